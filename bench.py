@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Graph-kernel benchmark on one MI355X: triangle count, BFS and PageRank on an R-MAT graph.
+
+Workload (default): R-MAT scale 22, edge factor 16, (a, b, c, d) = (0.57, 0.19, 0.19, 0.05),
+seeded, vertex ids permuted, symmetrized, deduplicated, self-loops removed, u32 CSR. One step =
+  - triangle_count (symmetric CSR; the count is checked against the committed golden value),
+  - BFS from `--sources` seeded random sources of nonzero degree (distances left in HBM),
+  - PageRank d = 0.85, `--pr-iterations` iterations, tol 1e-6 (ranks left in HBM).
+The graph is uploaded and prepared (degree-ordered layout) before the timed region.
+
+Reported (one JSON line): `value` = whole-step GTEPS with the reference's TEPS conventions
+(crates/grafeo-cuda-algos/src/bin/gpu_triangles.rs:127-132, gpu_bfs.rs:78-83,
+gpu_pagerank.rs:80-85): (m [TC] + m per BFS source + m x iterations [PageRank]) / step time,
+where m = CSR entries. Per-algorithm GTEPS, kernel event times, the roofline of the dominant
+kernel (HBM-bound: algorithmic bytes per launch / average launch time, against 8 TB/s) and a CPU
+baseline (the C oracle on the host cores, on a bounded sample) go into the same line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+from ctypes import byref, c_double, c_int32, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "grafeo-cuda_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
+GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pick_sources(offsets: np.ndarray, count: int, seed: int) -> list[int]:
+    """`count` distinct seeded random vertices of nonzero degree."""
+    deg = np.diff(offsets.astype(np.int64))
+    candidates = np.flatnonzero(deg > 0)
+    rng = np.random.default_rng(seed)
+    return [int(v) for v in rng.choice(candidates, size=min(count, candidates.size), replace=False)]
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--sources", type=int, default=8)
+    ap.add_argument("--pr-iterations", type=int, default=20)
+    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host threads (max 16)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(
+            json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
+                        "(see DESIGN.md), so --gpus must be 1", "n_gpus": args.gpus}),
+            flush=True,
+        )
+        return 2
+
+    import grafeo_cuda
+    from grafeo_cuda import _native
+    from grafeo_cuda._native import check, lib
+
+    t0 = time.time()
+    pairs = _native.gen_rmat(args.scale, args.edge_factor, args.seed)
+    offsets, targets = _native.from_edge_pairs(pairs, 1 << args.scale, True)
+    del pairs
+    n, m = offsets.size - 1, int(targets.size)
+    log(f"R-MAT-{args.scale}: n={n} m={m} built in {time.time() - t0:.1f}s")
+
+    dev = grafeo_cuda.GpuDevice(0)
+    t0 = time.time()
+    g = grafeo_cuda.GpuGraph.from_csr(offsets, targets, dev)
+    upload_s = time.time() - t0
+    prepare_s = g.prepare()
+    m_o, inter = c_uint64(), c_uint64()
+    check(lib.gc_graph_dag_stats(g._handle, byref(m_o), byref(inter)))
+    log(f"upload {upload_s:.2f}s, prepare {prepare_s:.2f}s, DAG entries {m_o.value}")
+
+    sources = pick_sources(offsets, args.sources, args.seed)
+    d_dist, d_rank = c_void_p(), c_void_p()
+    check(lib.gc_device_malloc(dev._handle, 4 * n, byref(d_dist)))
+    check(lib.gc_device_malloc(dev._handle, 4 * n, byref(d_rank)))
+
+    def step() -> dict:
+        times = {}
+        tri, secs = c_uint64(), c_double()
+        check(lib.gc_triangle_count(g._handle, byref(tri), byref(secs)))
+        times["tc"] = secs.value
+        bfs_s, reach_total = 0.0, 0
+        for s in sources:
+            reach, depth = c_uint32(), c_int32()
+            check(lib.gc_bfs_device(g._handle, s, d_dist, byref(reach), byref(depth), byref(secs)))
+            bfs_s += secs.value
+            reach_total += reach.value
+        times["bfs"] = bfs_s
+        it, conv = c_uint32(), c_int32()
+        check(lib.gc_pagerank_device(g._handle, 0.85, args.pr_iterations, 1e-6, d_rank, byref(it), byref(conv), byref(secs)))
+        times["pagerank"] = secs.value
+        return {"times": times, "triangles": tri.value, "pr_iterations": it.value, "reachable": reach_total}
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"warmup {i}: {r['times']}")
+    t_start = time.perf_counter()
+    results = [step() for _ in range(args.steps)]
+    elapsed = time.perf_counter() - t_start
+    log(f"timed {args.steps} steps in {elapsed:.3f}s")
+
+    # Per-kernel event timing in one extra (untimed) step.
+    dev.profile(True)
+    step()
+    kernels = {}
+    for name in ("tc_count", "pr_spmv", "pr_finalize", "bfs_topdown", "bfs_bottomup",
+                 "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "gather_u32", "fill_u32", "pr_init"):
+        ms, count = dev.kernel_time(name)
+        if count:
+            kernels[name] = {"total_ms": ms, "launches": count, "avg_us": 1000.0 * ms / count}
+    dev.profile(False)
+
+    iters = results[-1]["pr_iterations"]
+    edges_per_step = m * (1 + len(sources) + iters)
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = edges_per_step / (elapsed / args.steps) / 1e9
+    mean = {k: float(np.mean([r["times"][k] for r in results])) for k in ("tc", "bfs", "pagerank")}
+    per_algo = {
+        "tc_gteps": m / mean["tc"] / 1e9,
+        "bfs_gteps_per_source": m * len(sources) / mean["bfs"] / 1e9,
+        "pagerank_gteps": m * iters / mean["pagerank"] / 1e9,
+        "tc_ms": 1000 * mean["tc"],
+        "bfs_ms_per_source": 1000 * mean["bfs"] / len(sources),
+        "pagerank_ms": 1000 * mean["pagerank"],
+    }
+
+    # Algorithmic bytes per launch (HBM roofline for every kernel here).
+    spmv_bytes = 4 * (n + 1) + 4 * m + 4 * n + 4 * n
+    tc_bytes = 4 * (n + 1) + 4 * m_o.value + 4 * inter.value
+    rooflines = {}
+    if "pr_spmv" in kernels:
+        a = spmv_bytes / (kernels["pr_spmv"]["avg_us"] * 1e-6) / 1e9
+        rooflines["pr_spmv"] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": spmv_bytes}
+    if "tc_count" in kernels:
+        a = tc_bytes / (kernels["tc_count"]["avg_us"] * 1e-6) / 1e9
+        rooflines["tc_count"] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": tc_bytes}
+    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
+    roofline = rooflines.get(dominant) or rooflines.get("pr_spmv")
+
+    parity = {}
+    try:
+        with open(GOLDEN) as f:
+            golden = json.load(f)
+        key = f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}"
+        if key in golden:
+            parity["triangles_match_golden"] = results[-1]["triangles"] == golden[key]["triangles"]
+    except FileNotFoundError:
+        pass
+
+    cpu = None
+    if args.cpu_baseline == "on":
+        cpu = cpu_baseline(offsets, targets, sources, iters, args.cpu_threads, m)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (TC, BFS); f32 ranks, f64 reductions (PageRank)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"R-MAT scale {args.scale}, edge factor {args.edge_factor}: triangle count + "
+                        f"BFS x{len(sources)} sources + PageRank x{iters} iterations per step",
+            "num_nodes": n, "csr_entries": m, "dag_entries": m_o.value, "seed": args.seed,
+            "bfs_sources": len(sources), "pagerank_iterations": iters,
+            "parallelism": "single GPU",
+        },
+        "per_algorithm": per_algo,
+        "roofline": roofline,
+        "rooflines": rooflines,
+        "kernels": kernels,
+        "prepare_seconds": prepare_s,
+        "upload_seconds": upload_s,
+        "triangles": results[-1]["triangles"],
+        "parity": parity,
+        "cpu_baseline": cpu,
+        "device": dev.name,
+    }
+    check(lib.gc_device_free(dev._handle, d_dist))
+    check(lib.gc_device_free(dev._handle, d_rank))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(offsets, targets, sources, iters, threads, m) -> dict:
+    """The C oracle (a port of the reference's semantics, OpenMP) on the host cores, on a sample:
+    one triangle count, one BFS source, two PageRank iterations; scaled to the step's mix."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    t = time.perf_counter()
+    oracle.triangle_count(offsets, targets, fast=True)
+    tc_s = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.bfs(offsets, targets, sources[0])
+    bfs_s = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.pagerank(offsets, targets, 0.85, 2, 0.0)
+    pr_iter_s = (time.perf_counter() - t) / 2
+    step_s = tc_s + len(sources) * bfs_s + iters * pr_iter_s
+    value = m * (1 + len(sources) + iters) / step_s / 1e9
+    log(f"cpu baseline: tc {tc_s:.2f}s bfs {bfs_s:.2f}s pr/it {pr_iter_s:.3f}s on {threads} threads")
+    return {
+        "value": value,
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"1 triangle count, 1 BFS source, 2 PageRank iterations of the same graph, "
+                  f"scaled to the step's mix (tc {tc_s:.3f}s, bfs {bfs_s:.3f}s/source, "
+                  f"pagerank {pr_iter_s:.4f}s/iteration); host {platform.processor() or platform.machine()}",
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,171 @@
+// PageRank driver.
+//
+// Replaces pagerank::pagerank (crates/grafeo-cuda-algos/src/pagerank.rs:46-199):
+// r0 = 1/n; per iteration the dangling rank D, the pull update, the L1 change;
+// `iterations` counts performed iterations; converged once the change drops
+// below `tolerance`; the latest iterate is returned; n == 0 gives
+// {[], 0, converged}. Differences: FP64 accumulation and reductions (the
+// reference sums in FP32 with FP32 atomics), and the convergence test runs on
+// the device, the host reading the state once per batch of iterations.
+#include <algorithm>
+
+#include "../runtime/common.hpp"
+
+namespace gc {
+
+namespace {
+
+constexpr uint32_t kPrThreads = 256;
+constexpr uint32_t kPrMaxSegments = 10;
+constexpr uint32_t kPrBatch = 8;  // iterations launched between state reads
+
+struct PrSegment {
+  uint32_t row_begin;
+  uint32_t row_end;
+  uint32_t group;
+  uint32_t block_begin;
+};
+
+struct PrSegments {
+  PrSegment seg[kPrMaxSegments];
+  uint32_t count;
+  uint32_t total_blocks;
+};
+
+struct PrState {
+  double dangling;
+  double diff;
+  uint32_t iterations;
+  uint32_t done;
+};
+
+// Cuts the degree-sorted rows into segments of one lane-group size each.
+PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n) {
+  // Degree thresholds and the group size used below each.
+  static const uint32_t kLimit[] = {2, 4, 8, 16, 32, 64, 1024, 0xFFFFFFFFu};
+  static const uint32_t kGroup[] = {1, 2, 4, 8, 16, 32, 64, 256};
+  PrSegments s{};
+  uint32_t row = 0, block = 0;
+  auto degree = [&](uint32_t i) { return off[i + 1] - off[i]; };
+  for (int t = 0; t < 8 && row < n; ++t) {
+    // first row at or after `row` whose degree reaches kLimit[t]
+    uint32_t lo = row, hi = n;
+    while (lo < hi) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (degree(mid) >= kLimit[t]) hi = mid;
+      else lo = mid + 1;
+    }
+    if (t == 7) lo = n;
+    if (lo > row) {
+      PrSegment seg;
+      seg.row_begin = row;
+      seg.row_end = lo;
+      seg.group = kGroup[t];
+      seg.block_begin = block;
+      const uint32_t rows_per_block = kGroup[t] >= kPrThreads ? 1 : kPrThreads / kGroup[t];
+      block += (lo - row + rows_per_block - 1) / rows_per_block;
+      s.seg[s.count++] = seg;
+      row = lo;
+    }
+  }
+  s.total_blocks = block;
+  return s;
+}
+
+}  // namespace
+
+void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance, float* d_out,
+              uint32_t* iterations, int32_t* converged) {
+  Device& dev = *g.dev;
+  prepare(g);
+  const uint32_t n = g.n;
+  const PrSegments segs = plan_segments(g.rl.host_offsets, n);
+  float* rank[2] = {g.buffer<float>("pr.rank0", sizeof(float) * n),
+                    g.buffer<float>("pr.rank1", sizeof(float) * n)};
+  float* contrib[2] = {g.buffer<float>("pr.contrib0", sizeof(float) * n),
+                       g.buffer<float>("pr.contrib1", sizeof(float) * n)};
+  double* partials = g.buffer<double>("pr.partials", sizeof(double) * 2 * std::max(1u, segs.total_blocks));
+  PrState* state = g.buffer<PrState>("pr.state", sizeof(PrState));
+
+  const float r0 = 1.0f / static_cast<float>(n);
+  const uint32_t* off = g.rl.offsets.as<const uint32_t>();
+  const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
+  Grid1D grid = thread_per_item(n);
+  launch(dev, get_kernel(dev, "pagerank", "pr_init"), grid.blocks, grid.threads, 0, off, n, r0,
+         rank[0], contrib[0]);
+  PrState init{static_cast<double>(g.rl.dangling) * static_cast<double>(r0), 0.0, 0, 0};
+  copy_to_device(dev, state, &init, sizeof(init));
+
+  Kernel spmv = get_kernel(dev, "pagerank", "pr_spmv");
+  Kernel finalize = get_kernel(dev, "pagerank", "pr_finalize");
+  const double d = static_cast<double>(damping);
+  const double tol = static_cast<double>(tolerance);
+  PrState host_state = init;
+  for (uint32_t it = 0; it < max_iterations; ++it) {
+    const int in = it & 1, out = in ^ 1;
+    launch(dev, spmv, segs.total_blocks, kPrThreads, 0, off, tgt,
+           static_cast<const float*>(contrib[in]), static_cast<const float*>(rank[in]), contrib[out],
+           rank[out], n, d, segs, static_cast<const PrState*>(state), partials);
+    launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials),
+           segs.total_blocks, tol);
+    if ((it + 1) % kPrBatch == 0 || it + 1 == max_iterations) {
+      copy_to_host(dev, dev.pinned, state, sizeof(PrState));
+      std::memcpy(&host_state, dev.pinned, sizeof(PrState));
+      if (host_state.done) break;
+    }
+  }
+  const int latest = host_state.iterations & 1;
+  Grid1D ogrid = thread_per_item(n);
+  launch(dev, get_kernel(dev, "util", "gather_u32"), ogrid.blocks, ogrid.threads, 0,
+         reinterpret_cast<const uint32_t*>(rank[latest]), g.rl.inv.as<const uint32_t>(),
+         reinterpret_cast<uint32_t*>(d_out), static_cast<uint64_t>(n));
+  dev.sync();
+  *iterations = host_state.iterations;
+  *converged = host_state.done ? 1 : 0;
+}
+
+}  // namespace gc
+
+using gc::guard;
+
+extern "C" {
+
+gc_status gc_pagerank_device(gc_graph* h, float damping, uint32_t max_iterations, float tolerance,
+                             float* d_ranks, uint32_t* iterations, int32_t* converged,
+                             double* secs) {
+  return guard([&] {
+    gc::Stopwatch sw;
+    gc::GraphCall call(h);
+    uint32_t it = 0;
+    int32_t conv = 1;
+    if (call.g->n > 0) {
+      if (d_ranks == nullptr) gc::fail_kernel("ranks is null");
+      gc::pagerank(*call.g, damping, max_iterations, tolerance, d_ranks, &it, &conv);
+    }
+    if (iterations) *iterations = it;
+    if (converged) *converged = conv;
+    if (secs) *secs = sw.seconds();
+  });
+}
+
+gc_status gc_pagerank(gc_graph* h, float damping, uint32_t max_iterations, float tolerance,
+                      float* ranks, uint32_t* iterations, int32_t* converged, double* secs) {
+  return guard([&] {
+    gc::Stopwatch sw;
+    gc::GraphCall call(h);
+    gc::Graph& g = *call.g;
+    uint32_t it = 0;
+    int32_t conv = 1;
+    if (g.n > 0) {
+      if (ranks == nullptr) gc::fail_kernel("ranks is null");
+      float* d_out = g.buffer<float>("pr.out", sizeof(float) * g.n);
+      gc::pagerank(g, damping, max_iterations, tolerance, d_out, &it, &conv);
+      gc::copy_to_host(call.dev(), ranks, d_out, sizeof(float) * g.n);
+    }
+    if (iterations) *iterations = it;
+    if (converged) *converged = conv;
+    if (secs) *secs = sw.seconds();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// Direction-optimizing level-synchronous BFS.
+//
+// Same result as crates/grafeo-cuda-algos/kernels/bfs.cu:54-83 (distance =
+// BFS level along the CSR lists, -1 if unreachable): a vertex's level does
+// not depend on which frontier vertex discovers it, so any schedule below is
+// bit-exact with the reference's top-down expansion.
+//  - bfs_topdown: the frontier is a queue; a wave takes 64 frontier vertices,
+//    prefix-sums their degrees and spreads the edges evenly over its lanes
+//    (hubs and leaves in one pass); an unvisited neighbour is claimed with a
+//    compare-and-swap on its distance, and winners are appended to the next
+//    queue with one atomic per wave (ballot + popcount ranks).
+//  - bfs_bottomup: the frontier is a bitmap; each lane owns one unvisited
+//    vertex and scans its list from the end (highest degree-ordered ids, the
+//    hubs most likely to be in the frontier) until it finds a parent; the
+//    wave writes its 64 result bits as one 64-bit bitmap word, so the next
+//    bitmap needs no clearing and no atomics.
+// Both report the size of the next frontier and the degree sum of its
+// vertices, which drive the direction choice.
+#include "device.hpp"
+
+using namespace gcd;
+
+struct BfsCounters {
+  uint32_t next_size;
+  uint32_t pad;
+  unsigned long long next_edges;
+};
+
+extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__ dist, uint32_t n,
+                                                           uint32_t source, uint32_t* __restrict__ queue) {
+  for (uint64_t v = global_thread(); v < n; v += grid_threads()) dist[v] = v == source ? 0 : -1;
+  if (global_thread() == 0) queue[0] = source;
+}
+
+__device__ __forceinline__ void bfs_append(bool discovered, uint32_t v, uint32_t deg,
+                                           uint32_t* __restrict__ next_queue,
+                                           BfsCounters* __restrict__ counters) {
+  const uint64_t mask = __ballot(discovered);
+  if (mask == 0) return;
+  const uint32_t leader = __builtin_ctzll(mask);
+  uint32_t base = 0;
+  unsigned long long edges = wave_sum(static_cast<unsigned long long>(discovered ? deg : 0u));
+  if (lane_id() == leader) {
+    base = atomicAdd(&counters->next_size, static_cast<uint32_t>(__popcll(mask)));
+    atomicAdd(&counters->next_edges, edges);
+  }
+  base = __shfl(base, static_cast<int>(leader));
+  if (discovered && next_queue != nullptr) next_queue[base + lanes_below(mask)] = v;
+}
+
+extern "C" __global__ __launch_bounds__(256) void bfs_topdown(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    int32_t* __restrict__ dist, const uint32_t* __restrict__ queue, uint32_t queue_size,
+    uint32_t* __restrict__ next_queue, BfsCounters* __restrict__ counters, int32_t next_level) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t base = (global_thread() / GC_WAVE) * GC_WAVE; base < queue_size;
+       base += waves * GC_WAVE) {
+    const uint64_t idx = base + lane;
+    uint32_t st = 0, len = 0;
+    if (idx < queue_size) {
+      const uint32_t u = queue[idx];
+      st = offsets[u];
+      len = offsets[u + 1] - st;
+    }
+    const uint32_t incl = wave_inclusive_scan(len);
+    const uint32_t total = __shfl(incl, GC_WAVE - 1);
+    for (uint32_t t0 = 0; t0 < total; t0 += GC_WAVE) {
+      const uint32_t t = t0 + lane;
+      const bool active = t < total;
+      const uint32_t tt = active ? t : total - 1;
+      const uint32_t owner = wave_upper_bound(incl, tt);
+      const uint32_t prev = __shfl(incl, static_cast<int>(owner == 0 ? 0 : owner - 1));
+      const uint32_t excl = owner == 0 ? 0 : prev;
+      const uint32_t ost = __shfl(st, static_cast<int>(owner));
+      bool discovered = false;
+      uint32_t v = 0, deg = 0;
+      if (active) {
+        v = targets[ost + (tt - excl)];
+        if (dist[v] < 0) {
+          discovered = atomicCAS(&dist[v], -1, next_level) == -1;
+          if (discovered) deg = offsets[v + 1] - offsets[v];
+        }
+      }
+      bfs_append(discovered, v, deg, next_queue, counters);
+    }
+  }
+}
+
+extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    int32_t* __restrict__ dist, const uint32_t* __restrict__ front, uint64_t* __restrict__ next,
+    uint32_t n, BfsCounters* __restrict__ counters, int32_t next_level) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  const uint64_t chunks = (static_cast<uint64_t>(n) + GC_WAVE - 1) / GC_WAVE;
+  for (uint64_t chunk = global_thread() / GC_WAVE; chunk < chunks; chunk += waves) {
+    const uint64_t v64 = chunk * GC_WAVE + lane;
+    bool found = false;
+    uint32_t deg = 0;
+    if (v64 < n) {
+      const uint32_t v = static_cast<uint32_t>(v64);
+      if (dist[v] < 0) {
+        const uint32_t b = offsets[v], e = offsets[v + 1];
+        for (uint32_t i = e; i > b; --i) {
+          const uint32_t u = targets[i - 1];
+          if ((front[u >> 5] >> (u & 31)) & 1u) {
+            found = true;
+            break;
+          }
+        }
+        if (found) {
+          dist[v] = next_level;
+          deg = e - b;
+        }
+      }
+    }
+    const uint64_t mask = __ballot(found);
+    if (lane == 0) next[chunk] = mask;
+    if (mask != 0) {
+      unsigned long long edges = wave_sum(static_cast<unsigned long long>(deg));
+      if (lane == 0) {
+        atomicAdd(&counters->next_size, static_cast<uint32_t>(__popcll(mask)));
+        atomicAdd(&counters->next_edges, edges);
+      }
+    }
+  }
+}
+
+// Sets the bitmap bits of the queue's vertices (the bitmap is cleared first).
+extern "C" __global__ __launch_bounds__(256) void bfs_queue_to_bitmap(
+    const uint32_t* __restrict__ queue, uint32_t size, uint32_t* __restrict__ bitmap) {
+  for (uint64_t i = global_thread(); i < size; i += grid_threads()) {
+    const uint32_t v = queue[i];
+    atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+  }
+}
+
+// Lists the set bits of a bitmap as a queue (any order).
+extern "C" __global__ __launch_bounds__(256) void bfs_bitmap_to_queue(
+    const uint64_t* __restrict__ bitmap, uint64_t words, uint32_t* __restrict__ queue,
+    uint32_t* __restrict__ size) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t base = (global_thread() / GC_WAVE) * GC_WAVE; base < words; base += waves * GC_WAVE) {
+    const uint64_t w = base + lane;
+    const uint64_t bits = w < words ? bitmap[w] : 0ull;
+    const uint32_t c = static_cast<uint32_t>(__popcll(bits));
+    const uint32_t incl = wave_inclusive_scan(c);
+    const uint32_t total = __shfl(incl, GC_WAVE - 1);
+    uint32_t start = 0;
+    if (lane == 0 && total) start = atomicAdd(size, total);
+    start = __shfl(start, 0);
+    uint32_t pos = start + incl - c;
+    uint64_t rest = bits;
+    while (rest) {
+      const uint32_t bit = __builtin_ctzll(rest);
+      rest &= rest - 1;
+      queue[pos++] = static_cast<uint32_t>(w * 64 + bit);
+    }
+  }
+}

@@ -1,0 +1,138 @@
+// Triangle counting by sorted-list intersection, one 64-lane wave per vertex.
+//
+// Counts, over every vertex u and every v in L(u), the w in L(v) that are also
+// in L(u), where L(x) = targets[lo[x] .. hi[x]). The drivers choose L:
+//  - triangle_count (crates/grafeo-cuda-algos/kernels/triangle_counting.cu:246-315
+//    counts w > v > u with v, w in N(u) and w in N(v)): L(x) is the suffix of
+//    N(x) above x, and `after_only` restricts the search to the part of L(u)
+//    above v (w > v holds for every w in L(v)). On a symmetric graph the
+//    driver passes the degree-ordered relabelled CSR, where this suffix is
+//    the degree-oriented DAG;
+//  - triangle_count_oriented (triangle_counting.cu:331-376, the sum over
+//    entries u->v of |N(u) intersect N(v)|): L(x) = N(x), `after_only` = 0.
+//
+// Work layout (CDNA4): a workgroup is one wave. The wave stages L(u) in LDS
+// (up to TC_LDS_CAP entries, else it searches global memory), then walks the
+// lists L(v) of 64 neighbours v at a time as one flattened item range: a wave
+// prefix sum over their lengths spreads the items of short and long lists
+// evenly over the 64 lanes, every lane reads consecutive list entries
+// (coalesced), and each item is a binary search in the LDS copy of L(u).
+// Vertices are handed out dynamically: the first `heavy` positions one at a
+// time, the rest in chunks of `light_chunk`, so hubs (processed first when
+// `descending`) do not leave a tail. Counts are u64 per wave, added with one
+// atomic per wave at the end (integer sums are exact in any order).
+#include "device.hpp"
+
+using namespace gcd;
+
+#define TC_LDS_CAP 1024
+
+extern "C" __global__ __launch_bounds__(64) void tc_count(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+    const uint32_t* __restrict__ targets, uint32_t n, uint32_t descending, uint32_t after_only,
+    uint32_t heavy, uint32_t light_chunk, uint32_t* __restrict__ counters,
+    unsigned long long* __restrict__ result) {
+  __shared__ uint32_t s_a[TC_LDS_CAP];
+  __shared__ uint32_t s_pos[GC_WAVE];
+  __shared__ uint32_t s_start[GC_WAVE];
+  __shared__ uint32_t s_range[2];
+
+  const uint32_t lane = threadIdx.x;
+  unsigned long long count = 0;
+  bool heavy_left = heavy > 0;
+
+  for (;;) {
+    // ---- fetch the next range of positions [kb, ke) ----
+    if (lane == 0) {
+      uint32_t kb = n, ke = n;
+      if (heavy_left) {
+        uint32_t k = atomicAdd(&counters[0], 1u);
+        if (k < heavy) {
+          kb = k;
+          ke = k + 1;
+        } else {
+          heavy_left = false;
+        }
+      }
+      if (!heavy_left && kb == n) {
+        uint32_t c = atomicAdd(&counters[1], light_chunk);
+        uint64_t b = static_cast<uint64_t>(heavy) + c;
+        if (b < n) {
+          kb = static_cast<uint32_t>(b);
+          uint64_t e = b + light_chunk;
+          ke = e < n ? static_cast<uint32_t>(e) : n;
+        }
+      }
+      s_range[0] = kb;
+      s_range[1] = ke;
+    }
+    __syncthreads();
+    const uint32_t kb = s_range[0], ke = s_range[1];
+    heavy_left = __shfl(heavy_left ? 1 : 0, 0) != 0;
+    __syncthreads();
+    if (kb >= n) break;
+
+    for (uint32_t k = kb; k < ke; ++k) {
+      const uint32_t u = descending ? n - 1 - k : k;
+      const uint32_t a_lo = lo[u];
+      const uint32_t a = hi[u] - a_lo;
+      if (a == 0) continue;
+      const uint32_t* A = targets + a_lo;
+      const bool in_lds = a <= TC_LDS_CAP;
+      if (in_lds) {
+        for (uint32_t j = lane; j < a; j += GC_WAVE) s_a[j] = A[j];
+      }
+      __syncthreads();
+
+      for (uint32_t c = 0; c < a; c += GC_WAVE) {
+        const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - c);
+        uint32_t len = 0, st = 0;
+        if (lane < nv) {
+          const uint32_t v = in_lds ? s_a[c + lane] : A[c + lane];
+          st = lo[v];
+          len = hi[v] - st;
+        }
+        const uint32_t incl = wave_inclusive_scan(len);
+        s_pos[lane] = incl;
+        s_start[lane] = st;
+        const uint32_t total = __shfl(incl, GC_WAVE - 1);
+        __syncthreads();
+
+        for (uint32_t t = lane; t < total; t += GC_WAVE) {
+          // owner j: first lane whose inclusive end exceeds t
+          uint32_t l = 0, h = nv - 1;
+          while (l < h) {
+            uint32_t mid = (l + h) >> 1;
+            if (s_pos[mid] > t) h = mid;
+            else l = mid + 1;
+          }
+          const uint32_t excl = l ? s_pos[l - 1] : 0;
+          const uint32_t w = targets[s_start[l] + (t - excl)];
+          uint32_t from = after_only ? c + l + 1 : 0;
+          uint32_t to = a;
+          bool found;
+          if (in_lds) {
+            while (from < to) {
+              uint32_t mid = (from + to) >> 1;
+              if (s_a[mid] < w) from = mid + 1;
+              else to = mid;
+            }
+            found = from < a && s_a[from] == w;
+          } else {
+            while (from < to) {
+              uint32_t mid = (from + to) >> 1;
+              if (A[mid] < w) from = mid + 1;
+              else to = mid;
+            }
+            found = from < a && A[from] == w;
+          }
+          count += found ? 1ull : 0ull;
+        }
+        __syncthreads();
+      }
+    }
+  }
+
+  count = wave_sum(count);
+  if (lane == 0 && count != 0) atomicAdd(result, count);
+}

@@ -1,0 +1,264 @@
+// Internal runtime of libgrafeo_hip: errors, devices, device memory, kernel
+// launches and the graph object shared by the algorithm drivers.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "grafeo_hip.h"
+#include "launch.hpp"
+
+namespace gc {
+
+// ---------------------------------------------------------------- errors --
+
+// Thrown inside the library and turned into a gc_status at the C boundary.
+struct Error {
+  gc_status status;
+  std::string message;  // Display text, e.g. "invalid CSR: ..."
+};
+
+[[noreturn]] void fail(gc_status status, const std::string& detail);
+[[noreturn]] inline void fail_kernel(const std::string& detail) { fail(GC_ERR_KERNEL, detail); }
+[[noreturn]] inline void fail_csr(const std::string& detail) { fail(GC_ERR_INVALID_CSR, detail); }
+
+void hip_check(hipError_t err, const char* what);
+void set_last_error(const std::string& message);
+
+// Runs `body` and converts exceptions into a status (and gc_last_error text).
+gc_status guard(const std::function<void()>& body);
+
+// ---------------------------------------------------------------- timing --
+
+class Stopwatch {
+ public:
+  Stopwatch() : start_(std::chrono::steady_clock::now()) {}
+  double seconds() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - start_).count();
+  }
+
+ private:
+  std::chrono::steady_clock::time_point start_;
+};
+
+// ------------------------------------------------------- host parallelism --
+
+// Number of host threads used by builders and graph preparation
+// (GC_HOST_THREADS, default: hardware concurrency capped at 16).
+unsigned host_threads();
+
+// Runs fn(begin, end) over [0, n) split into contiguous chunks, one per
+// thread. Small ranges run on the calling thread.
+void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)>& fn,
+                  uint64_t min_chunk = 1u << 16);
+
+// ------------------------------------------------------------- profiling --
+
+struct ProfileRecord {
+  std::string name;
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+
+struct ProfileTotal {
+  std::string name;
+  double total_ms = 0.0;
+  uint64_t launches = 0;
+};
+
+// ---------------------------------------------------------------- device --
+
+struct LoadedModule;
+
+struct Device {
+  std::atomic<int> refs{1};
+  int ordinal = 0;
+  hipStream_t stream = nullptr;
+  std::string name;
+  uint64_t total_memory = 0;
+  int cu_count = 0;
+  std::recursive_mutex mu;  // serializes calls on this device
+  // Small pinned staging area for scalar read-backs (counters, states).
+  void* pinned = nullptr;
+  static constexpr size_t kPinnedBytes = 4096;
+  // Kernel timing.
+  bool profiling = false;
+  std::vector<ProfileRecord> pending;
+  std::vector<ProfileTotal> totals;
+  std::vector<hipEvent_t> event_pool;
+
+  void make_current() const;
+  void sync() const;
+};
+
+// A kernel handle resolved from an embedded code object.
+struct Kernel {
+  hipFunction_t fn = nullptr;
+  const char* name = "";
+};
+
+// Looks up kernel `name` in code object `family` on `dev` (loaded once per
+// device through the module cache).
+Kernel get_kernel(Device& dev, const char* family, const char* name);
+
+// Event-timing hooks around one launch (no-ops unless profiling is on).
+void profile_begin(Device& dev, const char* name, hipEvent_t* start);
+void profile_end(Device& dev, const char* name, hipEvent_t start);
+
+template <class T>
+inline void* karg(T& value) {
+  return static_cast<void*>(&value);
+}
+
+// Launches `k` with the given arguments (passed by value, each must match the
+// kernel parameter type exactly).
+template <class... Args>
+void launch(Device& dev, const Kernel& k, uint32_t grid, uint32_t block, uint32_t shmem,
+            Args... args) {
+  if (grid == 0) return;
+  void* params[] = {karg(args)..., nullptr};
+  hipEvent_t start = nullptr;
+  profile_begin(dev, k.name, &start);
+  hipError_t err = hipModuleLaunchKernel(k.fn, grid, 1, 1, block, 1, 1, shmem, dev.stream, params,
+                                         nullptr);
+  if (err != hipSuccess) {
+    fail(GC_ERR_KERNEL, std::string(k.name) + " launch failed: " + hipGetErrorString(err));
+  }
+  profile_end(dev, k.name, start);
+}
+
+// --------------------------------------------------------- device memory --
+
+// Owning device allocation (hipMalloc on the device current at allocation).
+class DeviceMemory {
+ public:
+  DeviceMemory() = default;
+  DeviceMemory(const DeviceMemory&) = delete;
+  DeviceMemory& operator=(const DeviceMemory&) = delete;
+  DeviceMemory(DeviceMemory&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+    o.ptr_ = nullptr;
+    o.bytes_ = 0;
+  }
+  DeviceMemory& operator=(DeviceMemory&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr_ = o.ptr_;
+      bytes_ = o.bytes_;
+      o.ptr_ = nullptr;
+      o.bytes_ = 0;
+    }
+    return *this;
+  }
+  ~DeviceMemory() { release(); }
+
+  void allocate(size_t bytes);          // replaces any previous allocation
+  void ensure(size_t bytes);            // grows (never shrinks) the allocation
+  void release();
+  void* get() const { return ptr_; }
+  size_t bytes() const { return bytes_; }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr_);
+  }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+void copy_to_device(Device& dev, void* dst, const void* src, size_t bytes);
+void copy_to_host(Device& dev, void* dst, const void* src, size_t bytes);  // synchronizes
+
+// ----------------------------------------------------------------- graph --
+
+// Degree-ordered relabelled copy of a graph: new id i is old vertex perm[i];
+// vertices are sorted by degree ascending, ties by old id, so hubs sit at the
+// top of the id range and the rank/distance vectors of the hubs are one
+// contiguous, cache-resident block. Each row is sorted by new id; for a
+// symmetric graph the entries with target > row form the degree-oriented DAG
+// (lower degree -> higher degree, ties by lower old id), which is exactly the
+// orientation of from_edge_pairs_oriented.
+struct Relabelled {
+  DeviceMemory perm;     // u32[n]   new -> old
+  DeviceMemory inv;      // u32[n]   old -> new
+  DeviceMemory offsets;  // u32[n+1]
+  DeviceMemory targets;  // u32[m]
+  DeviceMemory fwd;      // u32[n]   first index of row i with target > i
+  std::vector<uint32_t> host_inv;
+  std::vector<uint32_t> host_offsets;  // kept for host-side planning
+  uint64_t oriented_entries = 0;       // sum of (offsets[i+1] - fwd[i])
+  uint64_t intersect_entries = 0;      // sum over DAG entries i->j of (d+(i) + d+(j))
+  uint32_t dangling = 0;               // rows of degree 0
+  uint32_t max_degree = 0;
+};
+
+struct Graph {
+  Device* dev = nullptr;  // holds one reference
+  uint32_t n = 0;
+  uint32_t m = 0;
+  DeviceMemory offsets;  // u32[n+1]
+  DeviceMemory targets;  // u32[m]
+  // Host copy kept until the derived layouts are built.
+  std::vector<uint32_t> host_offsets;
+  std::vector<uint32_t> host_targets;
+
+  bool prepared = false;
+  bool symmetric = false;
+  uint32_t self_loops = 0;
+  double prepare_seconds = 0.0;
+  Relabelled rl;
+
+  // Scratch reused across calls (grown on demand, freed with the graph).
+  DeviceMemory fwd_original;  // u32[n] first index with target > row (original layout)
+  bool fwd_original_ready = false;
+  DeviceMemory ktruss_rev;    // u32[m] reverse entry of each forward relabelled entry
+  bool ktruss_rev_ready = false;
+  std::vector<std::pair<std::string, std::unique_ptr<DeviceMemory>>> buffers;
+
+  // Named scratch buffer of at least `bytes` bytes (contents unspecified).
+  template <class T>
+  T* buffer(const char* name, size_t bytes) {
+    for (auto& b : buffers) {
+      if (b.first == name) {
+        b.second->ensure(bytes == 0 ? 1 : bytes);
+        return b.second->as<T>();
+      }
+    }
+    buffers.emplace_back(name, std::unique_ptr<DeviceMemory>(new DeviceMemory()));
+    buffers.back().second->ensure(bytes == 0 ? 1 : bytes);
+    return buffers.back().second->as<T>();
+  }
+};
+
+Graph* graph_from_handle(gc_graph* h);
+
+// Locks the graph's device, makes it current and returns the graph.
+struct GraphCall {
+  Graph* g;
+  std::unique_lock<std::recursive_mutex> lock;
+  explicit GraphCall(gc_graph* h) : g(graph_from_handle(h)), lock(g->dev->mu) { g->dev->make_current(); }
+  Device& dev() { return *g->dev; }
+};
+
+// Fills `words` 32-bit words at `p` with `value` on the device's stream.
+void fill_words(Device& dev, void* p, uint64_t words, uint32_t value);
+
+// Builds the derived layouts if not done yet (host relabel + upload).
+void prepare(Graph& g);
+void ensure_fwd_original(Graph& g);
+
+// Host-side CSR validation with the reference's messages (throws InvalidCsr).
+void validate_csr(const uint32_t* offsets, size_t num_offsets, const uint32_t* targets,
+                  size_t num_targets);
+
+}  // namespace gc

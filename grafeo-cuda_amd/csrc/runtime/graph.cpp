@@ -1,0 +1,254 @@
+// Graph upload and derived device layouts.
+//
+// Upload follows GpuGraph::from_csr_slices / from_csr_arrays
+// (crates/grafeo-cuda-core/src/gpu_graph.rs:46-87): validate first (a
+// malformed CSR never reaches the device), then copy offsets and targets.
+//
+// The derived layout (built on first use or by gc_graph_prepare) has no
+// reference counterpart: a degree-ordered relabelling of the graph. Vertices
+// get new ids sorted by (degree, old id) ascending by a stable counting sort;
+// each relabelled row is sorted, and fwd[i] marks where its entries above i
+// start. The kernels that need locality (PageRank, BFS) or the
+// degree-oriented DAG (triangle counting, k-truss on symmetric graphs) run on
+// it, and results are mapped back to the caller's ids on the device.
+#include <algorithm>
+#include <numeric>
+
+#include "common.hpp"
+
+namespace gc {
+
+void prepare(Graph& g) {
+  if (g.prepared) return;
+  Stopwatch sw;
+  Device& dev = *g.dev;
+  const uint32_t n = g.n;
+  const uint64_t m = g.m;
+  const uint32_t* off = g.host_offsets.data();
+  const uint32_t* tgt = g.host_targets.data();
+  Relabelled& rl = g.rl;
+
+  // Symmetry and self-loops of the original CSR.
+  std::atomic<bool> symmetric{true};
+  std::atomic<uint32_t> loops{0};
+  parallel_for(n, [&](uint64_t b, uint64_t e) {
+    uint32_t local_loops = 0;
+    bool sym = true;
+    for (uint64_t u = b; u < e && sym; ++u) {
+      for (uint32_t p = off[u]; p < off[u + 1]; ++p) {
+        uint32_t v = tgt[p];
+        if (v == u) {
+          ++local_loops;
+          continue;
+        }
+        if (!std::binary_search(tgt + off[v], tgt + off[v + 1], static_cast<uint32_t>(u))) {
+          sym = false;
+          break;
+        }
+      }
+    }
+    if (!sym) symmetric = false;
+    loops += local_loops;
+  }, 1u << 12);
+  g.symmetric = symmetric.load();
+  g.self_loops = loops.load();
+
+  // Stable counting sort by degree.
+  std::vector<uint32_t> deg(n);
+  uint32_t max_deg = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    deg[v] = off[v + 1] - off[v];
+    max_deg = std::max(max_deg, deg[v]);
+  }
+  std::vector<uint32_t> start(static_cast<size_t>(max_deg) + 2, 0);
+  for (uint32_t v = 0; v < n; ++v) ++start[deg[v] + 1];
+  for (size_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
+  std::vector<uint32_t> perm(n), inv(n);
+  for (uint32_t v = 0; v < n; ++v) perm[start[deg[v]]++] = v;
+  parallel_for(n, [&](uint64_t b, uint64_t e) {
+    for (uint64_t i = b; i < e; ++i) inv[perm[i]] = static_cast<uint32_t>(i);
+  });
+  std::vector<uint32_t> r_off(static_cast<size_t>(n) + 1);
+  r_off[0] = 0;
+  for (uint32_t i = 0; i < n; ++i) r_off[i + 1] = r_off[i] + deg[perm[i]];
+  std::vector<uint32_t> r_tgt(m), fwd(n);
+  parallel_for(n, [&](uint64_t b, uint64_t e) {
+    for (uint64_t i = b; i < e; ++i) {
+      const uint32_t old = perm[i];
+      uint32_t* row = r_tgt.data() + r_off[i];
+      const uint32_t len = r_off[i + 1] - r_off[i];
+      const uint32_t* src = tgt + off[old];
+      for (uint32_t j = 0; j < len; ++j) row[j] = inv[src[j]];
+      std::sort(row, row + len);
+      fwd[i] = r_off[i] + static_cast<uint32_t>(std::upper_bound(row, row + len, static_cast<uint32_t>(i)) - row);
+    }
+  }, 1u << 12);
+  uint64_t oriented = 0;
+  uint32_t dangling = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    oriented += r_off[i + 1] - fwd[i];
+    dangling += r_off[i + 1] == r_off[i];
+  }
+  std::atomic<uint64_t> intersect{0};
+  parallel_for(n, [&](uint64_t b, uint64_t e) {
+    uint64_t local = 0;
+    for (uint64_t i = b; i < e; ++i) {
+      const uint64_t du = r_off[i + 1] - fwd[i];
+      for (uint32_t p = fwd[i]; p < r_off[i + 1]; ++p) {
+        const uint32_t j = r_tgt[p];
+        local += du + (r_off[j + 1] - fwd[j]);
+      }
+    }
+    intersect += local;
+  }, 1u << 12);
+
+  dev.make_current();
+  rl.perm.allocate(sizeof(uint32_t) * std::max<size_t>(n, 1));
+  rl.inv.allocate(sizeof(uint32_t) * std::max<size_t>(n, 1));
+  rl.offsets.allocate(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  rl.targets.allocate(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  rl.fwd.allocate(sizeof(uint32_t) * std::max<size_t>(n, 1));
+  copy_to_device(dev, rl.perm.get(), perm.data(), sizeof(uint32_t) * n);
+  copy_to_device(dev, rl.inv.get(), inv.data(), sizeof(uint32_t) * n);
+  copy_to_device(dev, rl.offsets.get(), r_off.data(), sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  copy_to_device(dev, rl.targets.get(), r_tgt.data(), sizeof(uint32_t) * m);
+  copy_to_device(dev, rl.fwd.get(), fwd.data(), sizeof(uint32_t) * n);
+  rl.host_inv = std::move(inv);
+  rl.host_offsets = std::move(r_off);
+  rl.oriented_entries = oriented;
+  rl.intersect_entries = intersect.load();
+  rl.dangling = dangling;
+  rl.max_degree = max_deg;
+
+  // The host copy of the original CSR is no longer needed.
+  std::vector<uint32_t>().swap(g.host_targets);
+  g.prepared = true;
+  g.prepare_seconds = sw.seconds();
+}
+
+void ensure_fwd_original(Graph& g) {
+  if (g.fwd_original_ready) return;
+  Device& dev = *g.dev;
+  g.fwd_original.allocate(sizeof(uint32_t) * std::max<size_t>(g.n, 1));
+  if (g.n > 0) {
+    Kernel k = get_kernel(dev, "util", "csr_forward_start");
+    Grid1D grid = thread_per_item(g.n);
+    launch(dev, k, grid.blocks, grid.threads, 0, g.offsets.as<const uint32_t>(),
+           g.targets.as<const uint32_t>(), g.n, g.fwd_original.as<uint32_t>());
+  }
+  g.fwd_original_ready = true;
+}
+
+static Graph* as_graph(gc_graph* h) {
+  if (h == nullptr) fail_kernel("graph is null");
+  return reinterpret_cast<Graph*>(h);
+}
+
+static void upload(gc_device* dev_handle, const uint32_t* offsets, size_t num_offsets,
+                   const uint32_t* targets, size_t num_targets, gc_graph** out) {
+  if (out == nullptr) fail_kernel("out is null");
+  *out = nullptr;
+  if (dev_handle == nullptr) fail_kernel("device is null");
+  if ((offsets == nullptr && num_offsets > 0) || (targets == nullptr && num_targets > 0)) {
+    fail_kernel("null array with non-zero length");
+  }
+  validate_csr(offsets, num_offsets, targets, num_targets);
+  auto* dev = reinterpret_cast<Device*>(dev_handle);
+  std::lock_guard<std::recursive_mutex> lock(dev->mu);
+  dev->make_current();
+  std::unique_ptr<Graph> g(new Graph());
+  g->n = static_cast<uint32_t>(num_offsets - 1);
+  g->m = static_cast<uint32_t>(num_targets);
+  g->offsets.allocate(sizeof(uint32_t) * num_offsets);
+  g->targets.allocate(sizeof(uint32_t) * std::max<size_t>(num_targets, 1));
+  copy_to_device(*dev, g->offsets.get(), offsets, sizeof(uint32_t) * num_offsets);
+  copy_to_device(*dev, g->targets.get(), targets, sizeof(uint32_t) * num_targets);
+  g->host_offsets.assign(offsets, offsets + num_offsets);
+  g->host_targets.assign(targets, targets + num_targets);
+  g->dev = reinterpret_cast<Device*>(gc_device_retain(dev_handle));
+  *out = reinterpret_cast<gc_graph*>(g.release());
+}
+
+}  // namespace gc
+
+using gc::guard;
+
+extern "C" {
+
+gc_status gc_graph_from_csr(gc_device* dev, const uint32_t* offsets, size_t num_offsets,
+                            const uint32_t* targets, size_t num_targets, gc_graph** out) {
+  return guard([&] { gc::upload(dev, offsets, num_offsets, targets, num_targets, out); });
+}
+
+gc_status gc_graph_from_csr_arrays(gc_device* dev, const gc_csr* csr, gc_graph** out) {
+  return guard([&] {
+    if (csr == nullptr || csr->offsets == nullptr) gc::fail_kernel("csr is null");
+    gc::upload(dev, csr->offsets, size_t(csr->num_nodes) + 1, csr->targets, csr->num_edges, out);
+  });
+}
+
+void gc_graph_free(gc_graph* h) {
+  if (h == nullptr) return;
+  auto* g = reinterpret_cast<gc::Graph*>(h);
+  gc::Device* dev = g->dev;
+  {
+    std::lock_guard<std::recursive_mutex> lock(dev->mu);
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipStreamSynchronize(dev->stream);
+    delete g;
+  }
+  gc_device_release(reinterpret_cast<gc_device*>(dev));
+}
+
+uint32_t gc_graph_num_nodes(const gc_graph* h) {
+  return h ? reinterpret_cast<const gc::Graph*>(h)->n : 0;
+}
+
+uint32_t gc_graph_num_edges(const gc_graph* h) {
+  return h ? reinterpret_cast<const gc::Graph*>(h)->m : 0;
+}
+
+const uint32_t* gc_graph_device_offsets(const gc_graph* h) {
+  return h ? reinterpret_cast<const gc::Graph*>(h)->offsets.as<const uint32_t>() : nullptr;
+}
+
+const uint32_t* gc_graph_device_targets(const gc_graph* h) {
+  return h ? reinterpret_cast<const gc::Graph*>(h)->targets.as<const uint32_t>() : nullptr;
+}
+
+gc_status gc_graph_prepare(gc_graph* h, double* secs) {
+  return guard([&] {
+    gc::Graph* g = gc::as_graph(h);
+    std::lock_guard<std::recursive_mutex> lock(g->dev->mu);
+    g->dev->make_current();
+    gc::prepare(*g);
+    if (secs) *secs = g->prepare_seconds;
+  });
+}
+
+gc_status gc_graph_dag_stats(gc_graph* h, uint64_t* oriented_entries, uint64_t* intersect_entries) {
+  return guard([&] {
+    gc::Graph* g = gc::as_graph(h);
+    std::lock_guard<std::recursive_mutex> lock(g->dev->mu);
+    g->dev->make_current();
+    gc::prepare(*g);
+    if (oriented_entries) *oriented_entries = g->rl.oriented_entries;
+    if (intersect_entries) *intersect_entries = g->rl.intersect_entries;
+  });
+}
+
+gc_status gc_graph_is_symmetric(gc_graph* h, int32_t* symmetric) {
+  return guard([&] {
+    gc::Graph* g = gc::as_graph(h);
+    std::lock_guard<std::recursive_mutex> lock(g->dev->mu);
+    g->dev->make_current();
+    gc::prepare(*g);
+    if (symmetric) *symmetric = g->symmetric ? 1 : 0;
+  });
+}
+
+}  // extern "C"
+
+namespace gc {
+Graph* graph_from_handle(gc_graph* h) { return as_graph(h); }
+}  // namespace gc

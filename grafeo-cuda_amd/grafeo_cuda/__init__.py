@@ -1,0 +1,214 @@
+"""grafeo_cuda - the Python surface of grafeo-cuda, on MI355X (gfx950).
+
+Drop-in for the reference's PyO3 module `grafeo_cuda` (crates/bindings/python/src/lib.rs:12-17,
+crates/bindings/python/src/gpu_graph.rs:11-155): the classes `GpuDevice` and `GpuGraph`, with
+the same constructors, properties, method names, keyword defaults, result dict keys, and
+`RuntimeError(str(CudaError))` on failure. Everything runs through the C ABI of
+libgrafeo_hip.so (include/grafeo_hip.h); there is no CPU fallback.
+
+Supersets of the reference surface: `from_csr` also accepts numpy arrays (no per-element
+conversion), `triangle_count_oriented`, `prepare`, `is_symmetric`, and array-returning
+variants (`*_numpy`) for large graphs. The GIL is released during device work.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int32, c_size_t, c_uint32
+from ctypes import c_uint64, c_void_p
+
+import numpy as np
+
+from . import _native
+from ._native import GcError, check, lib
+
+__all__ = ["GpuDevice", "GpuGraph", "GcError"]
+
+
+def _runtime_error(e: GcError) -> RuntimeError:
+    err = RuntimeError(str(e))
+    err.status = e.status
+    err.kind = e.kind
+    return err
+
+
+def _u32_arg(value, what: str) -> int:
+    if isinstance(value, bool) or not isinstance(value, (int, np.integer)):
+        raise TypeError(f"{what} must be an int")
+    if value < 0 or value > 0xFFFFFFFF:
+        raise OverflowError(f"{what} out of range for u32")
+    return int(value)
+
+
+class GpuDevice:
+    """A GPU device (GpuDevice, crates/bindings/python/src/gpu_graph.rs:11-37)."""
+
+    def __init__(self, ordinal: int = 0):
+        ordinal = _u32_arg(ordinal, "ordinal")
+        handle = c_void_p()
+        try:
+            check(lib.gc_device_new(ordinal, byref(handle)))
+        except GcError as e:
+            raise _runtime_error(e) from None
+        self._handle = handle
+        name = ctypes.create_string_buffer(256)
+        total, ordn = c_uint64(), c_uint32()
+        check(lib.gc_device_properties(handle, name, 256, byref(total), byref(ordn)))
+        self._name = name.value.decode(errors="replace")
+        self._total_memory = total.value
+        self._ordinal = ordn.value
+
+    @property
+    def name(self) -> str:
+        """Device name."""
+        return self._name
+
+    @property
+    def total_memory(self) -> int:
+        """Total GPU memory in bytes."""
+        return self._total_memory
+
+    @property
+    def ordinal(self) -> int:
+        return self._ordinal
+
+    def profile(self, on: bool = True) -> None:
+        """Starts (or stops) per-kernel event timing and clears accumulated figures."""
+        check(lib.gc_profile_enable(self._handle, 1 if on else 0))
+
+    def kernel_time(self, kernel: str) -> tuple[float, int]:
+        """(total milliseconds, launches) recorded for `kernel` since `profile()`."""
+        ms, count = c_double(), c_uint64()
+        check(lib.gc_profile_query(self._handle, kernel.encode(), byref(ms), byref(count)))
+        return ms.value, count.value
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            lib.gc_device_release(h)
+            self._handle = None
+
+    def __repr__(self) -> str:
+        return f"GpuDevice(ordinal={self._ordinal}, name={self._name!r})"
+
+
+class GpuGraph:
+    """A graph in GPU memory (GpuGraph, crates/bindings/python/src/gpu_graph.rs:39-155)."""
+
+    def __init__(self, handle: c_void_p, device: GpuDevice):
+        self._handle = handle
+        self._device = device  # keeps the Python device object alive too
+
+    @staticmethod
+    def from_csr(offsets, targets, device: GpuDevice) -> "GpuGraph":
+        """Create a GpuGraph from CSR arrays (lists of ints or numpy arrays)."""
+        if not isinstance(device, GpuDevice):
+            raise TypeError("device must be a GpuDevice")
+        off = _native.as_u32(offsets, "offsets")
+        tgt = _native.as_u32(targets, "targets")
+        handle = c_void_p()
+        try:
+            check(
+                lib.gc_graph_from_csr(
+                    device._handle, _native.u32_ptr(off), off.size, _native.u32_ptr(tgt), tgt.size,
+                    byref(handle),
+                )
+            )
+        except GcError as e:
+            raise _runtime_error(e) from None
+        return GpuGraph(handle, device)
+
+    @property
+    def num_nodes(self) -> int:
+        """Number of nodes."""
+        return lib.gc_graph_num_nodes(self._handle)
+
+    @property
+    def num_edges(self) -> int:
+        """Number of edge entries in CSR."""
+        return lib.gc_graph_num_edges(self._handle)
+
+    def _call(self, fn, *args) -> None:
+        try:
+            check(fn(self._handle, *args))
+        except GcError as e:
+            raise _runtime_error(e) from None
+
+    def prepare(self) -> float:
+        """Builds the derived device layouts now; returns the seconds it took."""
+        secs = c_double()
+        self._call(lib.gc_graph_prepare, byref(secs))
+        return secs.value
+
+    def is_symmetric(self) -> bool:
+        sym = c_int32()
+        self._call(lib.gc_graph_is_symmetric, byref(sym))
+        return bool(sym.value)
+
+    def triangle_count(self) -> dict:
+        """Count all triangles in the graph on GPU: {'total_triangles', 'time_seconds'}."""
+        total, secs = c_uint64(), c_double()
+        self._call(lib.gc_triangle_count, byref(total), byref(secs))
+        return {"total_triangles": total.value, "time_seconds": secs.value}
+
+    def triangle_count_oriented(self) -> dict:
+        """triangle_counting::triangle_count_oriented for a degree-oriented CSR."""
+        total, secs = c_uint64(), c_double()
+        self._call(lib.gc_triangle_count_oriented, byref(total), byref(secs))
+        return {"total_triangles": total.value, "time_seconds": secs.value}
+
+    def pagerank_numpy(self, damping: float = 0.85, max_iterations: int = 100, tolerance: float = 1e-6) -> dict:
+        """As pagerank, with 'ranks' as a float32 numpy array."""
+        max_iterations = _u32_arg(max_iterations, "max_iterations")
+        ranks = np.zeros(self.num_nodes, dtype=np.float32)
+        it, conv, secs = c_uint32(), c_int32(), c_double()
+        self._call(
+            lib.gc_pagerank, c_float(damping), max_iterations, c_float(tolerance),
+            ranks.ctypes.data_as(POINTER(c_float)), byref(it), byref(conv), byref(secs),
+        )
+        return {"ranks": ranks, "iterations": it.value, "converged": bool(conv.value), "time_seconds": secs.value}
+
+    def pagerank(self, damping: float = 0.85, max_iterations: int = 100, tolerance: float = 1e-6) -> dict:
+        """Run PageRank on GPU: {'ranks', 'iterations', 'converged', 'time_seconds'}."""
+        r = self.pagerank_numpy(damping, max_iterations, tolerance)
+        r["ranks"] = [float(x) for x in r["ranks"]]
+        return r
+
+    def bfs_numpy(self, source: int) -> dict:
+        """As bfs, with 'distances' as an int32 numpy array."""
+        source = _u32_arg(source, "source")
+        dist = np.zeros(self.num_nodes, dtype=np.int32)
+        reach, depth, secs = c_uint32(), c_int32(), c_double()
+        self._call(
+            lib.gc_bfs, source, dist.ctypes.data_as(POINTER(c_int32)), byref(reach), byref(depth), byref(secs)
+        )
+        return {"distances": dist, "reachable": reach.value, "max_depth": depth.value, "time_seconds": secs.value}
+
+    def bfs(self, source: int) -> dict:
+        """Run BFS from a source node: {'distances', 'reachable', 'max_depth', 'time_seconds'}."""
+        r = self.bfs_numpy(source)
+        r["distances"] = [int(x) for x in r["distances"]]
+        return r
+
+    def k_truss_numpy(self, k: int) -> dict:
+        """As k_truss, with 'edges' as an (E, 2) uint32 numpy array."""
+        k = _u32_arg(k, "k")
+        ptr, count, it, secs = POINTER(c_uint32)(), c_size_t(), c_uint32(), c_double()
+        self._call(lib.gc_k_truss, k, byref(ptr), byref(count), byref(it), byref(secs))
+        edges = _native.take_u32(ptr, 2 * count.value).reshape(-1, 2)
+        return {"edges": edges, "k": k, "iterations": it.value, "time_seconds": secs.value}
+
+    def k_truss(self, k: int) -> dict:
+        """Extract the k-truss: {'edges': list[tuple], 'k', 'iterations', 'time_seconds'}."""
+        r = self.k_truss_numpy(k)
+        r["edges"] = [(int(u), int(v)) for u, v in r["edges"]]
+        return r
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            lib.gc_graph_free(h)
+            self._handle = None
+
+    def __repr__(self) -> str:
+        return f"GpuGraph(num_nodes={self.num_nodes}, num_edges={self.num_edges})"

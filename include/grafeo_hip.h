@@ -1,0 +1,267 @@
+/*
+ * grafeo_hip.h - C ABI of the MI355X (gfx950) graph-kernel library.
+ *
+ * Drop-in replacement for the GPU path of grafeo-cuda: the Rust crates
+ * `grafeo-cuda-core` (device, CSR model, graph upload) and
+ * `grafeo-cuda-algos` (triangle counting, k-truss, BFS, PageRank), and the
+ * PyO3 module `grafeo_cuda` that wraps them. Every entry point below names the
+ * reference item it replaces (paths relative to the reference repository).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Host arrays are read or written by the
+ *    call and never retained; results are owned by the caller, except arrays
+ *    the library allocates (k-truss edges, builder output), which are released
+ *    with gc_free / gc_csr_free.
+ *  - Every call returns a gc_status. On failure gc_last_error() returns the
+ *    message of the failing call on this thread (the Display text of the
+ *    reference's CudaError).
+ *  - All device work of one gc_device runs on that device's own stream and
+ *    calls on one device are serialized; each call returns after its results
+ *    are written (host outputs) or after its device work is complete (device
+ *    outputs).
+ *  - "secs" outputs carry the wall-clock time of the call, as the reference's
+ *    `time_seconds` fields do.
+ */
+#ifndef GRAFEO_HIP_H
+#define GRAFEO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, one per variant of `CudaError`
+ * (crates/grafeo-cuda-core/src/error.rs:5-39). */
+typedef enum gc_status {
+  GC_OK = 0,
+  GC_ERR_NO_DEVICE = 1,     /* CudaError::NoDevice */
+  GC_ERR_DRIVER = 2,        /* CudaError::Driver (HIP runtime error) */
+  GC_ERR_OUT_OF_MEMORY = 3, /* CudaError::OutOfMemory */
+  GC_ERR_KERNEL = 4,        /* CudaError::Kernel (launch and argument errors) */
+  GC_ERR_EXPORT = 5,        /* CudaError::Export */
+  GC_ERR_INVALID_CSR = 6    /* CudaError::InvalidCsr */
+} gc_status;
+
+/* Message of the last failed call on the calling thread ("" if none).
+ * Replaces `impl Display for CudaError` (crates/grafeo-cuda-core/src/error.rs:5-39). */
+const char* gc_last_error(void);
+
+/* Releases memory the library allocated (k-truss edges, parsed edge pairs,
+ * generator output). */
+void gc_free(void* ptr);
+
+/* ------------------------------------------------------------------------ */
+/* Device                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gc_device gc_device;
+
+/* Number of visible GPUs (0 when the runtime finds none). */
+gc_status gc_device_count(uint32_t* count);
+
+/* Opens GPU `ordinal` with its own stream; refcount 1.
+ * Replaces GpuDevice::new (crates/grafeo-cuda-core/src/device.rs:30-51). */
+gc_status gc_device_new(uint32_t ordinal, gc_device** out);
+
+/* Adds a reference (GpuDevice is Clone, crates/grafeo-cuda-core/src/device.rs:20-26). */
+gc_device* gc_device_retain(gc_device* dev);
+
+/* Drops a reference; the last one frees the stream and cached modules. */
+void gc_device_release(gc_device* dev);
+
+/* DeviceProperties {name, ordinal, total_memory}
+ * (crates/grafeo-cuda-core/src/device.rs:8-16,55-58). `name` receives a
+ * NUL-terminated string truncated to `name_cap` bytes. */
+gc_status gc_device_properties(const gc_device* dev, char* name, size_t name_cap,
+                               uint64_t* total_memory, uint32_t* ordinal);
+
+/* ------------------------------------------------------------------------ */
+/* CSR model                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Library-owned CSR arrays: CsrArrays (crates/grafeo-cuda-core/src/csr_export.rs:15-30). */
+typedef struct gc_csr {
+  uint32_t* offsets;  /* num_nodes + 1 entries */
+  uint32_t* targets;  /* num_edges entries */
+  uint32_t num_nodes;
+  uint32_t num_edges;
+} gc_csr;
+
+void gc_csr_free(gc_csr* csr);
+
+/* validate_csr (crates/grafeo-cuda-core/src/csr_export.rs:71-120): offsets
+ * non-empty, offsets[0] == 0, non-decreasing, offsets[n] == num_targets, both
+ * lengths fit u32, targets < n, each list strictly increasing. Returns
+ * GC_ERR_INVALID_CSR with the reference's message otherwise. */
+gc_status gc_validate_csr(const uint32_t* offsets, size_t num_offsets,
+                          const uint32_t* targets, size_t num_targets);
+
+/* CsrArrays::validate (crates/grafeo-cuda-core/src/csr_export.rs:37-54). */
+gc_status gc_csr_validate(const gc_csr* csr);
+
+/* from_edge_pairs (crates/grafeo-cuda-core/src/csr_export.rs:202-239).
+ * `pairs` holds num_pairs (src, dst) pairs interleaved. If `symmetric`, each
+ * pair also adds (dst, src) unless it is a self-loop; the result is sorted
+ * and deduplicated. A node id >= num_nodes is GC_ERR_INVALID_CSR. */
+gc_status gc_csr_from_edge_pairs(const uint32_t* pairs, size_t num_pairs, uint32_t num_nodes,
+                                 int symmetric, gc_csr* out);
+
+/* from_edge_pairs_oriented (crates/grafeo-cuda-core/src/csr_export.rs:299-350):
+ * drops loops, deduplicates undirected edges, orients each from the
+ * lower-degree endpoint to the higher (ties: lower id first). */
+gc_status gc_csr_from_edge_pairs_oriented(const uint32_t* pairs, size_t num_pairs,
+                                          uint32_t num_nodes, gc_csr* out);
+
+/* parse_tsv_to_edges (crates/grafeo-cuda-core/src/csr_export.rs:246-287).
+ * `*pairs` is library-allocated (gc_free); ids are remapped in order of first
+ * appearance. A missing file is GC_ERR_EXPORT. */
+gc_status gc_parse_tsv_to_edges(const char* path, uint32_t** pairs, size_t* num_pairs,
+                                uint32_t* num_nodes);
+
+/* ------------------------------------------------------------------------ */
+/* Graph in device memory                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gc_graph gc_graph;
+
+/* GpuGraph::from_csr_slices (crates/grafeo-cuda-core/src/gpu_graph.rs:66-87):
+ * validates, then copies both arrays to the device. The graph keeps a
+ * reference to `dev`. */
+gc_status gc_graph_from_csr(gc_device* dev, const uint32_t* offsets, size_t num_offsets,
+                            const uint32_t* targets, size_t num_targets, gc_graph** out);
+
+/* GpuGraph::from_csr_arrays (crates/grafeo-cuda-core/src/gpu_graph.rs:46-60). */
+gc_status gc_graph_from_csr_arrays(gc_device* dev, const gc_csr* csr, gc_graph** out);
+
+void gc_graph_free(gc_graph* g);
+
+/* GpuGraph.num_nodes / num_edges (crates/grafeo-cuda-core/src/gpu_graph.rs:20-23). */
+uint32_t gc_graph_num_nodes(const gc_graph* g);
+uint32_t gc_graph_num_edges(const gc_graph* g);
+
+/* Device addresses of the uploaded CSR (GpuGraph.offsets / .targets,
+ * crates/grafeo-cuda-core/src/gpu_graph.rs:16-19). */
+const uint32_t* gc_graph_device_offsets(const gc_graph* g);
+const uint32_t* gc_graph_device_targets(const gc_graph* g);
+
+/* Builds the graph's derived device layouts now instead of on first use: the
+ * degree-ordered relabelled CSR used by the symmetric kernels, and the
+ * symmetry check. No reference counterpart (the reference keeps only the
+ * uploaded arrays); `secs` is the preparation time. */
+gc_status gc_graph_prepare(gc_graph* g, double* secs);
+
+/* 1 if every entry u->v of the CSR has the entry v->u, else 0 (prepares). */
+gc_status gc_graph_is_symmetric(gc_graph* g, int32_t* symmetric);
+
+/* Sizes of the degree-oriented DAG the symmetric kernels run on (prepares):
+ * its entry count m_o (entries u->v whose (degree, id) rank is above u's) and
+ * the sum over those entries of (d+(u) + d+(v)), the list entries a
+ * sorted-intersection triangle count streams. Used to state the triangle
+ * kernels' algorithmic bytes. No reference counterpart. */
+gc_status gc_graph_dag_stats(gc_graph* g, uint64_t* oriented_entries, uint64_t* intersect_entries);
+
+/* ------------------------------------------------------------------------ */
+/* Algorithms                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* triangle_counting::triangle_count (crates/grafeo-cuda-algos/src/triangle_counting.rs:40-152):
+ * number of (u, v, w), u < v < w, with v and w in N(u) and w in N(v)
+ * (the triangle count of a symmetric CSR; self-loops never count). */
+gc_status gc_triangle_count(gc_graph* g, uint64_t* total_triangles, double* secs);
+
+/* triangle_counting::triangle_count_oriented
+ * (crates/grafeo-cuda-algos/src/triangle_counting.rs:162-210): the sum over
+ * CSR entries u->v of |N(u) intersect N(v)| (the triangle count of a
+ * degree-oriented CSR). */
+gc_status gc_triangle_count_oriented(gc_graph* g, uint64_t* total_triangles, double* secs);
+
+/* bfs::bfs (crates/grafeo-cuda-algos/src/bfs.rs:42-131): level-synchronous BFS
+ * along the CSR lists. `distances` (host, num_nodes entries) receives -1 for
+ * unreachable nodes. A source >= num_nodes is GC_ERR_KERNEL. */
+gc_status gc_bfs(gc_graph* g, uint32_t source, int32_t* distances, uint32_t* reachable,
+                 int32_t* max_depth, double* secs);
+
+/* As gc_bfs, with `d_distances` a device array of num_nodes int32. */
+gc_status gc_bfs_device(gc_graph* g, uint32_t source, int32_t* d_distances, uint32_t* reachable,
+                        int32_t* max_depth, double* secs);
+
+/* pagerank::pagerank (crates/grafeo-cuda-algos/src/pagerank.rs:46-199):
+ * r0 = 1/n; r'[v] = (1-d)/n + d*(sum_{u in N(v), deg u > 0} r[u]/deg u + D/n)
+ * with D the rank of nodes of degree 0; stops when the L1 change drops below
+ * `tolerance` or after `max_iterations`. FP32 ranks, FP64 reductions.
+ * `ranks` (host, num_nodes floats) receives the latest iterate. */
+gc_status gc_pagerank(gc_graph* g, float damping, uint32_t max_iterations, float tolerance,
+                      float* ranks, uint32_t* iterations, int32_t* converged, double* secs);
+
+/* As gc_pagerank, with `d_ranks` a device array of num_nodes floats. */
+gc_status gc_pagerank_device(gc_graph* g, float damping, uint32_t max_iterations, float tolerance,
+                             float* d_ranks, uint32_t* iterations, int32_t* converged,
+                             double* secs);
+
+/* k_truss::k_truss (crates/grafeo-cuda-algos/src/k_truss.rs:42-161): peels
+ * edges with fewer than k-2 supporting triangles until none is removed.
+ * `*edges` (library-allocated, gc_free) receives `*num_edges` pairs (u, v),
+ * u < v, in CSR order; `iterations` counts the final round that removes
+ * nothing. k < 3 is GC_ERR_KERNEL. */
+gc_status gc_k_truss(gc_graph* g, uint32_t k, uint32_t** edges, size_t* num_edges,
+                     uint32_t* iterations, double* secs);
+
+/* Device memory for the *_device entry points' outputs (hipMalloc on the
+ * device of `dev`). No reference counterpart. */
+gc_status gc_device_malloc(gc_device* dev, size_t bytes, void** ptr);
+gc_status gc_device_free(gc_device* dev, void* ptr);
+
+/* Copies `bytes` from device address `src` to host memory `dst`. */
+gc_status gc_device_copy_to_host(gc_device* dev, void* dst, const void* src, size_t bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel timing (hipEvent pairs around each launch)                         */
+/* ------------------------------------------------------------------------ */
+
+/* Turns per-kernel event timing on (1) or off (0) for launches on `dev`, and
+ * clears the accumulated figures. No reference counterpart: the reference
+ * times whole calls only (crates/grafeo-cuda-algos/src/harness.rs:13-36). */
+gc_status gc_profile_enable(gc_device* dev, int32_t on);
+
+/* Total milliseconds and launch count recorded for kernel `name` since the
+ * last gc_profile_enable. Waits for outstanding launches. */
+gc_status gc_profile_query(gc_device* dev, const char* name, double* total_ms, uint64_t* launches);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic graph generators (benchmark inputs)                             */
+/* ------------------------------------------------------------------------ */
+
+/* Erdos-Renyi-style multigraph draws: `num_draws` pairs (u, v) uniform over
+ * [0, n)^2 from SplitMix64(seed); self-loops are dropped, so *num_pairs <=
+ * num_draws. No reference counterpart (the reference loads TSV files). */
+gc_status gc_gen_erdos_renyi(uint32_t n, uint64_t num_draws, uint64_t seed, uint32_t** pairs,
+                             size_t* num_pairs);
+
+/* R-MAT draws: edge_factor * 2^scale pairs, quadrant probabilities (a, b, c,
+ * 1-a-b-c) per bit level; if `permute`, vertex ids pass through a seeded
+ * random permutation; self-loops are dropped. */
+gc_status gc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                      uint64_t seed, int32_t permute, uint32_t** pairs, size_t* num_pairs);
+
+/* rows x cols 4-neighbour grid, node id r*cols + c, each undirected edge once. */
+gc_status gc_gen_grid(uint32_t rows, uint32_t cols, uint32_t** pairs, size_t* num_pairs);
+
+/* ------------------------------------------------------------------------ */
+/* Build information                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Number of embedded gfx950 code objects, and for entry `i` its kernel-family
+ * name, the SHA-256 of the HIP source it was built from and of the code
+ * object itself. Replaces the embedded-PTX checks
+ * (crates/grafeo-cuda-algos/src/lib.rs:26-63). */
+uint32_t gc_code_object_count(void);
+gc_status gc_code_object_info(uint32_t i, const char** name, const char** source_sha256,
+                              const char** object_sha256, uint64_t* size_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAFEO_HIP_H */

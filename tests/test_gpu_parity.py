@@ -1,0 +1,141 @@
+"""GPU results against the CPU oracle on the same CSR.
+
+Integer results (triangle counts, BFS distances, k-truss edge sets and round counts) must be
+bit-exact; PageRank (FP32 ranks, FP64 reductions) must be within L1 <= 1e-6 of the FP64 oracle
+run for the same number of iterations (the tolerance of the graph-kernel path's target).
+Comparisons follow crates/grafeo-cuda-algos/tests/correctness.rs:234-326 (BFS from every source,
+k = 3..8 for k-truss, triangle counts through both the symmetric and the oriented builder).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import grafeo_cuda
+import oracle
+from grafeo_cuda import _native
+from graphs import catalog, er_graph, random_graph, rmat_graph, undirected
+
+pytestmark = pytest.mark.gpu
+
+PR_L1_TOL = 1e-6
+
+
+def upload(device, off, tgt):
+    return grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+
+
+def check_triangles(name, device, off, tgt, g):
+    expected = oracle.triangle_count(off, tgt)
+    assert g.triangle_count()["total_triangles"] == expected, f"{name}: triangle_count"
+    n = len(off) - 1
+    pairs = [(u, int(v)) for u in range(n) for v in tgt[off[u]:off[u + 1]] if v > u]
+    o_off, o_tgt = _native.from_edge_pairs_oriented(pairs, n)
+    og = upload(device, o_off, o_tgt)
+    assert og.triangle_count_oriented()["total_triangles"] == expected, f"{name}: oriented"
+    assert og.triangle_count_oriented()["total_triangles"] == oracle.triangle_count_oriented(o_off, o_tgt)
+
+
+def check_bfs(name, off, tgt, g, source):
+    dist, reach, depth = oracle.bfs(off, tgt, source)
+    r = g.bfs_numpy(source)
+    assert np.array_equal(r["distances"], dist), f"{name}: BFS from {source}"
+    assert r["reachable"] == reach, f"{name}: reachable from {source}"
+    assert r["max_depth"] == depth, f"{name}: max depth from {source}"
+
+
+def check_pagerank(name, off, tgt, g):
+    r = g.pagerank_numpy(0.85, 1000, 1e-7)
+    n = len(off) - 1
+    assert len(r["ranks"]) == n
+    if n == 0:
+        return
+    ref, it, _ = oracle.pagerank(off, tgt, 0.85, r["iterations"], 0.0)
+    assert it == r["iterations"]
+    l1 = float(np.abs(r["ranks"].astype(np.float64) - ref).sum())
+    assert l1 <= PR_L1_TOL, f"{name}: L1 {l1}"
+    fixed, _, conv = oracle.pagerank(off, tgt, 0.85, 1000, 1e-12)
+    assert np.max(np.abs(r["ranks"] - fixed)) < 1e-4, f"{name}: far from the fixed point"
+    assert abs(float(r["ranks"].astype(np.float64).sum()) - 1.0) < 1e-3
+
+
+def check_k_truss(name, off, tgt, g, monkeypatch, ks=range(3, 9)):
+    for k in ks:
+        edges, iterations = oracle.k_truss(off, tgt, k)
+        for direct in ("0", "1"):
+            monkeypatch.setenv("GC_KTRUSS_DIRECT", direct)
+            r = g.k_truss(k)
+            assert r["edges"] == edges, f"{name}: {k}-truss (direct={direct})"
+            assert r["iterations"] == iterations, f"{name}: {k}-truss rounds (direct={direct})"
+            assert r["k"] == k
+    monkeypatch.delenv("GC_KTRUSS_DIRECT")
+
+
+def test_catalog(device, monkeypatch):
+    for name, off, tgt in catalog():
+        g = upload(device, off, tgt)
+        check_triangles(name, device, off, tgt, g)
+        for source in range(len(off) - 1):
+            check_bfs(name, off, tgt, g, source)
+        check_pagerank(name, off, tgt, g)
+        check_k_truss(name, off, tgt, g, monkeypatch)
+
+
+def test_random_graphs(device, monkeypatch):
+    """The reference's proptest (48 cases, n < 16, <= 48 pairs with loops), run 96 times."""
+    rng = np.random.default_rng(20240611)
+    cases = [(1, [])] + [random_graph(rng) for _ in range(95)]
+    for case, (n, edges) in enumerate(cases):
+        off, tgt = undirected(n, edges)
+        g = upload(device, off, tgt)
+        name = f"random #{case} (n={n}, edges={edges})"
+        check_triangles(name, device, off, tgt, g)
+        check_bfs(name, off, tgt, g, 0)
+        check_bfs(name, off, tgt, g, n - 1)
+        check_pagerank(name, off, tgt, g)
+        check_k_truss(name, off, tgt, g, monkeypatch, ks=(3, 4, 5))
+
+
+def test_directed_csr_semantics(device, monkeypatch):
+    """Non-symmetric CSRs take the reference's per-entry definitions (no relabelled DAG)."""
+    rng = np.random.default_rng(7)
+    for case in range(40):
+        n, edges = random_graph(rng, max_nodes=14, max_edges=40)
+        off, tgt = _native.from_edge_pairs(edges, n, False)
+        g = upload(device, off, tgt)
+        name = f"directed #{case}"
+        assert g.triangle_count()["total_triangles"] == oracle.triangle_count(off, tgt), name
+        assert g.triangle_count_oriented()["total_triangles"] == oracle.triangle_count_oriented(off, tgt), name
+        for source in range(n):
+            check_bfs(name, off, tgt, g, source)
+        r = g.pagerank_numpy(0.85, 30, 0.0)
+        ref, _, _ = oracle.pagerank(off, tgt, 0.85, 30, 0.0)
+        assert float(np.abs(r["ranks"] - ref).sum()) <= PR_L1_TOL, name
+        for k in (3, 4):
+            edges_o, it_o = oracle.k_truss(off, tgt, k)
+            r = g.k_truss(k)
+            assert (r["edges"], r["iterations"]) == (edges_o, it_o), f"{name} k={k}"
+
+
+def test_er_4096(device, monkeypatch):
+    """Triangle count, plain method: Erdos-Renyi n=4096, average degree 16."""
+    off, tgt = er_graph(4096, 16, seed=1)
+    g = upload(device, off, tgt)
+    check_triangles("ER-4096", device, off, tgt, g)
+    for source in (0, 17, 4095):
+        check_bfs("ER-4096", off, tgt, g, source)
+    check_pagerank("ER-4096", off, tgt, g)
+    check_k_truss("ER-4096", off, tgt, g, monkeypatch, ks=(3, 4))
+
+
+@pytest.mark.parametrize("scale", [10, 14])
+def test_rmat_small(device, monkeypatch, scale):
+    off, tgt = rmat_graph(scale, 16, seed=scale)
+    g = upload(device, off, tgt)
+    check_triangles(f"RMAT-{scale}", device, off, tgt, g)
+    n = len(off) - 1
+    for source in np.random.default_rng(scale).integers(0, n, size=4):
+        check_bfs(f"RMAT-{scale}", off, tgt, g, int(source))
+    check_pagerank(f"RMAT-{scale}", off, tgt, g)
+    check_k_truss(f"RMAT-{scale}", off, tgt, g, monkeypatch, ks=(3, 4, 6))
