@@ -87,8 +87,8 @@ def main() -> int:
     g = grafeo_cuda.GpuGraph.from_csr(offsets, targets, dev)
     upload_s = time.time() - t0
     prepare_s = g.prepare()
-    m_o, inter = c_uint64(), c_uint64()
-    check(lib.gc_graph_dag_stats(g._handle, byref(m_o), byref(inter)))
+    m_o, inter, wedges = c_uint64(), c_uint64(), c_uint64()
+    check(lib.gc_graph_dag_stats(g._handle, byref(m_o), byref(inter), byref(wedges)))
     log(f"upload {upload_s:.2f}s, prepare {prepare_s:.2f}s, DAG entries {m_o.value}")
 
     sources = pick_sources(offsets, args.sources, args.seed)
@@ -125,8 +125,9 @@ def main() -> int:
     dev.profile(True)
     step()
     kernels = {}
-    for name in ("tc_count", "pr_spmv", "pr_finalize", "bfs_topdown", "bfs_bottomup",
-                 "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "gather_u32", "fill_u32", "pr_init"):
+    for name in ("tc_wedge", "tc_count", "pr_spmv", "pr_finalize", "bfs_topdown_edges", "bfs_frontier_degrees",
+                 "bfs_bottomup", "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "scan_block_sums",
+                 "exclusive_scan_small", "scan_apply", "gather_u32", "fill_u32", "pr_init"):
         ms, count = dev.kernel_time(name)
         if count:
             kernels[name] = {"total_ms": ms, "launches": count, "avg_us": 1000.0 * ms / count}
@@ -147,19 +148,34 @@ def main() -> int:
     }
 
     # Algorithmic bytes per launch (HBM roofline for every kernel here).
+    # PageRank SpMV per iteration: offsets + targets + x read once + y written once.
     spmv_bytes = 4 * (n + 1) + 4 * m + 4 * n + 4 * n
-    tc_bytes = 4 * (n + 1) + 4 * m_o.value + 4 * inter.value
+    # Middle-vertex triangle count: offsets and DAG split (fwd) read once, every CSR entry
+    # read once (in-lists and out-lists), plus one 4-byte entry per DAG wedge streamed.
+    tc_bytes = 4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value
+    # BFS per traversal: offsets + targets read once, distances written once.
+    bfs_bytes = 4 * (n + 1) + 4 * m + 4 * n
     rooflines = {}
+
+    def add_roofline(name, bytes_per_launch, avg_us):
+        a = bytes_per_launch / (avg_us * 1e-6) / 1e9
+        rooflines[name] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_per_launch,
+                           "avg_launch_us": avg_us}
+
     if "pr_spmv" in kernels:
-        a = spmv_bytes / (kernels["pr_spmv"]["avg_us"] * 1e-6) / 1e9
-        rooflines["pr_spmv"] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": spmv_bytes}
-    if "tc_count" in kernels:
-        a = tc_bytes / (kernels["tc_count"]["avg_us"] * 1e-6) / 1e9
-        rooflines["tc_count"] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                 "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": tc_bytes}
+        add_roofline("pr_spmv", spmv_bytes, kernels["pr_spmv"]["avg_us"])
+    if "tc_wedge" in kernels:
+        add_roofline("tc_wedge", tc_bytes, kernels["tc_wedge"]["avg_us"])
+    bfs_kernels = [k for k in kernels if k.startswith("bfs_") or k in ("scan_block_sums", "scan_apply")]
+    if bfs_kernels:
+        per_traversal_us = 1000.0 * sum(kernels[k]["total_ms"] for k in bfs_kernels) / len(sources)
+        add_roofline("bfs_traversal", bfs_bytes, per_traversal_us)
+    # The dominant kernel: the one with the most event time in the profiled step.
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
-    roofline = rooflines.get(dominant) or rooflines.get("pr_spmv")
+    roofline = dict(rooflines.get(dominant) or rooflines.get("pr_spmv") or {})
+    if roofline:
+        roofline["kernel"] = dominant if dominant in rooflines else "pr_spmv"
 
     parity = {}
     try:

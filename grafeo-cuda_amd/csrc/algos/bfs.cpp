@@ -10,8 +10,12 @@
 // edges exceed 1/alpha of the unvisited vertices' edges, back top-down when
 // the frontier holds fewer than n/beta vertices (alpha = 14, beta = 24).
 // Distances do not depend on the schedule, so both directions give the
-// reference's result bit for bit.
+// reference's result bit for bit. A bottom-up step looks for parents in a
+// vertex's own list, which holds its in-neighbours only when the CSR is
+// symmetric; other CSRs stay top-down.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 #include "../runtime/common.hpp"
 
@@ -55,7 +59,10 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
 
   const uint32_t src = g.rl.host_inv[source];
   Kernel k_init = get_kernel(dev, "bfs", "bfs_init");
-  Kernel k_td = get_kernel(dev, "bfs", "bfs_topdown");
+  Kernel k_td = get_kernel(dev, "bfs", "bfs_topdown_edges");
+  Kernel k_deg = get_kernel(dev, "bfs", "bfs_frontier_degrees");
+  uint32_t* start = g.buffer<uint32_t>("bfs.start", sizeof(uint32_t) * n);
+  uint32_t* scan_buf = g.buffer<uint32_t>("bfs.scan", sizeof(uint32_t) * scan_scratch_words(n));
   Kernel k_bu = get_kernel(dev, "bfs", "bfs_bottomup");
   Kernel k_q2b = get_kernel(dev, "bfs", "bfs_queue_to_bitmap");
   Kernel k_b2q = get_kernel(dev, "bfs", "bfs_bitmap_to_queue");
@@ -71,8 +78,11 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
   int32_t level = 0;
   uint64_t reachable = 1;
   BfsOutcome outcome;
+  const char* trace_env = std::getenv("GC_BFS_TRACE");
+  const bool trace = trace_env != nullptr && trace_env[0] == '1';
+  Stopwatch level_clock;
   for (;;) {
-    if (top_down && static_cast<double>(m_f) > static_cast<double>(m_u) / kAlpha) {
+    if (top_down && g.symmetric && static_cast<double>(m_f) > static_cast<double>(m_u) / kAlpha) {
       fill_words(dev, bitmap[cur], 2 * words64, 0);
       Grid1D qg = thread_per_item(n_f);
       launch(dev, k_q2b, qg.blocks, qg.threads, 0, static_cast<const uint32_t*>(queue[cur]),
@@ -88,9 +98,16 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
     fill_words(dev, counters, sizeof(BfsCounters) / 4, 0);
     const int32_t next_level = level + 1;
     if (top_down) {
-      Grid1D tg = wave_per_item((n_f + 63) / 64);
+      // Edge-balanced expansion: prefix sum of the frontier's degrees, then
+      // 256 consecutive frontier edges per wave.
+      Grid1D qg = thread_per_item(n_f);
+      launch(dev, k_deg, qg.blocks, qg.threads, 0, off, static_cast<const uint32_t*>(queue[cur]),
+             static_cast<uint32_t>(n_f), start);
+      exclusive_scan_u32(dev, start, n_f, scan_buf);
+      Grid1D tg = wave_per_item((m_f + 255) / 256);
       launch(dev, k_td, std::min(tg.blocks, cu * 32), tg.threads, 0, off, tgt, dist,
-             static_cast<const uint32_t*>(queue[cur]), static_cast<uint32_t>(n_f), queue[cur ^ 1],
+             static_cast<const uint32_t*>(queue[cur]), static_cast<uint32_t>(n_f),
+             static_cast<const uint32_t*>(start), static_cast<uint32_t>(m_f), queue[cur ^ 1],
              counters, next_level);
     } else {
       Grid1D bg = wave_per_item(words64);
@@ -102,6 +119,13 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
     BfsCounters c;
     copy_to_host(dev, dev.pinned, counters, sizeof(BfsCounters));
     std::memcpy(&c, dev.pinned, sizeof(c));
+    if (trace) {
+      std::fprintf(stderr, "bfs level %d %s frontier %llu edges %llu -> next %u edges %llu (%.3f ms)\n",
+                   level, top_down ? "top-down " : "bottom-up", static_cast<unsigned long long>(n_f),
+                   static_cast<unsigned long long>(m_f), c.next_size, c.next_edges,
+                   level_clock.seconds() * 1e3);
+      level_clock = Stopwatch();
+    }
     if (c.next_size == 0) break;
     level = next_level;
     reachable += c.next_size;

@@ -104,17 +104,18 @@ uint32_t rounds_dag(Graph& g, uint32_t threshold, uint8_t* flags) {
   uint8_t* removed = g.buffer<uint8_t>("kt.removed", (static_cast<size_t>(m) + 3) / 4 * 4);
   uint32_t* count = g.buffer<uint32_t>("kt.count", 16);  // [0] removed, [1..2] work counters
   fill_words(dev, removed, (static_cast<uint64_t>(m) + 3) / 4, 0);
-  Kernel k_support = get_kernel(dev, "ktruss", "kt_dag_support");
+  Kernel k_support = get_kernel(dev, "ktruss", "kt_wedge_support");
   Kernel k_finish = get_kernel(dev, "ktruss", "kt_dag_finish");
-  const uint32_t heavy = std::min<uint32_t>(n, 65536);
-  const uint32_t light_chunk = 64;
-  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(std::max(1, dev.cu_count) * 32u, n));
+  const uint32_t bits = wedge_table_bits(g.rl);
+  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+  const uint32_t hub_chunk = Relabelled::kHubChunk;
   uint32_t iterations = 0;
   for (;;) {
     fill_words(dev, support, m, 0);
     fill_words(dev, count, 4, 0);
-    launch(dev, k_support, grid, kWaveSize, 0, off, tgt, fwd, static_cast<const uint8_t*>(removed),
-           n, heavy, light_chunk, count + 1, support);
+    launch(dev, k_support, grid, kWaveSize, (1u << bits) * 8u, off, tgt, fwd,
+           static_cast<const uint8_t*>(removed), n, g.rl.hub_units.as<const uint32_t>(),
+           g.rl.num_hub_units, hub_chunk, bits, count + 1, support);
     const uint32_t round1 = iterations == 0 ? 1 : 0;
     launch(dev, k_finish, vg.blocks, vg.threads, 0, off, tgt, fwd,
            g.ktruss_rev.as<const uint32_t>(), removed, n, static_cast<const uint32_t*>(support),

@@ -8,6 +8,8 @@
 // reference sums in FP32 with FP32 atomics), and the convergence test runs on
 // the device, the host reading the state once per batch of iterations.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "../runtime/common.hpp"
 
@@ -15,7 +17,8 @@ namespace gc {
 
 namespace {
 
-constexpr uint32_t kPrThreads = 256;
+constexpr uint32_t kPrThreads = 256;     // pr_spmv workgroup
+constexpr uint32_t kPrHotThreads = 1024; // pr_spmv_hot workgroup
 constexpr uint32_t kPrMaxSegments = 10;
 constexpr uint32_t kPrBatch = 8;  // iterations launched between state reads
 
@@ -39,11 +42,14 @@ struct PrState {
   uint32_t done;
 };
 
-// Cuts the degree-sorted rows into segments of one lane-group size each.
-PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n) {
+// Cuts the degree-sorted rows into segments of one lane-group size each; a
+// segment's rows are handed out in tiles of block_threads/group rows (one row
+// per tile for the last segment, whose group is the whole workgroup).
+PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t block_threads) {
   // Degree thresholds and the group size used below each.
   static const uint32_t kLimit[] = {2, 4, 8, 16, 32, 64, 1024, 0xFFFFFFFFu};
-  static const uint32_t kGroup[] = {1, 2, 4, 8, 16, 32, 64, 256};
+  const uint32_t kGroup[] = {1, 2, 4, 8, 16, 32, 64, block_threads};
+  const uint32_t kPrThreads = block_threads;
   PrSegments s{};
   uint32_t row = 0, block = 0;
   auto degree = [&](uint32_t i) { return off[i + 1] - off[i]; };
@@ -74,18 +80,39 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n) {
 
 }  // namespace
 
+// Which SpMV runs: the persistent LDS-hot kernel (default) or one workgroup
+// per row tile (GC_PR_KERNEL=rows).
+static bool use_row_kernel() {
+  const char* env = std::getenv("GC_PR_KERNEL");
+  return env != nullptr && std::strcmp(env, "rows") == 0;
+}
+
 void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance, float* d_out,
               uint32_t* iterations, int32_t* converged) {
   Device& dev = *g.dev;
   prepare(g);
   const uint32_t n = g.n;
-  const PrSegments segs = plan_segments(g.rl.host_offsets, n);
+  const bool rows_kernel = use_row_kernel();
+  const uint32_t block_threads = rows_kernel ? kPrThreads : kPrHotThreads;
+  const PrSegments segs = plan_segments(g.rl.host_offsets, n, block_threads);
+  // Persistent kernel: as many 1024-lane workgroups as fit at once; the hot
+  // set is what the LDS of one workgroup holds (leaving room for the
+  // reduction scratch), capped at n.
+  const uint32_t lds = static_cast<uint32_t>(std::min<size_t>(dev.max_lds_per_block, 160 * 1024));
+  uint32_t hot_count = lds >= 132 * 1024 ? 32768u : (lds >= 66 * 1024 ? 16384u : 8192u);
+  hot_count = std::min({hot_count, n, dev.pr_hot_count_limit});
+  const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
+  auto hot_grid = [&](uint32_t hot) { return cu * (hot * 4u > 80u * 1024u ? 1u : 2u); };
+  uint32_t grid_blocks = rows_kernel ? segs.total_blocks : hot_grid(hot_count);
   float* rank[2] = {g.buffer<float>("pr.rank0", sizeof(float) * n),
                     g.buffer<float>("pr.rank1", sizeof(float) * n)};
   float* contrib[2] = {g.buffer<float>("pr.contrib0", sizeof(float) * n),
                        g.buffer<float>("pr.contrib1", sizeof(float) * n)};
-  double* partials = g.buffer<double>("pr.partials", sizeof(double) * 2 * std::max(1u, segs.total_blocks));
+  double* partials = g.buffer<double>(
+      "pr.partials", sizeof(double) * 2 * std::max({1u, segs.total_blocks, 2 * cu}));
   PrState* state = g.buffer<PrState>("pr.state", sizeof(PrState));
+  uint32_t* work = g.buffer<uint32_t>("pr.work", 16);
+  fill_words(dev, work, 4, 0);
 
   const float r0 = 1.0f / static_cast<float>(n);
   const uint32_t* off = g.rl.offsets.as<const uint32_t>();
@@ -96,18 +123,36 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   PrState init{static_cast<double>(g.rl.dangling) * static_cast<double>(r0), 0.0, 0, 0};
   copy_to_device(dev, state, &init, sizeof(init));
 
-  Kernel spmv = get_kernel(dev, "pagerank", "pr_spmv");
+  Kernel spmv = get_kernel(dev, "pagerank", rows_kernel ? "pr_spmv" : "pr_spmv_hot");
   Kernel finalize = get_kernel(dev, "pagerank", "pr_finalize");
   const double d = static_cast<double>(damping);
   const double tol = static_cast<double>(tolerance);
   PrState host_state = init;
   for (uint32_t it = 0; it < max_iterations; ++it) {
     const int in = it & 1, out = in ^ 1;
-    launch(dev, spmv, segs.total_blocks, kPrThreads, 0, off, tgt,
-           static_cast<const float*>(contrib[in]), static_cast<const float*>(rank[in]), contrib[out],
-           rank[out], n, d, segs, static_cast<const PrState*>(state), partials);
-    launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials),
-           segs.total_blocks, tol);
+    if (rows_kernel) {
+      launch(dev, spmv, segs.total_blocks, kPrThreads, 0, off, tgt,
+             static_cast<const float*>(contrib[in]), static_cast<const float*>(rank[in]),
+             contrib[out], rank[out], n, d, segs, static_cast<const PrState*>(state), partials);
+    } else {
+      for (;;) {
+        hipError_t err = try_launch(dev, spmv, grid_blocks, kPrHotThreads, hot_count * 4u, off, tgt,
+                                    static_cast<const float*>(contrib[in]),
+                                    static_cast<const float*>(rank[in]), contrib[out], rank[out], n,
+                                    d, segs, static_cast<const PrState*>(state), work, hot_count,
+                                    partials);
+        if (err == hipSuccess) break;
+        // A device may refuse the dynamic LDS size: halve the hot set and retry.
+        if (hot_count <= 1024) {
+          fail_kernel(std::string("pr_spmv_hot launch failed: ") + hipGetErrorString(err));
+        }
+        hot_count /= 2;
+        dev.pr_hot_count_limit = hot_count;
+        grid_blocks = hot_grid(hot_count);
+      }
+    }
+    launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials), grid_blocks,
+           tol, work);
     if ((it + 1) % kPrBatch == 0 || it + 1 == max_iterations) {
       copy_to_host(dev, dev.pinned, state, sizeof(PrState));
       std::memcpy(&host_state, dev.pinned, sizeof(PrState));

@@ -5,6 +5,8 @@
 // same early return for an edgeless graph, no forward-edge ceiling (the
 // reference refuses more than 134,217,727 forward edges, launch.rs:27-41).
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "../runtime/common.hpp"
 
@@ -37,12 +39,64 @@ uint64_t run_tc_count(Graph& g, const uint32_t* lo, const uint32_t* hi, const ui
 
 }  // namespace
 
+// Triangles of a symmetric graph by enumeration at the middle vertex of the
+// relabelled DAG (tc_wedge).
+uint64_t run_tc_wedge(Graph& g) {
+  Device& dev = *g.dev;
+  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", 16);
+  fill_words(dev, scratch, 4, 0);
+  const uint32_t bits = wedge_table_bits(g.rl);
+  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+  auto* result = reinterpret_cast<unsigned long long*>(scratch);
+  const uint32_t hub_chunk = Relabelled::kHubChunk;
+  launch(dev, get_kernel(dev, "tc", "tc_wedge"), grid, kWaveSize, (1u << bits) * 4u,
+         g.rl.offsets.as<const uint32_t>(), g.rl.targets.as<const uint32_t>(),
+         g.rl.fwd.as<const uint32_t>(), g.n, g.rl.hub_units.as<const uint32_t>(), g.rl.num_hub_units,
+         hub_chunk, bits, scratch + 2, result);
+  uint64_t total = 0;
+  copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
+  std::memcpy(&total, dev.pinned, sizeof(uint64_t));
+  return total;
+}
+
+// Triangles of a symmetric graph by hash probes of each DAG row (tc_hash).
+uint64_t run_tc_hash(Graph& g) {
+  Device& dev = *g.dev;
+  const uint32_t n = g.n;
+  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", 16);
+  fill_words(dev, scratch, 4, 0);
+  const uint32_t heavy = std::min<uint32_t>(n, 65536);
+  const uint32_t light_chunk = 64;
+  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+  auto* result = reinterpret_cast<unsigned long long*>(scratch);
+  launch(dev, get_kernel(dev, "tc", "tc_hash"), grid, 4 * kWaveSize, 0,
+         g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
+         g.rl.targets.as<const uint32_t>(), n, heavy, light_chunk, scratch + 2, result);
+  uint64_t total = 0;
+  copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
+  std::memcpy(&total, dev.pinned, sizeof(uint64_t));
+  return total;
+}
+
+// Which symmetric-graph method runs (GC_TC_METHOD): "hash" (default, tc_hash),
+// "lists" (binary-search intersection, tc_count) or "wedge" (enumeration at
+// the middle vertex, tc_wedge). All three count the same triangles.
+static const char* tc_method() {
+  const char* env = std::getenv("GC_TC_METHOD");
+  return env != nullptr ? env : "hash";
+}
+
 uint64_t triangle_count_symmetric(Graph& g) {
   prepare(g);
   if (g.symmetric) {
-    // Degree-oriented DAG of the relabelled graph, hubs first.
-    return run_tc_count(g, g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
-                        g.rl.targets.as<const uint32_t>(), true, true);
+    const char* method = tc_method();
+    if (std::strcmp(method, "wedge") == 0) return run_tc_wedge(g);
+    if (std::strcmp(method, "lists") == 0) {
+      // Degree-oriented DAG of the relabelled graph, hubs first.
+      return run_tc_count(g, g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
+                          g.rl.targets.as<const uint32_t>(), true, true);
+    }
+    return run_tc_hash(g);
   }
   // Any other CSR: the suffix of each list above its row, in the caller's ids.
   ensure_fwd_original(g);

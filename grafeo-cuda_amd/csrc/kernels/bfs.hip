@@ -102,12 +102,17 @@ extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
       const uint32_t v = static_cast<uint32_t>(v64);
       if (dist[v] < 0) {
         const uint32_t b = offsets[v], e = offsets[v + 1];
-        for (uint32_t i = e; i > b; --i) {
-          const uint32_t u = targets[i - 1];
-          if ((front[u >> 5] >> (u & 31)) & 1u) {
-            found = true;
-            break;
-          }
+        // four candidates per step, their loads issued together
+        for (uint32_t i = e; i > b && !found;) {
+          const uint32_t cnt = min(4u, i - b);
+          uint32_t u[4], word[4];
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) u[k] = k < cnt ? targets[i - 1 - k] : 0u;
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) word[k] = k < cnt ? front[u[k] >> 5] : 0u;
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) found |= k < cnt && ((word[k] >> (u[k] & 31)) & 1u);
+          i -= cnt;
         }
         if (found) {
           dist[v] = next_level;
@@ -123,6 +128,68 @@ extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
         atomicAdd(&counters->next_size, static_cast<uint32_t>(__popcll(mask)));
         atomicAdd(&counters->next_edges, edges);
       }
+    }
+  }
+}
+
+// degree[i] = degree of queue[i] (input of the frontier's edge prefix sum).
+extern "C" __global__ __launch_bounds__(256) void bfs_frontier_degrees(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ queue, uint32_t size,
+    uint32_t* __restrict__ degree) {
+  for (uint64_t i = global_thread(); i < size; i += grid_threads()) {
+    const uint32_t u = queue[i];
+    degree[i] = offsets[u + 1] - offsets[u];
+  }
+}
+
+#define TD_EDGES_PER_WAVE 256
+
+// Top-down step balanced over the frontier's edges: `start` is the exclusive
+// prefix sum of the frontier's degrees (`total` edges in all). Each wave takes
+// TD_EDGES_PER_WAVE consecutive edge ranks; a lane finds the frontier vertex
+// owning its edge by binary search in `start`, restricted to the owners of
+// the wave's first and last edge, so a hub's edges spread over every CU.
+extern "C" __global__ __launch_bounds__(256) void bfs_topdown_edges(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    int32_t* __restrict__ dist, const uint32_t* __restrict__ queue, uint32_t queue_size,
+    const uint32_t* __restrict__ start, uint32_t total, uint32_t* __restrict__ next_queue,
+    BfsCounters* __restrict__ counters, int32_t next_level) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t base = (global_thread() / GC_WAVE) * TD_EDGES_PER_WAVE; base < total;
+       base += waves * TD_EDGES_PER_WAVE) {
+    const uint32_t first = static_cast<uint32_t>(base);
+    const uint32_t last = static_cast<uint32_t>(min(base + TD_EDGES_PER_WAVE, static_cast<uint64_t>(total)) - 1);
+    // owners of the first and last edge of this wave's range (largest j with start[j] <= t)
+    uint32_t probe = lane == 0 ? first : last;
+    uint32_t lo = 0, hi = queue_size - 1;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+      if (start[mid] <= probe) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t own_lo = __shfl(lo, 0);
+    const uint32_t own_hi = __shfl(lo, GC_WAVE - 1);
+    for (uint32_t k = 0; k < TD_EDGES_PER_WAVE; k += GC_WAVE) {
+      const uint32_t t = first + k + lane;
+      const bool active = t <= last;
+      bool discovered = false;
+      uint32_t v = 0, deg = 0;
+      if (active) {
+        uint32_t a = own_lo, b = own_hi;
+        while (a < b) {
+          const uint32_t mid = a + ((b - a + 1) >> 1);
+          if (start[mid] <= t) a = mid;
+          else b = mid - 1;
+        }
+        const uint32_t u = queue[a];
+        v = targets[offsets[u] + (t - start[a])];
+        if (dist[v] < 0) {
+          discovered = atomicCAS(&dist[v], -1, next_level) == -1;
+          if (discovered) deg = offsets[v + 1] - offsets[v];
+        }
+      }
+      bfs_append(discovered, v, deg, next_queue, counters);
     }
   }
 }

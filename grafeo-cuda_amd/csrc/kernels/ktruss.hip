@@ -10,18 +10,17 @@
 //  - direct (any CSR, the reference's per-entry definition): one thread per
 //    forward entry merges the two lists;
 //  - DAG (symmetric CSR, degree-ordered relabelling): every triangle is found
-//    once from its lowest vertex with the wave intersection of tc.hip and adds
-//    1 to the support of its three (forward) edges. On a symmetric graph this
+//    once, at its middle vertex (wedge.hpp), and adds 1 to the support of its
+//    three (forward) edges. On a symmetric graph this
 //    equals the reference's support except in round 1, where the reference's
 //    merge also counts w = u when u has an active self-loop and w = v when v
 //    has one (CHANGELOG.md:76-77); kt_dag_finish adds those two terms in round
 //    1. Self-loops get support 0 and are removed in round 1, so later rounds
 //    need no correction.
 #include "device.hpp"
+#include "wedge.hpp"
 
 using namespace gcd;
-
-#define KT_LDS_CAP 1024
 
 // ---------------------------------------------------------------- direct --
 
@@ -91,110 +90,15 @@ extern "C" __global__ __launch_bounds__(256) void kt_reverse_index(
   }
 }
 
-// Adds 1 to the support of each edge of every triangle whose three DAG
-// entries are active. Same wave layout and scheduling as tc_count.
-extern "C" __global__ __launch_bounds__(64) void kt_dag_support(
+// Adds 1 to the support of each forward entry of every triangle whose three
+// DAG entries are active (enumeration at the middle vertex, see wedge.hpp).
+extern "C" __global__ __launch_bounds__(64) void kt_wedge_support(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ fwd, const uint8_t* __restrict__ removed, uint32_t n,
-    uint32_t heavy, uint32_t light_chunk, uint32_t* __restrict__ counters,
-    uint32_t* __restrict__ support) {
-  __shared__ uint32_t s_a[KT_LDS_CAP];
-  __shared__ uint32_t s_pos[GC_WAVE];
-  __shared__ uint32_t s_start[GC_WAVE];
-  __shared__ uint32_t s_range[2];
-  const uint32_t lane = threadIdx.x;
-  bool heavy_left = heavy > 0;
-
-  for (;;) {
-    if (lane == 0) {
-      uint32_t kb = n, ke = n;
-      if (heavy_left) {
-        uint32_t k = atomicAdd(&counters[0], 1u);
-        if (k < heavy) {
-          kb = k;
-          ke = k + 1;
-        } else {
-          heavy_left = false;
-        }
-      }
-      if (!heavy_left && kb == n) {
-        uint32_t c = atomicAdd(&counters[1], light_chunk);
-        uint64_t b = static_cast<uint64_t>(heavy) + c;
-        if (b < n) {
-          kb = static_cast<uint32_t>(b);
-          uint64_t e = b + light_chunk;
-          ke = e < n ? static_cast<uint32_t>(e) : n;
-        }
-      }
-      s_range[0] = kb;
-      s_range[1] = ke;
-    }
-    __syncthreads();
-    const uint32_t kb = s_range[0], ke = s_range[1];
-    __syncthreads();
-    if (kb >= n) break;
-
-    for (uint32_t k = kb; k < ke; ++k) {
-      const uint32_t u = n - 1 - k;
-      const uint32_t a_lo = fwd[u];
-      const uint32_t a = offsets[u + 1] - a_lo;
-      if (a == 0) continue;
-      const uint32_t* A = targets + a_lo;
-      const bool in_lds = a <= KT_LDS_CAP;
-      if (in_lds) {
-        for (uint32_t j = lane; j < a; j += GC_WAVE) s_a[j] = A[j];
-      }
-      __syncthreads();
-      for (uint32_t c = 0; c < a; c += GC_WAVE) {
-        const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - c);
-        uint32_t len = 0, st = 0;
-        if (lane < nv && !removed[a_lo + c + lane]) {
-          const uint32_t v = in_lds ? s_a[c + lane] : A[c + lane];
-          st = fwd[v];
-          len = offsets[v + 1] - st;
-        }
-        const uint32_t incl = wave_inclusive_scan(len);
-        s_pos[lane] = incl;
-        s_start[lane] = st;
-        const uint32_t total = __shfl(incl, GC_WAVE - 1);
-        __syncthreads();
-        for (uint32_t t = lane; t < total; t += GC_WAVE) {
-          uint32_t l = 0, h = nv - 1;
-          while (l < h) {
-            uint32_t mid = (l + h) >> 1;
-            if (s_pos[mid] > t) h = mid;
-            else l = mid + 1;
-          }
-          const uint32_t excl = l ? s_pos[l - 1] : 0;
-          const uint32_t e2 = s_start[l] + (t - excl);
-          if (removed[e2]) continue;
-          const uint32_t w = targets[e2];
-          uint32_t from = c + l + 1, to = a;
-          if (in_lds) {
-            while (from < to) {
-              uint32_t mid = (from + to) >> 1;
-              if (s_a[mid] < w) from = mid + 1;
-              else to = mid;
-            }
-            if (!(from < a && s_a[from] == w)) continue;
-          } else {
-            while (from < to) {
-              uint32_t mid = (from + to) >> 1;
-              if (A[mid] < w) from = mid + 1;
-              else to = mid;
-            }
-            if (!(from < a && A[from] == w)) continue;
-          }
-          const uint32_t e3 = a_lo + from;
-          if (removed[e3]) continue;
-          atomicAdd(&support[a_lo + c + l], 1u);
-          atomicAdd(&support[e2], 1u);
-          atomicAdd(&support[e3], 1u);
-        }
-        __syncthreads();
-      }
-    }
-  }
+    const uint32_t* __restrict__ hub_units, uint32_t num_hub_units, uint32_t hub_chunk,
+    uint32_t table_bits, uint32_t* __restrict__ counters, uint32_t* __restrict__ support) {
+  wedge_body<true>(offsets, targets, fwd, removed, n, hub_units, num_hub_units, hub_chunk, table_bits,
+                   counters, nullptr, support);
 }
 
 __device__ __forceinline__ uint32_t has_loop(const uint32_t* __restrict__ offsets,

@@ -66,6 +66,30 @@ __device__ __forceinline__ void pr_finish_row(uint32_t row, uint32_t deg, double
   diff += fabs(static_cast<double>(r) - static_cast<double>(rank_in[row]));
 }
 
+// Sum of contrib[targets[i]] for i = first, first + STRIDE, ... < end. Eight
+// independent target loads are issued before their gathers so each lane keeps
+// several memory requests in flight (the pass is bound by memory latency
+// times requests in flight, not by arithmetic).
+template <int STRIDE>
+__device__ __forceinline__ double pr_row_sum(const uint32_t* __restrict__ targets,
+                                             const float* __restrict__ contrib, uint32_t first,
+                                             uint32_t end) {
+  double acc = 0.0;
+  uint32_t i = first;
+  for (; i + 7 * STRIDE < end; i += 8 * STRIDE) {
+    uint32_t t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = load_stream(targets + i + k * STRIDE);
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = contrib[t[k]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += static_cast<double>(x[k]);
+  }
+  for (; i < end; i += STRIDE) acc += static_cast<double>(contrib[load_stream(targets + i)]);
+  return acc;
+}
+
 template <int G>
 __device__ __forceinline__ void pr_rows_group(const PrSegment& seg, uint32_t local_block,
                                               const uint32_t* __restrict__ offsets,
@@ -84,7 +108,7 @@ __device__ __forceinline__ void pr_rows_group(const PrSegment& seg, uint32_t loc
   if (valid) {
     b = offsets[row];
     e = offsets[row + 1];
-    for (uint32_t i = b + g; i < e; i += G) acc += static_cast<double>(contrib_in[load_stream(targets + i)]);
+    acc = pr_row_sum<G>(targets, contrib_in, b + g, e);
   }
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -104,10 +128,7 @@ __device__ __forceinline__ void pr_row_block(const PrSegment& seg, uint32_t loca
                                              double* s_red) {
   const uint32_t row = seg.row_begin + local_block;
   const uint32_t b = offsets[row], e = offsets[row + 1];
-  double acc = 0.0;
-  for (uint32_t i = b + threadIdx.x; i < e; i += PR_THREADS) {
-    acc += static_cast<double>(contrib_in[load_stream(targets + i)]);
-  }
+  double acc = pr_row_sum<PR_THREADS>(targets, contrib_in, b + threadIdx.x, e);
   acc = wave_sum(acc);
   if (lane_id() == 0) s_red[threadIdx.x / GC_WAVE] = acc;
   __syncthreads();
@@ -164,12 +185,151 @@ extern "C" __global__ __launch_bounds__(PR_THREADS) void pr_spmv(
   }
 }
 
-// Adds the per-workgroup partials in a fixed order and advances the state.
+// ---- persistent variant: hot contributions in LDS ---------------------------
+//
+// The rank mass of a power-law graph is concentrated: on the degree-ordered
+// ids, the top 32K vertices receive about half of all gathers (R-MAT scale 22).
+// Each 1024-lane workgroup copies c[n-hot .. n) into LDS once per iteration
+// and then takes row tiles from a shared work counter, heaviest tiles first
+// (a tile is 1024/G rows of one segment, or one row of degree >= 1024 for the
+// whole workgroup). Gathers of hot vertices become LDS reads, which removes
+// them from the L1/L2 request stream that bounds the pass.
+
+#define PRH_THREADS 1024
+
+template <int STRIDE>
+__device__ __forceinline__ double prh_row_sum(const uint32_t* __restrict__ targets,
+                                              const float* __restrict__ contrib,
+                                              const float* __restrict__ s_hot, uint32_t hot_base,
+                                              uint32_t first, uint32_t end) {
+  double acc = 0.0;
+  uint32_t i = first;
+  for (; i + 7 * STRIDE < end; i += 8 * STRIDE) {
+    uint32_t t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = load_stream(targets + i + k * STRIDE);
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = t[k] >= hot_base ? s_hot[t[k] - hot_base] : contrib[t[k]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += static_cast<double>(x[k]);
+  }
+  for (; i < end; i += STRIDE) {
+    const uint32_t t = load_stream(targets + i);
+    acc += static_cast<double>(t >= hot_base ? s_hot[t - hot_base] : contrib[t]);
+  }
+  return acc;
+}
+
+template <int G>
+__device__ __forceinline__ void prh_tile_group(const PrSegment& seg, uint32_t local,
+                                               const uint32_t* __restrict__ offsets,
+                                               const uint32_t* __restrict__ targets,
+                                               const float* __restrict__ contrib_in,
+                                               const float* __restrict__ s_hot, uint32_t hot_base,
+                                               const float* __restrict__ rank_in,
+                                               float* __restrict__ contrib_out,
+                                               float* __restrict__ rank_out, double base,
+                                               double damping, double& dangling, double& diff) {
+  constexpr uint32_t kRowsPerTile = PRH_THREADS / G;
+  const uint32_t row = seg.row_begin + local * kRowsPerTile + threadIdx.x / G;
+  const uint32_t g = threadIdx.x % G;
+  const bool valid = row < seg.row_end;
+  double acc = 0.0;
+  uint32_t b = 0, e = 0;
+  if (valid) {
+    b = offsets[row];
+    e = offsets[row + 1];
+    acc = prh_row_sum<G>(targets, contrib_in, s_hot, hot_base, b + g, e);
+  }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (valid && g == 0) {
+    pr_finish_row(row, e - b, acc, base, damping, rank_in, rank_out, contrib_out, dangling, diff);
+  }
+}
+
+extern "C" __global__ __launch_bounds__(PRH_THREADS) void pr_spmv_hot(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    PrSegments tiles, const PrState* __restrict__ state, uint32_t* __restrict__ work,
+    uint32_t hot_count, double* __restrict__ partials) {
+  extern __shared__ float s_hot[];
+  __shared__ double s_red[PRH_THREADS / GC_WAVE];
+  __shared__ double s_part[2][PRH_THREADS / GC_WAVE];
+  __shared__ uint32_t s_task;
+  if (state->done) return;
+  const uint32_t hot_base = n - hot_count;
+  for (uint32_t i = threadIdx.x; i < hot_count; i += PRH_THREADS) s_hot[i] = contrib_in[hot_base + i];
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0) s_task = atomicAdd(work, 1u);
+    __syncthreads();
+    const uint32_t k = s_task;
+    __syncthreads();
+    if (k >= tiles.total_blocks) break;
+    const uint32_t t = tiles.total_blocks - 1 - k;  // heaviest tiles first
+    uint32_t s = 0;
+    while (s + 1 < tiles.count && t >= tiles.seg[s + 1].block_begin) ++s;
+    const PrSegment seg = tiles.seg[s];
+    const uint32_t local = t - seg.block_begin;
+    switch (seg.group) {
+      case 1: prh_tile_group<1>(seg, local, offsets, targets, contrib_in, s_hot, hot_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+      case 2: prh_tile_group<2>(seg, local, offsets, targets, contrib_in, s_hot, hot_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+      case 4: prh_tile_group<4>(seg, local, offsets, targets, contrib_in, s_hot, hot_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+      case 8: prh_tile_group<8>(seg, local, offsets, targets, contrib_in, s_hot, hot_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+      case 16: prh_tile_group<16>(seg, local, offsets, targets, contrib_in, s_hot, hot_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+      case 32: prh_tile_group<32>(seg, local, offsets, targets, contrib_in, s_hot, hot_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+      case 64: prh_tile_group<64>(seg, local, offsets, targets, contrib_in, s_hot, hot_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+      default: {
+        // one row for the whole workgroup
+        const uint32_t row = seg.row_begin + local;
+        const uint32_t b = offsets[row], e = offsets[row + 1];
+        double acc = prh_row_sum<PRH_THREADS>(targets, contrib_in, s_hot, hot_base, b + threadIdx.x, e);
+        acc = wave_sum(acc);
+        if (lane_id() == 0) s_red[threadIdx.x / GC_WAVE] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double total = 0.0;
+          for (int w = 0; w < PRH_THREADS / GC_WAVE; ++w) total += s_red[w];
+          pr_finish_row(row, e - b, total, base, damping, rank_in, rank_out, contrib_out, dangling, diff);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  dangling = wave_sum(dangling);
+  diff = wave_sum(diff);
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  if (lane_id() == 0) {
+    s_part[0][wave] = dangling;
+    s_part[1][wave] = diff;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    for (int w = 0; w < PRH_THREADS / GC_WAVE; ++w) {
+      d0 += s_part[0][w];
+      d1 += s_part[1][w];
+    }
+    partials[2 * static_cast<uint64_t>(blockIdx.x)] = d0;
+    partials[2 * static_cast<uint64_t>(blockIdx.x) + 1] = d1;
+  }
+}
+
+// Adds the per-workgroup partials in a fixed order and advances the state;
+// also rearms the work counter of pr_spmv_hot for the next iteration.
 extern "C" __global__ __launch_bounds__(1024) void pr_finalize(PrState* __restrict__ state,
                                                                const double* __restrict__ partials,
-                                                               uint32_t blocks, double tolerance) {
+                                                               uint32_t blocks, double tolerance,
+                                                               uint32_t* __restrict__ work) {
   __shared__ double s_a[1024 / GC_WAVE];
   __shared__ double s_b[1024 / GC_WAVE];
+  if (threadIdx.x == 0) *work = 0;
   if (state->done) return;
   double d0 = 0.0, d1 = 0.0;
   for (uint32_t i = threadIdx.x; i < blocks; i += 1024) {

@@ -22,6 +22,7 @@
 // `descending`) do not leave a tail. Counts are u64 per wave, added with one
 // atomic per wave at the end (integer sums are exact in any order).
 #include "device.hpp"
+#include "wedge.hpp"
 
 using namespace gcd;
 
@@ -135,4 +136,145 @@ extern "C" __global__ __launch_bounds__(64) void tc_count(
 
   count = wave_sum(count);
   if (lane == 0 && count != 0) atomicAdd(result, count);
+}
+
+// ---- hash-probe variant ------------------------------------------------------
+//
+// Same count as tc_count on the degree-oriented DAG (L = the suffix above the
+// row), with each wave of a 256-lane workgroup working on its own vertex u:
+// the wave inserts L(u) into a private LDS hash table (open addressing,
+// linear probing, at most half full), then streams the lists L(v) of its
+// neighbours v as one flattened item range (wave prefix sum over their
+// lengths) and tests each item with one hash probe instead of a binary
+// search. The lists L(v) of the DAG's high-rank vertices are short and shared
+// by many u, so they stay cache-resident. Rows with more than TCH_TABLE/2
+// entries fall back to a binary search in global memory.
+
+#define TCH_WAVES 4
+#define TCH_TABLE 1024
+
+__device__ __forceinline__ void wave_lds_sync() {
+  // Makes this wave's LDS writes visible to its own other lanes; waves of the
+  // workgroup work independently, so no workgroup barrier is needed.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+    const uint32_t* __restrict__ targets, uint32_t n, uint32_t heavy, uint32_t light_chunk,
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+  __shared__ uint32_t s_table_all[TCH_WAVES][TCH_TABLE];
+  __shared__ uint32_t s_pos_all[TCH_WAVES][GC_WAVE];
+  __shared__ uint32_t s_start_all[TCH_WAVES][GC_WAVE];
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  const uint32_t lane = lane_id();
+  uint32_t* s_table = s_table_all[wave];
+  uint32_t* s_pos = s_pos_all[wave];
+  uint32_t* s_start = s_start_all[wave];
+  unsigned long long count = 0;
+  bool heavy_left = heavy > 0;
+
+  for (;;) {
+    uint32_t kb = n, ke = n;
+    if (lane == 0) {
+      if (heavy_left) {
+        const uint32_t k = atomicAdd(&counters[0], 1u);
+        if (k < heavy) {
+          kb = k;
+          ke = k + 1;
+        } else {
+          heavy_left = false;
+        }
+      }
+      if (!heavy_left && kb == n) {
+        const uint32_t c = atomicAdd(&counters[1], light_chunk);
+        const uint64_t b = static_cast<uint64_t>(heavy) + c;
+        if (b < n) {
+          kb = static_cast<uint32_t>(b);
+          const uint64_t e = b + light_chunk;
+          ke = e < n ? static_cast<uint32_t>(e) : n;
+        }
+      }
+    }
+    kb = __shfl(kb, 0);
+    ke = __shfl(ke, 0);
+    if (kb >= n) break;
+
+    for (uint32_t k = kb; k < ke; ++k) {
+      const uint32_t u = n - 1 - k;
+      const uint32_t a_lo = lo[u];
+      const uint32_t a = hi[u] - a_lo;
+      if (a == 0) continue;
+      const uint32_t* A = targets + a_lo;
+      const bool in_table = 2 * a <= TCH_TABLE;
+      uint32_t bits = 6;
+      while ((1u << bits) < 2 * a && bits < 10) ++bits;
+      const uint32_t shift = 32u - bits, mask = (1u << bits) - 1;
+      if (in_table) {
+        for (uint32_t j = lane; j <= mask; j += GC_WAVE) s_table[j] = 0xFFFFFFFFu;
+        wave_lds_sync();
+        for (uint32_t j = lane; j < a; j += GC_WAVE) {
+          const uint32_t w = A[j];
+          uint32_t slot = (w * 0x9E3779B1u) >> shift;
+          while (atomicCAS(&s_table[slot], 0xFFFFFFFFu, w) != 0xFFFFFFFFu) slot = (slot + 1) & mask;
+        }
+        wave_lds_sync();
+      }
+      for (uint32_t c = 0; c < a; c += GC_WAVE) {
+        const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - c);
+        uint32_t len = 0, st = 0;
+        if (lane < nv) {
+          const uint32_t v = A[c + lane];
+          st = lo[v];
+          len = hi[v] - st;
+        }
+        const uint32_t incl = wave_inclusive_scan(len);
+        s_pos[lane] = incl;
+        s_start[lane] = st;
+        const uint32_t total = __shfl(incl, GC_WAVE - 1);
+        wave_lds_sync();
+        for (uint32_t t = lane; t < total; t += GC_WAVE) {
+          uint32_t l = 0, h = nv - 1;
+          while (l < h) {
+            const uint32_t mid = (l + h) >> 1;
+            if (s_pos[mid] > t) h = mid;
+            else l = mid + 1;
+          }
+          const uint32_t excl = l ? s_pos[l - 1] : 0;
+          const uint32_t w = targets[s_start[l] + (t - excl)];
+          bool found;
+          if (in_table) {
+            uint32_t slot = (w * 0x9E3779B1u) >> shift;
+            uint32_t key = s_table[slot];
+            while (key != w && key != 0xFFFFFFFFu) {
+              slot = (slot + 1) & mask;
+              key = s_table[slot];
+            }
+            found = key == w;
+          } else {
+            const uint32_t p = lower_bound_u32(A, 0, a, w);
+            found = p < a && A[p] == w;
+          }
+          count += found ? 1ull : 0ull;
+        }
+        wave_lds_sync();
+      }
+    }
+  }
+  count = wave_sum(count);
+  if (lane == 0 && count != 0) atomicAdd(result, count);
+}
+
+// Triangle count of a symmetric graph on its degree-ordered relabelling
+// (the count of crates/grafeo-cuda-algos/kernels/triangle_counting.cu:246-315),
+// by enumeration at the middle vertex; see wedge.hpp.
+extern "C" __global__ __launch_bounds__(64) void tc_wedge(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ fwd, uint32_t n, const uint32_t* __restrict__ hub_units,
+    uint32_t num_hub_units, uint32_t hub_chunk, uint32_t table_bits, uint32_t* __restrict__ counters,
+    unsigned long long* __restrict__ result) {
+  wedge_body<false>(offsets, targets, fwd, nullptr, n, hub_units, num_hub_units, hub_chunk, table_bits,
+                    counters, result, nullptr);
 }

@@ -130,3 +130,61 @@ extern "C" __global__ __launch_bounds__(COMPACT_THREADS) void compact_emit_pairs
     ++pos;
   }
 }
+
+// ---- exclusive prefix sum of u32 values, in place ---------------------------
+//
+// scan_block_sums: per-block totals of SCAN_PER_BLOCK consecutive values;
+// exclusive_scan_small over the totals; scan_apply: each block scans its own
+// values and adds its base. data[count] receives nothing; `total` gets the sum.
+
+#define SCAN_THREADS 256
+#define SCAN_PER_THREAD 16
+#define SCAN_PER_BLOCK (SCAN_THREADS * SCAN_PER_THREAD)
+
+extern "C" __global__ __launch_bounds__(SCAN_THREADS) void scan_block_sums(
+    const uint32_t* __restrict__ data, uint64_t count, uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t s_wave[SCAN_THREADS / GC_WAVE];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * SCAN_PER_BLOCK +
+                        static_cast<uint64_t>(threadIdx.x) * SCAN_PER_THREAD;
+  uint32_t c = 0;
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    const uint64_t i = base + k;
+    if (i < count) c += data[i];
+  }
+  c = wave_sum(c);
+  if (lane_id() == 0) s_wave[threadIdx.x / GC_WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < SCAN_THREADS / GC_WAVE; ++w) t += s_wave[w];
+    block_sums[blockIdx.x] = t;
+  }
+}
+
+extern "C" __global__ __launch_bounds__(SCAN_THREADS) void scan_apply(
+    uint32_t* __restrict__ data, uint64_t count, const uint32_t* __restrict__ block_bases) {
+  __shared__ uint32_t s_wave[SCAN_THREADS / GC_WAVE];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * SCAN_PER_BLOCK +
+                        static_cast<uint64_t>(threadIdx.x) * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    const uint64_t i = base + k;
+    v[k] = i < count ? data[i] : 0u;
+    c += v[k];
+  }
+  const uint32_t incl = wave_inclusive_scan(c);
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  if (lane_id() == GC_WAVE - 1) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t run = block_bases[blockIdx.x];
+  for (uint32_t w = 0; w < wave; ++w) run += s_wave[w];
+  run += incl - c;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    const uint64_t i = base + k;
+    if (i < count) data[i] = run;
+    run += v[k];
+  }
+}

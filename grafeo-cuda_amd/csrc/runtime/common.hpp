@@ -87,6 +87,8 @@ struct Device {
   std::string name;
   uint64_t total_memory = 0;
   int cu_count = 0;
+  size_t max_lds_per_block = 64 * 1024;
+  uint32_t pr_hot_count_limit = 0xFFFFFFFFu;  // lowered if a launch with a larger LDS set fails
   std::recursive_mutex mu;  // serializes calls on this device
   // Small pinned staging area for scalar read-backs (counters, states).
   void* pinned = nullptr;
@@ -135,6 +137,26 @@ void launch(Device& dev, const Kernel& k, uint32_t grid, uint32_t block, uint32_
     fail(GC_ERR_KERNEL, std::string(k.name) + " launch failed: " + hipGetErrorString(err));
   }
   profile_end(dev, k.name, start);
+}
+
+// As launch(), but returns the launch status instead of throwing (for launch
+// shapes that a device may refuse, such as large dynamic LDS).
+template <class... Args>
+hipError_t try_launch(Device& dev, const Kernel& k, uint32_t grid, uint32_t block, uint32_t shmem,
+                      Args... args) {
+  if (grid == 0) return hipSuccess;
+  void* params[] = {karg(args)..., nullptr};
+  hipEvent_t start = nullptr;
+  profile_begin(dev, k.name, &start);
+  hipError_t err = hipModuleLaunchKernel(k.fn, grid, 1, 1, block, 1, 1, shmem, dev.stream, params,
+                                         nullptr);
+  if (err == hipSuccess) {
+    profile_end(dev, k.name, start);
+  } else {
+    (void)hipGetLastError();
+    if (start != nullptr) dev.event_pool.push_back(start);
+  }
+  return err;
 }
 
 // --------------------------------------------------------- device memory --
@@ -198,8 +220,15 @@ struct Relabelled {
   std::vector<uint32_t> host_offsets;  // kept for host-side planning
   uint64_t oriented_entries = 0;       // sum of (offsets[i+1] - fwd[i])
   uint64_t intersect_entries = 0;      // sum over DAG entries i->j of (d+(i) + d+(j))
+  uint64_t wedges = 0;                 // sum over rows of d+(i) (d+(i) - 1) / 2
   uint32_t dangling = 0;               // rows of degree 0
   uint32_t max_degree = 0;
+  uint32_t max_out_degree = 0;         // max over rows of (offsets[i+1] - fwd[i])
+  // In-lists (entries below the row) longer than kHubChunk are cut into
+  // slices of kHubChunk entries: (vertex, begin, end) triples, heaviest first.
+  static constexpr uint32_t kHubChunk = 2048;
+  DeviceMemory hub_units;
+  uint32_t num_hub_units = 0;
 };
 
 struct Graph {
@@ -252,6 +281,17 @@ struct GraphCall {
 
 // Fills `words` 32-bit words at `p` with `value` on the device's stream.
 void fill_words(Device& dev, void* p, uint64_t words, uint32_t value);
+
+// In-place exclusive prefix sum of `count` u32 values on the device;
+// `block_buf` holds scan_scratch_words(count) words, the last of which
+// receives the total.
+uint64_t scan_scratch_words(uint64_t count);
+void exclusive_scan_u32(Device& dev, uint32_t* data, uint64_t count, uint32_t* block_buf);
+
+// log2 of the LDS hash-table size of the middle-vertex triangle kernels: the
+// smallest power of two holding twice the largest DAG out-degree, between 64
+// and 4096 entries (rows above that search global memory instead).
+uint32_t wedge_table_bits(const Relabelled& rl);
 
 // Builds the derived layouts if not done yet (host relabel + upload).
 void prepare(Graph& g);

@@ -84,22 +84,35 @@ void prepare(Graph& g) {
     }
   }, 1u << 12);
   uint64_t oriented = 0;
-  uint32_t dangling = 0;
-  for (uint32_t i = 0; i < n; ++i) {
+  uint32_t dangling = 0, max_out = 0;
+  std::vector<uint32_t> hubs;  // (vertex, begin, end) slices of long in-lists
+  for (uint32_t i = n; i-- > 0;) {
     oriented += r_off[i + 1] - fwd[i];
     dangling += r_off[i + 1] == r_off[i];
+    max_out = std::max(max_out, r_off[i + 1] - fwd[i]);
+    uint32_t in_end = fwd[i];
+    if (in_end > r_off[i] && r_tgt[in_end - 1] == i) --in_end;
+    if (in_end - r_off[i] > Relabelled::kHubChunk) {
+      for (uint32_t s = r_off[i]; s < in_end; s += Relabelled::kHubChunk) {
+        hubs.push_back(i);
+        hubs.push_back(s);
+        hubs.push_back(std::min(s + Relabelled::kHubChunk, in_end));
+      }
+    }
   }
-  std::atomic<uint64_t> intersect{0};
+  std::atomic<uint64_t> intersect{0}, wedges{0};
   parallel_for(n, [&](uint64_t b, uint64_t e) {
-    uint64_t local = 0;
+    uint64_t local = 0, local_wedges = 0;
     for (uint64_t i = b; i < e; ++i) {
       const uint64_t du = r_off[i + 1] - fwd[i];
+      local_wedges += du * (du ? du - 1 : 0) / 2;
       for (uint32_t p = fwd[i]; p < r_off[i + 1]; ++p) {
         const uint32_t j = r_tgt[p];
         local += du + (r_off[j + 1] - fwd[j]);
       }
     }
     intersect += local;
+    wedges += local_wedges;
   }, 1u << 12);
 
   dev.make_current();
@@ -117,8 +130,13 @@ void prepare(Graph& g) {
   rl.host_offsets = std::move(r_off);
   rl.oriented_entries = oriented;
   rl.intersect_entries = intersect.load();
+  rl.wedges = wedges.load();
   rl.dangling = dangling;
   rl.max_degree = max_deg;
+  rl.max_out_degree = max_out;
+  rl.num_hub_units = static_cast<uint32_t>(hubs.size() / 3);
+  rl.hub_units.allocate(sizeof(uint32_t) * std::max<size_t>(hubs.size(), 3));
+  copy_to_device(dev, rl.hub_units.get(), hubs.data(), sizeof(uint32_t) * hubs.size());
 
   // The host copy of the original CSR is no longer needed.
   std::vector<uint32_t>().swap(g.host_targets);
@@ -226,7 +244,8 @@ gc_status gc_graph_prepare(gc_graph* h, double* secs) {
   });
 }
 
-gc_status gc_graph_dag_stats(gc_graph* h, uint64_t* oriented_entries, uint64_t* intersect_entries) {
+gc_status gc_graph_dag_stats(gc_graph* h, uint64_t* oriented_entries, uint64_t* intersect_entries,
+                             uint64_t* wedges) {
   return guard([&] {
     gc::Graph* g = gc::as_graph(h);
     std::lock_guard<std::recursive_mutex> lock(g->dev->mu);
@@ -234,6 +253,7 @@ gc_status gc_graph_dag_stats(gc_graph* h, uint64_t* oriented_entries, uint64_t* 
     gc::prepare(*g);
     if (oriented_entries) *oriented_entries = g->rl.oriented_entries;
     if (intersect_entries) *intersect_entries = g->rl.intersect_entries;
+    if (wedges) *wedges = g->rl.wedges;
   });
 }
 

@@ -264,6 +264,28 @@ void fill_words(Device& dev, void* p, uint64_t words, uint32_t value) {
   launch(dev, k, grid.blocks, grid.threads, 0, static_cast<uint32_t*>(p), words, value);
 }
 
+uint64_t scan_scratch_words(uint64_t count) {
+  constexpr uint64_t kPerBlock = 256 * 16;  // SCAN_PER_BLOCK of util.hip
+  return (count + kPerBlock - 1) / kPerBlock + 1;
+}
+
+void exclusive_scan_u32(Device& dev, uint32_t* data, uint64_t count, uint32_t* block_buf) {
+  if (count == 0) return;
+  const uint32_t blocks = static_cast<uint32_t>(scan_scratch_words(count) - 1);
+  launch(dev, get_kernel(dev, "util", "scan_block_sums"), blocks, 256u, 0,
+         static_cast<const uint32_t*>(data), count, block_buf);
+  launch(dev, get_kernel(dev, "util", "exclusive_scan_small"), 1u, 1024u, 0, block_buf, blocks,
+         block_buf + blocks);
+  launch(dev, get_kernel(dev, "util", "scan_apply"), blocks, 256u, 0, data, count,
+         static_cast<const uint32_t*>(block_buf));
+}
+
+uint32_t wedge_table_bits(const Relabelled& rl) {
+  uint32_t bits = 6;
+  while (bits < 12 && (uint64_t(1) << bits) < 2ull * rl.max_out_degree) ++bits;
+  return bits;
+}
+
 void copy_to_device(Device& dev, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return;
   hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, dev.stream), "hipMemcpyAsync H2D");
@@ -321,8 +343,14 @@ gc_status gc_device_new(uint32_t ordinal, gc_device** out) {
     hipDeviceProp_t prop;
     gc::hip_check(hipGetDeviceProperties(&prop, dev->ordinal), "hipGetDeviceProperties");
     dev->name = prop.name;
+    if (dev->name.empty()) {
+      char buf[256] = {0};
+      if (hipDeviceGetName(buf, sizeof(buf) - 1, dev->ordinal) == hipSuccess) dev->name = buf;
+    }
+    if (dev->name.empty()) dev->name = std::string("AMD ") + prop.gcnArchName;
     dev->total_memory = prop.totalGlobalMem;
     dev->cu_count = prop.multiProcessorCount;
+    dev->max_lds_per_block = prop.sharedMemPerBlock;
     gc::hip_check(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking), "hipStreamCreate");
     gc::hip_check(hipHostMalloc(&dev->pinned, gc::Device::kPinnedBytes, hipHostMallocDefault),
                   "hipHostMalloc");

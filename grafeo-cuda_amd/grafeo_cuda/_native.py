@@ -78,7 +78,7 @@ def _load() -> ctypes.CDLL:
         "gc_graph_device_targets": (c_void_p, [c_void_p]),
         "gc_graph_prepare": (c_int, [c_void_p, POINTER(c_double)]),
         "gc_graph_is_symmetric": (c_int, [c_void_p, i32p]),
-        "gc_graph_dag_stats": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
+        "gc_graph_dag_stats": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
         "gc_device_malloc": (c_int, [c_void_p, c_size_t, POINTER(c_void_p)]),
         "gc_device_free": (c_int, [c_void_p, c_void_p]),
         "gc_device_copy_to_host": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),

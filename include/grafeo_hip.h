@@ -156,11 +156,14 @@ gc_status gc_graph_prepare(gc_graph* g, double* secs);
 gc_status gc_graph_is_symmetric(gc_graph* g, int32_t* symmetric);
 
 /* Sizes of the degree-oriented DAG the symmetric kernels run on (prepares):
- * its entry count m_o (entries u->v whose (degree, id) rank is above u's) and
- * the sum over those entries of (d+(u) + d+(v)), the list entries a
- * sorted-intersection triangle count streams. Used to state the triangle
- * kernels' algorithmic bytes. No reference counterpart. */
-gc_status gc_graph_dag_stats(gc_graph* g, uint64_t* oriented_entries, uint64_t* intersect_entries);
+ * its entry count m_o (entries u->v whose (degree, id) rank is above u's),
+ * the sum over those entries of (d+(u) + d+(v)) (the list entries a merge
+ * intersection per DAG edge streams), and the number of DAG wedges,
+ * sum over u of d+(u)(d+(u)-1)/2 (the list entries the middle-vertex triangle
+ * kernel streams). Used to state the triangle kernels' algorithmic bytes. No
+ * reference counterpart. */
+gc_status gc_graph_dag_stats(gc_graph* g, uint64_t* oriented_entries, uint64_t* intersect_entries,
+                             uint64_t* wedges);
 
 /* ------------------------------------------------------------------------ */
 /* Algorithms                                                                */

@@ -1,0 +1,89 @@
+"""The benchmark configurations at full size, checked on the GPU against golden fixtures that
+the CPU oracle wrote (tests/golden/rmat_counts.json) and, for PageRank, against the FP64 oracle
+run live for the same number of iterations.
+
+  - R-MAT scale 20 suite: triangle count, k-truss k=4, BFS from 64 seeded sources of nonzero
+    degree, PageRank d=0.85, at most 20 iterations, tol 1e-6 (reached after 17 iterations; the
+    iteration count and convergence flag must match the FP64 oracle's).
+  - R-MAT scale 22 PageRank precision: FP32 ranks, FP64 reductions, up to 50 iterations at tol
+    1e-7; L1 <= 1e-6 against FP64 at the same iteration count (max-abs is reported).
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import grafeo_cuda
+import oracle
+from graphs import csr_sha256, rmat_graph
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rmat_counts.json")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def bfs_is_valid(off, tgt, source, dist):
+    """Size-independent characterization of BFS distances on any CSR: d[source] = 0; for every
+    entry u->v with d[u] >= 0, 0 <= d[v] <= d[u] + 1; every v != source with d[v] > 0 has an
+    in-entry u->v with d[u] = d[v] - 1."""
+    n = len(off) - 1
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(off.astype(np.int64)))
+    dst = tgt.astype(np.int64)
+    du, dv = dist[src], dist[dst]
+    reached = du >= 0
+    ok = dist[source] == 0 and np.all(dv[reached] >= 0) and np.all(dv[reached] <= du[reached] + 1)
+    has_parent = np.zeros(n, bool)
+    has_parent[dst[reached & (dv == du + 1)]] = True
+    need = dist > 0
+    return bool(ok and np.all(has_parent[need]))
+
+
+def test_rmat20_suite(device, golden):
+    e = golden["rmat20_ef16_seed1"]
+    off, tgt = rmat_graph(20, 16, seed=1)
+    assert csr_sha256(off, tgt) == e["csr_sha256"]
+    g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+    assert g.triangle_count()["total_triangles"] == e["triangles"]
+
+    kt = g.k_truss_numpy(4)
+    assert len(kt["edges"]) == e["k_truss_4"]["edges"]
+    assert kt["iterations"] == e["k_truss_4"]["iterations"]
+    assert hashlib.sha256(kt["edges"].astype("<u4").tobytes()).hexdigest() == e["k_truss_4"]["sha256"]
+
+    for item in e["bfs"]:
+        r = g.bfs_numpy(item["source"])
+        assert r["reachable"] == item["reachable"] and r["max_depth"] == item["max_depth"]
+        assert hashlib.sha256(r["distances"].astype("<i4").tobytes()).hexdigest() == item["sha256"]
+    assert bfs_is_valid(off, tgt, e["bfs"][0]["source"], g.bfs_numpy(e["bfs"][0]["source"])["distances"])
+
+    pr = g.pagerank_numpy(0.85, 20, 1e-6)
+    _, it_fp64, conv_fp64 = oracle.pagerank(off, tgt, 0.85, 20, 1e-6)
+    assert (pr["iterations"], pr["converged"]) == (it_fp64, conv_fp64)
+    ref, it, _ = oracle.pagerank(off, tgt, 0.85, pr["iterations"], 0.0)
+    l1 = float(np.abs(pr["ranks"].astype(np.float64) - ref).sum())
+    assert l1 <= 1e-6, l1
+
+
+def test_rmat22_pagerank_precision(device, golden):
+    e = golden["rmat22_ef16_seed1"]
+    off, tgt = rmat_graph(22, 16, seed=1)
+    assert csr_sha256(off, tgt) == e["csr_sha256"]
+    g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+    pr = g.pagerank_numpy(0.85, 50, 1e-7)
+    ref, it, _ = oracle.pagerank(off, tgt, 0.85, pr["iterations"], 0.0)
+    err = pr["ranks"].astype(np.float64) - ref
+    l1, max_abs = float(np.abs(err).sum()), float(np.abs(err).max())
+    print(f"RMAT-22 PageRank: {pr['iterations']} iterations, L1 {l1:.3e}, max-abs {max_abs:.3e}")
+    assert l1 <= 1e-6
+    assert g.triangle_count()["total_triangles"] == e["triangles"]

