@@ -125,7 +125,7 @@ def main() -> int:
     dev.profile(True)
     step()
     kernels = {}
-    for name in ("tc_wedge", "tc_count", "pr_spmv", "pr_finalize", "bfs_topdown_edges", "bfs_frontier_degrees",
+    for name in ("tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_hot", "pr_spmv", "pr_hub_finish", "pr_finalize", "bfs_topdown_edges", "bfs_frontier_degrees",
                  "bfs_bottomup", "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "scan_block_sums",
                  "exclusive_scan_small", "scan_apply", "gather_u32", "fill_u32", "pr_init"):
         ms, count = dev.kernel_time(name)
@@ -150,9 +150,12 @@ def main() -> int:
     # Algorithmic bytes per launch (HBM roofline for every kernel here).
     # PageRank SpMV per iteration: offsets + targets + x read once + y written once.
     spmv_bytes = 4 * (n + 1) + 4 * m + 4 * n + 4 * n
-    # Middle-vertex triangle count: offsets and DAG split (fwd) read once, every CSR entry
-    # read once (in-lists and out-lists), plus one 4-byte entry per DAG wedge streamed.
-    tc_bytes = 4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value
+    # Triangle count by list intersection on the DAG (tc_hash, tc_count): offsets, the DAG's
+    # entries, and for each DAG edge u->v both out-lists, 4(n+1) + 4 m_o + 4 sum(d+u + d+v).
+    tc_bytes = 4 * (n + 1) + 4 * m_o.value + 4 * inter.value
+    # Middle-vertex triangle count (tc_wedge): offsets and DAG split (fwd) read once, every
+    # CSR entry read once (in-lists and out-lists), plus one 4-byte entry per DAG wedge.
+    wedge_bytes = 4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value
     # BFS per traversal: offsets + targets read once, distances written once.
     bfs_bytes = 4 * (n + 1) + 4 * m + 4 * n
     rooflines = {}
@@ -165,8 +168,13 @@ def main() -> int:
 
     if "pr_spmv" in kernels:
         add_roofline("pr_spmv", spmv_bytes, kernels["pr_spmv"]["avg_us"])
+    if "pr_spmv_hot" in kernels:
+        add_roofline("pr_spmv_hot", spmv_bytes, kernels["pr_spmv_hot"]["avg_us"])
+    for name in ("tc_bitmap", "tc_hash", "tc_count"):
+        if name in kernels:
+            add_roofline(name, tc_bytes, kernels[name]["avg_us"])
     if "tc_wedge" in kernels:
-        add_roofline("tc_wedge", tc_bytes, kernels["tc_wedge"]["avg_us"])
+        add_roofline("tc_wedge", wedge_bytes, kernels["tc_wedge"]["avg_us"])
     bfs_kernels = [k for k in kernels if k.startswith("bfs_") or k in ("scan_block_sums", "scan_apply")]
     if bfs_kernels:
         per_traversal_us = 1000.0 * sum(kernels[k]["total_ms"] for k in bfs_kernels) / len(sources)

@@ -19,14 +19,20 @@ namespace {
 
 constexpr uint32_t kPrThreads = 256;     // pr_spmv workgroup
 constexpr uint32_t kPrHotThreads = 1024; // pr_spmv_hot workgroup
-constexpr uint32_t kPrMaxSegments = 10;
+constexpr uint32_t kPrMaxSegments = 24;
 constexpr uint32_t kPrBatch = 8;  // iterations launched between state reads
+constexpr uint32_t kPrExact = 0x80000000u;  // group flag: rows of one exact degree
+constexpr uint32_t kPrExactMax = 16;        // exact-degree segments for degrees 0..15
+constexpr uint32_t kPrHubLimit = 1024;      // rows of at least this degree are summed in chunks
+constexpr uint32_t kPrHubChunk = 4096;      // PR_HUB_CHUNK of pagerank.hip
+constexpr uint32_t kPrBlocksPerCu = 8;      // resident 256-lane workgroups per CU (32 waves)
 
 struct PrSegment {
   uint32_t row_begin;
   uint32_t row_end;
   uint32_t group;
   uint32_t block_begin;
+  uint32_t entry_begin;
 };
 
 struct PrSegments {
@@ -42,10 +48,15 @@ struct PrState {
   uint32_t done;
 };
 
+// Rows per thread of an exact-degree segment (pr_rows_exact's R for degree d).
+uint32_t exact_rows_per_thread(uint32_t d) { return d <= 3 ? 4 : (d <= 7 ? 2 : 1); }
+
 // Cuts the degree-sorted rows into segments of one lane-group size each; a
 // segment's rows are handed out in tiles of block_threads/group rows (one row
-// per tile for the last segment, whose group is the whole workgroup).
-PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t block_threads) {
+// per tile for the last segment, whose group is the whole workgroup). With
+// `exact`, rows of degree below kPrExactMax form one segment per degree.
+PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t block_threads,
+                         bool exact) {
   // Degree thresholds and the group size used below each.
   static const uint32_t kLimit[] = {2, 4, 8, 16, 32, 64, 1024, 0xFFFFFFFFu};
   const uint32_t kGroup[] = {1, 2, 4, 8, 16, 32, 64, block_threads};
@@ -53,6 +64,26 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
   PrSegments s{};
   uint32_t row = 0, block = 0;
   auto degree = [&](uint32_t i) { return off[i + 1] - off[i]; };
+  auto first_with_degree = [&](uint32_t from, uint32_t limit) {
+    uint32_t lo = from, hi = n;
+    while (lo < hi) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (degree(mid) >= limit) hi = mid;
+      else lo = mid + 1;
+    }
+    return lo;
+  };
+  if (exact) {
+    for (uint32_t d = 0; d < kPrExactMax && row < n; ++d) {
+      const uint32_t end = first_with_degree(row, d + 1);
+      if (end > row) {
+        const uint32_t rows_per_block = kPrThreads * exact_rows_per_thread(d);
+        s.seg[s.count++] = PrSegment{row, end, kPrExact | d, block, off[row]};
+        block += (end - row + rows_per_block - 1) / rows_per_block;
+        row = end;
+      }
+    }
+  }
   for (int t = 0; t < 8 && row < n; ++t) {
     // first row at or after `row` whose degree reaches kLimit[t]
     uint32_t lo = row, hi = n;
@@ -68,6 +99,7 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
       seg.row_end = lo;
       seg.group = kGroup[t];
       seg.block_begin = block;
+      seg.entry_begin = off[row];
       const uint32_t rows_per_block = kGroup[t] >= kPrThreads ? 1 : kPrThreads / kGroup[t];
       block += (lo - row + rows_per_block - 1) / rows_per_block;
       s.seg[s.count++] = seg;
@@ -80,11 +112,12 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
 
 }  // namespace
 
-// Which SpMV runs: the persistent LDS-hot kernel (default) or one workgroup
-// per row tile (GC_PR_KERNEL=rows).
+// Which SpMV runs: one workgroup per row tile (default, pr_spmv) or the
+// persistent kernel with the hot contributions in LDS (GC_PR_KERNEL=hot,
+// pr_spmv_hot).
 static bool use_row_kernel() {
   const char* env = std::getenv("GC_PR_KERNEL");
-  return env != nullptr && std::strcmp(env, "rows") == 0;
+  return env == nullptr || std::strcmp(env, "hot") != 0;
 }
 
 void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance, float* d_out,
@@ -94,7 +127,45 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   const uint32_t n = g.n;
   const bool rows_kernel = use_row_kernel();
   const uint32_t block_threads = rows_kernel ? kPrThreads : kPrHotThreads;
-  const PrSegments segs = plan_segments(g.rl.host_offsets, n, block_threads);
+  // Row kernel: rows of degree >= kPrHubLimit are summed in chunks (hub plan).
+  const std::vector<uint32_t>& hoff = g.rl.host_offsets;
+  uint32_t hub_begin = n;
+  if (rows_kernel) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (hoff[mid + 1] - hoff[mid] >= kPrHubLimit) hi = mid;
+      else lo = mid + 1;
+    }
+    hub_begin = lo;
+  }
+  const PrSegments segs = plan_segments(hoff, hub_begin, block_threads, rows_kernel);
+  uint32_t num_hub_chunks = 0;
+  uint32_t* hub_chunks = nullptr;
+  uint32_t* hub_first = nullptr;
+  uint32_t* hub_count = nullptr;
+  double* hub_partials = nullptr;
+  if (rows_kernel && hub_begin < n) {
+    std::vector<uint32_t> chunks, first(n - hub_begin), count(n - hub_begin);
+    for (uint32_t r = n; r-- > hub_begin;) {  // heaviest rows first
+      first[r - hub_begin] = static_cast<uint32_t>(chunks.size() / 2);
+      uint32_t c = 0;
+      for (uint32_t e = hoff[r]; e < hoff[r + 1]; e += kPrHubChunk, ++c) {
+        chunks.push_back(e);
+        chunks.push_back(std::min(e + kPrHubChunk, hoff[r + 1]));
+      }
+      count[r - hub_begin] = c;
+    }
+    num_hub_chunks = static_cast<uint32_t>(chunks.size() / 2);
+    hub_chunks = g.buffer<uint32_t>("pr.hub_chunks", sizeof(uint32_t) * chunks.size());
+    hub_first = g.buffer<uint32_t>("pr.hub_first", sizeof(uint32_t) * first.size());
+    hub_count = g.buffer<uint32_t>("pr.hub_count", sizeof(uint32_t) * count.size());
+    hub_partials = g.buffer<double>("pr.hub_partials", sizeof(double) * num_hub_chunks);
+    copy_to_device(dev, hub_chunks, chunks.data(), sizeof(uint32_t) * chunks.size());
+    copy_to_device(dev, hub_first, first.data(), sizeof(uint32_t) * first.size());
+    copy_to_device(dev, hub_count, count.data(), sizeof(uint32_t) * count.size());
+  }
+  const uint32_t finish_blocks = rows_kernel ? (n - hub_begin + kPrThreads - 1) / kPrThreads : 0;
   // Persistent kernel: as many 1024-lane workgroups as fit at once; the hot
   // set is what the LDS of one workgroup holds (leaving room for the
   // reduction scratch), capped at n.
@@ -103,13 +174,24 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   hot_count = std::min({hot_count, n, dev.pr_hot_count_limit});
   const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
   auto hot_grid = [&](uint32_t hot) { return cu * (hot * 4u > 80u * 1024u ? 1u : 2u); };
-  uint32_t grid_blocks = rows_kernel ? segs.total_blocks : hot_grid(hot_count);
+  // Row kernel: one workgroup per virtual block by default (the hardware
+  // dispatcher balances them); GC_PR_RESIDENT=1 launches only the resident
+  // workgroups, which then take the virtual blocks round-robin.
+  const char* resident_env = std::getenv("GC_PR_RESIDENT");
+  const bool resident = resident_env != nullptr && std::strcmp(resident_env, "1") == 0;
+  const uint32_t virtual_blocks = num_hub_chunks + segs.total_blocks;
+  uint32_t grid_blocks = rows_kernel ? std::max(1u, resident ? std::min(virtual_blocks,
+                                                                        cu * kPrBlocksPerCu)
+                                                             : virtual_blocks)
+                                     : hot_grid(hot_count);
+  // partial sums: one pair per spmv workgroup, then one per pr_hub_finish workgroup
+  const uint32_t partial_blocks = rows_kernel ? grid_blocks + finish_blocks : grid_blocks;
   float* rank[2] = {g.buffer<float>("pr.rank0", sizeof(float) * n),
                     g.buffer<float>("pr.rank1", sizeof(float) * n)};
   float* contrib[2] = {g.buffer<float>("pr.contrib0", sizeof(float) * n),
                        g.buffer<float>("pr.contrib1", sizeof(float) * n)};
   double* partials = g.buffer<double>(
-      "pr.partials", sizeof(double) * 2 * std::max({1u, segs.total_blocks, 2 * cu}));
+      "pr.partials", sizeof(double) * 2 * std::max({1u, partial_blocks, 2 * cu}));
   PrState* state = g.buffer<PrState>("pr.state", sizeof(PrState));
   uint32_t* work = g.buffer<uint32_t>("pr.work", 16);
   fill_words(dev, work, 4, 0);
@@ -124,6 +206,7 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   copy_to_device(dev, state, &init, sizeof(init));
 
   Kernel spmv = get_kernel(dev, "pagerank", rows_kernel ? "pr_spmv" : "pr_spmv_hot");
+  Kernel hub_finish = get_kernel(dev, "pagerank", "pr_hub_finish");
   Kernel finalize = get_kernel(dev, "pagerank", "pr_finalize");
   const double d = static_cast<double>(damping);
   const double tol = static_cast<double>(tolerance);
@@ -131,9 +214,14 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   for (uint32_t it = 0; it < max_iterations; ++it) {
     const int in = it & 1, out = in ^ 1;
     if (rows_kernel) {
-      launch(dev, spmv, segs.total_blocks, kPrThreads, 0, off, tgt,
-             static_cast<const float*>(contrib[in]), static_cast<const float*>(rank[in]),
-             contrib[out], rank[out], n, d, segs, static_cast<const PrState*>(state), partials);
+      launch(dev, spmv, grid_blocks, kPrThreads, 0, off, tgt, static_cast<const float*>(contrib[in]),
+             static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d, segs,
+             static_cast<const PrState*>(state), partials, static_cast<const uint32_t*>(hub_chunks),
+             num_hub_chunks, hub_partials);
+      launch(dev, hub_finish, finish_blocks, kPrThreads, 0, off, static_cast<const float*>(rank[in]),
+             contrib[out], rank[out], n, d, static_cast<const PrState*>(state), hub_begin,
+             static_cast<const uint32_t*>(hub_first), static_cast<const uint32_t*>(hub_count),
+             static_cast<const double*>(hub_partials), partials, grid_blocks);
     } else {
       for (;;) {
         hipError_t err = try_launch(dev, spmv, grid_blocks, kPrHotThreads, hot_count * 4u, off, tgt,
@@ -151,7 +239,7 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
         grid_blocks = hot_grid(hot_count);
       }
     }
-    launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials), grid_blocks,
+    launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials), partial_blocks,
            tol, work);
     if ((it + 1) % kPrBatch == 0 || it + 1 == max_iterations) {
       copy_to_host(dev, dev.pinned, state, sizeof(PrState));

@@ -78,25 +78,49 @@ uint64_t run_tc_hash(Graph& g) {
   return total;
 }
 
-// Which symmetric-graph method runs (GC_TC_METHOD): "hash" (default, tc_hash),
-// "lists" (binary-search intersection, tc_count) or "wedge" (enumeration at
-// the middle vertex, tc_wedge). All three count the same triangles.
+// Triangles of a symmetric graph with the top-id bitmap of each DAG row (tc_bitmap).
+uint64_t run_tc_bitmap(Graph& g) {
+  Device& dev = *g.dev;
+  const uint32_t n = g.n;
+  constexpr uint32_t kHotBits = 32768;  // TCB_HOT_BITS of tc.hip
+  // hot range [hot_base, n): at most kHotBits ids, hot_base a multiple of 32
+  const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
+  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", 16);
+  fill_words(dev, scratch, 4, 0);
+  const uint32_t heavy = std::min<uint32_t>(n, 65536);
+  const uint32_t light_chunk = 64;
+  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+  auto* result = reinterpret_cast<unsigned long long*>(scratch);
+  launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid, 4 * kWaveSize, 0,
+         g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
+         g.rl.targets.as<const uint32_t>(), n, hot_base, heavy, light_chunk, scratch + 2, result);
+  uint64_t total = 0;
+  copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
+  std::memcpy(&total, dev.pinned, sizeof(uint64_t));
+  return total;
+}
+
+// Which symmetric-graph method runs (GC_TC_METHOD): "bitmap" (default,
+// tc_bitmap), "hash" (tc_hash), "lists" (binary-search intersection, tc_count)
+// or "wedge" (enumeration at the middle vertex, tc_wedge). All four count the
+// same triangles.
 static const char* tc_method() {
   const char* env = std::getenv("GC_TC_METHOD");
-  return env != nullptr ? env : "hash";
+  return env != nullptr ? env : "bitmap";
 }
 
 uint64_t triangle_count_symmetric(Graph& g) {
   prepare(g);
   if (g.symmetric) {
     const char* method = tc_method();
+    if (std::strcmp(method, "hash") == 0) return run_tc_hash(g);
     if (std::strcmp(method, "wedge") == 0) return run_tc_wedge(g);
     if (std::strcmp(method, "lists") == 0) {
       // Degree-oriented DAG of the relabelled graph, hubs first.
       return run_tc_count(g, g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
                           g.rl.targets.as<const uint32_t>(), true, true);
     }
-    return run_tc_hash(g);
+    return run_tc_bitmap(g);
   }
   // Any other CSR: the suffix of each list above its row, in the caller's ids.
   ensure_fwd_original(g);

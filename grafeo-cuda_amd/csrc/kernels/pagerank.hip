@@ -21,13 +21,15 @@
 using namespace gcd;
 
 #define PR_THREADS 256
-#define PR_MAX_SEGMENTS 10
+#define PR_MAX_SEGMENTS 24
+#define PR_EXACT 0x80000000u  // group flag: every row of the segment has degree (group & 0xFF)
 
 struct PrSegment {
   uint32_t row_begin;
   uint32_t row_end;
-  uint32_t group;        // lanes per row: 1, 2, ..., 64, or 256 (one workgroup)
+  uint32_t group;        // lanes per row: 1, 2, ..., 64, 256 (one workgroup), or PR_EXACT | degree
   uint32_t block_begin;  // first workgroup of the segment
+  uint32_t entry_begin;  // offsets[row_begin]
 };
 
 struct PrSegments {
@@ -117,6 +119,38 @@ __device__ __forceinline__ void pr_rows_group(const PrSegment& seg, uint32_t loc
   }
 }
 
+// Rows of one exact degree D (rows are degree-sorted, so they are contiguous
+// and row r's entries start at entry_begin + (r - row_begin) * D: no offsets
+// load). Each thread takes R consecutive rows and issues all R*D target loads,
+// then all R*D gathers, before summing.
+template <int D, int R>
+__device__ __forceinline__ void pr_rows_exact(const PrSegment& seg, uint32_t local,
+                                              const uint32_t* __restrict__ targets,
+                                              const float* __restrict__ contrib_in,
+                                              const float* __restrict__ rank_in,
+                                              float* __restrict__ contrib_out,
+                                              float* __restrict__ rank_out, double base,
+                                              double damping, double& dangling, double& diff) {
+  const uint32_t first = seg.row_begin + (local * PR_THREADS + threadIdx.x) * R;
+  if (first >= seg.row_end) return;
+  const uint32_t rows = min(static_cast<uint32_t>(R), seg.row_end - first);
+  const uint32_t e0 = seg.entry_begin + (first - seg.row_begin) * D;
+  uint32_t t[R * D > 0 ? R * D : 1];
+#pragma unroll
+  for (int k = 0; k < R * D; ++k) t[k] = k / (D > 0 ? D : 1) < static_cast<int>(rows) ? load_stream(targets + e0 + k) : 0u;
+  float x[R * D > 0 ? R * D : 1];
+#pragma unroll
+  for (int k = 0; k < R * D; ++k) x[k] = contrib_in[t[k]];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (r >= static_cast<int>(rows)) break;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc += static_cast<double>(x[r * D + j]);
+    pr_finish_row(first + r, D, acc, base, damping, rank_in, rank_out, contrib_out, dangling, diff);
+  }
+}
+
 __device__ __forceinline__ void pr_row_block(const PrSegment& seg, uint32_t local_block,
                                              const uint32_t* __restrict__ offsets,
                                              const uint32_t* __restrict__ targets,
@@ -140,23 +174,62 @@ __device__ __forceinline__ void pr_row_block(const PrSegment& seg, uint32_t loca
   __syncthreads();
 }
 
+// Workgroups [0, num_hub_chunks) each sum one chunk of a hub row (rows of
+// degree >= the hub limit are cut into chunks of at most PR_HUB_CHUNK entries,
+// heaviest rows first) into hub_partials; pr_hub_finish adds each row's chunks
+// in order. The remaining workgroups take the segments of the other rows,
+// highest-degree segment first, so no single long row sits at the tail.
+#define PR_HUB_CHUNK 4096
+
 extern "C" __global__ __launch_bounds__(PR_THREADS) void pr_spmv(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
     float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
-    PrSegments segs, const PrState* __restrict__ state, double* __restrict__ partials) {
+    PrSegments segs, const PrState* __restrict__ state, double* __restrict__ partials,
+    const uint32_t* __restrict__ hub_chunks, uint32_t num_hub_chunks,
+    double* __restrict__ hub_partials) {
   __shared__ double s_red[PR_THREADS / GC_WAVE];
   __shared__ double s_part[2][PR_THREADS / GC_WAVE];
   if (state->done) return;
-  const uint32_t blk = blockIdx.x;
-  uint32_t s = 0;
-  while (s + 1 < segs.count && blk >= segs.seg[s + 1].block_begin) ++s;
-  const PrSegment seg = segs.seg[s];
-  const uint32_t local = blk - seg.block_begin;
   const double inv_n = 1.0 / static_cast<double>(n);
   const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
   double dangling = 0.0, diff = 0.0;
-  switch (seg.group) {
+  // A resident workgroup takes virtual blocks round-robin: hub chunks first,
+  // then the segment tiles from the highest-degree segment down, so launch and
+  // reduction overheads are paid once per workgroup instead of once per tile.
+  const uint32_t num_virtual = num_hub_chunks + segs.total_blocks;
+  for (uint32_t vb = blockIdx.x; vb < num_virtual; vb += gridDim.x) {
+  if (vb < num_hub_chunks) {
+    const uint32_t e0 = hub_chunks[2 * vb], e1 = hub_chunks[2 * vb + 1];
+    double acc = pr_row_sum<PR_THREADS>(targets, contrib_in, e0 + threadIdx.x, e1);
+    acc = wave_sum(acc);
+    if (lane_id() == 0) s_red[threadIdx.x / GC_WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double total = 0.0;
+      for (int w = 0; w < PR_THREADS / GC_WAVE; ++w) total += s_red[w];
+      hub_partials[vb] = total;
+    }
+    __syncthreads();
+    continue;
+  }
+  const uint32_t vblk = segs.total_blocks - 1 - (vb - num_hub_chunks);  // heaviest segment first
+  uint32_t s = 0;
+  while (s + 1 < segs.count && vblk >= segs.seg[s + 1].block_begin) ++s;
+  const PrSegment seg = segs.seg[s];
+  const uint32_t local = vblk - seg.block_begin;
+  if (seg.group & PR_EXACT) {
+#define PR_EXACT_CASE(D, R) \
+  case D: pr_rows_exact<D, R>(seg, local, targets, contrib_in, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
+    switch (seg.group & 0xFFu) {
+      PR_EXACT_CASE(0, 4) PR_EXACT_CASE(1, 4) PR_EXACT_CASE(2, 4) PR_EXACT_CASE(3, 4)
+      PR_EXACT_CASE(4, 2) PR_EXACT_CASE(5, 2) PR_EXACT_CASE(6, 2) PR_EXACT_CASE(7, 2)
+      PR_EXACT_CASE(8, 1) PR_EXACT_CASE(9, 1) PR_EXACT_CASE(10, 1) PR_EXACT_CASE(11, 1)
+      PR_EXACT_CASE(12, 1) PR_EXACT_CASE(13, 1) PR_EXACT_CASE(14, 1) PR_EXACT_CASE(15, 1)
+      default: break;
+    }
+#undef PR_EXACT_CASE
+  } else switch (seg.group) {
     case 1: pr_rows_group<1>(seg, local, offsets, targets, contrib_in, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
     case 2: pr_rows_group<2>(seg, local, offsets, targets, contrib_in, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
     case 4: pr_rows_group<4>(seg, local, offsets, targets, contrib_in, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
@@ -165,6 +238,7 @@ extern "C" __global__ __launch_bounds__(PR_THREADS) void pr_spmv(
     case 32: pr_rows_group<32>(seg, local, offsets, targets, contrib_in, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
     case 64: pr_rows_group<64>(seg, local, offsets, targets, contrib_in, rank_in, contrib_out, rank_out, base, damping, dangling, diff); break;
     default: pr_row_block(seg, local, offsets, targets, contrib_in, rank_in, contrib_out, rank_out, base, damping, dangling, diff, s_red); break;
+  }
   }
   dangling = wave_sum(dangling);
   diff = wave_sum(diff);
@@ -180,8 +254,51 @@ extern "C" __global__ __launch_bounds__(PR_THREADS) void pr_spmv(
       d0 += s_part[0][w];
       d1 += s_part[1][w];
     }
-    partials[2 * static_cast<uint64_t>(blk)] = d0;
-    partials[2 * static_cast<uint64_t>(blk) + 1] = d1;
+    partials[2 * static_cast<uint64_t>(blockIdx.x)] = d0;
+    partials[2 * static_cast<uint64_t>(blockIdx.x) + 1] = d1;
+  }
+}
+
+// Finishes the hub rows [hub_begin, n): row hub_begin + i adds its chunk sums
+// hub_partials[chunk_first[i] .. chunk_first[i] + chunk_count[i]) in order.
+// Per-workgroup partials of the dangling and L1 sums go to partials from
+// `partial_base` on.
+extern "C" __global__ __launch_bounds__(PR_THREADS) void pr_hub_finish(
+    const uint32_t* __restrict__ offsets, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t hub_begin,
+    const uint32_t* __restrict__ chunk_first, const uint32_t* __restrict__ chunk_count,
+    const double* __restrict__ hub_partials, double* __restrict__ partials, uint32_t partial_base) {
+  __shared__ double s_part[2][PR_THREADS / GC_WAVE];
+  if (state->done) return;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  const uint32_t i = blockIdx.x * PR_THREADS + threadIdx.x;
+  if (i < n - hub_begin) {
+    const uint32_t row = hub_begin + i;
+    double acc = 0.0;
+    for (uint32_t c = 0; c < chunk_count[i]; ++c) acc += hub_partials[chunk_first[i] + c];
+    pr_finish_row(row, offsets[row + 1] - offsets[row], acc, base, damping, rank_in, rank_out,
+                  contrib_out, dangling, diff);
+  }
+  dangling = wave_sum(dangling);
+  diff = wave_sum(diff);
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  if (lane_id() == 0) {
+    s_part[0][wave] = dangling;
+    s_part[1][wave] = diff;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    for (int w = 0; w < PR_THREADS / GC_WAVE; ++w) {
+      d0 += s_part[0][w];
+      d1 += s_part[1][w];
+    }
+    const uint64_t slot = partial_base + blockIdx.x;
+    partials[2 * slot] = d0;
+    partials[2 * slot + 1] = d1;
   }
 }
 

@@ -144,14 +144,16 @@ extern "C" __global__ __launch_bounds__(64) void tc_count(
 // row), with each wave of a 256-lane workgroup working on its own vertex u:
 // the wave inserts L(u) into a private LDS hash table (open addressing,
 // linear probing, at most half full), then streams the lists L(v) of its
-// neighbours v as one flattened item range (wave prefix sum over their
-// lengths) and tests each item with one hash probe instead of a binary
-// search. The lists L(v) of the DAG's high-rank vertices are short and shared
-// by many u, so they stay cache-resident. Rows with more than TCH_TABLE/2
-// entries fall back to a binary search in global memory.
+// neighbours v one after the other, 64 consecutive entries per step, and
+// tests each entry with one hash probe. The work is instruction-bound, so the
+// per-item cost is what counts: no owner search and no binary search. The
+// lists L(v) of the DAG's high-rank vertices are short and shared by many u,
+// so they stay cache-resident. Rows with more than TCH_TABLE/2 entries fall
+// back to a binary search in global memory.
 
 #define TCH_WAVES 4
 #define TCH_TABLE 1024
+#define TCH_ILP 2
 
 __device__ __forceinline__ void wave_lds_sync() {
   // Makes this wave's LDS writes visible to its own other lanes; waves of the
@@ -166,13 +168,9 @@ extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
     const uint32_t* __restrict__ targets, uint32_t n, uint32_t heavy, uint32_t light_chunk,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
   __shared__ uint32_t s_table_all[TCH_WAVES][TCH_TABLE];
-  __shared__ uint32_t s_pos_all[TCH_WAVES][GC_WAVE];
-  __shared__ uint32_t s_start_all[TCH_WAVES][GC_WAVE];
   const uint32_t wave = threadIdx.x / GC_WAVE;
   const uint32_t lane = lane_id();
   uint32_t* s_table = s_table_all[wave];
-  uint32_t* s_pos = s_pos_all[wave];
-  uint32_t* s_start = s_start_all[wave];
   unsigned long long count = 0;
   bool heavy_left = heavy > 0;
 
@@ -230,37 +228,155 @@ extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
           st = lo[v];
           len = hi[v] - st;
         }
-        const uint32_t incl = wave_inclusive_scan(len);
-        s_pos[lane] = incl;
-        s_start[lane] = st;
-        const uint32_t total = __shfl(incl, GC_WAVE - 1);
-        wave_lds_sync();
-        for (uint32_t t = lane; t < total; t += GC_WAVE) {
-          uint32_t l = 0, h = nv - 1;
-          while (l < h) {
-            const uint32_t mid = (l + h) >> 1;
-            if (s_pos[mid] > t) h = mid;
-            else l = mid + 1;
-          }
-          const uint32_t excl = l ? s_pos[l - 1] : 0;
-          const uint32_t w = targets[s_start[l] + (t - excl)];
-          bool found;
-          if (in_table) {
-            uint32_t slot = (w * 0x9E3779B1u) >> shift;
-            uint32_t key = s_table[slot];
-            while (key != w && key != 0xFFFFFFFFu) {
-              slot = (slot + 1) & mask;
-              key = s_table[slot];
+        // One list L(v) at a time, all 64 lanes on consecutive entries: no
+        // per-item owner search (the bounds of the 64 lists were prefetched
+        // above and are broadcast from their lane), so an item costs one
+        // coalesced load and one hash probe.
+        for (uint32_t j = 0; j < nv; ++j) {
+          const uint32_t sj = __shfl(st, static_cast<int>(j));
+          const uint32_t lj = __shfl(len, static_cast<int>(j));
+          for (uint32_t k = lane; k < lj; k += TCH_ILP * GC_WAVE) {
+            uint32_t w[TCH_ILP];
+#pragma unroll
+            for (int q = 0; q < TCH_ILP; ++q) {
+              const uint32_t kk = k + q * GC_WAVE;
+              w[q] = kk < lj ? targets[sj + kk] : 0xFFFFFFFFu;
             }
-            found = key == w;
-          } else {
-            const uint32_t p = lower_bound_u32(A, 0, a, w);
-            found = p < a && A[p] == w;
+#pragma unroll
+            for (int q = 0; q < TCH_ILP; ++q) {
+              if (w[q] == 0xFFFFFFFFu) continue;
+              bool found;
+              if (in_table) {
+                uint32_t slot = (w[q] * 0x9E3779B1u) >> shift;
+                uint32_t key = s_table[slot];
+                while (key != w[q] && key != 0xFFFFFFFFu) {
+                  slot = (slot + 1) & mask;
+                  key = s_table[slot];
+                }
+                found = key == w[q];
+              } else {
+                const uint32_t p = lower_bound_u32(A, 0, a, w[q]);
+                found = p < a && A[p] == w[q];
+              }
+              count += found ? 1ull : 0ull;
+            }
           }
-          count += found ? 1ull : 0ull;
         }
-        wave_lds_sync();
       }
+    }
+  }
+  count = wave_sum(count);
+  if (lane == 0 && count != 0) atomicAdd(result, count);
+}
+
+// ---- hub-bitmap variant (default for symmetric graphs) ------------------------
+//
+// Same count as tc_hash. On the degree-ordered ids the lists L(v) point to
+// higher-degree vertices, so almost every list entry w lies among the top ids:
+// at R-MAT scale 22, 98.8% of the 2.9e10 entries probed fall in the top 32768
+// ids. Each wave keeps L(u) ∩ [hot_base, n) as a 32768-bit LDS bitmap (4 KB)
+// and tests such w with one LDS read and no branch; the few entries below
+// hot_base are searched in the (sorted) cold prefix of L(u). Bits are set
+// before and cleared after each u by walking L(u)'s hot suffix, so the
+// bitmap is never cleared whole.
+
+#define TCB_WAVES 4
+#define TCB_HOT_BITS 32768
+#define TCB_HOT_WORDS (TCB_HOT_BITS / 32)
+
+extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+    const uint32_t* __restrict__ targets, uint32_t n, uint32_t hot_base, uint32_t heavy,
+    uint32_t light_chunk, uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+  __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_HOT_WORDS];
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  const uint32_t lane = lane_id();
+  uint32_t* s_bits = s_bits_all[wave];
+  for (uint32_t j = lane; j < TCB_HOT_WORDS; j += GC_WAVE) s_bits[j] = 0;
+  wave_lds_sync();
+  unsigned long long count = 0;
+  bool heavy_left = heavy > 0;
+
+  for (;;) {
+    uint32_t kb = n, ke = n;
+    if (lane == 0) {
+      if (heavy_left) {
+        const uint32_t k = atomicAdd(&counters[0], 1u);
+        if (k < heavy) {
+          kb = k;
+          ke = k + 1;
+        } else {
+          heavy_left = false;
+        }
+      }
+      if (!heavy_left && kb == n) {
+        const uint32_t c = atomicAdd(&counters[1], light_chunk);
+        const uint64_t b = static_cast<uint64_t>(heavy) + c;
+        if (b < n) {
+          kb = static_cast<uint32_t>(b);
+          const uint64_t e = b + light_chunk;
+          ke = e < n ? static_cast<uint32_t>(e) : n;
+        }
+      }
+    }
+    kb = __shfl(kb, 0);
+    ke = __shfl(ke, 0);
+    if (kb >= n) break;
+
+    for (uint32_t k = kb; k < ke; ++k) {
+      const uint32_t u = n - 1 - k;
+      const uint32_t a_lo = lo[u];
+      const uint32_t a = hi[u] - a_lo;
+      if (a == 0) continue;
+      const uint32_t* A = targets + a_lo;
+      // cold prefix [0, c): entries below hot_base
+      const uint32_t c = lower_bound_u32(A, 0, a, hot_base);
+      for (uint32_t j = c + lane; j < a; j += GC_WAVE) {
+        const uint32_t x = A[j] - hot_base;
+        atomicOr(&s_bits[x >> 5], 1u << (x & 31));
+      }
+      wave_lds_sync();
+      uint32_t local = 0;
+      for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
+        const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
+        uint32_t st = 0, len = 0;
+        if (lane < nv) {
+          const uint32_t v = A[cv + lane];
+          st = lo[v];
+          len = hi[v] - st;
+        }
+        for (uint32_t jv = 0; jv < nv; ++jv) {
+          const uint32_t sj = __shfl(st, static_cast<int>(jv));
+          const uint32_t lj = __shfl(len, static_cast<int>(jv));
+          for (uint32_t kk = lane; kk < lj; kk += 2 * GC_WAVE) {
+            const uint32_t k2 = kk + GC_WAVE;
+            const uint32_t w0 = targets[sj + kk];
+            const uint32_t w1 = k2 < lj ? targets[sj + k2] : 0u;
+            uint32_t hit0, hit1 = 0;
+            if (w0 >= hot_base) {
+              const uint32_t x = w0 - hot_base;
+              hit0 = (s_bits[x >> 5] >> (x & 31)) & 1u;
+            } else {
+              const uint32_t p = lower_bound_u32(A, 0, c, w0);
+              hit0 = (p < c && A[p] == w0) ? 1u : 0u;
+            }
+            if (k2 < lj) {
+              if (w1 >= hot_base) {
+                const uint32_t x = w1 - hot_base;
+                hit1 = (s_bits[x >> 5] >> (x & 31)) & 1u;
+              } else {
+                const uint32_t p = lower_bound_u32(A, 0, c, w1);
+                hit1 = (p < c && A[p] == w1) ? 1u : 0u;
+              }
+            }
+            local += hit0 + hit1;
+          }
+        }
+      }
+      count += local;
+      wave_lds_sync();
+      for (uint32_t j = c + lane; j < a; j += GC_WAVE) s_bits[(A[j] - hot_base) >> 5] = 0;
+      wave_lds_sync();
     }
   }
   count = wave_sum(count);

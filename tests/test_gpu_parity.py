@@ -31,8 +31,9 @@ def upload(device, off, tgt):
 def check_triangles(name, device, off, tgt, g):
     expected = oracle.triangle_count(off, tgt)
     assert g.triangle_count()["total_triangles"] == expected, f"{name}: triangle_count"
-    # every triangle method (hash probes, binary-search lists, middle-vertex wedges) must agree
-    for method in ("lists", "wedge"):
+    # every triangle method (hub bitmap, hash probes, binary-search lists, middle-vertex
+    # wedges) must agree
+    for method in ("hash", "lists", "wedge"):
         os.environ["GC_TC_METHOD"] = method
         try:
             assert g.triangle_count()["total_triangles"] == expected, f"{name}: triangle_count ({method})"
