@@ -35,6 +35,10 @@ sys.path.insert(0, os.path.join(REPO, "grafeo-cuda_amd"))
 HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
+KERNELS = ("tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_hot", "pr_spmv", "pr_hub_finish",
+           "pr_finalize", "pr_init", "bfs_topdown_edges", "bfs_frontier_degrees", "bfs_bottomup",
+           "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "scan_block_sums",
+           "exclusive_scan_small", "scan_apply", "gather_u32", "fill_u32")
 
 
 def log(msg: str) -> None:
@@ -60,8 +64,15 @@ def main() -> int:
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--pr-iterations", type=int, default=20)
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--kernel-events", choices=["on", "off"], default="on",
+                    help="time every kernel launch of the timed steps with HIP events")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host threads (max 16)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes 5 times (the known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
 
     if args.gpus != 1:
         print(
@@ -116,22 +127,21 @@ def main() -> int:
     for i in range(args.warmup):
         r = step()
         log(f"warmup {i}: {r['times']}")
+    # Kernel durations are taken live, with HIP events recorded on the library's stream around
+    # every launch of the timed steps.
+    dev.profile(args.kernel_events == "on")
     t_start = time.perf_counter()
     results = [step() for _ in range(args.steps)]
     elapsed = time.perf_counter() - t_start
     log(f"timed {args.steps} steps in {elapsed:.3f}s")
-
-    # Per-kernel event timing in one extra (untimed) step.
-    dev.profile(True)
-    step()
     kernels = {}
-    for name in ("tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_hot", "pr_spmv", "pr_hub_finish", "pr_finalize", "bfs_topdown_edges", "bfs_frontier_degrees",
-                 "bfs_bottomup", "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "scan_block_sums",
-                 "exclusive_scan_small", "scan_apply", "gather_u32", "fill_u32", "pr_init"):
-        ms, count = dev.kernel_time(name)
-        if count:
-            kernels[name] = {"total_ms": ms, "launches": count, "avg_us": 1000.0 * ms / count}
-    dev.profile(False)
+    if args.kernel_events == "on":
+        for name in KERNELS:
+            ms, count = dev.kernel_time(name)
+            if count:
+                kernels[name] = {"total_ms": ms / args.steps, "launches": count / args.steps,
+                                 "avg_us": 1000.0 * ms / count}
+        dev.profile(False)
 
     iters = results[-1]["pr_iterations"]
     edges_per_step = m * (1 + len(sources) + iters)
@@ -159,11 +169,15 @@ def main() -> int:
     # BFS per traversal: offsets + targets read once, distances written once.
     bfs_bytes = 4 * (n + 1) + 4 * m + 4 * n
     rooflines = {}
+    # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
+    # (scaled by the readings of a copy of known size), when they have been recorded.
+    traffic = load_traffic(f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}")
 
     def add_roofline(name, bytes_per_launch, avg_us):
         a = bytes_per_launch / (avg_us * 1e-6) / 1e9
         rooflines[name] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_per_launch,
+                           "frac": a / HBM_PEAK_GBS, "traffic": traffic.get(name),
+                           "bytes_per_launch": bytes_per_launch,
                            "avg_launch_us": avg_us}
 
     if "pr_spmv" in kernels:
@@ -234,6 +248,33 @@ def main() -> int:
     check(lib.gc_device_free(dev._handle, d_rank))
     print(json.dumps(out), flush=True)
     return 0
+
+
+def calibrate_copy(nbytes: int) -> int:
+    """A streaming copy of a known size: reads `nbytes` and writes `nbytes` per launch."""
+    import torch
+
+    words = nbytes // 4
+    src = torch.arange(words, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel": "torch.neg (int32, elementwise)", "bytes_read": 4 * words,
+                      "bytes_written": 4 * words, "launches": 5}), flush=True)
+    return 0
+
+
+def load_traffic(workload_key: str) -> dict:
+    """Per-kernel HBM bytes per launch measured with rocprofv3 counters for this workload."""
+    try:
+        with open(os.path.join(REPO, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+    except (FileNotFoundError, ValueError):
+        return {}
+    if t.get("workload") != workload_key:
+        return {}
+    return {k: v["hbm_bytes_per_launch"] for k, v in t.get("kernels", {}).items()}
 
 
 def cpu_baseline(offsets, targets, sources, iters, threads, m) -> dict:
