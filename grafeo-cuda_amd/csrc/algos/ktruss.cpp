@@ -30,8 +30,10 @@ bool force_direct() {
   return env != nullptr && env[0] == '1';
 }
 
-// Compacts the flagged original entries into (u, v) pairs (host, malloc'd).
-void emit_pairs(Graph& g, const uint8_t* flags, uint32_t** edges, size_t* num_edges) {
+// Compacts the flagged original entries into (u, v) pairs (host, malloc'd),
+// and with `values`, values[entry] of the same entries into *out_values.
+void emit_pairs(Graph& g, const uint8_t* flags, uint32_t** edges, size_t* num_edges,
+                const uint32_t* values = nullptr, uint32_t** out_values = nullptr) {
   Device& dev = *g.dev;
   const uint64_t m = g.m;
   const uint32_t per_block = 256 * 16;
@@ -44,20 +46,47 @@ void emit_pairs(Graph& g, const uint8_t* flags, uint32_t** edges, size_t* num_ed
   const uint32_t total = read_u32(dev, total_dev);
   auto* host = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * 2 * std::max<size_t>(1, total)));
   if (host == nullptr) throw std::bad_alloc();
+  uint32_t* host_values = nullptr;
+  if (values != nullptr) {
+    host_values = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<size_t>(1, total)));
+    if (host_values == nullptr) {
+      std::free(host);
+      throw std::bad_alloc();
+    }
+  }
   if (total > 0) {
-    uint32_t* pairs = g.buffer<uint32_t>("kt.pairs", sizeof(uint32_t) * 2 * static_cast<size_t>(total));
-    launch(dev, get_kernel(dev, "util", "compact_emit_pairs"), blocks, 256u, 0, flags, m,
-           static_cast<const uint32_t*>(block_counts), g.offsets.as<const uint32_t>(),
-           g.targets.as<const uint32_t>(), g.n, pairs);
     try {
+      uint32_t* pairs = g.buffer<uint32_t>("kt.pairs", sizeof(uint32_t) * 2 * static_cast<size_t>(total));
+      uint32_t* dev_values =
+          values != nullptr ? g.buffer<uint32_t>("kt.values", sizeof(uint32_t) * static_cast<size_t>(total))
+                            : nullptr;
+      launch(dev, get_kernel(dev, "util", "compact_emit_pairs"), blocks, 256u, 0, flags, m,
+             static_cast<const uint32_t*>(block_counts), g.offsets.as<const uint32_t>(),
+             g.targets.as<const uint32_t>(), g.n, pairs, values, dev_values);
       copy_to_host(dev, host, pairs, sizeof(uint32_t) * 2 * static_cast<size_t>(total));
+      if (values != nullptr)
+        copy_to_host(dev, host_values, dev_values, sizeof(uint32_t) * static_cast<size_t>(total));
     } catch (...) {
       std::free(host);
+      std::free(host_values);
       throw;
     }
   }
   *edges = host;
   *num_edges = total;
+  if (out_values != nullptr) *out_values = host_values;
+}
+
+// Reverse entry of every forward relabelled entry (built once per graph).
+void ensure_reverse_index(Graph& g) {
+  if (g.ktruss_rev_ready) return;
+  Device& dev = *g.dev;
+  Grid1D vg = thread_per_item(g.n);
+  g.ktruss_rev.allocate(sizeof(uint32_t) * std::max<uint32_t>(g.m, 1));
+  launch(dev, get_kernel(dev, "ktruss", "kt_reverse_index"), vg.blocks, vg.threads, 0,
+         g.rl.offsets.as<const uint32_t>(), g.rl.targets.as<const uint32_t>(),
+         g.rl.fwd.as<const uint32_t>(), g.n, g.ktruss_rev.as<uint32_t>());
+  g.ktruss_rev_ready = true;
 }
 
 uint32_t rounds_direct(Graph& g, uint32_t threshold, uint8_t* flags) {
@@ -94,12 +123,7 @@ uint32_t rounds_dag(Graph& g, uint32_t threshold, uint8_t* flags) {
   const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
   const uint32_t* fwd = g.rl.fwd.as<const uint32_t>();
   Grid1D vg = thread_per_item(n);
-  if (!g.ktruss_rev_ready) {
-    g.ktruss_rev.allocate(sizeof(uint32_t) * std::max<uint32_t>(m, 1));
-    launch(dev, get_kernel(dev, "ktruss", "kt_reverse_index"), vg.blocks, vg.threads, 0, off, tgt,
-           fwd, n, g.ktruss_rev.as<uint32_t>());
-    g.ktruss_rev_ready = true;
-  }
+  ensure_reverse_index(g);
   uint32_t* support = g.buffer<uint32_t>("kt.support", sizeof(uint32_t) * m);
   uint8_t* removed = g.buffer<uint8_t>("kt.removed", (static_cast<size_t>(m) + 3) / 4 * 4);
   uint32_t* count = g.buffer<uint32_t>("kt.count", 16);  // [0] removed, [1..2] work counters
@@ -128,6 +152,88 @@ uint32_t rounds_dag(Graph& g, uint32_t threshold, uint8_t* flags) {
          g.offsets.as<const uint32_t>(), g.targets.as<const uint32_t>(), n, m,
          g.rl.inv.as<const uint32_t>(), off, tgt, static_cast<const uint8_t*>(removed), flags);
   return iterations;
+}
+
+// Trussness of every undirected edge of a symmetric graph (peeling on the
+// relabelled CSR; kernels kt_truss_* of ktruss.hip). Returns the largest
+// trussness (0 without edges); the per-entry values are left in the support
+// slots of the canonical entries.
+uint32_t truss_peel(Graph& g, uint32_t* support) {
+  Device& dev = *g.dev;
+  const uint32_t n = g.n, m = g.m;
+  const uint32_t* off = g.rl.offsets.as<const uint32_t>();
+  const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
+  const uint32_t* fwd = g.rl.fwd.as<const uint32_t>();
+  ensure_reverse_index(g);
+  const uint64_t m_o = g.rl.oriented_entries;
+  uint8_t* state = g.buffer<uint8_t>("kt.removed", (static_cast<size_t>(m) + 3) / 4 * 4);
+  uint32_t* canon = g.buffer<uint32_t>("kt.canon", sizeof(uint32_t) * m);
+  uint32_t* queue[2] = {g.buffer<uint32_t>("kt.queue0", sizeof(uint32_t) * std::max<uint64_t>(m_o, 1)),
+                        g.buffer<uint32_t>("kt.queue1", sizeof(uint32_t) * std::max<uint64_t>(m_o, 1))};
+  uint32_t* counters = g.buffer<uint32_t>("kt.count", 16);
+  // supports of the whole graph: every triangle once, at its middle vertex
+  fill_words(dev, state, (static_cast<uint64_t>(m) + 3) / 4, 0);
+  fill_words(dev, support, m, 0);
+  fill_words(dev, counters, 4, 0);
+  const uint32_t bits = wedge_table_bits(g.rl);
+  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+  launch(dev, get_kernel(dev, "ktruss", "kt_wedge_support"), grid, kWaveSize, (1u << bits) * 8u, off,
+         tgt, fwd, static_cast<const uint8_t*>(state), n, g.rl.hub_units.as<const uint32_t>(),
+         g.rl.num_hub_units, Relabelled::kHubChunk, bits, counters + 1, support);
+  Grid1D vg = thread_per_item(n);
+  launch(dev, get_kernel(dev, "ktruss", "kt_canonical_index"), vg.blocks, vg.threads, 0, off, fwd, n,
+         canon, state);
+  launch(dev, get_kernel(dev, "ktruss", "kt_canonical_reverse"), vg.blocks, vg.threads, 0, off, fwd,
+         g.ktruss_rev.as<const uint32_t>(), n, canon);
+
+  Kernel k_frontier = get_kernel(dev, "ktruss", "kt_truss_frontier");
+  Kernel k_peel = get_kernel(dev, "ktruss", "kt_truss_peel");
+  Kernel k_commit = get_kernel(dev, "ktruss", "kt_truss_commit");
+  const uint32_t scan_grid = std::max(1, dev.cu_count) * 8u;
+  const uint32_t peel_grid = std::max(1, dev.cu_count) * 8u;  // 4 waves per workgroup
+  uint64_t alive = m_o;
+  uint32_t k = 3, max_truss = m_o > 0 ? 2 : 0;
+  uint32_t host_counters[4];
+  auto read_counters = [&](uint32_t words) {
+    copy_to_host(dev, dev.pinned, counters, sizeof(uint32_t) * words);
+    std::memcpy(host_counters, dev.pinned, sizeof(uint32_t) * words);
+  };
+  while (alive > 0) {
+    const uint32_t t = k - 2;
+    const uint32_t init[2] = {0u, 0xFFFFFFFFu};
+    copy_to_device(dev, counters, init, sizeof(init));
+    launch(dev, k_frontier, scan_grid, 256u, 0, state, static_cast<const uint32_t*>(support),
+           static_cast<uint64_t>(m), t, queue[0], counters);
+    read_counters(2);
+    uint32_t size = host_counters[0];
+    if (size == 0) {
+      // no live edge below t: the next level that removes anything is min support + 3
+      if (host_counters[1] == 0xFFFFFFFFu) fail_kernel("truss peeling lost track of live edges");
+      k = std::max(k + 1, host_counters[1] + 3);
+      continue;
+    }
+    int cur = 0;
+    while (size > 0) {
+      fill_words(dev, counters + 2, 1, 0);
+      launch(dev, k_peel, peel_grid, 256u, 0, off, tgt, static_cast<const uint32_t*>(canon), n,
+             static_cast<const uint32_t*>(queue[cur]), size, state, support, t, queue[cur ^ 1],
+             counters + 2);
+      copy_to_host(dev, dev.pinned, counters + 2, sizeof(uint32_t));
+      uint32_t next = 0;
+      std::memcpy(&next, dev.pinned, sizeof(next));
+      const uint32_t commit_items = std::max(size, next);
+      Grid1D cg = thread_per_item(commit_items);
+      launch(dev, k_commit, cg.blocks, cg.threads, 0, static_cast<const uint32_t*>(queue[cur]), size,
+             static_cast<const uint32_t*>(queue[cur ^ 1]), next, state, support, k - 1);
+      if (size > alive) fail_kernel("truss peeling removed more edges than are live");
+      alive -= size;
+      size = next;
+      cur ^= 1;
+    }
+    max_truss = k - 1;
+    ++k;
+  }
+  return max_truss;
 }
 
 }  // namespace
@@ -161,6 +267,51 @@ gc_status gc_k_truss(gc_graph* h, uint32_t k, uint32_t** edges, size_t* num_edge
       gc::emit_pairs(g, flags, edges, num_edges);
     }
     if (iterations) *iterations = it;
+    if (secs) *secs = sw.seconds();
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+gc_status gc_truss_numbers(gc_graph* h, uint32_t** edges, uint32_t** truss, size_t* num_edges,
+                           uint32_t* max_truss, double* secs) {
+  return guard([&] {
+    gc::Stopwatch sw;
+    gc::GraphCall call(h);
+    if (edges == nullptr || truss == nullptr || num_edges == nullptr)
+      gc::fail_kernel("edges, truss or num_edges is null");
+    gc::Graph& g = *call.g;
+    *edges = nullptr;
+    *truss = nullptr;
+    *num_edges = 0;
+    uint32_t top = 0;
+    if (g.m == 0) {
+      *edges = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * 2));
+      *truss = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t)));
+      if (*edges == nullptr || *truss == nullptr) {
+        std::free(*edges);
+        std::free(*truss);
+        *edges = nullptr;
+        *truss = nullptr;
+        throw std::bad_alloc();
+      }
+    } else {
+      gc::prepare(g);
+      if (!g.symmetric) gc::fail_kernel("truss numbers need a symmetric CSR");
+      uint32_t* support = g.buffer<uint32_t>("kt.support", sizeof(uint32_t) * g.m);
+      top = gc::truss_peel(g, support);
+      uint8_t* flags = g.buffer<uint8_t>("kt.flags", g.m);
+      uint32_t* values = g.buffer<uint32_t>("kt.entry_values", sizeof(uint32_t) * g.m);
+      gc::Grid1D eg = gc::thread_per_item(g.m);
+      launch(*g.dev, gc::get_kernel(*g.dev, "ktruss", "kt_truss_values"), eg.blocks, eg.threads, 0,
+             g.offsets.as<const uint32_t>(), g.targets.as<const uint32_t>(), g.n, g.m,
+             g.rl.inv.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>(),
+             g.rl.targets.as<const uint32_t>(), static_cast<const uint32_t*>(support), flags, values);
+      gc::emit_pairs(g, flags, edges, num_edges, values, truss);
+    }
+    if (max_truss) *max_truss = top;
     if (secs) *secs = sw.seconds();
   });
 }

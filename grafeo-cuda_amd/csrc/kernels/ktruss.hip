@@ -163,3 +163,179 @@ extern "C" __global__ __launch_bounds__(256) void kt_emit_flags_relabelled(
     flags[e] = flag;
   }
 }
+
+// ----------------------------------------------------------- truss numbers --
+//
+// Per-edge trussness (the largest k whose k-truss holds the edge; 2 for an
+// edge in no triangle) of a symmetric graph, by peeling on the relabelled
+// CSR. Every undirected edge {a, b}, a < b, is identified by its canonical
+// entry: the forward entry a -> b. Level k (threshold t = k - 2) removes, in
+// batches, every live edge whose support (triangles of live edges) is below
+// t; the edges removed at level k have trussness k - 1. A batch is the
+// frontier of edges below t; for each frontier edge the triangles it closes
+// with two live edges are found by intersecting the two full rows, and each
+// such triangle is destroyed exactly once: an edge of the triangle outside
+// the frontier loses one support per destroyed triangle, and when two edges
+// of a triangle are in the frontier the one with the smaller entry index
+// does the decrement. An edge whose support falls to t - 1 joins the next
+// frontier. Triangles with an edge removed in an earlier batch were
+// destroyed then and are skipped.
+
+#define KT_ALIVE 0u
+#define KT_FRONTIER 1u
+#define KT_REMOVED 2u
+#define KT_NEXT 3u  // fell below the threshold in the current batch; still live in it
+#define KT_NONE 4u  // not a canonical entry (reverse direction or self-loop)
+
+// canon[i] = canonical entry of the undirected edge of entry i: i itself for a
+// forward entry, the forward entry of the reverse direction otherwise (self-
+// loops map to themselves and are never looked up).
+extern "C" __global__ __launch_bounds__(256) void kt_canonical_index(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ fwd, uint32_t n,
+    uint32_t* __restrict__ canon, uint8_t* __restrict__ state) {
+  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
+    const uint32_t u = static_cast<uint32_t>(uu);
+    const uint32_t f = fwd[u], e = offsets[u + 1];
+    for (uint32_t i = offsets[u]; i < f; ++i) {
+      canon[i] = i;  // overwritten below by the forward entry for non-loops
+      state[i] = static_cast<uint8_t>(KT_NONE);
+    }
+    for (uint32_t i = f; i < e; ++i) {
+      canon[i] = i;
+      state[i] = static_cast<uint8_t>(KT_ALIVE);
+    }
+  }
+}
+
+// Second pass (after kt_canonical_index has finished everywhere): the
+// reverse entry of each forward entry points at it.
+extern "C" __global__ __launch_bounds__(256) void kt_canonical_reverse(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ fwd,
+    const uint32_t* __restrict__ rev, uint32_t n, uint32_t* __restrict__ canon) {
+  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
+    const uint32_t u = static_cast<uint32_t>(uu);
+    for (uint32_t i = fwd[u]; i < offsets[u + 1]; ++i) canon[rev[i]] = i;
+  }
+}
+
+// Starts a level: every live edge with support < t joins the frontier. The
+// smallest support among the edges that stay live goes to counters[1], so
+// the host can skip levels that would remove nothing.
+extern "C" __global__ __launch_bounds__(256) void kt_truss_frontier(
+    uint8_t* __restrict__ state, const uint32_t* __restrict__ support, uint64_t m, uint32_t t,
+    uint32_t* __restrict__ queue, uint32_t* __restrict__ counters) {
+  uint32_t local_min = 0xFFFFFFFFu;
+  const uint64_t stride = grid_threads();
+  const uint64_t start = global_thread();
+  // every lane of a wave runs the same number of steps (wave-wide appends)
+  const uint64_t steps = (m + stride - 1) / stride;
+  for (uint64_t s = 0; s < steps; ++s) {
+    const uint64_t i = start + s * stride;
+    bool take = false;
+    if (i < m && state[i] == KT_ALIVE) {
+      const uint32_t sup = support[i];
+      if (sup < t) {
+        take = true;
+        state[i] = static_cast<uint8_t>(KT_FRONTIER);
+      } else {
+        local_min = min(local_min, sup);
+      }
+    }
+    const unsigned long long mask = __ballot(take);
+    if (mask) {
+      uint32_t base = 0;
+      if (lane_id() == 0) base = atomicAdd(&counters[0], static_cast<uint32_t>(__popcll(mask)));
+      base = __shfl(base, 0);
+      if (take) queue[base + lanes_below(mask)] = static_cast<uint32_t>(i);
+    }
+  }
+  for (int off = GC_WAVE / 2; off > 0; off >>= 1) local_min = min(local_min, __shfl_xor(local_min, off));
+  if (lane_id() == 0 && local_min != 0xFFFFFFFFu) atomicMin(&counters[1], local_min);
+}
+
+__device__ __forceinline__ void kt_truss_decrement(uint32_t x, uint32_t t,
+                                                   uint32_t* __restrict__ support,
+                                                   uint8_t* __restrict__ state,
+                                                   uint32_t* __restrict__ next_queue,
+                                                   uint32_t* __restrict__ next_count) {
+  const uint32_t old = atomicSub(&support[x], 1u);
+  if (old == t) {
+    state[x] = static_cast<uint8_t>(KT_NEXT);
+    next_queue[atomicAdd(next_count, 1u)] = x;
+  }
+}
+
+// One batch: one wave per frontier edge (u, v) walks the shorter of the two
+// full rows and binary-searches each neighbour w in the longer one.
+extern "C" __global__ __launch_bounds__(256) void kt_truss_peel(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ canon, uint32_t n, const uint32_t* __restrict__ queue,
+    uint32_t queue_size, uint8_t* __restrict__ state, uint32_t* __restrict__ support, uint32_t t,
+    uint32_t* __restrict__ next_queue, uint32_t* __restrict__ next_count) {
+  const uint32_t lane = lane_id();
+  const uint64_t wave = global_thread() / GC_WAVE;
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t q = wave; q < queue_size; q += waves) {
+    const uint32_t e = queue[q];
+    const uint32_t u = row_of_entry(offsets, n, e);
+    const uint32_t v = targets[e];
+    const uint32_t du = offsets[u + 1] - offsets[u], dv = offsets[v + 1] - offsets[v];
+    const uint32_t s = du <= dv ? u : v;
+    const uint32_t l = du <= dv ? v : u;
+    const uint32_t s_begin = offsets[s], s_end = offsets[s + 1];
+    const uint32_t l_begin = offsets[l], l_end = offsets[l + 1];
+    for (uint32_t es = s_begin + lane; es < s_end; es += GC_WAVE) {
+      const uint32_t w = targets[es];
+      if (w == u || w == v) continue;
+      const uint32_t p = lower_bound_u32(targets, l_begin, l_end, w);
+      if (p == l_end || targets[p] != w) continue;
+      const uint32_t c1 = canon[es], c2 = canon[p];
+      const uint32_t st1 = state[c1], st2 = state[c2];
+      if (st1 == KT_REMOVED || st2 == KT_REMOVED) continue;
+      const bool f1 = st1 == KT_FRONTIER, f2 = st2 == KT_FRONTIER;
+      if (!f1 && (!f2 || e < c2)) kt_truss_decrement(c1, t, support, state, next_queue, next_count);
+      if (!f2 && (!f1 || e < c1)) kt_truss_decrement(c2, t, support, state, next_queue, next_count);
+    }
+  }
+}
+
+// Ends a batch: the frontier is removed with trussness `truss` (stored in its
+// support slot), and the next frontier becomes the frontier.
+extern "C" __global__ __launch_bounds__(256) void kt_truss_commit(
+    const uint32_t* __restrict__ queue, uint32_t queue_size,
+    const uint32_t* __restrict__ next_queue, uint32_t next_size, uint8_t* __restrict__ state,
+    uint32_t* __restrict__ support, uint32_t truss) {
+  const uint64_t total = queue_size > next_size ? queue_size : next_size;
+  for (uint64_t i = global_thread(); i < total; i += grid_threads()) {
+    if (i < queue_size) {
+      const uint32_t x = queue[i];
+      state[x] = static_cast<uint8_t>(KT_REMOVED);
+      support[x] = truss;
+    }
+    if (i < next_size) state[next_queue[i]] = static_cast<uint8_t>(KT_FRONTIER);
+  }
+}
+
+// flags[e] = 1 and values[e] = trussness for original entries u -> v, v > u;
+// the relabelled canonical entry of {inv[u], inv[v]} holds the trussness.
+extern "C" __global__ __launch_bounds__(256) void kt_truss_values(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n,
+    uint32_t m, const uint32_t* __restrict__ inv, const uint32_t* __restrict__ r_offsets,
+    const uint32_t* __restrict__ r_targets, const uint32_t* __restrict__ truss,
+    uint8_t* __restrict__ flags, uint32_t* __restrict__ values) {
+  for (uint64_t i = global_thread(); i < m; i += grid_threads()) {
+    const uint32_t e = static_cast<uint32_t>(i);
+    const uint32_t u = row_of_entry(offsets, n, e);
+    const uint32_t v = targets[e];
+    uint8_t flag = 0;
+    uint32_t value = 0;
+    if (v > u) {
+      const uint32_t a = min(inv[u], inv[v]), b = max(inv[u], inv[v]);
+      const uint32_t p = lower_bound_u32(r_targets, r_offsets[a], r_offsets[a + 1], b);
+      flag = 1;
+      value = truss[p];
+    }
+    flags[e] = flag;
+    values[e] = value;
+  }
+}

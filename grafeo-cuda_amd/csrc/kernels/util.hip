@@ -32,7 +32,8 @@ extern "C" __global__ __launch_bounds__(256) void csr_forward_start(
 //
 // Three passes over `m` byte flags: per-block counts, one exclusive scan of
 // the counts, then each block writes its flagged entries as (row, target)
-// pairs at its scanned offset, in entry order.
+// pairs at its scanned offset, in entry order (and, when `values` is given,
+// values[entry] at the same position of `out_values`).
 
 #define COMPACT_THREADS 256
 #define COMPACT_PER_THREAD 16
@@ -96,7 +97,8 @@ extern "C" __global__ __launch_bounds__(1024) void exclusive_scan_small(uint32_t
 extern "C" __global__ __launch_bounds__(COMPACT_THREADS) void compact_emit_pairs(
     const uint8_t* __restrict__ flags, uint64_t m, const uint32_t* __restrict__ block_offsets,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n,
-    uint32_t* __restrict__ pairs) {
+    uint32_t* __restrict__ pairs, const uint32_t* __restrict__ values,
+    uint32_t* __restrict__ out_values) {
   __shared__ uint32_t s_wave[COMPACT_THREADS / GC_WAVE];
   uint64_t base = static_cast<uint64_t>(blockIdx.x) * COMPACT_PER_BLOCK +
                   static_cast<uint64_t>(threadIdx.x) * COMPACT_PER_THREAD;
@@ -127,6 +129,7 @@ extern "C" __global__ __launch_bounds__(COMPACT_THREADS) void compact_emit_pairs
     while (offsets[row + 1] <= e) ++row;
     pairs[2 * static_cast<uint64_t>(pos)] = row;
     pairs[2 * static_cast<uint64_t>(pos) + 1] = targets[e];
+    if (values != nullptr) out_values[pos] = values[e];
     ++pos;
   }
 }

@@ -82,6 +82,23 @@ class GpuDevice:
         check(lib.gc_profile_query(self._handle, kernel.encode(), byref(ms), byref(count)))
         return ms.value, count.value
 
+    def truss_numbers_numpy(self) -> dict:
+        """As truss_numbers, with 'edges' as an (E, 2) uint32 array and 'truss' as a uint32 array."""
+        ep, tp = POINTER(c_uint32)(), POINTER(c_uint32)()
+        count, top, secs = c_size_t(), c_uint32(), c_double()
+        self._call(lib.gc_truss_numbers, byref(ep), byref(tp), byref(count), byref(top), byref(secs))
+        edges = _native.take_u32(ep, 2 * count.value).reshape(-1, 2)
+        truss = _native.take_u32(tp, count.value)
+        return {"edges": edges, "truss": truss, "max_truss": top.value, "time_seconds": secs.value}
+
+    def truss_numbers(self) -> dict:
+        """Trussness of every edge: {'edges': list[tuple], 'truss': list[int], 'max_truss',
+        'time_seconds'}; k_truss(k)['edges'] lists exactly the edges with truss >= k."""
+        r = self.truss_numbers_numpy()
+        r["edges"] = [(int(u), int(v)) for u, v in r["edges"]]
+        r["truss"] = [int(t) for t in r["truss"]]
+        return r
+
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h is not None and h.value:
@@ -202,6 +219,23 @@ class GpuGraph:
         """Extract the k-truss: {'edges': list[tuple], 'k', 'iterations', 'time_seconds'}."""
         r = self.k_truss_numpy(k)
         r["edges"] = [(int(u), int(v)) for u, v in r["edges"]]
+        return r
+
+    def truss_numbers_numpy(self) -> dict:
+        """As truss_numbers, with 'edges' as an (E, 2) uint32 array and 'truss' as a uint32 array."""
+        ep, tp = POINTER(c_uint32)(), POINTER(c_uint32)()
+        count, top, secs = c_size_t(), c_uint32(), c_double()
+        self._call(lib.gc_truss_numbers, byref(ep), byref(tp), byref(count), byref(top), byref(secs))
+        edges = _native.take_u32(ep, 2 * count.value).reshape(-1, 2)
+        truss = _native.take_u32(tp, count.value)
+        return {"edges": edges, "truss": truss, "max_truss": top.value, "time_seconds": secs.value}
+
+    def truss_numbers(self) -> dict:
+        """Trussness of every edge: {'edges': list[tuple], 'truss': list[int], 'max_truss',
+        'time_seconds'}; k_truss(k)['edges'] lists exactly the edges with truss >= k."""
+        r = self.truss_numbers_numpy()
+        r["edges"] = [(int(u), int(v)) for u, v in r["edges"]]
+        r["truss"] = [int(t) for t in r["truss"]]
         return r
 
     def __del__(self):

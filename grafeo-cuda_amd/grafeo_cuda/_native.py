@@ -98,6 +98,10 @@ def _load() -> ctypes.CDLL:
             c_int,
             [c_void_p, c_uint32, POINTER(u32p), POINTER(c_size_t), u32p, POINTER(c_double)],
         ),
+        "gc_truss_numbers": (
+            c_int,
+            [c_void_p, POINTER(u32p), POINTER(u32p), POINTER(c_size_t), u32p, POINTER(c_double)],
+        ),
         "gc_profile_enable": (c_int, [c_void_p, c_int32]),
         "gc_profile_query": (c_int, [c_void_p, c_char_p, POINTER(c_double), POINTER(c_uint64)]),
         "gc_gen_erdos_renyi": (c_int, [c_uint32, c_uint64, c_uint64, POINTER(u32p), POINTER(c_size_t)]),
@@ -132,7 +136,7 @@ EXPORTED = [
     "gc_graph_device_targets", "gc_graph_prepare", "gc_graph_is_symmetric", "gc_graph_dag_stats",
     "gc_device_malloc", "gc_device_free", "gc_device_copy_to_host", "gc_triangle_count",
     "gc_triangle_count_oriented", "gc_bfs", "gc_bfs_device", "gc_pagerank", "gc_pagerank_device",
-    "gc_k_truss", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
+    "gc_k_truss", "gc_truss_numbers", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
     "gc_gen_grid", "gc_code_object_count", "gc_code_object_info",
 ]
 

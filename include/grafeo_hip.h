@@ -211,6 +211,17 @@ gc_status gc_pagerank_device(gc_graph* g, float damping, uint32_t max_iterations
 gc_status gc_k_truss(gc_graph* g, uint32_t k, uint32_t** edges, size_t* num_edges,
                      uint32_t* iterations, double* secs);
 
+/* Trussness of every edge (an addition with no reference function; it is
+ * consistent with k_truss::k_truss, crates/grafeo-cuda-algos/src/k_truss.rs:42-161):
+ * truss[i] is the largest k whose k-truss (as gc_k_truss computes it) holds
+ * edge i, or 2 for an edge in no triangle, so gc_k_truss(g, k) returns exactly
+ * the edges with truss >= k. `*edges` (pairs (u, v), u < v, in CSR order, as
+ * gc_k_truss emits them) and `*truss` are library-allocated (gc_free each);
+ * `*max_truss` is the largest value (0 for a graph without edges). Needs a
+ * symmetric CSR (GC_ERR_KERNEL otherwise). */
+gc_status gc_truss_numbers(gc_graph* g, uint32_t** edges, uint32_t** truss, size_t* num_edges,
+                           uint32_t* max_truss, double* secs);
+
 /* Device memory for the *_device entry points' outputs (hipMalloc on the
  * device of `dev`). No reference counterpart. */
 gc_status gc_device_malloc(gc_device* dev, size_t bytes, void** ptr);

@@ -516,3 +516,107 @@ uint32_t oracle_k_truss_fast(const uint32_t* off, const uint32_t* tgt, uint32_t 
   free_dag(&d);
   return iterations;
 }
+
+/* Trussness of every undirected edge of a symmetric CSR: the largest k whose
+ * k-truss holds the edge, 2 for an edge in no triangle. There is no reference
+ * function for it; it must agree with k_truss::k_truss
+ * (crates/grafeo-cuda-algos/src/k_truss.rs:42-161, restated above by
+ * oracle_k_truss_ref): the k-truss is exactly the edges with trussness >= k.
+ * Sequential peeling (Wang and Cheng, "Truss decomposition in massive
+ * networks", PVLDB 5(9), 2012): in phase k every live edge with at most k-2
+ * triangles is removed, one at a time, with trussness k; each removal
+ * destroys the triangles the edge closes with two live edges, taking one
+ * support from each of those two edges. Self-loops are ignored. `truss` gets
+ * one value per entry u -> v with u < v, in CSR order; returns that count, or
+ * -1 when the CSR is not symmetric. */
+int64_t oracle_truss_numbers(const uint32_t* off, const uint32_t* tgt, uint32_t n, uint32_t* truss) {
+  uint32_t m = off[n];
+  int64_t count = 0;
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint32_t e = off[u]; e < off[u + 1]; ++e) count += tgt[e] > u;
+  if (count == 0) return 0;
+  /* edge id of every entry: forward entries in CSR order, reverse entries via
+   * the forward entry, self-loops -1 */
+  int64_t* id = (int64_t*)malloc(sizeof(int64_t) * m);
+  uint32_t* eu = (uint32_t*)malloc(sizeof(uint32_t) * count);
+  uint32_t* ev = (uint32_t*)malloc(sizeof(uint32_t) * count);
+  int64_t next = 0;
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint32_t e = off[u]; e < off[u + 1]; ++e) {
+      id[e] = -1;
+      if (tgt[e] > u) {
+        eu[next] = u;
+        ev[next] = tgt[e];
+        id[e] = next++;
+      }
+    }
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint32_t e = off[u]; e < off[u + 1]; ++e)
+      if (tgt[e] != u) {
+        uint32_t f = find_entry(off, tgt, tgt[e], u);
+        if (f == UINT32_MAX) { /* not symmetric */
+          free(ev);
+          free(eu);
+          free(id);
+          return -1;
+        }
+        if (tgt[e] < u) id[e] = id[f];
+      }
+  int64_t* sup = (int64_t*)calloc(count, sizeof(int64_t));
+  uint8_t* gone = (uint8_t*)calloc(count, 1);
+  /* support: common neighbours other than the endpoints */
+  for (int64_t i = 0; i < count; ++i) {
+    uint32_t u = eu[i], v = ev[i];
+    uint32_t a = off[u], b = off[v];
+    while (a < off[u + 1] && b < off[v + 1]) {
+      if (tgt[a] < tgt[b]) ++a;
+      else if (tgt[a] > tgt[b]) ++b;
+      else {
+        if (tgt[a] != u && tgt[a] != v) ++sup[i];
+        ++a;
+        ++b;
+      }
+    }
+  }
+  int64_t* stack = (int64_t*)malloc(sizeof(int64_t) * count);
+  int64_t live = count;
+  for (int64_t k = 2; live > 0; ++k) {
+    int64_t top = 0;
+    for (int64_t i = 0; i < count; ++i)
+      if (!gone[i] && sup[i] <= k - 2) stack[top++] = i;
+    while (top > 0) {
+      int64_t i = stack[--top];
+      if (gone[i]) continue;
+      gone[i] = 1;
+      truss[i] = (uint32_t)k;
+      --live;
+      uint32_t u = eu[i], v = ev[i];
+      uint32_t a = off[u], b = off[v];
+      while (a < off[u + 1] && b < off[v + 1]) {
+        if (tgt[a] < tgt[b]) ++a;
+        else if (tgt[a] > tgt[b]) ++b;
+        else {
+          uint32_t w = tgt[a];
+          if (w != u && w != v) {
+            int64_t e1 = id[a], e2 = id[b];
+            if (!gone[e1] && !gone[e2]) {
+              /* an edge already at or below k-2 is on the stack; one that
+               * reaches k-2 now joins it */
+              if (--sup[e1] == k - 2) stack[top++] = e1;
+              if (--sup[e2] == k - 2) stack[top++] = e2;
+            }
+          }
+          ++a;
+          ++b;
+        }
+      }
+    }
+  }
+  free(stack);
+  free(gone);
+  free(sup);
+  free(ev);
+  free(eu);
+  free(id);
+  return count;
+}

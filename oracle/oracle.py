@@ -43,6 +43,7 @@ def _load() -> ctypes.CDLL:
         ),
         "oracle_k_truss_ref": (c_uint32, [u32p, u32p, c_uint32, c_uint32, POINTER(ctypes.c_uint8)]),
         "oracle_k_truss_fast": (c_uint32, [u32p, u32p, c_uint32, c_uint32, POINTER(ctypes.c_uint8)]),
+        "oracle_truss_numbers": (ctypes.c_int64, [u32p, u32p, c_uint32, u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -143,6 +144,19 @@ def k_truss(offsets, targets, k: int, fast: bool = False):
             if v > u and not removed[e]:
                 edges.append((u, v))
     return edges, iterations
+
+
+def truss_numbers(offsets, targets):
+    """(edges as list of (u, v) with u < v in CSR order, trussness per edge as a uint32 array)
+    of a symmetric CSR, by sequential peeling (oracle_truss_numbers)."""
+    off, tgt = _u32(offsets), _u32(targets)
+    n = off.size - 1
+    truss = np.zeros(max(int(off[-1]), 1), np.uint32)
+    count = int(lib.oracle_truss_numbers(_p(off), _p(tgt), n, _p(truss)))
+    if count < 0:
+        raise ValueError("truss numbers need a symmetric CSR")
+    edges = [(u, int(v)) for u in range(n) for v in tgt[off[u]:off[u + 1]] if v > u]
+    return edges, truss[:count].copy()
 
 
 def parse_tsv_to_edges(path: str):

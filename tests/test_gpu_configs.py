@@ -2,7 +2,8 @@
 the CPU oracle wrote (tests/golden/rmat_counts.json) and, for PageRank, against the FP64 oracle
 run live for the same number of iterations.
 
-  - R-MAT scale 20 suite: triangle count, k-truss k=4, BFS from 64 seeded sources of nonzero
+  - R-MAT scale 20 suite: triangle count, k-truss k=4, truss numbers (consistent with the 4-truss
+    and the top truss), BFS from 64 seeded sources of nonzero
     degree, PageRank d=0.85, at most 20 iterations, tol 1e-6 (reached after 17 iterations; the
     iteration count and convergence flag must match the FP64 oracle's).
   - R-MAT scale 22 PageRank precision: FP32 ranks, FP64 reductions, up to 50 iterations at tol
@@ -60,6 +61,18 @@ def test_rmat20_suite(device, golden):
     assert len(kt["edges"]) == e["k_truss_4"]["edges"]
     assert kt["iterations"] == e["k_truss_4"]["iterations"]
     assert hashlib.sha256(kt["edges"].astype("<u4").tobytes()).hexdigest() == e["k_truss_4"]["sha256"]
+
+    # truss numbers: the edges with trussness >= 4 are exactly the golden 4-truss, and the
+    # edges with trussness >= max are the max-truss (checked against the GPU k-truss)
+    tn = g.truss_numbers_numpy()
+    assert len(tn["edges"]) == (len(tgt) - int(np.sum(tgt == np.repeat(np.arange(len(off) - 1), np.diff(off))))) // 2
+    four = tn["edges"][tn["truss"] >= 4]
+    assert len(four) == e["k_truss_4"]["edges"]
+    assert hashlib.sha256(np.ascontiguousarray(four).astype("<u4").tobytes()).hexdigest() == e["k_truss_4"]["sha256"]
+    top = tn["max_truss"]
+    assert top == int(tn["truss"].max())
+    assert np.array_equal(g.k_truss_numpy(top)["edges"], tn["edges"][tn["truss"] >= top])
+    assert len(g.k_truss_numpy(top + 1)["edges"]) == 0
 
     for item in e["bfs"]:
         r = g.bfs_numpy(item["source"])

@@ -83,6 +83,14 @@ def check_k_truss(name, off, tgt, g, monkeypatch, ks=range(3, 9), both_paths=Tru
     monkeypatch.delenv("GC_KTRUSS_DIRECT")
 
 
+def check_truss_numbers(name, off, tgt, g):
+    edges, truss = oracle.truss_numbers(off, tgt)
+    r = g.truss_numbers_numpy()
+    assert [(int(u), int(v)) for u, v in r["edges"]] == edges, f"{name}: truss edges"
+    assert np.array_equal(r["truss"], truss), f"{name}: truss numbers"
+    assert r["max_truss"] == (int(truss.max()) if truss.size else 0), f"{name}: max truss"
+
+
 def test_catalog(device, monkeypatch):
     for name, off, tgt in catalog():
         g = upload(device, off, tgt)
@@ -91,6 +99,7 @@ def test_catalog(device, monkeypatch):
             check_bfs(name, off, tgt, g, source)
         check_pagerank(name, off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch)
+        check_truss_numbers(name, off, tgt, g)
 
 
 def test_random_graphs(device, monkeypatch):
@@ -106,6 +115,7 @@ def test_random_graphs(device, monkeypatch):
         check_bfs(name, off, tgt, g, n - 1)
         check_pagerank(name, off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch, ks=(3, 4, 5))
+        check_truss_numbers(name, off, tgt, g)
 
 
 def test_directed_csr_semantics(device, monkeypatch):
@@ -127,6 +137,9 @@ def test_directed_csr_semantics(device, monkeypatch):
             edges_o, it_o = oracle.k_truss(off, tgt, k)
             r = g.k_truss(k)
             assert (r["edges"], r["iterations"]) == (edges_o, it_o), f"{name} k={k}"
+        if not g.is_symmetric() and len(tgt):
+            with pytest.raises(RuntimeError, match="symmetric"):
+                g.truss_numbers()
 
 
 def test_er_4096(device, monkeypatch):
@@ -138,6 +151,7 @@ def test_er_4096(device, monkeypatch):
         check_bfs("ER-4096", off, tgt, g, source)
     check_pagerank("ER-4096", off, tgt, g)
     check_k_truss("ER-4096", off, tgt, g, monkeypatch, ks=(3, 4))
+    check_truss_numbers("ER-4096", off, tgt, g)
 
 
 @pytest.mark.parametrize("scale", [10, 14])
@@ -152,3 +166,4 @@ def test_rmat_small(device, monkeypatch, scale):
     # the per-entry path merges full lists (hub x hub), so it is checked at the small scale only
     check_k_truss(f"RMAT-{scale}", off, tgt, g, monkeypatch, ks=(3, 4, 6), both_paths=scale <= 10,
                   fast_oracle=scale > 10)
+    check_truss_numbers(f"RMAT-{scale}", off, tgt, g)

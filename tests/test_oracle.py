@@ -281,3 +281,64 @@ def test_k_truss_fast_matches_ref_at_scale():
     assert oracle.k_truss(off, tgt, 4, fast=True) == (ref, it)
     assert len(ref) == rm["k_truss_4"]["edges"]
     assert hashlib.sha256(np.asarray(ref, dtype="<u4").tobytes()).hexdigest() == rm["k_truss_4"]["sha256"]
+
+
+# ---- truss numbers ----------------------------------------------------------------
+
+
+def by_truss(edges, truss, k):
+    return [e for e, t in zip(edges, truss) if t >= k]
+
+
+def test_truss_numbers_complete_graphs():
+    for n in range(2, 9):
+        off, tgt = undirected(n, complete_edges(n))
+        edges, truss = oracle.truss_numbers(off, tgt)
+        assert edges == [tuple(e) for e in complete_edges(n)]
+        assert list(truss) == [max(n, 2)] * math.comb(n, 2), n
+    edges, truss = oracle.truss_numbers([0, 0, 0], [])
+    assert edges == [] and truss.size == 0
+
+
+def test_truss_numbers_agree_with_k_truss_on_catalog_and_random_graphs():
+    """truss(e) >= k exactly for the edges of the k-truss (oracle_k_truss_ref, itself pinned by
+    networkx and brute-force peeling above), for every k up to one past the largest value."""
+    cases = [(name, off, tgt) for name, off, tgt in catalog()]
+    rng = np.random.default_rng(21)
+    for i in range(120):
+        n, edges = random_graph(rng, max_nodes=16, max_edges=70)
+        off, tgt = undirected(n, edges)
+        cases.append((f"random #{i}", off, tgt))
+    for name, off, tgt in cases:
+        edges, truss = oracle.truss_numbers(off, tgt)
+        assert all(t >= 2 for t in truss), name
+        top = int(max(truss)) if len(truss) else 2
+        for k in range(3, top + 2):
+            ref, _ = oracle.k_truss(off, tgt, k)
+            assert by_truss(edges, truss, k) == ref, (name, k)
+        if len(off) <= 12:
+            for k in range(3, top + 2):
+                assert by_truss(edges, truss, k) == brute_truss(off, tgt, k), (name, k)
+
+
+def test_truss_numbers_by_hand():
+    # wheel: every edge in the 3-truss, none in the 4-truss; two K4 joined by a bridge: the
+    # bridge is in no triangle (2), the K4 edges are 4
+    cat = {name: (off, tgt) for name, off, tgt in catalog()}
+    edges, truss = oracle.truss_numbers(*cat["wheel"])
+    assert set(int(t) for t in truss) == {3}
+    edges, truss = oracle.truss_numbers(*cat["two K4 joined by a bridge"])
+    assert sorted(int(t) for t in truss) == [2] + [4] * 12
+
+
+def test_truss_numbers_rmat12_against_fast_k_truss():
+    off, tgt = rmat_graph(12, 16, seed=1)
+    edges, truss = oracle.truss_numbers(off, tgt)
+    for k in (3, 4, 6, 9, int(truss.max()), int(truss.max()) + 1):
+        ref, _ = oracle.k_truss(off, tgt, k, fast=True)
+        assert by_truss(edges, truss, k) == ref, k
+
+
+def test_truss_numbers_reject_directed_csr():
+    with pytest.raises(ValueError):
+        oracle.truss_numbers([0, 1, 1], [1])
