@@ -1,0 +1,249 @@
+// Multi-source BFS: up to 128 sources traversed together.
+//
+// Every vertex carries three 128-bit masks (bit s = source s), in the
+// degree-ordered relabelled ids: `seen` (sources that have reached it),
+// `visit` (sources whose frontier holds it at the current level) and `next`
+// (sources reaching it at the next level). A level is one of
+//   - top-down: each frontier vertex v ORs visit[v] & ~seen[w] into next[w]
+//     of every neighbour w (64-bit atomic ORs);
+//   - bottom-up: each vertex w still missing sources ORs visit[v] over its
+//     neighbours v and stops as soon as every missing source is covered (no
+//     atomics; needs a symmetric CSR so that w's list holds its parents);
+// followed by ms_finalize (new = next & ~seen; seen |= new; visit = new) and
+// ms_write_distances (distance = level for each new bit, in the caller's
+// ids, one array of n per source). Distances equal those of one BFS per
+// source (bfs.hip; crates/grafeo-cuda-algos/kernels/bfs.cu:54-83) whatever the
+// direction of each level.
+#include "device.hpp"
+
+using namespace gcd;
+
+struct MsCounters {
+  uint32_t frontier_vertices;  // vertices with a nonzero new mask
+  uint32_t pad;
+  unsigned long long frontier_edges;    // sum of their degrees
+  unsigned long long unexplored_edges;  // sum of degrees of vertices still missing a source
+};
+
+struct Mask {
+  unsigned long long lo, hi;
+};
+
+__device__ __forceinline__ Mask load_mask(const ulonglong2* p) {
+  const ulonglong2 v = *p;
+  return Mask{v.x, v.y};
+}
+
+__device__ __forceinline__ void store_mask(ulonglong2* p, Mask m) {
+  *p = make_ulonglong2(m.lo, m.hi);
+}
+
+__device__ __forceinline__ Mask wave_or(Mask m) {
+#pragma unroll
+  for (int o = GC_WAVE / 2; o > 0; o >>= 1) {
+    m.lo |= __shfl_xor(m.lo, o);
+    m.hi |= __shfl_xor(m.hi, o);
+  }
+  return m;
+}
+
+// Sets the bit of source s at its (relabelled) vertex; one thread, since two
+// sources may share a vertex.
+extern "C" __global__ __launch_bounds__(64) void ms_seed(const uint32_t* __restrict__ sources_rl,
+                                                         uint32_t count,
+                                                         ulonglong2* __restrict__ seen,
+                                                         ulonglong2* __restrict__ visit) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (uint32_t s = 0; s < count; ++s) {
+    const uint32_t v = sources_rl[s];
+    Mask m = load_mask(&seen[v]);
+    if (s < 64) m.lo |= 1ull << s;
+    else m.hi |= 1ull << (s - 64);
+    store_mask(&seen[v], m);
+    store_mask(&visit[v], m);
+  }
+}
+
+// distances[s * n + source_s] = 0 for every source (caller's ids).
+extern "C" __global__ __launch_bounds__(64) void ms_seed_distances(
+    const uint32_t* __restrict__ sources, uint32_t count, uint32_t n, int32_t* __restrict__ dist) {
+  const uint32_t s = threadIdx.x + blockIdx.x * blockDim.x;
+  if (s < count) dist[static_cast<uint64_t>(s) * n + sources[s]] = 0;
+}
+
+// Top-down: one wave per frontier vertex, lanes over its list.
+extern "C" __global__ __launch_bounds__(256) void ms_topdown(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ queue, uint32_t queue_size, const ulonglong2* __restrict__ visit,
+    const ulonglong2* __restrict__ seen, ulonglong2* __restrict__ next) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t q = global_thread() / GC_WAVE; q < queue_size; q += waves) {
+    const uint32_t v = queue[q];
+    const Mask mv = load_mask(&visit[v]);
+    const uint32_t end = offsets[v + 1];
+    for (uint32_t i = offsets[v] + lane; i < end; i += GC_WAVE) {
+      const uint32_t w = targets[i];
+      const Mask sw = load_mask(&seen[w]);
+      const unsigned long long add_lo = mv.lo & ~sw.lo, add_hi = mv.hi & ~sw.hi;
+      if (add_lo) atomicOr(&next[w].x, add_lo);
+      if (add_hi) atomicOr(&next[w].y, add_hi);
+    }
+  }
+}
+
+// Bottom-up for vertices [begin, end) of low degree: one thread per vertex.
+extern "C" __global__ __launch_bounds__(256) void ms_bottomup_thread(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t begin,
+    uint32_t end, unsigned long long active_lo, unsigned long long active_hi,
+    const ulonglong2* __restrict__ visit, const ulonglong2* __restrict__ seen,
+    ulonglong2* __restrict__ next) {
+  for (uint64_t ww = begin + global_thread(); ww < end; ww += grid_threads()) {
+    const uint32_t w = static_cast<uint32_t>(ww);
+    const Mask sw = load_mask(&seen[w]);
+    const unsigned long long miss_lo = active_lo & ~sw.lo, miss_hi = active_hi & ~sw.hi;
+    if ((miss_lo | miss_hi) == 0) continue;
+    unsigned long long acc_lo = 0, acc_hi = 0;
+    const uint32_t stop = offsets[w + 1];
+    for (uint32_t i = offsets[w]; i < stop; ++i) {
+      const Mask mv = load_mask(&visit[targets[i]]);
+      acc_lo |= mv.lo;
+      acc_hi |= mv.hi;
+      if ((acc_lo & miss_lo) == miss_lo && (acc_hi & miss_hi) == miss_hi) break;
+    }
+    acc_lo &= miss_lo;
+    acc_hi &= miss_hi;
+    if (acc_lo | acc_hi) store_mask(&next[w], Mask{acc_lo, acc_hi});
+  }
+}
+
+// Bottom-up for vertices [begin, end) of high degree: one wave per vertex,
+// 64 neighbours per step, stopping once the missing sources are covered.
+extern "C" __global__ __launch_bounds__(256) void ms_bottomup_wave(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t begin,
+    uint32_t end, unsigned long long active_lo, unsigned long long active_hi,
+    const ulonglong2* __restrict__ visit, const ulonglong2* __restrict__ seen,
+    ulonglong2* __restrict__ next) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t ww = begin + global_thread() / GC_WAVE; ww < end; ww += waves) {
+    const uint32_t w = static_cast<uint32_t>(ww);
+    const Mask sw = load_mask(&seen[w]);
+    const unsigned long long miss_lo = active_lo & ~sw.lo, miss_hi = active_hi & ~sw.hi;
+    if ((miss_lo | miss_hi) == 0) continue;
+    Mask acc{0, 0};
+    const uint32_t stop = offsets[w + 1];
+    for (uint32_t base = offsets[w]; base < stop; base += GC_WAVE) {
+      const uint32_t i = base + lane;
+      if (i < stop) {
+        const Mask mv = load_mask(&visit[targets[i]]);
+        acc.lo |= mv.lo;
+        acc.hi |= mv.hi;
+      }
+      acc = wave_or(acc);
+      if ((acc.lo & miss_lo) == miss_lo && (acc.hi & miss_hi) == miss_hi) break;
+    }
+    acc.lo &= miss_lo;
+    acc.hi &= miss_hi;
+    if (lane == 0 && (acc.lo | acc.hi)) store_mask(&next[w], acc);
+  }
+}
+
+// Ends a level: new = next & ~seen, seen |= new, visit = new, next = 0; the new
+// frontier is appended to `queue` and counted.
+extern "C" __global__ __launch_bounds__(256) void ms_finalize(
+    const uint32_t* __restrict__ offsets, uint32_t n, unsigned long long active_lo,
+    unsigned long long active_hi, ulonglong2* __restrict__ seen, ulonglong2* __restrict__ visit,
+    ulonglong2* __restrict__ next, uint32_t* __restrict__ queue, MsCounters* __restrict__ counters) {
+  unsigned long long f_edges = 0, u_edges = 0;
+  const uint64_t stride = grid_threads();
+  const uint64_t steps = (static_cast<uint64_t>(n) + stride - 1) / stride;
+  for (uint64_t k = 0; k < steps; ++k) {
+    const uint64_t ww = global_thread() + k * stride;
+    bool in_frontier = false;
+    if (ww < n) {
+      const uint32_t w = static_cast<uint32_t>(ww);
+      const Mask nx = load_mask(&next[w]);
+      Mask sw = load_mask(&seen[w]);
+      const Mask nw{nx.lo & ~sw.lo, nx.hi & ~sw.hi};
+      const Mask old_visit = load_mask(&visit[w]);
+      if (nx.lo | nx.hi) store_mask(&next[w], Mask{0, 0});
+      if (nw.lo | nw.hi) {
+        sw.lo |= nw.lo;
+        sw.hi |= nw.hi;
+        store_mask(&seen[w], sw);
+      }
+      if ((old_visit.lo | old_visit.hi | nw.lo | nw.hi) != 0) store_mask(&visit[w], nw);
+      const uint32_t deg = offsets[w + 1] - offsets[w];
+      if (nw.lo | nw.hi) {
+        in_frontier = true;
+        f_edges += deg;
+      }
+      if ((sw.lo & active_lo) != active_lo || (sw.hi & active_hi) != active_hi) u_edges += deg;
+    }
+    const unsigned long long mask = __ballot(in_frontier);
+    if (mask) {
+      uint32_t base = 0;
+      if (lane_id() == 0) base = atomicAdd(&counters->frontier_vertices, static_cast<uint32_t>(__popcll(mask)));
+      base = __shfl(base, 0);
+      if (in_frontier) queue[base + lanes_below(mask)] = static_cast<uint32_t>(ww);
+    }
+  }
+  f_edges = wave_sum(f_edges);
+  u_edges = wave_sum(u_edges);
+  if (lane_id() == 0) {
+    if (f_edges) atomicAdd(&counters->frontier_edges, f_edges);
+    if (u_edges) atomicAdd(&counters->unexplored_edges, u_edges);
+  }
+}
+
+// distances[s * n + u] = level for every source s newly reaching caller
+// vertex u (relabelled id inv[u]).
+extern "C" __global__ __launch_bounds__(256) void ms_write_distances(
+    const uint32_t* __restrict__ inv, uint32_t n, const ulonglong2* __restrict__ visit,
+    int32_t level, int32_t* __restrict__ dist) {
+  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
+    const uint32_t u = static_cast<uint32_t>(uu);
+    Mask m = load_mask(&visit[inv[u]]);
+    while (m.lo) {
+      const uint32_t s = __builtin_ctzll(m.lo);
+      m.lo &= m.lo - 1;
+      dist[static_cast<uint64_t>(s) * n + u] = level;
+    }
+    while (m.hi) {
+      const uint32_t s = 64u + __builtin_ctzll(m.hi);
+      m.hi &= m.hi - 1;
+      dist[static_cast<uint64_t>(s) * n + u] = level;
+    }
+  }
+}
+
+// Per source s (workgroups s * blocks_per_source ...): reachable[s] += count
+// of distances >= 0 and max_depth[s] = max distance, over the workgroup's
+// share of the n vertices.
+extern "C" __global__ __launch_bounds__(256) void ms_summarize(const int32_t* __restrict__ dist,
+                                                               uint32_t n,
+                                                               uint32_t blocks_per_source,
+                                                               uint32_t* __restrict__ reachable,
+                                                               int32_t* __restrict__ max_depth) {
+  const uint32_t s = blockIdx.x / blocks_per_source;
+  const uint32_t part = blockIdx.x % blocks_per_source;
+  const int32_t* d = dist + static_cast<uint64_t>(s) * n;
+  uint32_t count = 0;
+  int32_t deepest = 0;
+  for (uint64_t u = static_cast<uint64_t>(part) * blockDim.x + threadIdx.x; u < n;
+       u += static_cast<uint64_t>(blocks_per_source) * blockDim.x) {
+    const int32_t x = d[u];
+    if (x >= 0) {
+      ++count;
+      deepest = max(deepest, x);
+    }
+  }
+  count = wave_sum(count);
+#pragma unroll
+  for (int o = GC_WAVE / 2; o > 0; o >>= 1) deepest = max(deepest, __shfl_xor(deepest, o));
+  if (lane_id() == 0) {
+    if (count) atomicAdd(&reachable[s], count);
+    if (deepest) atomicMax(&max_depth[s], deepest);
+  }
+}

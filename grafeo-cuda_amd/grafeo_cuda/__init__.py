@@ -207,6 +207,37 @@ class GpuGraph:
         r["distances"] = [int(x) for x in r["distances"]]
         return r
 
+    def msbfs_numpy(self, sources) -> dict:
+        """BFS from every source in one call (128 sources per traversal): {'distances':
+        int32 array (len(sources), num_nodes), 'reachable': uint32 array, 'max_depth': int32
+        array, 'time_seconds'}; row s is what bfs(sources[s]) returns."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int64))
+        if src.ndim != 1:
+            raise TypeError("sources must be a sequence of node ids")
+        if src.size and (src.min() < 0 or src.max() > 0xFFFFFFFF):
+            raise OverflowError("source ids must fit in u32")
+        src = src.astype(np.uint32)
+        count = int(src.size)
+        dist = np.zeros((count, self.num_nodes), dtype=np.int32)
+        reach = np.zeros(count, dtype=np.uint32)
+        depth = np.zeros(count, dtype=np.int32)
+        secs = c_double()
+        self._call(
+            lib.gc_msbfs, src.ctypes.data_as(POINTER(c_uint32)), count,
+            dist.ctypes.data_as(POINTER(c_int32)), reach.ctypes.data_as(POINTER(c_uint32)),
+            depth.ctypes.data_as(POINTER(c_int32)), byref(secs),
+        )
+        return {"distances": dist, "reachable": reach, "max_depth": depth, "time_seconds": secs.value}
+
+    def msbfs(self, sources) -> dict:
+        """As msbfs_numpy with lists: {'distances': list of lists, 'reachable', 'max_depth',
+        'time_seconds'}."""
+        r = self.msbfs_numpy(sources)
+        r["distances"] = [[int(x) for x in row] for row in r["distances"]]
+        r["reachable"] = [int(x) for x in r["reachable"]]
+        r["max_depth"] = [int(x) for x in r["max_depth"]]
+        return r
+
     def k_truss_numpy(self, k: int) -> dict:
         """As k_truss, with 'edges' as an (E, 2) uint32 numpy array."""
         k = _u32_arg(k, "k")

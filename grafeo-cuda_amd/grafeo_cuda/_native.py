@@ -98,6 +98,14 @@ def _load() -> ctypes.CDLL:
             c_int,
             [c_void_p, c_uint32, POINTER(u32p), POINTER(c_size_t), u32p, POINTER(c_double)],
         ),
+        "gc_msbfs": (
+            c_int,
+            [c_void_p, u32p, c_uint32, i32p, u32p, i32p, POINTER(c_double)],
+        ),
+        "gc_msbfs_device": (
+            c_int,
+            [c_void_p, u32p, c_uint32, c_void_p, u32p, i32p, POINTER(c_double)],
+        ),
         "gc_truss_numbers": (
             c_int,
             [c_void_p, POINTER(u32p), POINTER(u32p), POINTER(c_size_t), u32p, POINTER(c_double)],
@@ -136,7 +144,7 @@ EXPORTED = [
     "gc_graph_device_targets", "gc_graph_prepare", "gc_graph_is_symmetric", "gc_graph_dag_stats",
     "gc_device_malloc", "gc_device_free", "gc_device_copy_to_host", "gc_triangle_count",
     "gc_triangle_count_oriented", "gc_bfs", "gc_bfs_device", "gc_pagerank", "gc_pagerank_device",
-    "gc_k_truss", "gc_truss_numbers", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
+    "gc_k_truss", "gc_truss_numbers", "gc_msbfs", "gc_msbfs_device", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
     "gc_gen_grid", "gc_code_object_count", "gc_code_object_info",
 ]
 

@@ -203,6 +203,24 @@ gc_status gc_pagerank_device(gc_graph* g, float damping, uint32_t max_iterations
                              float* d_ranks, uint32_t* iterations, int32_t* converged,
                              double* secs);
 
+/* Multi-source BFS (an addition with no reference function; for each source
+ * it returns what gc_bfs returns, bfs::bfs, crates/grafeo-cuda-algos/src/bfs.rs:42-131).
+ * `distances` (host) receives num_sources rows of num_nodes int32 distances,
+ * row s for sources[s] (-1 unreachable, 0 at the source); `reachable` and
+ * `max_depth` (host arrays of num_sources, each nullable) get every source's
+ * reachable count and last non-empty level. Up to 128 sources share one
+ * traversal (a 128-bit source mask per vertex); more run 128 at a time.
+ * Sources may repeat. A source >= num_nodes is GC_ERR_KERNEL with gc_bfs's
+ * message, checked before any work. */
+gc_status gc_msbfs(gc_graph* g, const uint32_t* sources, uint32_t num_sources, int32_t* distances,
+                   uint32_t* reachable, int32_t* max_depth, double* secs);
+
+/* As gc_msbfs, with `d_distances` a device array of num_sources * num_nodes
+ * int32 (row-major, one row per source). */
+gc_status gc_msbfs_device(gc_graph* g, const uint32_t* sources, uint32_t num_sources,
+                          int32_t* d_distances, uint32_t* reachable, int32_t* max_depth,
+                          double* secs);
+
 /* k_truss::k_truss (crates/grafeo-cuda-algos/src/k_truss.rs:42-161): peels
  * edges with fewer than k-2 supporting triangles until none is removed.
  * `*edges` (library-allocated, gc_free) receives `*num_edges` pairs (u, v),

@@ -80,6 +80,12 @@ def test_rmat20_suite(device, golden):
         assert hashlib.sha256(r["distances"].astype("<i4").tobytes()).hexdigest() == item["sha256"]
     assert bfs_is_valid(off, tgt, e["bfs"][0]["source"], g.bfs_numpy(e["bfs"][0]["source"])["distances"])
 
+    # multi-source BFS over the same 64 sources in one traversal: every row is the golden one
+    ms = g.msbfs_numpy([item["source"] for item in e["bfs"]])
+    for row, item in enumerate(e["bfs"]):
+        assert ms["reachable"][row] == item["reachable"] and ms["max_depth"][row] == item["max_depth"]
+        assert hashlib.sha256(ms["distances"][row].astype("<i4").tobytes()).hexdigest() == item["sha256"]
+
     pr = g.pagerank_numpy(0.85, 20, 1e-6)
     _, it_fp64, conv_fp64 = oracle.pagerank(off, tgt, 0.85, 20, 1e-6)
     assert (pr["iterations"], pr["converged"]) == (it_fp64, conv_fp64)
@@ -100,3 +106,25 @@ def test_rmat22_pagerank_precision(device, golden):
     print(f"RMAT-22 PageRank: {pr['iterations']} iterations, L1 {l1:.3e}, max-abs {max_abs:.3e}")
     assert l1 <= 1e-6
     assert g.triangle_count()["total_triangles"] == e["triangles"]
+
+
+def test_rmat22_msbfs_128_sources(device, golden):
+    """128-source multi-source BFS on R-MAT scale 22: sampled rows equal single-source BFS (itself
+    bit-exact against the oracle and the golden fixtures), one row passes the size-independent
+    BFS characterization, and every row's reachable / max_depth match its distances."""
+    e = golden["rmat22_ef16_seed1"]
+    off, tgt = rmat_graph(22, 16, seed=1)
+    assert csr_sha256(off, tgt) == e["csr_sha256"]
+    g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+    deg = np.diff(off.astype(np.int64))
+    sources = [int(v) for v in np.random.default_rng(128).choice(np.flatnonzero(deg > 0), size=128, replace=False)]
+    ms = g.msbfs_numpy(sources)
+    d = ms["distances"]
+    assert d.shape == (128, len(off) - 1)
+    for row in (0, 1, 64, 127):
+        single = g.bfs_numpy(sources[row])
+        assert np.array_equal(d[row], single["distances"]), row
+    assert bfs_is_valid(off, tgt, sources[5], d[5])
+    assert np.array_equal(ms["reachable"], (d >= 0).sum(axis=1).astype(np.uint32))
+    assert np.array_equal(ms["max_depth"], d.max(axis=1).astype(np.int32))
+    print(f"RMAT-22 MS-BFS 128 sources: {ms['time_seconds'] * 1e3:.1f} ms (host copy included)")

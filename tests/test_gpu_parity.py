@@ -83,6 +83,15 @@ def check_k_truss(name, off, tgt, g, monkeypatch, ks=range(3, 9), both_paths=Tru
     monkeypatch.delenv("GC_KTRUSS_DIRECT")
 
 
+def check_msbfs(name, off, tgt, g, sources):
+    r = g.msbfs_numpy(sources)
+    assert r["distances"].shape == (len(sources), len(off) - 1)
+    for row, source in enumerate(sources):
+        dist, reach, depth = oracle.bfs(off, tgt, int(source))
+        assert np.array_equal(r["distances"][row], dist), f"{name}: MS-BFS row {row} (source {source})"
+        assert r["reachable"][row] == reach and r["max_depth"][row] == depth, f"{name}: MS-BFS {source}"
+
+
 def check_truss_numbers(name, off, tgt, g):
     edges, truss = oracle.truss_numbers(off, tgt)
     r = g.truss_numbers_numpy()
@@ -100,6 +109,7 @@ def test_catalog(device, monkeypatch):
         check_pagerank(name, off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch)
         check_truss_numbers(name, off, tgt, g)
+        check_msbfs(name, off, tgt, g, list(range(len(off) - 1)))
 
 
 def test_random_graphs(device, monkeypatch):
@@ -116,6 +126,7 @@ def test_random_graphs(device, monkeypatch):
         check_pagerank(name, off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch, ks=(3, 4, 5))
         check_truss_numbers(name, off, tgt, g)
+        check_msbfs(name, off, tgt, g, list(range(n)) + [0, n - 1])
 
 
 def test_directed_csr_semantics(device, monkeypatch):
@@ -130,6 +141,7 @@ def test_directed_csr_semantics(device, monkeypatch):
         assert g.triangle_count_oriented()["total_triangles"] == oracle.triangle_count_oriented(off, tgt), name
         for source in range(n):
             check_bfs(name, off, tgt, g, source)
+        check_msbfs(name, off, tgt, g, list(range(n)))
         r = g.pagerank_numpy(0.85, 30, 0.0)
         ref, _, _ = oracle.pagerank(off, tgt, 0.85, 30, 0.0)
         assert float(np.abs(r["ranks"] - ref).sum()) <= PR_L1_TOL, name
@@ -152,6 +164,9 @@ def test_er_4096(device, monkeypatch):
     check_pagerank("ER-4096", off, tgt, g)
     check_k_truss("ER-4096", off, tgt, g, monkeypatch, ks=(3, 4))
     check_truss_numbers("ER-4096", off, tgt, g)
+    # 130 sources: one full traversal of 128 and one of 2, with repeats
+    sources = [int(x) for x in np.random.default_rng(4096).integers(0, 4096, size=127)] + [0, 0, 4095]
+    check_msbfs("ER-4096", off, tgt, g, sources)
 
 
 @pytest.mark.parametrize("scale", [10, 14])
@@ -167,3 +182,4 @@ def test_rmat_small(device, monkeypatch, scale):
     check_k_truss(f"RMAT-{scale}", off, tgt, g, monkeypatch, ks=(3, 4, 6), both_paths=scale <= 10,
                   fast_oracle=scale > 10)
     check_truss_numbers(f"RMAT-{scale}", off, tgt, g)
+    check_msbfs(f"RMAT-{scale}", off, tgt, g, [int(x) for x in np.random.default_rng(scale).integers(0, n, size=128)])
