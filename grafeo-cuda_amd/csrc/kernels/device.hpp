@@ -55,6 +55,14 @@ __device__ __forceinline__ uint32_t wave_upper_bound(uint32_t incl, uint32_t t) 
   return l;
 }
 
+// Makes this wave's LDS writes visible to its own other lanes; waves of a
+// workgroup that work independently need no workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Streaming load that should not displace reused data from the caches.
 __device__ __forceinline__ uint32_t load_stream(const uint32_t* p) {
   return __builtin_nontemporal_load(p);

@@ -155,14 +155,6 @@ extern "C" __global__ __launch_bounds__(64) void tc_count(
 #define TCH_TABLE 1024
 #define TCH_ILP 2
 
-__device__ __forceinline__ void wave_lds_sync() {
-  // Makes this wave's LDS writes visible to its own other lanes; waves of the
-  // workgroup work independently, so no workgroup barrier is needed.
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
     const uint32_t* __restrict__ targets, uint32_t n, uint32_t heavy, uint32_t light_chunk,

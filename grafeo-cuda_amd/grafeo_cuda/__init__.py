@@ -7,8 +7,9 @@ the same constructors, properties, method names, keyword defaults, result dict k
 libgrafeo_hip.so (include/grafeo_hip.h); there is no CPU fallback.
 
 Supersets of the reference surface: `from_csr` also accepts numpy arrays (no per-element
-conversion), `triangle_count_oriented`, `prepare`, `is_symmetric`, and array-returning
-variants (`*_numpy`) for large graphs. The GIL is released during device work.
+conversion), `triangle_count_oriented`, `prepare`, `is_symmetric`, `msbfs` (many BFS sources in
+one traversal), `truss_numbers`, `GpuBatch` (many graphs triangle-counted in one launch), and
+array-returning variants (`*_numpy`) for large graphs. The GIL is released during device work.
 """
 
 from __future__ import annotations
@@ -22,7 +23,7 @@ import numpy as np
 from . import _native
 from ._native import GcError, check, lib
 
-__all__ = ["GpuDevice", "GpuGraph", "GcError"]
+__all__ = ["GpuDevice", "GpuGraph", "GpuBatch", "GcError"]
 
 
 def _runtime_error(e: GcError) -> RuntimeError:
@@ -81,23 +82,6 @@ class GpuDevice:
         ms, count = c_double(), c_uint64()
         check(lib.gc_profile_query(self._handle, kernel.encode(), byref(ms), byref(count)))
         return ms.value, count.value
-
-    def truss_numbers_numpy(self) -> dict:
-        """As truss_numbers, with 'edges' as an (E, 2) uint32 array and 'truss' as a uint32 array."""
-        ep, tp = POINTER(c_uint32)(), POINTER(c_uint32)()
-        count, top, secs = c_size_t(), c_uint32(), c_double()
-        self._call(lib.gc_truss_numbers, byref(ep), byref(tp), byref(count), byref(top), byref(secs))
-        edges = _native.take_u32(ep, 2 * count.value).reshape(-1, 2)
-        truss = _native.take_u32(tp, count.value)
-        return {"edges": edges, "truss": truss, "max_truss": top.value, "time_seconds": secs.value}
-
-    def truss_numbers(self) -> dict:
-        """Trussness of every edge: {'edges': list[tuple], 'truss': list[int], 'max_truss',
-        'time_seconds'}; k_truss(k)['edges'] lists exactly the edges with truss >= k."""
-        r = self.truss_numbers_numpy()
-        r["edges"] = [(int(u), int(v)) for u, v in r["edges"]]
-        r["truss"] = [int(t) for t in r["truss"]]
-        return r
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -277,3 +261,85 @@ class GpuGraph:
 
     def __repr__(self) -> str:
         return f"GpuGraph(num_nodes={self.num_nodes}, num_edges={self.num_edges})"
+
+
+class GpuBatch:
+    """Many symmetric graphs triangle-counted together in one launch (gc_batch_*; an addition:
+    counts[i] equals GpuGraph.from_csr(*graphs[i]).triangle_count()['total_triangles'])."""
+
+    def __init__(self, handle: c_void_p, device: GpuDevice, keep):
+        self._handle = handle
+        self._device = device
+        self._keep = keep
+
+    @staticmethod
+    def from_csrs(graphs, device: GpuDevice) -> "GpuBatch":
+        """`graphs`: a sequence of (offsets, targets) pairs, each a symmetric CSR."""
+        if not isinstance(device, GpuDevice):
+            raise TypeError("device must be a GpuDevice")
+        arrays = [(_native.as_u32(o, "offsets"), _native.as_u32(t, "targets")) for o, t in graphs]
+        count = len(arrays)
+        for off, _ in arrays:
+            if off.size == 0:
+                raise RuntimeError("invalid CSR: offsets must have at least one element")
+        off_ptrs = (POINTER(c_uint32) * max(count, 1))(*[_native.u32_ptr(o) for o, _ in arrays])
+        tgt_ptrs = (POINTER(c_uint32) * max(count, 1))(*[_native.u32_ptr(t) for _, t in arrays])
+        nodes = np.asarray([o.size - 1 for o, _ in arrays] or [0], dtype=np.uint32)
+        sizes = np.asarray([t.size for _, t in arrays] or [0], dtype=np.uint64)
+        handle = c_void_p()
+        try:
+            check(
+                lib.gc_batch_from_csrs(
+                    device._handle, count, off_ptrs, nodes.ctypes.data_as(POINTER(c_uint32)), tgt_ptrs,
+                    sizes.ctypes.data_as(POINTER(c_uint64)), byref(handle),
+                )
+            )
+        except GcError as e:
+            raise _runtime_error(e) from None
+        return GpuBatch(handle, device, None)
+
+    @property
+    def num_graphs(self) -> int:
+        return lib.gc_batch_num_graphs(self._handle)
+
+    def _call(self, fn, *args) -> None:
+        try:
+            check(fn(self._handle, *args))
+        except GcError as e:
+            raise _runtime_error(e) from None
+
+    def triangle_count(self) -> dict:
+        """{'counts': uint64 array (one per graph), 'time_seconds'}."""
+        counts = np.zeros(max(self.num_graphs, 1), dtype=np.uint64)
+        secs = c_double()
+        self._call(lib.gc_batch_triangle_count, counts.ctypes.data_as(POINTER(c_uint64)), byref(secs))
+        return {"counts": counts[: self.num_graphs], "time_seconds": secs.value}
+
+    @property
+    def config(self) -> tuple:
+        """(bitmap out-degree threshold, DAG edges per merge task, rows per bitmap task,
+        workgroups per CU)."""
+        c = (c_uint32 * 4)()
+        self._call(lib.gc_batch_get_config, c)
+        return tuple(int(x) for x in c)
+
+    def set_config(self, config) -> None:
+        c = (c_uint32 * 4)(*[int(x) for x in config])
+        self._call(lib.gc_batch_set_config, c)
+
+    def tune(self) -> dict:
+        """Times candidate launch shapes on this batch and keeps the fastest:
+        {'config', 'best_ms'}."""
+        c = (c_uint32 * 4)()
+        ms = c_double()
+        self._call(lib.gc_batch_tune, c, byref(ms))
+        return {"config": tuple(int(x) for x in c), "best_ms": ms.value}
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            lib.gc_batch_free(h)
+            self._handle = None
+
+    def __repr__(self) -> str:
+        return f"GpuBatch(num_graphs={self.num_graphs})"

@@ -106,6 +106,16 @@ def _load() -> ctypes.CDLL:
             c_int,
             [c_void_p, u32p, c_uint32, c_void_p, u32p, i32p, POINTER(c_double)],
         ),
+        "gc_batch_from_csrs": (
+            c_int,
+            [c_void_p, c_uint32, POINTER(u32p), u32p, POINTER(u32p), POINTER(c_uint64), POINTER(c_void_p)],
+        ),
+        "gc_batch_free": (None, [c_void_p]),
+        "gc_batch_num_graphs": (c_uint32, [c_void_p]),
+        "gc_batch_triangle_count": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_double)]),
+        "gc_batch_set_config": (c_int, [c_void_p, u32p]),
+        "gc_batch_get_config": (c_int, [c_void_p, u32p]),
+        "gc_batch_tune": (c_int, [c_void_p, u32p, POINTER(c_double)]),
         "gc_truss_numbers": (
             c_int,
             [c_void_p, POINTER(u32p), POINTER(u32p), POINTER(c_size_t), u32p, POINTER(c_double)],
@@ -144,7 +154,9 @@ EXPORTED = [
     "gc_graph_device_targets", "gc_graph_prepare", "gc_graph_is_symmetric", "gc_graph_dag_stats",
     "gc_device_malloc", "gc_device_free", "gc_device_copy_to_host", "gc_triangle_count",
     "gc_triangle_count_oriented", "gc_bfs", "gc_bfs_device", "gc_pagerank", "gc_pagerank_device",
-    "gc_k_truss", "gc_truss_numbers", "gc_msbfs", "gc_msbfs_device", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
+    "gc_k_truss", "gc_truss_numbers", "gc_msbfs", "gc_msbfs_device", "gc_batch_from_csrs",
+    "gc_batch_free", "gc_batch_num_graphs", "gc_batch_triangle_count", "gc_batch_set_config",
+    "gc_batch_get_config", "gc_batch_tune", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
     "gc_gen_grid", "gc_code_object_count", "gc_code_object_info",
 ]
 

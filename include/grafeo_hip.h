@@ -240,6 +240,38 @@ gc_status gc_k_truss(gc_graph* g, uint32_t k, uint32_t** edges, size_t* num_edge
 gc_status gc_truss_numbers(gc_graph* g, uint32_t** edges, uint32_t** truss, size_t* num_edges,
                            uint32_t* max_truss, double* secs);
 
+/* ------------------------------------------------------------------------ */
+/* Batched triangle counting (an addition with no reference function: per   */
+/* graph it returns what gc_triangle_count returns,                           */
+/* triangle_counting::triangle_count, triangle_counting.rs:40-152)            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gc_batch gc_batch;
+
+/* Builds a batch of `num_graphs` symmetric CSRs (graph i: offsets[i] of
+ * num_nodes[i] + 1 entries, targets[i] of num_targets[i] entries). Every CSR
+ * is validated first (GC_ERR_INVALID_CSR with the graph index appended); a
+ * non-symmetric CSR is GC_ERR_KERNEL. The batch holds its own device copy of
+ * the degree-oriented DAGs and a reference to the device. */
+gc_status gc_batch_from_csrs(gc_device* dev, uint32_t num_graphs, const uint32_t* const* offsets,
+                             const uint32_t* num_nodes, const uint32_t* const* targets,
+                             const uint64_t* num_targets, gc_batch** out);
+void gc_batch_free(gc_batch* b);
+uint32_t gc_batch_num_graphs(const gc_batch* b);
+
+/* Triangles of every graph (`counts`, host, num_graphs values) from one replay
+ * of a captured HIP graph holding one persistent counting launch. */
+gc_status gc_batch_triangle_count(gc_batch* b, uint64_t* counts, double* secs);
+
+/* Launch shape: config[0] = out-degree from which a row uses the LDS bitmap
+ * (0: never), config[1] = DAG edges per merge task, config[2] = rows per
+ * bitmap task, config[3] = workgroups per CU. Results never depend on it.
+ * gc_batch_tune times candidate shapes on the batch (coordinate descent from
+ * the default), keeps the fastest and returns it with its time in ms. */
+gc_status gc_batch_set_config(gc_batch* b, const uint32_t* config);
+gc_status gc_batch_get_config(gc_batch* b, uint32_t* config);
+gc_status gc_batch_tune(gc_batch* b, uint32_t* config, double* best_ms);
+
 /* Device memory for the *_device entry points' outputs (hipMalloc on the
  * device of `dev`). No reference counterpart. */
 gc_status gc_device_malloc(gc_device* dev, size_t bytes, void** ptr);

@@ -151,10 +151,30 @@ def make_rmat(large: bool):
     return out
 
 
+def make_batch():
+    """The 256-graph uneven batch (grafeo_cuda.workloads.uneven_batch, seed 1): per graph its
+    size, CSR SHA-256 and triangle count from the C oracle's fast variant (checked against
+    networkx and the literal restatement by the CPU tests)."""
+    from grafeo_cuda import workloads
+
+    graphs = []
+    for off, tgt in workloads.uneven_batch(256, 1):
+        graphs.append({"num_nodes": len(off) - 1, "csr_entries": int(len(tgt)),
+                       "csr_sha256": csr_sha256(off, tgt),
+                       "triangles": int(oracle.triangle_count(off, tgt, fast=True))})
+    return {"uneven_batch_256_seed1": {"source": "oracle_triangle_count_fast", "graphs": graphs,
+                                       "total_triangles": sum(g["triangles"] for g in graphs)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--large", action="store_true", help="also R-MAT 16..22 (minutes)")
+    ap.add_argument("--only-batch", action="store_true", help="only batch_counts.json")
     args = ap.parse_args()
+    with open(os.path.join(HERE, "batch_counts.json"), "w") as f:
+        json.dump(make_batch(), f, indent=1)
+    if args.only_batch:
+        return
     with open(os.path.join(HERE, "catalog.json"), "w") as f:
         json.dump(make_catalog(), f, indent=1)
     with open(os.path.join(HERE, "er4096.json"), "w") as f:
