@@ -66,6 +66,10 @@ def main() -> int:
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--kernel-events", choices=["on", "off"], default="on",
                     help="time every kernel launch of the timed steps with HIP events")
+    ap.add_argument("--batched", choices=["on", "off"], default="on",
+                    help="also time the batched schedules against the one-at-a-time ones: 128-source "
+                         "multi-source BFS vs 128 single-source BFS, and the 256-graph triangle-count "
+                         "batch (default and autotuned shape) vs one call per graph")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host threads (max 16)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 5 times (the known-size kernel "
@@ -209,6 +213,10 @@ def main() -> int:
     except FileNotFoundError:
         pass
 
+    batched = None
+    if args.batched == "on":
+        batched = batched_section(dev, g, offsets, n, m, args)
+
     cpu = None
     if args.cpu_baseline == "on":
         cpu = cpu_baseline(offsets, targets, sources, iters, args.cpu_threads, m)
@@ -242,12 +250,121 @@ def main() -> int:
         "triangles": results[-1]["triangles"],
         "parity": parity,
         "cpu_baseline": cpu,
+        "batched": batched,
         "device": dev.name,
     }
     check(lib.gc_device_free(dev._handle, d_dist))
     check(lib.gc_device_free(dev._handle, d_rank))
     print(json.dumps(out), flush=True)
     return 0
+
+
+def batched_section(dev, g, offsets, n, m, args) -> dict:
+    """Batched schedules against the same work done one unit at a time, on the same GPU, same
+    inputs, bit-identical results (checked here)."""
+    import grafeo_cuda
+    from grafeo_cuda import workloads
+    from grafeo_cuda._native import check, lib
+
+    out = {}
+    # -- 128-source multi-source BFS vs 128 single-source BFS (same sources) --
+    sources = pick_sources(offsets, 128, args.seed + 128)
+    src = np.asarray(sources, dtype=np.uint32)
+    src_p = src.ctypes.data_as(ctypes.POINTER(c_uint32))
+    d_ms, d_one = c_void_p(), c_void_p()
+    check(lib.gc_device_malloc(dev._handle, 4 * n * len(sources), byref(d_ms)))
+    check(lib.gc_device_malloc(dev._handle, 4 * n, byref(d_one)))
+    reach = np.zeros(len(sources), np.uint32)
+    depth = np.zeros(len(sources), np.int32)
+    secs = c_double()
+
+    def run_ms():
+        check(lib.gc_msbfs_device(g._handle, src_p, len(sources), d_ms,
+                                  reach.ctypes.data_as(ctypes.POINTER(c_uint32)),
+                                  depth.ctypes.data_as(ctypes.POINTER(c_int32)), byref(secs)))
+        return secs.value
+
+    def run_single():
+        total, r, d = 0.0, c_uint32(), c_int32()
+        reach1 = []
+        for s in sources:
+            check(lib.gc_bfs_device(g._handle, s, d_one, byref(r), byref(d), byref(secs)))
+            total += secs.value
+            reach1.append(r.value)
+        return total, reach1
+
+    run_ms()
+    ms_t = min(run_ms() for _ in range(3))
+    run_single()
+    single_t, reach1 = min((run_single() for _ in range(2)), key=lambda x: x[0])
+    # row 0 of the batched distances against the single-source distances of source 0
+    row = np.empty(n, np.int32)
+    one = np.empty(n, np.int32)
+    check(lib.gc_device_copy_to_host(dev._handle, row.ctypes.data_as(c_void_p), d_ms, 4 * n))
+    rr, dd = c_uint32(), c_int32()
+    check(lib.gc_bfs_device(g._handle, sources[0], d_one, byref(rr), byref(dd), byref(secs)))
+    check(lib.gc_device_copy_to_host(dev._handle, one.ctypes.data_as(c_void_p), d_one, 4 * n))
+    check(lib.gc_device_free(dev._handle, d_ms))
+    check(lib.gc_device_free(dev._handle, d_one))
+    out["msbfs"] = {
+        "sources": len(sources), "ms": 1000 * ms_t, "gteps": m * len(sources) / ms_t / 1e9,
+        "single_source_ms": 1000 * single_t, "single_source_gteps": m * len(sources) / single_t / 1e9,
+        "speedup": single_t / ms_t,
+        "identical": bool(list(reach) == reach1 and np.array_equal(row, one)),
+    }
+    log(f"msbfs: {out['msbfs']}")
+
+    # -- 256-graph triangle-count batch vs one call per graph --
+    t0 = time.time()
+    graphs = workloads.uneven_batch(256, 1)
+    entries = sum(int(t.size) for _, t in graphs)
+    batch = grafeo_cuda.GpuBatch.from_csrs(graphs, dev)
+    singles = [grafeo_cuda.GpuGraph.from_csr(o, t, dev) for o, t in graphs]
+    for s_g in singles:
+        s_g.prepare()
+    log(f"batch of 256 graphs ({entries} CSR entries) built in {time.time() - t0:.1f}s")
+    expected = None
+    try:
+        with open(os.path.join(REPO, "tests", "golden", "batch_counts.json")) as f:
+            expected = [x["triangles"] for x in json.load(f)["uneven_batch_256_seed1"]["graphs"]]
+    except (FileNotFoundError, KeyError, ValueError):
+        pass
+
+    def time_batch(reps=10):
+        batch.triangle_count()
+        best, counts = 1e30, None
+        for _ in range(reps):
+            r = batch.triangle_count()
+            best, counts = min(best, r["time_seconds"]), r["counts"]
+        return best, [int(c) for c in counts]
+
+    default_cfg = batch.config
+    default_t, counts_default = time_batch()
+    tuned = batch.tune()
+    tuned_t, counts_tuned = time_batch()
+
+    def time_singles():
+        total, counts = 0.0, []
+        for s_g in singles:
+            r = s_g.triangle_count()
+            total += r["time_seconds"]
+            counts.append(r["total_triangles"])
+        return total, counts
+
+    time_singles()
+    single_t, counts_single = min((time_singles() for _ in range(2)), key=lambda x: x[0])
+    out["batch_tc"] = {
+        "graphs": len(graphs), "csr_entries": entries,
+        "default_config": list(default_cfg), "default_ms": 1000 * default_t,
+        "tuned_config": list(tuned["config"]), "tuned_ms": 1000 * tuned_t,
+        "tuned_gteps": entries / tuned_t / 1e9,
+        "one_call_per_graph_ms": 1000 * single_t, "speedup_vs_one_call_per_graph": single_t / tuned_t,
+        "speedup_tuned_vs_default": default_t / tuned_t,
+        "identical": counts_default == counts_tuned == counts_single,
+        "match_golden": None if expected is None else counts_tuned == expected,
+    }
+    log(f"batch_tc: {out['batch_tc']}")
+    return out
 
 
 def calibrate_copy(nbytes: int) -> int:
