@@ -74,9 +74,19 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 5 times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
+    ap.add_argument("--workload", choices=["step", "scheduler"], default="step",
+                    help="step: the R-MAT scale-22 step above; scheduler: the learned schedules "
+                         "against the fixed rules on graphs they were not trained on (R-MAT scale 24, "
+                         "4096 x 4096 grid)")
+    ap.add_argument("--model", default=None, help="learned schedules (default: the committed model)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
+    if args.workload == "scheduler":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU", "n_gpus": args.gpus}), flush=True)
+            return 2
+        return scheduler_workload(args)
 
     if args.gpus != 1:
         print(
@@ -365,6 +375,73 @@ def batched_section(dev, g, offsets, n, m, args) -> dict:
     }
     log(f"batch_tc: {out['batch_tc']}")
     return out
+
+
+def scheduler_workload(args) -> int:
+    """Learned schedules against the fixed rules (Beamer's direction switch for BFS; the bitmap
+    intersection for triangle counting) on the same GPU and inputs; results must be identical."""
+    import grafeo_cuda
+    from grafeo_cuda import scheduler
+    from grafeo_cuda._native import check, lib
+
+    dev = grafeo_cuda.GpuDevice(0)
+    bfs_s, tc_s = scheduler.load(dev, args.model or scheduler.MODEL)
+    out = {"metric": "learned vs fixed schedule (BFS per level, TC per degree bucket)",
+           "unit": "ms", "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "graphs": {}}
+    cases = [("rmat24", lambda: scheduler.rmat(24, 16, 1), 4), ("grid4096", lambda: scheduler.grid(4096, 4096), 0)]
+    for name, build, nsrc in cases:
+        t0 = time.time()
+        off, tgt = build()
+        n = off.size - 1
+        sources = pick_sources(off, nsrc, args.seed) if nsrc else [n // 2 + 2048, 0]
+        g = grafeo_cuda.GpuGraph.from_csr(off, tgt, dev)
+        del tgt
+        g.prepare()
+        log(f"{name}: n={n} m={g.num_edges} ready in {time.time() - t0:.1f}s")
+        d_dist = c_void_p()
+        check(lib.gc_device_malloc(dev._handle, 4 * n, byref(d_dist)))
+        host = np.empty(n, np.int32)
+
+        def bfs_times():
+            total, checks = 0.0, []
+            for s in sources:
+                secs, reach, depth = c_double(), c_uint32(), c_int32()
+                check(lib.gc_bfs_device(g._handle, s, d_dist, byref(reach), byref(depth), byref(secs)))
+                best = secs.value
+                check(lib.gc_bfs_device(g._handle, s, d_dist, byref(reach), byref(depth), byref(secs)))
+                best = min(best, secs.value)
+                check(lib.gc_device_copy_to_host(dev._handle, host.ctypes.data_as(c_void_p), d_dist, 4 * n))
+                checks.append((reach.value, depth.value, hash(host.tobytes())))
+                total += best
+            return total, checks
+
+        def tc_time():
+            g.triangle_count()
+            r = [g.triangle_count() for _ in range(2)]
+            return min(x["time_seconds"] for x in r), r[0]["total_triangles"]
+
+        g.set_scheduler()
+        fixed_bfs, fixed_checks = bfs_times()
+        fixed_tc, fixed_tri = tc_time()
+        g.set_scheduler(bfs_s, tc_s)
+        learned_bfs, learned_checks = bfs_times()
+        bfs_schedule = g.last_schedule()
+        learned_tc, learned_tri = tc_time()
+        tc_schedule = g.last_schedule()
+        check(lib.gc_device_free(dev._handle, d_dist))
+        out["graphs"][name] = {
+            "num_nodes": n, "csr_entries": g.num_edges, "bfs_sources": len(sources),
+            "bfs_fixed_ms": 1000 * fixed_bfs, "bfs_learned_ms": 1000 * learned_bfs,
+            "bfs_speedup": fixed_bfs / learned_bfs, "bfs_identical": fixed_checks == learned_checks,
+            "bfs_last_schedule": {str(c): bfs_schedule.count(c) for c in (0, 1, 2)},
+            "tc_fixed_ms": 1000 * fixed_tc, "tc_learned_ms": 1000 * learned_tc,
+            "tc_speedup": fixed_tc / learned_tc, "tc_identical": fixed_tri == learned_tri,
+            "tc_schedule": tc_schedule, "triangles": fixed_tri,
+        }
+        log(f"{name}: {out['graphs'][name]}")
+        del g
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def calibrate_copy(nbytes: int) -> int:

@@ -12,10 +12,14 @@
 // Distances do not depend on the schedule, so both directions give the
 // reference's result bit for bit. A bottom-up step looks for parents in a
 // vertex's own list, which holds its in-neighbours only when the CSR is
-// symmetric; other CSRs stay top-down.
+// symmetric; other CSRs stay top-down. With a learned scheduler attached
+// (sched.cpp) the network picks each level among top-down, bottom-up and the
+// device-chained levels (captured HIP graphs of many levels, read back once
+// per chain), from features computed on the device after each level.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 #include "../runtime/common.hpp"
 
@@ -29,118 +33,435 @@ struct BfsCounters {
   unsigned long long next_edges;
 };
 
+struct BfsChainState {  // bfs.hip
+  uint32_t size;
+  int32_t level;
+  uint32_t stop;
+  uint32_t levels;
+  uint32_t next_size;
+  uint32_t cap;
+  unsigned long long next_edges;
+  unsigned long long edges;
+  unsigned long long all_edges;
+  unsigned long long reached;
+  unsigned long long edge_cap;
+};
+
+struct BfsLevelState {  // mlp.hip
+  unsigned long long m, m_f, m_u, reached;
+  uint32_t n, n_f, prev_n_f, level, bitmap, max_degree, pad0, pad1;
+};
+
+struct SchedDecision {  // mlp.hip
+  float features[kSchedFeatures];
+  float predicted[kSchedChoices];
+  uint32_t choice;
+};
+
+// Level counters and the scheduler's decision, read back together.
+struct BfsReadback {
+  BfsCounters counters[2];
+  SchedDecision decision;
+};
+
 constexpr double kAlpha = 14.0;
 constexpr double kBeta = 24.0;
+constexpr uint32_t kTopDown = 0, kBottomUp = 1, kChained = 2;
+// Device-chained levels (choice 2): the largest frontier they take or keep
+// going with, in vertices and in edges, and the levels of one captured chain
+// (even, so that the frontier is back in the starting queue after a chain).
+constexpr uint32_t kChainCap = 1u << 18;
+constexpr uint64_t kChainEdges = 1ull << 20;
+constexpr uint32_t kChainRunLevels = 1u << 30;
+constexpr uint32_t kChainLevels = 16;
+
+// One traversal from one source on the relabelled graph: the frontier is a
+// queue (top-down, chained levels) or a bitmap (bottom-up), converted
+// on demand; each level's counters are read back before the next choice.
+struct Traversal {
+  Graph& g;
+  Device& dev;
+  uint32_t n;
+  uint64_t words64;
+  int32_t* dist;
+  uint32_t* queue[2];
+  uint64_t* bitmap[2];
+  BfsReadback* rb;
+  uint32_t* start;
+  uint32_t* scan_buf;
+  BfsChainState* chain_state;
+  const uint32_t* off;
+  const uint32_t* tgt;
+  Kernel k_td, k_deg, k_bu, k_q2b, k_b2q, k_chain_expand, k_chain_advance, k_decide;
+  uint32_t cu;
+  // state at the start of the next level
+  uint64_t n_f = 1, m_f = 0, m_u = 0, reached = 1, prev_n_f = 1;
+  int32_t level = 0;
+  bool in_bitmap = false;
+  int cur = 0;
+  bool done = false;
+  bool top_down_dir = true;  // the fixed rule's current direction
+
+  Traversal(Graph& graph, uint32_t src)
+      : g(graph), dev(*graph.dev), n(graph.n), words64((static_cast<uint64_t>(graph.n) + 63) / 64) {
+    dist = g.buffer<int32_t>("bfs.dist", sizeof(int32_t) * n);
+    queue[0] = g.buffer<uint32_t>("bfs.q0", sizeof(uint32_t) * n);
+    queue[1] = g.buffer<uint32_t>("bfs.q1", sizeof(uint32_t) * n);
+    bitmap[0] = g.buffer<uint64_t>("bfs.b0", sizeof(uint64_t) * words64);
+    bitmap[1] = g.buffer<uint64_t>("bfs.b1", sizeof(uint64_t) * words64);
+    rb = g.buffer<BfsReadback>("bfs.readback", sizeof(BfsReadback));
+    start = g.buffer<uint32_t>("bfs.start", sizeof(uint32_t) * n);
+    scan_buf = g.buffer<uint32_t>("bfs.scan", sizeof(uint32_t) * scan_scratch_words(n));
+    chain_state = g.buffer<BfsChainState>("bfs.chain", sizeof(BfsChainState));
+    off = g.rl.offsets.as<const uint32_t>();
+    tgt = g.rl.targets.as<const uint32_t>();
+    k_td = get_kernel(dev, "bfs", "bfs_topdown_edges");
+    k_deg = get_kernel(dev, "bfs", "bfs_frontier_degrees");
+    k_bu = get_kernel(dev, "bfs", "bfs_bottomup");
+    k_q2b = get_kernel(dev, "bfs", "bfs_queue_to_bitmap");
+    k_b2q = get_kernel(dev, "bfs", "bfs_bitmap_to_queue");
+    k_chain_expand = get_kernel(dev, "bfs", "bfs_chain_expand");
+    k_chain_advance = get_kernel(dev, "bfs", "bfs_chain_advance");
+    k_decide = get_kernel(dev, "mlp", "sched_bfs_decide");
+    cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
+    const std::vector<uint32_t>& hoff = g.rl.host_offsets;
+    Grid1D vgrid = thread_per_item(n);
+    launch(dev, get_kernel(dev, "bfs", "bfs_init"), vgrid.blocks, vgrid.threads, 0, dist, n, src,
+           queue[0]);
+    m_f = hoff[src + 1] - hoff[src];
+    m_u = static_cast<uint64_t>(g.m) - m_f;
+  }
+
+  uint32_t allowed_mask() const {
+    uint32_t mask = 1u << kTopDown;
+    if (g.symmetric) mask |= 1u << kBottomUp;
+    if (n_f <= kChainCap && m_f <= kChainEdges) mask |= 1u << kChained;
+    return mask;
+  }
+
+  BfsLevelState level_state() const {
+    BfsLevelState st{};
+    st.m = g.m;
+    st.m_f = m_f;
+    st.m_u = m_u;
+    st.reached = reached;
+    st.n = n;
+    st.n_f = static_cast<uint32_t>(n_f);
+    st.prev_n_f = static_cast<uint32_t>(prev_n_f);
+    st.level = static_cast<uint32_t>(level);
+    st.bitmap = in_bitmap ? 1u : 0u;
+    st.max_degree = g.rl.max_degree;
+    return st;
+  }
+
+  // The fixed rule: Beamer's direction switch (never the chained levels).
+  uint32_t fixed_choice() {
+    if (top_down_dir && g.symmetric && static_cast<double>(m_f) > static_cast<double>(m_u) / kAlpha) {
+      top_down_dir = false;
+    } else if (!top_down_dir && static_cast<double>(n_f) < static_cast<double>(n) / kBeta) {
+      top_down_dir = true;
+    }
+    return top_down_dir ? kTopDown : kBottomUp;
+  }
+
+  // Launches the decision kernel for the current state (or, with
+  // from_counters, for the state after the level whose counters are in rb).
+  void launch_decide(const float* params, bool from_counters) {
+    const SchedNorm norm = g.bfs_sched != nullptr ? g.bfs_sched->norm : SchedNorm{};
+    // the kernel drops the chained levels itself when the frontier is too large
+    const uint32_t mask = (1u << kTopDown) | (g.symmetric ? (1u << kBottomUp) : 0u) | (1u << kChained);
+    launch(dev, k_decide, 1u, 64u, 0, params, norm, level_state(),
+           static_cast<const BfsCounters*>(&rb->counters[0]), from_counters ? 1u : 0u, mask,
+           kChainCap, static_cast<unsigned long long>(kChainEdges), &rb->decision);
+  }
+
+  void to_queue() {
+    if (!in_bitmap) return;
+    fill_words(dev, &rb->counters[1], sizeof(BfsCounters) / 4, 0);
+    Grid1D bg = wave_per_item((words64 + 63) / 64);
+    launch(dev, k_b2q, std::min(bg.blocks, cu * 16), bg.threads, 0,
+           static_cast<const uint64_t*>(bitmap[cur]), words64, queue[cur], &rb->counters[1].next_size);
+    in_bitmap = false;
+  }
+
+  void to_bitmap() {
+    if (in_bitmap) return;
+    fill_words(dev, bitmap[cur], 2 * words64, 0);
+    Grid1D qg = thread_per_item(n_f);
+    launch(dev, k_q2b, qg.blocks, qg.threads, 0, static_cast<const uint32_t*>(queue[cur]),
+           static_cast<uint32_t>(n_f), reinterpret_cast<uint32_t*>(bitmap[cur]));
+    in_bitmap = true;
+  }
+
+  // Launches one top-down or bottom-up level (counters into rb->counters[0]).
+  void launch_level(uint32_t choice) {
+    fill_words(dev, &rb->counters[0], sizeof(BfsCounters) / 4, 0);
+    const int32_t next_level = level + 1;
+    if (choice == kTopDown) {
+      to_queue();
+      Grid1D qg = thread_per_item(n_f);
+      launch(dev, k_deg, qg.blocks, qg.threads, 0, off, static_cast<const uint32_t*>(queue[cur]),
+             static_cast<uint32_t>(n_f), start);
+      exclusive_scan_u32(dev, start, n_f, scan_buf);
+      Grid1D tg = wave_per_item((m_f + 255) / 256);
+      launch(dev, k_td, std::max(1u, std::min(tg.blocks, cu * 32)), tg.threads, 0, off, tgt, dist,
+             static_cast<const uint32_t*>(queue[cur]), static_cast<uint32_t>(n_f),
+             static_cast<const uint32_t*>(start), static_cast<uint32_t>(m_f), queue[cur ^ 1],
+             &rb->counters[0], next_level);
+    } else {
+      to_bitmap();
+      Grid1D bg = wave_per_item(words64);
+      launch(dev, k_bu, std::min(bg.blocks, cu * 32), bg.threads, 0, off, tgt, dist,
+             reinterpret_cast<const uint32_t*>(bitmap[cur]), bitmap[cur ^ 1], n, &rb->counters[0],
+             next_level);
+    }
+  }
+
+  // Applies the counters of the level just run (already copied to `c`).
+  void finish_level(uint32_t choice, const BfsCounters& c) {
+    if (c.next_size == 0) {
+      done = true;
+      return;
+    }
+    prev_n_f = n_f;
+    level += 1;
+    reached += c.next_size;
+    m_u -= std::min<uint64_t>(m_u, c.next_edges);
+    m_f = c.next_edges;
+    n_f = c.next_size;
+    in_bitmap = choice == kBottomUp;
+    cur ^= 1;
+  }
+
+  uint32_t chain_grid() const { return cu * 8; }
+
+  // The captured chain of kChainLevels levels starting from queue[start].
+  hipGraphExec_t chain_exec(int start) {
+    GraphExecs& ge = g.bfs_chain;
+    if (ge.exec[start] != nullptr) return ge.exec[start];
+    dev.sync();
+    hip_check(hipStreamBeginCapture(dev.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t err = hipSuccess;
+    for (uint32_t k = 0; k < kChainLevels && err == hipSuccess; ++k) {
+      const uint32_t* in = queue[start ^ static_cast<int>(k & 1)];
+      uint32_t* out = queue[start ^ static_cast<int>((k + 1) & 1)];
+      void* expand_params[] = {karg(off), karg(tgt), karg(dist), karg(in), karg(out), karg(chain_state)};
+      err = hipModuleLaunchKernel(k_chain_expand.fn, chain_grid(), 1, 1, 256, 1, 1, 0, dev.stream,
+                                  expand_params, nullptr);
+      if (err == hipSuccess) {
+        void* advance_params[] = {karg(chain_state)};
+        err = hipModuleLaunchKernel(k_chain_advance.fn, 1, 1, 1, 64, 1, 1, 0, dev.stream, advance_params,
+                                    nullptr);
+      }
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(dev.stream, &graph);
+    hip_check(err, "BFS level chain capture");
+    hip_check(end, "hipStreamEndCapture");
+    ge.graph[start] = graph;
+    hip_check(hipGraphInstantiate(&ge.exec[start], graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return ge.exec[start];
+  }
+
+  // Device-side levels, captured chains of kChainLevels levels replayed with
+  // one read-back each, until the traversal ends, hands back a large frontier
+  // or has built max_levels levels (checked after each chain); with
+  // max_levels == 1, one level launched directly. Returns the levels built.
+  uint32_t run_chain(uint32_t max_levels) {
+    to_queue();
+    BfsChainState init{};
+    init.size = static_cast<uint32_t>(n_f);
+    init.level = level;
+    init.cap = kChainCap;
+    init.edges = m_f;
+    init.edge_cap = kChainEdges;
+    copy_to_device(dev, chain_state, &init, sizeof(init));
+    BfsChainState st{};
+    for (;;) {
+      if (max_levels == 1) {
+        launch(dev, k_chain_expand, chain_grid(), 256u, 0, off, tgt, dist,
+               static_cast<const uint32_t*>(queue[cur]), queue[cur ^ 1], chain_state);
+        launch(dev, k_chain_advance, 1u, 64u, 0, chain_state);
+      } else {
+        hipEvent_t prof = nullptr;
+        profile_begin(dev, "bfs_chain", &prof);
+        hip_check(hipGraphLaunch(chain_exec(cur), dev.stream), "hipGraphLaunch");
+        profile_end(dev, "bfs_chain", prof);
+      }
+      copy_to_host(dev, dev.pinned, chain_state, sizeof(st));
+      std::memcpy(&st, dev.pinned, sizeof(st));
+      if (st.stop != 0 || max_levels == 1 || st.levels >= max_levels) break;
+      // a full chain has an even number of levels: the frontier is back in queue[cur]
+    }
+    reached += st.reached;
+    m_u -= std::min<uint64_t>(m_u, st.all_edges);
+    if (st.stop == 1) {
+      level = st.level - 1;
+      done = true;
+      return st.levels;
+    }
+    level = st.level;
+    prev_n_f = n_f;
+    n_f = st.size;
+    m_f = st.edges;
+    if (st.levels & 1u) cur ^= 1;
+    in_bitmap = false;
+    return st.levels;
+  }
+
+  // Runs one decision's worth of work: a level, or a run of chained levels.
+  // With `decide_next`, the scheduler's decision for the next level is
+  // computed on the device and read back with the counters.
+  void step(uint32_t choice, uint32_t chain_levels, bool decide_next, const float* params) {
+    if (choice == kChained) {
+      run_chain(chain_levels);
+      return;
+    }
+    launch_level(choice);
+    if (decide_next) launch_decide(params, true);
+    copy_to_host(dev, dev.pinned, rb, sizeof(BfsReadback));
+    BfsReadback host;
+    std::memcpy(&host, dev.pinned, sizeof(host));
+    last_decision = host.decision;
+    finish_level(choice, host.counters[0]);
+  }
+
+  // The scheduler's decision for the current state (one round trip).
+  uint32_t decide_now(const float* params) {
+    launch_decide(params, false);
+    copy_to_host(dev, dev.pinned, &rb->decision, sizeof(SchedDecision));
+    std::memcpy(&last_decision, dev.pinned, sizeof(SchedDecision));
+    return last_decision.choice;
+  }
+
+  SchedDecision last_decision{};
+};
 
 }  // namespace
 
 struct BfsOutcome {
   uint32_t reachable = 0;
   int32_t max_depth = 0;
-  uint32_t levels_bottom_up = 0;
 };
 
 // Runs BFS from `source` (caller's id) and writes caller-ordered distances to
-// the device array `d_out`.
+// the device array `d_out`. The scheduler attached to the graph, if any,
+// chooses each level's method; otherwise the fixed rule does.
 BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
-  Device& dev = *g.dev;
   prepare(g);
-  const uint32_t n = g.n;
-  const uint64_t words64 = (static_cast<uint64_t>(n) + 63) / 64;
-  int32_t* dist = g.buffer<int32_t>("bfs.dist", sizeof(int32_t) * n);
-  uint32_t* queue[2] = {g.buffer<uint32_t>("bfs.q0", sizeof(uint32_t) * n),
-                        g.buffer<uint32_t>("bfs.q1", sizeof(uint32_t) * n)};
-  uint64_t* bitmap[2] = {g.buffer<uint64_t>("bfs.b0", sizeof(uint64_t) * words64),
-                         g.buffer<uint64_t>("bfs.b1", sizeof(uint64_t) * words64)};
-  auto* counters = g.buffer<BfsCounters>("bfs.counters", 2 * sizeof(BfsCounters));
-  const uint32_t* off = g.rl.offsets.as<const uint32_t>();
-  const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
-  const std::vector<uint32_t>& hoff = g.rl.host_offsets;
-
-  const uint32_t src = g.rl.host_inv[source];
-  Kernel k_init = get_kernel(dev, "bfs", "bfs_init");
-  Kernel k_td = get_kernel(dev, "bfs", "bfs_topdown_edges");
-  Kernel k_deg = get_kernel(dev, "bfs", "bfs_frontier_degrees");
-  uint32_t* start = g.buffer<uint32_t>("bfs.start", sizeof(uint32_t) * n);
-  uint32_t* scan_buf = g.buffer<uint32_t>("bfs.scan", sizeof(uint32_t) * scan_scratch_words(n));
-  Kernel k_bu = get_kernel(dev, "bfs", "bfs_bottomup");
-  Kernel k_q2b = get_kernel(dev, "bfs", "bfs_queue_to_bitmap");
-  Kernel k_b2q = get_kernel(dev, "bfs", "bfs_bitmap_to_queue");
-  Grid1D vgrid = thread_per_item(n);
-  launch(dev, k_init, vgrid.blocks, vgrid.threads, 0, dist, n, src, queue[0]);
-
-  const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
-  uint64_t n_f = 1;
-  uint64_t m_f = hoff[src + 1] - hoff[src];
-  uint64_t m_u = static_cast<uint64_t>(g.m) - m_f;
-  bool top_down = true;
-  int cur = 0;  // index of the current frontier buffer (queue or bitmap)
-  int32_t level = 0;
-  uint64_t reachable = 1;
-  BfsOutcome outcome;
+  Device& dev = *g.dev;
+  Traversal t(g, g.rl.host_inv[source]);
   const char* trace_env = std::getenv("GC_BFS_TRACE");
   const bool trace = trace_env != nullptr && trace_env[0] == '1';
+  const float* params = g.bfs_sched != nullptr ? g.bfs_sched->params.as<const float>() : nullptr;
+  g.last_schedule.clear();
+  uint32_t choice = params != nullptr ? t.decide_now(params) : t.fixed_choice();
   Stopwatch level_clock;
-  for (;;) {
-    if (top_down && g.symmetric && static_cast<double>(m_f) > static_cast<double>(m_u) / kAlpha) {
-      fill_words(dev, bitmap[cur], 2 * words64, 0);
-      Grid1D qg = thread_per_item(n_f);
-      launch(dev, k_q2b, qg.blocks, qg.threads, 0, static_cast<const uint32_t*>(queue[cur]),
-             static_cast<uint32_t>(n_f), reinterpret_cast<uint32_t*>(bitmap[cur]));
-      top_down = false;
-    } else if (!top_down && static_cast<double>(n_f) < static_cast<double>(n) / kBeta) {
-      fill_words(dev, &counters[1], sizeof(BfsCounters) / 4, 0);
-      Grid1D bg = wave_per_item((words64 + 63) / 64);
-      launch(dev, k_b2q, std::min(bg.blocks, cu * 16), bg.threads, 0,
-             static_cast<const uint64_t*>(bitmap[cur]), words64, queue[cur], &counters[1].next_size);
-      top_down = true;
-    }
-    fill_words(dev, counters, sizeof(BfsCounters) / 4, 0);
-    const int32_t next_level = level + 1;
-    if (top_down) {
-      // Edge-balanced expansion: prefix sum of the frontier's degrees, then
-      // 256 consecutive frontier edges per wave.
-      Grid1D qg = thread_per_item(n_f);
-      launch(dev, k_deg, qg.blocks, qg.threads, 0, off, static_cast<const uint32_t*>(queue[cur]),
-             static_cast<uint32_t>(n_f), start);
-      exclusive_scan_u32(dev, start, n_f, scan_buf);
-      Grid1D tg = wave_per_item((m_f + 255) / 256);
-      launch(dev, k_td, std::min(tg.blocks, cu * 32), tg.threads, 0, off, tgt, dist,
-             static_cast<const uint32_t*>(queue[cur]), static_cast<uint32_t>(n_f),
-             static_cast<const uint32_t*>(start), static_cast<uint32_t>(m_f), queue[cur ^ 1],
-             counters, next_level);
-    } else {
-      Grid1D bg = wave_per_item(words64);
-      launch(dev, k_bu, std::min(bg.blocks, cu * 32), bg.threads, 0, off, tgt, dist,
-             reinterpret_cast<const uint32_t*>(bitmap[cur]), bitmap[cur ^ 1], n, counters,
-             next_level);
-      ++outcome.levels_bottom_up;
-    }
-    BfsCounters c;
-    copy_to_host(dev, dev.pinned, counters, sizeof(BfsCounters));
-    std::memcpy(&c, dev.pinned, sizeof(c));
+  while (!t.done) {
+    if (!(t.allowed_mask() >> choice & 1u)) choice = kTopDown;
+    g.last_schedule.push_back(choice);
+    const uint64_t n_f = t.n_f;
+    const int32_t level = t.level;
+    const bool learned_next = params != nullptr && choice != kChained;
+    t.step(choice, kChainRunLevels, learned_next, params);
     if (trace) {
-      std::fprintf(stderr, "bfs level %d %s frontier %llu edges %llu -> next %u edges %llu (%.3f ms)\n",
-                   level, top_down ? "top-down " : "bottom-up", static_cast<unsigned long long>(n_f),
-                   static_cast<unsigned long long>(m_f), c.next_size, c.next_edges,
+      std::fprintf(stderr, "bfs level %d %s frontier %llu -> level %d frontier %llu (%.3f ms)\n", level,
+                   choice == kTopDown ? "top-down " : (choice == kBottomUp ? "bottom-up" : "chained  "),
+                   static_cast<unsigned long long>(n_f), t.level, static_cast<unsigned long long>(t.n_f),
                    level_clock.seconds() * 1e3);
       level_clock = Stopwatch();
     }
-    if (c.next_size == 0) break;
-    level = next_level;
-    reachable += c.next_size;
-    m_u -= std::min<uint64_t>(m_u, c.next_edges);
-    m_f = c.next_edges;
-    n_f = c.next_size;
-    cur ^= 1;
+    if (t.done) break;
+    if (params == nullptr) choice = t.fixed_choice();
+    else if (learned_next) choice = t.last_decision.choice;
+    else choice = t.decide_now(params);
   }
+  Grid1D vgrid = thread_per_item(g.n);
   launch(dev, get_kernel(dev, "util", "gather_u32"), vgrid.blocks, vgrid.threads, 0,
-         reinterpret_cast<const uint32_t*>(dist), g.rl.inv.as<const uint32_t>(),
-         reinterpret_cast<uint32_t*>(d_out), static_cast<uint64_t>(n));
+         reinterpret_cast<const uint32_t*>(t.dist), g.rl.inv.as<const uint32_t>(),
+         reinterpret_cast<uint32_t*>(d_out), static_cast<uint64_t>(g.n));
   dev.sync();
-  outcome.reachable = static_cast<uint32_t>(reachable);
-  outcome.max_depth = level;
+  BfsOutcome outcome;
+  outcome.reachable = static_cast<uint32_t>(t.reached);
+  outcome.max_depth = t.level;
   return outcome;
+}
+
+// Training data: at every level, the features of the decision and the time of
+// each allowed choice from the same state (dist and frontier saved and
+// restored around each trial); then the fastest choice is run for one level.
+// Chained levels are timed as one captured chain divided by the levels it
+// built, their cost per level when chosen.
+uint32_t bfs_level_timings(Graph& g, uint32_t source, uint32_t max_levels, float* features,
+                           float* times_us) {
+  prepare(g);
+  Device& dev = *g.dev;
+  Traversal t(g, g.rl.host_inv[source]);
+  int32_t* save_dist = g.buffer<int32_t>("bfs.save_dist", sizeof(int32_t) * g.n);
+  uint64_t* save_front = g.buffer<uint64_t>("bfs.save_front", std::max<uint64_t>(sizeof(uint32_t) * g.n, 16));
+  uint32_t levels = 0;
+  while (!t.done && levels < max_levels) {
+    t.decide_now(nullptr);
+    std::memcpy(features + static_cast<uint64_t>(levels) * kSchedFeatures, t.last_decision.features,
+                sizeof(float) * kSchedFeatures);
+    const Traversal saved = t;
+    const size_t front_bytes = saved.in_bitmap ? sizeof(uint64_t) * t.words64 : sizeof(uint32_t) * t.n_f;
+    const void* front_src = saved.in_bitmap ? static_cast<const void*>(t.bitmap[t.cur])
+                                            : static_cast<const void*>(t.queue[t.cur]);
+    hip_check(hipMemcpyAsync(save_dist, t.dist, sizeof(int32_t) * g.n, hipMemcpyDeviceToDevice, dev.stream),
+              "hipMemcpyAsync");
+    if (front_bytes > 0)
+      hip_check(hipMemcpyAsync(save_front, front_src, front_bytes, hipMemcpyDeviceToDevice, dev.stream),
+                "hipMemcpyAsync");
+    auto restore = [&]() {
+      hip_check(hipMemcpyAsync(t.dist, save_dist, sizeof(int32_t) * g.n, hipMemcpyDeviceToDevice, dev.stream),
+                "hipMemcpyAsync");
+      if (front_bytes > 0)
+        hip_check(hipMemcpyAsync(const_cast<void*>(front_src), save_front, front_bytes,
+                                 hipMemcpyDeviceToDevice, dev.stream),
+                  "hipMemcpyAsync");
+      t.n_f = saved.n_f;
+      t.m_f = saved.m_f;
+      t.m_u = saved.m_u;
+      t.reached = saved.reached;
+      t.prev_n_f = saved.prev_n_f;
+      t.level = saved.level;
+      t.in_bitmap = saved.in_bitmap;
+      t.cur = saved.cur;
+      t.done = false;
+      dev.sync();
+    };
+    const uint32_t allowed = t.allowed_mask();
+    float best_time = 0.0f;
+    uint32_t best = kTopDown;
+    float* row = times_us + static_cast<uint64_t>(levels) * kSchedChoices;
+    for (uint32_t c = 0; c < kSchedChoices; ++c) {
+      row[c] = -1.0f;
+      if (!(allowed >> c & 1u)) continue;
+      float fastest = 0.0f;
+      for (int rep = 0; rep < 2; ++rep) {
+        restore();
+        Stopwatch sw;
+        float us;
+        if (c == kChained) {
+          // chained levels cost what one captured chain costs per level it builds
+          const uint32_t built = t.run_chain(kChainLevels);
+          us = static_cast<float>(sw.seconds() * 1e6) / static_cast<float>(std::max(1u, built));
+        } else {
+          t.step(c, 1, false, nullptr);
+          us = static_cast<float>(sw.seconds() * 1e6);
+        }
+        fastest = rep == 0 ? us : std::min(fastest, us);
+      }
+      row[c] = fastest;
+      if (best_time == 0.0f || fastest < best_time) best_time = fastest, best = c;
+    }
+    restore();
+    t.step(best, 1, false, nullptr);
+    ++levels;
+  }
+  dev.sync();
+  return levels;
 }
 
 static void check_source(const Graph& g, uint32_t source) {
@@ -184,6 +505,18 @@ gc_status gc_bfs(gc_graph* h, uint32_t source, int32_t* distances, uint32_t* rea
     if (reachable) *reachable = r.reachable;
     if (max_depth) *max_depth = r.max_depth;
     if (secs) *secs = sw.seconds();
+  });
+}
+
+gc_status gc_bfs_level_timings(gc_graph* h, uint32_t source, uint32_t max_levels, float* features,
+                               float* times_us, uint32_t* levels) {
+  return guard([&] {
+    gc::GraphCall call(h);
+    gc::check_source(*call.g, source);
+    if ((features == nullptr || times_us == nullptr) && max_levels > 0)
+      gc::fail_kernel("features or times_us is null");
+    const uint32_t got = gc::bfs_level_timings(*call.g, source, max_levels, features, times_us);
+    if (levels) *levels = got;
   });
 }
 

@@ -5,8 +5,10 @@
 // same early return for an edgeless graph, no forward-edge ceiling (the
 // reference refuses more than 134,217,727 forward edges, launch.rs:27-41).
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "../runtime/common.hpp"
 
@@ -14,27 +16,149 @@ namespace gc {
 
 namespace {
 
-// Runs tc_count over the lists L(x) = targets[lo[x] .. hi[x]) and returns the count.
-uint64_t run_tc_count(Graph& g, const uint32_t* lo, const uint32_t* hi, const uint32_t* targets,
-                      bool descending, bool after_only) {
-  Device& dev = *g.dev;
-  const uint32_t n = g.n;
-  // [0..1] u64 result, [2..3] work counters
-  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", 16);
-  fill_words(dev, scratch, 4, 0);
-  const uint32_t heavy = descending ? std::min<uint32_t>(n, 65536) : 0;
-  const uint32_t light_chunk = descending ? 64 : 16;
-  const uint32_t waves = std::max(1, dev.cu_count) * 32u;
-  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(waves, n));
-  Kernel k = get_kernel(dev, "tc", "tc_count");
-  auto* result = reinterpret_cast<unsigned long long*>(scratch);
-  uint32_t descending_u = descending ? 1 : 0, after_u = after_only ? 1 : 0;
-  launch(dev, k, grid, kWaveSize, 0, lo, hi, targets, n, descending_u, after_u, heavy, light_chunk,
-         scratch + 2, result);
+constexpr uint32_t kBitmap = 0, kHash = 1, kLists = 2;  // the scheduler's choices
+constexpr uint32_t kHotBits = 32768;                    // TCB_HOT_BITS of tc.hip
+constexpr uint32_t kMaxLaunches = 64;                   // work-counter pairs in the scratch
+
+uint64_t read_result(Device& dev, const unsigned long long* result) {
   uint64_t total = 0;
   copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
   std::memcpy(&total, dev.pinned, sizeof(uint64_t));
   return total;
+}
+
+// Runs tc_count over the lists L(x) = targets[lo[x] .. hi[x]) for rows
+// [row_lo, row_hi), adding into `result`; `counters` are two zeroed words.
+void launch_tc_count(Graph& g, const uint32_t* lo, const uint32_t* hi, const uint32_t* targets,
+                     uint32_t row_lo, uint32_t row_hi, bool descending, bool after_only,
+                     uint32_t* counters, unsigned long long* result) {
+  Device& dev = *g.dev;
+  const uint32_t rows = row_hi - row_lo;
+  const uint32_t heavy = descending ? std::min<uint32_t>(rows, 65536) : 0;
+  const uint32_t light_chunk = descending ? 64 : 16;
+  const uint32_t waves = std::max(1, dev.cu_count) * 32u;
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(waves, rows));
+  uint32_t descending_u = descending ? 1 : 0, after_u = after_only ? 1 : 0;
+  launch(dev, get_kernel(dev, "tc", "tc_count"), grid, kWaveSize, 0, lo, hi, targets, row_hi, row_lo,
+         descending_u, after_u, heavy, light_chunk, counters, result);
+}
+
+// One symmetric-graph method over rows [row_lo, row_hi) of the relabelled DAG.
+void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, uint32_t* counters,
+                   unsigned long long* result) {
+  Device& dev = *g.dev;
+  const uint32_t n = g.n;
+  if (row_hi <= row_lo) return;
+  const uint32_t* fwd = g.rl.fwd.as<const uint32_t>();
+  const uint32_t* end = g.rl.offsets.as<const uint32_t>() + 1;
+  const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
+  const uint32_t heavy = std::min<uint32_t>(row_hi - row_lo, 65536);
+  const uint32_t light_chunk = 64;
+  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+  if (method == kLists) {
+    launch_tc_count(g, fwd, end, tgt, row_lo, row_hi, true, true, counters, result);
+  } else if (method == kHash) {
+    launch(dev, get_kernel(dev, "tc", "tc_hash"), grid, 4 * kWaveSize, 0, fwd, end, tgt, row_hi, row_lo,
+           heavy, light_chunk, counters, result);
+  } else {
+    // hot range [hot_base, n): at most kHotBits ids, hot_base a multiple of 32
+    const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
+    launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid, 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
+           row_lo, hot_base, heavy, light_chunk, counters, result);
+  }
+}
+
+// Scratch: [0..1] u64 result, then kMaxLaunches pairs of work counters.
+uint32_t* tc_scratch(Graph& g) {
+  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", sizeof(uint32_t) * (2 + 2 * kMaxLaunches));
+  fill_words(*g.dev, scratch, 2 + 2 * kMaxLaunches, 0);
+  return scratch;
+}
+
+// Degree buckets of the relabelled rows: bucket b holds the rows of degree in
+// [2^b, 2^(b+1)) (contiguous, rows being sorted by degree); rows of degree 0
+// or 1 close no triangle and are left out. bounds has buckets + 1 entries.
+std::vector<uint32_t> bucket_bounds(const Graph& g) {
+  const std::vector<uint32_t>& off = g.rl.host_offsets;
+  const uint32_t n = g.n;
+  auto first_with_degree = [&](uint64_t d) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (off[mid + 1] - off[mid] >= d) hi = mid;
+      else lo = mid + 1;
+    }
+    return lo;
+  };
+  std::vector<uint32_t> bounds{first_with_degree(2)};
+  for (uint32_t b = 1; bounds.back() < n && b < 32; ++b) bounds.push_back(first_with_degree(2ull << b));
+  if (bounds.back() < n) bounds.push_back(n);
+  // drop empty buckets
+  std::vector<uint32_t> out{bounds[0]};
+  for (size_t i = 1; i < bounds.size(); ++i)
+    if (bounds[i] > out.back()) out.push_back(bounds[i]);
+  return out;
+}
+
+using BucketInfo = Graph::TcBucket;
+
+// Work of every bucket (cached per graph: the graph never changes).
+const std::vector<BucketInfo>& bucket_info(Graph& g) {
+  if (g.tc_buckets_ready) return g.tc_buckets;
+  Device& dev = *g.dev;
+  const std::vector<uint32_t> bounds = bucket_bounds(g);
+  std::vector<BucketInfo> info;
+  if (bounds.size() >= 2) {
+    const uint32_t buckets = static_cast<uint32_t>(bounds.size() - 1);
+    uint32_t* d_bounds = g.buffer<uint32_t>("tc.bucket_bounds", sizeof(uint32_t) * bounds.size());
+    auto* d_stats = g.buffer<unsigned long long>("tc.bucket_stats", sizeof(unsigned long long) * 4 * buckets);
+    copy_to_device(dev, d_bounds, bounds.data(), sizeof(uint32_t) * bounds.size());
+    fill_words(dev, d_stats, 8ull * buckets, 0);
+    const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+    launch(dev, get_kernel(dev, "tc", "tc_bucket_stats"), grid, 256u, 0, g.rl.fwd.as<const uint32_t>(),
+           g.rl.offsets.as<const uint32_t>() + 1, g.rl.targets.as<const uint32_t>(),
+           static_cast<const uint32_t*>(d_bounds), buckets, d_stats);
+    std::vector<unsigned long long> stats(4 * buckets);
+    copy_to_host(dev, stats.data(), d_stats, sizeof(unsigned long long) * stats.size());
+    for (uint32_t b = 0; b < buckets; ++b) {
+      info.push_back({bounds[b], bounds[b + 1], stats[4 * b], stats[4 * b + 1], stats[4 * b + 2],
+                      stats[4 * b + 3]});
+    }
+  }
+  g.tc_buckets.assign(info.begin(), info.end());
+  g.tc_buckets_ready = true;
+  return g.tc_buckets;
+}
+
+float lg2(double x) { return static_cast<float>(std::log2(x + 1.0)); }
+
+// The 12 features of one bucket's decision.
+void bucket_features(const Graph& g, const std::vector<BucketInfo>& all, size_t b, float* f) {
+  const BucketInfo& k = all[b];
+  uint64_t total_probes = 0;
+  for (const BucketInfo& x : all) total_probes += x.probes;
+  const uint32_t degree_lo = g.rl.host_offsets[k.row_lo + 1] - g.rl.host_offsets[k.row_lo];
+  f[0] = lg2(g.n);
+  f[1] = lg2(g.m);
+  f[2] = lg2(static_cast<double>(k.rows));
+  f[3] = lg2(static_cast<double>(k.entries));
+  f[4] = lg2(static_cast<double>(k.probes));
+  f[5] = lg2(static_cast<double>(k.max_out));
+  f[6] = lg2(static_cast<double>(k.entries) / std::max<uint64_t>(k.rows, 1));
+  f[7] = lg2(degree_lo);
+  f[8] = lg2(g.n - k.row_lo);
+  f[9] = lg2(static_cast<double>(k.probes) / std::max<uint64_t>(k.entries, 1));
+  f[10] = lg2(static_cast<double>(g.rl.oriented_entries));
+  f[11] = lg2(static_cast<double>(total_probes));
+}
+
+// Which symmetric-graph method the fixed rule runs (GC_TC_METHOD): "bitmap"
+// (default, tc_bitmap), "hash" (tc_hash), "lists" (binary-search
+// intersection, tc_count) or "wedge" (enumeration at the middle vertex,
+// tc_wedge). All four count the same triangles.
+const char* tc_method() {
+  const char* env = std::getenv("GC_TC_METHOD");
+  return env != nullptr ? env : "bitmap";
 }
 
 }  // namespace
@@ -43,8 +167,7 @@ uint64_t run_tc_count(Graph& g, const uint32_t* lo, const uint32_t* hi, const ui
 // relabelled DAG (tc_wedge).
 uint64_t run_tc_wedge(Graph& g) {
   Device& dev = *g.dev;
-  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", 16);
-  fill_words(dev, scratch, 4, 0);
+  uint32_t* scratch = tc_scratch(g);
   const uint32_t bits = wedge_table_bits(g.rl);
   const uint32_t grid = std::max(1, dev.cu_count) * 8u;
   auto* result = reinterpret_cast<unsigned long long*>(scratch);
@@ -53,87 +176,104 @@ uint64_t run_tc_wedge(Graph& g) {
          g.rl.offsets.as<const uint32_t>(), g.rl.targets.as<const uint32_t>(),
          g.rl.fwd.as<const uint32_t>(), g.n, g.rl.hub_units.as<const uint32_t>(), g.rl.num_hub_units,
          hub_chunk, bits, scratch + 2, result);
-  uint64_t total = 0;
-  copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
-  std::memcpy(&total, dev.pinned, sizeof(uint64_t));
-  return total;
-}
-
-// Triangles of a symmetric graph by hash probes of each DAG row (tc_hash).
-uint64_t run_tc_hash(Graph& g) {
-  Device& dev = *g.dev;
-  const uint32_t n = g.n;
-  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", 16);
-  fill_words(dev, scratch, 4, 0);
-  const uint32_t heavy = std::min<uint32_t>(n, 65536);
-  const uint32_t light_chunk = 64;
-  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
-  auto* result = reinterpret_cast<unsigned long long*>(scratch);
-  launch(dev, get_kernel(dev, "tc", "tc_hash"), grid, 4 * kWaveSize, 0,
-         g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
-         g.rl.targets.as<const uint32_t>(), n, heavy, light_chunk, scratch + 2, result);
-  uint64_t total = 0;
-  copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
-  std::memcpy(&total, dev.pinned, sizeof(uint64_t));
-  return total;
-}
-
-// Triangles of a symmetric graph with the top-id bitmap of each DAG row (tc_bitmap).
-uint64_t run_tc_bitmap(Graph& g) {
-  Device& dev = *g.dev;
-  const uint32_t n = g.n;
-  constexpr uint32_t kHotBits = 32768;  // TCB_HOT_BITS of tc.hip
-  // hot range [hot_base, n): at most kHotBits ids, hot_base a multiple of 32
-  const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
-  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", 16);
-  fill_words(dev, scratch, 4, 0);
-  const uint32_t heavy = std::min<uint32_t>(n, 65536);
-  const uint32_t light_chunk = 64;
-  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
-  auto* result = reinterpret_cast<unsigned long long*>(scratch);
-  launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid, 4 * kWaveSize, 0,
-         g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
-         g.rl.targets.as<const uint32_t>(), n, hot_base, heavy, light_chunk, scratch + 2, result);
-  uint64_t total = 0;
-  copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
-  std::memcpy(&total, dev.pinned, sizeof(uint64_t));
-  return total;
-}
-
-// Which symmetric-graph method runs (GC_TC_METHOD): "bitmap" (default,
-// tc_bitmap), "hash" (tc_hash), "lists" (binary-search intersection, tc_count)
-// or "wedge" (enumeration at the middle vertex, tc_wedge). All four count the
-// same triangles.
-static const char* tc_method() {
-  const char* env = std::getenv("GC_TC_METHOD");
-  return env != nullptr ? env : "bitmap";
+  return read_result(dev, result);
 }
 
 uint64_t triangle_count_symmetric(Graph& g) {
   prepare(g);
+  Device& dev = *g.dev;
+  g.last_schedule.clear();
   if (g.symmetric) {
-    const char* method = tc_method();
-    if (std::strcmp(method, "hash") == 0) return run_tc_hash(g);
-    if (std::strcmp(method, "wedge") == 0) return run_tc_wedge(g);
-    if (std::strcmp(method, "lists") == 0) {
-      // Degree-oriented DAG of the relabelled graph, hubs first.
-      return run_tc_count(g, g.rl.fwd.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
-                          g.rl.targets.as<const uint32_t>(), true, true);
+    if (g.tc_sched == nullptr) {
+      const char* method = tc_method();
+      if (std::strcmp(method, "wedge") == 0) return run_tc_wedge(g);
+      const uint32_t m = std::strcmp(method, "hash") == 0    ? kHash
+                         : std::strcmp(method, "lists") == 0 ? kLists
+                                                             : kBitmap;
+      uint32_t* scratch = tc_scratch(g);
+      auto* result = reinterpret_cast<unsigned long long*>(scratch);
+      launch_method(g, m, 0, g.n, scratch + 2, result);
+      return read_result(dev, result);
     }
-    return run_tc_bitmap(g);
+    // learned: one decision per degree bucket, adjacent buckets with the same
+    // method merged into one launch
+    const std::vector<BucketInfo>& buckets = bucket_info(g);
+    uint32_t* scratch = tc_scratch(g);
+    auto* result = reinterpret_cast<unsigned long long*>(scratch);
+    if (buckets.empty()) return 0;
+    std::vector<float> x(buckets.size() * kSchedFeatures), y(buckets.size() * kSchedChoices);
+    for (size_t b = 0; b < buckets.size(); ++b) bucket_features(g, buckets, b, &x[b * kSchedFeatures]);
+    sched_predict(*g.tc_sched, x.data(), static_cast<uint32_t>(buckets.size()), y.data());
+    std::vector<uint32_t> choice(buckets.size());
+    for (size_t b = 0; b < buckets.size(); ++b) {
+      const float* yb = &y[b * kSchedChoices];
+      uint32_t best = 0;
+      for (uint32_t c = 1; c < kSchedChoices; ++c)
+        if (yb[c] < yb[best]) best = c;
+      choice[b] = best;
+      g.last_schedule.push_back(best);
+    }
+    uint32_t launches = 0;
+    for (size_t b = 0; b < buckets.size();) {
+      size_t e = b + 1;
+      while (e < buckets.size() && choice[e] == choice[b]) ++e;
+      if (launches >= kMaxLaunches) fail_kernel("too many triangle-count launches");
+      launch_method(g, choice[b], buckets[b].row_lo, buckets[e - 1].row_hi, scratch + 2 + 2 * launches,
+                    result);
+      ++launches;
+      b = e;
+    }
+    return read_result(dev, result);
   }
   // Any other CSR: the suffix of each list above its row, in the caller's ids.
   ensure_fwd_original(g);
-  return run_tc_count(g, g.fwd_original.as<const uint32_t>(), g.offsets.as<const uint32_t>() + 1,
-                      g.targets.as<const uint32_t>(), false, true);
+  uint32_t* scratch = tc_scratch(g);
+  auto* result = reinterpret_cast<unsigned long long*>(scratch);
+  launch_tc_count(g, g.fwd_original.as<const uint32_t>(), g.offsets.as<const uint32_t>() + 1,
+                  g.targets.as<const uint32_t>(), 0, g.n, false, true, scratch + 2, result);
+  return read_result(dev, result);
 }
 
 uint64_t triangle_count_oriented(Graph& g) {
   prepare(g);
   // sum over entries u->v of |N(u) intersect N(v)| is invariant under the
   // relabelling; the degree order lets the heaviest rows start first.
-  return run_tc_count(g, g.rl.offsets.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
-                      g.rl.targets.as<const uint32_t>(), true, false);
+  uint32_t* scratch = tc_scratch(g);
+  auto* result = reinterpret_cast<unsigned long long*>(scratch);
+  launch_tc_count(g, g.rl.offsets.as<const uint32_t>(), g.rl.offsets.as<const uint32_t>() + 1,
+                  g.rl.targets.as<const uint32_t>(), 0, g.n, true, false, scratch + 2, result);
+  return read_result(*g.dev, result);
+}
+
+// Training data: every bucket's features and the time of each method run on
+// that bucket alone; the methods' counts must agree.
+uint32_t tc_bucket_timings(Graph& g, uint32_t max_buckets, float* features, float* times_us) {
+  prepare(g);
+  if (!g.symmetric) fail_kernel("triangle-count bucket timings need a symmetric CSR");
+  Device& dev = *g.dev;
+  const std::vector<BucketInfo>& buckets = bucket_info(g);
+  uint32_t count = 0;
+  for (size_t b = 0; b < buckets.size() && count < max_buckets; ++b, ++count) {
+    bucket_features(g, buckets, b, features + static_cast<uint64_t>(count) * kSchedFeatures);
+    uint64_t reference = 0;
+    for (uint32_t c = 0; c < kSchedChoices; ++c) {
+      float fastest = 0.0f;
+      for (int rep = 0; rep < 2; ++rep) {
+        uint32_t* scratch = tc_scratch(g);
+        auto* result = reinterpret_cast<unsigned long long*>(scratch);
+        dev.sync();
+        Stopwatch sw;
+        launch_method(g, c, buckets[b].row_lo, buckets[b].row_hi, scratch + 2, result);
+        const uint64_t got = read_result(dev, result);
+        const float us = static_cast<float>(sw.seconds() * 1e6);
+        fastest = rep == 0 ? us : std::min(fastest, us);
+        if (c == 0 && rep == 0) reference = got;
+        else if (got != reference) fail_kernel("triangle-count methods disagree on a bucket");
+      }
+      times_us[static_cast<uint64_t>(count) * kSchedChoices + c] = fastest;
+    }
+  }
+  return count;
 }
 
 }  // namespace gc
@@ -159,6 +299,18 @@ gc_status gc_triangle_count_oriented(gc_graph* h, uint64_t* total_triangles, dou
     if (total_triangles == nullptr) gc::fail_kernel("total_triangles is null");
     *total_triangles = call.g->m == 0 ? 0 : gc::triangle_count_oriented(*call.g);
     if (secs) *secs = sw.seconds();
+  });
+}
+
+gc_status gc_tc_bucket_timings(gc_graph* h, uint32_t max_buckets, float* features, float* times_us,
+                               uint32_t* buckets) {
+  return guard([&] {
+    gc::GraphCall call(h);
+    if ((features == nullptr || times_us == nullptr) && max_buckets > 0)
+      gc::fail_kernel("features or times_us is null");
+    uint32_t got = 0;
+    if (call.g->m > 0) got = gc::tc_bucket_timings(*call.g, max_buckets, features, times_us);
+    if (buckets) *buckets = got;
   });
 }
 

@@ -4,9 +4,9 @@
 // BFS level along the CSR lists, -1 if unreachable): a vertex's level does
 // not depend on which frontier vertex discovers it, so any schedule below is
 // bit-exact with the reference's top-down expansion.
-//  - bfs_topdown: the frontier is a queue; a wave takes 64 frontier vertices,
-//    prefix-sums their degrees and spreads the edges evenly over its lanes
-//    (hubs and leaves in one pass); an unvisited neighbour is claimed with a
+//  - bfs_topdown_edges: the frontier is a queue; the frontier's degrees are
+//    prefix-summed and each wave takes 256 consecutive frontier edges (hubs
+//    and leaves spread evenly); an unvisited neighbour is claimed with a
 //    compare-and-swap on its distance, and winners are appended to the next
 //    queue with one atomic per wave (ballot + popcount ranks).
 //  - bfs_bottomup: the frontier is a bitmap; each lane owns one unvisited
@@ -46,45 +46,6 @@ __device__ __forceinline__ void bfs_append(bool discovered, uint32_t v, uint32_t
   }
   base = __shfl(base, static_cast<int>(leader));
   if (discovered && next_queue != nullptr) next_queue[base + lanes_below(mask)] = v;
-}
-
-extern "C" __global__ __launch_bounds__(256) void bfs_topdown(
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
-    int32_t* __restrict__ dist, const uint32_t* __restrict__ queue, uint32_t queue_size,
-    uint32_t* __restrict__ next_queue, BfsCounters* __restrict__ counters, int32_t next_level) {
-  const uint32_t lane = lane_id();
-  const uint64_t waves = grid_threads() / GC_WAVE;
-  for (uint64_t base = (global_thread() / GC_WAVE) * GC_WAVE; base < queue_size;
-       base += waves * GC_WAVE) {
-    const uint64_t idx = base + lane;
-    uint32_t st = 0, len = 0;
-    if (idx < queue_size) {
-      const uint32_t u = queue[idx];
-      st = offsets[u];
-      len = offsets[u + 1] - st;
-    }
-    const uint32_t incl = wave_inclusive_scan(len);
-    const uint32_t total = __shfl(incl, GC_WAVE - 1);
-    for (uint32_t t0 = 0; t0 < total; t0 += GC_WAVE) {
-      const uint32_t t = t0 + lane;
-      const bool active = t < total;
-      const uint32_t tt = active ? t : total - 1;
-      const uint32_t owner = wave_upper_bound(incl, tt);
-      const uint32_t prev = __shfl(incl, static_cast<int>(owner == 0 ? 0 : owner - 1));
-      const uint32_t excl = owner == 0 ? 0 : prev;
-      const uint32_t ost = __shfl(st, static_cast<int>(owner));
-      bool discovered = false;
-      uint32_t v = 0, deg = 0;
-      if (active) {
-        v = targets[ost + (tt - excl)];
-        if (dist[v] < 0) {
-          discovered = atomicCAS(&dist[v], -1, next_level) == -1;
-          if (discovered) deg = offsets[v + 1] - offsets[v];
-        }
-      }
-      bfs_append(discovered, v, deg, next_queue, counters);
-    }
-  }
 }
 
 extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
@@ -226,4 +187,116 @@ extern "C" __global__ __launch_bounds__(256) void bfs_bitmap_to_queue(
       queue[pos++] = static_cast<uint32_t>(w * 64 + bit);
     }
   }
+}
+
+// ---- device-chained levels (a captured graph of many levels) -------------------
+//
+// A HIP graph holds a fixed number of levels, each = bfs_chain_expand (push
+// from the frontier queue named in the graph into the other one) followed by
+// bfs_chain_advance (one lane: the level's counts become the next frontier).
+// Both read the frontier size and the stop flag from device memory, so once a
+// level ends the traversal (no new vertex) or hands it back to the host (more
+// than `cap` vertices or `edge_cap` edges), the remaining nodes of the graph
+// return at once. The host replays the graph and reads the state once per
+// replay instead of once per level. The expansion walks lists of degree <=
+// CH_LANE_DEGREE with one lane each and longer lists with the whole wave, and
+// appends discoveries with one atomic per wave.
+
+#define CH_LANE_DEGREE 32
+
+// Appends the lanes' discoveries with one atomic per wave (every lane calls it).
+__device__ __forceinline__ void chain_append(bool found, uint32_t w, const uint32_t* __restrict__ offsets,
+                                             uint32_t* __restrict__ nxt, uint32_t* count,
+                                             unsigned long long* edges) {
+  const unsigned long long mask = __ballot(found);
+  if (mask == 0) return;
+  const unsigned long long e = wave_sum(found ? static_cast<unsigned long long>(offsets[w + 1] - offsets[w]) : 0ull);
+  uint32_t base = 0;
+  if (lane_id() == 0) {
+    base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
+    atomicAdd(edges, e);
+  }
+  base = __shfl(base, 0);
+  if (found) nxt[base + lanes_below(mask)] = w;
+}
+
+struct BfsChainState {
+  uint32_t size;        // current frontier
+  int32_t level;        // its level
+  uint32_t stop;        // 0 running, 1 finished (empty level), 2 frontier too large
+  uint32_t levels;      // levels built since the host last set the state
+  uint32_t next_size;
+  uint32_t cap;
+  unsigned long long next_edges;
+  unsigned long long edges;      // degree sum of the current frontier
+  unsigned long long all_edges;  // degree sums of every level built
+  unsigned long long reached;    // vertices discovered
+  unsigned long long edge_cap;
+};
+
+extern "C" __global__ __launch_bounds__(256) void bfs_chain_expand(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    int32_t* __restrict__ dist, const uint32_t* __restrict__ cur, uint32_t* __restrict__ nxt,
+    BfsChainState* __restrict__ st) {
+  if (st->stop != 0) return;
+  const uint32_t size = st->size;
+  const int32_t next_level = st->level + 1;
+  const uint32_t lane = lane_id();
+  const uint64_t wave = global_thread() / GC_WAVE;
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t base = wave * GC_WAVE; base < size; base += waves * GC_WAVE) {
+    const uint64_t q = base + lane;
+    uint32_t a = 0, e = 0;
+    if (q < size) {
+      const uint32_t v = cur[q];
+      a = offsets[v];
+      e = offsets[v + 1];
+    }
+    const bool long_list = e - a > CH_LANE_DEGREE;
+    const uint32_t my_deg = long_list ? 0u : e - a;
+    uint32_t max_deg = my_deg;
+#pragma unroll
+    for (int o = GC_WAVE / 2; o > 0; o >>= 1) max_deg = max(max_deg, static_cast<uint32_t>(__shfl_xor(max_deg, o)));
+    for (uint32_t k = 0; k < max_deg; ++k) {
+      bool found = false;
+      uint32_t w = 0;
+      if (k < my_deg) {
+        w = targets[a + k];
+        found = dist[w] == -1 && atomicCAS(&dist[w], -1, next_level) == -1;
+      }
+      chain_append(found, w, offsets, nxt, &st->next_size, &st->next_edges);
+    }
+    unsigned long long longs = __ballot(long_list);
+    while (longs) {
+      const int j = __builtin_ctzll(longs);
+      longs &= longs - 1;
+      const uint32_t ja = __shfl(a, j), je = __shfl(e, j);
+      for (uint32_t i0 = ja; i0 < je; i0 += GC_WAVE) {
+        const uint32_t i = i0 + lane;
+        bool found = false;
+        uint32_t w = 0;
+        if (i < je) {
+          w = targets[i];
+          found = dist[w] == -1 && atomicCAS(&dist[w], -1, next_level) == -1;
+        }
+        chain_append(found, w, offsets, nxt, &st->next_size, &st->next_edges);
+      }
+    }
+  }
+}
+
+extern "C" __global__ __launch_bounds__(64) void bfs_chain_advance(BfsChainState* __restrict__ st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || st->stop != 0) return;
+  const uint32_t s = st->next_size;
+  const unsigned long long e = st->next_edges;
+  st->next_size = 0;
+  st->next_edges = 0;
+  st->levels += 1;
+  st->level += 1;
+  st->reached += s;
+  st->all_edges += e;
+  st->size = s;
+  st->edges = e;
+  if (s == 0) st->stop = 1;
+  else if (s > st->cap || e > st->edge_cap) st->stop = 2;
 }

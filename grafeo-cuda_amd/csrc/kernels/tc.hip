@@ -21,6 +21,11 @@
 // time, the rest in chunks of `light_chunk`, so hubs (processed first when
 // `descending`) do not leave a tail. Counts are u64 per wave, added with one
 // atomic per wave at the end (integer sums are exact in any order).
+//
+// tc_count, tc_hash and tc_bitmap take the rows [row_lo, n) only, so that a
+// driver can give each degree bucket (a contiguous row range of the
+// degree-ordered layout) its own method; tc_bucket_stats sums each bucket's
+// work for the scheduler's features.
 #include "device.hpp"
 #include "wedge.hpp"
 
@@ -30,9 +35,10 @@ using namespace gcd;
 
 extern "C" __global__ __launch_bounds__(64) void tc_count(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
-    const uint32_t* __restrict__ targets, uint32_t n, uint32_t descending, uint32_t after_only,
-    uint32_t heavy, uint32_t light_chunk, uint32_t* __restrict__ counters,
+    const uint32_t* __restrict__ targets, uint32_t n, uint32_t row_lo, uint32_t descending,
+    uint32_t after_only, uint32_t heavy, uint32_t light_chunk, uint32_t* __restrict__ counters,
     unsigned long long* __restrict__ result) {
+  const uint32_t positions = n - row_lo;  // rows [row_lo, n)
   __shared__ uint32_t s_a[TC_LDS_CAP];
   __shared__ uint32_t s_pos[GC_WAVE];
   __shared__ uint32_t s_start[GC_WAVE];
@@ -45,7 +51,7 @@ extern "C" __global__ __launch_bounds__(64) void tc_count(
   for (;;) {
     // ---- fetch the next range of positions [kb, ke) ----
     if (lane == 0) {
-      uint32_t kb = n, ke = n;
+      uint32_t kb = positions, ke = positions;
       if (heavy_left) {
         uint32_t k = atomicAdd(&counters[0], 1u);
         if (k < heavy) {
@@ -55,13 +61,13 @@ extern "C" __global__ __launch_bounds__(64) void tc_count(
           heavy_left = false;
         }
       }
-      if (!heavy_left && kb == n) {
+      if (!heavy_left && kb == positions) {
         uint32_t c = atomicAdd(&counters[1], light_chunk);
         uint64_t b = static_cast<uint64_t>(heavy) + c;
-        if (b < n) {
+        if (b < positions) {
           kb = static_cast<uint32_t>(b);
           uint64_t e = b + light_chunk;
-          ke = e < n ? static_cast<uint32_t>(e) : n;
+          ke = e < positions ? static_cast<uint32_t>(e) : positions;
         }
       }
       s_range[0] = kb;
@@ -71,10 +77,10 @@ extern "C" __global__ __launch_bounds__(64) void tc_count(
     const uint32_t kb = s_range[0], ke = s_range[1];
     heavy_left = __shfl(heavy_left ? 1 : 0, 0) != 0;
     __syncthreads();
-    if (kb >= n) break;
+    if (kb >= positions) break;
 
     for (uint32_t k = kb; k < ke; ++k) {
-      const uint32_t u = descending ? n - 1 - k : k;
+      const uint32_t u = descending ? n - 1 - k : row_lo + k;
       const uint32_t a_lo = lo[u];
       const uint32_t a = hi[u] - a_lo;
       if (a == 0) continue;
@@ -157,8 +163,10 @@ extern "C" __global__ __launch_bounds__(64) void tc_count(
 
 extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
-    const uint32_t* __restrict__ targets, uint32_t n, uint32_t heavy, uint32_t light_chunk,
+    const uint32_t* __restrict__ targets, uint32_t n, uint32_t row_lo, uint32_t heavy,
+    uint32_t light_chunk,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+  const uint32_t positions = n - row_lo;  // rows [row_lo, n)
   __shared__ uint32_t s_table_all[TCH_WAVES][TCH_TABLE];
   const uint32_t wave = threadIdx.x / GC_WAVE;
   const uint32_t lane = lane_id();
@@ -167,7 +175,7 @@ extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
   bool heavy_left = heavy > 0;
 
   for (;;) {
-    uint32_t kb = n, ke = n;
+    uint32_t kb = positions, ke = positions;
     if (lane == 0) {
       if (heavy_left) {
         const uint32_t k = atomicAdd(&counters[0], 1u);
@@ -178,19 +186,19 @@ extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
           heavy_left = false;
         }
       }
-      if (!heavy_left && kb == n) {
+      if (!heavy_left && kb == positions) {
         const uint32_t c = atomicAdd(&counters[1], light_chunk);
         const uint64_t b = static_cast<uint64_t>(heavy) + c;
-        if (b < n) {
+        if (b < positions) {
           kb = static_cast<uint32_t>(b);
           const uint64_t e = b + light_chunk;
-          ke = e < n ? static_cast<uint32_t>(e) : n;
+          ke = e < positions ? static_cast<uint32_t>(e) : positions;
         }
       }
     }
     kb = __shfl(kb, 0);
     ke = __shfl(ke, 0);
-    if (kb >= n) break;
+    if (kb >= positions) break;
 
     for (uint32_t k = kb; k < ke; ++k) {
       const uint32_t u = n - 1 - k;
@@ -278,8 +286,10 @@ extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
 
 extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
-    const uint32_t* __restrict__ targets, uint32_t n, uint32_t hot_base, uint32_t heavy,
+    const uint32_t* __restrict__ targets, uint32_t n, uint32_t row_lo, uint32_t hot_base,
+    uint32_t heavy,
     uint32_t light_chunk, uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+  const uint32_t positions = n - row_lo;  // rows [row_lo, n)
   __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_HOT_WORDS];
   const uint32_t wave = threadIdx.x / GC_WAVE;
   const uint32_t lane = lane_id();
@@ -290,7 +300,7 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
   bool heavy_left = heavy > 0;
 
   for (;;) {
-    uint32_t kb = n, ke = n;
+    uint32_t kb = positions, ke = positions;
     if (lane == 0) {
       if (heavy_left) {
         const uint32_t k = atomicAdd(&counters[0], 1u);
@@ -301,19 +311,19 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
           heavy_left = false;
         }
       }
-      if (!heavy_left && kb == n) {
+      if (!heavy_left && kb == positions) {
         const uint32_t c = atomicAdd(&counters[1], light_chunk);
         const uint64_t b = static_cast<uint64_t>(heavy) + c;
-        if (b < n) {
+        if (b < positions) {
           kb = static_cast<uint32_t>(b);
           const uint64_t e = b + light_chunk;
-          ke = e < n ? static_cast<uint32_t>(e) : n;
+          ke = e < positions ? static_cast<uint32_t>(e) : positions;
         }
       }
     }
     kb = __shfl(kb, 0);
     ke = __shfl(ke, 0);
-    if (kb >= n) break;
+    if (kb >= positions) break;
 
     for (uint32_t k = kb; k < ke; ++k) {
       const uint32_t u = n - 1 - k;
@@ -385,4 +395,63 @@ extern "C" __global__ __launch_bounds__(64) void tc_wedge(
     unsigned long long* __restrict__ result) {
   wedge_body<false>(offsets, targets, fwd, nullptr, n, hub_units, num_hub_units, hub_chunk, table_bits,
                     counters, result, nullptr);
+}
+
+// Per degree bucket b (rows [bucket_lo[b], bucket_lo[b+1]) of the relabelled
+// DAG): stats[4b + 0] rows, + 1 DAG entries, + 2 probe items (sum over the
+// rows u of sum over v in L(u) of |L(v)|), + 3 largest |L(u)|. One atomic per
+// wave when the wave's rows share a bucket.
+extern "C" __global__ __launch_bounds__(256) void tc_bucket_stats(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+    const uint32_t* __restrict__ targets, const uint32_t* __restrict__ bucket_lo, uint32_t buckets,
+    unsigned long long* __restrict__ stats) {
+  const uint32_t first = bucket_lo[0], last = bucket_lo[buckets];
+  const uint64_t stride = grid_threads();
+  const uint64_t steps = (static_cast<uint64_t>(last - first) + stride - 1) / stride;
+  for (uint64_t k = 0; k < steps; ++k) {
+    const uint64_t uu = first + global_thread() + k * stride;
+    const bool valid = uu < last;
+    uint32_t b = 0;
+    unsigned long long d = 0, probes = 0;
+    if (valid) {
+      const uint32_t u = static_cast<uint32_t>(uu);
+      uint32_t l = 0, r = buckets;  // last bucket whose first row is <= u
+      while (r - l > 1) {
+        const uint32_t mid = (l + r) / 2;
+        if (bucket_lo[mid] <= u) l = mid;
+        else r = mid;
+      }
+      b = l;
+      const uint32_t a = lo[u], e = hi[u];
+      d = e - a;
+      for (uint32_t i = a; i < e; ++i) {
+        const uint32_t v = targets[i];
+        probes += hi[v] - lo[v];
+      }
+    }
+    const uint32_t b0 = __shfl(b, 0);
+    const bool uniform = __ballot(!valid || b == b0) == ~0ull && __shfl(valid ? 1u : 0u, 0) != 0u;
+    if (uniform) {
+      const unsigned long long rows = wave_sum(valid ? 1ull : 0ull);
+      const unsigned long long entries = wave_sum(d);
+      const unsigned long long items = wave_sum(probes);
+      unsigned long long mx = d;
+#pragma unroll
+      for (int o = GC_WAVE / 2; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(mx, o);
+        mx = other > mx ? other : mx;
+      }
+      if (lane_id() == 0) {
+        atomicAdd(&stats[4 * b0 + 0], rows);
+        atomicAdd(&stats[4 * b0 + 1], entries);
+        atomicAdd(&stats[4 * b0 + 2], items);
+        atomicMax(&stats[4 * b0 + 3], mx);
+      }
+    } else if (valid) {
+      atomicAdd(&stats[4 * b + 0], 1ull);
+      atomicAdd(&stats[4 * b + 1], d);
+      atomicAdd(&stats[4 * b + 2], probes);
+      atomicMax(&stats[4 * b + 3], d);
+    }
+  }
 }

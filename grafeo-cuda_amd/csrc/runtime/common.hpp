@@ -231,8 +231,59 @@ struct Relabelled {
   uint32_t num_hub_units = 0;
 };
 
+// The learned scheduler: the FP32 MLP of mlp.hip (12 -> 32 -> 32 -> 3) with
+// its input normalisation, resident on one device.
+constexpr uint32_t kSchedFeatures = 12;
+constexpr uint32_t kSchedChoices = 3;
+constexpr uint32_t kSchedParams = 32 * 12 + 32 + 32 * 32 + 32 + 3 * 32 + 3;  // 1571
+
+struct SchedNorm {  // MlpNorm of mlp.hip
+  float mean[kSchedFeatures];
+  float scale[kSchedFeatures];
+};
+
+struct Sched {
+  std::atomic<int> refs{1};
+  Device* dev = nullptr;  // holds one reference
+  DeviceMemory params;    // f32[kSchedParams]
+  DeviceMemory m1, m2;    // Adam moments
+  DeviceMemory grad;      // f32[kSchedParams] + loss
+  SchedNorm norm{};
+  uint32_t adam_step = 0;
+};
+
+void sched_release(Sched* s);
+// y[count][3] = the scheduler's network on x[count][12] (GPU, synchronizes).
+void sched_predict(Sched& s, const float* x, uint32_t count, float* y);
+
+// Instantiated HIP graphs owned by a graph (destroyed with it).
+struct GraphExecs {
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  GraphExecs() = default;
+  GraphExecs(const GraphExecs&) = delete;
+  GraphExecs& operator=(const GraphExecs&) = delete;
+  ~GraphExecs() {
+    for (int i = 0; i < 2; ++i) {
+      if (exec[i] != nullptr) (void)hipGraphExecDestroy(exec[i]);
+      if (graph[i] != nullptr) (void)hipGraphDestroy(graph[i]);
+    }
+  }
+};
+
 struct Graph {
   Device* dev = nullptr;  // holds one reference
+  Sched* bfs_sched = nullptr;  // learned BFS schedule (one reference), or null: the fixed rule
+  Sched* tc_sched = nullptr;   // learned triangle-count schedule (one reference), or null
+  std::vector<uint32_t> last_schedule;  // choices of the last BFS levels / TC buckets
+  GraphExecs bfs_chain;                 // captured BFS level chains, one per start queue
+  // Degree buckets of the relabelled rows and their work (triangle counting).
+  struct TcBucket {
+    uint32_t row_lo, row_hi;
+    uint64_t rows, entries, probes, max_out;
+  };
+  std::vector<TcBucket> tc_buckets;
+  bool tc_buckets_ready = false;
   uint32_t n = 0;
   uint32_t m = 0;
   DeviceMemory offsets;  // u32[n+1]

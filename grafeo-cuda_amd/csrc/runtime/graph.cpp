@@ -209,12 +209,16 @@ void gc_graph_free(gc_graph* h) {
   if (h == nullptr) return;
   auto* g = reinterpret_cast<gc::Graph*>(h);
   gc::Device* dev = g->dev;
+  gc::Sched* bfs_sched = g->bfs_sched;
+  gc::Sched* tc_sched = g->tc_sched;
   {
     std::lock_guard<std::recursive_mutex> lock(dev->mu);
     (void)hipSetDevice(dev->ordinal);
     (void)hipStreamSynchronize(dev->stream);
     delete g;
   }
+  if (bfs_sched != nullptr) gc::sched_release(bfs_sched);
+  if (tc_sched != nullptr) gc::sched_release(tc_sched);
   gc_device_release(reinterpret_cast<gc_device*>(dev));
 }
 

@@ -23,7 +23,7 @@ import numpy as np
 from . import _native
 from ._native import GcError, check, lib
 
-__all__ = ["GpuDevice", "GpuGraph", "GpuBatch", "GcError"]
+__all__ = ["GpuDevice", "GpuGraph", "GpuBatch", "GpuScheduler", "GcError"]
 
 
 def _runtime_error(e: GcError) -> RuntimeError:
@@ -236,6 +236,44 @@ class GpuGraph:
         r["edges"] = [(int(u), int(v)) for u, v in r["edges"]]
         return r
 
+    def set_scheduler(self, bfs=None, tc=None) -> None:
+        """Attaches GpuSchedulers: `bfs` picks each BFS level's method, `tc` each degree bucket's
+        triangle-count method; None keeps the fixed rule. Results do not change."""
+        self._call(lib.gc_graph_set_scheduler, bfs._handle if bfs is not None else None,
+                   tc._handle if tc is not None else None)
+        self._schedulers = (bfs, tc)
+
+    def last_schedule(self) -> list[int]:
+        """Choices of the last BFS (per level run: 0 top-down, 1 bottom-up, 2 device-chained levels)
+        or symmetric triangle count (per degree bucket: 0 bitmap, 1 hash, 2 binary search)."""
+        count = c_uint32()
+        self._call(lib.gc_graph_last_schedule, None, 0, byref(count))
+        out = np.zeros(max(count.value, 1), dtype=np.uint32)
+        self._call(lib.gc_graph_last_schedule, out.ctypes.data_as(POINTER(c_uint32)), count.value, byref(count))
+        return [int(x) for x in out[: count.value]]
+
+    def bfs_level_timings(self, source: int, max_levels: int = 100000) -> dict:
+        """Training data for the BFS decision: {'features': (L, 12), 'times_us': (L, 3), -1 where
+        a choice is not allowed}; each level's choices are all timed from the same state."""
+        source = _u32_arg(source, "source")
+        cap = max(1, min(int(max_levels), self.num_nodes + 1))
+        feats = np.zeros((cap, 12), dtype=np.float32)
+        times = np.zeros((cap, 3), dtype=np.float32)
+        levels = c_uint32()
+        self._call(lib.gc_bfs_level_timings, source, cap, feats.ctypes.data_as(POINTER(c_float)),
+                   times.ctypes.data_as(POINTER(c_float)), byref(levels))
+        return {"features": feats[: levels.value], "times_us": times[: levels.value]}
+
+    def tc_bucket_timings(self, max_buckets: int = 64) -> dict:
+        """Training data for the triangle-count decision: per degree bucket {'features': (B, 12),
+        'times_us': (B, 3)} (bitmap, hash, binary search, each timed alone on the bucket)."""
+        feats = np.zeros((max_buckets, 12), dtype=np.float32)
+        times = np.zeros((max_buckets, 3), dtype=np.float32)
+        count = c_uint32()
+        self._call(lib.gc_tc_bucket_timings, max_buckets, feats.ctypes.data_as(POINTER(c_float)),
+                   times.ctypes.data_as(POINTER(c_float)), byref(count))
+        return {"features": feats[: count.value], "times_us": times[: count.value]}
+
     def truss_numbers_numpy(self) -> dict:
         """As truss_numbers, with 'edges' as an (E, 2) uint32 array and 'truss' as a uint32 array."""
         ep, tp = POINTER(c_uint32)(), POINTER(c_uint32)()
@@ -343,3 +381,87 @@ class GpuBatch:
 
     def __repr__(self) -> str:
         return f"GpuBatch(num_graphs={self.num_graphs})"
+
+
+SCHED_PARAMS = 1571
+
+
+def _f32(a, shape_tail, what):
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if arr.ndim != len(shape_tail) + 1 or tuple(arr.shape[1:]) != tuple(shape_tail):
+        raise ValueError(f"{what} must have shape (count, {', '.join(map(str, shape_tail))})")
+    return arr
+
+
+class GpuScheduler:
+    """The learned scheduler (gc_sched_*; an addition): a 12 -> 32 -> 32 -> 3 FP32 MLP on the GPU
+    predicting the log-time of three choices, with GPU inference, loss gradient and training."""
+
+    def __init__(self, device: GpuDevice, seed: int = 1):
+        if not isinstance(device, GpuDevice):
+            raise TypeError("device must be a GpuDevice")
+        handle = c_void_p()
+        try:
+            check(lib.gc_sched_new(device._handle, int(seed), byref(handle)))
+        except GcError as e:
+            raise _runtime_error(e) from None
+        self._handle = handle
+        self._device = device
+
+    def _call(self, fn, *args) -> None:
+        try:
+            check(fn(self._handle, *args))
+        except GcError as e:
+            raise _runtime_error(e) from None
+
+    def get(self) -> dict:
+        """{'params': float32[1571], 'mean': float32[12], 'scale': float32[12]}."""
+        p = np.zeros(SCHED_PARAMS, np.float32)
+        mean = np.zeros(12, np.float32)
+        scale = np.zeros(12, np.float32)
+        fp = POINTER(c_float)
+        self._call(lib.gc_sched_get, p.ctypes.data_as(fp), mean.ctypes.data_as(fp), scale.ctypes.data_as(fp))
+        return {"params": p, "mean": mean, "scale": scale}
+
+    def set(self, params=None, mean=None, scale=None) -> None:
+        fp = POINTER(c_float)
+        arrays = []
+        for a, size in ((params, SCHED_PARAMS), (mean, 12), (scale, 12)):
+            if a is None:
+                arrays.append(None)
+                continue
+            arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32)).reshape(-1)
+            if arr.size != size:
+                raise ValueError(f"expected {size} values, got {arr.size}")
+            arrays.append(arr)
+        self._call(lib.gc_sched_set, *[None if a is None else a.ctypes.data_as(fp) for a in arrays])
+
+    def predict(self, x) -> np.ndarray:
+        x = _f32(x, (12,), "x")
+        y = np.zeros((x.shape[0], 3), np.float32)
+        fp = POINTER(c_float)
+        self._call(lib.gc_sched_predict, x.ctypes.data_as(fp), x.shape[0], y.ctypes.data_as(fp))
+        return y
+
+    def loss_grad(self, x, t, mask) -> tuple[float, np.ndarray]:
+        x, t, mask = _f32(x, (12,), "x"), _f32(t, (3,), "t"), _f32(mask, (3,), "mask")
+        grad = np.zeros(SCHED_PARAMS, np.float32)
+        loss = c_float()
+        fp = POINTER(c_float)
+        self._call(lib.gc_sched_loss_grad, x.ctypes.data_as(fp), t.ctypes.data_as(fp), mask.ctypes.data_as(fp),
+                   x.shape[0], byref(loss), grad.ctypes.data_as(fp))
+        return loss.value, grad
+
+    def train(self, x, t, mask, steps: int = 1000, lr: float = 1e-2) -> float:
+        x, t, mask = _f32(x, (12,), "x"), _f32(t, (3,), "t"), _f32(mask, (3,), "mask")
+        loss = c_float()
+        fp = POINTER(c_float)
+        self._call(lib.gc_sched_train, x.ctypes.data_as(fp), t.ctypes.data_as(fp), mask.ctypes.data_as(fp),
+                   x.shape[0], int(steps), c_float(lr), byref(loss))
+        return loss.value
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            lib.gc_sched_free(h)
+            self._handle = None

@@ -116,6 +116,17 @@ def _load() -> ctypes.CDLL:
         "gc_batch_set_config": (c_int, [c_void_p, u32p]),
         "gc_batch_get_config": (c_int, [c_void_p, u32p]),
         "gc_batch_tune": (c_int, [c_void_p, u32p, POINTER(c_double)]),
+        "gc_sched_new": (c_int, [c_void_p, c_uint64, POINTER(c_void_p)]),
+        "gc_sched_free": (None, [c_void_p]),
+        "gc_sched_get": (c_int, [c_void_p, f32p, f32p, f32p]),
+        "gc_sched_set": (c_int, [c_void_p, f32p, f32p, f32p]),
+        "gc_sched_predict": (c_int, [c_void_p, f32p, c_uint32, f32p]),
+        "gc_sched_loss_grad": (c_int, [c_void_p, f32p, f32p, f32p, c_uint32, f32p, f32p]),
+        "gc_sched_train": (c_int, [c_void_p, f32p, f32p, f32p, c_uint32, c_uint32, c_float, f32p]),
+        "gc_graph_set_scheduler": (c_int, [c_void_p, c_void_p, c_void_p]),
+        "gc_graph_last_schedule": (c_int, [c_void_p, u32p, c_uint32, u32p]),
+        "gc_bfs_level_timings": (c_int, [c_void_p, c_uint32, c_uint32, f32p, f32p, u32p]),
+        "gc_tc_bucket_timings": (c_int, [c_void_p, c_uint32, f32p, f32p, u32p]),
         "gc_truss_numbers": (
             c_int,
             [c_void_p, POINTER(u32p), POINTER(u32p), POINTER(c_size_t), u32p, POINTER(c_double)],
@@ -156,7 +167,10 @@ EXPORTED = [
     "gc_triangle_count_oriented", "gc_bfs", "gc_bfs_device", "gc_pagerank", "gc_pagerank_device",
     "gc_k_truss", "gc_truss_numbers", "gc_msbfs", "gc_msbfs_device", "gc_batch_from_csrs",
     "gc_batch_free", "gc_batch_num_graphs", "gc_batch_triangle_count", "gc_batch_set_config",
-    "gc_batch_get_config", "gc_batch_tune", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
+    "gc_batch_get_config", "gc_batch_tune", "gc_sched_new", "gc_sched_free", "gc_sched_get",
+    "gc_sched_set", "gc_sched_predict", "gc_sched_loss_grad", "gc_sched_train",
+    "gc_graph_set_scheduler", "gc_graph_last_schedule", "gc_bfs_level_timings",
+    "gc_tc_bucket_timings", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
     "gc_gen_grid", "gc_code_object_count", "gc_code_object_info",
 ]
 

@@ -272,6 +272,66 @@ gc_status gc_batch_set_config(gc_batch* b, const uint32_t* config);
 gc_status gc_batch_get_config(gc_batch* b, uint32_t* config);
 gc_status gc_batch_tune(gc_batch* b, uint32_t* config, double* best_ms);
 
+/* ------------------------------------------------------------------------ */
+/* Learned scheduler (an addition with no reference function; it changes no */
+/* result, only which kernels compute it)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* A 12 -> 32 -> 32 -> 3 FP32 MLP (ReLU hidden layers) resident on a device.
+ * Its 3 outputs predict the log-time (natural log of microseconds) of three
+ * choices for one decision; the smallest allowed prediction wins. Parameter
+ * layout (GC_SCHED_PARAMS floats): W1[32][12], b1[32], W2[32][32], b2[32],
+ * W3[3][32], b3[3]; inputs are normalised as (x - mean[i]) * scale[i]. */
+#define GC_SCHED_FEATURES 12
+#define GC_SCHED_CHOICES 3
+#define GC_SCHED_PARAMS 1571
+
+typedef struct gc_sched gc_sched;
+
+/* He-uniform weights from SplitMix64(seed), zero biases, mean 0, scale 1. */
+gc_status gc_sched_new(gc_device* dev, uint64_t seed, gc_sched** out);
+void gc_sched_free(gc_sched* s);
+/* Copies out / in the parameters and the input normalisation (each nullable;
+ * setting parameters also resets the optimizer state). */
+gc_status gc_sched_get(gc_sched* s, float* params, float* mean, float* scale);
+gc_status gc_sched_set(gc_sched* s, const float* params, const float* mean, const float* scale);
+/* y[count][3] = network(x[count][12]) on the GPU. */
+gc_status gc_sched_predict(gc_sched* s, const float* x, uint32_t count, float* y);
+/* Loss L = sum mask (y - t)^2 / sum mask over a batch (x[count][12],
+ * t[count][3], mask[count][3] in {0, 1}) and its gradient dL/dparams
+ * (GC_SCHED_PARAMS floats), by the GPU backward kernels. */
+gc_status gc_sched_loss_grad(gc_sched* s, const float* x, const float* t, const float* mask,
+                             uint32_t count, float* loss, float* grad);
+/* `steps` full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8) on the GPU;
+ * *final_loss is the loss after the last step. */
+gc_status gc_sched_train(gc_sched* s, const float* x, const float* t, const float* mask,
+                         uint32_t count, uint32_t steps, float lr, float* final_loss);
+
+/* Attaches schedulers to a graph (each holds a reference; null detaches):
+ * with `bfs`, BFS picks each level among top-down, bottom-up and the
+ * device-chained levels (captured HIP graphs of many levels with one
+ * read-back per graph); with `tc`, the
+ * symmetric triangle count picks, per degree bucket, among the bitmap, hash
+ * and binary-search intersections. Without them the fixed rules run (BFS:
+ * Beamer, Asanovic and Patterson's alpha = 14, beta = 24 switch; triangle
+ * count: the bitmap intersection over all rows). Results are identical. */
+gc_status gc_graph_set_scheduler(gc_graph* g, gc_sched* bfs, gc_sched* tc);
+/* The choices taken by the last BFS (one per level run) or triangle count
+ * (one per degree bucket) on this graph. */
+gc_status gc_graph_last_schedule(gc_graph* g, uint32_t* choices, uint32_t capacity, uint32_t* count);
+
+/* Training data. BFS from `source`: for each level (at most max_levels), the
+ * 12 features of the decision (features[level][12]) and the time of each
+ * choice run from the same state (times_us[level][3], -1 where the choice is
+ * not allowed); the traversal continues along the fastest choice. */
+gc_status gc_bfs_level_timings(gc_graph* g, uint32_t source, uint32_t max_levels, float* features,
+                               float* times_us, uint32_t* levels);
+/* Symmetric triangle count: per degree bucket (rows of degree [2^b, 2^(b+1))),
+ * its 12 features and the time of each intersection method on it alone;
+ * every method's count is checked against the others. */
+gc_status gc_tc_bucket_timings(gc_graph* g, uint32_t max_buckets, float* features, float* times_us,
+                               uint32_t* buckets);
+
 /* Device memory for the *_device entry points' outputs (hipMalloc on the
  * device of `dev`). No reference counterpart. */
 gc_status gc_device_malloc(gc_device* dev, size_t bytes, void** ptr);
