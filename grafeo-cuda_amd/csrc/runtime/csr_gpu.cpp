@@ -1,0 +1,188 @@
+// CSR construction on the device (kernels of sort.hip): the same arrays as
+// from_edge_pairs (crates/grafeo-cuda-core/src/csr_export.rs:202-239) -- the
+// reverse of every non-loop pair added when symmetric, sorted by (source,
+// target), duplicates removed -- built by a stable LSD radix sort of packed
+// keys on the GPU instead of the host.
+#include <algorithm>
+#include <cstring>
+#include <utility>
+
+#include "common.hpp"
+
+namespace gc {
+
+namespace {
+
+constexpr uint64_t kTile = 4096;  // RS_TILE of sort.hip and the compaction tile of util.hip
+
+unsigned bits_of(uint64_t max_value) {
+  unsigned b = 0;
+  while (b < 64 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+// Sorts the low `bits` bits of n keys (stable LSD, 8 bits per pass); returns
+// whichever of keys / tmp holds the result.
+unsigned long long* radix_sort(Device& dev, unsigned long long* keys, unsigned long long* tmp, uint64_t n,
+                               uint32_t bits) {
+  if (n <= 1) return keys;
+  const uint64_t blocks = (n + kTile - 1) / kTile;
+  if (blocks * 256 > 0xFFFFFFFFull) fail_kernel("too many keys for the device sort");
+  DeviceMemory hist, scan;
+  hist.allocate(sizeof(uint32_t) * 256 * blocks);
+  scan.allocate(sizeof(uint32_t) * scan_scratch_words(256 * blocks));
+  Kernel k_hist = get_kernel(dev, "sort", "rs_histogram");
+  Kernel k_scatter = get_kernel(dev, "sort", "rs_scatter");
+  const uint32_t grid = static_cast<uint32_t>(blocks);
+  for (uint32_t shift = 0; shift < bits; shift += 8) {
+    launch(dev, k_hist, grid, 256u, 0, static_cast<const unsigned long long*>(keys), n, shift,
+           hist.as<uint32_t>());
+    exclusive_scan_u32(dev, hist.as<uint32_t>(), 256 * blocks, scan.as<uint32_t>());
+    launch(dev, k_scatter, grid, 256u, 0, static_cast<const unsigned long long*>(keys),
+           static_cast<const uint32_t*>(nullptr), n, shift, static_cast<const uint32_t*>(hist.as<uint32_t>()),
+           tmp, static_cast<uint32_t*>(nullptr));
+    std::swap(keys, tmp);
+  }
+  return keys;
+}
+
+}  // namespace
+
+struct DeviceCsr {
+  DeviceMemory offsets;  // u32[n + 1]
+  DeviceMemory targets;  // u32[m]
+  uint32_t n = 0;
+  uint32_t m = 0;
+};
+
+// Builds the CSR of `num_pairs` host pairs on the device. Pairs naming a node
+// >= n are an InvalidCsr error with the host builder's message.
+void build_csr_on_device(Device& dev, const uint32_t* pairs, size_t num_pairs, uint32_t n, bool symmetric,
+                         DeviceCsr& out) {
+  const unsigned id_bits = std::max(1u, bits_of(n == 0 ? 0 : n - 1));
+  const uint64_t num_keys = symmetric ? 2ull * num_pairs : num_pairs;
+  DeviceMemory d_pairs, keys_a, keys_b, bad;
+  d_pairs.allocate(sizeof(uint32_t) * 2 * std::max<size_t>(num_pairs, 1));
+  keys_a.allocate(sizeof(unsigned long long) * std::max<uint64_t>(num_keys, 1));
+  keys_b.allocate(sizeof(unsigned long long) * std::max<uint64_t>(num_keys, 1));
+  bad.allocate(16);
+  fill_words(dev, bad.get(), 1, 0);
+  if (num_pairs > 0) copy_to_device(dev, d_pairs.get(), pairs, sizeof(uint32_t) * 2 * num_pairs);
+  Grid1D pg = thread_per_item(std::max<uint64_t>(num_pairs, 1));
+  launch(dev, get_kernel(dev, "sort", "csr_make_keys"), pg.blocks, pg.threads, 0,
+         static_cast<const uint32_t*>(d_pairs.as<uint32_t>()), static_cast<uint64_t>(num_pairs), n, id_bits,
+         symmetric ? 1u : 0u, keys_a.as<unsigned long long>(), bad.as<uint32_t>());
+  uint32_t bad_count = 0;
+  copy_to_host(dev, &bad_count, bad.get(), sizeof(uint32_t));
+  if (bad_count > 0) {
+    // the first offending pair in input order, with the host builder's message
+    for (size_t i = 0; i < num_pairs; ++i) {
+      const uint32_t s = pairs[2 * i], d = pairs[2 * i + 1];
+      if (s >= n || d >= n) {
+        fail_csr("edge (" + std::to_string(s) + ", " + std::to_string(d) +
+                 ") has a node id >= num_nodes " + std::to_string(n));
+      }
+    }
+  }
+  d_pairs.release();
+  unsigned long long* sorted =
+      radix_sort(dev, keys_a.as<unsigned long long>(), keys_b.as<unsigned long long>(), num_keys, 2 * id_bits);
+  unsigned long long* spare = sorted == keys_a.as<unsigned long long>() ? keys_b.as<unsigned long long>()
+                                                                        : keys_a.as<unsigned long long>();
+  // first copies of each key, compacted in order into `spare`
+  uint64_t unique = 0;
+  if (num_keys > 0) {
+    DeviceMemory flags, tile_counts;
+    flags.allocate(std::max<uint64_t>(num_keys, 4));
+    const uint64_t tiles = (num_keys + kTile - 1) / kTile;
+    tile_counts.allocate(sizeof(uint32_t) * (tiles + 1));
+    Grid1D kg = thread_per_item(num_keys);
+    launch(dev, get_kernel(dev, "sort", "csr_first_copies"), kg.blocks, kg.threads, 0,
+           static_cast<const unsigned long long*>(sorted), num_keys, flags.as<uint8_t>());
+    launch(dev, get_kernel(dev, "util", "compact_count"), static_cast<uint32_t>(tiles), 256u, 0,
+           static_cast<const uint8_t*>(flags.as<uint8_t>()), num_keys, tile_counts.as<uint32_t>());
+    launch(dev, get_kernel(dev, "util", "exclusive_scan_small"), 1u, 1024u, 0, tile_counts.as<uint32_t>(),
+           static_cast<uint32_t>(tiles), tile_counts.as<uint32_t>() + tiles);
+    uint32_t total = 0;
+    copy_to_host(dev, &total, tile_counts.as<uint32_t>() + tiles, sizeof(uint32_t));
+    unique = total;
+    launch(dev, get_kernel(dev, "sort", "csr_compact_keys"), static_cast<uint32_t>(tiles), 256u, 0,
+           static_cast<const unsigned long long*>(sorted), static_cast<const uint8_t*>(flags.as<uint8_t>()),
+           num_keys, static_cast<const uint32_t*>(tile_counts.as<uint32_t>()), spare);
+  }
+  if (unique > 0xFFFFFFFFull) {
+    fail_csr("graph too large for u32 indices: " + std::to_string(unique) + " targets");
+  }
+  out.n = n;
+  out.m = static_cast<uint32_t>(unique);
+  out.offsets.allocate(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  out.targets.allocate(sizeof(uint32_t) * std::max<uint64_t>(unique, 1));
+  Grid1D cg = thread_per_item(std::max<uint64_t>(unique, static_cast<uint64_t>(n) + 1));
+  launch(dev, get_kernel(dev, "sort", "csr_from_keys"), cg.blocks, cg.threads, 0,
+         static_cast<const unsigned long long*>(spare), unique, n, static_cast<uint32_t>(id_bits),
+         out.targets.as<uint32_t>(), out.offsets.as<uint32_t>());
+  dev.sync();
+}
+
+}  // namespace gc
+
+using gc::guard;
+
+extern "C" {
+
+gc_status gc_csr_from_edge_pairs_device(gc_device* dev_handle, const uint32_t* pairs, size_t num_pairs,
+                                        uint32_t num_nodes, int32_t symmetric, gc_csr* out) {
+  return guard([&] {
+    if (out == nullptr) gc::fail_kernel("out is null");
+    std::memset(out, 0, sizeof(*out));
+    if (dev_handle == nullptr) gc::fail_kernel("device is null");
+    if (pairs == nullptr && num_pairs > 0) gc::fail_kernel("pairs is null");
+    auto* dev = reinterpret_cast<gc::Device*>(dev_handle);
+    std::lock_guard<std::recursive_mutex> lock(dev->mu);
+    dev->make_current();
+    gc::DeviceCsr csr;
+    gc::build_csr_on_device(*dev, pairs, num_pairs, num_nodes, symmetric != 0, csr);
+    auto* offsets = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * (size_t(num_nodes) + 1)));
+    auto* targets = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<size_t>(1, csr.m)));
+    if (!offsets || !targets) {
+      std::free(offsets);
+      std::free(targets);
+      throw std::bad_alloc();
+    }
+    gc::copy_to_host(*dev, offsets, csr.offsets.get(), sizeof(uint32_t) * (size_t(num_nodes) + 1));
+    if (csr.m > 0) gc::copy_to_host(*dev, targets, csr.targets.get(), sizeof(uint32_t) * csr.m);
+    out->offsets = offsets;
+    out->targets = targets;
+    out->num_nodes = num_nodes;
+    out->num_edges = csr.m;
+  });
+}
+
+gc_status gc_graph_from_edge_pairs(gc_device* dev_handle, const uint32_t* pairs, size_t num_pairs,
+                                   uint32_t num_nodes, int32_t symmetric, gc_graph** out) {
+  return guard([&] {
+    if (out == nullptr) gc::fail_kernel("out is null");
+    *out = nullptr;
+    if (dev_handle == nullptr) gc::fail_kernel("device is null");
+    if (pairs == nullptr && num_pairs > 0) gc::fail_kernel("pairs is null");
+    auto* dev = reinterpret_cast<gc::Device*>(dev_handle);
+    std::lock_guard<std::recursive_mutex> lock(dev->mu);
+    dev->make_current();
+    gc::DeviceCsr csr;
+    gc::build_csr_on_device(*dev, pairs, num_pairs, num_nodes, symmetric != 0, csr);
+    std::unique_ptr<gc::Graph> g(new gc::Graph());
+    g->n = csr.n;
+    g->m = csr.m;
+    g->offsets = std::move(csr.offsets);
+    g->targets = std::move(csr.targets);
+    // host copies for the host-side preparation of the derived layouts
+    g->host_offsets.resize(size_t(g->n) + 1);
+    g->host_targets.resize(g->m);
+    gc::copy_to_host(*dev, g->host_offsets.data(), g->offsets.get(), sizeof(uint32_t) * g->host_offsets.size());
+    if (g->m > 0) gc::copy_to_host(*dev, g->host_targets.data(), g->targets.get(), sizeof(uint32_t) * g->m);
+    g->dev = reinterpret_cast<gc::Device*>(gc_device_retain(dev_handle));
+    *out = reinterpret_cast<gc_graph*>(g.release());
+  });
+}
+
+}  // extern "C"

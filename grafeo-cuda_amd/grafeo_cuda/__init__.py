@@ -119,6 +119,22 @@ class GpuGraph:
             raise _runtime_error(e) from None
         return GpuGraph(handle, device)
 
+    @staticmethod
+    def from_edge_pairs(edges, num_nodes: int, device: GpuDevice, symmetric: bool = True) -> "GpuGraph":
+        """A GpuGraph whose CSR is built on the GPU from (source, target) pairs (list or (E, 2)
+        array): from_edge_pairs' arrays without a host CSR."""
+        if not isinstance(device, GpuDevice):
+            raise TypeError("device must be a GpuDevice")
+        pairs = _native.pairs_array(edges)
+        handle = c_void_p()
+        try:
+            check(lib.gc_graph_from_edge_pairs(device._handle, _native.u32_ptr(pairs), pairs.shape[0],
+                                               _u32_arg(num_nodes, "num_nodes"), int(bool(symmetric)),
+                                               byref(handle)))
+        except GcError as e:
+            raise _runtime_error(e) from None
+        return GpuGraph(handle, device)
+
     @property
     def num_nodes(self) -> int:
         """Number of nodes."""

@@ -114,6 +114,14 @@ gc_status gc_csr_from_edge_pairs(const uint32_t* pairs, size_t num_pairs, uint32
 gc_status gc_csr_from_edge_pairs_oriented(const uint32_t* pairs, size_t num_pairs,
                                           uint32_t num_nodes, gc_csr* out);
 
+/* from_edge_pairs (crates/grafeo-cuda-core/src/csr_export.rs:202-239) built on
+ * the GPU (stable radix sort of packed keys, dedup, offsets by binary
+ * search): the same arrays as gc_csr_from_edge_pairs, returned to the host
+ * (gc_csr_free). Out-of-range ids are GC_ERR_INVALID_CSR with the same
+ * message. */
+gc_status gc_csr_from_edge_pairs_device(gc_device* dev, const uint32_t* pairs, size_t num_pairs,
+                                        uint32_t num_nodes, int32_t symmetric, gc_csr* out);
+
 /* parse_tsv_to_edges (crates/grafeo-cuda-core/src/csr_export.rs:246-287).
  * `*pairs` is library-allocated (gc_free); ids are remapped in order of first
  * appearance. A missing file is GC_ERR_EXPORT. */
@@ -331,6 +339,13 @@ gc_status gc_bfs_level_timings(gc_graph* g, uint32_t source, uint32_t max_levels
  * every method's count is checked against the others. */
 gc_status gc_tc_bucket_timings(gc_graph* g, uint32_t max_buckets, float* features, float* times_us,
                                uint32_t* buckets);
+
+/* GpuGraph from edge pairs without a host CSR: the CSR is built on the GPU
+ * (as gc_csr_from_edge_pairs_device) and becomes the graph's arrays; it is
+ * valid by construction, so no validate_csr pass runs (an addition: the
+ * reference builds the CSR on the host, harness.rs:84-115). */
+gc_status gc_graph_from_edge_pairs(gc_device* dev, const uint32_t* pairs, size_t num_pairs,
+                                   uint32_t num_nodes, int32_t symmetric, gc_graph** out);
 
 /* Device memory for the *_device entry points' outputs (hipMalloc on the
  * device of `dev`). No reference counterpart. */

@@ -1,0 +1,77 @@
+"""CSR construction on the GPU: the same arrays as the host builder (from_edge_pairs,
+crates/grafeo-cuda-core/src/csr_export.rs:202-239) for symmetric and directed inputs with
+self-loops, duplicates and isolated nodes, at small and full sizes, and the same error for
+out-of-range ids."""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+import grafeo_cuda
+from grafeo_cuda import _native
+from graphs import csr_sha256, random_graph
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rmat_counts.json")
+
+
+def same(device, pairs, n, symmetric):
+    host = _native.from_edge_pairs(pairs, n, symmetric)
+    dev = _native.from_edge_pairs_device(device._handle, pairs, n, symmetric)
+    assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1])
+    return host
+
+
+def test_small_inputs(device):
+    assert same(device, [], 0, True)[0].tolist() == [0]
+    assert same(device, [], 5, True)[0].tolist() == [0] * 6
+    off, tgt = same(device, [(0, 0), (0, 0)], 1, True)
+    assert off.tolist() == [0, 1] and tgt.tolist() == [0]
+    rng = np.random.default_rng(5)
+    for _ in range(60):
+        n, edges = random_graph(rng, max_nodes=40, max_edges=200)
+        for symmetric in (True, False):
+            same(device, edges, n, symmetric)
+
+
+def test_large_inputs(device):
+    rng = np.random.default_rng(6)
+    for n, count in ((1, 1000), (2, 5000), (300, 100000), (70000, 1_000_000), (1 << 20, 3_000_000)):
+        pairs = rng.integers(0, n, size=(count, 2), dtype=np.int64).astype(np.uint32)
+        for symmetric in (True, False):
+            same(device, pairs, n, symmetric)
+
+
+def test_rmat20_matches_the_golden_csr_and_counts(device):
+    with open(GOLDEN) as f:
+        golden = json.load(f)["rmat20_ef16_seed1"]
+    pairs = _native.gen_rmat(20, 16, 1)
+    t = time.perf_counter()
+    host = _native.from_edge_pairs(pairs, 1 << 20, True)
+    host_s = time.perf_counter() - t
+    t = time.perf_counter()
+    dev = _native.from_edge_pairs_device(device._handle, pairs, 1 << 20, True)
+    dev_s = time.perf_counter() - t
+    assert csr_sha256(*dev) == golden["csr_sha256"] == csr_sha256(*host)
+    print(f"RMAT-20 CSR build: host {host_s:.3f}s, device {dev_s:.3f}s (with transfers)")
+    g = grafeo_cuda.GpuGraph.from_edge_pairs(pairs, 1 << 20, device)
+    assert g.num_edges == len(host[1])
+    assert g.triangle_count()["total_triangles"] == golden["triangles"]
+
+
+def test_out_of_range_ids_are_rejected_like_the_host_builder(device):
+    pairs = [(0, 1), (2, 7), (9, 1)]
+    with pytest.raises(_native.GcError) as host_err:
+        _native.from_edge_pairs(pairs, 5, True)
+    with pytest.raises(_native.GcError) as dev_err:
+        _native.from_edge_pairs_device(device._handle, pairs, 5, True)
+    assert str(host_err.value) == str(dev_err.value)
+    assert dev_err.value.kind == "InvalidCsr"
+    with pytest.raises(RuntimeError, match="node id >= num_nodes"):
+        grafeo_cuda.GpuGraph.from_edge_pairs(pairs, 5, device)
