@@ -344,8 +344,15 @@ void exclusive_scan_u32(Device& dev, uint32_t* data, uint64_t count, uint32_t* b
 // and 4096 entries (rows above that search global memory instead).
 uint32_t wedge_table_bits(const Relabelled& rl);
 
-// Builds the derived layouts if not done yet (host relabel + upload).
+// Builds the derived layouts if not done yet (on the device; GC_PREPARE=host
+// runs the host restatement instead).
 void prepare(Graph& g);
+
+// Stable LSD radix sort of the low `bits` bits of n 64-bit keys, 8 bits per
+// pass, ping-ponging between keys and tmp; returns the buffer holding the
+// sorted keys (csr_gpu.cpp, kernels of sort.hip).
+unsigned long long* radix_sort(Device& dev, unsigned long long* keys, unsigned long long* tmp, uint64_t n,
+                               uint32_t bits);
 void ensure_fwd_original(Graph& g);
 
 // Host-side CSR validation with the reference's messages (throws InvalidCsr).

@@ -21,8 +21,8 @@ unsigned bits_of(uint64_t max_value) {
   return b;
 }
 
-// Sorts the low `bits` bits of n keys (stable LSD, 8 bits per pass); returns
-// whichever of keys / tmp holds the result.
+}  // namespace
+
 unsigned long long* radix_sort(Device& dev, unsigned long long* keys, unsigned long long* tmp, uint64_t n,
                                uint32_t bits) {
   if (n <= 1) return keys;
@@ -45,8 +45,6 @@ unsigned long long* radix_sort(Device& dev, unsigned long long* keys, unsigned l
   }
   return keys;
 }
-
-}  // namespace
 
 struct DeviceCsr {
   DeviceMemory offsets;  // u32[n + 1]
@@ -175,11 +173,9 @@ gc_status gc_graph_from_edge_pairs(gc_device* dev_handle, const uint32_t* pairs,
     g->m = csr.m;
     g->offsets = std::move(csr.offsets);
     g->targets = std::move(csr.targets);
-    // host copies for the host-side preparation of the derived layouts
+    // the offsets on the host, for planning (targets stay on the device)
     g->host_offsets.resize(size_t(g->n) + 1);
-    g->host_targets.resize(g->m);
     gc::copy_to_host(*dev, g->host_offsets.data(), g->offsets.get(), sizeof(uint32_t) * g->host_offsets.size());
-    if (g->m > 0) gc::copy_to_host(*dev, g->host_targets.data(), g->targets.get(), sizeof(uint32_t) * g->m);
     g->dev = reinterpret_cast<gc::Device*>(gc_device_retain(dev_handle));
     *out = reinterpret_cast<gc_graph*>(g.release());
   });

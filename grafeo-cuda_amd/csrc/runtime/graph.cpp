@@ -6,24 +6,157 @@
 //
 // The derived layout (built on first use or by gc_graph_prepare) has no
 // reference counterpart: a degree-ordered relabelling of the graph. Vertices
-// get new ids sorted by (degree, old id) ascending by a stable counting sort;
-// each relabelled row is sorted, and fwd[i] marks where its entries above i
-// start. The kernels that need locality (PageRank, BFS) or the
-// degree-oriented DAG (triangle counting, k-truss on symmetric graphs) run on
-// it, and results are mapped back to the caller's ids on the device.
+// get new ids sorted by (degree, old id) ascending; each relabelled row is
+// sorted, and fwd[i] marks where its entries above i start. The kernels that
+// need locality (PageRank, BFS) or the degree-oriented DAG (triangle
+// counting, k-truss on symmetric graphs) run on it, and results are mapped
+// back to the caller's ids on the device. It is built on the device (radix
+// sorts of (degree, id) and (new source, new target) keys, kernels of
+// prep.hip); GC_PREPARE=host builds the same arrays on the host instead.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <numeric>
 
 #include "common.hpp"
 
 namespace gc {
 
-void prepare(Graph& g) {
-  if (g.prepared) return;
-  Stopwatch sw;
+namespace {
+
+unsigned bits_needed(uint64_t max_value) {
+  unsigned b = 0;
+  while (b < 64 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+// (vertex, begin, end) slices of in-lists longer than Relabelled::kHubChunk,
+// heaviest vertices first.
+std::vector<uint32_t> hub_slices(const std::vector<uint32_t>& r_off, const std::vector<uint32_t>& in_end) {
+  std::vector<uint32_t> hubs;
+  for (size_t i = in_end.size(); i-- > 0;) {
+    if (in_end[i] - r_off[i] > Relabelled::kHubChunk) {
+      for (uint32_t s = r_off[i]; s < in_end[i]; s += Relabelled::kHubChunk) {
+        hubs.push_back(static_cast<uint32_t>(i));
+        hubs.push_back(s);
+        hubs.push_back(std::min(s + Relabelled::kHubChunk, in_end[i]));
+      }
+    }
+  }
+  return hubs;
+}
+
+void upload_hubs(Graph& g, const std::vector<uint32_t>& hubs) {
+  g.rl.num_hub_units = static_cast<uint32_t>(hubs.size() / 3);
+  g.rl.hub_units.allocate(sizeof(uint32_t) * std::max<size_t>(hubs.size(), 3));
+  if (!hubs.empty()) copy_to_device(*g.dev, g.rl.hub_units.get(), hubs.data(), sizeof(uint32_t) * hubs.size());
+}
+
+void prepare_device(Graph& g) {
   Device& dev = *g.dev;
   const uint32_t n = g.n;
   const uint64_t m = g.m;
+  Relabelled& rl = g.rl;
+  const uint32_t* off = g.offsets.as<const uint32_t>();
+  const uint32_t* tgt = g.targets.as<const uint32_t>();
+  uint32_t max_deg = 0;
+  for (uint32_t v = 0; v < n; ++v) max_deg = std::max(max_deg, g.host_offsets[v + 1] - g.host_offsets[v]);
+  const unsigned id_bits = std::max(1u, bits_needed(n == 0 ? 0 : n - 1));
+
+  // symmetry and self-loops
+  DeviceMemory counts;
+  counts.allocate(64);
+  fill_words(dev, counts.get(), 2, 0);
+  if (m > 0) {
+    Grid1D eg = thread_per_item(m);
+    launch(dev, get_kernel(dev, "prep", "prep_symmetry"), eg.blocks, eg.threads, 0, off, tgt, n, m,
+           counts.as<uint32_t>());
+  }
+  uint32_t host_counts[2] = {0, 0};
+  copy_to_host(dev, host_counts, counts.get(), sizeof(host_counts));
+  g.self_loops = host_counts[0];
+  g.symmetric = host_counts[1] == 0;
+
+  rl.perm.allocate(sizeof(uint32_t) * std::max<size_t>(n, 1));
+  rl.inv.allocate(sizeof(uint32_t) * std::max<size_t>(n, 1));
+  rl.offsets.allocate(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  rl.targets.allocate(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  rl.fwd.allocate(sizeof(uint32_t) * std::max<size_t>(n, 1));
+  {
+    // vertex order: sort (degree, id)
+    DeviceMemory ka, kb;
+    ka.allocate(sizeof(unsigned long long) * std::max<size_t>(n, 1));
+    kb.allocate(sizeof(unsigned long long) * std::max<size_t>(n, 1));
+    Grid1D vg = thread_per_item(std::max<uint32_t>(n, 1));
+    launch(dev, get_kernel(dev, "prep", "prep_degree_keys"), vg.blocks, vg.threads, 0, off, n, id_bits,
+           ka.as<unsigned long long>());
+    unsigned long long* sorted = radix_sort(dev, ka.as<unsigned long long>(), kb.as<unsigned long long>(), n,
+                                            id_bits + std::max(1u, bits_needed(max_deg)));
+    launch(dev, get_kernel(dev, "prep", "prep_perm"), vg.blocks, vg.threads, 0,
+           static_cast<const unsigned long long*>(sorted), n, id_bits, rl.perm.as<uint32_t>(),
+           rl.inv.as<uint32_t>());
+  }
+  {
+    // relabelled rows: sort (inv[u], inv[v]) over every entry
+    DeviceMemory ka, kb;
+    ka.allocate(sizeof(unsigned long long) * std::max<uint64_t>(m, 1));
+    kb.allocate(sizeof(unsigned long long) * std::max<uint64_t>(m, 1));
+    unsigned long long* sorted = ka.as<unsigned long long>();
+    if (m > 0) {
+      Grid1D eg = thread_per_item(m);
+      launch(dev, get_kernel(dev, "prep", "prep_entry_keys"), eg.blocks, eg.threads, 0, off, tgt, n, m,
+             static_cast<const uint32_t*>(rl.inv.as<uint32_t>()), static_cast<uint32_t>(id_bits),
+             ka.as<unsigned long long>());
+      sorted = radix_sort(dev, ka.as<unsigned long long>(), kb.as<unsigned long long>(), m, 2 * id_bits);
+    }
+    Grid1D cg = thread_per_item(std::max<uint64_t>(m, static_cast<uint64_t>(n) + 1));
+    launch(dev, get_kernel(dev, "sort", "csr_from_keys"), cg.blocks, cg.threads, 0,
+           static_cast<const unsigned long long*>(sorted), m, n, static_cast<uint32_t>(id_bits),
+           rl.targets.as<uint32_t>(), rl.offsets.as<uint32_t>());
+  }
+  if (n > 0) {
+    Grid1D vg = thread_per_item(n);
+    launch(dev, get_kernel(dev, "util", "csr_forward_start"), vg.blocks, vg.threads, 0,
+           rl.offsets.as<const uint32_t>(), rl.targets.as<const uint32_t>(), n, rl.fwd.as<uint32_t>());
+  }
+  // statistics and in-list ends
+  DeviceMemory stats, in_end;
+  stats.allocate(sizeof(unsigned long long) * 8);
+  in_end.allocate(sizeof(uint32_t) * std::max<size_t>(n, 1));
+  fill_words(dev, stats.get(), 16, 0);
+  if (n > 0) {
+    Grid1D vg = thread_per_item(n);
+    launch(dev, get_kernel(dev, "prep", "prep_stats"), vg.blocks, vg.threads, 0,
+           rl.offsets.as<const uint32_t>(), rl.targets.as<const uint32_t>(), rl.fwd.as<const uint32_t>(), n,
+           stats.as<unsigned long long>(), in_end.as<uint32_t>());
+  }
+  unsigned long long s[8];
+  copy_to_host(dev, s, stats.get(), sizeof(s));
+  rl.host_offsets.resize(static_cast<size_t>(n) + 1);
+  rl.host_inv.resize(n);
+  std::vector<uint32_t> host_in_end(n);
+  copy_to_host(dev, rl.host_offsets.data(), rl.offsets.get(), sizeof(uint32_t) * rl.host_offsets.size());
+  if (n > 0) {
+    copy_to_host(dev, rl.host_inv.data(), rl.inv.get(), sizeof(uint32_t) * n);
+    copy_to_host(dev, host_in_end.data(), in_end.get(), sizeof(uint32_t) * n);
+  }
+  rl.oriented_entries = s[0];
+  rl.intersect_entries = s[1];
+  rl.wedges = s[2];
+  rl.dangling = static_cast<uint32_t>(s[3]);
+  rl.max_degree = static_cast<uint32_t>(s[4]);
+  rl.max_out_degree = static_cast<uint32_t>(s[5]);
+  upload_hubs(g, hub_slices(rl.host_offsets, host_in_end));
+}
+
+void prepare_host(Graph& g) {
+  Device& dev = *g.dev;
+  const uint32_t n = g.n;
+  const uint64_t m = g.m;
+  if (g.host_targets.size() != m) {
+    g.host_targets.resize(m);
+    if (m > 0) copy_to_host(dev, g.host_targets.data(), g.targets.get(), sizeof(uint32_t) * m);
+  }
   const uint32_t* off = g.host_offsets.data();
   const uint32_t* tgt = g.host_targets.data();
   Relabelled& rl = g.rl;
@@ -134,12 +267,19 @@ void prepare(Graph& g) {
   rl.dangling = dangling;
   rl.max_degree = max_deg;
   rl.max_out_degree = max_out;
-  rl.num_hub_units = static_cast<uint32_t>(hubs.size() / 3);
-  rl.hub_units.allocate(sizeof(uint32_t) * std::max<size_t>(hubs.size(), 3));
-  copy_to_device(dev, rl.hub_units.get(), hubs.data(), sizeof(uint32_t) * hubs.size());
-
-  // The host copy of the original CSR is no longer needed.
+  upload_hubs(g, hubs);
+  // The host copy of the original targets is no longer needed.
   std::vector<uint32_t>().swap(g.host_targets);
+}
+
+}  // namespace
+
+void prepare(Graph& g) {
+  if (g.prepared) return;
+  Stopwatch sw;
+  const char* env = std::getenv("GC_PREPARE");
+  if (env != nullptr && std::strcmp(env, "host") == 0) prepare_host(g);
+  else prepare_device(g);
   g.prepared = true;
   g.prepare_seconds = sw.seconds();
 }
@@ -182,7 +322,6 @@ static void upload(gc_device* dev_handle, const uint32_t* offsets, size_t num_of
   copy_to_device(*dev, g->offsets.get(), offsets, sizeof(uint32_t) * num_offsets);
   copy_to_device(*dev, g->targets.get(), targets, sizeof(uint32_t) * num_targets);
   g->host_offsets.assign(offsets, offsets + num_offsets);
-  g->host_targets.assign(targets, targets + num_targets);
   g->dev = reinterpret_cast<Device*>(gc_device_retain(dev_handle));
   *out = reinterpret_cast<gc_graph*>(g.release());
 }
@@ -232,6 +371,20 @@ uint32_t gc_graph_num_edges(const gc_graph* h) {
 
 const uint32_t* gc_graph_device_offsets(const gc_graph* h) {
   return h ? reinterpret_cast<const gc::Graph*>(h)->offsets.as<const uint32_t>() : nullptr;
+}
+
+gc_status gc_graph_relabelled(gc_graph* h, const uint32_t** offsets, const uint32_t** targets,
+                              const uint32_t** perm, const uint32_t** fwd) {
+  return guard([&] {
+    gc::Graph* g = gc::as_graph(h);
+    std::lock_guard<std::recursive_mutex> lock(g->dev->mu);
+    g->dev->make_current();
+    gc::prepare(*g);
+    if (offsets) *offsets = g->rl.offsets.as<const uint32_t>();
+    if (targets) *targets = g->rl.targets.as<const uint32_t>();
+    if (perm) *perm = g->rl.perm.as<const uint32_t>();
+    if (fwd) *fwd = g->rl.fwd.as<const uint32_t>();
+  });
 }
 
 const uint32_t* gc_graph_device_targets(const gc_graph* h) {

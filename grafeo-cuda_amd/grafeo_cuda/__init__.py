@@ -157,6 +157,20 @@ class GpuGraph:
         self._call(lib.gc_graph_prepare, byref(secs))
         return secs.value
 
+    def relabelled_numpy(self) -> dict:
+        """The degree-ordered layout the kernels run on, copied to the host: {'offsets',
+        'targets', 'perm' (new id -> id), 'fwd' (first entry above each row)}."""
+        ptrs = [c_void_p() for _ in range(4)]
+        self._call(lib.gc_graph_relabelled, *[byref(p) for p in ptrs])
+        n, m = self.num_nodes, self.num_edges
+        out = {}
+        for name, p, size in zip(("offsets", "targets", "perm", "fwd"), ptrs, (n + 1, m, n, n)):
+            arr = np.zeros(max(size, 1), dtype=np.uint32)
+            if size:
+                check(lib.gc_device_copy_to_host(self._device._handle, arr.ctypes.data_as(c_void_p), p, 4 * size))
+            out[name] = arr[:size]
+        return out
+
     def is_symmetric(self) -> bool:
         sym = c_int32()
         self._call(lib.gc_graph_is_symmetric, byref(sym))

@@ -69,6 +69,8 @@ def _load() -> ctypes.CDLL:
         "gc_csr_from_edge_pairs": (c_int, [u32p, c_size_t, c_uint32, c_int, POINTER(Csr)]),
         "gc_csr_from_edge_pairs_device": (c_int, [c_void_p, u32p, c_size_t, c_uint32, c_int, POINTER(Csr)]),
         "gc_graph_from_edge_pairs": (c_int, [c_void_p, u32p, c_size_t, c_uint32, c_int, POINTER(c_void_p)]),
+        "gc_graph_relabelled": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p),
+                                        POINTER(c_void_p)]),
         "gc_csr_from_edge_pairs_oriented": (c_int, [u32p, c_size_t, c_uint32, POINTER(Csr)]),
         "gc_parse_tsv_to_edges": (c_int, [c_char_p, POINTER(u32p), POINTER(c_size_t), u32p]),
         "gc_graph_from_csr": (c_int, [c_void_p, u32p, c_size_t, u32p, c_size_t, POINTER(c_void_p)]),
@@ -162,7 +164,7 @@ EXPORTED = [
     "gc_last_error", "gc_free", "gc_device_count", "gc_device_new", "gc_device_retain",
     "gc_device_release", "gc_device_properties", "gc_csr_free", "gc_validate_csr",
     "gc_csr_validate", "gc_csr_from_edge_pairs", "gc_csr_from_edge_pairs_oriented",
-    "gc_csr_from_edge_pairs_device", "gc_graph_from_edge_pairs",
+    "gc_csr_from_edge_pairs_device", "gc_graph_from_edge_pairs", "gc_graph_relabelled",
     "gc_parse_tsv_to_edges", "gc_graph_from_csr", "gc_graph_from_csr_arrays", "gc_graph_free",
     "gc_graph_num_nodes", "gc_graph_num_edges", "gc_graph_device_offsets",
     "gc_graph_device_targets", "gc_graph_prepare", "gc_graph_is_symmetric", "gc_graph_dag_stats",

@@ -347,6 +347,13 @@ gc_status gc_tc_bucket_timings(gc_graph* g, uint32_t max_buckets, float* feature
 gc_status gc_graph_from_edge_pairs(gc_device* dev, const uint32_t* pairs, size_t num_pairs,
                                    uint32_t num_nodes, int32_t symmetric, gc_graph** out);
 
+/* The degree-ordered layout the kernels run on (built on first use; no
+ * reference counterpart), as device pointers owned by the graph: offsets
+ * (num_nodes + 1), targets (num_edges, each row sorted), perm (new id -> the
+ * caller's id) and fwd (per row, the first entry above the row). */
+gc_status gc_graph_relabelled(gc_graph* g, const uint32_t** offsets, const uint32_t** targets,
+                              const uint32_t** perm, const uint32_t** fwd);
+
 /* Device memory for the *_device entry points' outputs (hipMalloc on the
  * device of `dev`). No reference counterpart. */
 gc_status gc_device_malloc(gc_device* dev, size_t bytes, void** ptr);

@@ -75,3 +75,44 @@ def test_out_of_range_ids_are_rejected_like_the_host_builder(device):
     assert dev_err.value.kind == "InvalidCsr"
     with pytest.raises(RuntimeError, match="node id >= num_nodes"):
         grafeo_cuda.GpuGraph.from_edge_pairs(pairs, 5, device)
+
+
+def dag_stats(g):
+    from ctypes import byref, c_uint64
+
+    a, b, c = c_uint64(), c_uint64(), c_uint64()
+    _native.check(_native.lib.gc_graph_dag_stats(g._handle, byref(a), byref(b), byref(c)))
+    return a.value, b.value, c.value
+
+
+def test_device_preparation_builds_the_host_layout(device, monkeypatch):
+    """The degree-ordered layout built on the device equals the host restatement's array for
+    array (offsets, targets, perm, fwd), with the same symmetry flag and DAG statistics."""
+    from graphs import catalog, rmat_graph, undirected
+
+    cases = [(name, off, tgt) for name, off, tgt in catalog()]
+    rng = np.random.default_rng(77)
+    for i in range(30):
+        n, edges = random_graph(rng, max_nodes=30, max_edges=120)
+        cases.append((f"random #{i}", *undirected(n, edges)))
+        cases.append((f"directed #{i}", *_native.from_edge_pairs(edges, n, False)))
+    cases.append(("RMAT-14", *rmat_graph(14, 16, seed=2)))
+    cases.append(("RMAT-20", *rmat_graph(20, 16, seed=1)))
+    for name, off, tgt in cases:
+        monkeypatch.delenv("GC_PREPARE", raising=False)
+        gd = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+        t = time.perf_counter()
+        dev_layout = gd.relabelled_numpy()
+        dev_s = time.perf_counter() - t
+        monkeypatch.setenv("GC_PREPARE", "host")
+        gh = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+        t = time.perf_counter()
+        host_layout = gh.relabelled_numpy()
+        host_s = time.perf_counter() - t
+        for key in ("offsets", "targets", "perm", "fwd"):
+            assert np.array_equal(dev_layout[key], host_layout[key]), (name, key)
+        assert gd.is_symmetric() == gh.is_symmetric(), name
+        assert dag_stats(gd) == dag_stats(gh), name
+        if name.startswith("RMAT-20"):
+            print(f"RMAT-20 preparation: device {dev_s:.3f}s, host {host_s:.3f}s")
+    monkeypatch.delenv("GC_PREPARE", raising=False)
