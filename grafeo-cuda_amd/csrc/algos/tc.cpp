@@ -18,13 +18,51 @@ namespace {
 
 constexpr uint32_t kBitmap = 0, kHash = 1, kLists = 2;  // the scheduler's choices
 constexpr uint32_t kHotBits = 32768;                    // TCB_HOT_BITS of tc.hip
-constexpr uint32_t kMaxLaunches = 64;                   // work-counter pairs in the scratch
+constexpr uint32_t kMaxLaunches = 64;                   // launches per count
+constexpr uint32_t kMaxParts = 8;                       // middle-vertex parts of tc_bitmap (one per XCD)
+constexpr uint32_t kCounterStride = 64;                 // TCB_COUNTER_STRIDE of tc.hip
+constexpr uint32_t kCounterWords = kCounterStride * kMaxParts;  // work counters of one launch
 
 uint64_t read_result(Device& dev, const unsigned long long* result) {
   uint64_t total = 0;
   copy_to_host(dev, dev.pinned, result, sizeof(uint64_t));
   std::memcpy(&total, dev.pinned, sizeof(uint64_t));
   return total;
+}
+
+// Middle-vertex parts of tc_bitmap: GC_TC_PARTS (1..8), default 8 (one per
+// XCD). Measured on MI355X: R-MAT scale 22 40.3 ms at 1 part, 21.1 ms at 8;
+// scale 14 0.50 -> 0.45 ms; a 4096-vertex Erdos-Renyi graph 0.26 -> 0.14 ms.
+uint32_t tc_parts() {
+  const char* env = std::getenv("GC_TC_PARTS");
+  if (env == nullptr) return kMaxParts;
+  const long p = std::strtol(env, nullptr, 10);
+  return static_cast<uint32_t>(std::min<long>(std::max<long>(p, 1), kMaxParts));
+}
+
+// Rows handed out one at a time by tc_bitmap: the leading rows of [row_lo,
+// row_hi) in descending order whose degree is at least GC_TC_HEAVY_DEGREE
+// (default 64), at most 65536; the rest go in chunks of 64.
+uint32_t tc_heavy_rows(const Graph& g, uint32_t row_lo, uint32_t row_hi) {
+  const char* env = std::getenv("GC_TC_HEAVY_DEGREE");
+  const uint32_t degree = env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : 64u;
+  const std::vector<uint32_t>& off = g.rl.host_offsets;
+  // rows are sorted by degree: the first row of degree >= `degree`
+  uint32_t a = row_lo, b = row_hi;
+  while (a < b) {
+    const uint32_t mid = a + (b - a) / 2;
+    if (off[mid + 1] - off[mid] >= degree) b = mid;
+    else a = mid + 1;
+  }
+  return std::min<uint32_t>(row_hi - a, 65536);
+}
+
+// Shortest L(u) that tc_bitmap splits over the parts (shorter rows run whole,
+// once): GC_TC_SPLIT, default 64.
+uint32_t tc_split_min() {
+  const char* env = std::getenv("GC_TC_SPLIT");
+  if (env == nullptr) return 64;
+  return static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
 }
 
 // Runs tc_count over the lists L(x) = targets[lo[x] .. hi[x]) for rows
@@ -43,7 +81,8 @@ void launch_tc_count(Graph& g, const uint32_t* lo, const uint32_t* hi, const uin
          descending_u, after_u, heavy, light_chunk, counters, result);
 }
 
-// One symmetric-graph method over rows [row_lo, row_hi) of the relabelled DAG.
+// One symmetric-graph method over rows [row_lo, row_hi) of the relabelled DAG;
+// `counters` are kCounterWords zeroed words.
 void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, uint32_t* counters,
                    unsigned long long* result) {
   Device& dev = *g.dev;
@@ -64,14 +103,16 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     // hot range [hot_base, n): at most kHotBits ids, hot_base a multiple of 32
     const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
     launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid, 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
-           row_lo, hot_base, heavy, light_chunk, counters, result);
+           row_lo, hot_base, tc_heavy_rows(g, row_lo, row_hi), light_chunk, tc_parts(), tc_split_min(),
+           counters, result);
   }
 }
 
-// Scratch: [0..1] u64 result, then kMaxLaunches pairs of work counters.
+// Scratch: [0..1] u64 result, then kCounterWords work counters per launch.
 uint32_t* tc_scratch(Graph& g) {
-  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", sizeof(uint32_t) * (2 + 2 * kMaxLaunches));
-  fill_words(*g.dev, scratch, 2 + 2 * kMaxLaunches, 0);
+  constexpr uint64_t words = 2 + uint64_t(kCounterWords) * kMaxLaunches;
+  uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", sizeof(uint32_t) * words);
+  fill_words(*g.dev, scratch, words, 0);
   return scratch;
 }
 
@@ -218,7 +259,7 @@ uint64_t triangle_count_symmetric(Graph& g) {
       size_t e = b + 1;
       while (e < buckets.size() && choice[e] == choice[b]) ++e;
       if (launches >= kMaxLaunches) fail_kernel("too many triangle-count launches");
-      launch_method(g, choice[b], buckets[b].row_lo, buckets[e - 1].row_hi, scratch + 2 + 2 * launches,
+      launch_method(g, choice[b], buckets[b].row_lo, buckets[e - 1].row_hi, scratch + 2 + kCounterWords * launches,
                     result);
       ++launches;
       b = e;

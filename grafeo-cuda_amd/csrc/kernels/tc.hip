@@ -283,102 +283,183 @@ extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
 #define TCB_WAVES 4
 #define TCB_HOT_BITS 32768
 #define TCB_HOT_WORDS (TCB_HOT_BITS / 32)
+#define TCB_COUNTER_STRIDE 64  // words between two parts' counters (own 256-byte line)
 
+// 1 when w is a hot id whose bit is set in the row's bitmap. Branch-free: a
+// cold id (w - hot_base >= TCB_HOT_BITS, wrapped) reads the bitmap's extra
+// word TCB_HOT_WORDS, which stays zero, so the four lookups of a step issue
+// back to back.
+__device__ __forceinline__ uint32_t hot_bit(const uint32_t* s_bits, uint32_t w, uint32_t hot_base) {
+  const uint32_t x = w - hot_base;
+  const uint32_t word = s_bits[min(x >> 5, static_cast<uint32_t>(TCB_HOT_WORDS))];
+  return (word >> (x & 31)) & 1u;
+}
+
+// 1 when w is a cold id (below hot_base) found in the sorted cold prefix
+// A[0, c) of the row.
+__device__ __forceinline__ uint32_t cold_hit(const uint32_t* A, uint32_t c, uint32_t w, uint32_t hot_base) {
+  if (w >= hot_base) return 0u;
+  const uint32_t p = lower_bound_u32(A, 0, c, w);
+  return (p < c && A[p] == w) ? 1u : 0u;
+}
+
+// Hits of the row's lists among the entries L[0, lj) (lj wave-uniform), in
+// whole steps of 4 x 64 entries (four loads in flight per lane) and then a
+// 64-entry tail. (Loading the lists of low ids with the nontemporal hint, to
+// keep the lists of high ids cached, measured slower: 18.5 -> 22.8 ms at R-MAT
+// scale 22.)
+__device__ __forceinline__ uint32_t probe_list(const uint32_t* L, uint32_t lj, const uint32_t* s_bits,
+                                               uint32_t hot_base, const uint32_t* A, uint32_t c) {
+  const uint32_t lane = lane_id();
+  uint32_t local = 0;
+  uint32_t kk = 0;
+  for (; kk + 4 * GC_WAVE <= lj; kk += 4 * GC_WAVE) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      w[q] = L[kk + q * GC_WAVE + lane];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) local += hot_bit(s_bits, w[q], hot_base);
+    if (c != 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) local += cold_hit(A, c, w[q], hot_base);
+    }
+  }
+  for (; kk < lj; kk += GC_WAVE) {
+    if (kk + lane < lj) {
+      const uint32_t w = L[kk + lane];
+      local += hot_bit(s_bits, w, hot_base);
+      if (c != 0) local += cold_hit(A, c, w, hot_base);
+    }
+  }
+  return local;
+}
+
+// Partitions by middle vertex (`parts` > 1): a row u with |L(u)| >= split_min
+// is processed once per part p, and part p walks only the lists L(v) of the
+// v in L(u) with v % parts == p; the parts' sums add up to the same count.
+// Workgroup b serves part b % parts first, so with parts = 8 (the XCD count)
+// and the round-robin placement of workgroups over XCDs, each XCD streams one
+// eighth of the lists through its own L2 instead of all of them: the lists of
+// the top ids, which carry most of the streaming, then fit the L2s together
+// (at R-MAT scale 22 the L2 hit rate goes from 19% to 55% and the kernel from
+// 35 to 18 ms). A shorter row is processed whole, once, by part u % parts:
+// building its bitmap once per part costs more than its streaming saves (at
+// scale 22 the rows with |L(u)| < 64 are 2.1 M of the 2.4 M non-empty rows
+// and 22% of the streaming). Every part has its own pair of work counters
+// over all rows; a wave that finds its part drained takes rows of the others,
+// so an uneven part does not leave XCDs idle (where a row runs changes its
+// speed, never its count). Part p's counters are
+// counters[TCB_COUNTER_STRIDE * p + 0, 1], each on a 256-byte line of its own
+// (sharing one line, the parts' atomics serialized: 25 ms instead of 21).
 extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
     const uint32_t* __restrict__ targets, uint32_t n, uint32_t row_lo, uint32_t hot_base,
-    uint32_t heavy,
-    uint32_t light_chunk, uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+    uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
   const uint32_t positions = n - row_lo;  // rows [row_lo, n)
-  __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_HOT_WORDS];
+  // per wave: the row's hot bitmap and one word that stays zero (for cold ids)
+  __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_HOT_WORDS + 1];
   const uint32_t wave = threadIdx.x / GC_WAVE;
   const uint32_t lane = lane_id();
   uint32_t* s_bits = s_bits_all[wave];
-  for (uint32_t j = lane; j < TCB_HOT_WORDS; j += GC_WAVE) s_bits[j] = 0;
+  for (uint32_t j = lane; j <= TCB_HOT_WORDS; j += GC_WAVE) s_bits[j] = 0;
   wave_lds_sync();
   unsigned long long count = 0;
-  bool heavy_left = heavy > 0;
+  const uint32_t home = blockIdx.x % parts;
 
-  for (;;) {
-    uint32_t kb = positions, ke = positions;
-    if (lane == 0) {
-      if (heavy_left) {
-        const uint32_t k = atomicAdd(&counters[0], 1u);
-        if (k < heavy) {
-          kb = k;
-          ke = k + 1;
-        } else {
-          heavy_left = false;
+  for (uint32_t q = 0; q < parts; ++q) {
+    const uint32_t part = (home + q) % parts;
+    uint32_t* part_counters = counters + TCB_COUNTER_STRIDE * part;
+    bool heavy_left = heavy > 0;
+    for (;;) {
+      uint32_t kb = positions, ke = positions;
+      if (lane == 0) {
+        if (heavy_left) {
+          const uint32_t k = atomicAdd(&part_counters[0], 1u);
+          if (k < heavy) {
+            kb = k;
+            ke = k + 1;
+          } else {
+            heavy_left = false;
+          }
         }
-      }
-      if (!heavy_left && kb == positions) {
-        const uint32_t c = atomicAdd(&counters[1], light_chunk);
-        const uint64_t b = static_cast<uint64_t>(heavy) + c;
-        if (b < positions) {
-          kb = static_cast<uint32_t>(b);
-          const uint64_t e = b + light_chunk;
-          ke = e < positions ? static_cast<uint32_t>(e) : positions;
-        }
-      }
-    }
-    kb = __shfl(kb, 0);
-    ke = __shfl(ke, 0);
-    if (kb >= positions) break;
-
-    for (uint32_t k = kb; k < ke; ++k) {
-      const uint32_t u = n - 1 - k;
-      const uint32_t a_lo = lo[u];
-      const uint32_t a = hi[u] - a_lo;
-      if (a == 0) continue;
-      const uint32_t* A = targets + a_lo;
-      // cold prefix [0, c): entries below hot_base
-      const uint32_t c = lower_bound_u32(A, 0, a, hot_base);
-      for (uint32_t j = c + lane; j < a; j += GC_WAVE) {
-        const uint32_t x = A[j] - hot_base;
-        atomicOr(&s_bits[x >> 5], 1u << (x & 31));
-      }
-      wave_lds_sync();
-      uint32_t local = 0;
-      for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
-        const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
-        uint32_t st = 0, len = 0;
-        if (lane < nv) {
-          const uint32_t v = A[cv + lane];
-          st = lo[v];
-          len = hi[v] - st;
-        }
-        for (uint32_t jv = 0; jv < nv; ++jv) {
-          const uint32_t sj = __shfl(st, static_cast<int>(jv));
-          const uint32_t lj = __shfl(len, static_cast<int>(jv));
-          for (uint32_t kk = lane; kk < lj; kk += 2 * GC_WAVE) {
-            const uint32_t k2 = kk + GC_WAVE;
-            const uint32_t w0 = targets[sj + kk];
-            const uint32_t w1 = k2 < lj ? targets[sj + k2] : 0u;
-            uint32_t hit0, hit1 = 0;
-            if (w0 >= hot_base) {
-              const uint32_t x = w0 - hot_base;
-              hit0 = (s_bits[x >> 5] >> (x & 31)) & 1u;
-            } else {
-              const uint32_t p = lower_bound_u32(A, 0, c, w0);
-              hit0 = (p < c && A[p] == w0) ? 1u : 0u;
-            }
-            if (k2 < lj) {
-              if (w1 >= hot_base) {
-                const uint32_t x = w1 - hot_base;
-                hit1 = (s_bits[x >> 5] >> (x & 31)) & 1u;
-              } else {
-                const uint32_t p = lower_bound_u32(A, 0, c, w1);
-                hit1 = (p < c && A[p] == w1) ? 1u : 0u;
-              }
-            }
-            local += hit0 + hit1;
+        if (!heavy_left && kb == positions) {
+          const uint32_t c = atomicAdd(&part_counters[1], light_chunk);
+          const uint64_t b = static_cast<uint64_t>(heavy) + c;
+          if (b < positions) {
+            kb = static_cast<uint32_t>(b);
+            const uint64_t e = b + light_chunk;
+            ke = e < positions ? static_cast<uint32_t>(e) : positions;
           }
         }
       }
-      count += local;
-      wave_lds_sync();
-      for (uint32_t j = c + lane; j < a; j += GC_WAVE) s_bits[(A[j] - hot_base) >> 5] = 0;
-      wave_lds_sync();
+      kb = __shfl(kb, 0);
+      ke = __shfl(ke, 0);
+      if (kb >= positions) break;
+
+      // the task's rows, 64 at a time: lane j reads row kb + j's list bounds
+      for (uint32_t kc = kb; kc < ke; kc += GC_WAVE) {
+        uint32_t row_lo_e = 0, row_a = 0;
+        bool take = false;
+        if (kc + lane < ke) {
+          const uint32_t u = n - 1 - (kc + lane);
+          row_lo_e = lo[u];
+          row_a = hi[u] - row_lo_e;
+          take = row_a != 0 && (row_a >= split_min || u % parts == part);
+        }
+        unsigned long long rows = __ballot(take);
+        while (rows != 0) {
+          const int jr = __ffsll(static_cast<long long>(rows)) - 1;
+          rows &= rows - 1;
+          const uint32_t a = __builtin_amdgcn_readfirstlane(__shfl(row_a, jr));
+          const uint32_t* A = targets + __builtin_amdgcn_readfirstlane(__shfl(row_lo_e, jr));
+          const bool whole = a < split_min;  // processed by one part, every v
+          // cold prefix [0, c): entries below hot_base; set the hot suffix's bits
+          uint32_t c;
+          if (a <= GC_WAVE) {
+            const uint32_t x = lane < a ? A[lane] : 0xFFFFFFFFu;
+            c = static_cast<uint32_t>(__popcll(__ballot(x < hot_base)));  // uniform
+            if (lane >= c && lane < a) {
+              const uint32_t y = x - hot_base;
+              atomicOr(&s_bits[y >> 5], 1u << (y & 31));
+            }
+          } else {
+            c = __builtin_amdgcn_readfirstlane(lower_bound_u32(A, 0, a, hot_base));
+            for (uint32_t j = c + lane; j < a; j += GC_WAVE) {
+              const uint32_t y = A[j] - hot_base;
+              atomicOr(&s_bits[y >> 5], 1u << (y & 31));
+            }
+          }
+          wave_lds_sync();
+          uint32_t local = 0;
+          for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
+            const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
+            uint32_t st = 0, len = 0;
+            if (lane < nv) {
+              const uint32_t v = A[cv + lane];
+              if (whole || v % parts == part) {
+                st = lo[v];
+                len = hi[v] - st;
+              }
+            }
+            // the lanes holding a non-empty list to walk
+            unsigned long long pending = __ballot(len != 0);
+            while (pending != 0) {
+              const int jv = __ffsll(static_cast<long long>(pending)) - 1;
+              pending &= pending - 1;
+              const uint32_t* L = targets + __builtin_amdgcn_readfirstlane(__shfl(st, jv));
+              const uint32_t lj = __builtin_amdgcn_readfirstlane(__shfl(len, jv));
+              local += probe_list(L, lj, s_bits, hot_base, A, c);
+            }
+          }
+          count += local;
+          wave_lds_sync();
+          for (uint32_t j = c + lane; j < a; j += GC_WAVE) s_bits[(A[j] - hot_base) >> 5] = 0;
+          wave_lds_sync();
+        }
+      }
     }
   }
   count = wave_sum(count);
