@@ -184,6 +184,18 @@ struct Traversal {
     in_bitmap = false;
   }
 
+  // The transposed four last entries of every row (built once per graph).
+  const uint32_t* top_entries() {
+    if (!g.bfs_top_ready) {
+      g.bfs_top.allocate(sizeof(uint32_t) * 4 * std::max<uint64_t>(n, 1));
+      Grid1D vg = thread_per_item(n);
+      launch(dev, get_kernel(dev, "bfs", "bfs_top_build"), vg.blocks, vg.threads, 0, off, tgt, n,
+             g.bfs_top.as<uint32_t>());
+      g.bfs_top_ready = true;
+    }
+    return g.bfs_top.as<const uint32_t>();
+  }
+
   void to_bitmap() {
     if (in_bitmap) return;
     fill_words(dev, bitmap[cur], 2 * words64, 0);
@@ -211,7 +223,7 @@ struct Traversal {
     } else {
       to_bitmap();
       Grid1D bg = wave_per_item(words64);
-      launch(dev, k_bu, std::min(bg.blocks, cu * 32), bg.threads, 0, off, tgt, dist,
+      launch(dev, k_bu, std::min(bg.blocks, cu * 8), bg.threads, 0, off, tgt, top_entries(), dist,
              reinterpret_cast<const uint32_t*>(bitmap[cur]), bitmap[cur ^ 1], n, &rb->counters[0],
              next_level);
     }

@@ -303,6 +303,8 @@ struct Graph {
   bool fwd_original_ready = false;
   DeviceMemory ktruss_rev;    // u32[m] reverse entry of each forward relabelled entry
   bool ktruss_rev_ready = false;
+  DeviceMemory bfs_top;       // u32[4][n] the four highest neighbours of each relabelled row
+  bool bfs_top_ready = false;
   std::vector<std::pair<std::string, std::unique_ptr<DeviceMemory>>> buffers;
 
   // Named scratch buffer of at least `bytes` bytes (contents unspecified).
