@@ -102,9 +102,9 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
   } else {
     // hot range [hot_base, n): at most kHotBits ids, hot_base a multiple of 32
     const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
-    launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid, 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
-           row_lo, hot_base, tc_heavy_rows(g, row_lo, row_hi), light_chunk, tc_parts(), tc_split_min(),
-           counters, result);
+    launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid, 4 * kWaveSize, 0, fwd, end, tgt, row_hi, row_lo,
+           hot_base, tc_heavy_rows(g, row_lo, row_hi), light_chunk, tc_parts(), tc_split_min(), counters,
+           result);
   }
 }
 

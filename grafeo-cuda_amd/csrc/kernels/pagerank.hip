@@ -181,7 +181,7 @@ __device__ __forceinline__ void pr_row_block(const PrSegment& seg, uint32_t loca
 // highest-degree segment first, so no single long row sits at the tail.
 #define PR_HUB_CHUNK 4096
 
-extern "C" __global__ __launch_bounds__(PR_THREADS) void pr_spmv(
+extern "C" __global__ __launch_bounds__(PR_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void pr_spmv(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
     float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
@@ -302,15 +302,18 @@ extern "C" __global__ __launch_bounds__(PR_THREADS) void pr_hub_finish(
   }
 }
 
-// ---- persistent variant: hot contributions in LDS ---------------------------
+// ---- persistent variant (the default): hot contributions in LDS -------------
 //
 // The rank mass of a power-law graph is concentrated: on the degree-ordered
-// ids, the top 32K vertices receive about half of all gathers (R-MAT scale 22).
+// ids, the top 19K vertices receive 42% of all gathers (R-MAT scale 22).
 // Each 1024-lane workgroup copies c[n-hot .. n) into LDS once per iteration
-// and then takes row tiles from a shared work counter, heaviest tiles first
+// and then takes row tiles round-robin over the workgroups, heaviest first
 // (a tile is 1024/G rows of one segment, or one row of degree >= 1024 for the
 // whole workgroup). Gathers of hot vertices become LDS reads, which removes
-// them from the L1/L2 request stream that bounds the pass.
+// them from the L1 request stream (one tag lookup per distinct line, the
+// texture path 60-70% busy in pr_spmv) that bounds the pass. Two workgroups
+// per CU keep 32 waves resident: with one workgroup and twice the hot set the
+// pass is latency-bound and no faster than pr_spmv.
 
 #define PRH_THREADS 1024
 
@@ -375,7 +378,6 @@ extern "C" __global__ __launch_bounds__(PRH_THREADS) void pr_spmv_hot(
   extern __shared__ float s_hot[];
   __shared__ double s_red[PRH_THREADS / GC_WAVE];
   __shared__ double s_part[2][PRH_THREADS / GC_WAVE];
-  __shared__ uint32_t s_task;
   if (state->done) return;
   const uint32_t hot_base = n - hot_count;
   for (uint32_t i = threadIdx.x; i < hot_count; i += PRH_THREADS) s_hot[i] = contrib_in[hot_base + i];
@@ -383,13 +385,9 @@ extern "C" __global__ __launch_bounds__(PRH_THREADS) void pr_spmv_hot(
   const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
   double dangling = 0.0, diff = 0.0;
   __syncthreads();
-  for (;;) {
-    if (threadIdx.x == 0) s_task = atomicAdd(work, 1u);
-    __syncthreads();
-    const uint32_t k = s_task;
-    __syncthreads();
-    if (k >= tiles.total_blocks) break;
-    const uint32_t t = tiles.total_blocks - 1 - k;  // heaviest tiles first
+  // tiles in round-robin order over the workgroups, heaviest first
+  for (uint32_t k = blockIdx.x; k < tiles.total_blocks; k += gridDim.x) {
+    const uint32_t t = tiles.total_blocks - 1 - k;
     uint32_t s = 0;
     while (s + 1 < tiles.count && t >= tiles.seg[s + 1].block_begin) ++s;
     const PrSegment seg = tiles.seg[s];

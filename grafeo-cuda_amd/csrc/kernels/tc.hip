@@ -285,14 +285,19 @@ extern "C" __global__ __launch_bounds__(TCH_WAVES * GC_WAVE) void tc_hash(
 #define TCB_HOT_WORDS (TCB_HOT_BITS / 32)
 #define TCB_COUNTER_STRIDE 64  // words between two parts' counters (own 256-byte line)
 
+// hot_base is a multiple of 32, so w - hot_base and w share their low five
+// bits: the bit is extracted with w itself (v_bfe_u32 reads only the offset's
+// low five bits).
+__device__ __forceinline__ uint32_t word_bit(uint32_t word, uint32_t w) {
+  return __builtin_amdgcn_ubfe(word, w, 1u);
+}
+
 // 1 when w is a hot id whose bit is set in the row's bitmap. Branch-free: a
 // cold id (w - hot_base >= TCB_HOT_BITS, wrapped) reads the bitmap's extra
 // word TCB_HOT_WORDS, which stays zero, so the four lookups of a step issue
 // back to back.
 __device__ __forceinline__ uint32_t hot_bit(const uint32_t* s_bits, uint32_t w, uint32_t hot_base) {
-  const uint32_t x = w - hot_base;
-  const uint32_t word = s_bits[min(x >> 5, static_cast<uint32_t>(TCB_HOT_WORDS))];
-  return (word >> (x & 31)) & 1u;
+  return word_bit(s_bits[min((w - hot_base) >> 5, static_cast<uint32_t>(TCB_HOT_WORDS))], w);
 }
 
 // 1 when w is a cold id (below hot_base) found in the sorted cold prefix
@@ -303,11 +308,22 @@ __device__ __forceinline__ uint32_t cold_hit(const uint32_t* A, uint32_t c, uint
   return (p < c && A[p] == w) ? 1u : 0u;
 }
 
-// Hits of the row's lists among the entries L[0, lj) (lj wave-uniform), in
-// whole steps of 4 x 64 entries (four loads in flight per lane) and then a
-// 64-entry tail. (Loading the lists of low ids with the nontemporal hint, to
-// keep the lists of high ids cached, measured slower: 18.5 -> 22.8 ms at R-MAT
-// scale 22.)
+// Hit of one entry w of L(v): the clamped bitmap lookup plus, when the row
+// has a cold prefix (c != 0, wave-uniform), its search. (A variant that skips
+// the clamp and the cold test for the lists of hot v, whose entries are all
+// hot, measured no faster.)
+__device__ __forceinline__ uint32_t entry_hit(const uint32_t* s_bits, uint32_t w, uint32_t hot_base,
+                                              const uint32_t* A, uint32_t c) {
+  uint32_t hit = hot_bit(s_bits, w, hot_base);
+  if (c != 0) hit += cold_hit(A, c, w, hot_base);
+  return hit;
+}
+
+// Hits of the row's lists among the entries L[0, lj) (lj wave-uniform): whole
+// steps of 4 x 64 entries (four loads in flight per lane), then one guarded
+// step for the rest (a single load per lane when at most 64 entries are left).
+// (Loading the lists of low ids with the nontemporal hint, to keep the lists
+// of high ids cached, measured slower: 18.5 -> 22.8 ms at R-MAT scale 22.)
 __device__ __forceinline__ uint32_t probe_list(const uint32_t* L, uint32_t lj, const uint32_t* s_bits,
                                                uint32_t hot_base, const uint32_t* A, uint32_t c) {
   const uint32_t lane = lane_id();
@@ -316,21 +332,25 @@ __device__ __forceinline__ uint32_t probe_list(const uint32_t* L, uint32_t lj, c
   for (; kk + 4 * GC_WAVE <= lj; kk += 4 * GC_WAVE) {
     uint32_t w[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      w[q] = L[kk + q * GC_WAVE + lane];
-    }
+    for (int q = 0; q < 4; ++q) w[q] = L[kk + q * GC_WAVE + lane];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) local += hot_bit(s_bits, w[q], hot_base);
-    if (c != 0) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) local += cold_hit(A, c, w[q], hot_base);
-    }
+    for (int q = 0; q < 4; ++q) local += entry_hit(s_bits, w[q], hot_base, A, c);
   }
-  for (; kk < lj; kk += GC_WAVE) {
-    if (kk + lane < lj) {
-      const uint32_t w = L[kk + lane];
-      local += hot_bit(s_bits, w, hot_base);
-      if (c != 0) local += cold_hit(A, c, w, hot_base);
+  if (kk < lj) {
+    // indexes clamped to the list's last entry (always a valid load), hits
+    // masked to the entries left: no branches between the loads and lookups
+    const uint32_t left = lj - kk;  // 1 .. 255, wave-uniform
+    const uint32_t* T = L + kk;
+    if (left <= GC_WAVE) {
+      const uint32_t w = T[min(lane, left - 1)];
+      local += lane < left ? entry_hit(s_bits, w, hot_base, A, c) : 0u;
+    } else {
+      uint32_t w[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[q] = T[min(q * GC_WAVE + lane, left - 1)];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        local += q * GC_WAVE + lane < left ? entry_hit(s_bits, w[q], hot_base, A, c) : 0u;
     }
   }
   return local;
@@ -353,9 +373,10 @@ __device__ __forceinline__ uint32_t probe_list(const uint32_t* L, uint32_t lj, c
 // speed, never its count). Part p's counters are
 // counters[TCB_COUNTER_STRIDE * p + 0, 1], each on a 256-byte line of its own
 // (sharing one line, the parts' atomics serialized: 25 ms instead of 21).
+// (Packing lo and hi into one uint2 array, one 8-byte load per list instead
+// of two 4-byte ones, measured slower: 14.7 -> 15.4 ms at R-MAT scale 22.)
 extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
-    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
-    const uint32_t* __restrict__ targets, uint32_t n, uint32_t row_lo, uint32_t hot_base,
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets, uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
   const uint32_t positions = n - row_lo;  // rows [row_lo, n)
@@ -413,8 +434,8 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
         while (rows != 0) {
           const int jr = __ffsll(static_cast<long long>(rows)) - 1;
           rows &= rows - 1;
-          const uint32_t a = __builtin_amdgcn_readfirstlane(__shfl(row_a, jr));
-          const uint32_t* A = targets + __builtin_amdgcn_readfirstlane(__shfl(row_lo_e, jr));
+          const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(row_a, jr));
+          const uint32_t* A = targets + static_cast<uint32_t>(__builtin_amdgcn_readlane(row_lo_e, jr));
           const bool whole = a < split_min;  // processed by one part, every v
           // cold prefix [0, c): entries below hot_base; set the hot suffix's bits
           uint32_t c;
@@ -449,8 +470,9 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
             while (pending != 0) {
               const int jv = __ffsll(static_cast<long long>(pending)) - 1;
               pending &= pending - 1;
-              const uint32_t* L = targets + __builtin_amdgcn_readfirstlane(__shfl(st, jv));
-              const uint32_t lj = __builtin_amdgcn_readfirstlane(__shfl(len, jv));
+              // jv is wave-uniform: read the lane's values into scalars
+              const uint32_t* L = targets + static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+              const uint32_t lj = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
               local += probe_list(L, lj, s_bits, hot_base, A, c);
             }
           }
