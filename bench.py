@@ -197,10 +197,19 @@ def main() -> int:
                            # the measured bytes over the same launch time, against the same peak
                            "traffic_frac": (t / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if t else None}
 
-    if "pr_spmv" in kernels:
-        add_roofline("pr_spmv", spmv_bytes, kernels["pr_spmv"]["avg_us"])
-    if "pr_spmv_hot" in kernels:
-        add_roofline("pr_spmv_hot", spmv_bytes, kernels["pr_spmv_hot"]["avg_us"])
+    # PageRank launches in batches and the launches after convergence return at once: its
+    # SpMV time per iteration is the kernel's time per step over the iterations that ran.
+    for name in ("pr_spmv", "pr_spmv_hot"):
+        if name in kernels:
+            add_roofline(name, spmv_bytes, 1000.0 * kernels[name]["total_ms"] / max(1, iters))
+            rooflines[name]["note"] = ("avg_launch_us and traffic are per iteration that ran (the "
+                                       "launches after convergence return at once)")
+            if rooflines[name]["traffic"] is not None:
+                # traffic.json averages over all launches of one step: per iteration that ran
+                per_iter = rooflines[name]["traffic"] * kernels[name]["launches"] / max(1, iters)
+                rooflines[name]["traffic"] = per_iter
+                rooflines[name]["traffic_frac"] = (per_iter / (rooflines[name]["avg_launch_us"] * 1e-6)
+                                                   / 1e9 / HBM_PEAK_GBS)
     for name in ("tc_bitmap", "tc_hash", "tc_count"):
         if name in kernels:
             add_roofline(name, tc_bytes, kernels[name]["avg_us"])
