@@ -20,6 +20,7 @@ constexpr uint32_t kBitmap = 0, kHash = 1, kLists = 2;  // the scheduler's choic
 constexpr uint32_t kHotBits = 32768;                    // TCB_HOT_BITS of tc.hip
 constexpr uint32_t kMaxLaunches = 64;                   // launches per count
 constexpr uint32_t kMaxParts = 8;                       // middle-vertex parts of tc_bitmap (one per XCD)
+constexpr uint32_t kDefaultParts = 4;
 constexpr uint32_t kCounterStride = 64;                 // TCB_COUNTER_STRIDE of tc.hip
 constexpr uint32_t kCounterWords = kCounterStride * kMaxParts;  // work counters of one launch
 
@@ -30,12 +31,14 @@ uint64_t read_result(Device& dev, const unsigned long long* result) {
   return total;
 }
 
-// Middle-vertex parts of tc_bitmap: GC_TC_PARTS (1..8), default 8 (one per
-// XCD). Measured on MI355X: R-MAT scale 22 40.3 ms at 1 part, 21.1 ms at 8;
-// scale 14 0.50 -> 0.45 ms; a 4096-vertex Erdos-Renyi graph 0.26 -> 0.14 ms.
+// Middle-vertex parts of tc_bitmap: GC_TC_PARTS (1..8), default 4 (each
+// pair of XCDs shares a part). Measured on MI355X, R-MAT scale 22: 40.3 ms
+// at 1 part, 21.1 at 8 (first version of the kernel); with the current
+// kernel 4 parts beat 8 by 2-4% at scales 20 and 22 (half the per-part
+// bitmap builds) and tie at scale 24 (145.6 against 146.0 ms).
 uint32_t tc_parts() {
   const char* env = std::getenv("GC_TC_PARTS");
-  if (env == nullptr) return kMaxParts;
+  if (env == nullptr) return kDefaultParts;
   const long p = std::strtol(env, nullptr, 10);
   return static_cast<uint32_t>(std::min<long>(std::max<long>(p, 1), kMaxParts));
 }

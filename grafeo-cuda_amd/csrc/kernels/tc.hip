@@ -322,8 +322,10 @@ __device__ __forceinline__ uint32_t entry_hit(const uint32_t* s_bits, uint32_t w
 // Hits of the row's lists among the entries L[0, lj) (lj wave-uniform): whole
 // steps of 4 x 64 entries (four loads in flight per lane), then one guarded
 // step for the rest (a single load per lane when at most 64 entries are left).
-// (Loading the lists of low ids with the nontemporal hint, to keep the lists
-// of high ids cached, measured slower: 18.5 -> 22.8 ms at R-MAT scale 22.)
+// Measured slower at R-MAT scale 22: nontemporal loads for the lists of low
+// ids (to keep the lists of high ids cached), fewer loads in the last step
+// (one whole step of 2 x 64, then at most two guarded loads), and steps of
+// 8 x 64 or 16 x 64 entries.
 __device__ __forceinline__ uint32_t probe_list(const uint32_t* L, uint32_t lj, const uint32_t* s_bits,
                                                uint32_t hot_base, const uint32_t* A, uint32_t c) {
   const uint32_t lane = lane_id();
@@ -356,15 +358,57 @@ __device__ __forceinline__ uint32_t probe_list(const uint32_t* L, uint32_t lj, c
   return local;
 }
 
+// Two lists walked in lockstep (eight loads in flight per lane): whole steps
+// of 4 x 64 entries of each while both have them, then guarded steps (index
+// clamped to the list's last entry, hit masked) until the longer one ends.
+// Consecutive pending lists belong to neighbouring ids, which the degree
+// order gives similar lengths, so little of the guarded work is wasted. At
+// R-MAT scale 22 pairs took the kernel from 17.7 to 15.8 ms (same box);
+// groups of three or four lists measured slower than pairs.
+__device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, const uint32_t* L2, uint32_t l2,
+                                               const uint32_t* s_bits, uint32_t hot_base, const uint32_t* A,
+                                               uint32_t c) {
+  const uint32_t lane = lane_id();
+  uint32_t local = 0;
+  uint32_t kk = 0;
+  const uint32_t lmin = min(l1, l2), lmax = max(l1, l2);
+  for (; kk + 4 * GC_WAVE <= lmin; kk += 4 * GC_WAVE) {
+    uint32_t w[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      w[q] = L1[kk + q * GC_WAVE + lane];
+      w[4 + q] = L2[kk + q * GC_WAVE + lane];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) local += entry_hit(s_bits, w[q], hot_base, A, c);
+  }
+  for (; kk < lmax; kk += 4 * GC_WAVE) {
+    uint32_t w[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t i = kk + q * GC_WAVE + lane;
+      w[q] = L1[min(i, l1 - 1)];
+      w[4 + q] = L2[min(i, l2 - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t i = kk + q * GC_WAVE + lane;
+      local += i < l1 ? entry_hit(s_bits, w[q], hot_base, A, c) : 0u;
+      local += i < l2 ? entry_hit(s_bits, w[4 + q], hot_base, A, c) : 0u;
+    }
+  }
+  return local;
+}
+
 // Partitions by middle vertex (`parts` > 1): a row u with |L(u)| >= split_min
 // is processed once per part p, and part p walks only the lists L(v) of the
 // v in L(u) with v % parts == p; the parts' sums add up to the same count.
-// Workgroup b serves part b % parts first, so with parts = 8 (the XCD count)
-// and the round-robin placement of workgroups over XCDs, each XCD streams one
-// eighth of the lists through its own L2 instead of all of them: the lists of
-// the top ids, which carry most of the streaming, then fit the L2s together
-// (at R-MAT scale 22 the L2 hit rate goes from 19% to 55% and the kernel from
-// 35 to 18 ms). A shorter row is processed whole, once, by part u % parts:
+// Workgroup b serves part b % parts first, so with parts dividing 8 (the XCD
+// count) and the round-robin placement of workgroups over XCDs, each XCD
+// streams parts-th of the lists through its own L2 instead of all of them:
+// the lists of the top ids, which carry most of the streaming, then fit the
+// L2s together (at R-MAT scale 22 with 8 parts the L2 hit rate goes from 20%
+// to 52% and the kernel from 35 to 18 ms). A shorter row is processed whole, once, by part u % parts:
 // building its bitmap once per part costs more than its streaming saves (at
 // scale 22 the rows with |L(u)| < 64 are 2.1 M of the 2.4 M non-empty rows
 // and 22% of the streaming). Every part has its own pair of work counters
@@ -376,7 +420,8 @@ __device__ __forceinline__ uint32_t probe_list(const uint32_t* L, uint32_t lj, c
 // (Packing lo and hi into one uint2 array, one 8-byte load per list instead
 // of two 4-byte ones, measured slower: 14.7 -> 15.4 ms at R-MAT scale 22.)
 extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
-    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets, uint32_t n, uint32_t row_lo, uint32_t hot_base,
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
+    uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
   const uint32_t positions = n - row_lo;  // rows [row_lo, n)
@@ -473,7 +518,15 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
               // jv is wave-uniform: read the lane's values into scalars
               const uint32_t* L = targets + static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
               const uint32_t lj = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
-              local += probe_list(L, lj, s_bits, hot_base, A, c);
+              if (pending != 0) {  // a second list: walk the two in lockstep
+                const int jv2 = __ffsll(static_cast<long long>(pending)) - 1;
+                pending &= pending - 1;
+                const uint32_t* L2 = targets + static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv2));
+                const uint32_t l2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv2));
+                local += probe_pair(L, lj, L2, l2, s_bits, hot_base, A, c);
+              } else {
+                local += probe_list(L, lj, s_bits, hot_base, A, c);
+              }
             }
           }
           count += local;

@@ -47,6 +47,32 @@ def check_triangles(name, device, off, tgt, g):
     assert og.triangle_count_oriented()["total_triangles"] == oracle.triangle_count_oriented(o_off, o_tgt)
 
 
+def check_tc_partitions(name, g, expected, monkeypatch):
+    """The bitmap kernel splits the middle vertices into parts (one per XCD or pair of XCDs) and
+    runs short rows whole: every split must give the same count."""
+    for parts in ("1", "2", "3", "8"):
+        for split in ("0", "64", "4294967295"):
+            monkeypatch.setenv("GC_TC_PARTS", parts)
+            monkeypatch.setenv("GC_TC_SPLIT", split)
+            assert g.triangle_count()["total_triangles"] == expected, f"{name}: parts={parts} split={split}"
+    monkeypatch.delenv("GC_TC_PARTS")
+    monkeypatch.delenv("GC_TC_SPLIT")
+
+
+def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
+    """Both SpMV kernels (hot set in LDS, the default, and one workgroup per row tile), and a
+    hot set smaller than the graph, against the oracle."""
+    for kernel, hot in (("rows", None), ("hot", "1024"), ("hot", None)):
+        monkeypatch.setenv("GC_PR_KERNEL", kernel)
+        if hot is None:
+            monkeypatch.delenv("GC_PR_HOT", raising=False)
+        else:
+            monkeypatch.setenv("GC_PR_HOT", hot)
+        check_pagerank(f"{name} ({kernel}, hot={hot})", off, tgt, g)
+    monkeypatch.delenv("GC_PR_KERNEL")
+    monkeypatch.delenv("GC_PR_HOT", raising=False)
+
+
 def check_bfs(name, off, tgt, g, source):
     dist, reach, depth = oracle.bfs(off, tgt, source)
     r = g.bfs_numpy(source)
@@ -159,6 +185,8 @@ def test_er_4096(device, monkeypatch):
     off, tgt = er_graph(4096, 16, seed=1)
     g = upload(device, off, tgt)
     check_triangles("ER-4096", device, off, tgt, g)
+    check_tc_partitions("ER-4096", g, oracle.triangle_count(off, tgt), monkeypatch)
+    check_pagerank_kernels("ER-4096", off, tgt, g, monkeypatch)
     for source in (0, 17, 4095):
         check_bfs("ER-4096", off, tgt, g, source)
     check_pagerank("ER-4096", off, tgt, g)
@@ -174,6 +202,8 @@ def test_rmat_small(device, monkeypatch, scale):
     off, tgt = rmat_graph(scale, 16, seed=1)
     g = upload(device, off, tgt)
     check_triangles(f"RMAT-{scale}", device, off, tgt, g)
+    check_tc_partitions(f"RMAT-{scale}", g, oracle.triangle_count(off, tgt, fast=True), monkeypatch)
+    check_pagerank_kernels(f"RMAT-{scale}", off, tgt, g, monkeypatch)
     n = len(off) - 1
     for source in np.random.default_rng(scale).integers(0, n, size=4):
         check_bfs(f"RMAT-{scale}", off, tgt, g, int(source))
