@@ -56,8 +56,10 @@ uint32_t exact_rows_per_thread(uint32_t d) { return d <= 3 ? 4 : (d <= 7 ? 2 : 1
 // segment's rows are handed out in tiles of block_threads/group rows (one row
 // per tile for the last segment, whose group is the whole workgroup). With
 // `exact`, rows of degree below kPrExactMax form one segment per degree.
+// With `shift`, the degree thresholds are scaled by 2^shift (for passes that
+// see a 2^-shift share of every row).
 PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t block_threads,
-                         bool exact) {
+                         bool exact, uint32_t shift = 0) {
   // Degree thresholds and the group size used below each.
   static const uint32_t kLimit[] = {2, 4, 8, 16, 32, 64, 1024, 0xFFFFFFFFu};
   const uint32_t kGroup[] = {1, 2, 4, 8, 16, 32, 64, block_threads};
@@ -86,11 +88,12 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
     }
   }
   for (int t = 0; t < 8 && row < n; ++t) {
-    // first row at or after `row` whose degree reaches kLimit[t]
+    // first row at or after `row` whose degree reaches kLimit[t] (scaled)
+    const uint64_t limit = static_cast<uint64_t>(kLimit[t]) << shift;
     uint32_t lo = row, hi = n;
     while (lo < hi) {
       uint32_t mid = lo + (hi - lo) / 2;
-      if (degree(mid) >= kLimit[t]) hi = mid;
+      if (degree(mid) >= limit) hi = mid;
       else lo = mid + 1;
     }
     if (t == 7) lo = n;
@@ -116,7 +119,9 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
 // Which SpMV runs: the persistent kernel with the hot contributions in LDS
 // (default, pr_spmv_hot) or one workgroup per row tile (GC_PR_KERNEL=rows,
 // pr_spmv). Measured on MI355X, 17 iterations at R-MAT scale 22: 11.3 ms
-// against 14.0 ms; scale 20: 3.0 against 3.8 ms.
+// against 14.0 ms; scale 20: 3.0 against 3.8 ms. (A source-partitioned pass,
+// one part of the sources per XCD with eight FP32 partial sums per row,
+// halved the L2 requests but was no faster: 12.1 against 10.2-10.5 ms.)
 static bool use_row_kernel() {
   const char* env = std::getenv("GC_PR_KERNEL");
   return env != nullptr && std::strcmp(env, "rows") == 0;
@@ -141,7 +146,15 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
     }
     hub_begin = lo;
   }
-  const PrSegments segs = plan_segments(hoff, hub_begin, block_threads, rows_kernel);
+  // The hot kernel gives a row a quarter of the lanes of the row kernel (its
+  // degree thresholds scaled by 4): with the eight-deep unrolled loop and one
+  // guarded step for the rest, a lane then has 4-8 entries in flight at once.
+  // Measured at R-MAT scale 22 (17 iterations, one box): thresholds x1 12.9
+  // ms, x2 11.1, x4 10.2, x8 10.5.
+  uint32_t hot_shift = 2;
+  if (const char* shift_env = std::getenv("GC_PR_SHIFT"))
+    hot_shift = static_cast<uint32_t>(std::strtoul(shift_env, nullptr, 10));
+  const PrSegments segs = plan_segments(hoff, hub_begin, block_threads, rows_kernel, rows_kernel ? 0u : hot_shift);
   uint32_t num_hub_chunks = 0;
   uint32_t* hub_chunks = nullptr;
   uint32_t* hub_first = nullptr;

@@ -334,9 +334,18 @@ __device__ __forceinline__ double prh_row_sum(const uint32_t* __restrict__ targe
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc += static_cast<double>(x[k]);
   }
-  for (; i < end; i += STRIDE) {
-    const uint32_t t = load_stream(targets + i);
-    acc += static_cast<double>(t >= hot_base ? s_hot[t - hot_base] : contrib[t]);
+  if (i < end) {
+    // fewer than 8 entries per lane left: one guarded step (index clamped to
+    // the row's last entry, value masked), so all its loads are in flight at
+    // once instead of one target-then-gather round trip per entry
+    uint32_t t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = load_stream(targets + min(i + k * STRIDE, end - 1));
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = t[k] >= hot_base ? s_hot[t[k] - hot_base] : contrib[t[k]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += i + k * STRIDE < end ? static_cast<double>(x[k]) : 0.0;
   }
   return acc;
 }
