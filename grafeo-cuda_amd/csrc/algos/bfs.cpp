@@ -367,7 +367,11 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
   const bool trace = trace_env != nullptr && trace_env[0] == '1';
   const float* params = g.bfs_sched != nullptr ? g.bfs_sched->params.as<const float>() : nullptr;
   g.last_schedule.clear();
-  uint32_t choice = params != nullptr ? t.decide_now(params) : t.fixed_choice();
+  // The first level (the source alone) runs top-down under either rule; the
+  // network decides from the next level on, its decision computed on the
+  // device after each level and read back with the level's counters (one
+  // round trip per level, as for the fixed rule).
+  uint32_t choice = params != nullptr ? kTopDown : t.fixed_choice();
   Stopwatch level_clock;
   while (!t.done) {
     if (!(t.allowed_mask() >> choice & 1u)) choice = kTopDown;

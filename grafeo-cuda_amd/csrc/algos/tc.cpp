@@ -19,6 +19,7 @@ namespace {
 constexpr uint32_t kBitmap = 0, kHash = 1, kLists = 2;  // the scheduler's choices
 constexpr uint32_t kHotBits = 32768;                    // TCB_HOT_BITS of tc.hip
 constexpr uint32_t kMaxLaunches = 64;                   // launches per count
+constexpr float kSwitchUs = 20.0f;                      // predicted saving that pays for a launch
 constexpr uint32_t kMaxParts = 8;                       // middle-vertex parts of tc_bitmap (one per XCD)
 constexpr uint32_t kDefaultParts = 4;
 constexpr uint32_t kCounterStride = 64;                 // TCB_COUNTER_STRIDE of tc.hip
@@ -94,20 +95,30 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
   const uint32_t* fwd = g.rl.fwd.as<const uint32_t>();
   const uint32_t* end = g.rl.offsets.as<const uint32_t>() + 1;
   const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
-  const uint32_t heavy = std::min<uint32_t>(row_hi - row_lo, 65536);
+  const uint32_t rows = row_hi - row_lo;
+  const uint32_t heavy = std::min<uint32_t>(rows, 65536);
   const uint32_t light_chunk = 64;
-  const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+  const uint32_t full_grid = std::max(1, dev.cu_count) * 8u;
+  // Workgroups of four waves, no more than the tasks (single heavy rows and
+  // light chunks, per part) can keep busy: a small row range does not pay
+  // for thousands of waves that only find the work counters drained.
+  auto grid_for = [&](uint64_t tasks) {
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(full_grid, (tasks + 3) / 4)));
+  };
   if (method == kLists) {
     launch_tc_count(g, fwd, end, tgt, row_lo, row_hi, true, true, counters, result);
   } else if (method == kHash) {
-    launch(dev, get_kernel(dev, "tc", "tc_hash"), grid, 4 * kWaveSize, 0, fwd, end, tgt, row_hi, row_lo,
-           heavy, light_chunk, counters, result);
+    const uint64_t tasks = heavy + (static_cast<uint64_t>(rows - heavy) + light_chunk - 1) / light_chunk;
+    launch(dev, get_kernel(dev, "tc", "tc_hash"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
+           row_lo, heavy, light_chunk, counters, result);
   } else {
     // hot range [hot_base, n): at most kHotBits ids, hot_base a multiple of 32
     const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
-    launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid, 4 * kWaveSize, 0, fwd, end, tgt, row_hi, row_lo,
-           hot_base, tc_heavy_rows(g, row_lo, row_hi), light_chunk, tc_parts(), tc_split_min(), counters,
-           result);
+    const uint32_t single = tc_heavy_rows(g, row_lo, row_hi), parts = tc_parts();
+    const uint64_t tasks =
+        parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
+    launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
+           row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result);
   }
 }
 
@@ -236,7 +247,8 @@ uint64_t triangle_count_symmetric(Graph& g) {
                                                              : kBitmap;
       uint32_t* scratch = tc_scratch(g);
       auto* result = reinterpret_cast<unsigned long long*>(scratch);
-      launch_method(g, m, 0, g.n, scratch + 2, result);
+      // rows of degree 0 or 1 (sorted first) close no triangle
+      launch_method(g, m, bucket_bounds(g).front(), g.n, scratch + 2, result);
       return read_result(dev, result);
     }
     // learned: one decision per degree bucket, adjacent buckets with the same
@@ -254,6 +266,10 @@ uint64_t triangle_count_symmetric(Graph& g) {
       uint32_t best = 0;
       for (uint32_t c = 1; c < kSchedChoices; ++c)
         if (yb[c] < yb[best]) best = c;
+      // Leaving the bitmap for a bucket costs a launch of its own (adjacent
+      // buckets of one method share a launch): only worth it when the
+      // predicted saving exceeds kSwitchUs.
+      if (best != kBitmap && std::exp(yb[kBitmap]) - std::exp(yb[best]) < kSwitchUs) best = kBitmap;
       choice[b] = best;
       g.last_schedule.push_back(best);
     }

@@ -207,7 +207,9 @@ extern "C" __global__ __launch_bounds__(64) void sched_bfs_decide(
     const float* __restrict__ params, MlpNorm norm, BfsLevelState st,
     const BfsCountersView* __restrict__ counters, uint32_t use_counters, uint32_t allowed_mask,
     uint32_t chain_cap, unsigned long long chain_edge_cap, SchedDecision* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (blockIdx.x != 0) return;
+  const uint32_t lane = lane_id();
+  // every lane derives the same state and features
   if (use_counters) {
     const BfsCountersView c = *counters;
     st.prev_n_f = st.n_f;
@@ -230,15 +232,38 @@ extern "C" __global__ __launch_bounds__(64) void sched_bfs_decide(
   f[9] = st.bitmap ? 1.0f : 0.0f;
   f[10] = lg(static_cast<double>(st.m) / (st.n > 0 ? st.n : 1));
   f[11] = lg(st.max_degree);
-  for (int i = 0; i < MLP_IN; ++i) out->features[i] = f[i];
+  if (lane < MLP_IN) out->features[lane] = f[lane];
   // chained levels take bounded frontiers only
   if (st.n_f > chain_cap || st.m_f > chain_edge_cap) allowed_mask &= ~4u;
   if (params == nullptr) {
-    out->choice = 0;
+    if (lane == 0) out->choice = 0;
     return;
   }
-  float xn[MLP_IN], h1[MLP_H], h2[MLP_H], y[MLP_OUT];
-  mlp_forward_one(params, f, norm, xn, h1, h2, y);
+  // The forward pass with one hidden neuron per lane; each neuron adds its
+  // inputs in the order of mlp_forward_one, so the predictions are the same
+  // numbers (a single lane did the whole pass before: about 1,500 dependent
+  // multiply-adds per level).
+  float xn[MLP_IN];
+#pragma unroll
+  for (int i = 0; i < MLP_IN; ++i) xn[i] = (f[i] - norm.mean[i]) * norm.scale[i];
+  const uint32_t j = lane < MLP_H ? lane : 0u;
+  float a = params[MLP_B1 + j];
+#pragma unroll
+  for (int i = 0; i < MLP_IN; ++i) a = fmaf(params[MLP_W1 + j * MLP_IN + i], xn[i], a);
+  const float h1 = a > 0.0f ? a : 0.0f;
+  a = params[MLP_B2 + j];
+#pragma unroll 8
+  for (int i = 0; i < MLP_H; ++i) a = fmaf(params[MLP_W2 + j * MLP_H + i], __shfl(h1, i), a);
+  const float h2 = a > 0.0f ? a : 0.0f;
+  float y[MLP_OUT];
+#pragma unroll
+  for (int k = 0; k < MLP_OUT; ++k) {
+    float o = params[MLP_B3 + k];
+#pragma unroll 8
+    for (int i = 0; i < MLP_H; ++i) o = fmaf(params[MLP_W3 + k * MLP_H + i], __shfl(h2, i), o);
+    y[k] = o;
+  }
+  if (lane != 0) return;
   uint32_t best = 0xFFFFFFFFu;
   for (uint32_t k = 0; k < MLP_OUT; ++k) {
     out->predicted[k] = y[k];
