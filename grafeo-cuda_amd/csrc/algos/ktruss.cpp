@@ -116,41 +116,90 @@ uint32_t rounds_direct(Graph& g, uint32_t threshold, uint8_t* flags) {
   return iterations;
 }
 
+// The reference's rounds on a symmetric graph, with the supports computed
+// once: every triangle is enumerated at its middle vertex, and each later
+// round only destroys the triangles of the edges it removes (kt_truss_peel:
+// the surviving edges of a destroyed triangle lose one support each, once per
+// triangle). Round r removes every live edge whose support, among live edges,
+// is below the threshold; the first round also counts the self-loops of an
+// edge's endpoints and removes the self-loops. An edge falls below the
+// threshold in round r >= 3 only by losing support in round r - 1, so the
+// peel's list of such edges is the next frontier; round 2 adds the edges that
+// only the self-loops held above it. `iterations` counts the last round,
+// which removes nothing. (Recomputing every support each round, as the
+// reference does, cost 8 triangle enumerations for the 4-truss of R-MAT
+// scale 22.)
 uint32_t rounds_dag(Graph& g, uint32_t threshold, uint8_t* flags) {
   Device& dev = *g.dev;
   const uint32_t n = g.n, m = g.m;
   const uint32_t* off = g.rl.offsets.as<const uint32_t>();
   const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
   const uint32_t* fwd = g.rl.fwd.as<const uint32_t>();
-  Grid1D vg = thread_per_item(n);
   ensure_reverse_index(g);
+  const uint64_t m_o = g.rl.oriented_entries;
   uint32_t* support = g.buffer<uint32_t>("kt.support", sizeof(uint32_t) * m);
-  uint8_t* removed = g.buffer<uint8_t>("kt.removed", (static_cast<size_t>(m) + 3) / 4 * 4);
-  uint32_t* count = g.buffer<uint32_t>("kt.count", 16);  // [0] removed, [1..2] work counters
-  fill_words(dev, removed, (static_cast<uint64_t>(m) + 3) / 4, 0);
-  Kernel k_support = get_kernel(dev, "ktruss", "kt_wedge_support");
-  Kernel k_finish = get_kernel(dev, "ktruss", "kt_dag_finish");
+  uint8_t* state = g.buffer<uint8_t>("kt.removed", (static_cast<size_t>(m) + 3) / 4 * 4);
+  uint32_t* canon = g.buffer<uint32_t>("kt.canon", sizeof(uint32_t) * m);
+  uint32_t* queue[2] = {g.buffer<uint32_t>("kt.queue0", sizeof(uint32_t) * std::max<uint64_t>(m_o, 1)),
+                        g.buffer<uint32_t>("kt.queue1", sizeof(uint32_t) * std::max<uint64_t>(m_o, 1))};
+  uint32_t* counters = g.buffer<uint32_t>("kt.count", 16);
+  fill_words(dev, state, (static_cast<uint64_t>(m) + 3) / 4, 0);
+  fill_words(dev, support, m, 0);
+  fill_words(dev, counters, 4, 0);
   const uint32_t bits = wedge_table_bits(g.rl);
   const uint32_t grid = std::max(1, dev.cu_count) * 8u;
-  const uint32_t hub_chunk = Relabelled::kHubChunk;
-  uint32_t iterations = 0;
-  for (;;) {
-    fill_words(dev, support, m, 0);
-    fill_words(dev, count, 4, 0);
-    launch(dev, k_support, grid, kWaveSize, (1u << bits) * 8u, off, tgt, fwd,
-           static_cast<const uint8_t*>(removed), n, g.rl.hub_units.as<const uint32_t>(),
-           g.rl.num_hub_units, hub_chunk, bits, count + 1, support);
-    const uint32_t round1 = iterations == 0 ? 1 : 0;
-    launch(dev, k_finish, vg.blocks, vg.threads, 0, off, tgt, fwd,
-           g.ktruss_rev.as<const uint32_t>(), removed, n, static_cast<const uint32_t*>(support),
-           threshold, round1, count);
-    ++iterations;
-    if (read_u32(dev, count) == 0) break;
+  // every entry is active: no removal flags to read
+  launch(dev, get_kernel(dev, "ktruss", "kt_wedge_support"), grid, kWaveSize, (1u << bits) * 8u, off, tgt,
+         fwd, static_cast<const uint8_t*>(nullptr), n, g.rl.hub_units.as<const uint32_t>(), g.rl.num_hub_units,
+         Relabelled::kHubChunk, bits, counters + 1, support);
+  Grid1D vg = thread_per_item(n);
+  launch(dev, get_kernel(dev, "ktruss", "kt_canonical_index"), vg.blocks, vg.threads, 0, off, fwd, n, canon,
+         state);
+  launch(dev, get_kernel(dev, "ktruss", "kt_canonical_reverse"), vg.blocks, vg.threads, 0, off, fwd,
+         g.ktruss_rev.as<const uint32_t>(), n, canon);
+
+  Kernel k_frontier = get_kernel(dev, "ktruss", "kt_truss_frontier");
+  Kernel k_peel = get_kernel(dev, "ktruss", "kt_truss_peel");
+  Kernel k_commit = get_kernel(dev, "ktruss", "kt_truss_commit");
+  const uint32_t t = threshold;
+  // appends the live edges below t (with the self-loop terms when `loops`)
+  // to queue[q] after its first `already` entries; returns the new length
+  auto frontier = [&](int q, uint32_t already, uint32_t loops) {
+    const uint32_t init[2] = {already, 0xFFFFFFFFu};
+    copy_to_device(dev, counters, init, sizeof(init));
+    launch(dev, k_frontier, grid, 256u, 0, off, tgt, fwd, n, loops, state, static_cast<const uint32_t*>(support),
+           static_cast<uint64_t>(m), t, queue[q], counters);
+    uint32_t size = 0;
+    copy_to_host(dev, dev.pinned, counters, sizeof(uint32_t));
+    std::memcpy(&size, dev.pinned, sizeof(size));
+    return size;
+  };
+  int cur = 0;
+  uint32_t size = frontier(cur, 0, 1);
+  uint32_t iterations = 1;
+  if (size > 0 || g.self_loops > 0) {  // round 1 removed something
+    for (uint32_t round = 1;; ++round) {
+      uint32_t next = 0;
+      if (size > 0) {
+        fill_words(dev, counters + 2, 1, 0);
+        launch(dev, k_peel, grid, 256u, 0, off, tgt, static_cast<const uint32_t*>(canon), n,
+               static_cast<const uint32_t*>(queue[cur]), size, state, support, t, queue[cur ^ 1], counters + 2);
+        copy_to_host(dev, dev.pinned, counters + 2, sizeof(uint32_t));
+        std::memcpy(&next, dev.pinned, sizeof(next));
+        Grid1D cg = thread_per_item(std::max(size, next));
+        launch(dev, k_commit, cg.blocks, cg.threads, 0, static_cast<const uint32_t*>(queue[cur]), size,
+               static_cast<const uint32_t*>(queue[cur ^ 1]), next, state, support, 0u);
+      }
+      ++iterations;  // round + 1 runs
+      size = round == 1 ? frontier(cur ^ 1, next, 0) : next;
+      cur ^= 1;
+      if (size == 0) break;  // that round removes nothing
+    }
   }
   Grid1D eg = thread_per_item(m);
-  launch(dev, get_kernel(dev, "ktruss", "kt_emit_flags_relabelled"), eg.blocks, eg.threads, 0,
-         g.offsets.as<const uint32_t>(), g.targets.as<const uint32_t>(), n, m,
-         g.rl.inv.as<const uint32_t>(), off, tgt, static_cast<const uint8_t*>(removed), flags);
+  launch(dev, get_kernel(dev, "ktruss", "kt_emit_flags_state"), eg.blocks, eg.threads, 0,
+         g.offsets.as<const uint32_t>(), g.targets.as<const uint32_t>(), n, m, g.rl.inv.as<const uint32_t>(), off,
+         tgt, static_cast<const uint32_t*>(canon), static_cast<const uint8_t*>(state), flags);
   return iterations;
 }
 
@@ -178,7 +227,7 @@ uint32_t truss_peel(Graph& g, uint32_t* support) {
   const uint32_t bits = wedge_table_bits(g.rl);
   const uint32_t grid = std::max(1, dev.cu_count) * 8u;
   launch(dev, get_kernel(dev, "ktruss", "kt_wedge_support"), grid, kWaveSize, (1u << bits) * 8u, off,
-         tgt, fwd, static_cast<const uint8_t*>(state), n, g.rl.hub_units.as<const uint32_t>(),
+         tgt, fwd, static_cast<const uint8_t*>(nullptr), n, g.rl.hub_units.as<const uint32_t>(),
          g.rl.num_hub_units, Relabelled::kHubChunk, bits, counters + 1, support);
   Grid1D vg = thread_per_item(n);
   launch(dev, get_kernel(dev, "ktruss", "kt_canonical_index"), vg.blocks, vg.threads, 0, off, fwd, n,
@@ -202,8 +251,8 @@ uint32_t truss_peel(Graph& g, uint32_t* support) {
     const uint32_t t = k - 2;
     const uint32_t init[2] = {0u, 0xFFFFFFFFu};
     copy_to_device(dev, counters, init, sizeof(init));
-    launch(dev, k_frontier, scan_grid, 256u, 0, state, static_cast<const uint32_t*>(support),
-           static_cast<uint64_t>(m), t, queue[0], counters);
+    launch(dev, k_frontier, scan_grid, 256u, 0, off, tgt, fwd, n, 0u, state,
+           static_cast<const uint32_t*>(support), static_cast<uint64_t>(m), t, queue[0], counters);
     read_counters(2);
     uint32_t size = host_counters[0];
     if (size == 0) {

@@ -11,11 +11,13 @@
 //    forward entry merges the two lists;
 //  - DAG (symmetric CSR, degree-ordered relabelling): every triangle is found
 //    once, at its middle vertex (wedge.hpp), and adds 1 to the support of its
-//    three (forward) edges. On a symmetric graph this
-//    equals the reference's support except in round 1, where the reference's
-//    merge also counts w = u when u has an active self-loop and w = v when v
-//    has one (CHANGELOG.md:76-77); kt_dag_finish adds those two terms in round
-//    1. Self-loops get support 0 and are removed in round 1, so later rounds
+//    three (forward) edges, once; the rounds then peel (kt_truss_frontier,
+//    kt_truss_peel, kt_truss_commit) and only destroy the triangles of the
+//    edges they remove. On a symmetric graph this equals the reference's
+//    support except in round 1, where the reference's merge also counts w = u
+//    when u has an active self-loop and w = v when v has one
+//    (CHANGELOG.md:76-77); the first frontier adds those two terms.
+//    Self-loops get support 0 and are removed in round 1, so later rounds
 //    need no correction.
 #include "device.hpp"
 #include "wedge.hpp"
@@ -108,62 +110,6 @@ __device__ __forceinline__ uint32_t has_loop(const uint32_t* __restrict__ offset
   return (f > offsets[x] && targets[f - 1] == x) ? 1u : 0u;
 }
 
-// Removes active forward entries (and their reverse entries) whose support
-// is below `threshold`; in round 1 also every self-loop, and the support of
-// an edge gains the self-loops of its two endpoints first.
-extern "C" __global__ __launch_bounds__(256) void kt_dag_finish(
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
-    const uint32_t* __restrict__ fwd, const uint32_t* __restrict__ rev,
-    uint8_t* __restrict__ removed, uint32_t n, const uint32_t* __restrict__ support,
-    uint32_t threshold, uint32_t round1, uint32_t* __restrict__ removed_count) {
-  uint32_t local = 0;
-  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
-    const uint32_t u = static_cast<uint32_t>(uu);
-    const uint32_t f = fwd[u], e = offsets[u + 1];
-    uint32_t loop_u = 0;
-    if (round1) {
-      loop_u = has_loop(offsets, targets, fwd, u);
-      if (loop_u && !removed[f - 1]) {
-        removed[f - 1] = 1;
-        ++local;
-      }
-    }
-    for (uint32_t i = f; i < e; ++i) {
-      if (removed[i]) continue;
-      uint32_t s = support[i];
-      if (round1) s += loop_u + has_loop(offsets, targets, fwd, targets[i]);
-      if (s < threshold) {
-        removed[i] = 1;
-        removed[rev[i]] = 1;
-        local += 2;
-      }
-    }
-  }
-  local = wave_sum(local);
-  if (lane_id() == 0 && local) atomicAdd(removed_count, local);
-}
-
-// flags[e] = 1 for original entries u->v, v > u, whose relabelled entry
-// inv[u] -> inv[v] survived.
-extern "C" __global__ __launch_bounds__(256) void kt_emit_flags_relabelled(
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n,
-    uint32_t m, const uint32_t* __restrict__ inv, const uint32_t* __restrict__ r_offsets,
-    const uint32_t* __restrict__ r_targets, const uint8_t* __restrict__ removed,
-    uint8_t* __restrict__ flags) {
-  for (uint64_t i = global_thread(); i < m; i += grid_threads()) {
-    const uint32_t e = static_cast<uint32_t>(i);
-    const uint32_t u = row_of_entry(offsets, n, e);
-    const uint32_t v = targets[e];
-    uint8_t flag = 0;
-    if (v > u) {
-      const uint32_t iu = inv[u], iv = inv[v];
-      const uint32_t p = lower_bound_u32(r_targets, r_offsets[iu], r_offsets[iu + 1], iv);
-      flag = removed[p] ? 0 : 1;
-    }
-    flags[e] = flag;
-  }
-}
-
 // ----------------------------------------------------------- truss numbers --
 //
 // Per-edge trussness (the largest k whose k-truss holds the edge; 2 for an
@@ -218,10 +164,15 @@ extern "C" __global__ __launch_bounds__(256) void kt_canonical_reverse(
   }
 }
 
-// Starts a level: every live edge with support < t joins the frontier. The
-// smallest support among the edges that stay live goes to counters[1], so
-// the host can skip levels that would remove nothing.
+// Starts a level: every live edge with support < t joins the frontier,
+// appended to queue from counters[0] on. The smallest support among the edges
+// that stay live goes to counters[1], so the host can skip levels that would
+// remove nothing. With `loops` (the first k-truss round), an edge's support
+// also counts the self-loops of its two endpoints, as the reference's first
+// round does.
 extern "C" __global__ __launch_bounds__(256) void kt_truss_frontier(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ fwd, uint32_t n, uint32_t loops,
     uint8_t* __restrict__ state, const uint32_t* __restrict__ support, uint64_t m, uint32_t t,
     uint32_t* __restrict__ queue, uint32_t* __restrict__ counters) {
   uint32_t local_min = 0xFFFFFFFFu;
@@ -233,7 +184,11 @@ extern "C" __global__ __launch_bounds__(256) void kt_truss_frontier(
     const uint64_t i = start + s * stride;
     bool take = false;
     if (i < m && state[i] == KT_ALIVE) {
-      const uint32_t sup = support[i];
+      uint32_t sup = support[i];
+      if (loops) {
+        const uint32_t u = row_of_entry(offsets, n, static_cast<uint32_t>(i));
+        sup += has_loop(offsets, targets, fwd, u) + has_loop(offsets, targets, fwd, targets[i]);
+      }
       if (sup < t) {
         take = true;
         state[i] = static_cast<uint8_t>(KT_FRONTIER);
@@ -313,6 +268,27 @@ extern "C" __global__ __launch_bounds__(256) void kt_truss_commit(
       support[x] = truss;
     }
     if (i < next_size) state[next_queue[i]] = static_cast<uint8_t>(KT_FRONTIER);
+  }
+}
+
+// flags[e] = 1 for original entries u -> v, v > u, whose undirected edge
+// {inv[u], inv[v]} is still live (its canonical entry not KT_REMOVED).
+extern "C" __global__ __launch_bounds__(256) void kt_emit_flags_state(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n,
+    uint32_t m, const uint32_t* __restrict__ inv, const uint32_t* __restrict__ r_offsets,
+    const uint32_t* __restrict__ r_targets, const uint32_t* __restrict__ canon,
+    const uint8_t* __restrict__ state, uint8_t* __restrict__ flags) {
+  for (uint64_t i = global_thread(); i < m; i += grid_threads()) {
+    const uint32_t e = static_cast<uint32_t>(i);
+    const uint32_t u = row_of_entry(offsets, n, e);
+    const uint32_t v = targets[e];
+    uint8_t flag = 0;
+    if (v > u) {
+      const uint32_t iu = inv[u], iv = inv[v];
+      const uint32_t p = lower_bound_u32(r_targets, r_offsets[iu], r_offsets[iu + 1], iv);
+      flag = state[canon[p]] == KT_REMOVED ? 0 : 1;
+    }
+    flags[e] = flag;
   }
 }
 

@@ -18,9 +18,15 @@
 // by the host first (one per fetch), then every other vertex in descending id
 // order (heaviest first) in chunks.
 //
-// SUPPORT mode (k-truss): only active entries take part (the table holds the
-// active N+(v) with their entry indices), and each triangle adds 1 to the
-// support of its three forward entries u->v, u->w and v->w.
+// SUPPORT mode (k-truss): only active entries take part (all of them when
+// `removed` is null: the byte loads of the removal flags, one per wedge,
+// cost more than the enumeration itself), and each triangle
+// adds 1 to the support of its three forward entries u->v, u->w and v->w.
+// Only u->w takes a global atomic per triangle: v->w is counted in the LDS
+// table next to w (added to the support once per unit, after a binary search
+// for its entry), and u->v in an LDS counter per in-neighbour lane (added
+// once per chunk of 64 in-neighbours). Three global atomics per triangle made
+// the pass at R-MAT scale 22 take 420 ms.
 #pragma once
 
 #include "device.hpp"
@@ -58,8 +64,10 @@ __device__ __forceinline__ void wedge_body(
   __shared__ uint32_t s_start[GC_WAVE];
   __shared__ uint32_t s_edge[GC_WAVE];
   __shared__ uint32_t s_range[3];
+  __shared__ uint32_t s_ucnt[GC_WAVE];  // SUPPORT: triangles of in-neighbour lane l's entry u->v
 
   const uint32_t lane = threadIdx.x;
+  if (SUPPORT) s_ucnt[lane] = 0;
   unsigned long long count = 0;
   bool hubs_left = num_hub_units > 0;
 
@@ -121,13 +129,13 @@ __device__ __forceinline__ void wedge_body(
         __syncthreads();
         for (uint32_t j = lane; j < a; j += GC_WAVE) {
           const uint32_t e = a_lo + j;
-          if (SUPPORT && removed[e]) continue;
+          if (SUPPORT && removed != nullptr && removed[e]) continue;
           const uint32_t w = targets[e];
           uint32_t slot = wedge_hash(w, shift_v);
           for (;;) {
             const uint32_t prev = atomicCAS(&s_key[slot], WEDGE_EMPTY, w);
             if (prev == WEDGE_EMPTY) {
-              if (SUPPORT) s_val[slot] = e;
+              if (SUPPORT) s_val[slot] = 0;  // triangles of v -> w in this unit
               break;
             }
             slot = (slot + 1) & mask_v;
@@ -143,7 +151,7 @@ __device__ __forceinline__ void wedge_body(
           const uint32_t u = targets[c + lane];
           const uint32_t hi = offsets[u + 1];
           const uint32_t q = lower_bound_u32(targets, fwd[u], hi, v);
-          if (!SUPPORT || !removed[q]) {
+          if (!SUPPORT || removed == nullptr || !removed[q]) {
             st = q + 1;
             len = hi - st;
             e1 = q;
@@ -164,7 +172,7 @@ __device__ __forceinline__ void wedge_body(
           }
           const uint32_t excl = l ? s_pos[l - 1] : 0;
           const uint32_t e2 = s_start[l] + (t - excl);
-          if (SUPPORT && removed[e2]) continue;
+          if (SUPPORT && removed != nullptr && removed[e2]) continue;
           const uint32_t w = targets[e2];
           uint32_t e3 = WEDGE_EMPTY;
           if (in_table) {
@@ -172,7 +180,7 @@ __device__ __forceinline__ void wedge_body(
             for (;;) {
               const uint32_t key = s_key[slot];
               if (key == w) {
-                e3 = SUPPORT ? s_val[slot] : 0u;
+                e3 = slot;
                 break;
               }
               if (key == WEDGE_EMPTY) break;
@@ -180,16 +188,32 @@ __device__ __forceinline__ void wedge_body(
             }
           } else {
             const uint32_t p = lower_bound_u32(targets, a_lo, a_lo + a, w);
-            if (p < a_lo + a && targets[p] == w && !(SUPPORT && removed[p])) e3 = p;
+            if (p < a_lo + a && targets[p] == w && !(SUPPORT && removed != nullptr && removed[p])) e3 = p;
           }
           if (e3 == WEDGE_EMPTY) continue;
           if (SUPPORT) {
-            atomicAdd(&support[s_edge[l]], 1u);
+            atomicAdd(&s_ucnt[l], 1u);
             atomicAdd(&support[e2], 1u);
-            atomicAdd(&support[e3], 1u);
+            if (in_table) atomicAdd(&s_val[e3], 1u);
+            else atomicAdd(&support[e3], 1u);
           } else {
             ++count;
           }
+        }
+        __syncthreads();
+        if (SUPPORT) {  // the chunk's u -> v counts
+          if (lane < nv && s_ucnt[lane] != 0) {
+            atomicAdd(&support[s_edge[lane]], s_ucnt[lane]);
+            s_ucnt[lane] = 0;
+          }
+          __syncthreads();
+        }
+      }
+      if (SUPPORT && in_table) {  // the unit's v -> w counts
+        for (uint32_t j = lane; j < size_v; j += GC_WAVE) {
+          const uint32_t key = s_key[j];
+          if (key != WEDGE_EMPTY && s_val[j] != 0)
+            atomicAdd(&support[lower_bound_u32(targets, a_lo, a_lo + a, key)], s_val[j]);
         }
         __syncthreads();
       }
