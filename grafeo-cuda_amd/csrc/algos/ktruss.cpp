@@ -162,13 +162,14 @@ uint32_t rounds_dag(Graph& g, uint32_t threshold, uint8_t* flags) {
   Kernel k_peel = get_kernel(dev, "ktruss", "kt_truss_peel");
   Kernel k_commit = get_kernel(dev, "ktruss", "kt_truss_commit");
   const uint32_t t = threshold;
+  const uint32_t frontier_blocks = static_cast<uint32_t>(std::max<uint64_t>(1, (static_cast<uint64_t>(m) + 4095) / 4096));
   // appends the live edges below t (with the self-loop terms when `loops`)
   // to queue[q] after its first `already` entries; returns the new length
   auto frontier = [&](int q, uint32_t already, uint32_t loops) {
     const uint32_t init[2] = {already, 0xFFFFFFFFu};
     copy_to_device(dev, counters, init, sizeof(init));
-    launch(dev, k_frontier, grid, 256u, 0, off, tgt, fwd, n, loops, state, static_cast<const uint32_t*>(support),
-           static_cast<uint64_t>(m), t, queue[q], counters);
+    launch(dev, k_frontier, frontier_blocks, 256u, 0, off, tgt, fwd, n, loops, state,
+           static_cast<const uint32_t*>(support), static_cast<uint64_t>(m), t, queue[q], counters);
     uint32_t size = 0;
     copy_to_host(dev, dev.pinned, counters, sizeof(uint32_t));
     std::memcpy(&size, dev.pinned, sizeof(size));
@@ -238,7 +239,7 @@ uint32_t truss_peel(Graph& g, uint32_t* support) {
   Kernel k_frontier = get_kernel(dev, "ktruss", "kt_truss_frontier");
   Kernel k_peel = get_kernel(dev, "ktruss", "kt_truss_peel");
   Kernel k_commit = get_kernel(dev, "ktruss", "kt_truss_commit");
-  const uint32_t scan_grid = std::max(1, dev.cu_count) * 8u;
+  const uint32_t frontier_blocks = static_cast<uint32_t>(std::max<uint64_t>(1, (static_cast<uint64_t>(m) + 4095) / 4096));
   const uint32_t peel_grid = std::max(1, dev.cu_count) * 8u;  // 4 waves per workgroup
   uint64_t alive = m_o;
   uint32_t k = 3, max_truss = m_o > 0 ? 2 : 0;
@@ -251,7 +252,7 @@ uint32_t truss_peel(Graph& g, uint32_t* support) {
     const uint32_t t = k - 2;
     const uint32_t init[2] = {0u, 0xFFFFFFFFu};
     copy_to_device(dev, counters, init, sizeof(init));
-    launch(dev, k_frontier, scan_grid, 256u, 0, off, tgt, fwd, n, 0u, state,
+    launch(dev, k_frontier, frontier_blocks, 256u, 0, off, tgt, fwd, n, 0u, state,
            static_cast<const uint32_t*>(support), static_cast<uint64_t>(m), t, queue[0], counters);
     read_counters(2);
     uint32_t size = host_counters[0];

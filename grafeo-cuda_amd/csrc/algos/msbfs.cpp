@@ -123,8 +123,9 @@ void msbfs_chunk(Graph& g, const uint32_t* sources, uint32_t count, int32_t* d_d
       }
     }
     fill_words(dev, counters, sizeof(MsCounters) / 4, 0);
-    launch(dev, k_fin, std::min<uint32_t>((n + 255) / 256, resident * 4), 256u, 0, off, n,
-           active_lo, active_hi, seen, visit, next, queue, counters);
+    // one workgroup per MSF_RANGE (4096) vertices
+    launch(dev, k_fin, static_cast<uint32_t>(std::max<uint64_t>(1, (static_cast<uint64_t>(n) + 4095) / 4096)),
+           256u, 0, off, n, active_lo, active_hi, seen, visit, next, queue, counters);
     MsCounters c;
     copy_to_host(dev, dev.pinned, counters, sizeof(MsCounters));
     std::memcpy(&c, dev.pinned, sizeof(c));

@@ -170,19 +170,23 @@ extern "C" __global__ __launch_bounds__(256) void kt_canonical_reverse(
 // remove nothing. With `loops` (the first k-truss round), an edge's support
 // also counts the self-loops of its two endpoints, as the reference's first
 // round does.
+#define KTF_ITEMS 16  // entries per thread: a workgroup owns 4096 consecutive entries
+
 extern "C" __global__ __launch_bounds__(256) void kt_truss_frontier(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ fwd, uint32_t n, uint32_t loops,
     uint8_t* __restrict__ state, const uint32_t* __restrict__ support, uint64_t m, uint32_t t,
     uint32_t* __restrict__ queue, uint32_t* __restrict__ counters) {
+  // one queue atomic and one minimum atomic per workgroup (per wave and step,
+  // all on one line, they serialized: several ms per call at R-MAT scale 22)
+  __shared__ uint32_t s_wave_count[256 / GC_WAVE];
+  __shared__ uint32_t s_wave_min[256 / GC_WAVE];
+  __shared__ uint32_t s_base;
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * 256 * KTF_ITEMS;
   uint32_t local_min = 0xFFFFFFFFu;
-  const uint64_t stride = grid_threads();
-  const uint64_t start = global_thread();
-  // every lane of a wave runs the same number of steps (wave-wide appends)
-  const uint64_t steps = (m + stride - 1) / stride;
-  for (uint64_t s = 0; s < steps; ++s) {
-    const uint64_t i = start + s * stride;
-    bool take = false;
+  uint32_t flags = 0;  // bit k: item k joins the frontier
+  for (uint32_t k = 0; k < KTF_ITEMS; ++k) {
+    const uint64_t i = first + static_cast<uint64_t>(k) * 256 + threadIdx.x;
     if (i < m && state[i] == KT_ALIVE) {
       uint32_t sup = support[i];
       if (loops) {
@@ -190,22 +194,37 @@ extern "C" __global__ __launch_bounds__(256) void kt_truss_frontier(
         sup += has_loop(offsets, targets, fwd, u) + has_loop(offsets, targets, fwd, targets[i]);
       }
       if (sup < t) {
-        take = true;
+        flags |= 1u << k;
         state[i] = static_cast<uint8_t>(KT_FRONTIER);
       } else {
         local_min = min(local_min, sup);
       }
     }
-    const unsigned long long mask = __ballot(take);
-    if (mask) {
-      uint32_t base = 0;
-      if (lane_id() == 0) base = atomicAdd(&counters[0], static_cast<uint32_t>(__popcll(mask)));
-      base = __shfl(base, 0);
-      if (take) queue[base + lanes_below(mask)] = static_cast<uint32_t>(i);
-    }
   }
+  const uint32_t mine = static_cast<uint32_t>(__popc(flags));
+  const uint32_t incl = wave_inclusive_scan(mine);
   for (int off = GC_WAVE / 2; off > 0; off >>= 1) local_min = min(local_min, __shfl_xor(local_min, off));
-  if (lane_id() == 0 && local_min != 0xFFFFFFFFu) atomicMin(&counters[1], local_min);
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  if (lane_id() == GC_WAVE - 1) s_wave_count[wave] = incl;
+  if (lane_id() == 0) s_wave_min[wave] = local_min;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0, lowest = 0xFFFFFFFFu;
+    for (uint32_t w = 0; w < 256 / GC_WAVE; ++w) {
+      total += s_wave_count[w];
+      lowest = min(lowest, s_wave_min[w]);
+    }
+    s_base = total ? atomicAdd(&counters[0], total) : 0u;
+    if (lowest != 0xFFFFFFFFu) atomicMin(&counters[1], lowest);
+  }
+  __syncthreads();
+  uint32_t pos = s_base + incl - mine;
+  for (uint32_t w = 0; w < wave; ++w) pos += s_wave_count[w];
+  while (flags != 0) {
+    const uint32_t k = static_cast<uint32_t>(__ffs(flags)) - 1;
+    flags &= flags - 1;
+    queue[pos++] = static_cast<uint32_t>(first + static_cast<uint64_t>(k) * 256 + threadIdx.x);
+  }
 }
 
 __device__ __forceinline__ void kt_truss_decrement(uint32_t x, uint32_t t,
