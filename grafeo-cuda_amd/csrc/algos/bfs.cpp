@@ -67,6 +67,7 @@ struct BfsReadback {
 constexpr double kAlpha = 14.0;
 constexpr double kBeta = 24.0;
 constexpr uint32_t kTopDown = 0, kBottomUp = 1, kChained = 2;
+constexpr uint64_t kSmallFrontier = 16384;  // frontiers scanned by one workgroup
 // Device-chained levels (choice 2): the largest frontier they take or keep
 // going with, in vertices and in edges, and the levels of one captured chain
 // (even, so that the frontier is back in the starting queue after a chain).
@@ -92,7 +93,7 @@ struct Traversal {
   BfsChainState* chain_state;
   const uint32_t* off;
   const uint32_t* tgt;
-  Kernel k_td, k_deg, k_bu, k_q2b, k_b2q, k_chain_expand, k_chain_advance, k_decide;
+  Kernel k_td, k_deg, k_scan_small, k_bu, k_q2b, k_b2q, k_chain_expand, k_chain_advance, k_decide;
   uint32_t cu;
   // state at the start of the next level
   uint64_t n_f = 1, m_f = 0, m_u = 0, reached = 1, prev_n_f = 1;
@@ -117,6 +118,7 @@ struct Traversal {
     tgt = g.rl.targets.as<const uint32_t>();
     k_td = get_kernel(dev, "bfs", "bfs_topdown_edges");
     k_deg = get_kernel(dev, "bfs", "bfs_frontier_degrees");
+    k_scan_small = get_kernel(dev, "bfs", "bfs_frontier_scan_small");
     k_bu = get_kernel(dev, "bfs", "bfs_bottomup");
     k_q2b = get_kernel(dev, "bfs", "bfs_queue_to_bitmap");
     k_b2q = get_kernel(dev, "bfs", "bfs_bitmap_to_queue");
@@ -207,20 +209,26 @@ struct Traversal {
 
   // Launches one top-down or bottom-up level (counters into rb->counters[0]).
   void launch_level(uint32_t choice) {
-    fill_words(dev, &rb->counters[0], sizeof(BfsCounters) / 4, 0);
     const int32_t next_level = level + 1;
     if (choice == kTopDown) {
       to_queue();
-      Grid1D qg = thread_per_item(n_f);
-      launch(dev, k_deg, qg.blocks, qg.threads, 0, off, static_cast<const uint32_t*>(queue[cur]),
-             static_cast<uint32_t>(n_f), start);
-      exclusive_scan_u32(dev, start, n_f, scan_buf);
+      if (n_f <= kSmallFrontier) {
+        launch(dev, k_scan_small, 1u, 1024u, 0, off, static_cast<const uint32_t*>(queue[cur]),
+               static_cast<uint32_t>(n_f), start, &rb->counters[0]);
+      } else {
+        fill_words(dev, &rb->counters[0], sizeof(BfsCounters) / 4, 0);
+        Grid1D qg = thread_per_item(n_f);
+        launch(dev, k_deg, qg.blocks, qg.threads, 0, off, static_cast<const uint32_t*>(queue[cur]),
+               static_cast<uint32_t>(n_f), start);
+        exclusive_scan_u32(dev, start, n_f, scan_buf);
+      }
       Grid1D tg = wave_per_item((m_f + 255) / 256);
       launch(dev, k_td, std::max(1u, std::min(tg.blocks, cu * 32)), tg.threads, 0, off, tgt, dist,
              static_cast<const uint32_t*>(queue[cur]), static_cast<uint32_t>(n_f),
              static_cast<const uint32_t*>(start), static_cast<uint32_t>(m_f), queue[cur ^ 1],
              &rb->counters[0], next_level);
     } else {
+      fill_words(dev, &rb->counters[0], sizeof(BfsCounters) / 4, 0);
       to_bitmap();
       Grid1D bg = wave_per_item(words64);
       launch(dev, k_bu, std::min(bg.blocks, cu * 8), bg.threads, 0, off, tgt, top_entries(), dist,

@@ -150,6 +150,47 @@ extern "C" __global__ __launch_bounds__(256) void bfs_frontier_degrees(
   }
 }
 
+// For a small frontier (one workgroup's work): start[i] = exclusive prefix
+// sum of the degrees of queue[0..size), and the level's counters cleared, in
+// one launch instead of a fill, the degree kernel and the three scan kernels
+// (about 30 us of launches and kernels per top-down level).
+#define FS_THREADS 1024
+
+extern "C" __global__ __launch_bounds__(FS_THREADS) void bfs_frontier_scan_small(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ queue, uint32_t size,
+    uint32_t* __restrict__ start, BfsCounters* __restrict__ counters) {
+  __shared__ uint32_t s_wave[FS_THREADS / GC_WAVE];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) {
+    s_carry = 0;
+    counters->next_size = 0;
+    counters->next_edges = 0;
+  }
+  __syncthreads();
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  for (uint32_t chunk = 0; chunk < size; chunk += FS_THREADS) {
+    const uint32_t i = chunk + threadIdx.x;
+    uint32_t deg = 0;
+    if (i < size) {
+      const uint32_t u = queue[i];
+      deg = offsets[u + 1] - offsets[u];
+    }
+    const uint32_t incl = wave_inclusive_scan(deg);
+    if (lane_id() == GC_WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = s_carry;
+    for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+    if (i < size) start[i] = before + incl - deg;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t total = 0;
+      for (uint32_t w = 0; w < FS_THREADS / GC_WAVE; ++w) total += s_wave[w];
+      s_carry += total;
+    }
+    __syncthreads();
+  }
+}
+
 #define TD_EDGES_PER_WAVE 256
 
 // Top-down step balanced over the frontier's edges: `start` is the exclusive
