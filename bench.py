@@ -37,7 +37,7 @@ METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
 KERNELS = ("tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_hot", "pr_spmv", "pr_hub_finish",
            "pr_finalize", "pr_init", "bfs_topdown_edges", "bfs_frontier_degrees", "bfs_bottomup",
-           "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "scan_block_sums",
+           "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "bfs_frontier_scan_small", "scan_block_sums",
            "exclusive_scan_small", "scan_apply", "gather_u32", "fill_u32")
 
 

@@ -103,12 +103,29 @@ extern "C" __global__ __launch_bounds__(256) void prep_stats(
     max_deg = max(max_deg, static_cast<unsigned long long>(__shfl_xor(max_deg, o)));
     max_out = max(max_out, static_cast<unsigned long long>(__shfl_xor(max_out, o)));
   }
+  // one set of six atomics per workgroup (per wave, on one line, they took
+  // most of the kernel's 5 ms at R-MAT scale 22)
+  __shared__ unsigned long long s_v[6][256 / GC_WAVE];
+  const uint32_t wave = threadIdx.x / GC_WAVE;
   if (lane_id() == 0) {
-    atomicAdd(&stats[0], oriented);
-    atomicAdd(&stats[1], intersect);
-    atomicAdd(&stats[2], wedges);
-    atomicAdd(&stats[3], empty);
-    atomicMax(&stats[4], max_deg);
-    atomicMax(&stats[5], max_out);
+    s_v[0][wave] = oriented;
+    s_v[1][wave] = intersect;
+    s_v[2][wave] = wedges;
+    s_v[3][wave] = empty;
+    s_v[4][wave] = max_deg;
+    s_v[5][wave] = max_out;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t w = 0; w < blockDim.x / GC_WAVE; ++w) {
+      for (int k = 0; k < 4; ++k) t[k] += s_v[k][w];
+      t[4] = max(t[4], s_v[4][w]);
+      t[5] = max(t[5], s_v[5][w]);
+    }
+    for (int k = 0; k < 4; ++k)
+      if (t[k]) atomicAdd(&stats[k], t[k]);
+    if (t[4]) atomicMax(&stats[4], t[4]);
+    if (t[5]) atomicMax(&stats[5], t[5]);
   }
 }
