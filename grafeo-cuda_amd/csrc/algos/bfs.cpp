@@ -231,9 +231,11 @@ struct Traversal {
       fill_words(dev, &rb->counters[0], sizeof(BfsCounters) / 4, 0);
       to_bitmap();
       Grid1D bg = wave_per_item(words64);
+      // the rows of degree 0 come first in the degree order: skip their chunks
+      const uint32_t first_chunk = g.rl.dangling / 64;
       launch(dev, k_bu, std::min(bg.blocks, cu * 8), bg.threads, 0, off, tgt, top_entries(), dist,
-             reinterpret_cast<const uint32_t*>(bitmap[cur]), bitmap[cur ^ 1], n, &rb->counters[0],
-             next_level);
+             reinterpret_cast<const uint32_t*>(bitmap[cur]), bitmap[cur ^ 1], n, first_chunk,
+             &rb->counters[0], next_level);
     }
   }
 

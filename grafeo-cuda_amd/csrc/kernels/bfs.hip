@@ -76,13 +76,17 @@ __device__ __forceinline__ void block_counters_add(uint32_t count, unsigned long
 extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ top, int32_t* __restrict__ dist, const uint32_t* __restrict__ front,
-    uint64_t* __restrict__ next, uint32_t n, BfsCounters* __restrict__ counters, int32_t next_level) {
+    uint64_t* __restrict__ next, uint32_t n, uint32_t first_chunk, BfsCounters* __restrict__ counters,
+    int32_t next_level) {
   const uint32_t lane = lane_id();
   uint32_t found_count = 0;  // this lane's discoveries and their degrees
   unsigned long long found_edges = 0;
   const uint64_t waves = grid_threads() / GC_WAVE;
   const uint64_t chunks = (static_cast<uint64_t>(n) + GC_WAVE - 1) / GC_WAVE;
-  for (uint64_t chunk = global_thread() / GC_WAVE; chunk < chunks; chunk += waves) {
+  // Chunks below first_chunk hold only vertices of degree 0 (sorted first),
+  // which no level ever reaches: their words of `next` are only cleared.
+  for (uint64_t c = global_thread(); c < first_chunk; c += grid_threads()) next[c] = 0;
+  for (uint64_t chunk = first_chunk + global_thread() / GC_WAVE; chunk < chunks; chunk += waves) {
     const uint64_t v64 = chunk * GC_WAVE + lane;
     bool found = false;
     uint32_t deg = 0;
