@@ -61,6 +61,27 @@ uint32_t tc_heavy_rows(const Graph& g, uint32_t row_lo, uint32_t row_hi) {
   return std::min<uint32_t>(row_hi - a, 65536);
 }
 
+// tc_bitmap16 (16-bit hot lists) unless GC_TC_HOT16=0 (tc_bitmap).
+bool tc_hot16() {
+  const char* env = std::getenv("GC_TC_HOT16");
+  return env == nullptr || env[0] != '0';
+}
+
+// The 16-bit hot lists of tc_bitmap16, built once per graph (2 bytes per CSR
+// entry and 4 per vertex).
+void ensure_hot16(Graph& g, uint32_t hot_base) {
+  if (g.tc_hot16_ready) return;
+  Device& dev = *g.dev;
+  const uint64_t m = g.rl.host_offsets[g.n];
+  g.tc_split.allocate(sizeof(uint32_t) * std::max<uint64_t>(g.n, 1));
+  g.tc_t16.allocate(sizeof(uint16_t) * std::max<uint64_t>(m, 2));
+  Grid1D grid = thread_per_item(std::max<uint64_t>(m, g.n));
+  launch(dev, get_kernel(dev, "tc", "tc_hot16_build"), grid.blocks, grid.threads, 0, g.rl.fwd.as<const uint32_t>(),
+         static_cast<const uint32_t*>(g.rl.offsets.as<const uint32_t>() + 1), g.rl.targets.as<const uint32_t>(),
+         g.n, m, hot_base, g.tc_split.as<uint32_t>(), g.tc_t16.as<uint16_t>());
+  g.tc_hot16_ready = true;
+}
+
 // Shortest L(u) that tc_bitmap splits over the parts (shorter rows run whole,
 // once): GC_TC_SPLIT, default 64.
 uint32_t tc_split_min() {
@@ -117,8 +138,15 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     const uint32_t single = tc_heavy_rows(g, row_lo, row_hi), parts = tc_parts();
     const uint64_t tasks =
         parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
-    launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
-           row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result);
+    if (tc_hot16()) {
+      ensure_hot16(g, hot_base);
+      launch(dev, get_kernel(dev, "tc", "tc_bitmap16"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt,
+             g.tc_split.as<const uint32_t>(), g.tc_t16.as<const uint16_t>(), row_hi, row_lo, hot_base, single,
+             light_chunk, parts, tc_split_min(), counters, result);
+    } else {
+      launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
+             row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result);
+    }
   }
 }
 

@@ -416,6 +416,92 @@ __device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, 
   return local;
 }
 
+// ---- 16-bit hot lists (tc_bitmap16) ----
+// A hot entry y = w - hot_base (< TCB_HOT_BITS) is one LDS read and no clamp.
+__device__ __forceinline__ uint32_t hot16_hit(const uint32_t* s_bits, uint32_t y) {
+  return word_bit(s_bits[y >> 5], y);
+}
+
+// As probe_list over the 16-bit hot part H[0, lj) of a list.
+__device__ __forceinline__ uint32_t probe16_list(const uint16_t* H, uint32_t lj, const uint32_t* s_bits) {
+  const uint32_t lane = lane_id();
+  uint32_t local = 0;
+  uint32_t kk = 0;
+  for (; kk + 4 * GC_WAVE <= lj; kk += 4 * GC_WAVE) {
+    uint32_t y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q] = H[kk + q * GC_WAVE + lane];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) local += hot16_hit(s_bits, y[q]);
+  }
+  if (kk < lj) {
+    const uint32_t left = lj - kk;  // 1 .. 255, wave-uniform
+    const uint16_t* T = H + kk;
+    if (left <= GC_WAVE) {
+      const uint32_t y = T[min(lane, left - 1)];
+      local += lane < left ? hot16_hit(s_bits, y) : 0u;
+    } else {
+      uint32_t y[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) y[q] = T[min(q * GC_WAVE + lane, left - 1)];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) local += q * GC_WAVE + lane < left ? hot16_hit(s_bits, y[q]) : 0u;
+    }
+  }
+  return local;
+}
+
+// As probe_pair over two 16-bit hot parts.
+__device__ __forceinline__ uint32_t probe16_pair(const uint16_t* H1, uint32_t l1, const uint16_t* H2, uint32_t l2,
+                                                 const uint32_t* s_bits) {
+  const uint32_t lane = lane_id();
+  uint32_t local = 0;
+  uint32_t kk = 0;
+  const uint32_t lmin = min(l1, l2), lmax = max(l1, l2);
+  for (; kk + 4 * GC_WAVE <= lmin; kk += 4 * GC_WAVE) {
+    uint32_t y[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      y[q] = H1[kk + q * GC_WAVE + lane];
+      y[4 + q] = H2[kk + q * GC_WAVE + lane];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) local += hot16_hit(s_bits, y[q]);
+  }
+  for (; kk < lmax; kk += 4 * GC_WAVE) {
+    uint32_t y[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t i = kk + q * GC_WAVE + lane;
+      y[q] = H1[min(i, l1 - 1)];
+      y[4 + q] = H2[min(i, l2 - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t i = kk + q * GC_WAVE + lane;
+      local += i < l1 ? hot16_hit(s_bits, y[q]) : 0u;
+      local += i < l2 ? hot16_hit(s_bits, y[4 + q]) : 0u;
+    }
+  }
+  return local;
+}
+
+// Hits of the cold part L[0, lj) of a list (every entry below hot_base)
+// among the row's cold prefix A[0, c): the Bloom filter, then a search.
+__device__ __forceinline__ uint32_t probe_cold(const uint32_t* L, uint32_t lj, const uint32_t* s_bits,
+                                               const uint32_t* A, uint32_t c) {
+  uint32_t local = 0;
+  for (uint32_t k = lane_id(); k < lj; k += GC_WAVE) {
+    const uint32_t w = L[k];
+    const uint32_t h = bloom_hash(w);
+    if ((s_bits[TCB_BLOOM_BASE + (h >> 5)] >> (h & 31)) & 1u) {
+      const uint32_t p = lower_bound_u32(A, 0, c, w);
+      local += (p < c && A[p] == w) ? 1u : 0u;
+    }
+  }
+  return local;
+}
+
 // Partitions by middle vertex (`parts` > 1): a row u with |L(u)| >= split_min
 // is processed once per part p, and part p walks only the lists L(v) of the
 // v in L(u) with v % parts == p; the parts' sums add up to the same count.
@@ -435,18 +521,15 @@ __device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, 
 // (sharing one line, the parts' atomics serialized: 25 ms instead of 21).
 // (Packing lo and hi into one uint2 array, one 8-byte load per list instead
 // of two 4-byte ones, measured slower: 14.7 -> 15.4 ms at R-MAT scale 22.)
-extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
+template <bool kHot16>
+__device__ __forceinline__ void tc_bitmap_body(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ split, const uint16_t* __restrict__ t16,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
-    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits) {
   const uint32_t positions = n - row_lo;  // rows [row_lo, n)
-  // per wave: the row's hot bitmap, one word that stays zero (for cold ids)
-  // and the Bloom filter of the row's cold prefix
-  __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
-  const uint32_t wave = threadIdx.x / GC_WAVE;
   const uint32_t lane = lane_id();
-  uint32_t* s_bits = s_bits_all[wave];
   for (uint32_t j = lane; j < TCB_BLOOM_BASE + TCB_BLOOM_WORDS; j += GC_WAVE) s_bits[j] = 0;
   wave_lds_sync();
   unsigned long long count = 0;
@@ -526,12 +609,24 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
           uint32_t local = 0;
           for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
             const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
-            uint32_t st = 0, len = 0;
+            // kHot16: [st, st + len) is the hot part of L(v) in t16 and
+            // [cst, cst + clen) its cold part in targets (walked only when
+            // the row has a cold prefix); otherwise all of L(v) in targets
+            uint32_t st = 0, len = 0, cst = 0, clen = 0;
             if (lane < nv) {
               const uint32_t v = A[cv + lane];
               if (whole || v % parts == part) {
-                st = lo[v];
-                len = hi[v] - st;
+                if (kHot16) {
+                  st = split[v];
+                  len = hi[v] - st;
+                  if (c != 0) {
+                    cst = lo[v];
+                    clen = st - cst;
+                  }
+                } else {
+                  st = lo[v];
+                  len = hi[v] - st;
+                }
               }
             }
             // the lanes holding a non-empty list to walk
@@ -540,16 +635,28 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
               const int jv = __ffsll(static_cast<long long>(pending)) - 1;
               pending &= pending - 1;
               // jv is wave-uniform: read the lane's values into scalars
-              const uint32_t* L = targets + static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+              const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
               const uint32_t lj = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
               if (pending != 0) {  // a second list: walk the two in lockstep
                 const int jv2 = __ffsll(static_cast<long long>(pending)) - 1;
                 pending &= pending - 1;
-                const uint32_t* L2 = targets + static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv2));
+                const uint32_t s2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv2));
                 const uint32_t l2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv2));
-                local += probe_pair(L, lj, L2, l2, s_bits, hot_base, A, c);
+                if (kHot16) local += probe16_pair(t16 + s1, lj, t16 + s2, l2, s_bits);
+                else local += probe_pair(targets + s1, lj, targets + s2, l2, s_bits, hot_base, A, c);
               } else {
-                local += probe_list(L, lj, s_bits, hot_base, A, c);
+                if (kHot16) local += probe16_list(t16 + s1, lj, s_bits);
+                else local += probe_list(targets + s1, lj, s_bits, hot_base, A, c);
+              }
+            }
+            if (kHot16 && c != 0) {
+              unsigned long long cold = __ballot(clen != 0);
+              while (cold != 0) {
+                const int jv = __ffsll(static_cast<long long>(cold)) - 1;
+                cold &= cold - 1;
+                const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(cst, jv));
+                const uint32_t lj = static_cast<uint32_t>(__builtin_amdgcn_readlane(clen, jv));
+                local += probe_cold(targets + s1, lj, s_bits, A, c);
               }
             }
           }
@@ -564,6 +671,50 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
   }
   count = wave_sum(count);
   if (lane == 0 && count != 0) atomicAdd(result, count);
+}
+
+extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
+    uint32_t n, uint32_t row_lo, uint32_t hot_base,
+    uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+  // per wave: the row's hot bitmap, one word that stays zero (for cold ids)
+  // and the Bloom filter of the row's cold prefix
+  __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
+  tc_bitmap_body<false>(lo, hi, targets, nullptr, nullptr, n, row_lo, hot_base, heavy, light_chunk, parts,
+                        split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE]);
+}
+
+// As tc_bitmap, with the hot part of every list read as 16-bit offsets from
+// hot_base (t16, index-aligned with targets; split[v] = first entry of L(v)
+// at or above hot_base): half the bytes for the entries that carry almost all
+// of the streaming, no clamp and no cold test per hot entry, and the cold
+// parts walked apart, only for the rows that have a cold prefix.
+extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ split, const uint16_t* __restrict__ t16,
+    uint32_t n, uint32_t row_lo, uint32_t hot_base,
+    uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+  __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
+  tc_bitmap_body<true>(lo, hi, targets, split, t16, n, row_lo, hot_base, heavy, light_chunk, parts,
+                       split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE]);
+}
+
+// The 16-bit hot lists of tc_bitmap16: t16[i] = targets[i] - hot_base for the
+// entries at or above hot_base (0 for the others, never read), and split[v] =
+// the first index of [lo[v], hi[v]) whose target is at or above hot_base.
+extern "C" __global__ __launch_bounds__(256) void tc_hot16_build(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
+    uint32_t n, uint64_t m, uint32_t hot_base, uint32_t* __restrict__ split, uint16_t* __restrict__ t16) {
+  const uint64_t total = m > n ? m : static_cast<uint64_t>(n);
+  for (uint64_t i = global_thread(); i < total; i += grid_threads()) {
+    if (i < m) {
+      const uint32_t w = targets[i];
+      t16[i] = static_cast<uint16_t>(w >= hot_base ? w - hot_base : 0u);
+    }
+    if (i < n) split[i] = lower_bound_u32(targets, lo[i], hi[i], hot_base);
+  }
 }
 
 // Triangle count of a symmetric graph on its degree-ordered relabelling

@@ -95,6 +95,7 @@ struct Device {
   static constexpr size_t kPinnedBytes = 4096;
   // Kernel timing.
   bool profiling = false;
+  std::vector<std::string> profile_only;  // kernels timed when profiling (empty: all)
   std::vector<ProfileRecord> pending;
   std::vector<ProfileTotal> totals;
   std::vector<hipEvent_t> event_pool;
@@ -305,6 +306,9 @@ struct Graph {
   bool ktruss_rev_ready = false;
   DeviceMemory bfs_top;       // u32[4][n] the four highest neighbours of each relabelled row
   bool bfs_top_ready = false;
+  DeviceMemory tc_split;      // u32[n] first DAG entry of each relabelled row in the top 32768 ids
+  DeviceMemory tc_t16;        // u16[m] relabelled targets as offsets from the first of those ids
+  bool tc_hot16_ready = false;
   std::vector<std::pair<std::string, std::unique_ptr<DeviceMemory>>> buffers;
 
   // Named scratch buffer of at least `bytes` bytes (contents unspecified).

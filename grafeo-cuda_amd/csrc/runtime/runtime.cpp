@@ -178,8 +178,11 @@ Kernel get_kernel(Device& dev, const char* family, const char* name) {
 
 // ------------------------------------------------------------- profiling --
 
-void profile_begin(Device& dev, const char* /*name*/, hipEvent_t* start) {
+void profile_begin(Device& dev, const char* name, hipEvent_t* start) {
   if (!dev.profiling) return;
+  if (!dev.profile_only.empty() &&
+      std::find(dev.profile_only.begin(), dev.profile_only.end(), name) == dev.profile_only.end())
+    return;
   hipEvent_t ev;
   if (!dev.event_pool.empty()) {
     ev = dev.event_pool.back();
@@ -436,6 +439,26 @@ gc_status gc_profile_enable(gc_device* handle, int32_t on) {
     gc::profile_collect(*dev);
     dev->totals.clear();
     dev->profiling = on != 0;
+  });
+}
+
+gc_status gc_profile_only(gc_device* handle, const char* names) {
+  return guard([&] {
+    if (handle == nullptr) gc::fail_kernel("device is null");
+    auto* dev = reinterpret_cast<gc::Device*>(handle);
+    std::lock_guard<std::recursive_mutex> lock(dev->mu);
+    dev->profile_only.clear();
+    if (names == nullptr) return;
+    std::string item;
+    for (const char* p = names;; ++p) {
+      if (*p == ',' || *p == '\0') {
+        if (!item.empty()) dev->profile_only.push_back(item);
+        item.clear();
+        if (*p == '\0') break;
+      } else if (*p != ' ') {
+        item.push_back(*p);
+      }
+    }
   });
 }
 

@@ -73,8 +73,11 @@ class GpuDevice:
     def ordinal(self) -> int:
         return self._ordinal
 
-    def profile(self, on: bool = True) -> None:
-        """Starts (or stops) per-kernel event timing and clears accumulated figures."""
+    def profile(self, on: bool = True, only=None) -> None:
+        """Starts (or stops) per-kernel event timing and clears accumulated figures; `only`
+        (kernel names) limits the timing to those kernels, None times every kernel."""
+        names = ",".join(only) if only else ""
+        check(lib.gc_profile_only(self._handle, names.encode()))
         check(lib.gc_profile_enable(self._handle, 1 if on else 0))
 
     def kernel_time(self, kernel: str) -> tuple[float, int]:

@@ -137,6 +137,7 @@ def _load() -> ctypes.CDLL:
         ),
         "gc_profile_enable": (c_int, [c_void_p, c_int32]),
         "gc_profile_query": (c_int, [c_void_p, c_char_p, POINTER(c_double), POINTER(c_uint64)]),
+        "gc_profile_only": (c_int, [c_void_p, c_char_p]),
         "gc_gen_erdos_renyi": (c_int, [c_uint32, c_uint64, c_uint64, POINTER(u32p), POINTER(c_size_t)]),
         "gc_gen_rmat": (
             c_int,
@@ -175,7 +176,7 @@ EXPORTED = [
     "gc_batch_get_config", "gc_batch_tune", "gc_sched_new", "gc_sched_free", "gc_sched_get",
     "gc_sched_set", "gc_sched_predict", "gc_sched_loss_grad", "gc_sched_train",
     "gc_graph_set_scheduler", "gc_graph_last_schedule", "gc_bfs_level_timings",
-    "gc_tc_bucket_timings", "gc_profile_enable", "gc_profile_query", "gc_gen_erdos_renyi", "gc_gen_rmat",
+    "gc_tc_bucket_timings", "gc_profile_enable", "gc_profile_query", "gc_profile_only", "gc_gen_erdos_renyi", "gc_gen_rmat",
     "gc_gen_grid", "gc_code_object_count", "gc_code_object_info",
 ]
 

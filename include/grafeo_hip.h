@@ -371,6 +371,12 @@ gc_status gc_device_copy_to_host(gc_device* dev, void* dst, const void* src, siz
  * times whole calls only (crates/grafeo-cuda-algos/src/harness.rs:13-36). */
 gc_status gc_profile_enable(gc_device* dev, int32_t on);
 
+/* Restricts event timing to the comma-separated kernel `names` (NULL or ""
+ * times every kernel again). Each timed launch adds two event records to the
+ * stream, so a benchmark times only the kernels it reports. No reference
+ * counterpart (as gc_profile_enable). */
+gc_status gc_profile_only(gc_device* dev, const char* names);
+
 /* Total milliseconds and launch count recorded for kernel `name` since the
  * last gc_profile_enable. Waits for outstanding launches. */
 gc_status gc_profile_query(gc_device* dev, const char* name, double* total_ms, uint64_t* launches);

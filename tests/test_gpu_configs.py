@@ -50,12 +50,22 @@ def bfs_is_valid(off, tgt, source, dist):
     return bool(ok and np.all(has_parent[need]))
 
 
-def test_rmat20_suite(device, golden):
+def test_rmat20_suite(device, golden, monkeypatch):
     e = golden["rmat20_ef16_seed1"]
     off, tgt = rmat_graph(20, 16, seed=1)
     assert csr_sha256(off, tgt) == e["csr_sha256"]
     g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
     assert g.triangle_count()["total_triangles"] == e["triangles"]
+    # both list layouts of the bitmap kernel (16-bit hot lists, the default, and 32-bit lists) with
+    # the middle vertices in 1, 3 and 8 parts: at n = 2^20 the rows have cold prefixes (ids below
+    # the top 32768), which the smaller parity graphs do not
+    for hot16 in ("0", "1"):
+        for parts in ("1", "3", "8"):
+            monkeypatch.setenv("GC_TC_HOT16", hot16)
+            monkeypatch.setenv("GC_TC_PARTS", parts)
+            assert g.triangle_count()["total_triangles"] == e["triangles"], f"hot16={hot16} parts={parts}"
+    monkeypatch.delenv("GC_TC_HOT16")
+    monkeypatch.delenv("GC_TC_PARTS")
 
     kt = g.k_truss_numpy(4)
     assert len(kt["edges"]) == e["k_truss_4"]["edges"]
