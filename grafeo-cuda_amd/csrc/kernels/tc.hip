@@ -417,70 +417,81 @@ __device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, 
 }
 
 // ---- 16-bit hot lists (tc_bitmap16) ----
-// A hot entry y = w - hot_base (< TCB_HOT_BITS) is one LDS read and no clamp.
+// The hot part of a list is stored as 32-bit words of two offsets y = w -
+// hot_base (< TCB_HOT_BITS), low half first; an odd-length part ends in the
+// pad offset TCB_HOT_BITS, whose lookup reads the bitmap's zero word. So one
+// load carries two entries, and no entry needs a mask or a clamp.
+#define TCB_PAD16 TCB_HOT_BITS
+
 __device__ __forceinline__ uint32_t hot16_hit(const uint32_t* s_bits, uint32_t y) {
   return word_bit(s_bits[y >> 5], y);
 }
 
-// As probe_list over the 16-bit hot part H[0, lj) of a list.
-__device__ __forceinline__ uint32_t probe16_list(const uint16_t* H, uint32_t lj, const uint32_t* s_bits) {
+__device__ __forceinline__ uint32_t hot16x2_hits(const uint32_t* s_bits, uint32_t x) {
+  return hot16_hit(s_bits, x & 0xFFFFu) + hot16_hit(s_bits, x >> 16);
+}
+
+// As probe_list over the words W[0, nw) of a hot part: whole steps of
+// 4 x 64 words (512 entries, four loads in flight per lane), then one
+// guarded step (word index clamped to the last word, hits masked by word).
+__device__ __forceinline__ uint32_t probe16_list(const uint32_t* W, uint32_t nw, const uint32_t* s_bits) {
   const uint32_t lane = lane_id();
   uint32_t local = 0;
   uint32_t kk = 0;
-  for (; kk + 4 * GC_WAVE <= lj; kk += 4 * GC_WAVE) {
-    uint32_t y[4];
+  for (; kk + 4 * GC_WAVE <= nw; kk += 4 * GC_WAVE) {
+    uint32_t x[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) y[q] = H[kk + q * GC_WAVE + lane];
+    for (int q = 0; q < 4; ++q) x[q] = W[kk + q * GC_WAVE + lane];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) local += hot16_hit(s_bits, y[q]);
+    for (int q = 0; q < 4; ++q) local += hot16x2_hits(s_bits, x[q]);
   }
-  if (kk < lj) {
-    const uint32_t left = lj - kk;  // 1 .. 255, wave-uniform
-    const uint16_t* T = H + kk;
+  if (kk < nw) {
+    const uint32_t left = nw - kk;  // 1 .. 255, wave-uniform
+    const uint32_t* T = W + kk;
     if (left <= GC_WAVE) {
-      const uint32_t y = T[min(lane, left - 1)];
-      local += lane < left ? hot16_hit(s_bits, y) : 0u;
+      const uint32_t x = T[min(lane, left - 1)];
+      local += lane < left ? hot16x2_hits(s_bits, x) : 0u;
     } else {
-      uint32_t y[4];
+      uint32_t x[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) y[q] = T[min(q * GC_WAVE + lane, left - 1)];
+      for (int q = 0; q < 4; ++q) x[q] = T[min(q * GC_WAVE + lane, left - 1)];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) local += q * GC_WAVE + lane < left ? hot16_hit(s_bits, y[q]) : 0u;
+      for (int q = 0; q < 4; ++q) local += q * GC_WAVE + lane < left ? hot16x2_hits(s_bits, x[q]) : 0u;
     }
   }
   return local;
 }
 
-// As probe_pair over two 16-bit hot parts.
-__device__ __forceinline__ uint32_t probe16_pair(const uint16_t* H1, uint32_t l1, const uint16_t* H2, uint32_t l2,
+// As probe_pair over the words of two hot parts.
+__device__ __forceinline__ uint32_t probe16_pair(const uint32_t* W1, uint32_t n1, const uint32_t* W2, uint32_t n2,
                                                  const uint32_t* s_bits) {
   const uint32_t lane = lane_id();
   uint32_t local = 0;
   uint32_t kk = 0;
-  const uint32_t lmin = min(l1, l2), lmax = max(l1, l2);
-  for (; kk + 4 * GC_WAVE <= lmin; kk += 4 * GC_WAVE) {
-    uint32_t y[8];
+  const uint32_t nmin = min(n1, n2), nmax = max(n1, n2);
+  for (; kk + 4 * GC_WAVE <= nmin; kk += 4 * GC_WAVE) {
+    uint32_t x[8];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      y[q] = H1[kk + q * GC_WAVE + lane];
-      y[4 + q] = H2[kk + q * GC_WAVE + lane];
+      x[q] = W1[kk + q * GC_WAVE + lane];
+      x[4 + q] = W2[kk + q * GC_WAVE + lane];
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) local += hot16_hit(s_bits, y[q]);
+    for (int q = 0; q < 8; ++q) local += hot16x2_hits(s_bits, x[q]);
   }
-  for (; kk < lmax; kk += 4 * GC_WAVE) {
-    uint32_t y[8];
+  for (; kk < nmax; kk += 4 * GC_WAVE) {
+    uint32_t x[8];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t i = kk + q * GC_WAVE + lane;
-      y[q] = H1[min(i, l1 - 1)];
-      y[4 + q] = H2[min(i, l2 - 1)];
+      x[q] = W1[min(i, n1 - 1)];
+      x[4 + q] = W2[min(i, n2 - 1)];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t i = kk + q * GC_WAVE + lane;
-      local += i < l1 ? hot16_hit(s_bits, y[q]) : 0u;
-      local += i < l2 ? hot16_hit(s_bits, y[4 + q]) : 0u;
+      local += i < n1 ? hot16x2_hits(s_bits, x[q]) : 0u;
+      local += i < n2 ? hot16x2_hits(s_bits, x[4 + q]) : 0u;
     }
   }
   return local;
@@ -524,7 +535,7 @@ __device__ __forceinline__ uint32_t probe_cold(const uint32_t* L, uint32_t lj, c
 template <bool kHot16>
 __device__ __forceinline__ void tc_bitmap_body(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    const uint32_t* __restrict__ split, const uint16_t* __restrict__ t16,
+    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint32_t* __restrict__ words,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits) {
@@ -609,7 +620,7 @@ __device__ __forceinline__ void tc_bitmap_body(
           uint32_t local = 0;
           for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
             const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
-            // kHot16: [st, st + len) is the hot part of L(v) in t16 and
+            // kHot16: [st, st + len) are the words of L(v)'s hot part and
             // [cst, cst + clen) its cold part in targets (walked only when
             // the row has a cold prefix); otherwise all of L(v) in targets
             uint32_t st = 0, len = 0, cst = 0, clen = 0;
@@ -617,11 +628,12 @@ __device__ __forceinline__ void tc_bitmap_body(
               const uint32_t v = A[cv + lane];
               if (whole || v % parts == part) {
                 if (kHot16) {
-                  st = split[v];
-                  len = hi[v] - st;
+                  const uint2 h = hw[v];
+                  st = h.x;
+                  len = h.y;
                   if (c != 0) {
                     cst = lo[v];
-                    clen = st - cst;
+                    clen = split[v] - cst;
                   }
                 } else {
                   st = lo[v];
@@ -642,10 +654,10 @@ __device__ __forceinline__ void tc_bitmap_body(
                 pending &= pending - 1;
                 const uint32_t s2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv2));
                 const uint32_t l2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv2));
-                if (kHot16) local += probe16_pair(t16 + s1, lj, t16 + s2, l2, s_bits);
+                if (kHot16) local += probe16_pair(words + s1, lj, words + s2, l2, s_bits);
                 else local += probe_pair(targets + s1, lj, targets + s2, l2, s_bits, hot_base, A, c);
               } else {
-                if (kHot16) local += probe16_list(t16 + s1, lj, s_bits);
+                if (kHot16) local += probe16_list(words + s1, lj, s_bits);
                 else local += probe_list(targets + s1, lj, s_bits, hot_base, A, c);
               }
             }
@@ -681,39 +693,60 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
   // per wave: the row's hot bitmap, one word that stays zero (for cold ids)
   // and the Bloom filter of the row's cold prefix
   __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
-  tc_bitmap_body<false>(lo, hi, targets, nullptr, nullptr, n, row_lo, hot_base, heavy, light_chunk, parts,
+  tc_bitmap_body<false>(lo, hi, targets, nullptr, nullptr, nullptr, n, row_lo, hot_base, heavy, light_chunk, parts,
                         split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE]);
 }
 
 // As tc_bitmap, with the hot part of every list read as 16-bit offsets from
-// hot_base (t16, index-aligned with targets; split[v] = first entry of L(v)
-// at or above hot_base): half the bytes for the entries that carry almost all
-// of the streaming, no clamp and no cold test per hot entry, and the cold
-// parts walked apart, only for the rows that have a cold prefix.
+// hot_base, two to a 32-bit word (words; hw[v] = {first word, words} of
+// L(v)'s hot part, split[v] = the first entry of L(v) at or above hot_base):
+// half the bytes and half the loads for the entries that carry almost all of
+// the streaming, no clamp and no cold test per hot entry, and the cold parts
+// walked apart, only for the rows that have a cold prefix.
 extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    const uint32_t* __restrict__ split, const uint16_t* __restrict__ t16,
+    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint32_t* __restrict__ words,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
   __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
-  tc_bitmap_body<true>(lo, hi, targets, split, t16, n, row_lo, hot_base, heavy, light_chunk, parts,
+  tc_bitmap_body<true>(lo, hi, targets, split, hw, words, n, row_lo, hot_base, heavy, light_chunk, parts,
                        split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE]);
 }
 
-// The 16-bit hot lists of tc_bitmap16: t16[i] = targets[i] - hot_base for the
-// entries at or above hot_base (0 for the others, never read), and split[v] =
-// the first index of [lo[v], hi[v]) whose target is at or above hot_base.
-extern "C" __global__ __launch_bounds__(256) void tc_hot16_build(
+// The hot lists of tc_bitmap16, in two passes around an exclusive scan of
+// nwords: tc_hot16_rows sets split[v] (the first index of [lo[v], hi[v]) whose
+// target is at or above hot_base) and nwords[v] = the words of the hot part;
+// tc_hot16_fill writes every hot entry's offset into its row's words (at
+// word woff[v], the scanned nwords), the pad of an odd part, and hw[v].
+extern "C" __global__ __launch_bounds__(256) void tc_hot16_rows(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    uint32_t n, uint64_t m, uint32_t hot_base, uint32_t* __restrict__ split, uint16_t* __restrict__ t16) {
+    uint32_t n, uint32_t hot_base, uint32_t* __restrict__ split, uint32_t* __restrict__ nwords) {
+  for (uint64_t v = global_thread(); v < n; v += grid_threads()) {
+    const uint32_t b = hi[v];
+    const uint32_t sp = lower_bound_u32(targets, lo[v], b, hot_base);
+    split[v] = sp;
+    nwords[v] = (b - sp + 1) >> 1;
+  }
+}
+
+extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n, uint64_t m,
+    uint32_t hot_base, const uint32_t* __restrict__ split, const uint32_t* __restrict__ woff,
+    uint2* __restrict__ hw, uint16_t* __restrict__ half) {
   const uint64_t total = m > n ? m : static_cast<uint64_t>(n);
   for (uint64_t i = global_thread(); i < total; i += grid_threads()) {
     if (i < m) {
-      const uint32_t w = targets[i];
-      t16[i] = static_cast<uint16_t>(w >= hot_base ? w - hot_base : 0u);
+      const uint32_t v = row_of_entry(offsets, n, static_cast<uint32_t>(i));
+      const uint32_t sp = split[v];
+      if (i >= sp && i < offsets[v + 1])
+        half[2ull * woff[v] + (i - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
     }
-    if (i < n) split[i] = lower_bound_u32(targets, lo[i], hi[i], hot_base);
+    if (i < n) {
+      const uint32_t len = offsets[i + 1] - split[i];
+      hw[i] = make_uint2(woff[i], (len + 1) >> 1);
+      if (len & 1u) half[2ull * woff[i] + len] = static_cast<uint16_t>(TCB_PAD16);
+    }
   }
 }
 

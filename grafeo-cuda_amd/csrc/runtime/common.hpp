@@ -306,8 +306,11 @@ struct Graph {
   bool ktruss_rev_ready = false;
   DeviceMemory bfs_top;       // u32[4][n] the four highest neighbours of each relabelled row
   bool bfs_top_ready = false;
-  DeviceMemory tc_split;      // u32[n] first DAG entry of each relabelled row in the top 32768 ids
-  DeviceMemory tc_t16;        // u16[m] relabelled targets as offsets from the first of those ids
+  // The DAG lists' entries among the top 32768 ids as 16-bit offsets, two to
+  // a word, each row from a word boundary (tc_bitmap16).
+  DeviceMemory tc_split;      // u32[n] first such entry of each relabelled row
+  DeviceMemory tc_hw;         // uint2[n] {first word, words} of each row
+  DeviceMemory tc_words;      // u32 words of all rows
   bool tc_hot16_ready = false;
   std::vector<std::pair<std::string, std::unique_ptr<DeviceMemory>>> buffers;
 
