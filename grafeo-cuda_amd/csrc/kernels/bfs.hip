@@ -35,14 +35,17 @@ extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__
   if (global_thread() == 0) queue[0] = source;
 }
 
-// top[k * n + v] = the k-th last entry of row v (k < 4; 0 past the row's start).
+// top[k * n + v] = the k-th last entry of row v (k < 4); past the row's start,
+// v itself: a vertex that bottom-up checks is unvisited, so never in the
+// frontier, and its four candidates need no bound from the row's offsets.
 extern "C" __global__ __launch_bounds__(256) void bfs_top_build(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n,
     uint32_t* __restrict__ top) {
   for (uint64_t v = global_thread(); v < n; v += grid_threads()) {
     const uint32_t b = offsets[v], e = offsets[v + 1];
 #pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) top[k * static_cast<uint64_t>(n) + v] = e - b > k ? targets[e - 1 - k] : 0u;
+    for (uint32_t k = 0; k < 4; ++k)
+      top[k * static_cast<uint64_t>(n) + v] = e - b > k ? targets[e - 1 - k] : static_cast<uint32_t>(v);
   }
 }
 
@@ -73,6 +76,12 @@ __device__ __forceinline__ void block_counters_add(uint32_t count, unsigned long
   }
 }
 
+// A wave takes two chunks of 64 vertices at a time (chunk and chunk + waves),
+// their loads interleaved: dist of both, then the four candidates and row
+// bounds of both, then the candidates' frontier words, so that each wave has
+// twice the loads in flight along the chain (one chunk at a time, the
+// level's launches ran at about 0.6 TB/s on three dependent round trips per
+// chunk).
 extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ top, int32_t* __restrict__ dist, const uint32_t* __restrict__ front,
@@ -86,47 +95,70 @@ extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
   // Chunks below first_chunk hold only vertices of degree 0 (sorted first),
   // which no level ever reaches: their words of `next` are only cleared.
   for (uint64_t c = global_thread(); c < first_chunk; c += grid_threads()) next[c] = 0;
-  for (uint64_t chunk = first_chunk + global_thread() / GC_WAVE; chunk < chunks; chunk += waves) {
-    const uint64_t v64 = chunk * GC_WAVE + lane;
-    bool found = false;
-    uint32_t deg = 0;
-    if (v64 < n) {
-      const uint32_t v = static_cast<uint32_t>(v64);
-      if (dist[v] < 0) {
-        const uint32_t b = offsets[v], e = offsets[v + 1];
-        // the row's four last entries, from the transposed copy
-        const uint32_t first = min(4u, e - b);
-        {
-          uint32_t u[4], word[4];
+  for (uint64_t chunk = first_chunk + global_thread() / GC_WAVE; chunk < chunks; chunk += 2 * waves) {
+    uint64_t ch[2] = {chunk, chunk + waves};
+    uint32_t v[2];
+    bool open[2];
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k) u[k] = k < first ? top[k * static_cast<uint64_t>(n) + v] : 0u;
+    for (int j = 0; j < 2; ++j) {
+      const uint64_t v64 = ch[j] * GC_WAVE + lane;
+      v[j] = static_cast<uint32_t>(v64 < n ? v64 : 0);
+      open[j] = v64 < n;
+    }
+    int32_t d[2];
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k) word[k] = k < first ? front[u[k] >> 5] : 0u;
+    for (int j = 0; j < 2; ++j) d[j] = open[j] ? dist[v[j]] : 0;
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k) found |= k < first && ((word[k] >> (u[k] & 31)) & 1u);
-        }
-        // then four candidates per step, their loads issued together
-        for (uint32_t i = e - first; i > b && !found;) {
+    for (int j = 0; j < 2; ++j) open[j] = open[j] && d[j] < 0;
+    // the four candidates (the row's four last entries, from the transposed
+    // copy) and the row bounds of both vertices
+    uint32_t u[2][4], rb[2], re[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) u[j][k] = open[j] ? top[k * static_cast<uint64_t>(n) + v[j]] : 0u;
+      rb[j] = open[j] ? offsets[v[j]] : 0u;
+      re[j] = open[j] ? offsets[v[j] + 1] : 0u;
+    }
+    uint32_t word[2][4];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) word[j][k] = open[j] ? front[u[j][k] >> 5] : 0u;
+    bool found[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      found[j] = false;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) found[j] |= ((word[j][k] >> (u[j][k] & 31)) & 1u) != 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      // then four candidates per step, their loads issued together
+      const uint32_t b = rb[j], e = re[j];
+      if (open[j] && !found[j]) {
+        for (uint32_t i = e - min(4u, e - b); i > b && !found[j];) {
           const uint32_t cnt = min(4u, i - b);
-          uint32_t u[4], word[4];
+          uint32_t w[4], bits[4];
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k) u[k] = k < cnt ? targets[i - 1 - k] : 0u;
+          for (uint32_t k = 0; k < 4; ++k) w[k] = k < cnt ? targets[i - 1 - k] : 0u;
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k) word[k] = k < cnt ? front[u[k] >> 5] : 0u;
+          for (uint32_t k = 0; k < 4; ++k) bits[k] = k < cnt ? front[w[k] >> 5] : 0u;
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k) found |= k < cnt && ((word[k] >> (u[k] & 31)) & 1u);
+          for (uint32_t k = 0; k < 4; ++k) found[j] |= k < cnt && ((bits[k] >> (w[k] & 31)) & 1u);
           i -= cnt;
         }
-        if (found) {
-          dist[v] = next_level;
-          deg = e - b;
-        }
       }
+      uint32_t deg = 0;
+      if (found[j]) {
+        dist[v[j]] = next_level;
+        deg = e - b;
+      }
+      const uint64_t mask = __ballot(found[j]);
+      if (lane == 0 && ch[j] < chunks) next[ch[j]] = mask;
+      found_count += found[j] ? 1u : 0u;
+      found_edges += deg;
     }
-    const uint64_t mask = __ballot(found);
-    if (lane == 0) next[chunk] = mask;
-    found_count += found ? 1u : 0u;
-    found_edges += deg;
   }
   block_counters_add(found_count, found_edges, counters);
 }
