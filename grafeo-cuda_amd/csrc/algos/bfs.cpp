@@ -64,6 +64,16 @@ struct BfsReadback {
   SchedDecision decision;
 };
 
+// BfsPublished of bfs.hip: a level's counters (and decision) as a kernel
+// writes them to pinned host memory, `seq` last.
+struct BfsPublished {
+  BfsCounters counters;
+  SchedDecision decision;
+  uint32_t seq;
+  uint32_t pad[3];
+};
+static_assert(sizeof(BfsPublished) == 96, "BfsPublished layout");
+
 constexpr double kAlpha = 14.0;
 constexpr double kBeta = 24.0;
 constexpr uint32_t kTopDown = 0, kBottomUp = 1, kChained = 2;
@@ -93,7 +103,9 @@ struct Traversal {
   BfsChainState* chain_state;
   const uint32_t* off;
   const uint32_t* tgt;
-  Kernel k_td, k_deg, k_scan_small, k_bu, k_q2b, k_b2q, k_chain_expand, k_chain_advance, k_decide;
+  BfsPublished* pub;      // pinned host memory the level's publication lands in
+  BfsPublished* pub_dev;  // the same, as kernels address it
+  Kernel k_td, k_deg, k_scan_small, k_bu, k_q2b, k_b2q, k_chain_expand, k_chain_advance, k_decide, k_publish;
   uint32_t cu;
   // state at the start of the next level
   uint64_t n_f = 1, m_f = 0, m_u = 0, reached = 1, prev_n_f = 1;
@@ -125,11 +137,14 @@ struct Traversal {
     k_chain_expand = get_kernel(dev, "bfs", "bfs_chain_expand");
     k_chain_advance = get_kernel(dev, "bfs", "bfs_chain_advance");
     k_decide = get_kernel(dev, "mlp", "sched_bfs_decide");
+    k_publish = get_kernel(dev, "bfs", "bfs_publish");
+    pub = reinterpret_cast<BfsPublished*>(static_cast<char*>(dev.pinned) + Device::kPinnedKernelOffset);
+    pub_dev = reinterpret_cast<BfsPublished*>(static_cast<char*>(dev.pinned_device) + Device::kPinnedKernelOffset);
     cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
     const std::vector<uint32_t>& hoff = g.rl.host_offsets;
     Grid1D vgrid = thread_per_item(n);
     launch(dev, get_kernel(dev, "bfs", "bfs_init"), vgrid.blocks, vgrid.threads, 0, dist, n, src,
-           queue[0]);
+           queue[0], &rb->counters[0]);
     m_f = hoff[src + 1] - hoff[src];
     m_u = static_cast<uint64_t>(g.m) - m_f;
   }
@@ -167,14 +182,35 @@ struct Traversal {
   }
 
   // Launches the decision kernel for the current state (or, with
-  // from_counters, for the state after the level whose counters are in rb).
-  void launch_decide(const float* params, bool from_counters) {
+  // from_counters, for the state after the level whose counters are in rb;
+  // with `publish`, it also publishes them and its decision to `pub` under
+  // sequence number `seq`).
+  void launch_decide(const float* params, bool from_counters, bool publish = false, uint32_t seq = 0) {
     const SchedNorm norm = g.bfs_sched != nullptr ? g.bfs_sched->norm : SchedNorm{};
     // the kernel drops the chained levels itself when the frontier is too large
     const uint32_t mask = (1u << kTopDown) | (g.symmetric ? (1u << kBottomUp) : 0u) | (1u << kChained);
-    launch(dev, k_decide, 1u, 64u, 0, params, norm, level_state(),
-           static_cast<const BfsCounters*>(&rb->counters[0]), from_counters ? 1u : 0u, mask,
-           kChainCap, static_cast<unsigned long long>(kChainEdges), &rb->decision);
+    launch(dev, k_decide, 1u, 64u, 0, params, norm, level_state(), &rb->counters[0], from_counters ? 1u : 0u,
+           mask, kChainCap, static_cast<unsigned long long>(kChainEdges), &rb->decision,
+           publish ? pub_dev : static_cast<BfsPublished*>(nullptr), seq);
+  }
+
+  // Waits until a kernel has published sequence number `seq` to `pub`: the
+  // host polls the sequence word instead of waiting for the stream (about 5
+  // us less per level), and checks the stream every few thousand polls so
+  // that a failed launch is reported instead of waited for.
+  void wait_published(uint32_t seq) {
+    for (uint32_t spins = 1;; ++spins) {
+      if (__atomic_load_n(&pub->seq, __ATOMIC_ACQUIRE) == seq) return;
+      if ((spins & 4095u) == 0) {
+        const hipError_t err = hipStreamQuery(dev.stream);
+        if (err == hipSuccess) {
+          // the stream is idle, so the publishing kernel has ended
+          if (__atomic_load_n(&pub->seq, __ATOMIC_ACQUIRE) == seq) return;
+          fail_kernel("BFS level counters were not published");
+        }
+        if (err != hipErrorNotReady) hip_check(err, "BFS level");
+      }
+    }
   }
 
   void to_queue() {
@@ -212,11 +248,11 @@ struct Traversal {
     const int32_t next_level = level + 1;
     if (choice == kTopDown) {
       to_queue();
+      // (the level counters are clear: bfs_init and every publication clear them)
       if (n_f <= kSmallFrontier) {
         launch(dev, k_scan_small, 1u, 1024u, 0, off, static_cast<const uint32_t*>(queue[cur]),
                static_cast<uint32_t>(n_f), start, &rb->counters[0]);
       } else {
-        fill_words(dev, &rb->counters[0], sizeof(BfsCounters) / 4, 0);
         Grid1D qg = thread_per_item(n_f);
         launch(dev, k_deg, qg.blocks, qg.threads, 0, off, static_cast<const uint32_t*>(queue[cur]),
                static_cast<uint32_t>(n_f), start);
@@ -228,7 +264,6 @@ struct Traversal {
              static_cast<const uint32_t*>(start), static_cast<uint32_t>(m_f), queue[cur ^ 1],
              &rb->counters[0], next_level);
     } else {
-      fill_words(dev, &rb->counters[0], sizeof(BfsCounters) / 4, 0);
       to_bitmap();
       Grid1D bg = wave_per_item(words64);
       // the rows of degree 0 come first in the degree order: skip their chunks
@@ -340,12 +375,17 @@ struct Traversal {
       return;
     }
     launch_level(choice);
-    if (decide_next) launch_decide(params, true);
-    copy_to_host(dev, dev.pinned, rb, sizeof(BfsReadback));
-    BfsReadback host;
-    std::memcpy(&host, dev.pinned, sizeof(host));
-    last_decision = host.decision;
-    finish_level(choice, host.counters[0]);
+    const uint32_t seq = ++dev.publish_seq;
+    if (decide_next) {
+      launch_decide(params, true, true, seq);
+    } else {
+      launch(dev, k_publish, 1u, 64u, 0, &rb->counters[0], pub_dev, seq);
+    }
+    wait_published(seq);
+    BfsCounters counters;
+    std::memcpy(&counters, &pub->counters, sizeof(counters));
+    if (decide_next) std::memcpy(&last_decision, &pub->decision, sizeof(last_decision));
+    finish_level(choice, counters);
   }
 
   // The scheduler's decision for the current state (one round trip).

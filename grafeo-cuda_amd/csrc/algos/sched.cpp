@@ -181,6 +181,7 @@ gc_status gc_sched_set(gc_sched* h, const float* params, const float* mean, cons
     }
     if (mean) std::memcpy(s.norm.mean, mean, sizeof(s.norm.mean));
     if (scale) std::memcpy(s.norm.scale, scale, sizeof(s.norm.scale));
+    ++s.version;
     s.dev->sync();
   });
 }
@@ -224,6 +225,7 @@ gc_status gc_sched_train(gc_sched* h, const float* x, const float* t, const floa
     for (uint32_t i = 0; i < steps; ++i) {
       gc::loss_grad(s, b, count);
       ++s.adam_step;
+      ++s.version;
       gc::launch(dev, adam, (gc::kSchedParams + 255) / 256, 256u, 0, s.params.as<float>(),
                  static_cast<const float*>(s.grad.as<float>()), s.m1.as<float>(), s.m2.as<float>(),
                  s.adam_step, lr, 0.9f, 0.999f, 1e-8f);
@@ -252,6 +254,7 @@ gc_status gc_graph_set_scheduler(gc_graph* gh, gc_sched* bfs, gc_sched* tc) {
     gc::Sched* old_t = call.g->tc_sched;
     call.g->bfs_sched = b;
     call.g->tc_sched = t;
+    call.g->tc_choice_sched = nullptr;  // decisions of another network are stale
     if (old_b != nullptr) gc::sched_release(old_b);
     if (old_t != nullptr) gc::sched_release(old_t);
   });

@@ -253,7 +253,7 @@ void bucket_features(const Graph& g, const std::vector<BucketInfo>& all, size_t 
 }
 
 // Which symmetric-graph method the fixed rule runs (GC_TC_METHOD): "bitmap"
-// (default, tc_bitmap), "hash" (tc_hash), "lists" (binary-search
+// (default, tc_bitmap16 or tc_bitmap), "hash" (tc_hash), "lists" (binary-search
 // intersection, tc_count) or "wedge" (enumeration at the middle vertex,
 // tc_wedge). All four count the same triangles.
 const char* tc_method() {
@@ -297,27 +297,34 @@ uint64_t triangle_count_symmetric(Graph& g) {
       return read_result(dev, result);
     }
     // learned: one decision per degree bucket, adjacent buckets with the same
-    // method merged into one launch
+    // method merged into one launch. The decisions depend on the graph and
+    // the network only, so they are computed once per network version.
     const std::vector<BucketInfo>& buckets = bucket_info(g);
     uint32_t* scratch = tc_scratch(g);
     auto* result = reinterpret_cast<unsigned long long*>(scratch);
     if (buckets.empty()) return 0;
-    std::vector<float> x(buckets.size() * kSchedFeatures), y(buckets.size() * kSchedChoices);
-    for (size_t b = 0; b < buckets.size(); ++b) bucket_features(g, buckets, b, &x[b * kSchedFeatures]);
-    sched_predict(*g.tc_sched, x.data(), static_cast<uint32_t>(buckets.size()), y.data());
-    std::vector<uint32_t> choice(buckets.size());
-    for (size_t b = 0; b < buckets.size(); ++b) {
-      const float* yb = &y[b * kSchedChoices];
-      uint32_t best = 0;
-      for (uint32_t c = 1; c < kSchedChoices; ++c)
-        if (yb[c] < yb[best]) best = c;
-      // Leaving the bitmap for a bucket costs a launch of its own (adjacent
-      // buckets of one method share a launch): only worth it when the
-      // predicted saving exceeds kSwitchUs.
-      if (best != kBitmap && std::exp(yb[kBitmap]) - std::exp(yb[best]) < kSwitchUs) best = kBitmap;
-      choice[b] = best;
-      g.last_schedule.push_back(best);
+    if (g.tc_choice_sched != g.tc_sched || g.tc_choice_version != g.tc_sched->version ||
+        g.tc_choice.size() != buckets.size()) {
+      std::vector<float> x(buckets.size() * kSchedFeatures), y(buckets.size() * kSchedChoices);
+      for (size_t b = 0; b < buckets.size(); ++b) bucket_features(g, buckets, b, &x[b * kSchedFeatures]);
+      sched_predict(*g.tc_sched, x.data(), static_cast<uint32_t>(buckets.size()), y.data());
+      g.tc_choice.assign(buckets.size(), kBitmap);
+      for (size_t b = 0; b < buckets.size(); ++b) {
+        const float* yb = &y[b * kSchedChoices];
+        uint32_t best = 0;
+        for (uint32_t c = 1; c < kSchedChoices; ++c)
+          if (yb[c] < yb[best]) best = c;
+        // Leaving the bitmap for a bucket costs a launch of its own (adjacent
+        // buckets of one method share a launch): only worth it when the
+        // predicted saving exceeds kSwitchUs.
+        if (best != kBitmap && std::exp(yb[kBitmap]) - std::exp(yb[best]) < kSwitchUs) best = kBitmap;
+        g.tc_choice[b] = best;
+      }
+      g.tc_choice_sched = g.tc_sched;
+      g.tc_choice_version = g.tc_sched->version;
     }
+    const std::vector<uint32_t>& choice = g.tc_choice;
+    g.last_schedule = choice;
     uint32_t launches = 0;
     for (size_t b = 0; b < buckets.size();) {
       size_t e = b + 1;

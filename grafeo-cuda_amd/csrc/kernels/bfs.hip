@@ -29,10 +29,42 @@ struct BfsCounters {
   unsigned long long next_edges;
 };
 
+// The level's counters as the driver reads them: written to pinned host
+// memory by bfs_publish (or by the scheduler's decision kernel, which adds
+// its decision), `seq` last, which the driver polls instead of waiting for
+// the stream and copying.
+struct BfsPublished {
+  BfsCounters counters;
+  uint32_t decision[16];  // the decision kernel's SchedDecision
+  uint32_t seq;
+  uint32_t pad[3];
+};
+
+// Also clears the level counters, which every level's publication clears
+// again for the next one.
 extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__ dist, uint32_t n,
-                                                           uint32_t source, uint32_t* __restrict__ queue) {
+                                                           uint32_t source, uint32_t* __restrict__ queue,
+                                                           BfsCounters* __restrict__ counters) {
   for (uint64_t v = global_thread(); v < n; v += grid_threads()) dist[v] = v == source ? 0 : -1;
-  if (global_thread() == 0) queue[0] = source;
+  if (global_thread() == 0) {
+    queue[0] = source;
+    counters->next_size = 0;
+    counters->next_edges = 0;
+  }
+}
+
+// Publishes the level's counters to the driver (one lane): copies them to
+// `host`, clears them for the next level, then sets host->seq = seq after a
+// system-scope fence, so the counters are visible before the sequence word.
+extern "C" __global__ __launch_bounds__(64) void bfs_publish(BfsCounters* __restrict__ counters,
+                                                             BfsPublished* __restrict__ host, uint32_t seq) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const BfsCounters c = *counters;
+  host->counters = c;
+  counters->next_size = 0;
+  counters->next_edges = 0;
+  __threadfence_system();
+  __hip_atomic_store(&host->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // top[k * n + v] = the k-th last entry of row v (k < 4); past the row's start,

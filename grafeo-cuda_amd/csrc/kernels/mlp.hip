@@ -195,6 +195,15 @@ struct SchedDecision {
   uint32_t choice;
 };
 
+// BfsPublished of bfs.hip: the level counters, this decision, and the
+// sequence word the driver polls.
+struct BfsPublishedView {
+  BfsCountersView counters;
+  SchedDecision decision;
+  uint32_t seq;
+  uint32_t pad[3];
+};
+
 __device__ __forceinline__ float lg(double x) { return static_cast<float>(log2(x + 1.0)); }
 
 // Features of the level about to run, from `st`, or - with use_counters - from
@@ -202,11 +211,14 @@ __device__ __forceinline__ float lg(double x) { return static_cast<float>(log2(x
 // null only the features are written; otherwise the allowed choice
 // (allowed_mask bit c; choice 2 also needs n_f <= chain_cap and m_f <=
 // chain_edge_cap) with the
-// smallest predicted log-time.
+// smallest predicted log-time. With `host` (and use_counters), lane 0 also
+// publishes the counters and the decision there as bfs_publish does, clears
+// the counters for the next level and sets host->seq = seq last.
 extern "C" __global__ __launch_bounds__(64) void sched_bfs_decide(
     const float* __restrict__ params, MlpNorm norm, BfsLevelState st,
-    const BfsCountersView* __restrict__ counters, uint32_t use_counters, uint32_t allowed_mask,
-    uint32_t chain_cap, unsigned long long chain_edge_cap, SchedDecision* __restrict__ out) {
+    BfsCountersView* __restrict__ counters, uint32_t use_counters, uint32_t allowed_mask,
+    uint32_t chain_cap, unsigned long long chain_edge_cap, SchedDecision* __restrict__ out,
+    BfsPublishedView* __restrict__ host, uint32_t seq) {
   if (blockIdx.x != 0) return;
   const uint32_t lane = lane_id();
   // every lane derives the same state and features
@@ -272,4 +284,14 @@ extern "C" __global__ __launch_bounds__(64) void sched_bfs_decide(
     }
   }
   out->choice = best == 0xFFFFFFFFu ? 0u : best;
+  if (host != nullptr && use_counters) {
+    host->counters = *counters;
+    for (int i = 0; i < MLP_IN; ++i) host->decision.features[i] = f[i];
+    for (int k = 0; k < MLP_OUT; ++k) host->decision.predicted[k] = y[k];
+    host->decision.choice = best == 0xFFFFFFFFu ? 0u : best;
+    counters->next_size = 0;
+    counters->next_edges = 0;
+    __threadfence_system();
+    __hip_atomic_store(&host->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }

@@ -92,7 +92,12 @@ struct Device {
   std::recursive_mutex mu;  // serializes calls on this device
   // Small pinned staging area for scalar read-backs (counters, states).
   void* pinned = nullptr;
+  void* pinned_device = nullptr;  // the same buffer's address for kernels
   static constexpr size_t kPinnedBytes = 4096;
+  // Bytes [kPinnedKernelOffset, kPinnedBytes) are written by kernels (BFS
+  // level counters, polled by the host); the bytes below by read-back copies.
+  static constexpr size_t kPinnedKernelOffset = 2048;
+  uint32_t publish_seq = 0;  // last sequence number given to a publishing kernel
   // Kernel timing.
   bool profiling = false;
   std::vector<std::string> profile_only;  // kernels timed when profiling (empty: all)
@@ -251,6 +256,7 @@ struct Sched {
   DeviceMemory grad;      // f32[kSchedParams] + loss
   SchedNorm norm{};
   uint32_t adam_step = 0;
+  uint64_t version = 0;  // incremented whenever params or norm change
 };
 
 void sched_release(Sched* s);
@@ -312,6 +318,11 @@ struct Graph {
   DeviceMemory tc_hw;         // uint2[n] {first word, words} of each row
   DeviceMemory tc_words;      // u32 words of all rows
   bool tc_hot16_ready = false;
+  // The learned triangle-count schedule (method per degree bucket): a function
+  // of the graph and the network, kept with the network and version it came from.
+  std::vector<uint32_t> tc_choice;
+  const Sched* tc_choice_sched = nullptr;
+  uint64_t tc_choice_version = 0;
   std::vector<std::pair<std::string, std::unique_ptr<DeviceMemory>>> buffers;
 
   // Named scratch buffer of at least `bytes` bytes (contents unspecified).

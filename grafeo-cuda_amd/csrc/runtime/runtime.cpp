@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 
 #include "code_objects.hpp"
@@ -357,6 +358,8 @@ gc_status gc_device_new(uint32_t ordinal, gc_device** out) {
     gc::hip_check(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking), "hipStreamCreate");
     gc::hip_check(hipHostMalloc(&dev->pinned, gc::Device::kPinnedBytes, hipHostMallocDefault),
                   "hipHostMalloc");
+    std::memset(dev->pinned, 0, gc::Device::kPinnedBytes);
+    gc::hip_check(hipHostGetDevicePointer(&dev->pinned_device, dev->pinned, 0), "hipHostGetDevicePointer");
     *out = reinterpret_cast<gc_device*>(dev.release());
   });
 }
