@@ -67,10 +67,10 @@ bool tc_hot16() {
   return env == nullptr || env[0] != '0';
 }
 
-// The 16-bit hot lists of tc_bitmap16, built once per graph: at most
-// (m + n) / 2 + 1 words, and 12 bytes per vertex.
-void ensure_hot16(Graph& g, uint32_t hot_base) {
-  if (g.tc_hot16_ready) return;
+// The 16-bit hot lists of tc_bitmap16, built once per graph (again if the
+// parts change): at most (m + n) / 2 + 1 words, and 12 bytes per vertex.
+void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts) {
+  if (g.tc_hot16_ready && g.tc_hot16_parts == parts) return;
   Device& dev = *g.dev;
   const uint32_t n = g.n;
   if (n == 0) return;  // no rows, nothing to count
@@ -88,21 +88,24 @@ void ensure_hot16(Graph& g, uint32_t hot_base) {
   const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
   Grid1D vg = thread_per_item(rows);
   launch(dev, get_kernel(dev, "tc", "tc_hot16_rows"), vg.blocks, vg.threads, 0, g.rl.fwd.as<const uint32_t>(),
-         offsets + 1, tgt, n, hot_base, g.tc_split.as<uint32_t>(), woff.as<uint32_t>());
+         offsets + 1, tgt, n, hot_base, parts, g.tc_split.as<uint32_t>(), woff.as<uint32_t>());
   exclusive_scan_u32(dev, woff.as<uint32_t>(), n, scan.as<uint32_t>());
   Grid1D eg = thread_per_item(std::max<uint64_t>(m, n));
   launch(dev, get_kernel(dev, "tc", "tc_hot16_fill"), eg.blocks, eg.threads, 0, offsets, tgt, n, m, hot_base,
-         static_cast<const uint32_t*>(g.tc_split.as<uint32_t>()), static_cast<const uint32_t*>(woff.as<uint32_t>()),
+         parts, static_cast<const uint32_t*>(g.tc_split.as<uint32_t>()), static_cast<const uint32_t*>(woff.as<uint32_t>()),
          g.tc_hw.as<uint32_t>() /* uint2[n] */, g.tc_words.as<uint16_t>());
   dev.sync();  // woff and scan are released on return
   g.tc_hot16_ready = true;
+  g.tc_hot16_parts = parts;
 }
 
 // Shortest L(u) that tc_bitmap splits over the parts (shorter rows run whole,
-// once): GC_TC_SPLIT, default 64.
+// once): GC_TC_SPLIT, default 256. With the 16-bit lists, R-MAT scale 22
+// counted in 8.92 ms at 256 against 9.13 at 64 (scale 24: 86.0 against
+// 87.3 ms; scale 20 unchanged): a split row builds its bitmap once per part.
 uint32_t tc_split_min() {
   const char* env = std::getenv("GC_TC_SPLIT");
-  if (env == nullptr) return 64;
+  if (env == nullptr) return 256;
   return static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
 }
 
@@ -155,7 +158,7 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     const uint64_t tasks =
         parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
     if (tc_hot16()) {
-      ensure_hot16(g, hot_base);
+      ensure_hot16(g, hot_base, parts);
       launch(dev, get_kernel(dev, "tc", "tc_bitmap16"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt,
              g.tc_split.as<const uint32_t>(), g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>(), row_hi,
              row_lo, hot_base, single,

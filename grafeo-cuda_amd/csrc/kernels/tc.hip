@@ -524,7 +524,8 @@ __device__ __forceinline__ uint32_t probe_cold(const uint32_t* L, uint32_t lj, c
 // to 52% and the kernel from 35 to 18 ms). A shorter row is processed whole, once, by part u % parts:
 // building its bitmap once per part costs more than its streaming saves (at
 // scale 22 the rows with |L(u)| < 64 are 2.1 M of the 2.4 M non-empty rows
-// and 22% of the streaming). Every part has its own pair of work counters
+// and 22% of the streaming; with the 16-bit lists the best threshold is
+// 256). Every part has its own pair of work counters
 // over all rows; a wave that finds its part drained takes rows of the others,
 // so an uneven part does not leave XCDs idle (where a row runs changes its
 // speed, never its count). Part p's counters are
@@ -716,23 +717,31 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
 
 // The hot lists of tc_bitmap16, in two passes around an exclusive scan of
 // nwords: tc_hot16_rows sets split[v] (the first index of [lo[v], hi[v]) whose
-// target is at or above hot_base) and nwords[v] = the words of the hot part;
-// tc_hot16_fill writes every hot entry's offset into its row's words (at
-// word woff[v], the scanned nwords), the pad of an odd part, and hw[v].
+// target is at or above hot_base) and the words of the hot part; tc_hot16_fill
+// writes every hot entry's offset into its row's words (from the scanned
+// word offset), the pad of an odd part, and hw[v]. The rows are laid out part
+// by part (part v % parts, then v), in the middle-vertex parts of tc_bitmap16:
+// each part's lists are then contiguous, and an XCD's L2 fetches no line
+// shared with another part's lists. part_slot gives row v's place there.
+__device__ __forceinline__ uint32_t part_slot(uint32_t v, uint32_t n, uint32_t parts) {
+  const uint32_t p = v % parts;
+  return p * (n / parts) + min(p, n % parts) + v / parts;
+}
+
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_rows(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    uint32_t n, uint32_t hot_base, uint32_t* __restrict__ split, uint32_t* __restrict__ nwords) {
+    uint32_t n, uint32_t hot_base, uint32_t parts, uint32_t* __restrict__ split, uint32_t* __restrict__ nwords) {
   for (uint64_t v = global_thread(); v < n; v += grid_threads()) {
     const uint32_t b = hi[v];
     const uint32_t sp = lower_bound_u32(targets, lo[v], b, hot_base);
     split[v] = sp;
-    nwords[v] = (b - sp + 1) >> 1;
+    nwords[part_slot(static_cast<uint32_t>(v), n, parts)] = (b - sp + 1) >> 1;
   }
 }
 
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n, uint64_t m,
-    uint32_t hot_base, const uint32_t* __restrict__ split, const uint32_t* __restrict__ woff,
+    uint32_t hot_base, uint32_t parts, const uint32_t* __restrict__ split, const uint32_t* __restrict__ woff,
     uint2* __restrict__ hw, uint16_t* __restrict__ half) {
   const uint64_t total = m > n ? m : static_cast<uint64_t>(n);
   for (uint64_t i = global_thread(); i < total; i += grid_threads()) {
@@ -740,12 +749,14 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
       const uint32_t v = row_of_entry(offsets, n, static_cast<uint32_t>(i));
       const uint32_t sp = split[v];
       if (i >= sp && i < offsets[v + 1])
-        half[2ull * woff[v] + (i - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
+        half[2ull * woff[part_slot(v, n, parts)] + (i - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
     }
     if (i < n) {
-      const uint32_t len = offsets[i + 1] - split[i];
-      hw[i] = make_uint2(woff[i], (len + 1) >> 1);
-      if (len & 1u) half[2ull * woff[i] + len] = static_cast<uint16_t>(TCB_PAD16);
+      const uint32_t v = static_cast<uint32_t>(i);
+      const uint32_t w = woff[part_slot(v, n, parts)];
+      const uint32_t len = offsets[v + 1] - split[v];
+      hw[v] = make_uint2(w, (len + 1) >> 1);
+      if (len & 1u) half[2ull * w + len] = static_cast<uint16_t>(TCB_PAD16);
     }
   }
 }

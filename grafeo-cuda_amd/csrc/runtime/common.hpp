@@ -316,8 +316,9 @@ struct Graph {
   // a word, each row from a word boundary (tc_bitmap16).
   DeviceMemory tc_split;      // u32[n] first such entry of each relabelled row
   DeviceMemory tc_hw;         // uint2[n] {first word, words} of each row
-  DeviceMemory tc_words;      // u32 words of all rows
+  DeviceMemory tc_words;      // u32 words of all rows, part by part
   bool tc_hot16_ready = false;
+  uint32_t tc_hot16_parts = 0;  // the middle-vertex parts the words are grouped by
   // The learned triangle-count schedule (method per degree bucket): a function
   // of the graph and the network, kept with the network and version it came from.
   std::vector<uint32_t> tc_choice;
