@@ -185,7 +185,8 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   // occupancy), each holding the hot set in half of the CU's LDS less 2 KB
   // for the reduction scratch: 19,456 ids (76 KB) on a 160 KB CU, which at
   // R-MAT scale 22 serve 42% of the gathers. One workgroup per CU with twice
-  // the hot set (16 waves) measured slower: 14.2 ms against 11.3 ms.
+  // the hot set (16 waves) measured slower: 14.2 ms against 11.3 ms, and
+  // still 9.64 against 9.33 ms with sixteen loads in flight per lane.
   // GC_PR_HOT overrides the hot-set size.
   const uint32_t lds = static_cast<uint32_t>(std::min<size_t>(dev.max_lds_per_block, 160 * 1024));
   uint32_t hot_count = lds >= 8 * 1024 ? ((lds / 2 - 2048) / 4) & ~1023u : 1024u;
@@ -205,7 +206,7 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
                                                              : virtual_blocks)
                                      : hot_grid(hot_count);
   // partial sums: one pair per spmv workgroup, then one per pr_hub_finish workgroup
-  const uint32_t partial_blocks = rows_kernel ? grid_blocks + finish_blocks : grid_blocks;
+  uint32_t partial_blocks = rows_kernel ? grid_blocks + finish_blocks : grid_blocks;
   float* rank[2] = {g.buffer<float>("pr.rank0", sizeof(float) * n),
                     g.buffer<float>("pr.rank1", sizeof(float) * n)};
   float* contrib[2] = {g.buffer<float>("pr.contrib0", sizeof(float) * n),
@@ -258,6 +259,7 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
         hot_count /= 2;
         dev.pr_hot_count_limit = hot_count;
         grid_blocks = hot_grid(hot_count);
+        partial_blocks = grid_blocks;  // one partial pair per workgroup of the new grid
       }
     }
     launch(dev, finalize, kPrFinishGroups, 256u, 0, state, static_cast<const double*>(partials),
