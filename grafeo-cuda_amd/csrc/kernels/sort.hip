@@ -117,6 +117,85 @@ extern "C" __global__ __launch_bounds__(256) void csr_make_keys(
   if (lane_id() == 0 && local_bad) atomicAdd(bad, local_bad);
 }
 
+// ---- oriented CSR from edge pairs -------------------------------------------------
+// (the steps of from_edge_pairs_oriented, crates/grafeo-cuda-core/src/csr_export.rs:299-350)
+
+// Canonical undirected keys min << id_bits | max, one per pair (a loop keeps
+// its key min == max and is dropped by csr_unique_edges); `bad` counts pairs
+// with an endpoint >= n.
+extern "C" __global__ __launch_bounds__(256) void csr_undirected_keys(
+    const uint32_t* __restrict__ pairs, uint64_t num_pairs, uint32_t n, uint32_t id_bits,
+    unsigned long long* __restrict__ keys, uint32_t* __restrict__ bad) {
+  uint32_t local_bad = 0;
+  for (uint64_t p = global_thread(); p < num_pairs; p += grid_threads()) {
+    const uint32_t s = pairs[2 * p], d = pairs[2 * p + 1];
+    if (s >= n || d >= n) ++local_bad;
+    const unsigned long long lo = min(s, d), hi = max(s, d);
+    keys[p] = (lo << id_bits) | hi;
+  }
+  local_bad = wave_sum(local_bad);
+  if (lane_id() == 0 && local_bad) atomicAdd(bad, local_bad);
+}
+
+// flags[i] = 1 for the first copy of each sorted undirected key that is not a
+// loop (its two ids differ).
+extern "C" __global__ __launch_bounds__(256) void csr_unique_edges(
+    const unsigned long long* __restrict__ keys, uint64_t count, uint32_t id_bits, uint8_t* __restrict__ flags) {
+  const unsigned long long mask = (1ull << id_bits) - 1ull;
+  for (uint64_t i = global_thread(); i < count; i += grid_threads()) {
+    const unsigned long long k = keys[i];
+    const bool first = i == 0 || k != keys[i - 1];
+    flags[i] = (first && (k >> id_bits) != (k & mask)) ? 1 : 0;
+  }
+}
+
+// degree[x] += the unique undirected edges at x (degree cleared by the
+// caller). The keys are sorted, so the lower ids come in runs: each run adds
+// its length with one atomic (a hub's run would otherwise serialize on one
+// address); the higher ids add one each.
+extern "C" __global__ __launch_bounds__(256) void csr_edge_degrees(
+    const unsigned long long* __restrict__ keys, uint64_t count, uint32_t id_bits,
+    uint32_t* __restrict__ degree) {
+  const unsigned long long mask = (1ull << id_bits) - 1ull;
+  const uint32_t lane = lane_id();
+  const uint64_t waves = grid_threads() / GC_WAVE;
+  for (uint64_t base = (global_thread() / GC_WAVE) * GC_WAVE; base < count; base += waves * GC_WAVE) {
+    const uint64_t i = base + lane;
+    const bool valid = i < count;
+    const unsigned long long k = valid ? keys[i] : 0ull;
+    const uint32_t a = static_cast<uint32_t>(k >> id_bits);
+    const uint32_t b = static_cast<uint32_t>(k & mask);
+    const uint32_t prev_a = __shfl_up(a, 1);
+    // heads of the runs of equal lower ids within the wave
+    const bool head = valid && (lane == 0 || prev_a != a);
+    const unsigned long long heads = __ballot(head);
+    if (head) {
+      // the run ends at the next head or at the last valid lane
+      const unsigned long long later = lane == GC_WAVE - 1 ? 0ull : heads >> (lane + 1);
+      const uint64_t valid_lanes = count - base < GC_WAVE ? count - base : GC_WAVE;
+      const uint32_t end = later != 0 ? lane + 1 + static_cast<uint32_t>(__ffsll(static_cast<long long>(later)) - 1)
+                                      : static_cast<uint32_t>(valid_lanes);
+      atomicAdd(&degree[a], end - lane);
+    }
+    if (valid) atomicAdd(&degree[b], 1u);
+  }
+}
+
+// Each unique undirected edge (a, b), a < b, oriented from the endpoint of
+// lower degree to the higher, ties from the lower id: a << id_bits | b or
+// b << id_bits | a.
+extern "C" __global__ __launch_bounds__(256) void csr_orient_keys(
+    const unsigned long long* __restrict__ keys, uint64_t count, uint32_t id_bits,
+    const uint32_t* __restrict__ degree, unsigned long long* __restrict__ out) {
+  const unsigned long long mask = (1ull << id_bits) - 1ull;
+  for (uint64_t i = global_thread(); i < count; i += grid_threads()) {
+    const unsigned long long k = keys[i];
+    const unsigned long long a = k >> id_bits, b = k & mask;
+    const uint32_t da = degree[a], db = degree[b];
+    out[i] = (da < db || (da == db && a < b)) ? k : ((b << id_bits) | a);
+  }
+}
+
 // flags[i] = 1 when sorted key i differs from key i-1 (first copies of each key).
 extern "C" __global__ __launch_bounds__(256) void csr_first_copies(
     const unsigned long long* __restrict__ keys, uint64_t count, uint8_t* __restrict__ flags) {

@@ -122,6 +122,97 @@ void build_csr_on_device(Device& dev, const uint32_t* pairs, size_t num_pairs, u
   dev.sync();
 }
 
+// Compacts the flagged keys of sorted[0, count) in order into `out` and
+// returns how many there are (tiles of kTile entries: compact_count and
+// exclusive_scan_small of util.hip, csr_compact_keys of sort.hip).
+uint64_t compact_flagged(Device& dev, const unsigned long long* sorted, uint64_t count, const uint8_t* flags,
+                         unsigned long long* out) {
+  if (count == 0) return 0;
+  const uint64_t tiles = (count + kTile - 1) / kTile;
+  DeviceMemory tile_counts;
+  tile_counts.allocate(sizeof(uint32_t) * (tiles + 1));
+  launch(dev, get_kernel(dev, "util", "compact_count"), static_cast<uint32_t>(tiles), 256u, 0, flags, count,
+         tile_counts.as<uint32_t>());
+  launch(dev, get_kernel(dev, "util", "exclusive_scan_small"), 1u, 1024u, 0, tile_counts.as<uint32_t>(),
+         static_cast<uint32_t>(tiles), tile_counts.as<uint32_t>() + tiles);
+  uint32_t total = 0;
+  copy_to_host(dev, &total, tile_counts.as<uint32_t>() + tiles, sizeof(uint32_t));
+  launch(dev, get_kernel(dev, "sort", "csr_compact_keys"), static_cast<uint32_t>(tiles), 256u, 0, sorted, flags,
+         count, static_cast<const uint32_t*>(tile_counts.as<uint32_t>()), out);
+  return total;
+}
+
+// Builds the oriented CSR of from_edge_pairs_oriented
+// (crates/grafeo-cuda-core/src/csr_export.rs:299-350) on the device: canonical
+// (min, max) keys, sorted, first copies of the non-loop keys kept, undirected
+// degrees, each edge oriented from the lower degree to the higher (ties from
+// the lower id), sorted again, offsets by binary search.
+void build_oriented_csr_on_device(Device& dev, const uint32_t* pairs, size_t num_pairs, uint32_t n,
+                                  DeviceCsr& out) {
+  const unsigned id_bits = std::max(1u, bits_of(n == 0 ? 0 : n - 1));
+  const uint64_t slots = std::max<uint64_t>(num_pairs, 1);
+  DeviceMemory d_pairs, keys_a, keys_b, bad;
+  d_pairs.allocate(sizeof(uint32_t) * 2 * slots);
+  keys_a.allocate(sizeof(unsigned long long) * slots);
+  keys_b.allocate(sizeof(unsigned long long) * slots);
+  bad.allocate(16);
+  fill_words(dev, bad.get(), 1, 0);
+  if (num_pairs > 0) copy_to_device(dev, d_pairs.get(), pairs, sizeof(uint32_t) * 2 * num_pairs);
+  Grid1D pg = thread_per_item(slots);
+  launch(dev, get_kernel(dev, "sort", "csr_undirected_keys"), pg.blocks, pg.threads, 0,
+         static_cast<const uint32_t*>(d_pairs.as<uint32_t>()), static_cast<uint64_t>(num_pairs), n, id_bits,
+         keys_a.as<unsigned long long>(), bad.as<uint32_t>());
+  uint32_t bad_count = 0;
+  copy_to_host(dev, &bad_count, bad.get(), sizeof(uint32_t));
+  if (bad_count > 0) {
+    for (size_t i = 0; i < num_pairs; ++i) {
+      const uint32_t s = pairs[2 * i], d = pairs[2 * i + 1];
+      if (s >= n || d >= n) {
+        fail_csr("edge (" + std::to_string(s) + ", " + std::to_string(d) + ") has a node id >= num_nodes " +
+                 std::to_string(n));
+      }
+    }
+  }
+  d_pairs.release();
+  unsigned long long* sorted =
+      radix_sort(dev, keys_a.as<unsigned long long>(), keys_b.as<unsigned long long>(), num_pairs, 2 * id_bits);
+  unsigned long long* spare = sorted == keys_a.as<unsigned long long>() ? keys_b.as<unsigned long long>()
+                                                                        : keys_a.as<unsigned long long>();
+  uint64_t unique = 0;
+  if (num_pairs > 0) {
+    DeviceMemory flags;
+    flags.allocate(std::max<uint64_t>(num_pairs, 4));
+    Grid1D kg = thread_per_item(num_pairs);
+    launch(dev, get_kernel(dev, "sort", "csr_unique_edges"), kg.blocks, kg.threads, 0,
+           static_cast<const unsigned long long*>(sorted), static_cast<uint64_t>(num_pairs), id_bits,
+           flags.as<uint8_t>());
+    unique = compact_flagged(dev, sorted, num_pairs, flags.as<uint8_t>(), spare);
+  }
+  if (unique > 0xFFFFFFFFull) fail_csr("graph too large for u32 indices: " + std::to_string(unique) + " targets");
+  unsigned long long* oriented = sorted;  // free again: the unique keys are in `spare`
+  if (unique > 0) {
+    DeviceMemory degree;
+    degree.allocate(sizeof(uint32_t) * std::max<uint64_t>(n, 1));
+    fill_words(dev, degree.get(), std::max<uint64_t>(n, 1), 0);
+    Grid1D ug = thread_per_item(unique);
+    launch(dev, get_kernel(dev, "sort", "csr_edge_degrees"), ug.blocks, ug.threads, 0,
+           static_cast<const unsigned long long*>(spare), unique, id_bits, degree.as<uint32_t>());
+    launch(dev, get_kernel(dev, "sort", "csr_orient_keys"), ug.blocks, ug.threads, 0,
+           static_cast<const unsigned long long*>(spare), unique, id_bits,
+           static_cast<const uint32_t*>(degree.as<uint32_t>()), oriented);
+    oriented = radix_sort(dev, oriented, spare, unique, 2 * id_bits);
+  }
+  out.n = n;
+  out.m = static_cast<uint32_t>(unique);
+  out.offsets.allocate(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  out.targets.allocate(sizeof(uint32_t) * std::max<uint64_t>(unique, 1));
+  Grid1D cg = thread_per_item(std::max<uint64_t>(unique, static_cast<uint64_t>(n) + 1));
+  launch(dev, get_kernel(dev, "sort", "csr_from_keys"), cg.blocks, cg.threads, 0,
+         static_cast<const unsigned long long*>(oriented), unique, n, static_cast<uint32_t>(id_bits),
+         out.targets.as<uint32_t>(), out.offsets.as<uint32_t>());
+  dev.sync();
+}
+
 }  // namespace gc
 
 using gc::guard;
@@ -140,6 +231,34 @@ gc_status gc_csr_from_edge_pairs_device(gc_device* dev_handle, const uint32_t* p
     dev->make_current();
     gc::DeviceCsr csr;
     gc::build_csr_on_device(*dev, pairs, num_pairs, num_nodes, symmetric != 0, csr);
+    auto* offsets = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * (size_t(num_nodes) + 1)));
+    auto* targets = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<size_t>(1, csr.m)));
+    if (!offsets || !targets) {
+      std::free(offsets);
+      std::free(targets);
+      throw std::bad_alloc();
+    }
+    gc::copy_to_host(*dev, offsets, csr.offsets.get(), sizeof(uint32_t) * (size_t(num_nodes) + 1));
+    if (csr.m > 0) gc::copy_to_host(*dev, targets, csr.targets.get(), sizeof(uint32_t) * csr.m);
+    out->offsets = offsets;
+    out->targets = targets;
+    out->num_nodes = num_nodes;
+    out->num_edges = csr.m;
+  });
+}
+
+gc_status gc_csr_from_edge_pairs_oriented_device(gc_device* dev_handle, const uint32_t* pairs, size_t num_pairs,
+                                                 uint32_t num_nodes, gc_csr* out) {
+  return guard([&] {
+    if (out == nullptr) gc::fail_kernel("out is null");
+    std::memset(out, 0, sizeof(*out));
+    if (dev_handle == nullptr) gc::fail_kernel("device is null");
+    if (pairs == nullptr && num_pairs > 0) gc::fail_kernel("pairs is null");
+    auto* dev = reinterpret_cast<gc::Device*>(dev_handle);
+    std::lock_guard<std::recursive_mutex> lock(dev->mu);
+    dev->make_current();
+    gc::DeviceCsr csr;
+    gc::build_oriented_csr_on_device(*dev, pairs, num_pairs, num_nodes, csr);
     auto* offsets = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * (size_t(num_nodes) + 1)));
     auto* targets = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<size_t>(1, csr.m)));
     if (!offsets || !targets) {

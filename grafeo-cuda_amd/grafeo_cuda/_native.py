@@ -68,6 +68,7 @@ def _load() -> ctypes.CDLL:
         "gc_csr_validate": (c_int, [POINTER(Csr)]),
         "gc_csr_from_edge_pairs": (c_int, [u32p, c_size_t, c_uint32, c_int, POINTER(Csr)]),
         "gc_csr_from_edge_pairs_device": (c_int, [c_void_p, u32p, c_size_t, c_uint32, c_int, POINTER(Csr)]),
+        "gc_csr_from_edge_pairs_oriented_device": (c_int, [c_void_p, u32p, c_size_t, c_uint32, POINTER(Csr)]),
         "gc_graph_from_edge_pairs": (c_int, [c_void_p, u32p, c_size_t, c_uint32, c_int, POINTER(c_void_p)]),
         "gc_graph_relabelled": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p),
                                         POINTER(c_void_p)]),
@@ -165,7 +166,8 @@ EXPORTED = [
     "gc_last_error", "gc_free", "gc_device_count", "gc_device_new", "gc_device_retain",
     "gc_device_release", "gc_device_properties", "gc_csr_free", "gc_validate_csr",
     "gc_csr_validate", "gc_csr_from_edge_pairs", "gc_csr_from_edge_pairs_oriented",
-    "gc_csr_from_edge_pairs_device", "gc_graph_from_edge_pairs", "gc_graph_relabelled",
+    "gc_csr_from_edge_pairs_device", "gc_csr_from_edge_pairs_oriented_device", "gc_graph_from_edge_pairs",
+    "gc_graph_relabelled",
     "gc_parse_tsv_to_edges", "gc_graph_from_csr", "gc_graph_from_csr_arrays", "gc_graph_free",
     "gc_graph_num_nodes", "gc_graph_num_edges", "gc_graph_device_offsets",
     "gc_graph_device_targets", "gc_graph_prepare", "gc_graph_is_symmetric", "gc_graph_dag_stats",
@@ -264,6 +266,15 @@ def from_edge_pairs_device(device_handle, edges, num_nodes: int, symmetric: bool
     csr = Csr()
     check(lib.gc_csr_from_edge_pairs_device(device_handle, u32_ptr(pairs), pairs.shape[0], num_nodes,
                                             int(bool(symmetric)), byref(csr)))
+    return csr_to_numpy(csr)
+
+
+def from_edge_pairs_oriented_device(device_handle, edges, num_nodes: int):
+    """from_edge_pairs_oriented built on the GPU of `device_handle` (same arrays)."""
+    pairs = pairs_array(edges)
+    csr = Csr()
+    check(lib.gc_csr_from_edge_pairs_oriented_device(device_handle, u32_ptr(pairs), pairs.shape[0], num_nodes,
+                                                     byref(csr)))
     return csr_to_numpy(csr)
 
 

@@ -17,7 +17,8 @@ reference's BFS binary only saves its per-source documents; here they are printe
 
 Loading: the dataset is a TSV edge list read with parse_tsv_to_edges and built into a CSR with
 from_edge_pairs (symmetric) or from_edge_pairs_oriented, as the reference's --direct path does
-(harness.rs:79-160). The reference's other path imports the file into a Grafeo store first; Grafeo
+(harness.rs:79-160). `--csr device` (the default) builds those same arrays on the GPU (radix sort,
+deduplication, orientation); `--csr host` builds them on the host as the reference does. The reference's other path imports the file into a Grafeo store first; Grafeo
 is not part of this library, so both paths load the same way here. A dataset may also name a
 seeded synthetic graph: rmat:SCALE[:EDGE_FACTOR[:SEED]], er:N:AVG_DEGREE[:SEED] or grid:ROWS:COLS.
 """
@@ -61,20 +62,28 @@ def edges_of(dataset: str):
     return pairs, n
 
 
-def load_direct(dataset: str, device: GpuDevice, oriented: bool = False):
+def load_direct(dataset: str, device: GpuDevice, oriented: bool = False, csr: str = "device"):
     """(GpuGraph, seconds for parse + CSR build + transfer)."""
     start = time.perf_counter()
     pairs, n = edges_of(dataset)
     log(f"  Parsed {n} nodes, {len(pairs)} edges in {time.perf_counter() - start:.2f}s from {dataset}")
     t = time.perf_counter()
-    if oriented:
-        off, tgt = _native.from_edge_pairs_oriented(pairs, n)
+    if csr == "device" and not oriented:
+        # built on the GPU and kept there: the pairs go up, no CSR comes back
+        g = GpuGraph.from_edge_pairs(pairs, n, device)
+        log(f"  CSR build on the GPU (with the pairs' transfer): {time.perf_counter() - t:.4f}s "
+            f"({n} nodes, {g.num_edges} CSR entries)")
     else:
-        off, tgt = _native.from_edge_pairs(pairs, n, True)
-    log(f"  CSR build: {time.perf_counter() - t:.4f}s ({n} nodes, {len(tgt)} CSR entries)")
-    t = time.perf_counter()
-    g = GpuGraph.from_csr(off, tgt, device)
-    log(f"  GPU transfer: {time.perf_counter() - t:.4f}s")
+        if csr == "device":
+            off, tgt = _native.from_edge_pairs_oriented_device(device._handle, pairs, n)
+        elif oriented:
+            off, tgt = _native.from_edge_pairs_oriented(pairs, n)
+        else:
+            off, tgt = _native.from_edge_pairs(pairs, n, True)
+        log(f"  CSR build ({csr}): {time.perf_counter() - t:.4f}s ({n} nodes, {len(tgt)} CSR entries)")
+        t = time.perf_counter()
+        g = GpuGraph.from_csr(off, tgt, device)
+        log(f"  GPU transfer: {time.perf_counter() - t:.4f}s")
     total = time.perf_counter() - start
     log(f"  Total (parse+CSR+transfer): {total:.4f}s  [{g.num_nodes} vertices, {g.num_edges} CSR entries]")
     return g, total
@@ -135,7 +144,7 @@ def timed(call, warmup: int, runs: int):
 
 def run_triangles(args, device) -> list[dict]:
     oriented = args.oriented or args.direct
-    g, transfer = load_direct(args.dataset, device, oriented=oriented)
+    g, transfer = load_direct(args.dataset, device, oriented=oriented, csr=args.csr)
     call = g.triangle_count_oriented if oriented else g.triangle_count
     times, last = timed(call, args.warmup, args.runs)
     best = min(times)
@@ -153,7 +162,7 @@ def run_triangles(args, device) -> list[dict]:
 
 
 def run_bfs(args, device) -> list[dict]:
-    g, transfer = load_direct(args.dataset, device)
+    g, transfer = load_direct(args.dataset, device, csr=args.csr)
     n = g.num_nodes
     if n == 0:
         log("ERROR: Graph is empty")
@@ -184,7 +193,7 @@ def run_bfs(args, device) -> list[dict]:
 
 
 def run_pagerank(args, device) -> list[dict]:
-    g, transfer = load_direct(args.dataset, device)
+    g, transfer = load_direct(args.dataset, device, csr=args.csr)
     call = lambda: g.pagerank_numpy(args.damping, args.max_iterations, args.tolerance)  # noqa: E731
     times, last = timed(call, args.warmup, args.runs)
     best = min(times)
@@ -201,7 +210,7 @@ def run_pagerank(args, device) -> list[dict]:
 
 
 def run_ktruss(args, device) -> list[dict]:
-    g, transfer = load_direct(args.dataset, device)
+    g, transfer = load_direct(args.dataset, device, csr=args.csr)
     times, last = timed(lambda: g.k_truss_numpy(args.k), args.warmup, args.runs)
     best = min(times)
     teps = g.num_edges / best if best > 0 else 0.0
@@ -225,6 +234,8 @@ def parser() -> argparse.ArgumentParser:
         if direct:
             p.add_argument("--direct", action="store_true", help="TSV -> CSR -> GPU (the only path here)")
         p.add_argument("--save", action="store_true", help="write results/raw/gpu_<algorithm>_<stem>.json")
+        p.add_argument("--csr", choices=["device", "host"], default="device",
+                       help="build the CSR on the GPU (default) or on the host as the reference does")
 
     t = sub.add_parser("triangles", help="gpu_triangles")
     common(t)

@@ -122,6 +122,15 @@ gc_status gc_csr_from_edge_pairs_oriented(const uint32_t* pairs, size_t num_pair
 gc_status gc_csr_from_edge_pairs_device(gc_device* dev, const uint32_t* pairs, size_t num_pairs,
                                         uint32_t num_nodes, int32_t symmetric, gc_csr* out);
 
+/* from_edge_pairs_oriented (crates/grafeo-cuda-core/src/csr_export.rs:299-350)
+ * built on the GPU: canonical (min, max) keys radix-sorted, loops and copies
+ * dropped, undirected degrees, each edge oriented from the lower degree to the
+ * higher (ties: lower id first), sorted again. The same arrays as
+ * gc_csr_from_edge_pairs_oriented, returned to the host (gc_csr_free);
+ * out-of-range ids are GC_ERR_INVALID_CSR with the same message. */
+gc_status gc_csr_from_edge_pairs_oriented_device(gc_device* dev, const uint32_t* pairs, size_t num_pairs,
+                                                 uint32_t num_nodes, gc_csr* out);
+
 /* parse_tsv_to_edges (crates/grafeo-cuda-core/src/csr_export.rs:246-287).
  * `*pairs` is library-allocated (gc_free); ids are remapped in order of first
  * appearance. A missing file is GC_ERR_EXPORT. */

@@ -116,3 +116,65 @@ def test_device_preparation_builds_the_host_layout(device, monkeypatch):
         if name.startswith("RMAT-20"):
             print(f"RMAT-20 preparation: device {dev_s:.3f}s, host {host_s:.3f}s")
     monkeypatch.delenv("GC_PREPARE", raising=False)
+
+
+# ---- oriented builder (from_edge_pairs_oriented, csr_export.rs:299-350) ----------
+
+
+def same_oriented(device, pairs, n, with_oracle=False):
+    host = _native.from_edge_pairs_oriented(pairs, n)
+    dev = _native.from_edge_pairs_oriented_device(device._handle, pairs, n)
+    assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1])
+    if with_oracle:
+        import oracle
+
+        ref = oracle.from_edge_pairs_oriented(pairs, n)
+        assert np.array_equal(ref[0], dev[0]) and np.array_equal(ref[1], dev[1])
+    return dev
+
+
+def test_oriented_small_inputs(device):
+    assert same_oriented(device, [], 0)[0].tolist() == [0]
+    assert same_oriented(device, [], 5)[0].tolist() == [0] * 6
+    # loops only: no edge survives
+    off, tgt = same_oriented(device, [(0, 0), (3, 3), (3, 3)], 4, with_oracle=True)
+    assert off.tolist() == [0] * 5 and tgt.size == 0
+    # a star: every edge leaves its leaf (degree 1) for the centre (degree 3); a duplicate
+    # in both directions counts once
+    off, tgt = same_oriented(device, [(0, 1), (2, 0), (0, 3), (1, 0)], 4, with_oracle=True)
+    assert off.tolist() == [0, 0, 1, 2, 3] and tgt.tolist() == [0, 0, 0]
+    # equal degrees (a triangle): from the lower id
+    off, tgt = same_oriented(device, [(2, 1), (1, 0), (0, 2)], 3, with_oracle=True)
+    assert off.tolist() == [0, 2, 3, 3] and tgt.tolist() == [1, 2, 2]
+    rng = np.random.default_rng(15)
+    for _ in range(60):
+        n, edges = random_graph(rng, max_nodes=40, max_edges=200)
+        same_oriented(device, edges, n, with_oracle=True)
+
+
+def test_oriented_large_inputs(device):
+    rng = np.random.default_rng(16)
+    for n, count in ((1, 1000), (2, 5000), (300, 100000), (70000, 1_000_000), (1 << 20, 3_000_000)):
+        pairs = rng.integers(0, n, size=(count, 2), dtype=np.int64).astype(np.uint32)
+        same_oriented(device, pairs, n)
+
+
+def test_oriented_rmat20_triangles(device):
+    """The oriented R-MAT 20 CSR built on the GPU equals the host builder's, and its oriented
+    triangle count is the golden count."""
+    with open(GOLDEN) as f:
+        golden = json.load(f)["rmat20_ef16_seed1"]
+    pairs = _native.gen_rmat(20, 16, 1)
+    off, tgt = same_oriented(device, pairs, 1 << 20)
+    g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+    assert g.triangle_count_oriented()["total_triangles"] == golden["triangles"]
+
+
+def test_oriented_out_of_range_ids_are_rejected_like_the_host_builder(device):
+    pairs = [(0, 1), (2, 7), (9, 1)]
+    with pytest.raises(_native.GcError) as host_err:
+        _native.from_edge_pairs_oriented(pairs, 5)
+    with pytest.raises(_native.GcError) as dev_err:
+        _native.from_edge_pairs_oriented_device(device._handle, pairs, 5)
+    assert str(host_err.value) == str(dev_err.value)
+    assert dev_err.value.kind == "InvalidCsr"
