@@ -252,3 +252,155 @@ extern "C" __global__ __launch_bounds__(256) void csr_from_keys(
     }
   }
 }
+
+// ---- TSV edge lists on the device -----------------------------------------------
+// The parse of parse_tsv_to_edges (crates/grafeo-cuda-core/src/csr_export.rs:
+// 246-287): lines split at '\n', trimmed of ASCII whitespace; blank lines and
+// lines starting with '#' or '%' skipped; the first two whitespace-separated
+// fields parsed as u64 the way Rust's u64::from_str does (an optional '+',
+// then decimal digits, no overflow), the line skipped when either fails. The
+// file bytes sit in a buffer padded with zeros to a multiple of TSV_CHUNK
+// plus TSV_CHUNK, so every chunk's loads stay inside it.
+
+#define TSV_CHUNK 64
+
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {
+  // bytes of x that are zero, counted exactly (no carries between bytes)
+  uint32_t k = 0;
+#pragma unroll
+  for (int s = 0; s < 32; s += 8) k += ((x >> s) & 0xFFu) == 0u;
+  return k;
+}
+
+// counts[c] = line breaks in bytes [c * TSV_CHUNK, (c + 1) * TSV_CHUNK).
+extern "C" __global__ __launch_bounds__(256) void tsv_count_breaks(const uint4* __restrict__ bytes, uint64_t chunks,
+                                                                   uint32_t* __restrict__ counts) {
+  for (uint64_t c = global_thread(); c < chunks; c += grid_threads()) {
+    uint32_t k = 0;
+#pragma unroll
+    for (int q = 0; q < TSV_CHUNK / 16; ++q) {
+      const uint4 w = bytes[c * (TSV_CHUNK / 16) + q];
+      k += zero_bytes(w.x ^ 0x0A0A0A0Au) + zero_bytes(w.y ^ 0x0A0A0A0Au) + zero_bytes(w.z ^ 0x0A0A0A0Au) +
+           zero_bytes(w.w ^ 0x0A0A0A0Au);
+    }
+    counts[c] = k;
+  }
+}
+
+// starts[first[c] + j + 1] = the byte after the j-th line break of chunk c
+// (first = the exclusive scan of the counts; starts[0] = 0 is set apart).
+extern "C" __global__ __launch_bounds__(256) void tsv_line_starts(const uint8_t* __restrict__ bytes, uint64_t chunks,
+                                                                  const uint32_t* __restrict__ first,
+                                                                  uint32_t* __restrict__ starts) {
+  for (uint64_t c = global_thread(); c < chunks; c += grid_threads()) {
+    uint32_t at = first[c];
+    const uint64_t b = c * TSV_CHUNK;
+    for (uint32_t i = 0; i < TSV_CHUNK; ++i)
+      if (bytes[b + i] == '\n') starts[++at] = static_cast<uint32_t>(b + i + 1);
+  }
+}
+
+__device__ __forceinline__ bool tsv_space(uint8_t c) {
+  return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f';
+}
+
+__device__ __forceinline__ bool tsv_parse_u64(const uint8_t* bytes, uint32_t b, uint32_t e, unsigned long long* out) {
+  if (b < e && bytes[b] == '+') ++b;
+  if (b == e) return false;
+  unsigned long long v = 0;
+  for (uint32_t p = b; p < e; ++p) {
+    const uint32_t c = bytes[p];
+    if (c < '0' || c > '9') return false;
+    const unsigned long long digit = c - '0';
+    if (v > (0xFFFFFFFFFFFFFFFFull - digit) / 10ull) return false;
+    v = v * 10ull + digit;
+  }
+  *out = v;
+  return true;
+}
+
+// Line i = bytes [starts[i], starts[i + 1] - 1), the last one up to `size`:
+// its two ids, and whether it holds an edge.
+extern "C" __global__ __launch_bounds__(256) void tsv_parse_lines(
+    const uint8_t* __restrict__ bytes, uint32_t size, const uint32_t* __restrict__ starts, uint32_t lines,
+    unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst, uint8_t* __restrict__ valid) {
+  for (uint64_t li = global_thread(); li < lines; li += grid_threads()) {
+    uint32_t b = starts[li];
+    uint32_t e = li + 1 < lines ? starts[li + 1] - 1 : size;
+    while (b < e && tsv_space(bytes[b])) ++b;
+    while (e > b && tsv_space(bytes[e - 1])) --e;
+    bool ok = b < e && bytes[b] != '#' && bytes[b] != '%';
+    unsigned long long s = 0, d = 0;
+    if (ok) {
+      uint32_t p = b;
+      while (p < e && !tsv_space(bytes[p])) ++p;
+      const uint32_t f1e = p;
+      while (p < e && tsv_space(bytes[p])) ++p;
+      const uint32_t f2b = p;
+      while (p < e && !tsv_space(bytes[p])) ++p;
+      ok = tsv_parse_u64(bytes, b, f1e, &s) && tsv_parse_u64(bytes, f2b, p, &d);
+    }
+    src[li] = s;
+    dst[li] = d;
+    valid[li] = ok ? 1 : 0;
+  }
+}
+
+// The id occurrences in order of appearance: keys[2e] = src[e], keys[2e + 1] =
+// dst[e], pos[p] = p; and the largest id (for the sort's bit count).
+extern "C" __global__ __launch_bounds__(256) void tsv_occurrences(
+    const unsigned long long* __restrict__ src, const unsigned long long* __restrict__ dst, uint64_t edges,
+    unsigned long long* __restrict__ keys, uint32_t* __restrict__ pos, unsigned long long* __restrict__ max_id) {
+  unsigned long long m = 0;
+  for (uint64_t e = global_thread(); e < edges; e += grid_threads()) {
+    const unsigned long long s = src[e], d = dst[e];
+    keys[2 * e] = s;
+    keys[2 * e + 1] = d;
+    pos[2 * e] = static_cast<uint32_t>(2 * e);
+    pos[2 * e + 1] = static_cast<uint32_t>(2 * e + 1);
+    m = max(m, max(s, d));
+  }
+#pragma unroll
+  for (int o = GC_WAVE / 2; o > 0; o >>= 1) m = max(m, static_cast<unsigned long long>(__shfl_xor(m, o)));
+  if (lane_id() == 0 && m != 0) atomicMax(max_id, m);
+}
+
+// heads[i] = 1 where the sorted ids change (the first occurrence of an id, the
+// earliest in the stable sort).
+extern "C" __global__ __launch_bounds__(256) void tsv_heads(const unsigned long long* __restrict__ keys,
+                                                            uint64_t count, uint32_t* __restrict__ heads) {
+  for (uint64_t i = global_thread(); i < count; i += grid_threads())
+    heads[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// With run[i] = (exclusive scan of heads)[i] + heads[i] - 1, the id run of
+// sorted occurrence i: first[run] = the position of each run's head.
+extern "C" __global__ __launch_bounds__(256) void tsv_run_first(const uint32_t* __restrict__ scanned,
+                                                                const uint32_t* __restrict__ heads_in,
+                                                                const uint32_t* __restrict__ pos, uint64_t count,
+                                                                unsigned long long* __restrict__ first) {
+  for (uint64_t i = global_thread(); i < count; i += grid_threads())
+    if (heads_in[i]) first[scanned[i]] = pos[i];
+}
+
+// new_id[run[r]] = r over the runs sorted by first position.
+extern "C" __global__ __launch_bounds__(256) void tsv_rank_runs(const uint32_t* __restrict__ runs_by_first,
+                                                                uint32_t count, uint32_t* __restrict__ new_id) {
+  for (uint64_t r = global_thread(); r < count; r += grid_threads()) new_id[runs_by_first[r]] = static_cast<uint32_t>(r);
+}
+
+// pairs[pos[i]] = new_id[run of i]: the pairs, in line order, with ids
+// numbered in order of first appearance.
+extern "C" __global__ __launch_bounds__(256) void tsv_write_pairs(const uint32_t* __restrict__ scanned,
+                                                                  const uint32_t* __restrict__ heads_in,
+                                                                  const uint32_t* __restrict__ pos, uint64_t count,
+                                                                  const uint32_t* __restrict__ new_id,
+                                                                  uint32_t* __restrict__ pairs) {
+  for (uint64_t i = global_thread(); i < count; i += grid_threads())
+    pairs[pos[i]] = new_id[scanned[i] + heads_in[i] - 1];
+}
+
+// out[i] = i.
+extern "C" __global__ __launch_bounds__(256) void tsv_iota(uint32_t* __restrict__ out, uint64_t count) {
+  for (uint64_t i = global_thread(); i < count; i += grid_threads()) out[i] = static_cast<uint32_t>(i);
+}

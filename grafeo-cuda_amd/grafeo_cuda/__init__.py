@@ -15,6 +15,7 @@ array-returning variants (`*_numpy`) for large graphs. The GIL is released durin
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int32, c_size_t, c_uint32
 from ctypes import c_uint64, c_void_p
 
@@ -137,6 +138,23 @@ class GpuGraph:
         except GcError as e:
             raise _runtime_error(e) from None
         return GpuGraph(handle, device)
+
+    @staticmethod
+    def from_tsv(path: str, device: GpuDevice, symmetric: bool = True) -> "GpuGraph":
+        """A GpuGraph from a TSV edge list parsed and built on the GPU: parse_tsv_to_edges' pairs
+        (ids in order of first appearance), then from_edge_pairs' CSR, with no host CSR. The number
+        of edge lines parsed is in `.parsed_pairs`."""
+        if not isinstance(device, GpuDevice):
+            raise TypeError("device must be a GpuDevice")
+        handle, count = c_void_p(), c_size_t()
+        try:
+            check(lib.gc_graph_from_tsv(device._handle, os.fsencode(path), int(bool(symmetric)), byref(handle),
+                                        byref(count)))
+        except GcError as e:
+            raise _runtime_error(e) from None
+        g = GpuGraph(handle, device)
+        g.parsed_pairs = count.value
+        return g
 
     @property
     def num_nodes(self) -> int:

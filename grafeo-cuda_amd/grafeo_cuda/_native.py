@@ -69,6 +69,9 @@ def _load() -> ctypes.CDLL:
         "gc_csr_from_edge_pairs": (c_int, [u32p, c_size_t, c_uint32, c_int, POINTER(Csr)]),
         "gc_csr_from_edge_pairs_device": (c_int, [c_void_p, u32p, c_size_t, c_uint32, c_int, POINTER(Csr)]),
         "gc_csr_from_edge_pairs_oriented_device": (c_int, [c_void_p, u32p, c_size_t, c_uint32, POINTER(Csr)]),
+        "gc_parse_tsv_to_edges_device": (c_int, [c_void_p, c_char_p, POINTER(POINTER(c_uint32)), POINTER(c_size_t),
+                                                 POINTER(c_uint32)]),
+        "gc_graph_from_tsv": (c_int, [c_void_p, c_char_p, c_int32, POINTER(c_void_p), POINTER(c_size_t)]),
         "gc_graph_from_edge_pairs": (c_int, [c_void_p, u32p, c_size_t, c_uint32, c_int, POINTER(c_void_p)]),
         "gc_graph_relabelled": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p),
                                         POINTER(c_void_p)]),
@@ -166,7 +169,8 @@ EXPORTED = [
     "gc_last_error", "gc_free", "gc_device_count", "gc_device_new", "gc_device_retain",
     "gc_device_release", "gc_device_properties", "gc_csr_free", "gc_validate_csr",
     "gc_csr_validate", "gc_csr_from_edge_pairs", "gc_csr_from_edge_pairs_oriented",
-    "gc_csr_from_edge_pairs_device", "gc_csr_from_edge_pairs_oriented_device", "gc_graph_from_edge_pairs",
+    "gc_csr_from_edge_pairs_device", "gc_csr_from_edge_pairs_oriented_device", "gc_parse_tsv_to_edges_device",
+    "gc_graph_from_tsv", "gc_graph_from_edge_pairs",
     "gc_graph_relabelled",
     "gc_parse_tsv_to_edges", "gc_graph_from_csr", "gc_graph_from_csr_arrays", "gc_graph_free",
     "gc_graph_num_nodes", "gc_graph_num_edges", "gc_graph_device_offsets",
@@ -283,6 +287,13 @@ def from_edge_pairs_oriented(edges, num_nodes: int):
     csr = Csr()
     check(lib.gc_csr_from_edge_pairs_oriented(u32_ptr(pairs), pairs.shape[0], num_nodes, byref(csr)))
     return csr_to_numpy(csr)
+
+
+def parse_tsv_to_edges_device(device_handle, path: str):
+    """parse_tsv_to_edges on the GPU of `device_handle` (same pairs and node count)."""
+    ptr, count, nodes = POINTER(c_uint32)(), c_size_t(), c_uint32()
+    check(lib.gc_parse_tsv_to_edges_device(device_handle, os.fsencode(path), byref(ptr), byref(count), byref(nodes)))
+    return take_u32(ptr, 2 * count.value).reshape(-1, 2), nodes.value
 
 
 def parse_tsv_to_edges(path: str):

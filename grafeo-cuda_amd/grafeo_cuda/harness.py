@@ -17,8 +17,9 @@ reference's BFS binary only saves its per-source documents; here they are printe
 
 Loading: the dataset is a TSV edge list read with parse_tsv_to_edges and built into a CSR with
 from_edge_pairs (symmetric) or from_edge_pairs_oriented, as the reference's --direct path does
-(harness.rs:79-160). `--csr device` (the default) builds those same arrays on the GPU (radix sort,
-deduplication, orientation); `--csr host` builds them on the host as the reference does. The reference's other path imports the file into a Grafeo store first; Grafeo
+(harness.rs:79-160). `--csr device` (the default) parses the file and builds those same arrays on
+the GPU (line breaks and fields per lane, ids numbered by radix sorts, CSR by radix sort,
+deduplication, orientation); `--csr host` does both on the host as the reference does. The reference's other path imports the file into a Grafeo store first; Grafeo
 is not part of this library, so both paths load the same way here. A dataset may also name a
 seeded synthetic graph: rmat:SCALE[:EDGE_FACTOR[:SEED]], er:N:AVG_DEGREE[:SEED] or grid:ROWS:COLS.
 """
@@ -65,7 +66,18 @@ def edges_of(dataset: str):
 def load_direct(dataset: str, device: GpuDevice, oriented: bool = False, csr: str = "device"):
     """(GpuGraph, seconds for parse + CSR build + transfer)."""
     start = time.perf_counter()
-    pairs, n = edges_of(dataset)
+    synthetic = dataset.startswith(("rmat:", "er:", "grid:"))
+    if csr == "device" and not synthetic and not oriented:
+        # parsed and built on the GPU: the file's bytes go up, no pairs or CSR come back
+        g = GpuGraph.from_tsv(dataset, device)
+        total = time.perf_counter() - start
+        log(f"  Parsed and built on the GPU: {g.num_nodes} nodes, {g.parsed_pairs} edges, {g.num_edges} CSR "
+            f"entries in {total:.4f}s from {dataset}")
+        return g, total
+    if csr == "device" and not synthetic:
+        pairs, n = _native.parse_tsv_to_edges_device(device._handle, dataset)
+    else:
+        pairs, n = edges_of(dataset)
     log(f"  Parsed {n} nodes, {len(pairs)} edges in {time.perf_counter() - start:.2f}s from {dataset}")
     t = time.perf_counter()
     if csr == "device" and not oriented:

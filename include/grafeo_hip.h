@@ -131,6 +131,15 @@ gc_status gc_csr_from_edge_pairs_device(gc_device* dev, const uint32_t* pairs, s
 gc_status gc_csr_from_edge_pairs_oriented_device(gc_device* dev, const uint32_t* pairs, size_t num_pairs,
                                                  uint32_t num_nodes, gc_csr* out);
 
+/* parse_tsv_to_edges (crates/grafeo-cuda-core/src/csr_export.rs:246-287)
+ * parsed on the GPU: the same pairs and num_nodes as gc_parse_tsv_to_edges
+ * (ids in order of first appearance), `*pairs` library-allocated (gc_free).
+ * The file is read on the host and parsed on the device (line breaks, one lane
+ * per line, ids numbered by stable radix sorts). Files of 4 GiB or more are a
+ * GC_ERR_KERNEL error; a missing file is GC_ERR_EXPORT. */
+gc_status gc_parse_tsv_to_edges_device(gc_device* dev, const char* path, uint32_t** pairs, size_t* num_pairs,
+                                       uint32_t* num_nodes);
+
 /* parse_tsv_to_edges (crates/grafeo-cuda-core/src/csr_export.rs:246-287).
  * `*pairs` is library-allocated (gc_free); ids are remapped in order of first
  * appearance. A missing file is GC_ERR_EXPORT. */
@@ -355,6 +364,14 @@ gc_status gc_tc_bucket_timings(gc_graph* g, uint32_t max_buckets, float* feature
  * reference builds the CSR on the host, harness.rs:84-115). */
 gc_status gc_graph_from_edge_pairs(gc_device* dev, const uint32_t* pairs, size_t num_pairs,
                                    uint32_t num_nodes, int32_t symmetric, gc_graph** out);
+
+/* GpuGraph from a TSV edge list with no host CSR (an addition: the reference
+ * parses and builds on the host, harness.rs:84-115): parsed as
+ * gc_parse_tsv_to_edges_device, then built as gc_graph_from_edge_pairs
+ * (`symmetric` as there); the pairs never leave the device. `num_pairs`
+ * (optional) receives the edge lines parsed. */
+gc_status gc_graph_from_tsv(gc_device* dev, const char* path, int32_t symmetric, gc_graph** out,
+                            size_t* num_pairs);
 
 /* The degree-ordered layout the kernels run on (built on first use; no
  * reference counterpart), as device pointers owned by the graph: offsets
