@@ -68,33 +68,34 @@ bool tc_hot16() {
 }
 
 // The 16-bit hot lists of tc_bitmap16, built once per graph (again if the
-// parts change): at most (m + n) / 2 + 1 words, and 12 bytes per vertex.
+// parts change): at most (m + 3n) / 4 + 1 quads of 8 bytes, and 12 bytes per
+// vertex.
 void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts) {
   if (g.tc_hot16_ready && g.tc_hot16_parts == parts) return;
   Device& dev = *g.dev;
   const uint32_t n = g.n;
   if (n == 0) return;  // no rows, nothing to count
   const uint64_t m = g.rl.host_offsets[n];
-  const uint64_t max_words = (m + n) / 2 + 1;
-  if (max_words > 0xFFFFFFFFull) fail_kernel("graph too large for the 16-bit triangle-count lists");
+  const uint64_t max_quads = (m + 3ull * n) / 4 + 1;
+  if (max_quads > 0xFFFFFFFFull) fail_kernel("graph too large for the 16-bit triangle-count lists");
   const uint64_t rows = std::max<uint64_t>(n, 1);
   g.tc_split.allocate(sizeof(uint32_t) * rows);
   g.tc_hw.allocate(sizeof(uint32_t) * 2 * rows);
-  g.tc_words.allocate(sizeof(uint32_t) * max_words);
-  DeviceMemory woff, scan;
-  woff.allocate(sizeof(uint32_t) * rows);
+  g.tc_words.allocate(sizeof(uint16_t) * 4 * max_quads);
+  DeviceMemory qoff, scan;
+  qoff.allocate(sizeof(uint32_t) * rows);
   scan.allocate(sizeof(uint32_t) * scan_scratch_words(rows));
   const uint32_t* offsets = g.rl.offsets.as<const uint32_t>();
   const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
   Grid1D vg = thread_per_item(rows);
   launch(dev, get_kernel(dev, "tc", "tc_hot16_rows"), vg.blocks, vg.threads, 0, g.rl.fwd.as<const uint32_t>(),
-         offsets + 1, tgt, n, hot_base, parts, g.tc_split.as<uint32_t>(), woff.as<uint32_t>());
-  exclusive_scan_u32(dev, woff.as<uint32_t>(), n, scan.as<uint32_t>());
+         offsets + 1, tgt, n, hot_base, parts, g.tc_split.as<uint32_t>(), qoff.as<uint32_t>());
+  exclusive_scan_u32(dev, qoff.as<uint32_t>(), n, scan.as<uint32_t>());
   Grid1D eg = thread_per_item(std::max<uint64_t>(m, n));
   launch(dev, get_kernel(dev, "tc", "tc_hot16_fill"), eg.blocks, eg.threads, 0, offsets, tgt, n, m, hot_base,
-         parts, static_cast<const uint32_t*>(g.tc_split.as<uint32_t>()), static_cast<const uint32_t*>(woff.as<uint32_t>()),
+         parts, static_cast<const uint32_t*>(g.tc_split.as<uint32_t>()), static_cast<const uint32_t*>(qoff.as<uint32_t>()),
          g.tc_hw.as<uint32_t>() /* uint2[n] */, g.tc_words.as<uint16_t>());
-  dev.sync();  // woff and scan are released on return
+  dev.sync();  // qoff and scan are released on return
   g.tc_hot16_ready = true;
   g.tc_hot16_parts = parts;
 }
@@ -160,7 +161,7 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     if (tc_hot16()) {
       ensure_hot16(g, hot_base, parts);
       launch(dev, get_kernel(dev, "tc", "tc_bitmap16"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt,
-             g.tc_split.as<const uint32_t>(), g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>(), row_hi,
+             g.tc_split.as<const uint32_t>(), g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint2 quads */, row_hi,
              row_lo, hot_base, single,
              light_chunk, parts, tc_split_min(), counters, result);
     } else {

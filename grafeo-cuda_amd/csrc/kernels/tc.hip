@@ -417,81 +417,84 @@ __device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, 
 }
 
 // ---- 16-bit hot lists (tc_bitmap16) ----
-// The hot part of a list is stored as 32-bit words of two offsets y = w -
-// hot_base (< TCB_HOT_BITS), low half first; an odd-length part ends in the
-// pad offset TCB_HOT_BITS, whose lookup reads the bitmap's zero word. So one
-// load carries two entries, and no entry needs a mask or a clamp.
+// The hot part of a list is stored as quads: 8 bytes holding four offsets
+// y = w - hot_base (< TCB_HOT_BITS), lowest half first. A part whose length is
+// not a multiple of four ends in pad offsets TCB_HOT_BITS, whose lookups read
+// the bitmap's zero word. So one 8-byte load carries four entries (the
+// texture path's work goes with the lane loads, not their width), and no
+// entry needs a mask or a clamp.
 #define TCB_PAD16 TCB_HOT_BITS
 
 __device__ __forceinline__ uint32_t hot16_hit(const uint32_t* s_bits, uint32_t y) {
   return word_bit(s_bits[y >> 5], y);
 }
 
-__device__ __forceinline__ uint32_t hot16x2_hits(const uint32_t* s_bits, uint32_t x) {
-  return hot16_hit(s_bits, x & 0xFFFFu) + hot16_hit(s_bits, x >> 16);
+__device__ __forceinline__ uint32_t hot16x4_hits(const uint32_t* s_bits, uint2 x) {
+  return hot16_hit(s_bits, x.x & 0xFFFFu) + hot16_hit(s_bits, x.x >> 16) + hot16_hit(s_bits, x.y & 0xFFFFu) +
+         hot16_hit(s_bits, x.y >> 16);
 }
 
-// As probe_list over the words W[0, nw) of a hot part: whole steps of
-// 4 x 64 words (512 entries, four loads in flight per lane), then one
-// guarded step (word index clamped to the last word, hits masked by word).
-__device__ __forceinline__ uint32_t probe16_list(const uint32_t* W, uint32_t nw, const uint32_t* s_bits) {
+// As probe_list over the quads Q[0, nq) of a hot part: whole steps of
+// 2 x 64 quads (512 entries, two loads in flight per lane), then one guarded
+// step (quad index clamped to the last quad, hits masked by quad).
+__device__ __forceinline__ uint32_t probe16_list(const uint2* Q, uint32_t nq, const uint32_t* s_bits) {
   const uint32_t lane = lane_id();
   uint32_t local = 0;
   uint32_t kk = 0;
-  for (; kk + 4 * GC_WAVE <= nw; kk += 4 * GC_WAVE) {
-    uint32_t x[4];
+  for (; kk + 2 * GC_WAVE <= nq; kk += 2 * GC_WAVE) {
+    uint2 x[2];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = W[kk + q * GC_WAVE + lane];
+    for (int q = 0; q < 2; ++q) x[q] = Q[kk + q * GC_WAVE + lane];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) local += hot16x2_hits(s_bits, x[q]);
+    for (int q = 0; q < 2; ++q) local += hot16x4_hits(s_bits, x[q]);
   }
-  if (kk < nw) {
-    const uint32_t left = nw - kk;  // 1 .. 255, wave-uniform
-    const uint32_t* T = W + kk;
+  if (kk < nq) {
+    const uint32_t left = nq - kk;  // 1 .. 127, wave-uniform
+    const uint2* T = Q + kk;
     if (left <= GC_WAVE) {
-      const uint32_t x = T[min(lane, left - 1)];
-      local += lane < left ? hot16x2_hits(s_bits, x) : 0u;
+      const uint2 x = T[min(lane, left - 1)];
+      local += lane < left ? hot16x4_hits(s_bits, x) : 0u;
     } else {
-      uint32_t x[4];
+      uint2 x[2];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) x[q] = T[min(q * GC_WAVE + lane, left - 1)];
+      for (int q = 0; q < 2; ++q) x[q] = T[min(q * GC_WAVE + lane, left - 1)];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) local += q * GC_WAVE + lane < left ? hot16x2_hits(s_bits, x[q]) : 0u;
+      for (int q = 0; q < 2; ++q) local += q * GC_WAVE + lane < left ? hot16x4_hits(s_bits, x[q]) : 0u;
     }
   }
   return local;
 }
 
-// As probe_pair over the words of two hot parts.
-__device__ __forceinline__ uint32_t probe16_pair(const uint32_t* W1, uint32_t n1, const uint32_t* W2, uint32_t n2,
+// As probe_pair over the quads of two hot parts (four loads in flight per lane).
+__device__ __forceinline__ uint32_t probe16_pair(const uint2* Q1, uint32_t n1, const uint2* Q2, uint32_t n2,
                                                  const uint32_t* s_bits) {
   const uint32_t lane = lane_id();
   uint32_t local = 0;
   uint32_t kk = 0;
   const uint32_t nmin = min(n1, n2), nmax = max(n1, n2);
-  for (; kk + 4 * GC_WAVE <= nmin; kk += 4 * GC_WAVE) {
-    uint32_t x[8];
+  for (; kk + 2 * GC_WAVE <= nmin; kk += 2 * GC_WAVE) {
+    uint2 x[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      x[q] = W1[kk + q * GC_WAVE + lane];
-      x[4 + q] = W2[kk + q * GC_WAVE + lane];
+    for (int q = 0; q < 2; ++q) {
+      x[q] = Q1[kk + q * GC_WAVE + lane];
+      x[2 + q] = Q2[kk + q * GC_WAVE + lane];
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) local += hot16x2_hits(s_bits, x[q]);
+    for (int q = 0; q < 4; ++q) local += hot16x4_hits(s_bits, x[q]);
   }
-  for (; kk < nmax; kk += 4 * GC_WAVE) {
-    uint32_t x[8];
+  for (; kk < nmax; kk += 2 * GC_WAVE) {
+    uint2 x[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 2; ++q) {
       const uint32_t i = kk + q * GC_WAVE + lane;
-      x[q] = W1[min(i, n1 - 1)];
-      x[4 + q] = W2[min(i, n2 - 1)];
+      x[q] = Q1[min(i, n1 - 1)];
+      x[2 + q] = Q2[min(i, n2 - 1)];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 2; ++q) {
       const uint32_t i = kk + q * GC_WAVE + lane;
-      local += i < n1 ? hot16x2_hits(s_bits, x[q]) : 0u;
-      local += i < n2 ? hot16x2_hits(s_bits, x[4 + q]) : 0u;
+      local += i < n1 ? hot16x4_hits(s_bits, x[q]) : 0u;
+      local += i < n2 ? hot16x4_hits(s_bits, x[2 + q]) : 0u;
     }
   }
   return local;
@@ -536,7 +539,7 @@ __device__ __forceinline__ uint32_t probe_cold(const uint32_t* L, uint32_t lj, c
 template <bool kHot16>
 __device__ __forceinline__ void tc_bitmap_body(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint32_t* __restrict__ words,
+    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint2* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits) {
@@ -621,7 +624,7 @@ __device__ __forceinline__ void tc_bitmap_body(
           uint32_t local = 0;
           for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
             const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
-            // kHot16: [st, st + len) are the words of L(v)'s hot part and
+            // kHot16: [st, st + len) are the quads of L(v)'s hot part and
             // [cst, cst + clen) its cold part in targets (walked only when
             // the row has a cold prefix); otherwise all of L(v) in targets
             uint32_t st = 0, len = 0, cst = 0, clen = 0;
@@ -655,10 +658,10 @@ __device__ __forceinline__ void tc_bitmap_body(
                 pending &= pending - 1;
                 const uint32_t s2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv2));
                 const uint32_t l2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv2));
-                if (kHot16) local += probe16_pair(words + s1, lj, words + s2, l2, s_bits);
+                if (kHot16) local += probe16_pair(quads + s1, lj, quads + s2, l2, s_bits);
                 else local += probe_pair(targets + s1, lj, targets + s2, l2, s_bits, hot_base, A, c);
               } else {
-                if (kHot16) local += probe16_list(words + s1, lj, s_bits);
+                if (kHot16) local += probe16_list(quads + s1, lj, s_bits);
                 else local += probe_list(targets + s1, lj, s_bits, hot_base, A, c);
               }
             }
@@ -699,27 +702,27 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
 }
 
 // As tc_bitmap, with the hot part of every list read as 16-bit offsets from
-// hot_base, two to a 32-bit word (words; hw[v] = {first word, words} of
+// hot_base, four to an 8-byte quad (quads; hw[v] = {first quad, quads} of
 // L(v)'s hot part, split[v] = the first entry of L(v) at or above hot_base):
-// half the bytes and half the loads for the entries that carry almost all of
-// the streaming, no clamp and no cold test per hot entry, and the cold parts
-// walked apart, only for the rows that have a cold prefix.
+// half the bytes and a quarter of the loads for the entries that carry almost
+// all of the streaming, no clamp and no cold test per hot entry, and the cold
+// parts walked apart, only for the rows that have a cold prefix.
 extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint32_t* __restrict__ words,
+    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint2* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
   __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
-  tc_bitmap_body<true>(lo, hi, targets, split, hw, words, n, row_lo, hot_base, heavy, light_chunk, parts,
+  tc_bitmap_body<true>(lo, hi, targets, split, hw, quads, n, row_lo, hot_base, heavy, light_chunk, parts,
                        split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE]);
 }
 
 // The hot lists of tc_bitmap16, in two passes around an exclusive scan of
-// nwords: tc_hot16_rows sets split[v] (the first index of [lo[v], hi[v]) whose
-// target is at or above hot_base) and the words of the hot part; tc_hot16_fill
-// writes every hot entry's offset into its row's words (from the scanned
-// word offset), the pad of an odd part, and hw[v]. The rows are laid out part
+// nquads: tc_hot16_rows sets split[v] (the first index of [lo[v], hi[v]) whose
+// target is at or above hot_base) and the quads of the hot part; tc_hot16_fill
+// writes every hot entry's offset into its row's quads (from the scanned
+// quad offset), the pads of the last quad, and hw[v]. The rows are laid out part
 // by part (part v % parts, then v), in the middle-vertex parts of tc_bitmap16:
 // each part's lists are then contiguous, and an XCD's L2 fetches no line
 // shared with another part's lists. part_slot gives row v's place there.
@@ -730,18 +733,18 @@ __device__ __forceinline__ uint32_t part_slot(uint32_t v, uint32_t n, uint32_t p
 
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_rows(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    uint32_t n, uint32_t hot_base, uint32_t parts, uint32_t* __restrict__ split, uint32_t* __restrict__ nwords) {
+    uint32_t n, uint32_t hot_base, uint32_t parts, uint32_t* __restrict__ split, uint32_t* __restrict__ nquads) {
   for (uint64_t v = global_thread(); v < n; v += grid_threads()) {
     const uint32_t b = hi[v];
     const uint32_t sp = lower_bound_u32(targets, lo[v], b, hot_base);
     split[v] = sp;
-    nwords[part_slot(static_cast<uint32_t>(v), n, parts)] = (b - sp + 1) >> 1;
+    nquads[part_slot(static_cast<uint32_t>(v), n, parts)] = (b - sp + 3) >> 2;
   }
 }
 
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n, uint64_t m,
-    uint32_t hot_base, uint32_t parts, const uint32_t* __restrict__ split, const uint32_t* __restrict__ woff,
+    uint32_t hot_base, uint32_t parts, const uint32_t* __restrict__ split, const uint32_t* __restrict__ qoff,
     uint2* __restrict__ hw, uint16_t* __restrict__ half) {
   const uint64_t total = m > n ? m : static_cast<uint64_t>(n);
   for (uint64_t i = global_thread(); i < total; i += grid_threads()) {
@@ -749,14 +752,15 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
       const uint32_t v = row_of_entry(offsets, n, static_cast<uint32_t>(i));
       const uint32_t sp = split[v];
       if (i >= sp && i < offsets[v + 1])
-        half[2ull * woff[part_slot(v, n, parts)] + (i - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
+        half[4ull * qoff[part_slot(v, n, parts)] + (i - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
     }
     if (i < n) {
       const uint32_t v = static_cast<uint32_t>(i);
-      const uint32_t w = woff[part_slot(v, n, parts)];
+      const uint32_t q = qoff[part_slot(v, n, parts)];
       const uint32_t len = offsets[v + 1] - split[v];
-      hw[v] = make_uint2(w, (len + 1) >> 1);
-      if (len & 1u) half[2ull * w + len] = static_cast<uint16_t>(TCB_PAD16);
+      const uint32_t nq = (len + 3) >> 2;
+      hw[v] = make_uint2(q, nq);
+      for (uint32_t k = len; k < 4 * nq; ++k) half[4ull * q + k] = static_cast<uint16_t>(TCB_PAD16);
     }
   }
 }
