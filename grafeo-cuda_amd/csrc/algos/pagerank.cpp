@@ -26,7 +26,6 @@ constexpr uint32_t kPrExactMax = 16;        // exact-degree segments for degrees
 constexpr uint32_t kPrHubLimit = 1024;      // rows of at least this degree are summed in chunks
 constexpr uint32_t kPrHubChunk = 4096;      // PR_HUB_CHUNK of pagerank.hip
 constexpr uint32_t kPrBlocksPerCu = 8;      // resident 256-lane workgroups per CU (32 waves)
-constexpr uint32_t kPrFinishGroups = 64;    // workgroups of pr_finalize
 
 struct PrSegment {
   uint32_t row_begin;
@@ -215,7 +214,6 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
       "pr.partials", sizeof(double) * 2 * std::max({1u, partial_blocks, 2 * cu}));
   PrState* state = g.buffer<PrState>("pr.state", sizeof(PrState));
   uint32_t* work = g.buffer<uint32_t>("pr.work", 16);
-  double* wg_sums = g.buffer<double>("pr.wg_sums", sizeof(double) * 2 * kPrFinishGroups);
   fill_words(dev, work, 4, 0);
 
   const float r0 = 1.0f / static_cast<float>(n);
@@ -262,8 +260,7 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
         partial_blocks = grid_blocks;  // one partial pair per workgroup of the new grid
       }
     }
-    launch(dev, finalize, kPrFinishGroups, 256u, 0, state, static_cast<const double*>(partials),
-           partial_blocks, tol, work, wg_sums);
+    launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials), partial_blocks, tol, work);
     if ((it + 1) % kPrBatch == 0 || it + 1 == max_iterations) {
       copy_to_host(dev, dev.pinned, state, sizeof(PrState));
       std::memcpy(&host_state, dev.pinned, sizeof(PrState));

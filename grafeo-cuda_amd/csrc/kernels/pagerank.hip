@@ -446,26 +446,21 @@ extern "C" __global__ __launch_bounds__(PRH_THREADS) void pr_spmv_hot(
 }
 
 // Adds the per-workgroup partials in a fixed order and advances the state;
-// also rearms the work counter of pr_spmv_hot for the next iteration.
-// Fixed-order FP64 sums of the partials in two stages: workgroup b adds the
-// partials of its contiguous range (lane-strided, then a fixed reduction
-// tree) into wg_sums[b]; the last workgroup to finish (atomic ticket in
-// work[1]) adds wg_sums in index order and updates the iteration state. The
-// result does not depend on the order in which workgroups finish. work[0]
-// (the persistent kernel's task counter) is cleared for the next iteration.
-#define PR_FIN_THREADS 256
+// also rearms the work counters for the next iteration. One 1024-lane
+// workgroup: a lane adds the partials it strides over, the waves reduce in a
+// fixed tree and lane 0 adds the 16 wave sums in order, so the result does
+// not depend on timing (64 workgroups with a last-arriver fence took 9 us per
+// iteration for the 512 partial pairs).
+#define PR_FIN_THREADS 1024
 
 extern "C" __global__ __launch_bounds__(PR_FIN_THREADS) void pr_finalize(
     PrState* __restrict__ state, const double* __restrict__ partials, uint32_t blocks,
-    double tolerance, uint32_t* __restrict__ work, double* __restrict__ wg_sums) {
+    double tolerance, uint32_t* __restrict__ work) {
   __shared__ double s_a[PR_FIN_THREADS / GC_WAVE];
   __shared__ double s_b[PR_FIN_THREADS / GC_WAVE];
-  __shared__ uint32_t s_last;
   if (state->done) return;
-  const uint32_t per = (blocks + gridDim.x - 1) / gridDim.x;
-  const uint32_t lo = min(blocks, blockIdx.x * per), hi = min(blocks, lo + per);
   double d0 = 0.0, d1 = 0.0;
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += PR_FIN_THREADS) {
+  for (uint32_t i = threadIdx.x; i < blocks; i += PR_FIN_THREADS) {
     d0 += partials[2 * static_cast<uint64_t>(i)];
     d1 += partials[2 * static_cast<uint64_t>(i) + 1];
   }
@@ -481,19 +476,6 @@ extern "C" __global__ __launch_bounds__(PR_FIN_THREADS) void pr_finalize(
     for (int w = 0; w < PR_FIN_THREADS / GC_WAVE; ++w) {
       a += s_a[w];
       b += s_b[w];
-    }
-    wg_sums[2 * blockIdx.x] = a;
-    wg_sums[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    s_last = atomicAdd(&work[1], 1u) == gridDim.x - 1 ? 1u : 0u;
-  }
-  __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    double a = 0.0, b = 0.0;
-    for (uint32_t w = 0; w < gridDim.x; ++w) {
-      a += wg_sums[2 * w];
-      b += wg_sums[2 * w + 1];
     }
     state->dangling = a;
     state->diff = b;
