@@ -142,9 +142,10 @@ struct Traversal {
     pub_dev = reinterpret_cast<BfsPublished*>(static_cast<char*>(dev.pinned_device) + Device::kPinnedKernelOffset);
     cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
     const std::vector<uint32_t>& hoff = g.rl.host_offsets;
-    Grid1D vgrid = thread_per_item(n);
-    launch(dev, get_kernel(dev, "bfs", "bfs_init"), vgrid.blocks, vgrid.threads, 0, dist, n, src,
-           queue[0], &rb->counters[0]);
+    // four entries per lane and store, a grid-stride loop over at most 16 workgroups per CU
+    Grid1D vgrid = thread_per_item(std::max<uint64_t>(n / 4, 1));
+    launch(dev, get_kernel(dev, "bfs", "bfs_init"), std::min(vgrid.blocks, cu * 16), vgrid.threads, 0, dist, n,
+           src, queue[0], &rb->counters[0]);
     m_f = hoff[src + 1] - hoff[src];
     m_u = static_cast<uint64_t>(g.m) - m_f;
   }
@@ -442,7 +443,7 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
     else if (learned_next) choice = t.last_decision.choice;
     else choice = t.decide_now(params);
   }
-  Grid1D vgrid = thread_per_item(g.n);
+  Grid1D vgrid = thread_per_item((static_cast<uint64_t>(g.n) + 3) / 4);  // four entries per lane
   launch(dev, get_kernel(dev, "util", "gather_u32"), vgrid.blocks, vgrid.threads, 0,
          reinterpret_cast<const uint32_t*>(t.dist), g.rl.inv.as<const uint32_t>(),
          reinterpret_cast<uint32_t*>(d_out), static_cast<uint64_t>(g.n));

@@ -268,7 +268,7 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
     }
   }
   const int latest = host_state.iterations & 1;
-  Grid1D ogrid = thread_per_item(n);
+  Grid1D ogrid = thread_per_item((static_cast<uint64_t>(n) + 3) / 4);  // four entries per lane
   launch(dev, get_kernel(dev, "util", "gather_u32"), ogrid.blocks, ogrid.threads, 0,
          reinterpret_cast<const uint32_t*>(rank[latest]), g.rl.inv.as<const uint32_t>(),
          reinterpret_cast<uint32_t*>(d_out), static_cast<uint64_t>(n));

@@ -42,10 +42,18 @@ struct BfsPublished {
 
 // Also clears the level counters, which every level's publication clears
 // again for the next one.
+// dist is written four entries (16 bytes) per store, over a grid-stride loop.
 extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__ dist, uint32_t n,
                                                            uint32_t source, uint32_t* __restrict__ queue,
                                                            BfsCounters* __restrict__ counters) {
-  for (uint64_t v = global_thread(); v < n; v += grid_threads()) dist[v] = v == source ? 0 : -1;
+  const uint64_t quads = n / 4;
+  int4* d4 = reinterpret_cast<int4*>(dist);
+  for (uint64_t q = global_thread(); q < quads; q += grid_threads()) {
+    const uint64_t v = 4 * q;
+    d4[q] = make_int4(v == source ? 0 : -1, v + 1 == source ? 0 : -1, v + 2 == source ? 0 : -1,
+                      v + 3 == source ? 0 : -1);
+  }
+  for (uint64_t v = 4 * quads + global_thread(); v < n; v += grid_threads()) dist[v] = v == source ? 0 : -1;
   if (global_thread() == 0) {
     queue[0] = source;
     counters->next_size = 0;
