@@ -742,6 +742,20 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_rows(
   }
 }
 
+// Within a row, entries are placed by blocks of 256 (64 quads; the last block
+// holds Q = ceil(rest / 4) quads): entry r of a block goes to quad r % Q,
+// half r / Q. A probe's lane reads quad q of a block and looks up its four
+// halves in turn, so each LDS lookup instruction of the wave covers Q
+// consecutive entries of the list (close ids, few bitmap words and bank
+// conflicts) rather than entries four apart. The count does not depend on the
+// order of a list's entries.
+__device__ __forceinline__ uint64_t quad_slot(uint32_t e, uint32_t len) {
+  const uint32_t b = e >> 8, r = e & 255u;
+  const uint32_t rest = len - (b << 8);
+  const uint32_t q_count = rest >= 256u ? 64u : (rest + 3u) >> 2;
+  return 4ull * (64ull * b + r % q_count) + r / q_count;
+}
+
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n, uint64_t m,
     uint32_t hot_base, uint32_t parts, const uint32_t* __restrict__ split, const uint32_t* __restrict__ qoff,
@@ -751,8 +765,10 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
     if (i < m) {
       const uint32_t v = row_of_entry(offsets, n, static_cast<uint32_t>(i));
       const uint32_t sp = split[v];
-      if (i >= sp && i < offsets[v + 1])
-        half[4ull * qoff[part_slot(v, n, parts)] + (i - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
+      const uint32_t end = offsets[v + 1];
+      if (i >= sp && i < end)
+        half[4ull * qoff[part_slot(v, n, parts)] + quad_slot(static_cast<uint32_t>(i) - sp, end - sp)] =
+            static_cast<uint16_t>(targets[i] - hot_base);
     }
     if (i < n) {
       const uint32_t v = static_cast<uint32_t>(i);
@@ -760,7 +776,14 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
       const uint32_t len = offsets[v + 1] - split[v];
       const uint32_t nq = (len + 3) >> 2;
       hw[v] = make_uint2(q, nq);
-      for (uint32_t k = len; k < 4 * nq; ++k) half[4ull * q + k] = static_cast<uint16_t>(TCB_PAD16);
+      // the last block's free halves (at most three) hold the pad
+      if (len & 3u) {
+        const uint32_t last = (len - 1) >> 8;
+        const uint32_t rest = len - (last << 8);
+        const uint32_t q_count = (rest + 3u) >> 2;
+        for (uint32_t r = rest; r < 4 * q_count; ++r)
+          half[4ull * (q + 64ull * last + r % q_count) + r / q_count] = static_cast<uint16_t>(TCB_PAD16);
+      }
     }
   }
 }
