@@ -68,7 +68,7 @@ bool tc_hot16() {
 }
 
 // The 16-bit hot lists of tc_bitmap16, built once per graph (again if the
-// parts change): at most (m + 3n) / 4 + 1 quads of 8 bytes, and 12 bytes per
+// parts change): at most (m + 7n) / 8 + 1 octs of 16 bytes, and 12 bytes per
 // vertex.
 void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts) {
   if (g.tc_hot16_ready && g.tc_hot16_parts == parts) return;
@@ -76,12 +76,12 @@ void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts) {
   const uint32_t n = g.n;
   if (n == 0) return;  // no rows, nothing to count
   const uint64_t m = g.rl.host_offsets[n];
-  const uint64_t max_quads = (m + 3ull * n) / 4 + 1;
+  const uint64_t max_quads = (m + 7ull * n) / 8 + 1;  // 16-byte octs of eight offsets
   if (max_quads > 0xFFFFFFFFull) fail_kernel("graph too large for the 16-bit triangle-count lists");
   const uint64_t rows = std::max<uint64_t>(n, 1);
   g.tc_split.allocate(sizeof(uint32_t) * rows);
   g.tc_hw.allocate(sizeof(uint32_t) * 2 * rows);
-  g.tc_words.allocate(sizeof(uint16_t) * 4 * max_quads);
+  g.tc_words.allocate(sizeof(uint16_t) * 8 * max_quads);
   DeviceMemory qoff, scan;
   qoff.allocate(sizeof(uint32_t) * rows);
   scan.allocate(sizeof(uint32_t) * scan_scratch_words(rows));
@@ -161,7 +161,7 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     if (tc_hot16()) {
       ensure_hot16(g, hot_base, parts);
       launch(dev, get_kernel(dev, "tc", "tc_bitmap16"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt,
-             g.tc_split.as<const uint32_t>(), g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint2 quads */, row_hi,
+             g.tc_split.as<const uint32_t>(), g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
              row_lo, hot_base, single,
              light_chunk, parts, tc_split_min(), counters, result);
     } else {

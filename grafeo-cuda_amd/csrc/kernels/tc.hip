@@ -417,85 +417,62 @@ __device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, 
 }
 
 // ---- 16-bit hot lists (tc_bitmap16) ----
-// The hot part of a list is stored as quads: 8 bytes holding four offsets
-// y = w - hot_base (< TCB_HOT_BITS), lowest half first. A part whose length is
-// not a multiple of four ends in pad offsets TCB_HOT_BITS, whose lookups read
-// the bitmap's zero word. So one 8-byte load carries four entries (the
-// texture path's work goes with the lane loads, not their width), and no
-// entry needs a mask or a clamp.
+// The hot part of a list is stored as octs: 16 bytes holding eight offsets
+// y = w - hot_base (< TCB_HOT_BITS). A part whose length is not a multiple of
+// eight ends in pad offsets TCB_HOT_BITS, whose lookups read the bitmap's zero
+// word. So one 16-byte load carries eight entries (the texture path's work
+// goes with the lane loads, not their width), and no entry needs a mask or a
+// clamp.
 #define TCB_PAD16 TCB_HOT_BITS
 
 __device__ __forceinline__ uint32_t hot16_hit(const uint32_t* s_bits, uint32_t y) {
   return word_bit(s_bits[y >> 5], y);
 }
 
-__device__ __forceinline__ uint32_t hot16x4_hits(const uint32_t* s_bits, uint2 x) {
-  return hot16_hit(s_bits, x.x & 0xFFFFu) + hot16_hit(s_bits, x.x >> 16) + hot16_hit(s_bits, x.y & 0xFFFFu) +
-         hot16_hit(s_bits, x.y >> 16);
+__device__ __forceinline__ uint32_t hot16x2_hits(const uint32_t* s_bits, uint32_t x) {
+  return hot16_hit(s_bits, x & 0xFFFFu) + hot16_hit(s_bits, x >> 16);
 }
 
-// As probe_list over the quads Q[0, nq) of a hot part: whole steps of
-// 2 x 64 quads (512 entries, two loads in flight per lane), then one guarded
-// step (quad index clamped to the last quad, hits masked by quad).
-__device__ __forceinline__ uint32_t probe16_list(const uint2* Q, uint32_t nq, const uint32_t* s_bits) {
+__device__ __forceinline__ uint32_t hot16x8_hits(const uint32_t* s_bits, uint4 x) {
+  return hot16x2_hits(s_bits, x.x) + hot16x2_hits(s_bits, x.y) + hot16x2_hits(s_bits, x.z) +
+         hot16x2_hits(s_bits, x.w);
+}
+
+// As probe_list over the octs O[0, no) of a hot part: whole steps of 64 octs
+// (512 entries), then one guarded step (oct index clamped to the last oct,
+// hits masked by oct).
+__device__ __forceinline__ uint32_t probe16_list(const uint4* O, uint32_t no, const uint32_t* s_bits) {
   const uint32_t lane = lane_id();
   uint32_t local = 0;
   uint32_t kk = 0;
-  for (; kk + 2 * GC_WAVE <= nq; kk += 2 * GC_WAVE) {
-    uint2 x[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) x[q] = Q[kk + q * GC_WAVE + lane];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) local += hot16x4_hits(s_bits, x[q]);
-  }
-  if (kk < nq) {
-    const uint32_t left = nq - kk;  // 1 .. 127, wave-uniform
-    const uint2* T = Q + kk;
-    if (left <= GC_WAVE) {
-      const uint2 x = T[min(lane, left - 1)];
-      local += lane < left ? hot16x4_hits(s_bits, x) : 0u;
-    } else {
-      uint2 x[2];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) x[q] = T[min(q * GC_WAVE + lane, left - 1)];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) local += q * GC_WAVE + lane < left ? hot16x4_hits(s_bits, x[q]) : 0u;
-    }
+  for (; kk + GC_WAVE <= no; kk += GC_WAVE) local += hot16x8_hits(s_bits, O[kk + lane]);
+  if (kk < no) {
+    const uint32_t left = no - kk;  // 1 .. 63, wave-uniform
+    const uint4 x = O[kk + min(lane, left - 1)];
+    local += lane < left ? hot16x8_hits(s_bits, x) : 0u;
   }
   return local;
 }
 
-// As probe_pair over the quads of two hot parts (four loads in flight per lane).
-__device__ __forceinline__ uint32_t probe16_pair(const uint2* Q1, uint32_t n1, const uint2* Q2, uint32_t n2,
+// As probe_pair over the octs of two hot parts (two 16-byte loads in flight
+// per lane, sixteen entries).
+__device__ __forceinline__ uint32_t probe16_pair(const uint4* O1, uint32_t n1, const uint4* O2, uint32_t n2,
                                                  const uint32_t* s_bits) {
   const uint32_t lane = lane_id();
   uint32_t local = 0;
   uint32_t kk = 0;
   const uint32_t nmin = min(n1, n2), nmax = max(n1, n2);
-  for (; kk + 2 * GC_WAVE <= nmin; kk += 2 * GC_WAVE) {
-    uint2 x[4];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      x[q] = Q1[kk + q * GC_WAVE + lane];
-      x[2 + q] = Q2[kk + q * GC_WAVE + lane];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) local += hot16x4_hits(s_bits, x[q]);
+  for (; kk + GC_WAVE <= nmin; kk += GC_WAVE) {
+    const uint4 x1 = O1[kk + lane];
+    const uint4 x2 = O2[kk + lane];
+    local += hot16x8_hits(s_bits, x1) + hot16x8_hits(s_bits, x2);
   }
-  for (; kk < nmax; kk += 2 * GC_WAVE) {
-    uint2 x[4];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const uint32_t i = kk + q * GC_WAVE + lane;
-      x[q] = Q1[min(i, n1 - 1)];
-      x[2 + q] = Q2[min(i, n2 - 1)];
-    }
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const uint32_t i = kk + q * GC_WAVE + lane;
-      local += i < n1 ? hot16x4_hits(s_bits, x[q]) : 0u;
-      local += i < n2 ? hot16x4_hits(s_bits, x[2 + q]) : 0u;
-    }
+  for (; kk < nmax; kk += GC_WAVE) {
+    const uint32_t i = kk + lane;
+    const uint4 x1 = O1[min(i, n1 - 1)];
+    const uint4 x2 = O2[min(i, n2 - 1)];
+    local += i < n1 ? hot16x8_hits(s_bits, x1) : 0u;
+    local += i < n2 ? hot16x8_hits(s_bits, x2) : 0u;
   }
   return local;
 }
@@ -539,7 +516,7 @@ __device__ __forceinline__ uint32_t probe_cold(const uint32_t* L, uint32_t lj, c
 template <bool kHot16>
 __device__ __forceinline__ void tc_bitmap_body(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint2* __restrict__ quads,
+    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint4* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits) {
@@ -624,7 +601,7 @@ __device__ __forceinline__ void tc_bitmap_body(
           uint32_t local = 0;
           for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
             const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
-            // kHot16: [st, st + len) are the quads of L(v)'s hot part and
+            // kHot16: [st, st + len) are the octs of L(v)'s hot part and
             // [cst, cst + clen) its cold part in targets (walked only when
             // the row has a cold prefix); otherwise all of L(v) in targets
             uint32_t st = 0, len = 0, cst = 0, clen = 0;
@@ -702,14 +679,14 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
 }
 
 // As tc_bitmap, with the hot part of every list read as 16-bit offsets from
-// hot_base, four to an 8-byte quad (quads; hw[v] = {first quad, quads} of
+// hot_base, eight to a 16-byte oct (quads; hw[v] = {first oct, octs} of
 // L(v)'s hot part, split[v] = the first entry of L(v) at or above hot_base):
-// half the bytes and a quarter of the loads for the entries that carry almost
+// half the bytes and an eighth of the loads for the entries that carry almost
 // all of the streaming, no clamp and no cold test per hot entry, and the cold
 // parts walked apart, only for the rows that have a cold prefix.
 extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint2* __restrict__ quads,
+    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint4* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
@@ -720,9 +697,9 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
 
 // The hot lists of tc_bitmap16, in two passes around an exclusive scan of
 // nquads: tc_hot16_rows sets split[v] (the first index of [lo[v], hi[v]) whose
-// target is at or above hot_base) and the quads of the hot part; tc_hot16_fill
-// writes every hot entry's offset into its row's quads (from the scanned
-// quad offset), the pads of the last quad, and hw[v]. The rows are laid out part
+// target is at or above hot_base) and the octs of the hot part; tc_hot16_fill
+// writes every hot entry's offset into its row's octs (from the scanned oct
+// offset), the pads of the last oct, and hw[v]. The rows are laid out part
 // by part (part v % parts, then v), in the middle-vertex parts of tc_bitmap16:
 // each part's lists are then contiguous, and an XCD's L2 fetches no line
 // shared with another part's lists. part_slot gives row v's place there.
@@ -738,22 +715,22 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_rows(
     const uint32_t b = hi[v];
     const uint32_t sp = lower_bound_u32(targets, lo[v], b, hot_base);
     split[v] = sp;
-    nquads[part_slot(static_cast<uint32_t>(v), n, parts)] = (b - sp + 3) >> 2;
+    nquads[part_slot(static_cast<uint32_t>(v), n, parts)] = (b - sp + 7) >> 3;
   }
 }
 
-// Within a row, entries are placed by blocks of 256 (64 quads; the last block
-// holds Q = ceil(rest / 4) quads): entry r of a block goes to quad r % Q,
-// half r / Q. A probe's lane reads quad q of a block and looks up its four
-// halves in turn, so each LDS lookup instruction of the wave covers Q
+// Within a row, entries are placed by blocks of 512 (64 octs; the last block
+// holds Q = ceil(rest / 8) octs): entry r of a block goes to oct r % Q, slot
+// r / Q. A probe's lane reads oct q of a block and looks up its eight
+// offsets in turn, so each LDS lookup instruction of the wave covers Q
 // consecutive entries of the list (close ids, few bitmap words and bank
-// conflicts) rather than entries four apart. The count does not depend on the
-// order of a list's entries.
+// conflicts) rather than entries eight apart. The count does not depend on
+// the order of a list's entries.
 __device__ __forceinline__ uint64_t quad_slot(uint32_t e, uint32_t len) {
-  const uint32_t b = e >> 8, r = e & 255u;
-  const uint32_t rest = len - (b << 8);
-  const uint32_t q_count = rest >= 256u ? 64u : (rest + 3u) >> 2;
-  return 4ull * (64ull * b + r % q_count) + r / q_count;
+  const uint32_t b = e >> 9, r = e & 511u;
+  const uint32_t rest = len - (b << 9);
+  const uint32_t q_count = rest >= 512u ? 64u : (rest + 7u) >> 3;
+  return 8ull * (64ull * b + r % q_count) + r / q_count;
 }
 
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
@@ -767,22 +744,22 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
       const uint32_t sp = split[v];
       const uint32_t end = offsets[v + 1];
       if (i >= sp && i < end)
-        half[4ull * qoff[part_slot(v, n, parts)] + quad_slot(static_cast<uint32_t>(i) - sp, end - sp)] =
+        half[8ull * qoff[part_slot(v, n, parts)] + quad_slot(static_cast<uint32_t>(i) - sp, end - sp)] =
             static_cast<uint16_t>(targets[i] - hot_base);
     }
     if (i < n) {
       const uint32_t v = static_cast<uint32_t>(i);
       const uint32_t q = qoff[part_slot(v, n, parts)];
       const uint32_t len = offsets[v + 1] - split[v];
-      const uint32_t nq = (len + 3) >> 2;
+      const uint32_t nq = (len + 7) >> 3;
       hw[v] = make_uint2(q, nq);
-      // the last block's free halves (at most three) hold the pad
-      if (len & 3u) {
-        const uint32_t last = (len - 1) >> 8;
-        const uint32_t rest = len - (last << 8);
-        const uint32_t q_count = (rest + 3u) >> 2;
-        for (uint32_t r = rest; r < 4 * q_count; ++r)
-          half[4ull * (q + 64ull * last + r % q_count) + r / q_count] = static_cast<uint16_t>(TCB_PAD16);
+      // the last block's free slots (at most seven) hold the pad
+      if (len & 7u) {
+        const uint32_t last = (len - 1) >> 9;
+        const uint32_t rest = len - (last << 9);
+        const uint32_t q_count = (rest + 7u) >> 3;
+        for (uint32_t r = rest; r < 8 * q_count; ++r)
+          half[8ull * (q + 64ull * last + r % q_count) + r / q_count] = static_cast<uint16_t>(TCB_PAD16);
       }
     }
   }

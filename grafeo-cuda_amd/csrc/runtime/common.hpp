@@ -312,11 +312,11 @@ struct Graph {
   bool ktruss_rev_ready = false;
   DeviceMemory bfs_top;       // u32[4][n] the four highest neighbours of each relabelled row
   bool bfs_top_ready = false;
-  // The DAG lists' entries among the top 32768 ids as 16-bit offsets, four to
-  // an 8-byte quad, each row from a quad boundary (tc_bitmap16).
+  // The DAG lists' entries among the top 32768 ids as 16-bit offsets, eight to
+  // a 16-byte oct, each row from an oct boundary (tc_bitmap16).
   DeviceMemory tc_split;      // u32[n] first such entry of each relabelled row
-  DeviceMemory tc_hw;         // uint2[n] {first quad, quads} of each row
-  DeviceMemory tc_words;      // 8-byte quads of four offsets of all rows, part by part
+  DeviceMemory tc_hw;         // uint2[n] {first oct, octs} of each row
+  DeviceMemory tc_words;      // 16-byte octs of eight offsets of all rows, part by part
   bool tc_hot16_ready = false;
   uint32_t tc_hot16_parts = 0;  // the middle-vertex parts the words are grouped by
   // The learned triangle-count schedule (method per degree bucket): a function
