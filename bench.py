@@ -94,8 +94,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(
-            json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
-                        "(see DESIGN.md), so --gpus must be 1", "n_gpus": args.gpus}),
+            json.dumps({"error": "this library runs on one GPU (the sharding across GPUs is a design, "
+                        "not code), so --gpus must be 1", "n_gpus": args.gpus}),
             flush=True,
         )
         return 2
