@@ -587,7 +587,15 @@ __device__ __forceinline__ void tc_bitmap_body(
               atomicOr(&s_bits[TCB_BLOOM_BASE + (h >> 5)], 1u << (h & 31));
             }
           } else {
-            c = __builtin_amdgcn_readfirstlane(lower_bound_u32(A, 0, a, hot_base));
+            // the row's first hot entry: one load of split with the 16-bit
+            // lists, else a binary search of the row
+            if (kHot16) {
+              const uint32_t u = n - 1 - (kc + static_cast<uint32_t>(jr));
+              c = __builtin_amdgcn_readfirstlane(split[u] -
+                                                 static_cast<uint32_t>(__builtin_amdgcn_readlane(row_lo_e, jr)));
+            } else {
+              c = __builtin_amdgcn_readfirstlane(lower_bound_u32(A, 0, a, hot_base));
+            }
             for (uint32_t j = c + lane; j < a; j += GC_WAVE) {
               const uint32_t y = A[j] - hot_base;
               atomicOr(&s_bits[y >> 5], 1u << (y & 31));
