@@ -29,9 +29,9 @@ struct BfsCounters {
   unsigned long long next_edges;
 };
 
-// The level's counters as the driver reads them: written to pinned host
+// The level's counters as the host reads them: written to pinned host
 // memory by bfs_publish (or by the scheduler's decision kernel, which adds
-// its decision), `seq` last, which the driver polls instead of waiting for
+// its decision), `seq` last, which the host polls instead of waiting for
 // the stream and copying.
 struct BfsPublished {
   BfsCounters counters;
@@ -61,7 +61,7 @@ extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__
   }
 }
 
-// Publishes the level's counters to the driver (one lane): copies them to
+// Publishes the level's counters to the host (one lane): copies them to
 // `host`, clears them for the next level, then sets host->seq = seq after a
 // system-scope fence, so the counters are visible before the sequence word.
 extern "C" __global__ __launch_bounds__(64) void bfs_publish(BfsCounters* __restrict__ counters,

@@ -196,7 +196,7 @@ struct SchedDecision {
 };
 
 // BfsPublished of bfs.hip: the level counters, this decision, and the
-// sequence word the driver polls.
+// sequence word the host polls.
 struct BfsPublishedView {
   BfsCountersView counters;
   SchedDecision decision;

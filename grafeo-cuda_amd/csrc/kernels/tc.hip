@@ -1,7 +1,7 @@
 // Triangle counting by sorted-list intersection, one 64-lane wave per vertex.
 //
 // Counts, over every vertex u and every v in L(u), the w in L(v) that are also
-// in L(u), where L(x) = targets[lo[x] .. hi[x]). The drivers choose L:
+// in L(u), where L(x) = targets[lo[x] .. hi[x]). The host code chooses L:
 //  - triangle_count (crates/grafeo-cuda-algos/kernels/triangle_counting.cu:246-315
 //    counts w > v > u with v, w in N(u) and w in N(v)): L(x) is the suffix of
 //    N(x) above x, and `after_only` restricts the search to the part of L(u)

@@ -65,8 +65,10 @@ def test_empty_and_comment_only_files(device, tmp_path):
 
 def test_random_files(device, tmp_path):
     rng = np.random.default_rng(21)
+    # the last four span several 64-byte chunks of the file: a long comment, fields far apart,
+    # ids with many leading zeros (valid for u64::from_str) and a long field that overflows
     kinds = ["{a}\t{b}", "{a} {b}", "  {a}  {b}  ", "{a} {b} {c}", "# {a}", "%", "", "x {b}", "{a}", "+{a} {b}",
-             "{a}\t{b}\r"]
+             "{a}\t{b}\r", "#" + "x" * 300, "{a}" + " " * 150 + "{b}", "0" * 90 + "{a} 000{b}", "9" * 70 + " {b}"]
     for t in range(12):
         lines = []
         ids = rng.integers(0, [50, 10**6, 2**40][t % 3], size=3000)
