@@ -233,6 +233,14 @@ def main() -> int:
     # each level's counters), not kernel time alone.
     add_roofline("bfs_traversal", bfs_bytes, 1000.0 * per_algo["bfs_ms_per_source"])
     rooflines["bfs_traversal"]["note"] = "avg_launch_us is the whole BFS call per source"
+    bfs_traffic = bfs_call_traffic(f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}")
+    if bfs_traffic is not None:
+        r = rooflines["bfs_traversal"]
+        r["traffic"] = bfs_traffic
+        r["traffic_frac"] = bfs_traffic / (r["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS
+        r["note"] += ("; traffic sums the measured bytes of the level kernels (bfs_*, without the "
+                      "once-per-graph bfs_top_build) over the profiled calls, per call (bfs_init "
+                      "dispatches)")
     # The dominant kernel: the one with the most event time in the profiled step.
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
     roofline = dict(rooflines.get(dominant) or rooflines.get("pr_spmv") or {})
@@ -495,6 +503,25 @@ def load_traffic(workload_key: str) -> dict:
     if t.get("workload") != workload_key:
         return {}
     return {k: v["hbm_bytes_per_launch"] for k, v in t.get("kernels", {}).items()}
+
+
+def bfs_call_traffic(workload_key: str):
+    """HBM bytes of one BFS call: the measured bytes of every level kernel (bfs_*) over the
+    profiled calls, divided by the calls (one bfs_init each); None without a measurement."""
+    try:
+        with open(os.path.join(REPO, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+    except (FileNotFoundError, ValueError):
+        return None
+    if t.get("workload") != workload_key:
+        return None
+    ks = t.get("kernels", {})
+    calls = ks.get("bfs_init", {}).get("dispatches", 0)
+    if not calls:
+        return None
+    total = sum(v["hbm_bytes_per_launch"] * v["dispatches"] for k, v in ks.items()
+                if k.startswith("bfs_") and k != "bfs_top_build")
+    return total / calls
 
 
 def cpu_baseline(offsets, targets, sources, iters, threads, m) -> dict:
