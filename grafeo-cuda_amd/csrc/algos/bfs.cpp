@@ -85,6 +85,14 @@ constexpr uint32_t kChainCap = 1u << 18;
 constexpr uint64_t kChainEdges = 1ull << 20;
 constexpr uint32_t kChainRunLevels = 1u << 30;
 constexpr uint32_t kChainLevels = 16;
+// Bottom-up levels the fixed rule still runs, launched ahead, after Beamer's
+// rule would return to top-down (their frontiers are small and each costs
+// less than a top-down level with its read-back; the cap bounds path-like
+// graphs, where bottom-up rescans the unvisited vertices at every level).
+constexpr uint32_t kTailBottomUp = 3;
+// Publication slots in the pinned area, one per sequence-number parity: a
+// level launched ahead publishes into the other slot than the one read.
+constexpr size_t kPubStride = 128;
 
 // One traversal from one source on the relabelled graph: the frontier is a
 // queue (top-down, chained levels) or a bitmap (bottom-up), converted
@@ -103,8 +111,16 @@ struct Traversal {
   BfsChainState* chain_state;
   const uint32_t* off;
   const uint32_t* tgt;
-  BfsPublished* pub;      // pinned host memory the level's publication lands in
+  BfsPublished* pub;      // pinned host memory the levels' publications land in (two slots)
   BfsPublished* pub_dev;  // the same, as kernels address it
+
+  // The slot of publication `seq`, as the host reads it and as kernels write it.
+  BfsPublished* slot(uint32_t seq) const {
+    return reinterpret_cast<BfsPublished*>(reinterpret_cast<char*>(pub) + (seq & 1u) * kPubStride);
+  }
+  BfsPublished* slot_dev(uint32_t seq) const {
+    return reinterpret_cast<BfsPublished*>(reinterpret_cast<char*>(pub_dev) + (seq & 1u) * kPubStride);
+  }
   Kernel k_td, k_deg, k_scan_small, k_bu, k_q2b, k_b2q, k_chain_expand, k_chain_advance, k_decide, k_publish;
   uint32_t cu;
   // state at the start of the next level
@@ -138,6 +154,8 @@ struct Traversal {
     k_chain_advance = get_kernel(dev, "bfs", "bfs_chain_advance");
     k_decide = get_kernel(dev, "mlp", "sched_bfs_decide");
     k_publish = get_kernel(dev, "bfs", "bfs_publish");
+    static_assert(sizeof(BfsPublished) <= kPubStride, "publication slot size");
+    static_assert(Device::kPinnedKernelOffset + 2 * kPubStride <= Device::kPinnedBytes, "pinned area size");
     pub = reinterpret_cast<BfsPublished*>(static_cast<char*>(dev.pinned) + Device::kPinnedKernelOffset);
     pub_dev = reinterpret_cast<BfsPublished*>(static_cast<char*>(dev.pinned_device) + Device::kPinnedKernelOffset);
     cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
@@ -192,7 +210,7 @@ struct Traversal {
     const uint32_t mask = (1u << kTopDown) | (g.symmetric ? (1u << kBottomUp) : 0u) | (1u << kChained);
     launch(dev, k_decide, 1u, 64u, 0, params, norm, level_state(), &rb->counters[0], from_counters ? 1u : 0u,
            mask, kChainCap, static_cast<unsigned long long>(kChainEdges), &rb->decision,
-           publish ? pub_dev : static_cast<BfsPublished*>(nullptr), seq);
+           publish ? slot_dev(seq) : static_cast<BfsPublished*>(nullptr), seq);
   }
 
   // Waits until a kernel has published sequence number `seq` to `pub`: the
@@ -200,13 +218,14 @@ struct Traversal {
   // us less per level), and checks the stream every few thousand polls so
   // that a failed launch is reported instead of waited for.
   void wait_published(uint32_t seq) {
+    BfsPublished* p = slot(seq);
     for (uint32_t spins = 1;; ++spins) {
-      if (__atomic_load_n(&pub->seq, __ATOMIC_ACQUIRE) == seq) return;
+      if (__atomic_load_n(&p->seq, __ATOMIC_ACQUIRE) == seq) return;
       if ((spins & 4095u) == 0) {
         const hipError_t err = hipStreamQuery(dev.stream);
         if (err == hipSuccess) {
           // the stream is idle, so the publishing kernel has ended
-          if (__atomic_load_n(&pub->seq, __ATOMIC_ACQUIRE) == seq) return;
+          if (__atomic_load_n(&p->seq, __ATOMIC_ACQUIRE) == seq) return;
           fail_kernel("BFS level counters were not published");
         }
         if (err != hipErrorNotReady) hip_check(err, "BFS level");
@@ -266,13 +285,64 @@ struct Traversal {
              &rb->counters[0], next_level);
     } else {
       to_bitmap();
-      Grid1D bg = wave_per_item(words64);
-      // the rows of degree 0 come first in the degree order: skip their chunks
-      const uint32_t first_chunk = g.rl.dangling / 64;
-      launch(dev, k_bu, std::min(bg.blocks, cu * 8), bg.threads, 0, off, tgt, top_entries(), dist,
-             reinterpret_cast<const uint32_t*>(bitmap[cur]), bitmap[cur ^ 1], n, first_chunk,
-             &rb->counters[0], next_level);
+      launch_bottom_up(cur, next_level);
     }
+  }
+
+  // One bottom-up level from the frontier bitmap[in] into bitmap[in ^ 1]; its
+  // grid depends on n only, not on the frontier.
+  void launch_bottom_up(int in, int32_t next_level) {
+    Grid1D bg = wave_per_item(words64);
+    // the rows of degree 0 come first in the degree order: skip their chunks
+    const uint32_t first_chunk = g.rl.dangling / 64;
+    launch(dev, k_bu, std::min(bg.blocks, cu * 8), bg.threads, 0, off, tgt, top_entries(), dist,
+           reinterpret_cast<const uint32_t*>(bitmap[in]), bitmap[in ^ 1], n, first_chunk, &rb->counters[0],
+           next_level);
+  }
+
+  // Publishes (and clears) the level counters under a new sequence number.
+  uint32_t publish_counters() {
+    const uint32_t seq = ++dev.publish_seq;
+    launch(dev, k_publish, 1u, 64u, 0, &rb->counters[0], slot_dev(seq), seq);
+    return seq;
+  }
+
+  // Bottom-up levels under the fixed rule, each launched before the counters
+  // of the level before it are read, so the read-back overlaps a level on the
+  // GPU instead of leaving it idle. A level launched ahead is a complete BFS
+  // level whatever those counters say: if the traversal has ended, it finds
+  // an empty frontier and changes nothing. Continues while Beamer's rule
+  // stays bottom-up and then for at most kTailBottomUp levels; returns when
+  // the traversal ends or with the next level to run top-down. `on_level` is
+  // called after each level's counters are applied.
+  template <class OnLevel>
+  void run_bottom_up(OnLevel&& on_level) {
+    to_bitmap();
+    int in = cur;
+    int32_t lvl = level;  // the level whose frontier is bitmap[in]
+    auto launch_next = [&]() {
+      launch_bottom_up(in, lvl + 1);
+      in ^= 1;
+      ++lvl;
+      return publish_counters();
+    };
+    uint32_t pending = launch_next();
+    uint32_t tail = 0;
+    bool ahead = true;
+    for (;;) {
+      const uint32_t next = ahead ? launch_next() : 0u;
+      wait_published(pending);
+      BfsCounters counters;
+      std::memcpy(&counters, &slot(pending)->counters, sizeof(counters));
+      finish_level(kBottomUp, counters);
+      on_level();
+      if (done || !ahead) break;
+      // the level after `next`: bottom-up while the rule says so, then the tail
+      if (static_cast<double>(n_f) < static_cast<double>(n) / kBeta) ++tail;
+      ahead = tail < kTailBottomUp;
+      pending = next;
+    }
+    top_down_dir = true;
   }
 
   // Applies the counters of the level just run (already copied to `c`).
@@ -376,16 +446,17 @@ struct Traversal {
       return;
     }
     launch_level(choice);
-    const uint32_t seq = ++dev.publish_seq;
+    uint32_t seq;
     if (decide_next) {
+      seq = ++dev.publish_seq;
       launch_decide(params, true, true, seq);
     } else {
-      launch(dev, k_publish, 1u, 64u, 0, &rb->counters[0], pub_dev, seq);
+      seq = publish_counters();
     }
     wait_published(seq);
     BfsCounters counters;
-    std::memcpy(&counters, &pub->counters, sizeof(counters));
-    if (decide_next) std::memcpy(&last_decision, &pub->decision, sizeof(last_decision));
+    std::memcpy(&counters, &slot(seq)->counters, sizeof(counters));
+    if (decide_next) std::memcpy(&last_decision, &slot(seq)->decision, sizeof(last_decision));
     finish_level(choice, counters);
   }
 
@@ -423,9 +494,30 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
   // device after each level and read back with the level's counters (one
   // round trip per level, as for the fixed rule).
   uint32_t choice = params != nullptr ? kTopDown : t.fixed_choice();
+  // Under the fixed rule, bottom-up levels are launched ahead of the read-back
+  // (GC_BFS_AHEAD=0: one level per round trip, as the learned rule runs).
+  const char* ahead_env = std::getenv("GC_BFS_AHEAD");
+  const bool ahead = params == nullptr && (ahead_env == nullptr || ahead_env[0] != '0');
   Stopwatch level_clock;
   while (!t.done) {
     if (!(t.allowed_mask() >> choice & 1u)) choice = kTopDown;
+    if (ahead && choice == kBottomUp) {
+      t.run_bottom_up([&] {
+        g.last_schedule.push_back(kBottomUp);
+        if (trace) {
+          if (t.done) {
+            std::fprintf(stderr, "bfs level %d bottom-up (launched ahead) -> done (%.3f ms)\n", t.level,
+                         level_clock.seconds() * 1e3);
+          } else {
+            std::fprintf(stderr, "bfs level %d bottom-up (launched ahead) -> level %d frontier %llu (%.3f ms)\n",
+                         t.level - 1, t.level, static_cast<unsigned long long>(t.n_f), level_clock.seconds() * 1e3);
+          }
+          level_clock = Stopwatch();
+        }
+      });
+      choice = kTopDown;
+      continue;
+    }
     g.last_schedule.push_back(choice);
     const uint64_t n_f = t.n_f;
     const int32_t level = t.level;
