@@ -130,6 +130,7 @@ struct Traversal {
   int cur = 0;
   bool done = false;
   bool top_down_dir = true;  // the fixed rule's current direction
+  double alpha = kAlpha;     // the fixed rule's top-down -> bottom-up threshold (GC_BFS_ALPHA)
 
   Traversal(Graph& graph, uint32_t src)
       : g(graph), dev(*graph.dev), n(graph.n), words64((static_cast<uint64_t>(graph.n) + 63) / 64) {
@@ -165,6 +166,7 @@ struct Traversal {
     launch(dev, get_kernel(dev, "bfs", "bfs_init"), std::min(vgrid.blocks, cu * 16), vgrid.threads, 0, dist, n,
            src, queue[0], &rb->counters[0]);
     m_f = hoff[src + 1] - hoff[src];
+    if (const char* a = std::getenv("GC_BFS_ALPHA")) alpha = std::max(1.0, std::atof(a));
     m_u = static_cast<uint64_t>(g.m) - m_f;
   }
 
@@ -192,7 +194,7 @@ struct Traversal {
 
   // The fixed rule: Beamer's direction switch (never the chained levels).
   uint32_t fixed_choice() {
-    if (top_down_dir && g.symmetric && static_cast<double>(m_f) > static_cast<double>(m_u) / kAlpha) {
+    if (top_down_dir && g.symmetric && static_cast<double>(m_f) > static_cast<double>(m_u) / alpha) {
       top_down_dir = false;
     } else if (!top_down_dir && static_cast<double>(n_f) < static_cast<double>(n) / kBeta) {
       top_down_dir = true;
