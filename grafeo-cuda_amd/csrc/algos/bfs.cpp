@@ -496,10 +496,12 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
   // device after each level and read back with the level's counters (one
   // round trip per level, as for the fixed rule).
   uint32_t choice = params != nullptr ? kTopDown : t.fixed_choice();
-  // Under the fixed rule, bottom-up levels are launched ahead of the read-back
-  // (GC_BFS_AHEAD=0: one level per round trip, as the learned rule runs).
+  // Bottom-up levels are launched ahead of the read-back (GC_BFS_AHEAD=0: one
+  // level per round trip). Under the learned rule, a bottom-up choice starts
+  // such a run, which ends as under the fixed rule; the network then decides
+  // the next level from the state after it (one round trip).
   const char* ahead_env = std::getenv("GC_BFS_AHEAD");
-  const bool ahead = params == nullptr && (ahead_env == nullptr || ahead_env[0] != '0');
+  const bool ahead = ahead_env == nullptr || ahead_env[0] != '0';
   Stopwatch level_clock;
   while (!t.done) {
     if (!(t.allowed_mask() >> choice & 1u)) choice = kTopDown;
@@ -517,7 +519,8 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
           level_clock = Stopwatch();
         }
       });
-      choice = kTopDown;
+      if (t.done) break;
+      choice = params != nullptr ? t.decide_now(params) : kTopDown;
       continue;
     }
     g.last_schedule.push_back(choice);
