@@ -34,6 +34,9 @@ HOST_SOURCES = sorted((HERE / "csrc" / "runtime").glob("*.cpp")) + sorted(
     (HERE / "csrc" / "algos").glob("*.cpp")
 )
 HOST_HEADERS = sorted((HERE / "csrc" / "runtime").glob("*.hpp")) + [REPO / "include" / "grafeo_hip.h"]
+# Host sources built by hipcc: the static device scratch of the allocation-free entry point
+# (a __device__ array whose address the HIP runtime resolves), with its gfx950 code object.
+HIP_HOST_SOURCES = sorted((HERE / "csrc" / "runtime").glob("*.hip"))
 CXX = "g++"
 CXXFLAGS = [
     "-std=c++17",
@@ -107,6 +110,16 @@ def compile_host(src: Path, force: bool) -> Path:
     return out
 
 
+def compile_hip_host(src: Path, force: bool) -> Path:
+    rel = src.relative_to(HERE).with_suffix(".o")
+    out = BUILD / str(rel).replace(os.sep, "_")
+    if stale(out, [src, *HOST_HEADERS, Path(co.__file__)], force):
+        flags = [f for f in co.HIPCC_FLAGS if f != "--genco"]
+        run([co.HIPCC, *flags, "-fPIC", "-D__HIP_PLATFORM_AMD__", f"-I{HERE / 'csrc' / 'runtime'}", "-c",
+             str(src), "-o", str(out)])
+    return out
+
+
 def build(force: bool = False, jobs: int | None = None) -> Path:
     BUILD.mkdir(exist_ok=True)
     jobs = jobs or min(16, os.cpu_count() or 4)
@@ -118,6 +131,7 @@ def build(force: bool = False, jobs: int | None = None) -> Path:
     with cf.ThreadPoolExecutor(jobs) as pool:
         co_futs = {name: pool.submit(build_code_object, name, src, force) for name, src in sources}
         host_futs = [pool.submit(compile_host, src, force) for src in HOST_SOURCES]
+        host_futs += [pool.submit(compile_hip_host, src, force) for src in HIP_HOST_SOURCES]
         objects = {name: f.result() for name, f in co_futs.items()}
         entries = [(name, objects[name], co.source_digest(src)) for name, src in sources]
         s_path, c_path = write_blob(entries)

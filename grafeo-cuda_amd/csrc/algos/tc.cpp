@@ -8,9 +8,12 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../runtime/common.hpp"
+#include "../runtime/noalloc.hpp"
 
 namespace gc {
 
@@ -127,9 +130,12 @@ void launch_tc_count(Graph& g, const uint32_t* lo, const uint32_t* hi, const uin
 }
 
 // One symmetric-graph method over rows [row_lo, row_hi) of the relabelled DAG;
-// `counters` are kCounterWords zeroed words.
+// `counters` are kCounterWords zeroed words. With `no_alloc` (bitmap method
+// only) nothing is built, allocated or timed: tc_bitmap16 runs if its 16-bit
+// lists already exist for the current parts, tc_bitmap (which needs none)
+// otherwise; both count the same triangles.
 void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, uint32_t* counters,
-                   unsigned long long* result) {
+                   unsigned long long* result, bool no_alloc = false) {
   Device& dev = *g.dev;
   const uint32_t n = g.n;
   if (row_hi <= row_lo) return;
@@ -158,15 +164,20 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     const uint32_t single = tc_heavy_rows(g, row_lo, row_hi), parts = tc_parts();
     const uint64_t tasks =
         parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
-    if (tc_hot16()) {
-      ensure_hot16(g, hot_base, parts);
-      launch(dev, get_kernel(dev, "tc", "tc_bitmap16"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt,
-             g.tc_split.as<const uint32_t>(), g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
-             row_lo, hot_base, single,
-             light_chunk, parts, tc_split_min(), counters, result);
+    auto go = [&](const char* name, auto... args) {
+      const Kernel k = get_kernel(dev, "tc", name);
+      if (no_alloc) launch_untimed(dev, k, grid_for(tasks), 4 * kWaveSize, 0, args...);
+      else launch(dev, k, grid_for(tasks), 4 * kWaveSize, 0, args...);
+    };
+    const bool hot16 = tc_hot16() && (!no_alloc || (g.tc_hot16_ready && g.tc_hot16_parts == parts));
+    if (hot16) {
+      if (!no_alloc) ensure_hot16(g, hot_base, parts);
+      go("tc_bitmap16", fwd, end, tgt, g.tc_split.as<const uint32_t>(),
+         g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
+         row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result);
     } else {
-      launch(dev, get_kernel(dev, "tc", "tc_bitmap"), grid_for(tasks), 4 * kWaveSize, 0, fwd, end, tgt, row_hi,
-             row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result);
+      go("tc_bitmap", fwd, end, tgt, row_hi, row_lo, hot_base, single, light_chunk, parts, tc_split_min(),
+         counters, result);
     }
   }
 }
@@ -177,6 +188,18 @@ uint32_t* tc_scratch(Graph& g) {
   uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", sizeof(uint32_t) * words);
   fill_words(*g.dev, scratch, words, 0);
   return scratch;
+}
+
+// First relabelled row of degree >= d (rows are sorted by degree), or n.
+uint32_t first_row_of_degree(const Graph& g, uint64_t d) {
+  const std::vector<uint32_t>& off = g.rl.host_offsets;
+  uint32_t lo = 0, hi = g.n;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (off[mid + 1] - off[mid] >= d) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
 }
 
 // Degree buckets of the relabelled rows: bucket b holds the rows of degree in
@@ -350,6 +373,40 @@ uint64_t triangle_count_symmetric(Graph& g) {
   return read_result(dev, result);
 }
 
+// Calls of the allocation-free count on one GPU share its static scratch:
+// they are serialized per device ordinal (also across gc_device handles).
+std::mutex& noalloc_mutex(int ordinal) {
+  static std::mutex mutexes[64];
+  return mutexes[static_cast<unsigned>(ordinal) % 64u];
+}
+
+// gc_triangle_count_noalloc: the fixed rule's bitmap count of a prepared
+// symmetric graph with its u64 result and one launch's work counters in the
+// device's static scratch (zeroed first by the fill kernel), so the call
+// allocates, builds and loads nothing.
+uint64_t triangle_count_noalloc(Graph& g) {
+  Device& dev = *g.dev;
+  if (dev.noalloc_scratch == nullptr)
+    fail(GC_ERR_DRIVER, "the allocation-free scratch is not available on this device: " + dev.noalloc_error);
+  if (!g.prepared)
+    fail_kernel("gc_triangle_count_noalloc needs a prepared graph: call gc_graph_prepare first");
+  if (!g.symmetric) fail_kernel("gc_triangle_count_noalloc needs a symmetric CSR");
+  constexpr uint64_t words = 2 + uint64_t(kCounterWords);  // u64 result, then the work counters
+  if (words * sizeof(uint32_t) > kNoAllocScratchBytes) {
+    fail_kernel("the triangle count needs " + std::to_string(words * sizeof(uint32_t)) +
+                " bytes of scratch, more than the " + std::to_string(kNoAllocScratchBytes) +
+                "-byte static scratch");
+  }
+  const std::lock_guard<std::mutex> lock(noalloc_mutex(dev.ordinal));
+  uint32_t* scratch = static_cast<uint32_t*>(dev.noalloc_scratch);
+  const Grid1D fg = thread_per_item(words);
+  launch_untimed(dev, get_kernel(dev, "util", "fill_u32"), fg.blocks, fg.threads, 0, scratch, words, 0u);
+  auto* result = reinterpret_cast<unsigned long long*>(scratch);
+  // rows of degree 0 or 1 (sorted first) close no triangle
+  launch_method(g, kBitmap, first_row_of_degree(g, 2), g.n, scratch + 2, result, true);
+  return read_result(dev, result);
+}
+
 uint64_t triangle_count_oriented(Graph& g) {
   prepare(g);
   // sum over entries u->v of |N(u) intersect N(v)| is invariant under the
@@ -405,6 +462,14 @@ gc_status gc_triangle_count(gc_graph* h, uint64_t* total_triangles, double* secs
     if (total_triangles == nullptr) gc::fail_kernel("total_triangles is null");
     *total_triangles = call.g->m == 0 ? 0 : gc::triangle_count_symmetric(*call.g);
     if (secs) *secs = sw.seconds();
+  });
+}
+
+gc_status gc_triangle_count_noalloc(gc_graph* h, uint64_t* total_triangles) {
+  return guard([&] {
+    gc::GraphCall call(h);
+    if (total_triangles == nullptr) gc::fail_kernel("total_triangles is null");
+    *total_triangles = call.g->m == 0 ? 0 : gc::triangle_count_noalloc(*call.g);
   });
 }
 

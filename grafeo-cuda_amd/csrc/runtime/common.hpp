@@ -98,6 +98,10 @@ struct Device {
   // level counters, polled by the host); the bytes below by read-back copies.
   static constexpr size_t kPinnedKernelOffset = 2048;
   uint32_t publish_seq = 0;  // last sequence number given to a publishing kernel
+  // Device address of the static scratch array of the allocation-free entry
+  // point on this device (null if it could not be resolved: noalloc_error).
+  void* noalloc_scratch = nullptr;
+  std::string noalloc_error;
   // Kernel timing.
   bool profiling = false;
   std::vector<std::string> profile_only;  // kernels timed when profiling (empty: all)
@@ -144,6 +148,26 @@ void launch(Device& dev, const Kernel& k, uint32_t grid, uint32_t block, uint32_
   }
   profile_end(dev, k.name, start);
 }
+
+// As launch(), without the event-timing hooks (which may create events): for
+// the allocation-free entry point.
+template <class... Args>
+void launch_untimed(Device& dev, const Kernel& k, uint32_t grid, uint32_t block, uint32_t shmem,
+                    Args... args) {
+  if (grid == 0) return;
+  void* params[] = {karg(args)..., nullptr};
+  hipError_t err = hipModuleLaunchKernel(k.fn, grid, 1, 1, block, 1, 1, shmem, dev.stream, params,
+                                         nullptr);
+  if (err != hipSuccess) {
+    fail(GC_ERR_KERNEL, std::string(k.name) + " launch failed: " + hipGetErrorString(err));
+  }
+}
+
+// Resolves the device's allocation-free scratch (noalloc.hip) and loads the
+// code objects the allocation-free entry point launches, so that its calls
+// load and allocate nothing. Called once per device when it is opened; a
+// failure is kept in noalloc_error and reported by that entry point only.
+void noalloc_setup(Device& dev);
 
 // As launch(), but returns the launch status instead of throwing (for launch
 // shapes that a device may refuse, such as large dynamic LDS).

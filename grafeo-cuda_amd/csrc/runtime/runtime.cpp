@@ -14,6 +14,7 @@
 #include "code_objects.hpp"
 #include "common.hpp"
 #include "module_cache.hpp"
+#include "noalloc.hpp"
 
 namespace gc {
 
@@ -175,6 +176,27 @@ Kernel get_kernel(Device& dev, const char* family, const char* name) {
     it = module->functions.emplace(name, fn).first;
   }
   return Kernel{it->second, it->first.c_str()};
+}
+
+void noalloc_setup(Device& dev) {
+  dev.make_current();
+  void* ptr = nullptr;
+  const hipError_t err = noalloc_scratch_address(&ptr);
+  if (err != hipSuccess || ptr == nullptr) {
+    (void)hipGetLastError();
+    dev.noalloc_error = std::string("hipGetSymbolAddress of the static scratch failed: ") + hipGetErrorString(err);
+    return;
+  }
+  try {
+    // the kernels of gc_triangle_count_noalloc, loaded now so that it loads nothing
+    (void)get_kernel(dev, "util", "fill_u32");
+    (void)get_kernel(dev, "tc", "tc_bitmap16");
+    (void)get_kernel(dev, "tc", "tc_bitmap");
+  } catch (const Error& e) {
+    dev.noalloc_error = e.message;
+    return;
+  }
+  dev.noalloc_scratch = ptr;
 }
 
 // ------------------------------------------------------------- profiling --
@@ -360,6 +382,7 @@ gc_status gc_device_new(uint32_t ordinal, gc_device** out) {
                   "hipHostMalloc");
     std::memset(dev->pinned, 0, gc::Device::kPinnedBytes);
     gc::hip_check(hipHostGetDevicePointer(&dev->pinned_device, dev->pinned, 0), "hipHostGetDevicePointer");
+    gc::noalloc_setup(*dev);
     *out = reinterpret_cast<gc_device*>(dev.release());
   });
 }

@@ -91,6 +91,7 @@ def _load() -> ctypes.CDLL:
         "gc_device_free": (c_int, [c_void_p, c_void_p]),
         "gc_device_copy_to_host": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
         "gc_triangle_count": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_double)]),
+        "gc_triangle_count_noalloc": (c_int, [c_void_p, POINTER(c_uint64)]),
         "gc_triangle_count_oriented": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_double)]),
         "gc_bfs": (c_int, [c_void_p, c_uint32, i32p, u32p, i32p, POINTER(c_double)]),
         "gc_bfs_device": (c_int, [c_void_p, c_uint32, c_void_p, u32p, i32p, POINTER(c_double)]),
@@ -175,7 +176,7 @@ EXPORTED = [
     "gc_parse_tsv_to_edges", "gc_graph_from_csr", "gc_graph_from_csr_arrays", "gc_graph_free",
     "gc_graph_num_nodes", "gc_graph_num_edges", "gc_graph_device_offsets",
     "gc_graph_device_targets", "gc_graph_prepare", "gc_graph_is_symmetric", "gc_graph_dag_stats",
-    "gc_device_malloc", "gc_device_free", "gc_device_copy_to_host", "gc_triangle_count",
+    "gc_device_malloc", "gc_device_free", "gc_device_copy_to_host", "gc_triangle_count", "gc_triangle_count_noalloc",
     "gc_triangle_count_oriented", "gc_bfs", "gc_bfs_device", "gc_pagerank", "gc_pagerank_device",
     "gc_k_truss", "gc_truss_numbers", "gc_msbfs", "gc_msbfs_device", "gc_batch_from_csrs",
     "gc_batch_free", "gc_batch_num_graphs", "gc_batch_triangle_count", "gc_batch_set_config",

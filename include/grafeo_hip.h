@@ -200,6 +200,28 @@ gc_status gc_graph_dag_stats(gc_graph* g, uint64_t* oriented_entries, uint64_t* 
  * (the triangle count of a symmetric CSR; self-loops never count). */
 gc_status gc_triangle_count(gc_graph* g, uint64_t* total_triangles, double* secs);
 
+/* Allocation-free triangle count. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It returns what
+ * gc_triangle_count returns, computed by the fixed rule's bitmap kernel. It
+ * allocates no device memory, loads no code object, builds nothing and
+ * creates no events. The kernel's work counters and u64 result live in a static
+ * per-GPU array of GC_NOALLOC_SCRATCH_BYTES bytes. The array is resolved when
+ * the device is opened (gc_device_new) and held until the process ends. Every
+ * call on one GPU shares that array, which keeps the last call's contents:
+ * each call clears it first, and the library serializes such calls per GPU
+ * (also across gc_device handles of one ordinal). A caller that bypasses the
+ * library to use the array concurrently would race.
+ * Preconditions: a symmetric graph already prepared by gc_graph_prepare
+ * (otherwise GC_ERR_KERNEL, before any device work). A graph with no edges
+ * gives 0. If the library's 16-bit triangle lists exist (built by an earlier
+ * gc_triangle_count), the faster tc_bitmap16 kernel runs; otherwise tc_bitmap
+ * runs. GC_ERR_DRIVER if the scratch could not be resolved on this device.
+ * The scratch this count needs does not grow with the graph; a need above the
+ * array's size is GC_ERR_KERNEL. */
+#define GC_NOALLOC_SCRATCH_BYTES 4096
+gc_status gc_triangle_count_noalloc(gc_graph* g, uint64_t* total_triangles);
+
 /* triangle_counting::triangle_count_oriented
  * (crates/grafeo-cuda-algos/src/triangle_counting.rs:162-210): the sum over
  * CSR entries u->v of |N(u) intersect N(v)| (the triangle count of a
