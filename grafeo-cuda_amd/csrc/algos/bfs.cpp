@@ -109,6 +109,7 @@ struct Traversal {
   uint32_t* start;
   uint32_t* scan_buf;
   BfsChainState* chain_state;
+  BfsCounters* bu_counters;  // the two counter sets that bottom-up runs alternate between
   const uint32_t* off;
   const uint32_t* tgt;
   BfsPublished* pub;      // pinned host memory the levels' publications land in (two slots)
@@ -131,6 +132,11 @@ struct Traversal {
   bool done = false;
   bool top_down_dir = true;  // the fixed rule's current direction
   double alpha = kAlpha;     // the fixed rule's top-down -> bottom-up threshold (GC_BFS_ALPHA)
+  // In queue form, whether bitmap[cur] holds exactly the queue's vertices and
+  // bitmap[cur ^ 1] is empty (top-down levels then keep it so, and a switch to
+  // bottom-up needs no conversion). Kept for symmetric graphs only.
+  bool synced = false;
+  bool first_level = true;   // start[0] = 0 from bfs_init: the first level needs no scan
 
   Traversal(Graph& graph, uint32_t src)
       : g(graph), dev(*graph.dev), n(graph.n), words64((static_cast<uint64_t>(graph.n) + 63) / 64) {
@@ -143,6 +149,7 @@ struct Traversal {
     start = g.buffer<uint32_t>("bfs.start", sizeof(uint32_t) * n);
     scan_buf = g.buffer<uint32_t>("bfs.scan", sizeof(uint32_t) * scan_scratch_words(n));
     chain_state = g.buffer<BfsChainState>("bfs.chain", sizeof(BfsChainState));
+    bu_counters = g.buffer<BfsCounters>("bfs.bu_counters", 2 * sizeof(BfsCounters));
     off = g.rl.offsets.as<const uint32_t>();
     tgt = g.rl.targets.as<const uint32_t>();
     k_td = get_kernel(dev, "bfs", "bfs_topdown_edges");
@@ -164,7 +171,8 @@ struct Traversal {
     // four entries per lane and store, a grid-stride loop over at most 16 workgroups per CU
     Grid1D vgrid = thread_per_item(std::max<uint64_t>(n / 4, 1));
     launch(dev, get_kernel(dev, "bfs", "bfs_init"), std::min(vgrid.blocks, cu * 16), vgrid.threads, 0, dist, n,
-           src, queue[0], &rb->counters[0]);
+           src, queue[0], &rb->counters[0], bu_counters, start, bitmap[0], bitmap[1], words64);
+    synced = g.symmetric;
     m_f = hoff[src + 1] - hoff[src];
     if (const char* a = std::getenv("GC_BFS_ALPHA")) alpha = std::max(1.0, std::atof(a));
     m_u = static_cast<uint64_t>(g.m) - m_f;
@@ -238,10 +246,14 @@ struct Traversal {
   void to_queue() {
     if (!in_bitmap) return;
     fill_words(dev, &rb->counters[1], sizeof(BfsCounters) / 4, 0);
+    // the other bitmap (an earlier frontier) is cleared, so the queue and its
+    // bitmap are in step for the top-down levels that follow
+    fill_words(dev, bitmap[cur ^ 1], 2 * words64, 0);
     Grid1D bg = wave_per_item((words64 + 63) / 64);
     launch(dev, k_b2q, std::min(bg.blocks, cu * 16), bg.threads, 0,
            static_cast<const uint64_t*>(bitmap[cur]), words64, queue[cur], &rb->counters[1].next_size);
     in_bitmap = false;
+    synced = g.symmetric;
   }
 
   // The transposed four last entries of every row (built once per graph).
@@ -258,6 +270,10 @@ struct Traversal {
 
   void to_bitmap() {
     if (in_bitmap) return;
+    if (synced) {
+      in_bitmap = true;  // the top-down levels kept bitmap[cur] in step with the queue
+      return;
+    }
     fill_words(dev, bitmap[cur], 2 * words64, 0);
     Grid1D qg = thread_per_item(n_f);
     launch(dev, k_q2b, qg.blocks, qg.threads, 0, static_cast<const uint32_t*>(queue[cur]),
@@ -268,10 +284,14 @@ struct Traversal {
   // Launches one top-down or bottom-up level (counters into rb->counters[0]).
   void launch_level(uint32_t choice) {
     const int32_t next_level = level + 1;
+    const bool first = first_level;
+    first_level = false;
     if (choice == kTopDown) {
       to_queue();
       // (the level counters are clear: bfs_init and every publication clear them)
-      if (n_f <= kSmallFrontier) {
+      if (first && n_f == 1) {
+        // the source alone: bfs_init wrote its degree prefix start[0] = 0
+      } else if (n_f <= kSmallFrontier) {
         launch(dev, k_scan_small, 1u, 1024u, 0, off, static_cast<const uint32_t*>(queue[cur]),
                static_cast<uint32_t>(n_f), start, &rb->counters[0]);
       } else {
@@ -281,25 +301,31 @@ struct Traversal {
         exclusive_scan_u32(dev, start, n_f, scan_buf);
       }
       Grid1D tg = wave_per_item((m_f + 255) / 256);
+      uint32_t* cur_bits = synced ? reinterpret_cast<uint32_t*>(bitmap[cur]) : nullptr;
+      uint32_t* next_bits = synced ? reinterpret_cast<uint32_t*>(bitmap[cur ^ 1]) : nullptr;
       launch(dev, k_td, std::max(1u, std::min(tg.blocks, cu * 32)), tg.threads, 0, off, tgt, dist,
              static_cast<const uint32_t*>(queue[cur]), static_cast<uint32_t>(n_f),
              static_cast<const uint32_t*>(start), static_cast<uint32_t>(m_f), queue[cur ^ 1],
-             &rb->counters[0], next_level);
+             &rb->counters[0], next_level, cur_bits, next_bits);
     } else {
       to_bitmap();
-      launch_bottom_up(cur, next_level);
+      launch_bottom_up(cur, next_level, &rb->counters[0]);
     }
   }
 
   // One bottom-up level from the frontier bitmap[in] into bitmap[in ^ 1]; its
-  // grid depends on n only, not on the frontier.
-  void launch_bottom_up(int in, int32_t next_level) {
+  // grid depends on n only, not on the frontier. Its counts go to `counters`;
+  // with `pub_slot`, it first publishes the finished counters `pub` of the
+  // level before it under sequence number `seq`.
+  void launch_bottom_up(int in, int32_t next_level, BfsCounters* counters, BfsCounters* pub = nullptr,
+                        BfsPublished* pub_slot = nullptr, uint32_t seq = 0) {
     Grid1D bg = wave_per_item(words64);
     // the rows of degree 0 come first in the degree order: skip their chunks
     const uint32_t first_chunk = g.rl.dangling / 64;
     launch(dev, k_bu, std::min(bg.blocks, cu * 8), bg.threads, 0, off, tgt, top_entries(), dist,
-           reinterpret_cast<const uint32_t*>(bitmap[in]), bitmap[in ^ 1], n, first_chunk, &rb->counters[0],
-           next_level);
+           reinterpret_cast<const uint32_t*>(bitmap[in]), bitmap[in ^ 1], n, first_chunk, counters,
+           next_level, pub, pub_slot, seq);
+    synced = false;  // the queues no longer hold the frontier
   }
 
   // Publishes (and clears) the level counters under a new sequence number.
@@ -317,32 +343,43 @@ struct Traversal {
   // stays bottom-up and then for at most kTailBottomUp levels; returns when
   // the traversal ends or with the next level to run top-down. `on_level` is
   // called after each level's counters are applied.
+  // The levels alternate between the two bu_counters sets; each level
+  // launched ahead publishes the set of the level before it (so no
+  // publishing launch sits between two levels), and the run's last level is
+  // published by bfs_publish. A set is cleared by its publication; a level
+  // launched after the traversal ended counts nothing, and bfs_init clears
+  // both sets anyway.
   template <class OnLevel>
   void run_bottom_up(OnLevel&& on_level) {
     to_bitmap();
     int in = cur;
     int32_t lvl = level;  // the level whose frontier is bitmap[in]
-    auto launch_next = [&]() {
-      launch_bottom_up(in, lvl + 1);
-      in ^= 1;
-      ++lvl;
-      return publish_counters();
-    };
-    uint32_t pending = launch_next();
+    int set = 0;          // the counter set of the oldest level not yet read
+    launch_bottom_up(in, lvl + 1, &bu_counters[set]);
+    in ^= 1;
+    ++lvl;
+    uint32_t seq = ++dev.publish_seq;  // the publication of that level's set
     uint32_t tail = 0;
     bool ahead = true;
     for (;;) {
-      const uint32_t next = ahead ? launch_next() : 0u;
-      wait_published(pending);
+      if (ahead) {
+        launch_bottom_up(in, lvl + 1, &bu_counters[set ^ 1], &bu_counters[set], slot_dev(seq), seq);
+        in ^= 1;
+        ++lvl;
+      } else {
+        launch(dev, k_publish, 1u, 64u, 0, &bu_counters[set], slot_dev(seq), seq);
+      }
+      wait_published(seq);
       BfsCounters counters;
-      std::memcpy(&counters, &slot(pending)->counters, sizeof(counters));
+      std::memcpy(&counters, &slot(seq)->counters, sizeof(counters));
       finish_level(kBottomUp, counters);
       on_level();
       if (done || !ahead) break;
-      // the level after `next`: bottom-up while the rule says so, then the tail
+      // the level after the one just launched: bottom-up while the rule says so, then the tail
       if (static_cast<double>(n_f) < static_cast<double>(n) / kBeta) ++tail;
       ahead = tail < kTailBottomUp;
-      pending = next;
+      set ^= 1;
+      seq = ++dev.publish_seq;
     }
     top_down_dir = true;
   }
@@ -399,6 +436,8 @@ struct Traversal {
   // max_levels == 1, one level launched directly. Returns the levels built.
   uint32_t run_chain(uint32_t max_levels) {
     to_queue();
+    synced = false;  // the chained levels keep no bitmap
+    first_level = false;
     BfsChainState init{};
     init.size = static_cast<uint32_t>(n_f);
     init.level = level;
@@ -593,6 +632,8 @@ uint32_t bfs_level_timings(Graph& g, uint32_t source, uint32_t max_levels, float
       t.in_bitmap = saved.in_bitmap;
       t.cur = saved.cur;
       t.done = false;
+      t.synced = false;  // trials change the bitmaps; conversions rebuild them
+      t.first_level = false;
       dev.sync();
     };
     const uint32_t allowed = t.allowed_mask();

@@ -58,7 +58,7 @@ uint32_t exact_rows_per_thread(uint32_t d) { return d <= 3 ? 4 : (d <= 7 ? 2 : 1
 // With `shift`, the degree thresholds are scaled by 2^shift (for passes that
 // see a 2^-shift share of every row).
 PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t block_threads,
-                         bool exact, uint32_t shift = 0) {
+                         bool exact, uint32_t shift = 0, uint32_t rows_per_group = 1) {
   // Degree thresholds and the group size used below each.
   static const uint32_t kLimit[] = {2, 4, 8, 16, 32, 64, 1024, 0xFFFFFFFFu};
   const uint32_t kGroup[] = {1, 2, 4, 8, 16, 32, 64, block_threads};
@@ -103,7 +103,7 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
       seg.group = kGroup[t];
       seg.block_begin = block;
       seg.entry_begin = off[row];
-      const uint32_t rows_per_block = kGroup[t] >= kPrThreads ? 1 : kPrThreads / kGroup[t];
+      const uint32_t rows_per_block = kGroup[t] >= kPrThreads ? 1 : rows_per_group * (kPrThreads / kGroup[t]);
       block += (lo - row + rows_per_block - 1) / rows_per_block;
       s.seg[s.count++] = seg;
       row = lo;
@@ -115,15 +115,116 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
 
 }  // namespace
 
-// Which SpMV runs: the persistent kernel with the hot contributions in LDS
-// (default, pr_spmv_hot) or one workgroup per row tile (GC_PR_KERNEL=rows,
-// pr_spmv). Measured on MI355X, 17 iterations at R-MAT scale 22: 11.3 ms
-// against 14.0 ms; scale 20: 3.0 against 3.8 ms. (A source-partitioned pass,
-// one part of the sources per XCD with eight FP32 partial sums per row,
-// halved the L2 requests but was no faster: 12.1 against 10.2-10.5 ms.)
-static bool use_row_kernel() {
+// Which SpMV runs (GC_PR_KERNEL): on a symmetric graph the push-pull kernel
+// (default, pr_spmv_push); otherwise, or with GC_PR_KERNEL=hot, the persistent
+// pull kernel with the hot contributions in LDS (pr_spmv_hot); with
+// GC_PR_KERNEL=rows one workgroup per row tile (pr_spmv). Measured on MI355X,
+// 17 iterations at R-MAT scale 22: hot 11.3 ms against rows 14.0 ms; scale
+// 20: 3.0 against 3.8 ms. (A source-partitioned pass, one part of the sources
+// per XCD with eight FP32 partial sums per row, halved the L2 requests but was
+// no faster: 12.1 against 10.2-10.5 ms.)
+enum class PrKernel { kPush, kHot, kRows };
+
+static PrKernel choose_kernel(const Graph& g) {
   const char* env = std::getenv("GC_PR_KERNEL");
-  return env != nullptr && std::strcmp(env, "rows") == 0;
+  if (env != nullptr && std::strcmp(env, "rows") == 0) return PrKernel::kRows;
+  if (env != nullptr && std::strcmp(env, "hot") == 0) return PrKernel::kHot;
+  return g.symmetric ? PrKernel::kPush : PrKernel::kHot;
+}
+
+static uint32_t env_u32(const char* name, uint32_t fallback) {
+  const char* env = std::getenv(name);
+  return env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : fallback;
+}
+
+// Push-pull iterations on a symmetric graph (pr_spmv_push, pr_push_finish,
+// pr_finalize; the kernels' comments give the method). One 1024-lane
+// workgroup per CU holds push_count 8-byte accumulators and hot_count FP32
+// contributions in LDS: 8,192 and 22,528 (152 KB of the 160 KB). At R-MAT
+// scale 22 that leaves 36% of the entries to gather from global memory
+// (58% for the pull kernel's 19,456 hot ids) and skips 23% of the target
+// stream (the hubs' entries outside the push set). GC_PR_PUSH and GC_PR_HOT
+// override the two sizes.
+void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float tolerance, float* d_out,
+                   uint32_t* iterations, int32_t* converged) {
+  Device& dev = *g.dev;
+  const uint32_t n = g.n;
+  const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
+  // workgroups per CU (GC_PR_WGS, 1 or 2): each holds its own sets in its share of the LDS
+  const uint32_t wgs = std::min(2u, std::max(1u, env_u32("GC_PR_WGS", 1u)));
+  // lane groups take this many rows of a tile each, their loads in flight together (1 or 2,
+  // GC_PR_ROWS; pr_spmv_push's tiles have this many slices). Measured at R-MAT scale 22,
+  // one box: 357 us per iteration with 1 row, 382 with 2.
+  const uint32_t rows_per_group = env_u32("GC_PR_ROWS", 1u) >= 2 ? 2u : 1u;
+  const size_t lds_cap = std::min<size_t>(dev.max_lds_per_block, 160 * 1024) / wgs - 1024;  // static LDS
+  // at most 1/3 of the LDS for the accumulators, the rest for the contributions
+  const uint32_t push_cap = static_cast<uint32_t>(lds_cap / 3 / 8);
+  const uint32_t push_count = std::min({n, std::max(1u, env_u32("GC_PR_PUSH", 8192u)), push_cap});
+  const uint32_t hot_cap = static_cast<uint32_t>((lds_cap - 8ull * push_count) / 4);
+  const uint32_t hot_count = std::min({n, std::max(push_count, env_u32("GC_PR_HOT", 22528u)), hot_cap});
+  const uint32_t push_base = n - push_count;
+  const uint32_t shmem = static_cast<uint32_t>(8ull * push_count + 4ull * hot_count);
+  const PrSegments segs = plan_segments(g.rl.host_offsets, push_base, kPrHotThreads, false,
+                                        env_u32("GC_PR_SHIFT", 2u), rows_per_group);
+  const uint32_t push_tiles = (push_count + 15) / 16;  // 16 hub rows (one wave each) per tile
+  const uint32_t grid = cu * wgs;
+  const uint32_t finish_blocks = (push_count + 15) / 16;
+  const uint32_t partial_blocks = grid + finish_blocks;
+  float* rank[2] = {g.buffer<float>("pr.rank0", sizeof(float) * n),
+                    g.buffer<float>("pr.rank1", sizeof(float) * n)};
+  float* contrib[2] = {g.buffer<float>("pr.contrib0", sizeof(float) * n),
+                       g.buffer<float>("pr.contrib1", sizeof(float) * n)};
+  double* partials = g.buffer<double>("pr.partials", sizeof(double) * 2 * partial_blocks);
+  PrState* state = g.buffer<PrState>("pr.state", sizeof(PrState));
+  uint32_t* work = g.buffer<uint32_t>("pr.work", 16);
+  uint32_t* psplit = g.buffer<uint32_t>("pr.psplit", sizeof(uint32_t) * push_count);
+  auto* push_partials =
+      g.buffer<unsigned long long>("pr.push_partials", sizeof(unsigned long long) * grid * push_count);
+  fill_words(dev, work, 4, 0);
+
+  const float r0 = 1.0f / static_cast<float>(n);
+  const uint32_t* off = g.rl.offsets.as<const uint32_t>();
+  const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
+  Grid1D igrid = thread_per_item(n);
+  launch(dev, get_kernel(dev, "pagerank", "pr_init"), igrid.blocks, igrid.threads, 0, off, n, r0, rank[0],
+         contrib[0]);
+  launch(dev, get_kernel(dev, "pagerank", "pr_push_split"), (push_count + 255) / 256, 256u, 0, off, tgt, n,
+         push_count, psplit);
+  PrState init{static_cast<double>(g.rl.dangling) * static_cast<double>(r0), 0.0, 0, 0};
+  copy_to_device(dev, state, &init, sizeof(init));
+
+  Kernel spmv = get_kernel(dev, "pagerank", "pr_spmv_push");
+  Kernel finish = get_kernel(dev, "pagerank", "pr_push_finish");
+  Kernel finalize = get_kernel(dev, "pagerank", "pr_finalize");
+  const double d = static_cast<double>(damping);
+  const double tol = static_cast<double>(tolerance);
+  PrState host_state = init;
+  for (uint32_t it = 0; it < max_iterations; ++it) {
+    const int in = it & 1, out = in ^ 1;
+    hipError_t err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt,
+                                static_cast<const float*>(contrib[in]), static_cast<const float*>(rank[in]),
+                                contrib[out], rank[out], n, d, segs, static_cast<const PrState*>(state),
+                                hot_count, push_count, static_cast<const uint32_t*>(psplit), push_tiles,
+                                rows_per_group, push_partials, partials);
+    if (err != hipSuccess) fail_kernel(std::string("pr_spmv_push launch failed: ") + hipGetErrorString(err));
+    launch(dev, finish, finish_blocks, 256u, 0, off, static_cast<const float*>(rank[in]), contrib[out], rank[out],
+           n, d, static_cast<const PrState*>(state), push_count,
+           static_cast<const unsigned long long*>(push_partials), grid, partials, grid);
+    launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials), partial_blocks, tol, work);
+    if ((it + 1) % kPrBatch == 0 || it + 1 == max_iterations) {
+      copy_to_host(dev, dev.pinned, state, sizeof(PrState));
+      std::memcpy(&host_state, dev.pinned, sizeof(PrState));
+      if (host_state.done) break;
+    }
+  }
+  const int latest = host_state.iterations & 1;
+  Grid1D ogrid = thread_per_item((static_cast<uint64_t>(n) + 3) / 4);  // four entries per lane
+  launch(dev, get_kernel(dev, "util", "gather_u32"), ogrid.blocks, ogrid.threads, 0,
+         reinterpret_cast<const uint32_t*>(rank[latest]), g.rl.inv.as<const uint32_t>(),
+         reinterpret_cast<uint32_t*>(d_out), static_cast<uint64_t>(n));
+  dev.sync();
+  *iterations = host_state.iterations;
+  *converged = host_state.done ? 1 : 0;
 }
 
 void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance, float* d_out,
@@ -131,7 +232,12 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   Device& dev = *g.dev;
   prepare(g);
   const uint32_t n = g.n;
-  const bool rows_kernel = use_row_kernel();
+  const PrKernel kind = choose_kernel(g);
+  if (kind == PrKernel::kPush) {
+    pagerank_push(g, damping, max_iterations, tolerance, d_out, iterations, converged);
+    return;
+  }
+  const bool rows_kernel = kind == PrKernel::kRows;
   const uint32_t block_threads = rows_kernel ? kPrThreads : kPrHotThreads;
   // Row kernel: rows of degree >= kPrHubLimit are summed in chunks (hub plan).
   const std::vector<uint32_t>& hoff = g.rl.host_offsets;

@@ -40,12 +40,20 @@ struct BfsPublished {
   uint32_t pad[3];
 };
 
-// Also clears the level counters, which every level's publication clears
-// again for the next one.
+// Also clears the level counters (the top-down ones, which every level's
+// publication clears again for the next one, and the two of the bottom-up
+// runs), sets start[0] = 0 (the degree prefix of the one-vertex first
+// frontier, so the first level needs no scan), and sets up the frontier
+// bitmaps that top-down levels keep in step with their queues: bits0 = the
+// source alone, bits1 = empty (`words` 64-bit words each).
 // dist is written four entries (16 bytes) per store, over a grid-stride loop.
 extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__ dist, uint32_t n,
                                                            uint32_t source, uint32_t* __restrict__ queue,
-                                                           BfsCounters* __restrict__ counters) {
+                                                           BfsCounters* __restrict__ counters,
+                                                           BfsCounters* __restrict__ bu_counters,
+                                                           uint32_t* __restrict__ start,
+                                                           uint64_t* __restrict__ bits0,
+                                                           uint64_t* __restrict__ bits1, uint64_t words) {
   const uint64_t quads = n / 4;
   int4* d4 = reinterpret_cast<int4*>(dist);
   for (uint64_t q = global_thread(); q < quads; q += grid_threads()) {
@@ -54,25 +62,40 @@ extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__
                       v + 3 == source ? 0 : -1);
   }
   for (uint64_t v = 4 * quads + global_thread(); v < n; v += grid_threads()) dist[v] = v == source ? 0 : -1;
+  for (uint64_t w = global_thread(); w < words; w += grid_threads()) {
+    bits0[w] = w == source / 64 ? 1ull << (source % 64) : 0ull;
+    bits1[w] = 0ull;
+  }
   if (global_thread() == 0) {
     queue[0] = source;
+    start[0] = 0;
     counters->next_size = 0;
     counters->next_edges = 0;
+    for (int i = 0; i < 2; ++i) {
+      bu_counters[i].next_size = 0;
+      bu_counters[i].next_edges = 0;
+    }
   }
 }
 
-// Publishes the level's counters to the host (one lane): copies them to
-// `host`, clears them for the next level, then sets host->seq = seq after a
-// system-scope fence, so the counters are visible before the sequence word.
-extern "C" __global__ __launch_bounds__(64) void bfs_publish(BfsCounters* __restrict__ counters,
-                                                             BfsPublished* __restrict__ host, uint32_t seq) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Copies a finished level's counters to `host` (one lane), clears them for
+// the next level, then sets host->seq = seq after a system-scope fence, so
+// the counters are visible before the sequence word.
+__device__ __forceinline__ void publish_counters(BfsCounters* __restrict__ counters,
+                                                 BfsPublished* __restrict__ host, uint32_t seq) {
   const BfsCounters c = *counters;
   host->counters = c;
   counters->next_size = 0;
   counters->next_edges = 0;
   __threadfence_system();
   __hip_atomic_store(&host->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Publishes the level's counters to the host (one lane).
+extern "C" __global__ __launch_bounds__(64) void bfs_publish(BfsCounters* __restrict__ counters,
+                                                             BfsPublished* __restrict__ host, uint32_t seq) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  publish_counters(counters, host, seq);
 }
 
 // top[k * n + v] = the k-th last entry of row v (k < 4); past the row's start,
@@ -126,7 +149,12 @@ extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ top, int32_t* __restrict__ dist, const uint32_t* __restrict__ front,
     uint64_t* __restrict__ next, uint32_t n, uint32_t first_chunk, BfsCounters* __restrict__ counters,
-    int32_t next_level) {
+    int32_t next_level, BfsCounters* __restrict__ pub_counters, BfsPublished* __restrict__ pub_host,
+    uint32_t pub_seq) {
+  // A level launched ahead publishes the counters of the level before it
+  // (finished, since launches on the stream run in order): no publishing
+  // launch of its own between the two levels.
+  if (pub_host != nullptr && blockIdx.x == 0 && threadIdx.x == 0) publish_counters(pub_counters, pub_host, pub_seq);
   const uint32_t lane = lane_id();
   uint32_t found_count = 0;  // this lane's discoveries and their degrees
   unsigned long long found_edges = 0;
@@ -265,7 +293,15 @@ extern "C" __global__ __launch_bounds__(256) void bfs_topdown_edges(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     int32_t* __restrict__ dist, const uint32_t* __restrict__ queue, uint32_t queue_size,
     const uint32_t* __restrict__ start, uint32_t total, uint32_t* __restrict__ next_queue,
-    BfsCounters* __restrict__ counters, int32_t next_level) {
+    BfsCounters* __restrict__ counters, int32_t next_level, uint32_t* __restrict__ cur_bits,
+    uint32_t* __restrict__ next_bits) {
+  // With cur_bits / next_bits (the frontier's bitmap, which holds exactly the
+  // queue's vertices, and an empty one), the level also clears its frontier's
+  // bits and sets the bits of the vertices it discovers, so that afterwards
+  // next_bits holds the next frontier and cur_bits is empty: a switch to
+  // bottom-up needs no conversion. Every frontier vertex has an edge (it was
+  // reached over one, and the bitmaps are kept only for symmetric graphs),
+  // so the lane of its first edge clears its bit.
   // The workgroup's discoveries of one block of 4 x TD_EDGES_PER_WAVE edges are
   // gathered in LDS and appended with one atomic (one per wave and 64 edges,
   // all on one counter, serialized: a 370K-edge level took 368 us).
@@ -304,9 +340,13 @@ extern "C" __global__ __launch_bounds__(256) void bfs_topdown_edges(
           }
           const uint32_t u = queue[a];
           v = targets[offsets[u] + (t - start[a])];
+          if (cur_bits != nullptr && t == start[a]) atomicAnd(&cur_bits[u >> 5], ~(1u << (u & 31)));
           if (dist[v] < 0) {
             discovered = atomicCAS(&dist[v], -1, next_level) == -1;
-            if (discovered) found_edges += offsets[v + 1] - offsets[v];
+            if (discovered) {
+              found_edges += offsets[v + 1] - offsets[v];
+              if (next_bits != nullptr) atomicOr(&next_bits[v >> 5], 1u << (v & 31));
+            }
           }
         }
         const unsigned long long mask = __ballot(discovered);

@@ -485,3 +485,313 @@ extern "C" __global__ __launch_bounds__(PR_FIN_THREADS) void pr_finalize(
     work[1] = 0;
   }
 }
+
+// ---- push-pull variant for symmetric graphs (the default there) ------------
+//
+// On a symmetric CSR, y[h] = sum_{u in N(h)} c[u] is also the sum of c[u] over
+// the rows u that hold an entry h. For the push set P (the top push_count ids,
+// the hubs) the pass therefore pushes instead of pulling: every row u adds
+// c[u] into an LDS accumulator of each of its entries h in P, and the hubs'
+// own lists are read only for their entries inside P (which push c[h] into
+// the other hubs). The long in-lists of the hubs, mostly low-degree ids that
+// miss every cache, are never gathered, and a row outside P still pulls its
+// entries: from the LDS copy of the top hot_count contributions (H, which
+// contains P) or from global memory. Accumulators are unsigned 64-bit fixed
+// point with 2^-62 resolution (every c <= 1 and every y <= sum of c <= 1, so
+// nothing overflows), which makes the per-workgroup sums and their reduction
+// exact and independent of the order of the atomics: the result is
+// deterministic. Each workgroup writes its accumulators once; pr_push_finish
+// adds them and finishes the hub rows.
+
+#define PRP_THREADS 1024
+#define PRP_FIX_SCALE 4611686018427387904.0  // 2^62
+
+__device__ __forceinline__ unsigned long long pr_to_fixed(float c) {
+  return static_cast<unsigned long long>(__double2ull_rn(static_cast<double>(c) * PRP_FIX_SCALE));
+}
+
+// Pull sums of R rows at once: lane entries first[r], first[r] + STRIDE, ...
+// < end[r] of each row r, eight per row per step and all R rows' steps in
+// flight together (R x 8 target loads, then R x 8 gathers), pushing cfix[r]
+// (row r's own contribution) into s_push for every entry in P. A step past a
+// row's last entry is masked (its loads read the row's last entry again).
+template <int STRIDE, int R>
+__device__ __forceinline__ void prp_rows_sum(const uint32_t* __restrict__ targets,
+                                             const float* __restrict__ contrib,
+                                             const float* __restrict__ s_hot, uint32_t hot_base,
+                                             unsigned long long* __restrict__ s_push,
+                                             uint32_t push_base, const unsigned long long (&cfix)[R],
+                                             const uint32_t (&first)[R], const uint32_t (&end)[R],
+                                             double (&acc)[R]) {
+  uint32_t steps[R];
+  uint32_t most = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    steps[r] = first[r] < end[r] ? (end[r] - first[r] + 8 * STRIDE - 1) / (8 * STRIDE) : 0u;
+    most = max(most, steps[r]);
+  }
+  for (uint32_t s = 0; s < most; ++s) {
+    uint32_t t[R][8];
+    bool ok[R][8];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t idx = first[r] + (s * 8 + k) * STRIDE;
+        ok[r][k] = s < steps[r] && idx < end[r];
+        t[r][k] = s < steps[r] ? load_stream(targets + min(idx, end[r] - 1)) : hot_base;
+      }
+    }
+    float x[R][8];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[r][k] = t[r][k] >= hot_base ? s_hot[t[r][k] - hot_base] : contrib[t[r][k]];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (ok[r][k] && t[r][k] >= push_base) atomicAdd(s_push + (t[r][k] - push_base), cfix[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[r] += ok[r][k] ? static_cast<double>(x[r][k]) : 0.0;
+    }
+  }
+}
+
+// As pr_finish_row, with rank_in[row] already loaded (rin).
+__device__ __forceinline__ void pr_finish_row_rin(uint32_t row, uint32_t deg, double acc, double base,
+                                                  double damping, float rin, float* __restrict__ rank_out,
+                                                  float* __restrict__ contrib_out, double& dangling,
+                                                  double& diff) {
+  const float r = static_cast<float>(base + damping * acc);
+  rank_out[row] = r;
+  contrib_out[row] = deg ? r / static_cast<float>(deg) : 0.0f;
+  if (deg == 0) dangling += static_cast<double>(r);
+  diff += fabs(static_cast<double>(r) - static_cast<double>(rin));
+}
+
+// One tile of a segment with G lanes per row: R slices of PRP_THREADS / G
+// consecutive rows, lane group i of the workgroup taking row i of every
+// slice. All rows' offsets, contributions and input ranks are loaded first.
+template <int G, int R>
+__device__ __forceinline__ void prp_tile_group(const PrSegment& seg, uint32_t local,
+                                               const uint32_t* __restrict__ offsets,
+                                               const uint32_t* __restrict__ targets,
+                                               const float* __restrict__ contrib_in,
+                                               const float* __restrict__ s_hot, uint32_t hot_base,
+                                               unsigned long long* __restrict__ s_push,
+                                               uint32_t push_base, const float* __restrict__ rank_in,
+                                               float* __restrict__ contrib_out,
+                                               float* __restrict__ rank_out, double base,
+                                               double damping, double& dangling, double& diff) {
+  constexpr uint32_t kRowsPerSlice = PRP_THREADS / G;
+  const uint32_t g = threadIdx.x % G;
+  uint32_t row[R], b[R], e[R], first[R];
+  unsigned long long cfix[R];
+  float rin[R];
+  double acc[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    row[j] = seg.row_begin + (local * R + j) * kRowsPerSlice + threadIdx.x / G;
+    b[j] = e[j] = 0;
+    cfix[j] = 0ull;
+    rin[j] = 0.0f;
+    acc[j] = 0.0;
+    if (row[j] < seg.row_end) {
+      b[j] = offsets[row[j]];
+      e[j] = offsets[row[j] + 1];
+      cfix[j] = pr_to_fixed(contrib_in[row[j]]);
+      if (g == 0) rin[j] = rank_in[row[j]];
+    }
+    first[j] = b[j] + g;
+  }
+  prp_rows_sum<G, R>(targets, contrib_in, s_hot, hot_base, s_push, push_base, cfix, first, e, acc);
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o);
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    if (row[j] < seg.row_end && g == 0) {
+      pr_finish_row_rin(row[j], e[j] - b[j], acc[j], base, damping, rin[j], rank_out, contrib_out, dangling,
+                        diff);
+    }
+  }
+}
+
+// Tiles: first `push_tiles` tiles of 16 hub rows each (one wave per row, the
+// top rows first), each pushing c[h] into the hubs of its list inside P (from
+// psplit[h - push_base], the first such entry); then the segments' tiles of
+// the rows below P, heaviest first. A workgroup takes tiles round-robin.
+extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_push(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    PrSegments tiles, const PrState* __restrict__ state, uint32_t hot_count, uint32_t push_count,
+    const uint32_t* __restrict__ psplit, uint32_t push_tiles, uint32_t rows_per_group,
+    unsigned long long* __restrict__ push_partials, double* __restrict__ partials) {
+  extern __shared__ unsigned long long s_dyn[];
+  unsigned long long* s_push = s_dyn;                          // push_count accumulators
+  float* s_hot = reinterpret_cast<float*>(s_dyn + push_count);  // hot_count contributions
+  __shared__ double s_red[PRP_THREADS / GC_WAVE];
+  __shared__ double s_part[2][PRP_THREADS / GC_WAVE];
+  if (state->done) return;
+  const uint32_t hot_base = n - hot_count;
+  const uint32_t push_base = n - push_count;
+  for (uint32_t i = threadIdx.x; i < hot_count; i += PRP_THREADS) s_hot[i] = contrib_in[hot_base + i];
+  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) s_push[i] = 0ull;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  __syncthreads();
+  const uint32_t total = push_tiles + tiles.total_blocks;
+  for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
+    if (k < push_tiles) {
+      // hub row h (one wave): push c[h] into its list's hubs
+      const uint32_t r = k * (PRP_THREADS / GC_WAVE) + threadIdx.x / GC_WAVE;
+      if (r < push_count) {
+        const uint32_t h = n - 1 - r;
+        const unsigned long long cfix = pr_to_fixed(s_hot[h - hot_base]);
+        const uint32_t e = offsets[h + 1];
+        uint32_t i = psplit[h - push_base] + lane_id();
+        for (; i + 7 * GC_WAVE < e; i += 8 * GC_WAVE) {
+          uint32_t t[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) t[j] = load_stream(targets + i + j * GC_WAVE);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) atomicAdd(s_push + (t[j] - push_base), cfix);
+        }
+        for (; i < e; i += GC_WAVE) atomicAdd(s_push + (load_stream(targets + i) - push_base), cfix);
+      }
+      continue;
+    }
+    const uint32_t t = tiles.total_blocks - 1 - (k - push_tiles);
+    uint32_t s = 0;
+    while (s + 1 < tiles.count && t >= tiles.seg[s + 1].block_begin) ++s;
+    const PrSegment seg = tiles.seg[s];
+    const uint32_t local = t - seg.block_begin;
+#define PRP_ARGS seg, local, offsets, targets, contrib_in, s_hot, hot_base, s_push, push_base, rank_in, contrib_out, rank_out, base, damping, dangling, diff
+#define PRP_GROUP(G)                                                        \
+  do {                                                                      \
+    /* R = 4 needs more than 128 VGPRs */                                  \
+    if (rows_per_group >= 2) prp_tile_group<G, 2>(PRP_ARGS);                \
+    else prp_tile_group<G, 1>(PRP_ARGS);                                    \
+  } while (0)
+    switch (seg.group) {
+      case 1: PRP_GROUP(1); break;
+      case 2: PRP_GROUP(2); break;
+      case 4: PRP_GROUP(4); break;
+      case 8: PRP_GROUP(8); break;
+      case 16: PRP_GROUP(16); break;
+      case 32: PRP_GROUP(32); break;
+      case 64: PRP_GROUP(64); break;
+      default: {
+        // one row for the whole workgroup
+        const uint32_t row = seg.row_begin + local;
+        const uint32_t b = offsets[row], e = offsets[row + 1];
+        const unsigned long long cfix[1] = {pr_to_fixed(contrib_in[row])};
+        const uint32_t first[1] = {b + threadIdx.x}, end[1] = {e};
+        double accs[1] = {0.0};
+        prp_rows_sum<PRP_THREADS, 1>(targets, contrib_in, s_hot, hot_base, s_push, push_base, cfix, first, end, accs);
+        double acc = wave_sum(accs[0]);
+        if (lane_id() == 0) s_red[threadIdx.x / GC_WAVE] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double total_acc = 0.0;
+          for (int w = 0; w < PRP_THREADS / GC_WAVE; ++w) total_acc += s_red[w];
+          pr_finish_row(row, e - b, total_acc, base, damping, rank_in, rank_out, contrib_out, dangling, diff);
+        }
+        __syncthreads();
+      }
+    }
+#undef PRP_GROUP
+#undef PRP_ARGS
+  }
+  dangling = wave_sum(dangling);
+  diff = wave_sum(diff);
+  const uint32_t wave = threadIdx.x / GC_WAVE;
+  if (lane_id() == 0) {
+    s_part[0][wave] = dangling;
+    s_part[1][wave] = diff;
+  }
+  __syncthreads();  // also: every push of this workgroup has landed
+  unsigned long long* out = push_partials + static_cast<uint64_t>(blockIdx.x) * push_count;
+  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) out[i] = s_push[i];
+  if (threadIdx.x == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    for (int w = 0; w < PRP_THREADS / GC_WAVE; ++w) {
+      d0 += s_part[0][w];
+      d1 += s_part[1][w];
+    }
+    partials[2 * static_cast<uint64_t>(blockIdx.x)] = d0;
+    partials[2 * static_cast<uint64_t>(blockIdx.x) + 1] = d1;
+  }
+}
+
+// psplit[i] = first entry of hub row n - push_count + i whose target is in P
+// (rows are sorted): one lane per hub row, binary search.
+extern "C" __global__ __launch_bounds__(256) void pr_push_split(const uint32_t* __restrict__ offsets,
+                                                                const uint32_t* __restrict__ targets,
+                                                                uint32_t n, uint32_t push_count,
+                                                                uint32_t* __restrict__ psplit) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= push_count) return;
+  const uint32_t push_base = n - push_count;
+  const uint32_t h = push_base + i;
+  psplit[i] = lower_bound_u32(targets, offsets[h], offsets[h + 1], push_base);
+}
+
+// Finishes the hub rows: y[h] = sum over the `parts` workgroups' accumulators
+// (exact: integer sums), then rank, contribution, dangling and L1 terms as
+// pr_finish_row. 16 hubs per 256-lane workgroup, 16 lanes per hub each adding
+// every 16th workgroup's value. Per-workgroup FP64 partials of the dangling
+// and L1 sums go to partials from `partial_base` on.
+#define PRF_HUBS 16
+extern "C" __global__ __launch_bounds__(256) void pr_push_finish(
+    const uint32_t* __restrict__ offsets, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t push_count,
+    const unsigned long long* __restrict__ push_partials, uint32_t parts, double* __restrict__ partials,
+    uint32_t partial_base) {
+  __shared__ unsigned long long s_sum[256];
+  __shared__ double s_part[2][PRF_HUBS];
+  if (state->done) return;
+  const uint32_t hub = threadIdx.x % PRF_HUBS, j = threadIdx.x / PRF_HUBS;
+  const uint32_t i = blockIdx.x * PRF_HUBS + hub;
+  unsigned long long sum = 0ull;
+  if (i < push_count) {
+    for (uint32_t w = j; w < parts; w += 256 / PRF_HUBS) sum += push_partials[static_cast<uint64_t>(w) * push_count + i];
+  }
+  s_sum[threadIdx.x] = sum;
+  __syncthreads();
+  if (j == 0) {
+    double dangling = 0.0, diff = 0.0;
+    if (i < push_count) {
+      unsigned long long y = 0ull;
+      for (uint32_t q = 0; q < 256 / PRF_HUBS; ++q) y += s_sum[q * PRF_HUBS + hub];
+      const double inv_n = 1.0 / static_cast<double>(n);
+      const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+      const uint32_t row = n - push_count + i;
+      pr_finish_row(row, offsets[row + 1] - offsets[row], static_cast<double>(y) / PRP_FIX_SCALE, base,
+                    damping, rank_in, rank_out, contrib_out, dangling, diff);
+    }
+    s_part[0][hub] = dangling;
+    s_part[1][hub] = diff;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    for (int h = 0; h < PRF_HUBS; ++h) {
+      d0 += s_part[0][h];
+      d1 += s_part[1][h];
+    }
+    const uint64_t slot = partial_base + blockIdx.x;
+    partials[2 * slot] = d0;
+    partials[2 * slot + 1] = d1;
+  }
+}

@@ -17,7 +17,7 @@ import pytest
 import grafeo_cuda
 import oracle
 from grafeo_cuda import _native
-from graphs import catalog, er_graph, random_graph, rmat_graph, undirected
+from graphs import catalog, er_graph, grid_graph, random_graph, rmat_graph, undirected
 
 pytestmark = pytest.mark.gpu
 
@@ -60,17 +60,21 @@ def check_tc_partitions(name, g, expected, monkeypatch):
 
 
 def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
-    """Both SpMV kernels (hot set in LDS, the default, and one workgroup per row tile), and a
-    hot set smaller than the graph, against the oracle."""
-    for kernel, hot in (("rows", None), ("hot", "1024"), ("hot", None)):
-        monkeypatch.setenv("GC_PR_KERNEL", kernel)
-        if hot is None:
-            monkeypatch.delenv("GC_PR_HOT", raising=False)
-        else:
-            monkeypatch.setenv("GC_PR_HOT", hot)
-        check_pagerank(f"{name} ({kernel}, hot={hot})", off, tgt, g)
-    monkeypatch.delenv("GC_PR_KERNEL")
-    monkeypatch.delenv("GC_PR_HOT", raising=False)
+    """Every SpMV kernel against the oracle: the default (push-pull on a symmetric graph, with
+    its push and hot sets at their defaults, at a few dozen ids, and at one id), the pull
+    kernel with its hot set in LDS (default size and a set smaller than the graph), and one
+    workgroup per row tile."""
+    cases = ((None, None, None), (None, "64", "96"), (None, "1", "1"), (None, "37", None),
+             ("rows", None, None), ("hot", None, "1024"), ("hot", None, None))
+    for kernel, push, hot in cases:
+        for var, value in (("GC_PR_KERNEL", kernel), ("GC_PR_PUSH", push), ("GC_PR_HOT", hot)):
+            if value is None:
+                monkeypatch.delenv(var, raising=False)
+            else:
+                monkeypatch.setenv(var, value)
+        check_pagerank(f"{name} (kernel={kernel}, push={push}, hot={hot})", off, tgt, g)
+    for var in ("GC_PR_KERNEL", "GC_PR_PUSH", "GC_PR_HOT"):
+        monkeypatch.delenv(var, raising=False)
 
 
 def check_bfs(name, off, tgt, g, source):
@@ -213,3 +217,31 @@ def test_rmat_small(device, monkeypatch, scale):
                   fast_oracle=scale > 10)
     check_truss_numbers(f"RMAT-{scale}", off, tgt, g)
     check_msbfs(f"RMAT-{scale}", off, tgt, g, [int(x) for x in np.random.default_rng(scale).integers(0, n, size=128)])
+
+
+def test_bfs_schedule_switches(device, monkeypatch):
+    """Distances under other fixed-rule settings against the oracle: bottom-up levels launched
+    ahead or one per read-back (GC_BFS_AHEAD), and Beamer's top-down -> bottom-up threshold at
+    1 (bottom-up as soon as allowed), 14 (the default) and 1000 (almost never), on the catalog,
+    a grid (many levels) and R-MAT 14 (hubs). The schedule changes which kernels run, never
+    the distances."""
+    graphs = [(name, off, tgt) for name, off, tgt in catalog()]
+    graphs.append(("grid 40x30",) + tuple(grid_graph(40, 30)))
+    graphs.append(("RMAT-14",) + tuple(rmat_graph(14, 16, seed=3)))
+    for name, off, tgt in graphs:
+        g = upload(device, off, tgt)
+        n = len(off) - 1
+        sources = sorted({0, n // 2, n - 1}) if n else []
+        expected = {s: oracle.bfs(off, tgt, s) for s in sources}
+        for ahead in ("0", "1"):
+            for alpha in ("1", "14", "1000"):
+                monkeypatch.setenv("GC_BFS_AHEAD", ahead)
+                monkeypatch.setenv("GC_BFS_ALPHA", alpha)
+                for s in sources:
+                    dist, reach, depth = expected[s]
+                    r = g.bfs_numpy(s)
+                    tag = f"{name} from {s} (ahead={ahead}, alpha={alpha})"
+                    assert np.array_equal(r["distances"], dist), tag
+                    assert (r["reachable"], r["max_depth"]) == (reach, depth), tag
+    monkeypatch.delenv("GC_BFS_AHEAD")
+    monkeypatch.delenv("GC_BFS_ALPHA")
