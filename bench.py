@@ -11,16 +11,28 @@ The graph is uploaded and prepared (degree-ordered layout) before the timed regi
 Reported (one JSON line): `value` = whole-step GTEPS with the reference's TEPS conventions
 (crates/grafeo-cuda-algos/src/bin/gpu_triangles.rs:127-132, gpu_bfs.rs:78-83,
 gpu_pagerank.rs:80-85): (m [TC] + m per BFS source + m x iterations [PageRank]) / step time,
-where m = CSR entries. Per-algorithm GTEPS, kernel event times, the roofline of the dominant
-kernel (HBM-bound: algorithmic bytes per launch / average launch time, against 8 TB/s) and a CPU
-baseline (the C oracle on the host cores, on a bounded sample) go into the same line.
+where m = CSR entries. Beside it, outside the timed steps:
+  - per-algorithm GTEPS, kernel event times and the roofline of every reported kernel (HBM-bound:
+    the stated model's bytes per launch / average launch time, against 8 TB/s), the dominant
+    kernel's as `roofline`;
+  - the reference's own convention (host outputs, best of 5 calls after 1 warmup, BFS TEPS as
+    the harmonic mean over sources) and a Graph500-style BFS GTEPS;
+  - the batched schedules against one call at a time (128-source multi-source BFS, the
+    256-graph triangle-count batch), by call time and by kernel time;
+  - the learned schedules against the fixed rules and every forced choice on R-MAT 22, R-MAT 24
+    and the 4096 x 4096 grid (>= 5 runs each, 95% confidence intervals);
+  - the CPU baseline (the C oracle on the host's threads and on one thread, on a bounded sample)
+    and the PageRank precision study at R-MAT 22 (50 iterations, tol 1e-7, against the FP64
+    oracle).
 """
 
 from __future__ import annotations
 
 import argparse
 import ctypes
+import hashlib
 import json
+import math
 import os
 import platform
 import sys
@@ -35,13 +47,29 @@ sys.path.insert(0, os.path.join(REPO, "grafeo-cuda_amd"))
 HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
-KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_hot", "pr_spmv", "pr_hub_finish",
-           "pr_finalize", "pr_init", "bfs_topdown_edges", "bfs_frontier_degrees", "bfs_bottomup",
-           "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_init", "bfs_frontier_scan_small", "scan_block_sums",
-           "exclusive_scan_small", "scan_apply", "gather_u32", "fill_u32")
+KERNEL_FAMILIES = {
+    "tc": ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count"),
+    "pagerank": ("pr_spmv_push", "pr_push_finish", "pr_spmv_hot", "pr_spmv", "pr_hub_finish", "pr_finalize",
+                 "pr_init", "pr_push_split"),
+    "bfs": ("bfs_init", "bfs_topdown_edges", "bfs_frontier_degrees", "bfs_frontier_scan_small", "bfs_bottomup",
+            "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_publish", "bfs_chain_expand", "bfs_chain_advance",
+            "bfs_top_build", "bfs_chain"),
+    "msbfs": ("ms_seed", "ms_seed_distances", "ms_topdown", "ms_bottomup_thread", "ms_bottomup_wave",
+              "ms_finalize", "ms_write_distances", "ms_summarize"),
+    "ktruss": ("kt_wedge_support", "kt_truss_frontier", "kt_truss_peel", "kt_truss_commit", "kt_truss_decrement",
+               "kt_truss_values", "kt_filter", "kt_support_direct", "kt_reverse_index", "kt_canonical_index",
+               "kt_canonical_reverse", "kt_emit_flags_direct", "kt_emit_flags_state"),
+    "batch": ("bt_count",),
+    "util": ("scan_block_sums", "exclusive_scan_small", "scan_apply", "gather_u32", "fill_u32", "compact_count",
+             "compact_emit_pairs", "csr_forward_start"),
+}
+KERNELS = tuple(k for ks in KERNEL_FAMILIES.values() for k in ks)
 # The kernels timed with events in a default run: those the rooflines report. Each timed launch
 # adds two event records to the stream (about 2 ms per step if every launch were timed).
-ROOFLINE_KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_hot", "pr_spmv")
+ROOFLINE_KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_push", "pr_spmv_hot",
+                    "pr_spmv")
+# two-sided 95% quantiles of Student's t by degrees of freedom
+T975 = {1: 12.706, 2: 4.303, 3: 3.182, 4: 2.776, 5: 2.571, 6: 2.447, 7: 2.365, 8: 2.306, 9: 2.262, 10: 2.228}
 
 
 def log(msg: str) -> None:
@@ -56,6 +84,86 @@ def pick_sources(offsets: np.ndarray, count: int, seed: int) -> list[int]:
     return [int(v) for v in rng.choice(candidates, size=min(count, candidates.size), replace=False)]
 
 
+def mean_ci(xs) -> dict:
+    """Mean, sample standard deviation and the half-width of the 95% confidence interval."""
+    a = np.asarray(xs, dtype=np.float64)
+    n = a.size
+    m = float(a.mean())
+    sd = float(a.std(ddof=1)) if n > 1 else 0.0
+    half = T975.get(n - 1, 1.96) * sd / math.sqrt(n) if n > 1 else 0.0
+    return {"mean": m, "sd": sd, "ci95": half, "runs": int(n), "min": float(a.min())}
+
+
+def ratio_ci(num, den, seed: int = 0, draws: int = 2000) -> dict:
+    """mean(num) / mean(den) with a 95% bootstrap interval (runs resampled with replacement)."""
+    a, b = np.asarray(num, np.float64), np.asarray(den, np.float64)
+    rng = np.random.default_rng(seed)
+    ra = a[rng.integers(0, a.size, (draws, a.size))].mean(axis=1)
+    rb = b[rng.integers(0, b.size, (draws, b.size))].mean(axis=1)
+    r = ra / rb
+    return {"ratio": float(a.mean() / b.mean()), "lo": float(np.percentile(r, 2.5)), "hi": float(np.percentile(r, 97.5))}
+
+
+def roofline_entry(bytes_per_launch: float, avg_us: float, traffic=None, note: str = "") -> dict:
+    a = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    out = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+           "traffic": traffic, "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_us,
+           "traffic_frac": (traffic / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if traffic else None}
+    if note:
+        out["note"] = note
+    return out
+
+
+def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bits: int = 32768) -> dict:
+    """Bytes tc_bitmap16 reads per launch, by the model that fits the kernel (on the degree-ordered
+    DAG the kernel runs on, `rl` = GpuGraph.relabelled_numpy()): offsets and the DAG split read
+    once, 4(n+1) + 4n; L(u) at 4 bytes per entry once per part it is split into (rows of at
+    least `split_min` DAG entries run in `parts` parts, shorter rows once); and for every DAG
+    edge u -> v the list L(v) at its stored width: its entries among the top `hot_bits` ids as
+    16-bit offsets in whole 16-byte octs of eight, its other entries at 4 bytes."""
+    off = rl["offsets"].astype(np.int64)
+    fwd = rl["fwd"].astype(np.int64)
+    tgt = rl["targets"]
+    n = off.size - 1
+    hot_base = ((n - hot_bits) + 31) & ~31 if n > hot_bits else 0
+    dplus = off[1:] - fwd
+    # DAG entries: targets[fwd[u] .. off[u+1]) of every row u
+    row = np.repeat(np.arange(n, dtype=np.int32), dplus)
+    starts = np.repeat(fwd - np.concatenate(([0], np.cumsum(dplus)[:-1])), dplus)
+    dag_v = tgt[np.arange(row.size, dtype=np.int64) + starts].astype(np.int64)
+    hot = np.bincount(row, weights=(dag_v >= hot_base), minlength=n).astype(np.int64)
+    cold = dplus - hot
+    octs = (hot + 7) // 8
+    indeg = np.bincount(dag_v, minlength=n).astype(np.int64)
+    streamed = int((indeg * (16 * octs + 4 * cold)).sum())
+    row_reads = int((4 * dplus * np.where(dplus >= split_min, parts, 1)).sum())
+    total = 4 * (n + 1) + 4 * n + row_reads + streamed
+    return {"bytes": total, "list_bytes": streamed, "list_bytes_32bit": int((indeg * 4 * dplus).sum()),
+            "row_bytes": row_reads, "hot_base": int(hot_base),
+            "hot_fraction_of_streamed_entries": float((indeg * hot).sum() / max(1, (indeg * dplus).sum()))}
+
+
+def dag_merge_bytes(off: np.ndarray, tgt: np.ndarray) -> int:
+    """The merge model of one symmetric graph's triangle count: 4(n+1) + 4 m_o + 4 sum over DAG
+    edges u->v of (d+(u) + d+(v)), on the degree-ordered DAG (u -> v when v ranks above u by
+    (degree, id))."""
+    off = off.astype(np.int64)
+    n = off.size - 1
+    deg = np.diff(off)
+    order = np.argsort(deg, kind="stable")
+    rank = np.empty(n, np.int64)
+    rank[order] = np.arange(n)
+    row = np.repeat(np.arange(n), deg)
+    up = rank[tgt.astype(np.int64)] > rank[row]
+    dplus = np.bincount(row[up], minlength=n)
+    u, v = row[up], tgt[up].astype(np.int64)
+    return int(4 * (n + 1) + 4 * up.sum() + 4 * (dplus[u] + dplus[v]).sum())
+
+
+def sum_kernels(dev, family: str) -> float:
+    return sum(dev.kernel_time(k)[0] for k in KERNEL_FAMILIES[family])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +174,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--pr-iterations", type=int, default=20)
-    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on",
+                    help="the C oracle on the host (bounded sample) and the PageRank precision study")
     ap.add_argument("--kernel-events", choices=["on", "all", "off"], default="on",
                     help="time the roofline kernels (on) or every kernel launch (all) of the timed "
                          "steps with HIP events")
@@ -74,24 +183,23 @@ def main() -> int:
                     help="also time the batched schedules against the one-at-a-time ones: 128-source "
                          "multi-source BFS vs 128 single-source BFS, and the 256-graph triangle-count "
                          "batch (default and autotuned shape) vs one call per graph")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host threads (max 16)")
+    ap.add_argument("--scheduler", choices=["on", "off", "quick"], default="on",
+                    help="the learned schedules against the fixed rules and the forced choices on R-MAT "
+                         "22 (this graph), R-MAT 24 and the 4096 x 4096 grid (quick: this graph only)")
+    ap.add_argument("--reference-convention", choices=["on", "off"], default="on",
+                    help="the reference's timing convention: host outputs, best of 5 after 1 warmup")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: the host threads this process may use (OMP_NUM_THREADS, else all)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 5 times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     ap.add_argument("--workload", choices=["step", "scheduler"], default="step",
-                    help="step: the R-MAT scale-22 step above; scheduler: the learned schedules "
-                         "against the fixed rules on graphs they were not trained on (R-MAT scale 24, "
-                         "4096 x 4096 grid)")
+                    help="step: the R-MAT step above; scheduler: only the learned-versus-fixed "
+                         "comparison on R-MAT 24 and the 4096 x 4096 grid")
     ap.add_argument("--model", default=None, help="learned schedules (default: the committed model)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
-    if args.workload == "scheduler":
-        if args.gpus != 1:
-            print(json.dumps({"error": "this build runs on one GPU", "n_gpus": args.gpus}), flush=True)
-            return 2
-        return scheduler_workload(args)
-
     if args.gpus != 1:
         print(
             json.dumps({"error": "this library runs on one GPU (the sharding across GPUs is a design, "
@@ -103,6 +211,14 @@ def main() -> int:
     import grafeo_cuda
     from grafeo_cuda import _native
     from grafeo_cuda._native import check, lib
+
+    if args.workload == "scheduler":
+        dev = grafeo_cuda.GpuDevice(0)
+        out = {"metric": "learned vs fixed schedule (BFS per level, TC per degree bucket)", "unit": "ms",
+               "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+               "scheduler": scheduler_section(dev, None, None, args, with_large=True)}
+        print(json.dumps(out), flush=True)
+        return 0
 
     t0 = time.time()
     pairs = _native.gen_rmat(args.scale, args.edge_factor, args.seed)
@@ -176,76 +292,62 @@ def main() -> int:
         "pagerank_ms": 1000 * mean["pagerank"],
     }
 
-    # Algorithmic bytes per launch (HBM roofline for every kernel here).
+    # ---- rooflines (HBM bandwidth for every kernel here; MFMA is not applicable) ----
     # PageRank SpMV per iteration: offsets + targets + x read once + y written once.
     spmv_bytes = 4 * (n + 1) + 4 * m + 4 * n + 4 * n
-    # Triangle count by list intersection on the DAG (tc_hash, tc_count): offsets, the DAG's
-    # entries, and for each DAG edge u->v both out-lists, 4(n+1) + 4 m_o + 4 sum(d+u + d+v).
-    tc_bytes = 4 * (n + 1) + 4 * m_o.value + 4 * inter.value
-    # Middle-vertex triangle count (tc_wedge): offsets and DAG split (fwd) read once, every
-    # CSR entry read once (in-lists and out-lists), plus one 4-byte entry per DAG wedge.
-    wedge_bytes = 4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value
     # BFS per traversal: offsets + targets read once, distances written once.
     bfs_bytes = 4 * (n + 1) + 4 * m + 4 * n
-    rooflines = {}
-    # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
-    # (scaled by the readings of a copy of known size), when they have been recorded.
+    # The merge model of triangle counting (tc_hash, tc_count): offsets, the DAG's entries, and
+    # for each DAG edge u->v both out-lists at 4 bytes an entry.
+    tc_merge_bytes = 4 * (n + 1) + 4 * m_o.value + 4 * inter.value
+    wedge_bytes = 4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value
+    t0 = time.time()
+    tc16 = tc_stored_width_bytes(g.relabelled_numpy())
+    log(f"tc_bitmap16 model: {tc16['bytes'] / 1e9:.2f} GB per launch ({time.time() - t0:.1f}s)")
     traffic = load_traffic(f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}")
-
-    def add_roofline(name, bytes_per_launch, avg_us):
-        a = bytes_per_launch / (avg_us * 1e-6) / 1e9
-        t = traffic.get(name)
-        rooflines[name] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": a / HBM_PEAK_GBS, "traffic": t,
-                           "bytes_per_launch": bytes_per_launch,
-                           "avg_launch_us": avg_us,
-                           # the measured bytes over the same launch time, against the same peak
-                           "traffic_frac": (t / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if t else None}
-
+    rooflines = {}
     # PageRank launches in batches and the launches after convergence return at once: its
     # SpMV time per iteration is the kernel's time per step over the iterations that ran.
-    for name in ("pr_spmv", "pr_spmv_hot"):
+    for name in ("pr_spmv_push", "pr_spmv_hot", "pr_spmv"):
         if name in kernels:
-            add_roofline(name, spmv_bytes, 1000.0 * kernels[name]["total_ms"] / max(1, iters))
-            rooflines[name]["note"] = ("avg_launch_us and traffic are per iteration that ran (the "
-                                       "launches after convergence return at once)")
-            if rooflines[name]["traffic"] is not None:
-                # traffic.json averages over all launches of one step: per iteration that ran
-                per_iter = rooflines[name]["traffic"] * kernels[name]["launches"] / max(1, iters)
-                rooflines[name]["traffic"] = per_iter
-                rooflines[name]["traffic_frac"] = (per_iter / (rooflines[name]["avg_launch_us"] * 1e-6)
-                                                   / 1e9 / HBM_PEAK_GBS)
-    for name in ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_count"):
+            per_iter_us = 1000.0 * kernels[name]["total_ms"] / max(1, iters)
+            t = traffic.get(name)
+            t = t * kernels[name]["launches"] / max(1, iters) if t else None
+            rooflines[name] = roofline_entry(
+                spmv_bytes, per_iter_us, t,
+                "bytes_per_launch = 4(n+1) + 4m + 8n (offsets, targets, x read once, y written once); "
+                "avg_launch_us is per iteration that ran (launches after convergence return at once)"
+                + ("; the push-pull kernel reads the hubs' lists only inside the push set (77% of the "
+                   "target stream at R-MAT 22)" if name == "pr_spmv_push" else ""))
+    for name in ("tc_hash", "tc_count"):
         if name in kernels:
-            add_roofline(name, tc_bytes, kernels[name]["avg_us"])
+            rooflines[name] = roofline_entry(tc_merge_bytes, kernels[name]["avg_us"], traffic.get(name),
+                                             "merge model: 4(n+1) + 4 m_o + 4 sum over DAG edges (d+u + d+v)")
     for name in ("tc_bitmap16", "tc_bitmap"):
-        if name in rooflines:
-            rooflines[name]["note"] = (
-                "bytes_per_launch is the merge model (every DAG edge u->v streams both out-lists, "
-                "4 bytes an entry); the bitmap kernel reads L(u) once per row into an LDS bitmap "
-                "and, with the middle vertices split over the XCDs, serves most L(v) reads from "
-                "the L2s (tc_bitmap16 reads the entries of the top 32768 ids as 2-byte offsets), "
-                "so it moves fewer bytes (traffic) than the model and frac can exceed 1; "
-                "traffic_frac is the measured rate")
+        if name in kernels:
+            # tc_bitmap streams the same lists at 4 bytes per entry
+            model = tc16["bytes"] if name == "tc_bitmap16" else (
+                tc16["bytes"] - tc16["list_bytes"] + tc16["list_bytes_32bit"])
+            rooflines[name] = roofline_entry(
+                model, kernels[name]["avg_us"], traffic.get(name),
+                "stored-width model: 4(n+1) + 4n, L(u) at 4 bytes once per part it is split into, and "
+                "for every DAG edge u->v the list L(v) at its stored width (entries among the top 32768 "
+                "ids as 16-bit offsets in 16-byte octs of eight, others at 4 bytes); the merge model "
+                f"(both lists at 4 bytes, {tc_merge_bytes} bytes) would count data the kernel never reads")
+            rooflines[name]["merge_model_bytes"] = tc_merge_bytes
     if "tc_wedge" in kernels:
-        add_roofline("tc_wedge", wedge_bytes, kernels["tc_wedge"]["avg_us"])
+        rooflines["tc_wedge"] = roofline_entry(wedge_bytes, kernels["tc_wedge"]["avg_us"], traffic.get("tc_wedge"))
     # BFS over the whole call per source (every level's kernels, launch gaps and the read-back of
     # each level's counters), not kernel time alone.
-    add_roofline("bfs_traversal", bfs_bytes, 1000.0 * per_algo["bfs_ms_per_source"])
-    rooflines["bfs_traversal"]["note"] = "avg_launch_us is the whole BFS call per source"
-    bfs_traffic = bfs_call_traffic(f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}")
-    if bfs_traffic is not None:
-        r = rooflines["bfs_traversal"]
-        r["traffic"] = bfs_traffic
-        r["traffic_frac"] = bfs_traffic / (r["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS
-        r["note"] += ("; traffic sums the measured bytes of the level kernels (bfs_*, without the "
-                      "once-per-graph bfs_top_build) over the profiled calls, per call (bfs_init "
-                      "dispatches)")
+    rooflines["bfs_traversal"] = roofline_entry(
+        bfs_bytes, 1000.0 * per_algo["bfs_ms_per_source"], bfs_call_traffic(
+            f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}"),
+        "4(n+1) + 4m + 4n per traversal; avg_launch_us is the whole BFS call per source")
     # The dominant kernel: the one with the most event time in the profiled step.
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
-    roofline = dict(rooflines.get(dominant) or rooflines.get("pr_spmv") or {})
+    roofline = dict(rooflines.get(dominant) or {})
     if roofline:
-        roofline["kernel"] = dominant if dominant in rooflines else "pr_spmv"
+        roofline["kernel"] = dominant
 
     parity = {}
     try:
@@ -256,14 +358,36 @@ def main() -> int:
             parity["triangles_match_golden"] = results[-1]["triangles"] == golden[key]["triangles"]
     except FileNotFoundError:
         pass
+    bfs_golden = os.path.join(REPO, "tests", "golden", "rmat22_bfs.json")
+    if args.scale == 22 and args.edge_factor == 16 and args.seed == 1 and os.path.exists(bfs_golden):
+        with open(bfs_golden) as f:
+            want = json.load(f)["sources"]
+        host = np.empty(n, np.int32)
+        ok = True
+        for s in sources:
+            check(lib.gc_bfs_device(g._handle, s, d_dist, byref(c_uint32()), byref(c_int32()), byref(c_double())))
+            check(lib.gc_device_copy_to_host(dev._handle, host.ctypes.data_as(c_void_p), d_dist, 4 * n))
+            ok = ok and want.get(str(s), {}).get("sha256") == hashlib.sha256(host.tobytes()).hexdigest()
+        parity["bfs_distances_match_golden"] = ok
+
+    reference = None
+    if args.reference_convention == "on":
+        reference = reference_convention(dev, g, offsets, sources, args, m)
+        rooflines.update(reference.pop("rooflines"))
 
     batched = None
     if args.batched == "on":
         batched = batched_section(dev, g, offsets, n, m, args)
+        rooflines.update(batched.pop("rooflines"))
 
-    cpu = None
+    sched = None
+    if args.scheduler != "off":
+        sched = scheduler_section(dev, g, offsets, args, with_large=args.scheduler == "on")
+
+    cpu, precision = None, None
     if args.cpu_baseline == "on":
         cpu = cpu_baseline(offsets, targets, sources, iters, args.cpu_threads, m)
+        precision = pagerank_precision(g, offsets, targets, dev)
 
     out = {
         "metric": METRIC,
@@ -293,8 +417,11 @@ def main() -> int:
         "upload_seconds": upload_s,
         "triangles": results[-1]["triangles"],
         "parity": parity,
+        "reference_convention": reference,
         "cpu_baseline": cpu,
+        "pagerank_precision": precision,
         "batched": batched,
+        "scheduler": sched,
         "device": dev.name,
     }
     check(lib.gc_device_free(dev._handle, d_dist))
@@ -303,14 +430,88 @@ def main() -> int:
     return 0
 
 
+def reference_convention(dev, g, offsets, sources, args, m) -> dict:
+    """The reference's timing convention (crates/grafeo-cuda-algos/src/bin/gpu_*.rs): each
+    algorithm's public call with host outputs (the distances and ranks copied back inside the
+    call, as its time_seconds includes them, bfs.rs:121-130, pagerank.rs:190), warmup 1, runs 5,
+    the best run; BFS TEPS per source and their harmonic mean over the sources
+    (gpu_bfs.rs:116-124), plus a Graph500-style figure (undirected edges of the reached
+    component over the time). Also the k-truss (k = 4) call and its roofline."""
+    deg = np.diff(offsets.astype(np.int64))
+
+    def best(call, warmup=1, runs=5):
+        for _ in range(warmup):
+            call()
+        return min(call() for _ in range(runs))
+
+    out = {"convention": "host outputs, best of 5 calls after 1 warmup"}
+    out["tc"] = {"time_s": best(lambda: g.triangle_count()["time_seconds"])}
+    out["tc"]["gteps"] = m / out["tc"]["time_s"] / 1e9
+    per_source, g500 = [], []
+    for s in sources:
+        last = {}
+
+        def call():
+            r = g.bfs_numpy(s)
+            last["r"] = r
+            return r["time_seconds"]
+
+        t = best(call)
+        per_source.append(t)
+        reached = last["r"]["distances"] >= 0
+        g500.append(float(deg[reached].sum()) / 2.0 / t)
+    out["bfs"] = {"time_s_per_source": per_source,
+                  "gteps_harmonic_mean": len(sources) / sum(t / m for t in per_source) / 1e9,
+                  "graph500_gteps_harmonic_mean": len(g500) / sum(1.0 / x for x in g500) / 1e9}
+    pr = {}
+
+    def pr_call():
+        r = g.pagerank_numpy(0.85, args.pr_iterations, 1e-6)
+        pr["it"] = r["iterations"]
+        return r["time_seconds"]
+
+    out["pagerank"] = {"time_s": best(pr_call)}
+    out["pagerank"]["iterations"] = pr["it"]
+    out["pagerank"]["gteps"] = m * pr["it"] / out["pagerank"]["time_s"] / 1e9
+    # k-truss, k = 4 (config of the R-MAT 20 suite; gpu_ktruss.rs:51-56 reports entries / time)
+    kt = {}
+
+    def kt_call():
+        r = g.k_truss(4)
+        kt["r"] = r
+        return r["time_seconds"]
+
+    dev.profile(True, only=None)
+    out["ktruss"] = {"k": 4, "time_s": best(kt_call, warmup=1, runs=5)}
+    kt_kernel_ms = sum_kernels(dev, "ktruss") / 6.0
+    dev.profile(False)
+    rounds = kt["r"]["iterations"]
+    out["ktruss"].update({"gteps": m / out["ktruss"]["time_s"] / 1e9, "rounds": rounds,
+                          "edges": len(kt["r"]["edges"]), "kernel_ms_per_call": kt_kernel_ms})
+    # The survey's model: triangle-count bytes per support pass times passes, plus 5m per peel
+    # round. This library counts supports once (triangle enumeration on the DAG, tc_wedge's bytes)
+    # and then only peels.
+    n = offsets.size - 1
+    from grafeo_cuda._native import check, lib
+    m_o, inter, wedges = c_uint64(), c_uint64(), c_uint64()
+    check(lib.gc_graph_dag_stats(g._handle, byref(m_o), byref(inter), byref(wedges)))
+    kt_bytes = (4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value) + 5 * m * rounds
+    out["rooflines"] = {"ktruss": roofline_entry(
+        kt_bytes, 1000.0 * kt_kernel_ms,
+        note="one support pass at tc_wedge's bytes (4(n+1) + 4n + 4m + 4 wedges) plus 5m per peel round; "
+             "avg_launch_us is the kernels' event time per k-truss call (k = 4)")}
+    log(f"reference convention: {json.dumps({k: v for k, v in out.items() if k != 'rooflines'})[:400]}")
+    return out
+
+
 def batched_section(dev, g, offsets, n, m, args) -> dict:
     """Batched schedules against the same work done one unit at a time, on the same GPU, same
-    inputs, bit-identical results (checked here)."""
+    inputs, bit-identical results (checked here); speed-ups by call time and by kernel time."""
     import grafeo_cuda
     from grafeo_cuda import workloads
     from grafeo_cuda._native import check, lib
 
-    out = {}
+    out = {"rooflines": {}}
     # -- 128-source multi-source BFS vs 128 single-source BFS (same sources) --
     sources = pick_sources(offsets, 128, args.seed + 128)
     src = np.asarray(sources, dtype=np.uint32)
@@ -339,8 +540,16 @@ def batched_section(dev, g, offsets, n, m, args) -> dict:
 
     run_ms()
     ms_t = min(run_ms() for _ in range(3))
+    dev.profile(True, only=None)
+    run_ms()
+    ms_kernel_ms = sum_kernels(dev, "msbfs") + sum_kernels(dev, "util")
+    dev.profile(False)
     run_single()
     single_t, reach1 = min((run_single() for _ in range(2)), key=lambda x: x[0])
+    dev.profile(True, only=None)
+    run_single()
+    single_kernel_ms = sum_kernels(dev, "bfs") + sum_kernels(dev, "util")
+    dev.profile(False)
     # row 0 of the batched distances against the single-source distances of source 0
     row = np.empty(n, np.int32)
     one = np.empty(n, np.int32)
@@ -350,12 +559,24 @@ def batched_section(dev, g, offsets, n, m, args) -> dict:
     check(lib.gc_device_copy_to_host(dev._handle, one.ctypes.data_as(c_void_p), d_one, 4 * n))
     check(lib.gc_device_free(dev._handle, d_ms))
     check(lib.gc_device_free(dev._handle, d_one))
+    levels = int(depth.max()) + 1
     out["msbfs"] = {
         "sources": len(sources), "ms": 1000 * ms_t, "gteps": m * len(sources) / ms_t / 1e9,
+        "kernel_ms": ms_kernel_ms,
         "single_source_ms": 1000 * single_t, "single_source_gteps": m * len(sources) / single_t / 1e9,
-        "speedup": single_t / ms_t,
+        "single_source_kernel_ms": single_kernel_ms,
+        "speedup": single_t / ms_t, "speedup_kernel_time": single_kernel_ms / ms_kernel_ms,
+        "levels": levels,
         "identical": bool(list(reach) == reach1 and np.array_equal(row, one)),
     }
+    # Model that fits the multi-source kernels: per level the offsets, the adjacency once and the
+    # three 16-byte masks per vertex (seen, visit, next), plus the distances of every source once.
+    ms_bytes = levels * (4 * (n + 1) + 4 * m + 48 * n) + 4 * len(sources) * n
+    out["rooflines"]["msbfs"] = roofline_entry(
+        ms_bytes, 1000.0 * ms_kernel_ms,
+        note="levels x (4(n+1) + 4m + 48n) + 4 S n (structure and three 16-byte masks per level, "
+             "distances once); avg_launch_us is the kernels' event time per call; the per-source sum "
+             f"of the BFS model would be {len(sources) * (4 * (n + 1) + 8 * n + 4 * m)} bytes")
     log(f"msbfs: {out['msbfs']}")
 
     # -- 256-graph triangle-count batch vs one call per graph --
@@ -366,6 +587,7 @@ def batched_section(dev, g, offsets, n, m, args) -> dict:
     singles = [grafeo_cuda.GpuGraph.from_csr(o, t, dev) for o, t in graphs]
     for s_g in singles:
         s_g.prepare()
+    model_bytes = sum(dag_merge_bytes(o, t) for o, t in graphs)
     log(f"batch of 256 graphs ({entries} CSR entries) built in {time.time() - t0:.1f}s")
     expected = None
     try:
@@ -386,6 +608,12 @@ def batched_section(dev, g, offsets, n, m, args) -> dict:
     default_t, counts_default = time_batch()
     tuned = batch.tune()
     tuned_t, counts_tuned = time_batch()
+    dev.profile(True, only=["bt_count"])
+    for _ in range(5):
+        batch.triangle_count()
+    bt_ms, bt_n = dev.kernel_time("bt_count")
+    dev.profile(False)
+    bt_kernel_ms = bt_ms / max(1, bt_n)
 
     def time_singles():
         total, counts = 0.0, []
@@ -397,85 +625,142 @@ def batched_section(dev, g, offsets, n, m, args) -> dict:
 
     time_singles()
     single_t, counts_single = min((time_singles() for _ in range(2)), key=lambda x: x[0])
+    dev.profile(True, only=None)
+    time_singles()
+    singles_kernel_ms = sum_kernels(dev, "tc") + sum_kernels(dev, "util")
+    dev.profile(False)
     out["batch_tc"] = {
         "graphs": len(graphs), "csr_entries": entries,
         "default_config": list(default_cfg), "default_ms": 1000 * default_t,
         "tuned_config": list(tuned["config"]), "tuned_ms": 1000 * tuned_t,
-        "tuned_gteps": entries / tuned_t / 1e9,
-        "one_call_per_graph_ms": 1000 * single_t, "speedup_vs_one_call_per_graph": single_t / tuned_t,
+        "tuned_gteps": entries / tuned_t / 1e9, "kernel_ms": bt_kernel_ms,
+        "one_call_per_graph_ms": 1000 * single_t, "one_call_per_graph_kernel_ms": singles_kernel_ms,
+        "speedup_vs_one_call_per_graph": single_t / tuned_t,
+        "speedup_kernel_time": singles_kernel_ms / bt_kernel_ms,
         "speedup_tuned_vs_default": default_t / tuned_t,
         "identical": counts_default == counts_tuned == counts_single,
         "match_golden": None if expected is None else counts_tuned == expected,
     }
+    out["rooflines"]["batch_tc"] = roofline_entry(
+        model_bytes, 1000.0 * bt_kernel_ms,
+        note="the sum over the 256 graphs of the merge model 4(n+1) + 4 m_o + 4 sum (d+u + d+v); "
+             "avg_launch_us is bt_count's event time per replay")
     log(f"batch_tc: {out['batch_tc']}")
     return out
 
 
-def scheduler_workload(args) -> int:
-    """Learned schedules against the fixed rules (Beamer's direction switch for BFS; the bitmap
-    intersection for triangle counting) on the same GPU and inputs; results must be identical."""
+def forced_scheduler(dev, choice: int):
+    """A network whose outputs make `choice` the cheapest wherever it is allowed."""
+    import grafeo_cuda
+
+    s = grafeo_cuda.GpuScheduler(dev, seed=1)
+    p = np.zeros(1571, np.float32)
+    # ln-microsecond outputs 10 apart: far above the triangle count's 20 us switching margin
+    b3 = np.full(3, 10.0, np.float32)
+    b3[choice] = 0.0
+    p[-3:] = b3
+    s.set(params=p)
+    return s
+
+
+def scheduler_section(dev, g22, off22, args, with_large: bool) -> dict:
+    """The learned schedules against the fixed rules on the same GPU and inputs: per graph, the
+    BFS total time per source (mean and 95% confidence interval over 5 runs after a warmup, each
+    run over all the graph's sources), the time per level, and the triangle count under each
+    schedule; distances and counts must be identical under every schedule. Rows: Beamer's
+    switch alone, Beamer with device-chained top-down levels (the fixed default), each choice
+    forced, and the learned network. Graphs: R-MAT 22 (the step's graph, 8 sources), and with
+    `with_large` R-MAT 24 (4 sources) and the 4096 x 4096 grid (centre and corner), which the
+    network was not trained on."""
     import grafeo_cuda
     from grafeo_cuda import scheduler
     from grafeo_cuda._native import check, lib
 
-    dev = grafeo_cuda.GpuDevice(0)
     bfs_s, tc_s = scheduler.load(dev, args.model or scheduler.MODEL)
-    out = {"metric": "learned vs fixed schedule (BFS per level, TC per degree bucket)",
-           "unit": "ms", "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "graphs": {}}
-    cases = [("rmat24", lambda: scheduler.rmat(24, 16, 1), 4), ("grid4096", lambda: scheduler.grid(4096, 4096), 0)]
-    for name, build, nsrc in cases:
-        t0 = time.time()
-        off, tgt = build()
+    forced = [forced_scheduler(dev, c) for c in range(3)]
+    runs = 5
+    out = {"runs": runs, "graphs": {}}
+
+    def graph_cases():
+        if g22 is not None:
+            yield "rmat22", g22, off22, pick_sources(off22, args.sources, args.seed)
+        if with_large:
+            for name, pairs_fn, n in (("rmat24", lambda: grafeo_cuda._native.gen_rmat(24, 16, 1), 1 << 24),
+                                      ("grid4096", lambda: grafeo_cuda._native.gen_grid(4096, 4096), 4096 * 4096)):
+                t0 = time.time()
+                pairs = pairs_fn()
+                gg = grafeo_cuda.GpuGraph.from_edge_pairs(pairs, n, dev)
+                del pairs
+                gg.prepare()
+                off = np.empty(n + 1, np.uint32)
+                check(lib.gc_device_copy_to_host(dev._handle, off.ctypes.data_as(c_void_p),
+                                                 c_void_p(lib.gc_graph_device_offsets(gg._handle)), 4 * (n + 1)))
+                srcs = pick_sources(off, 4, args.seed) if name == "rmat24" else [2048 * 4096 + 2048, 0]
+                log(f"{name}: n={n} m={gg.num_edges} ready in {time.time() - t0:.1f}s")
+                yield name, gg, off, srcs
+                del gg
+
+    bfs_rows = [("beamer", {"GC_BFS_RULE": "beamer"}, None), ("beamer+chain (fixed default)", {}, None),
+                ("forced top-down", {}, forced[0]), ("forced bottom-up", {}, forced[1]),
+                ("forced chained", {}, forced[2]), ("learned", {}, bfs_s)]
+    tc_rows = [("bitmap (fixed)", None), ("forced hash", forced[1]), ("forced lists", forced[2]),
+               ("learned", tc_s)]
+    for name, g, off, srcs in graph_cases():
         n = off.size - 1
-        sources = pick_sources(off, nsrc, args.seed) if nsrc else [n // 2 + 2048, 0]
-        g = grafeo_cuda.GpuGraph.from_csr(off, tgt, dev)
-        del tgt
-        g.prepare()
-        log(f"{name}: n={n} m={g.num_edges} ready in {time.time() - t0:.1f}s")
         d_dist = c_void_p()
         check(lib.gc_device_malloc(dev._handle, 4 * n, byref(d_dist)))
         host = np.empty(n, np.int32)
-
-        def bfs_times():
-            total, checks = 0.0, []
-            for s in sources:
-                secs, reach, depth = c_double(), c_uint32(), c_int32()
+        entry = {"num_nodes": n, "csr_entries": g.num_edges, "bfs_sources": len(srcs), "bfs": {}, "tc": {}}
+        reference_hashes = None
+        for label, env, sched in bfs_rows:
+            os.environ.pop("GC_BFS_RULE", None)
+            os.environ.update(env)
+            g.set_scheduler(bfs=sched)
+            hashes, levels = [], 0
+            for s in srcs:  # warmup, with the distances checked
+                reach, depth, secs = c_uint32(), c_int32(), c_double()
                 check(lib.gc_bfs_device(g._handle, s, d_dist, byref(reach), byref(depth), byref(secs)))
-                best = secs.value
-                check(lib.gc_bfs_device(g._handle, s, d_dist, byref(reach), byref(depth), byref(secs)))
-                best = min(best, secs.value)
                 check(lib.gc_device_copy_to_host(dev._handle, host.ctypes.data_as(c_void_p), d_dist, 4 * n))
-                checks.append((reach.value, depth.value, hash(host.tobytes())))
-                total += best
-            return total, checks
-
-        def tc_time():
-            g.triangle_count()
-            r = [g.triangle_count() for _ in range(2)]
-            return min(x["time_seconds"] for x in r), r[0]["total_triangles"]
-
+                hashes.append((reach.value, depth.value, hashlib.sha256(host.tobytes()).hexdigest()))
+                levels += depth.value + 1
+            if reference_hashes is None:
+                reference_hashes = hashes
+            samples = []
+            for _ in range(runs):
+                total = 0.0
+                for s in srcs:
+                    secs = c_double()
+                    check(lib.gc_bfs_device(g._handle, s, d_dist, byref(c_uint32()), byref(c_int32()), byref(secs)))
+                    total += secs.value
+                samples.append(1000.0 * total / len(srcs))
+            st = mean_ci(samples)
+            entry["bfs"][label] = {"ms_per_source": st, "levels_per_source": levels / len(srcs),
+                                   "us_per_level": 1000.0 * st["mean"] * len(srcs) / levels,
+                                   "samples_ms": samples, "identical": hashes == reference_hashes,
+                                   "last_schedule": {str(c): g.last_schedule().count(c) for c in (0, 1, 2)}}
+            log(f"{name} BFS {label}: {st['mean']:.3f} +- {st['ci95']:.3f} ms/source")
+        os.environ.pop("GC_BFS_RULE", None)
         g.set_scheduler()
-        fixed_bfs, fixed_checks = bfs_times()
-        fixed_tc, fixed_tri = tc_time()
-        g.set_scheduler(bfs_s, tc_s)
-        learned_bfs, learned_checks = bfs_times()
-        bfs_schedule = g.last_schedule()
-        learned_tc, learned_tri = tc_time()
-        tc_schedule = g.last_schedule()
+        b = entry["bfs"]
+        entry["bfs_learned_speedup"] = {
+            other: ratio_ci(b[other]["samples_ms"], b["learned"]["samples_ms"])
+            for other in ("beamer", "beamer+chain (fixed default)", "forced top-down", "forced bottom-up",
+                          "forced chained")}
+        counts = None
+        for label, sched in tc_rows:
+            g.set_scheduler(tc=sched)
+            first = g.triangle_count()["total_triangles"]
+            samples = [1000.0 * g.triangle_count()["time_seconds"] for _ in range(runs)]
+            counts = first if counts is None else counts
+            entry["tc"][label] = {"ms": mean_ci(samples), "samples_ms": samples, "identical": first == counts,
+                                  "triangles": first, "schedule": g.last_schedule()}
+            log(f"{name} TC {label}: {entry['tc'][label]['ms']['mean']:.2f} ms")
+        g.set_scheduler()
+        entry["tc_learned_speedup_vs_fixed"] = ratio_ci(entry["tc"]["bitmap (fixed)"]["samples_ms"],
+                                                        entry["tc"]["learned"]["samples_ms"])
         check(lib.gc_device_free(dev._handle, d_dist))
-        out["graphs"][name] = {
-            "num_nodes": n, "csr_entries": g.num_edges, "bfs_sources": len(sources),
-            "bfs_fixed_ms": 1000 * fixed_bfs, "bfs_learned_ms": 1000 * learned_bfs,
-            "bfs_speedup": fixed_bfs / learned_bfs, "bfs_identical": fixed_checks == learned_checks,
-            "bfs_last_schedule": {str(c): bfs_schedule.count(c) for c in (0, 1, 2)},
-            "tc_fixed_ms": 1000 * fixed_tc, "tc_learned_ms": 1000 * learned_tc,
-            "tc_speedup": fixed_tc / learned_tc, "tc_identical": fixed_tri == learned_tri,
-            "tc_schedule": tc_schedule, "triangles": fixed_tri,
-        }
-        log(f"{name}: {out['graphs'][name]}")
-        del g
-    print(json.dumps(out), flush=True)
-    return 0
+        out["graphs"][name] = entry
+    return out
 
 
 def calibrate_copy(nbytes: int) -> int:
@@ -524,35 +809,109 @@ def bfs_call_traffic(workload_key: str):
     return total / calls
 
 
+def host_cpu() -> dict:
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(offsets, targets, sources, iters, threads, m) -> dict:
-    """The C oracle (a port of the reference's semantics, OpenMP) on the host cores, on a sample:
-    one triangle count, one BFS source, two PageRank iterations; scaled to the step's mix."""
+    """The C oracle (a port of the reference's semantics, OpenMP) on the host, on a bounded
+    sample of the step: the triangle count (its degree-ordered DAG built once, then the merge over
+    every stride-th DAG row, times the stride), one BFS source, and PageRank iterations; each
+    sample best of 5 after 1 warmup (the reference's defaults, gpu_triangles.rs:14-20), on all the
+    threads this process may use and on one thread; scaled to the step's mix."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    threads = threads or min(16, os.cpu_count() or 1)
-    oracle.set_threads(threads)
-    t = time.perf_counter()
-    oracle.triangle_count(offsets, targets, fast=True)
-    tc_s = time.perf_counter() - t
-    t = time.perf_counter()
-    oracle.bfs(offsets, targets, sources[0])
-    bfs_s = time.perf_counter() - t
-    t = time.perf_counter()
-    oracle.pagerank(offsets, targets, 0.85, 2, 0.0)
-    pr_iter_s = (time.perf_counter() - t) / 2
-    step_s = tc_s + len(sources) * bfs_s + iters * pr_iter_s
-    value = m * (1 + len(sources) + iters) / step_s / 1e9
-    log(f"cpu baseline: tc {tc_s:.2f}s bfs {bfs_s:.2f}s pr/it {pr_iter_s:.3f}s on {threads} threads")
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+    def best(fn, warmup=1, runs=5):
+        for _ in range(warmup):
+            fn()
+        out = []
+        for _ in range(runs):
+            t = time.perf_counter()
+            fn()
+            out.append(time.perf_counter() - t)
+        return min(out)
+
+    def measure(th: int, tc_stride: int, pr_its: int) -> dict:
+        oracle.set_threads(th)
+        t = time.perf_counter()
+        sample = oracle.TriangleSample(offsets, targets)
+        build = time.perf_counter() - t
+        try:
+            count = best(lambda: sample.count_rows(tc_stride, 0))
+        finally:
+            sample.close()
+        tc = build + tc_stride * count
+        bfs = best(lambda: oracle.bfs(offsets, targets, sources[0]))
+        pr = best(lambda: oracle.pagerank(offsets, targets, 0.85, pr_its, 0.0)) / pr_its
+        step = tc + len(sources) * bfs + iters * pr
+        return {"threads": th, "tc_s": tc, "tc_dag_build_s": build, "tc_row_stride": tc_stride,
+                "bfs_s_per_source": bfs, "pagerank_s_per_iteration": pr, "step_s": step,
+                "gteps": m * (1 + len(sources) + iters) / step / 1e9}
+
+    t0 = time.time()
+    full = measure(threads, 8, 2)
+    one = measure(1, 512, 1)
+    log(f"cpu baseline ({time.time() - t0:.1f}s): {threads} threads {full['gteps']:.3f} GTEPS, "
+        f"1 thread {one['gteps']:.3f} GTEPS")
     return {
-        "value": value,
+        "value": full["gteps"],
         "unit": "GTEPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"1 triangle count, 1 BFS source, 2 PageRank iterations of the same graph, "
-                  f"scaled to the step's mix (tc {tc_s:.3f}s, bfs {bfs_s:.3f}s/source, "
-                  f"pagerank {pr_iter_s:.4f}s/iteration); host {platform.processor() or platform.machine()}",
+        "host": host_cpu(),
+        "one_thread": one,
+        "all_threads": full,
+        "sample": (f"the step's mix from samples, each best of 5 after 1 warmup: triangle count = DAG build "
+                   f"+ 8 x the merge over every 8th DAG row ({threads} threads; every 512th row on one "
+                   f"thread), 1 BFS source, PageRank 2 iterations (1 on one thread); scaled to 1 triangle "
+                   f"count + {len(sources)} BFS + {iters} PageRank iterations"),
     }
+
+
+def pagerank_precision(g, offsets, targets, dev) -> dict:
+    """The PageRank precision study at the step's graph: d = 0.85, at most 50 iterations, tol
+    1e-7, FP32 ranks against the FP64 oracle run with the same settings (iteration count and
+    convergence must agree), max-abs and L1 error at that iteration, and the SpMV's rate."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    dev.profile(True, only=["pr_spmv_push", "pr_spmv_hot", "pr_spmv"])
+    r = g.pagerank_numpy(0.85, 50, 1e-7)
+    name = next(k for k in ("pr_spmv_push", "pr_spmv_hot", "pr_spmv") if dev.kernel_time(k)[1])
+    ms, launches = dev.kernel_time(name)
+    dev.profile(False)
+    _, it, conv = oracle.pagerank(offsets, targets, 0.85, 50, 1e-7)
+    k = r["iterations"]
+    ref = oracle.pagerank(offsets, targets, 0.85, k, 0.0)[0]  # the FP64 iterate of the same count
+    diff = np.abs(r["ranks"].astype(np.float64) - ref)
+    prev = g.pagerank_numpy(0.85, max(0, k - 1), 0.0)["ranks"].astype(np.float64)
+    n, m = offsets.size - 1, targets.size
+    per_iter_us = 1000.0 * ms / max(1, r["iterations"])
+    spmv_bytes = 4 * (n + 1) + 4 * m + 8 * n
+    out = {"damping": 0.85, "max_iterations": 50, "tolerance": 1e-7,
+           "iterations": k, "converged": r["converged"],
+           "oracle_iterations": it, "oracle_converged": conv,
+           "iterations_within_one": abs(k - it) <= 1 and bool(r["converged"]) == bool(conv),
+           "fp32_l1_change_at_stop": float(np.abs(r["ranks"].astype(np.float64) - prev).sum()),
+           "note": "L1 and max-abs against the FP64 oracle's iterate of the same count; the L1 change of "
+                   "FP32 iterates has a rounding floor near this tolerance, so the counts may differ by one",
+           "l1": float(diff.sum()), "max_abs": float(diff.max()), "l1_gate": 1e-6,
+           "spmv_kernel": name, "spmv_us_per_iteration": per_iter_us,
+           "spmv_gbs": spmv_bytes / (per_iter_us * 1e-6) / 1e9}
+    log(f"pagerank precision: {out}")
+    return out
 
 
 if __name__ == "__main__":

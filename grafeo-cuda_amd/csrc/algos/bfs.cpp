@@ -45,6 +45,9 @@ struct BfsChainState {  // bfs.hip
   unsigned long long all_edges;
   unsigned long long reached;
   unsigned long long edge_cap;
+  unsigned long long avg_cap;
+  unsigned long long unexplored;
+  double alpha;
 };
 
 struct BfsLevelState {  // mlp.hip
@@ -85,6 +88,17 @@ constexpr uint32_t kChainCap = 1u << 18;
 constexpr uint64_t kChainEdges = 1ull << 20;
 constexpr uint32_t kChainRunLevels = 1u << 30;
 constexpr uint32_t kChainLevels = 16;
+// The hand rule's chains also need a frontier of at most this many edges per
+// vertex: the chained expansion walks a list with one lane (up to 32 entries)
+// or one wave, so a frontier holding a hub would leave one wave walking it.
+constexpr uint64_t kChainAvgDegree = 32;
+
+// The fixed rules (GC_BFS_RULE): Beamer's direction switch alone ("beamer"), or the
+// same with top-down levels run as device-chained levels while the frontier fits
+// the chains ("chain", the default): at most kChainCap vertices, kChainEdges edges
+// and kChainAvgDegree edges per vertex; a chain hands back as soon as one of these
+// fails or Beamer's rule would switch to bottom-up.
+enum class FixedRule { kBeamer, kBeamerChain };
 // Bottom-up levels the fixed rule still runs, launched ahead, after Beamer's
 // rule would return to top-down (their frontiers are small and each costs
 // less than a top-down level with its read-back; the cap bounds path-like
@@ -132,6 +146,7 @@ struct Traversal {
   bool done = false;
   bool top_down_dir = true;  // the fixed rule's current direction
   double alpha = kAlpha;     // the fixed rule's top-down -> bottom-up threshold (GC_BFS_ALPHA)
+  FixedRule rule = FixedRule::kBeamerChain;
   // In queue form, whether bitmap[cur] holds exactly the queue's vertices and
   // bitmap[cur ^ 1] is empty (top-down levels then keep it so, and a switch to
   // bottom-up needs no conversion). Kept for symmetric graphs only.
@@ -175,6 +190,11 @@ struct Traversal {
     synced = g.symmetric;
     m_f = hoff[src + 1] - hoff[src];
     if (const char* a = std::getenv("GC_BFS_ALPHA")) alpha = std::max(1.0, std::atof(a));
+    if (const char* r = std::getenv("GC_BFS_RULE")) {
+      if (std::strcmp(r, "beamer") == 0) rule = FixedRule::kBeamer;
+      else if (std::strcmp(r, "chain") == 0) rule = FixedRule::kBeamerChain;
+      else fail_kernel(std::string("GC_BFS_RULE must be beamer or chain, not ") + r);
+    }
     m_u = static_cast<uint64_t>(g.m) - m_f;
   }
 
@@ -200,14 +220,25 @@ struct Traversal {
     return st;
   }
 
-  // The fixed rule: Beamer's direction switch (never the chained levels).
+  // The fixed rule: Beamer's direction switch; under the chain rule a
+  // top-down level whose frontier fits the chains runs as chained levels.
   uint32_t fixed_choice() {
     if (top_down_dir && g.symmetric && static_cast<double>(m_f) > static_cast<double>(m_u) / alpha) {
       top_down_dir = false;
     } else if (!top_down_dir && static_cast<double>(n_f) < static_cast<double>(n) / kBeta) {
       top_down_dir = true;
     }
-    return top_down_dir ? kTopDown : kBottomUp;
+    if (!top_down_dir) return kBottomUp;
+    if (rule == FixedRule::kBeamerChain && chain_fits()) return kChained;
+    return kTopDown;
+  }
+
+  // Whether the current frontier fits the hand rule's chains. The first level
+  // (the source alone) always runs top-down: one vertex's degree says nothing
+  // of the traversal's depth, and on a power-law graph a low-degree source's
+  // next frontier holds hubs (measured: chaining it cost 5-9% of an R-MAT BFS).
+  bool chain_fits() const {
+    return level >= 1 && n_f <= kChainCap && m_f <= kChainEdges && m_f <= kChainAvgDegree * n_f;
   }
 
   // Launches the decision kernel for the current state (or, with
@@ -434,7 +465,10 @@ struct Traversal {
   // one read-back each, until the traversal ends, hands back a large frontier
   // or has built max_levels levels (checked after each chain); with
   // max_levels == 1, one level launched directly. Returns the levels built.
-  uint32_t run_chain(uint32_t max_levels) {
+  // With `fixed_rule`, the chain also hands back when the hand rule would stop
+  // chaining (a frontier of more than kChainAvgDegree edges per vertex) or
+  // switch to bottom-up (Beamer's test on the device).
+  uint32_t run_chain(uint32_t max_levels, bool fixed_rule = false) {
     to_queue();
     synced = false;  // the chained levels keep no bitmap
     first_level = false;
@@ -444,23 +478,34 @@ struct Traversal {
     init.cap = kChainCap;
     init.edges = m_f;
     init.edge_cap = kChainEdges;
+    if (fixed_rule) {
+      init.avg_cap = kChainAvgDegree;
+      init.unexplored = m_u;
+      init.alpha = g.symmetric ? alpha : 0.0;
+    }
     copy_to_device(dev, chain_state, &init, sizeof(init));
     BfsChainState st{};
+    // The hand rule's first level runs as one direct launch: when the chain
+    // hands back after it (a hub next to a low-degree source), no replay of
+    // kChainLevels levels pays for the levels that return at once.
+    bool direct = max_levels == 1 || fixed_rule;
     for (;;) {
-      if (max_levels == 1) {
+      const int in = cur ^ static_cast<int>(st.levels & 1u);  // the queue holding the frontier
+      if (direct) {
         launch(dev, k_chain_expand, chain_grid(), 256u, 0, off, tgt, dist,
-               static_cast<const uint32_t*>(queue[cur]), queue[cur ^ 1], chain_state);
+               static_cast<const uint32_t*>(queue[in]), queue[in ^ 1], chain_state);
         launch(dev, k_chain_advance, 1u, 64u, 0, chain_state);
       } else {
         hipEvent_t prof = nullptr;
         profile_begin(dev, "bfs_chain", &prof);
-        hip_check(hipGraphLaunch(chain_exec(cur), dev.stream), "hipGraphLaunch");
+        hip_check(hipGraphLaunch(chain_exec(in), dev.stream), "hipGraphLaunch");
         profile_end(dev, "bfs_chain", prof);
       }
       copy_to_host(dev, dev.pinned, chain_state, sizeof(st));
       std::memcpy(&st, dev.pinned, sizeof(st));
       if (st.stop != 0 || max_levels == 1 || st.levels >= max_levels) break;
-      // a full chain has an even number of levels: the frontier is back in queue[cur]
+      direct = false;
+      // a full chain has an even number of levels, so `in` changes only after a direct level
     }
     reached += st.reached;
     m_u -= std::min<uint64_t>(m_u, st.all_edges);
@@ -483,7 +528,7 @@ struct Traversal {
   // computed on the device and read back with the counters.
   void step(uint32_t choice, uint32_t chain_levels, bool decide_next, const float* params) {
     if (choice == kChained) {
-      run_chain(chain_levels);
+      run_chain(chain_levels, params == nullptr);
       return;
     }
     launch_level(choice);
@@ -519,6 +564,25 @@ struct BfsOutcome {
   int32_t max_depth = 0;
 };
 
+// The fixed rule as the learned rule's fallback: the network's choice stands
+// only where it predicts it at least `margin` (in ln microseconds; 0.26, about
+// 30%, GC_SCHED_MARGIN) cheaper than the fixed rule's choice for the same
+// state, and the fixed rule decides elsewhere. A network trained on R-MAT
+// scales 16-22 went bottom-up from a 763-vertex frontier at scale 24 (0.50
+// against 0.21 ms for that level); the margin keeps such extrapolations from
+// overriding the rule unless the predicted gain is large.
+static float learned_margin() {
+  const char* env = std::getenv("GC_SCHED_MARGIN");
+  return env != nullptr ? static_cast<float>(std::atof(env)) : 0.26f;
+}
+
+static uint32_t with_fallback(Traversal& t, uint32_t learned, float margin) {
+  const uint32_t fixed = t.fixed_choice();  // also advances the rule's direction state
+  if (learned == fixed) return learned;
+  const float* p = t.last_decision.predicted;
+  return p[learned] + margin < p[fixed] ? learned : fixed;
+}
+
 // Runs BFS from `source` (caller's id) and writes caller-ordered distances to
 // the device array `d_out`. The scheduler attached to the graph, if any,
 // chooses each level's method; otherwise the fixed rule does.
@@ -541,6 +605,7 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
   // the next level from the state after it (one round trip).
   const char* ahead_env = std::getenv("GC_BFS_AHEAD");
   const bool ahead = ahead_env == nullptr || ahead_env[0] != '0';
+  const float margin = learned_margin();
   Stopwatch level_clock;
   while (!t.done) {
     if (!(t.allowed_mask() >> choice & 1u)) choice = kTopDown;
@@ -559,7 +624,7 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
         }
       });
       if (t.done) break;
-      choice = params != nullptr ? t.decide_now(params) : kTopDown;
+      choice = params != nullptr ? with_fallback(t, t.decide_now(params), margin) : kTopDown;
       continue;
     }
     g.last_schedule.push_back(choice);
@@ -576,8 +641,8 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
     }
     if (t.done) break;
     if (params == nullptr) choice = t.fixed_choice();
-    else if (learned_next) choice = t.last_decision.choice;
-    else choice = t.decide_now(params);
+    else if (learned_next) choice = with_fallback(t, t.last_decision.choice, margin);
+    else choice = with_fallback(t, t.decide_now(params), margin);
   }
   Grid1D vgrid = thread_per_item((static_cast<uint64_t>(g.n) + 3) / 4);  // four entries per lane
   launch(dev, get_kernel(dev, "util", "gather_u32"), vgrid.blocks, vgrid.threads, 0,
@@ -593,13 +658,17 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
 // Training data: at every level, the features of the decision and the time of
 // each allowed choice from the same state (dist and frontier saved and
 // restored around each trial); then the fastest choice is run for one level.
-// Chained levels are timed as one captured chain divided by the levels it
-// built, their cost per level when chosen.
+// Each choice is timed the way bfs() runs it when chosen, per level: chained
+// levels as one captured chain divided by the levels it built, bottom-up as
+// the run of levels launched ahead that the choice starts (unless
+// GC_BFS_AHEAD=0) divided by its levels.
 uint32_t bfs_level_timings(Graph& g, uint32_t source, uint32_t max_levels, float* features,
                            float* times_us) {
   prepare(g);
   Device& dev = *g.dev;
   Traversal t(g, g.rl.host_inv[source]);
+  const char* ahead_env = std::getenv("GC_BFS_AHEAD");
+  const bool ahead = ahead_env == nullptr || ahead_env[0] != '0';
   int32_t* save_dist = g.buffer<int32_t>("bfs.save_dist", sizeof(int32_t) * g.n);
   uint64_t* save_front = g.buffer<uint64_t>("bfs.save_front", std::max<uint64_t>(sizeof(uint32_t) * g.n, 16));
   uint32_t levels = 0;
@@ -634,6 +703,7 @@ uint32_t bfs_level_timings(Graph& g, uint32_t source, uint32_t max_levels, float
       t.done = false;
       t.synced = false;  // trials change the bitmaps; conversions rebuild them
       t.first_level = false;
+      t.top_down_dir = saved.top_down_dir;
       dev.sync();
     };
     const uint32_t allowed = t.allowed_mask();
@@ -651,6 +721,10 @@ uint32_t bfs_level_timings(Graph& g, uint32_t source, uint32_t max_levels, float
         if (c == kChained) {
           // chained levels cost what one captured chain costs per level it builds
           const uint32_t built = t.run_chain(kChainLevels);
+          us = static_cast<float>(sw.seconds() * 1e6) / static_cast<float>(std::max(1u, built));
+        } else if (c == kBottomUp && ahead) {
+          uint32_t built = 0;
+          t.run_bottom_up([&] { ++built; });
           us = static_cast<float>(sw.seconds() * 1e6) / static_cast<float>(std::max(1u, built));
         } else {
           t.step(c, 1, false, nullptr);

@@ -448,6 +448,12 @@ struct BfsChainState {
   unsigned long long all_edges;  // degree sums of every level built
   unsigned long long reached;    // vertices discovered
   unsigned long long edge_cap;
+  // Hand-back tests of the fixed rule's chains (0 = off): a frontier of more
+  // than avg_cap edges per vertex (the lists a lane or a wave walks alone),
+  // and Beamer's switch to bottom-up, next edges * alpha > unexplored edges.
+  unsigned long long avg_cap;
+  unsigned long long unexplored;  // edges of the vertices not yet reached
+  double alpha;
 };
 
 extern "C" __global__ __launch_bounds__(256) void bfs_chain_expand(
@@ -513,6 +519,9 @@ extern "C" __global__ __launch_bounds__(64) void bfs_chain_advance(BfsChainState
   st->all_edges += e;
   st->size = s;
   st->edges = e;
+  st->unexplored -= min(st->unexplored, e);
   if (s == 0) st->stop = 1;
   else if (s > st->cap || e > st->edge_cap) st->stop = 2;
+  else if (st->avg_cap != 0 && e > st->avg_cap * s) st->stop = 2;
+  else if (st->alpha > 0.0 && static_cast<double>(e) * st->alpha > static_cast<double>(st->unexplored)) st->stop = 2;
 }

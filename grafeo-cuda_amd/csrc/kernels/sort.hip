@@ -300,8 +300,66 @@ extern "C" __global__ __launch_bounds__(256) void tsv_line_starts(const uint8_t*
   }
 }
 
-__device__ __forceinline__ bool tsv_space(uint8_t c) {
-  return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f';
+// Byte length of the White_Space code point (Unicode's property, which Rust's
+// str::trim and split_whitespace use) starting at bytes[p], p < e, or 0:
+// U+0009..000D and U+0020 (one byte), U+0085 and U+00A0 (two), U+1680,
+// U+2000..200A, U+2028, U+2029, U+202F, U+205F and U+3000 (three).
+__device__ __forceinline__ uint32_t tsv_space_at(const uint8_t* bytes, uint32_t p, uint32_t e) {
+  const uint32_t c = bytes[p];
+  if (c == ' ' || (c >= 0x09 && c <= 0x0D)) return 1;
+  if (c < 0xC2 || c > 0xE3 || p + 1 >= e) return 0;
+  const uint32_t c1 = bytes[p + 1];
+  if (c == 0xC2) return c1 == 0x85 || c1 == 0xA0 ? 2 : 0;
+  if (p + 2 >= e) return 0;
+  const uint32_t c2 = bytes[p + 2];
+  if (c == 0xE1) return c1 == 0x9A && c2 == 0x80 ? 3 : 0;
+  if (c == 0xE2) {
+    if (c1 == 0x80) return (c2 <= 0x8A && c2 >= 0x80) || c2 == 0xA8 || c2 == 0xA9 || c2 == 0xAF ? 3 : 0;
+    return c1 == 0x81 && c2 == 0x9F ? 3 : 0;
+  }
+  if (c == 0xE3) return c1 == 0x80 && c2 == 0x80 ? 3 : 0;
+  return 0;
+}
+
+// Byte length of a White_Space code point ending at bytes[e - 1], b < e, or 0
+// (the line is valid UTF-8, so a matching suffix is that code point).
+__device__ __forceinline__ uint32_t tsv_space_before(const uint8_t* bytes, uint32_t b, uint32_t e) {
+  if (tsv_space_at(bytes, e - 1, e) == 1) return 1;
+  if (e - b >= 2 && tsv_space_at(bytes, e - 2, e) == 2) return 2;
+  if (e - b >= 3 && tsv_space_at(bytes, e - 3, e) == 3) return 3;
+  return 0;
+}
+
+// Whether bytes [b, e) are well-formed UTF-8 (no overlong forms, no
+// surrogates, nothing above U+10FFFF), the check of Rust's BufRead::lines.
+__device__ __forceinline__ bool tsv_utf8_ok(const uint8_t* bytes, uint32_t b, uint32_t e) {
+  for (uint32_t i = b; i < e;) {
+    const uint32_t c = bytes[i];
+    if (c < 0x80) {
+      ++i;
+      continue;
+    }
+    uint32_t len, lo = 0x80, hi = 0xBF;
+    if (c >= 0xC2 && c <= 0xDF) len = 2;
+    else if (c >= 0xE0 && c <= 0xEF) {
+      len = 3;
+      if (c == 0xE0) lo = 0xA0;
+      if (c == 0xED) hi = 0x9F;
+    } else if (c >= 0xF0 && c <= 0xF4) {
+      len = 4;
+      if (c == 0xF0) lo = 0x90;
+      if (c == 0xF4) hi = 0x8F;
+    } else {
+      return false;
+    }
+    if (e - i < len) return false;
+    const uint32_t c1 = bytes[i + 1];
+    if (c1 < lo || c1 > hi) return false;
+    for (uint32_t k = 2; k < len; ++k)
+      if ((bytes[i + k] & 0xC0u) != 0x80u) return false;
+    i += len;
+  }
+  return true;
 }
 
 __device__ __forceinline__ bool tsv_parse_u64(const uint8_t* bytes, uint32_t b, uint32_t e, unsigned long long* out) {
@@ -320,24 +378,30 @@ __device__ __forceinline__ bool tsv_parse_u64(const uint8_t* bytes, uint32_t b, 
 }
 
 // Line i = bytes [starts[i], starts[i + 1] - 1), the last one up to `size`:
-// its two ids, and whether it holds an edge.
+// its two ids, and whether it holds an edge. As Rust's lines / trim /
+// split_whitespace: the line is trimmed of White_Space at both ends, fields
+// are separated by White_Space runs, and a line that is not valid UTF-8 sets
+// *bad_utf8 (the whole parse then fails, as BufRead::lines does).
 extern "C" __global__ __launch_bounds__(256) void tsv_parse_lines(
     const uint8_t* __restrict__ bytes, uint32_t size, const uint32_t* __restrict__ starts, uint32_t lines,
-    unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst, uint8_t* __restrict__ valid) {
+    unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst, uint8_t* __restrict__ valid,
+    uint32_t* __restrict__ bad_utf8) {
   for (uint64_t li = global_thread(); li < lines; li += grid_threads()) {
     uint32_t b = starts[li];
     uint32_t e = li + 1 < lines ? starts[li + 1] - 1 : size;
-    while (b < e && tsv_space(bytes[b])) ++b;
-    while (e > b && tsv_space(bytes[e - 1])) --e;
+    if (!tsv_utf8_ok(bytes, b, e)) atomicOr(bad_utf8, 1u);
+    uint32_t k;
+    while (b < e && (k = tsv_space_at(bytes, b, e)) != 0) b += k;
+    while (e > b && (k = tsv_space_before(bytes, b, e)) != 0) e -= k;
     bool ok = b < e && bytes[b] != '#' && bytes[b] != '%';
     unsigned long long s = 0, d = 0;
     if (ok) {
       uint32_t p = b;
-      while (p < e && !tsv_space(bytes[p])) ++p;
+      while (p < e && tsv_space_at(bytes, p, e) == 0) ++p;
       const uint32_t f1e = p;
-      while (p < e && tsv_space(bytes[p])) ++p;
+      while (p < e && (k = tsv_space_at(bytes, p, e)) != 0) p += k;
       const uint32_t f2b = p;
-      while (p < e && !tsv_space(bytes[p])) ++p;
+      while (p < e && tsv_space_at(bytes, p, e) == 0) ++p;
       ok = tsv_parse_u64(bytes, b, f1e, &s) && tsv_parse_u64(bytes, f2b, p, &d);
     }
     src[li] = s;

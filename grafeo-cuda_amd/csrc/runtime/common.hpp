@@ -29,6 +29,10 @@ struct Error {
 };
 
 [[noreturn]] void fail(gc_status status, const std::string& detail);
+
+// The io::Error text Rust's BufRead::lines gives for a line that is not UTF-8
+// (parse_tsv_to_edges fails with it).
+inline constexpr const char* kInvalidUtf8 = "stream did not contain valid UTF-8";
 [[noreturn]] inline void fail_kernel(const std::string& detail) { fail(GC_ERR_KERNEL, detail); }
 [[noreturn]] inline void fail_csr(const std::string& detail) { fail(GC_ERR_INVALID_CSR, detail); }
 

@@ -279,6 +279,72 @@ gc_status gc_csr_from_edge_pairs_oriented(const uint32_t* pairs, size_t num_pair
   });
 }
 
+namespace {
+
+// Byte length of the White_Space code point (the property Rust's str::trim and
+// split_whitespace use) starting at p < e, or 0: U+0009..000D and U+0020 (one
+// byte), U+0085 and U+00A0 (two), U+1680, U+2000..200A, U+2028, U+2029,
+// U+202F, U+205F and U+3000 (three).
+size_t space_at(const unsigned char* p, const unsigned char* e) {
+  const unsigned c = p[0];
+  if (c == ' ' || (c >= 0x09 && c <= 0x0D)) return 1;
+  if (c < 0xC2 || c > 0xE3 || e - p < 2) return 0;
+  const unsigned c1 = p[1];
+  if (c == 0xC2) return c1 == 0x85 || c1 == 0xA0 ? 2 : 0;
+  if (e - p < 3) return 0;
+  const unsigned c2 = p[2];
+  if (c == 0xE1) return c1 == 0x9A && c2 == 0x80 ? 3 : 0;
+  if (c == 0xE2) {
+    if (c1 == 0x80) return (c2 >= 0x80 && c2 <= 0x8A) || c2 == 0xA8 || c2 == 0xA9 || c2 == 0xAF ? 3 : 0;
+    return c1 == 0x81 && c2 == 0x9F ? 3 : 0;
+  }
+  if (c == 0xE3) return c1 == 0x80 && c2 == 0x80 ? 3 : 0;
+  return 0;
+}
+
+// Byte length of a White_Space code point ending at e[-1], b < e, or 0 (the
+// line is valid UTF-8, so a matching suffix is that code point).
+size_t space_before(const unsigned char* b, const unsigned char* e) {
+  if (space_at(e - 1, e) == 1) return 1;
+  if (e - b >= 2 && space_at(e - 2, e) == 2) return 2;
+  if (e - b >= 3 && space_at(e - 3, e) == 3) return 3;
+  return 0;
+}
+
+// Whether [b, e) is well-formed UTF-8 (no overlong forms, no surrogates,
+// nothing above U+10FFFF), the check of Rust's BufRead::lines.
+bool utf8_ok(const unsigned char* b, const unsigned char* e) {
+  for (const unsigned char* p = b; p < e;) {
+    const unsigned c = *p;
+    if (c < 0x80) {
+      ++p;
+      continue;
+    }
+    ptrdiff_t len;
+    unsigned lo = 0x80, hi = 0xBF;
+    if (c >= 0xC2 && c <= 0xDF) {
+      len = 2;
+    } else if (c >= 0xE0 && c <= 0xEF) {
+      len = 3;
+      if (c == 0xE0) lo = 0xA0;
+      if (c == 0xED) hi = 0x9F;
+    } else if (c >= 0xF0 && c <= 0xF4) {
+      len = 4;
+      if (c == 0xF0) lo = 0x90;
+      if (c == 0xF4) hi = 0x8F;
+    } else {
+      return false;
+    }
+    if (e - p < len || p[1] < lo || p[1] > hi) return false;
+    for (ptrdiff_t k = 2; k < len; ++k)
+      if ((p[k] & 0xC0u) != 0x80u) return false;
+    p += len;
+  }
+  return true;
+}
+
+}  // namespace
+
 gc_status gc_parse_tsv_to_edges(const char* path, uint32_t** pairs, size_t* num_pairs,
                                 uint32_t* num_nodes) {
   return guard([&] {
@@ -289,6 +355,7 @@ gc_status gc_parse_tsv_to_edges(const char* path, uint32_t** pairs, size_t* num_
     if (f == nullptr) {
       gc::fail(GC_ERR_EXPORT, std::string("cannot open ") + path + ": " + std::strerror(errno));
     }
+    std::unique_ptr<FILE, int (*)(FILE*)> file(f, &std::fclose);  // closed on every path
     std::unordered_map<uint64_t, uint32_t> ids;
     std::vector<uint32_t> out;
     uint32_t next_id = 0;
@@ -313,23 +380,26 @@ gc_status gc_parse_tsv_to_edges(const char* path, uint32_t** pairs, size_t* num_
       *value = v;
       return true;
     };
-    auto is_space = [](char c) {
-      return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f';
-    };
     std::vector<char> buf(1 << 20);
     std::string line;
-    auto handle_line = [&](const char* b, const char* e) {
-      while (b < e && is_space(*b)) ++b;
-      while (e > b && is_space(e[-1])) --e;
+    // As Rust's lines / trim / split_whitespace: a line that is not UTF-8
+    // fails the parse; White_Space is trimmed and separates the fields.
+    auto handle_line = [&](const char* cb, const char* ce) {
+      const auto* b = reinterpret_cast<const unsigned char*>(cb);
+      const auto* e = reinterpret_cast<const unsigned char*>(ce);
+      if (!utf8_ok(b, e)) gc::fail(GC_ERR_EXPORT, gc::kInvalidUtf8);
+      size_t k;
+      while (b < e && (k = space_at(b, e)) != 0) b += k;
+      while (e > b && (k = space_before(b, e)) != 0) e -= k;
       if (b == e || *b == '#' || *b == '%') return;
-      const char* f1b = b;
-      const char* p = b;
-      while (p < e && !is_space(*p)) ++p;
-      const char* f1e = p;
-      while (p < e && is_space(*p)) ++p;
-      const char* f2b = p;
-      while (p < e && !is_space(*p)) ++p;
-      const char* f2e = p;
+      const char* f1b = reinterpret_cast<const char*>(b);
+      const unsigned char* p = b;
+      while (p < e && space_at(p, e) == 0) ++p;
+      const char* f1e = reinterpret_cast<const char*>(p);
+      while (p < e && (k = space_at(p, e)) != 0) p += k;
+      const char* f2b = reinterpret_cast<const char*>(p);
+      while (p < e && space_at(p, e) == 0) ++p;
+      const char* f2e = reinterpret_cast<const char*>(p);
       uint64_t s, d;
       if (!parse_u64(f1b, f1e, &s)) return;
       if (!parse_u64(f2b, f2e, &d)) return;
@@ -356,7 +426,7 @@ gc_status gc_parse_tsv_to_edges(const char* path, uint32_t** pairs, size_t* num_
       line.append(buf.data() + start, got - start);
     }
     bool read_error = std::ferror(f) != 0;
-    std::fclose(f);
+    file.reset();
     if (read_error) gc::fail(GC_ERR_EXPORT, std::string("cannot read ") + path);
     if (!line.empty()) handle_line(line.data(), line.data() + line.size());
     auto* result = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<size_t>(1, out.size())));

@@ -307,11 +307,18 @@ void parse_tsv_on_device(Device& dev, const char* path, DeviceMemory& pairs, uin
   src.allocate(sizeof(unsigned long long) * lines);
   dst.allocate(sizeof(unsigned long long) * lines);
   valid.allocate(std::max<uint64_t>(lines, 4));
+  DeviceMemory bad_utf8;
+  bad_utf8.allocate(sizeof(uint32_t));
+  fill_words(dev, bad_utf8.get(), 1, 0);
   Grid1D lg = thread_per_item(lines);
   launch(dev, get_kernel(dev, "sort", "tsv_parse_lines"), lg.blocks, lg.threads, 0,
          static_cast<const uint8_t*>(bytes.as<uint8_t>()), static_cast<uint32_t>(size),
          static_cast<const uint32_t*>(starts.as<uint32_t>()), lines, src.as<unsigned long long>(),
-         dst.as<unsigned long long>(), valid.as<uint8_t>());
+         dst.as<unsigned long long>(), valid.as<uint8_t>(), bad_utf8.as<uint32_t>());
+  uint32_t bad = 0;
+  copy_to_host(dev, dev.pinned, bad_utf8.get(), sizeof(uint32_t));
+  std::memcpy(&bad, dev.pinned, sizeof(uint32_t));
+  if (bad != 0) fail(GC_ERR_EXPORT, kInvalidUtf8);
   bytes.release();
   starts.release();
   // the edge lines, in order

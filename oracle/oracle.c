@@ -272,6 +272,44 @@ uint64_t oracle_triangle_count_fast(const uint32_t* off, const uint32_t* tgt, ui
   return total;
 }
 
+/* The count of oracle_triangle_count_fast in two timed halves, for the CPU
+ * baseline's bounded sample: oracle_tc_prepare builds the degree-ordered DAG
+ * (an opaque handle), oracle_tc_count_rows runs the same merge over the DAG
+ * rows r with r % stride == phase only (the phases of one stride add up to the
+ * full count), oracle_tc_release frees the DAG. */
+void* oracle_tc_prepare(const uint32_t* off, const uint32_t* tgt, uint32_t n) {
+  dag_t* d = (dag_t*)malloc(sizeof(dag_t));
+  *d = build_dag(off, tgt, n, NULL);
+  return d;
+}
+
+uint64_t oracle_tc_count_rows(const void* handle, uint32_t stride, uint32_t phase) {
+  const dag_t* d = (const dag_t*)handle;
+  const uint32_t n = d->n;
+  if (stride == 0) stride = 1;
+  uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+  for (int64_t rr = phase; rr < (int64_t)n; rr += stride) {
+    uint32_t u = (uint32_t)rr;
+    for (uint32_t i = d->off[u]; i < d->off[u + 1]; ++i) {
+      uint32_t v = d->tgt[i];
+      uint32_t j = i + 1, je = d->off[u + 1], k = d->off[v], ke = d->off[v + 1];
+      while (j < je && k < ke) {
+        if (d->tgt[j] == d->tgt[k]) { ++total; ++j; ++k; }
+        else if (d->tgt[j] < d->tgt[k]) ++j;
+        else ++k;
+      }
+    }
+  }
+  return total;
+}
+
+void oracle_tc_release(void* handle) {
+  dag_t* d = (dag_t*)handle;
+  free_dag(d);
+  free(d);
+}
+
 /* ---------------------------------------------------------------- BFS -- */
 
 /* bfs::bfs (crates/grafeo-cuda-algos/src/bfs.rs:42-131, kernels/bfs.cu:54-83):

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import re
 import subprocess
 from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_int32, c_size_t, c_uint32, c_uint64
 
@@ -44,6 +45,9 @@ def _load() -> ctypes.CDLL:
         "oracle_k_truss_ref": (c_uint32, [u32p, u32p, c_uint32, c_uint32, POINTER(ctypes.c_uint8)]),
         "oracle_k_truss_fast": (c_uint32, [u32p, u32p, c_uint32, c_uint32, POINTER(ctypes.c_uint8)]),
         "oracle_truss_numbers": (ctypes.c_int64, [u32p, u32p, c_uint32, u32p]),
+        "oracle_tc_prepare": (ctypes.c_void_p, [u32p, u32p, c_uint32]),
+        "oracle_tc_count_rows": (c_uint64, [ctypes.c_void_p, c_uint32, c_uint32]),
+        "oracle_tc_release": (None, [ctypes.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -99,6 +103,27 @@ def triangle_count(offsets, targets, fast: bool = False) -> int:
     off, tgt = _u32(offsets), _u32(targets)
     fn = lib.oracle_triangle_count_fast if fast else lib.oracle_triangle_count_ref
     return int(fn(_p(off), _p(tgt), off.size - 1))
+
+
+class TriangleSample:
+    """triangle_count(fast=True) in two timed halves (the CPU baseline's bounded sample): the
+    degree-ordered DAG is built once; count_rows(stride, phase) merges only the DAG rows r with
+    r % stride == phase, and the phases of one stride add up to the full count."""
+
+    def __init__(self, offsets, targets):
+        self._off, self._tgt = _u32(offsets), _u32(targets)
+        self._h = lib.oracle_tc_prepare(_p(self._off), _p(self._tgt), self._off.size - 1)
+
+    def count_rows(self, stride: int, phase: int = 0) -> int:
+        return int(lib.oracle_tc_count_rows(self._h, int(stride), int(phase)))
+
+    def close(self) -> None:
+        if self._h:
+            lib.oracle_tc_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def triangle_count_oriented(offsets, targets) -> int:
@@ -159,11 +184,20 @@ def truss_numbers(offsets, targets):
     return edges, truss[:count].copy()
 
 
+# Unicode's White_Space property, which Rust's str::trim and split_whitespace use (Python's
+# str.isspace also counts U+001C..001F, so it is not used here).
+RUST_WHITESPACE = ("\t\n\x0b\x0c\r \x85\xa0\u1680\u2000\u2001\u2002\u2003\u2004\u2005\u2006\u2007"
+                   "\u2008\u2009\u200a\u2028\u2029\u202f\u205f\u3000")
+_RUST_WS_RUN = re.compile("[" + re.escape(RUST_WHITESPACE) + "]+")
+
+
 def parse_tsv_to_edges(path: str):
     """parse_tsv_to_edges (crates/grafeo-cuda-core/src/csr_export.rs:246-287) in Python:
-    skip blank, '#' and '%' lines; the first two whitespace fields as u64 (Rust's
-    u64::from_str: optional '+', digits only, no overflow); ids remapped by first
-    appearance; lines that do not parse are skipped."""
+    lines split at '\n' as BufRead::lines does (a line that is not UTF-8 fails the whole
+    parse: UnicodeDecodeError here, an io::Error there); each trimmed of White_Space (str::trim);
+    skip blank, '#' and '%' lines; the first two White_Space-separated fields as u64 (Rust's
+    u64::from_str: optional '+', digits only, no overflow); ids remapped by first appearance;
+    lines that do not parse are skipped."""
 
     def parse_u64(s: str):
         t = s[1:] if s.startswith("+") else s
@@ -176,10 +210,10 @@ def parse_tsv_to_edges(path: str):
     edges = []
     with open(path, "rb") as f:
         for raw in f.read().split(b"\n"):
-            line = raw.decode("utf-8", errors="replace").strip()
+            line = raw.decode("utf-8").strip(RUST_WHITESPACE)
             if not line or line.startswith("#") or line.startswith("%"):
                 continue
-            parts = line.split()
+            parts = _RUST_WS_RUN.split(line)
             if len(parts) < 2:
                 continue
             s, d = parse_u64(parts[0]), parse_u64(parts[1])

@@ -151,6 +151,40 @@ def make_rmat(large: bool):
     return out
 
 
+def bench_sources(off, count: int, seed: int) -> list[int]:
+    """The BFS sources of bench.py's step: `count` distinct seeded random vertices of nonzero
+    degree (numpy PCG64 seeded with `seed`, choice without replacement)."""
+    deg = np.diff(off.astype(np.int64))
+    rng = np.random.default_rng(seed)
+    return [int(v) for v in rng.choice(np.flatnonzero(deg > 0), size=count, replace=False)]
+
+
+def make_rmat22_bfs():
+    """R-MAT 22 (the benchmark's graph): the oracle's distances from the step's 8 sources, by
+    SHA-256 of the int32 array, with reachable counts and depths."""
+    off, tgt = rmat_graph(22, 16, seed=1)
+    oracle.set_threads(os.cpu_count() or 1)
+    out = {"graph": "rmat22_ef16_seed1", "csr_sha256": csr_sha256(off, tgt), "source": "oracle_bfs",
+           "sources": {}}
+    for s in bench_sources(off, 8, 1):
+        dist, reach, depth = oracle.bfs(off, tgt, s)
+        out["sources"][str(s)] = {"reachable": reach, "max_depth": depth,
+                                  "sha256": hashlib.sha256(dist.astype("<i4").tobytes()).hexdigest()}
+        print("rmat22 bfs", s, reach, depth, flush=True)
+    return out
+
+
+def make_rmat24():
+    """R-MAT 24 (the scheduler's evaluation graph): its triangle count from the oracle's fast
+    variant and its CSR SHA-256."""
+    off, tgt = rmat_graph(24, 16, seed=1)
+    oracle.set_threads(os.cpu_count() or 1)
+    e = {"scale": 24, "edge_factor": 16, "seed": 1, "csr_sha256": csr_sha256(off, tgt), "num_edges": int(len(tgt)),
+         "triangles": oracle.triangle_count(off, tgt, fast=True), "source": "oracle (fast)"}
+    print("rmat24", e, flush=True)
+    return {"rmat24_ef16_seed1": e}
+
+
 def make_batch():
     """The 256-graph uneven batch (grafeo_cuda.workloads.uneven_batch, seed 1): per graph its
     size, CSR SHA-256 and triangle count from the C oracle's fast variant (checked against
@@ -170,7 +204,20 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--large", action="store_true", help="also R-MAT 16..22 (minutes)")
     ap.add_argument("--only-batch", action="store_true", help="only batch_counts.json")
+    ap.add_argument("--only-rmat22-bfs", action="store_true", help="only rmat22_bfs.json")
+    ap.add_argument("--only-rmat24", action="store_true", help="only R-MAT 24's entry of rmat_counts.json")
     args = ap.parse_args()
+    path = os.path.join(HERE, "rmat_counts.json")
+    if args.only_rmat22_bfs:
+        with open(os.path.join(HERE, "rmat22_bfs.json"), "w") as f:
+            json.dump(make_rmat22_bfs(), f, indent=1)
+        return
+    if args.only_rmat24:
+        existing = json.load(open(path)) if os.path.exists(path) else {}
+        existing.update(make_rmat24())
+        with open(path, "w") as f:
+            json.dump(existing, f, indent=1)
+        return
     with open(os.path.join(HERE, "batch_counts.json"), "w") as f:
         json.dump(make_batch(), f, indent=1)
     if args.only_batch:
@@ -179,7 +226,6 @@ def main():
         json.dump(make_catalog(), f, indent=1)
     with open(os.path.join(HERE, "er4096.json"), "w") as f:
         json.dump(make_er(), f, indent=1)
-    path = os.path.join(HERE, "rmat_counts.json")
     existing = json.load(open(path)) if os.path.exists(path) else {}
     existing.update(make_rmat(args.large))
     with open(path, "w") as f:

@@ -209,6 +209,64 @@ def test_tsv_parser_matches_the_python_restatement(tmp_path):
     assert [tuple(e) for e in edges.tolist()] == o_edges and nodes == o_nodes
 
 
+# Rust's parse on text with Unicode: fields separated and lines trimmed by the White_Space
+# code points (U+0085, U+00A0, U+1680, U+2000..200A, U+2028, U+2029, U+202F, U+205F, U+3000
+# besides ASCII's), not by U+001C..001F or U+200B (zero-width space, not White_Space), which
+# then belong to the field and make it fail to parse.
+UNICODE_LINES = [
+    ("1\u00a02", (1, 2)),
+    ("2\u20033", (2, 3)),
+    ("\u30003\u3000\u30004\u3000", (3, 4)),
+    ("4\u16805\u0085", (4, 5)),
+    ("5\u20286", (5, 6)),
+    ("6\u202f\u205f7", (6, 7)),
+    ("7\x1c8", None),
+    ("8\u200b9", None),
+    ("\u00a0# comment after a no-break space", None),
+    ("9 \u00e910", None),
+    ("10\u200a11", (10, 11)),
+]
+
+
+def unicode_file(tmp_path, name="unicode.tsv"):
+    p = tmp_path / name
+    p.write_bytes("\n".join(line for line, _ in UNICODE_LINES).encode("utf-8"))
+    ids, want = {}, []
+    for _, pair in UNICODE_LINES:
+        if pair:
+            want.append((ids.setdefault(pair[0], len(ids)), ids.setdefault(pair[1], len(ids))))
+    return p, want, len(ids)
+
+
+def test_tsv_parser_unicode_whitespace(tmp_path):
+    p, want, nodes = unicode_file(tmp_path)
+    edges, n = _native.parse_tsv_to_edges(str(p))
+    assert [tuple(e) for e in edges.tolist()] == want and n == nodes
+    assert oracle.parse_tsv_to_edges(str(p)) == (want, nodes)
+
+
+@pytest.mark.parametrize("bad", [b"\xff", b"\xc0\x80", b"\xed\xa0\x80", b"\xf4\x90\x80\x80", b"\xe2\x80",
+                                 b"\x80", b"\xf0\x8f\xbf\xbf"])
+def test_tsv_parser_rejects_invalid_utf8(tmp_path, bad):
+    """Rust's BufRead::lines fails on a line that is not UTF-8 (a lone continuation byte, an
+    overlong form, a surrogate, a code point above U+10FFFF, a truncated sequence), even in a
+    comment line, and the whole parse fails with it."""
+    p = tmp_path / "bad.tsv"
+    p.write_bytes(b"1 2\n# comment " + bad + b"\n3 4\n")
+    with pytest.raises(GcError) as info:
+        _native.parse_tsv_to_edges(str(p))
+    assert info.value.kind == "Export" and "valid UTF-8" in str(info.value)
+    with pytest.raises(UnicodeDecodeError):
+        oracle.parse_tsv_to_edges(str(p))
+
+
+def test_tsv_parser_accepts_every_valid_utf8_form(tmp_path):
+    p = tmp_path / "ok.tsv"
+    p.write_bytes("# \u00e9 \u20ac \U0001f600 \U0010ffff \ud7ff \ue000\n1 2\n".encode("utf-8"))
+    edges, n = _native.parse_tsv_to_edges(str(p))
+    assert (edges.tolist(), n) == ([[0, 1]], 2)
+
+
 # ---- generators ------------------------------------------------------------------
 
 

@@ -1,0 +1,124 @@
+"""The learned scheduler's evaluation graphs at full size on the GPU: R-MAT scale 24 (edge factor
+16, about 521M CSR entries) and the 4096 x 4096 4-neighbour grid (diameter 8190), the graphs
+the committed network was not trained on.
+
+  - R-MAT 24: the CSR built on the GPU equals the host builder's (the oracle's input, by
+    SHA-256); the triangle count equals the oracle's (tests/golden/rmat_counts.json) under the
+    fixed and the learned triangle-count schedules; from 4 seeded sources the distances pass
+    the size-independent BFS characterization and are identical under Beamer's rule, the
+    default fixed rule (Beamer with chained top-down levels) and the learned schedule.
+  - The grid: distances from the centre and a corner equal the Manhattan closed form under the
+    same three schedules.
+The reference has no test at this size (crates/grafeo-cuda-algos/tests/roundtrip.rs:186-222,
+2,000,002 nodes, is its largest BFS case).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from ctypes import c_void_p
+
+import numpy as np
+import pytest
+
+import grafeo_cuda
+from grafeo_cuda import _native, scheduler
+from grafeo_cuda._native import check, lib
+from graphs import csr_sha256
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rmat_counts.json")
+
+
+def host_csr(g):
+    n, m = g.num_nodes, g.num_edges
+    off = np.empty(n + 1, np.uint32)
+    tgt = np.empty(m, np.uint32)
+    check(lib.gc_device_copy_to_host(g._device._handle, off.ctypes.data_as(c_void_p),
+                                     c_void_p(lib.gc_graph_device_offsets(g._handle)), 4 * (n + 1)))
+    check(lib.gc_device_copy_to_host(g._device._handle, tgt.ctypes.data_as(c_void_p),
+                                     c_void_p(lib.gc_graph_device_targets(g._handle)), 4 * m))
+    return off, tgt
+
+
+def bfs_checker(off, tgt):
+    """bfs_is_valid of test_gpu_configs with the entries' endpoints computed once: d[source] = 0;
+    every entry u->v with d[u] >= 0 has 0 <= d[v] <= d[u] + 1; every v with d[v] > 0 has an
+    entry u->v with d[u] = d[v] - 1."""
+    n = len(off) - 1
+    src = np.repeat(np.arange(n, dtype=np.int32), np.diff(off.astype(np.int64)))
+
+    def valid(source, dist):
+        du, dv = dist[src], dist[tgt]
+        reached = du >= 0
+        if dist[source] != 0 or not np.all(dv[reached] >= 0) or not np.all(dv[reached] <= du[reached] + 1):
+            return False
+        has_parent = np.zeros(n, bool)
+        has_parent[tgt[reached & (dv == du + 1)]] = True
+        return bool(np.all(has_parent[dist > 0]))
+
+    return valid
+
+
+def test_rmat24_schedules(device, monkeypatch):
+    with open(GOLDEN) as f:
+        e = json.load(f)["rmat24_ef16_seed1"]
+    g = grafeo_cuda.GpuGraph.from_edge_pairs(_native.gen_rmat(24, 16, 1), 1 << 24, device)
+    off, tgt = host_csr(g)
+    assert csr_sha256(off, tgt) == e["csr_sha256"]
+    g.prepare()
+    bfs_s, tc_s = scheduler.load(device)
+    assert g.triangle_count()["total_triangles"] == e["triangles"]
+    g.set_scheduler(tc=tc_s)
+    assert g.triangle_count()["total_triangles"] == e["triangles"]
+    g.set_scheduler()
+    valid = bfs_checker(off, tgt)
+    deg = np.diff(off.astype(np.int64))
+    sources = [int(v) for v in np.random.default_rng(1).choice(np.flatnonzero(deg > 0), size=4, replace=False)]
+    for s in sources:
+        monkeypatch.delenv("GC_BFS_RULE", raising=False)
+        g.set_scheduler()
+        fixed = g.bfs_numpy(s)
+        assert valid(s, fixed["distances"]), s
+        assert fixed["reachable"] == int((fixed["distances"] >= 0).sum())
+        assert fixed["max_depth"] == int(fixed["distances"].max())
+        monkeypatch.setenv("GC_BFS_RULE", "beamer")
+        beamer = g.bfs_numpy(s)
+        monkeypatch.delenv("GC_BFS_RULE")
+        g.set_scheduler(bfs=bfs_s)
+        learned = g.bfs_numpy(s)
+        g.set_scheduler()
+        for other in (beamer, learned):
+            assert np.array_equal(other["distances"], fixed["distances"]), s
+            assert (other["reachable"], other["max_depth"]) == (fixed["reachable"], fixed["max_depth"])
+        print(f"R-MAT 24 BFS from {s}: reached {fixed['reachable']}, depth {fixed['max_depth']}; "
+              f"{fixed['time_seconds'] * 1e3:.2f} ms fixed, {beamer['time_seconds'] * 1e3:.2f} Beamer, "
+              f"{learned['time_seconds'] * 1e3:.2f} learned")
+
+
+def test_grid4096_schedules(device, monkeypatch):
+    side = 4096
+    n = side * side
+    g = grafeo_cuda.GpuGraph.from_edge_pairs(_native.gen_grid(side, side), n, device)
+    assert g.num_edges == 2 * 2 * side * (side - 1)
+    g.prepare()
+    bfs_s, _ = scheduler.load(device)
+    r, c = np.divmod(np.arange(n, dtype=np.int32), side)
+    for r0, c0 in ((side // 2, side // 2), (0, 0)):
+        s = r0 * side + c0
+        expected = np.abs(r - r0) + np.abs(c - c0)
+        times = {}
+        for label in ("fixed", "beamer", "learned"):
+            monkeypatch.delenv("GC_BFS_RULE", raising=False)
+            if label == "beamer":
+                monkeypatch.setenv("GC_BFS_RULE", "beamer")
+            g.set_scheduler(bfs=bfs_s if label == "learned" else None)
+            out = g.bfs_numpy(s)
+            assert np.array_equal(out["distances"], expected), (label, s)
+            assert out["reachable"] == n and out["max_depth"] == int(expected.max()), (label, s)
+            times[label] = out["time_seconds"]
+        g.set_scheduler()
+        print(f"grid {side}^2 BFS from ({r0}, {c0}): " + ", ".join(f"{k} {v * 1e3:.1f} ms" for k, v in times.items()))
+    monkeypatch.delenv("GC_BFS_RULE", raising=False)

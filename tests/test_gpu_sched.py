@@ -5,11 +5,14 @@
   - Any schedule gives the reference's results: every BFS level method and every triangle-count
     method is forced in turn (output biases that make one choice win), and schedulers with random
     weights pick arbitrary mixtures; results stay bit-exact against the oracle.
-  - A network trained on the GPU, on timings the kernels produce on small graphs, beats the fixed
-    rule on a high-diameter grid it was not trained on, with identical distances.
+  - A network trained on the GPU, on timings the kernels produce on small graphs, beats Beamer's
+    rule on a high-diameter grid it was not trained on, with identical distances, and stays
+    within 1.5x of the hand rule that chains small frontiers (the default fixed rule).
 """
 
 from __future__ import annotations
+
+import os
 
 import numpy as np
 import pytest
@@ -68,7 +71,8 @@ def forced(device, choice):
     """A scheduler whose outputs make `choice` the cheapest wherever it is allowed."""
     s = grafeo_cuda.GpuScheduler(device, seed=1)
     p = np.zeros(1571, np.float32)
-    b3 = np.ones(3, np.float32)
+    # ln-microsecond outputs 10 apart: far above the triangle count's 20 us switching margin
+    b3 = np.full(3, 10.0, np.float32)
     b3[choice] = 0.0
     p[-3:] = b3
     s.set(params=p)
@@ -123,15 +127,22 @@ def test_random_networks_are_bit_exact(device):
             check_graph(f"{name} (random net {seed})", off, tgt, g, [0, n - 1])
 
 
-def test_detaching_restores_the_fixed_rule(device):
+def test_detaching_restores_the_fixed_rule(device, monkeypatch):
     off, tgt = rmat_graph(12, 16, seed=3)
     g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+    fresh = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
+    fresh.bfs_numpy(0)
     g.set_scheduler(bfs=forced(device, 2))
     g.bfs_numpy(0)
     assert 2 in g.last_schedule()
     g.set_scheduler()
     g.bfs_numpy(0)
+    assert g.last_schedule() == fresh.last_schedule()
+    # Beamer's rule alone never chains
+    monkeypatch.setenv("GC_BFS_RULE", "beamer")
+    g.bfs_numpy(0)
     assert 2 not in g.last_schedule()
+    monkeypatch.delenv("GC_BFS_RULE")
 
 
 def collect(device, graphs_and_sources):
@@ -171,13 +182,27 @@ def test_trained_network_beats_the_fixed_rule_on_a_grid(device):
     off, tgt = grid_graph(1024, 1024)
     g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
     source = 1024 * 512 + 512
-    fixed = g.bfs_numpy(source)
-    fixed_t = min(g.bfs_numpy(source)["time_seconds"] for _ in range(2))
+
+    def timed():
+        first = g.bfs_numpy(source)
+        return first, min(g.bfs_numpy(source)["time_seconds"] for _ in range(3))
+
+    os.environ["GC_BFS_RULE"] = "beamer"
+    try:
+        beamer, beamer_t = timed()
+    finally:
+        del os.environ["GC_BFS_RULE"]
+    chain, chain_t = timed()  # the default fixed rule: Beamer with chained top-down levels
     g.set_scheduler(bfs=s)
-    learned = g.bfs_numpy(source)
-    learned_t = min(g.bfs_numpy(source)["time_seconds"] for _ in range(2))
-    assert np.array_equal(learned["distances"], fixed["distances"])
+    learned, learned_t = timed()
+    assert np.array_equal(learned["distances"], beamer["distances"])
+    assert np.array_equal(chain["distances"], beamer["distances"])
     r, c = np.divmod(np.arange(1024 * 1024), 1024)
-    assert np.array_equal(fixed["distances"], np.abs(r - 512) + np.abs(c - 512))
-    print(f"grid 1024x1024 BFS: fixed {fixed_t * 1e3:.1f} ms, learned {learned_t * 1e3:.1f} ms")
-    assert learned_t < 0.8 * fixed_t, (fixed_t, learned_t)
+    assert np.array_equal(beamer["distances"], np.abs(r - 512) + np.abs(c - 512))
+    print(f"grid 1024x1024 BFS: Beamer {beamer_t * 1e3:.1f} ms, Beamer + chain {chain_t * 1e3:.1f} ms, "
+          f"learned {learned_t * 1e3:.1f} ms")
+    # Trained on timings alone, the network finds the chained levels on a graph it never saw
+    # (Beamer's rule has no such choice); the hand rule that chains small frontiers is the bar
+    # it is compared with in the benchmark, where it does not beat it.
+    assert learned_t < 0.8 * beamer_t, (beamer_t, learned_t)
+    assert learned_t < 1.5 * chain_t, (chain_t, learned_t)

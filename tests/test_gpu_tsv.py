@@ -2,7 +2,8 @@
 pairs and node count as the host parser (parse_tsv_to_edges,
 crates/grafeo-cuda-core/src/csr_export.rs:246-287, itself pinned by the reference's own parse
 cases in test_csr.py) on edge cases, random files and a full-size file, and the same error for a
-missing file."""
+missing file; the same White_Space separators and the same error for a file that is not
+UTF-8, as Rust's lines, trim and split_whitespace give the reference."""
 
 from __future__ import annotations
 
@@ -105,3 +106,45 @@ def test_missing_file_is_an_export_error(device, tmp_path):
     with pytest.raises(_native.GcError) as dev_err:
         _native.parse_tsv_to_edges_device(device._handle, path)
     assert host_err.value.kind == dev_err.value.kind == "Export"
+
+
+UNICODE_SEPARATORS = ["\u00a0", "\u2003", "\u3000", "\u1680", "\u2000", "\u200a", "\u2028", "\u205f",
+                      "\u0085", " \u202f\t"]
+NOT_SEPARATORS = ["\x1c", "\u200b", "\u00e9"]
+
+
+def test_unicode_whitespace(device, tmp_path):
+    """White_Space code points separate fields and are trimmed (Rust's trim and
+    split_whitespace), in random lines with every separator, some spanning 64-byte chunks;
+    other code points stay inside their field, which then fails to parse. The device parser
+    gives the host parser's pairs, which the CPU tests check against the restatement."""
+    rng = np.random.default_rng(31)
+    for t in range(6):
+        lines = []
+        for k in range(2000):
+            a, b = (int(x) for x in rng.integers(0, 400, size=2))
+            sep = UNICODE_SEPARATORS[int(rng.integers(0, len(UNICODE_SEPARATORS)))]
+            lead = UNICODE_SEPARATORS[int(rng.integers(0, len(UNICODE_SEPARATORS)))] * int(rng.integers(0, 3))
+            if k % 17 == 0:
+                sep = NOT_SEPARATORS[k % len(NOT_SEPARATORS)]
+            if k % 101 == 0:
+                sep = sep * 40
+            lines.append(f"{lead}{a}{sep}{b}{lead}")
+        path = tmp_path / f"u{t}.tsv"
+        path.write_bytes("\n".join(lines).encode("utf-8"))
+        pairs, n = same(device, path)
+        assert len(pairs) > 1000
+
+
+@pytest.mark.parametrize("bad", [b"\xff", b"\xc0\x80", b"\xed\xa0\x80", b"\xf4\x90\x80\x80", b"\xe2\x80", b"\x80"])
+def test_invalid_utf8_is_an_export_error(device, tmp_path, bad):
+    path = tmp_path / "bad.tsv"
+    path.write_bytes(b"1 2\n" + b"x" * 100 + b" # " + bad + b"\n3 4\n")
+    with pytest.raises(_native.GcError) as host_err:
+        _native.parse_tsv_to_edges(str(path))
+    with pytest.raises(_native.GcError) as dev_err:
+        _native.parse_tsv_to_edges_device(device._handle, str(path))
+    assert host_err.value.kind == dev_err.value.kind == "Export"
+    assert str(host_err.value) == str(dev_err.value)
+    with pytest.raises(RuntimeError, match="valid UTF-8"):  # the Python surface's error type
+        grafeo_cuda.GpuGraph.from_tsv(str(path), device)

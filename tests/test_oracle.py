@@ -342,3 +342,42 @@ def test_truss_numbers_rmat12_against_fast_k_truss():
 def test_truss_numbers_reject_directed_csr():
     with pytest.raises(ValueError):
         oracle.truss_numbers([0, 1, 1], [1])
+
+
+def test_pagerank_first_iterates_by_hand_with_a_dangling_vertex():
+    """A star 0-{1,2,3} plus an isolated vertex 4 (degree 0, dangling), d = 0.85, worked by hand
+    from r0 = 1/5: D = r0[4] = 0.2, so every vertex gets (1-d)/5 + d*D/5 = 0.064, plus
+    d * 3 * 0.2 / 1 = 0.51 at the centre and d * 0.2 / 3 at each leaf. The second iterate takes
+    D = r1[4] = 0.064, the dangling mass of the input iterate (taking it from the output iterate
+    would give another second iterate). L1 changes 0.748 and 0.49708 decide convergence."""
+    off, tgt = undirected(5, [(0, 1), (0, 2), (0, 3)])
+    r1, it, conv = oracle.pagerank(off, tgt, 0.85, 1, 0.0)
+    assert it == 1 and not conv
+    assert np.allclose(r1, [0.574, 0.03 + 6.8 / 75, 0.03 + 6.8 / 75, 0.03 + 6.8 / 75, 0.064], rtol=0, atol=1e-15)
+    r2, it, conv = oracle.pagerank(off, tgt, 0.85, 2, 0.0)
+    assert it == 2 and not conv
+    leaf2 = 0.03 + 0.85 * (0.574 / 3 + 0.0128)
+    assert np.allclose(r2, [0.34858, leaf2, leaf2, leaf2, 0.04088], rtol=0, atol=1e-14)
+    assert abs(leaf2 - 0.20351333333333333) < 1e-15
+    # L1 change of the first iterate 0.748, of the second 0.49708
+    r, it, conv = oracle.pagerank(off, tgt, 0.85, 10, 0.75)
+    assert (it, conv) == (1, True) and np.allclose(r, r1, rtol=0, atol=1e-15)
+    r, it, conv = oracle.pagerank(off, tgt, 0.85, 10, 0.5)
+    assert (it, conv) == (2, True) and np.allclose(r, r2, rtol=0, atol=1e-14)
+    # third change 0.4146572: a tolerance between the second and third changes stops at 3
+    r, it, conv = oracle.pagerank(off, tgt, 0.85, 10, 0.497)
+    assert (it, conv) == (3, True)
+
+
+def test_triangle_sample_phases_add_up_to_the_full_count():
+    """The CPU baseline's sampled count: the phases of one stride partition the DAG rows."""
+    for name, off, tgt in [("ER-4096", *er_graph(4096, 16, 1)), ("RMAT-12", *rmat_graph(12, 16, 1))]:
+        full = oracle.triangle_count(off, tgt, fast=True)
+        sample = oracle.TriangleSample(off, tgt)
+        try:
+            assert sample.count_rows(1) == full, name
+            for stride in (2, 7, 64):
+                assert sum(sample.count_rows(stride, p) for p in range(stride)) == full, (name, stride)
+        finally:
+            sample.close()
+    assert full == oracle.triangle_count(off, tgt)
