@@ -49,8 +49,8 @@ METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
 KERNEL_FAMILIES = {
     "tc": ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count"),
-    "pagerank": ("pr_spmv_push", "pr_push_finish", "pr_spmv_hot", "pr_spmv", "pr_hub_finish", "pr_finalize",
-                 "pr_init", "pr_push_split"),
+    "pagerank": ("pr_spmv_merge", "pr_merge_finish", "pr_spmv_push", "pr_push_finish", "pr_spmv_hot", "pr_spmv",
+                 "pr_hub_finish", "pr_finalize", "pr_init", "pr_push_split", "pr_merge_starts", "pr_merge_chunks"),
     "bfs": ("bfs_init", "bfs_topdown_edges", "bfs_frontier_degrees", "bfs_frontier_scan_small", "bfs_bottomup",
             "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_publish", "bfs_chain_expand", "bfs_chain_advance",
             "bfs_top_build", "bfs_chain"),
@@ -66,8 +66,8 @@ KERNEL_FAMILIES = {
 KERNELS = tuple(k for ks in KERNEL_FAMILIES.values() for k in ks)
 # The kernels timed with events in a default run: those the rooflines report. Each timed launch
 # adds two event records to the stream (about 2 ms per step if every launch were timed).
-ROOFLINE_KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_push", "pr_spmv_hot",
-                    "pr_spmv")
+ROOFLINE_KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_merge", "pr_spmv_push",
+                    "pr_spmv_hot", "pr_spmv")
 # two-sided 95% quantiles of Student's t by degrees of freedom
 T975 = {1: 12.706, 2: 4.303, 3: 3.182, 4: 2.776, 5: 2.571, 6: 2.447, 7: 2.365, 8: 2.306, 9: 2.262, 10: 2.228}
 
@@ -308,7 +308,7 @@ def main() -> int:
     rooflines = {}
     # PageRank launches in batches and the launches after convergence return at once: its
     # SpMV time per iteration is the kernel's time per step over the iterations that ran.
-    for name in ("pr_spmv_push", "pr_spmv_hot", "pr_spmv"):
+    for name in ("pr_spmv_merge", "pr_spmv_push", "pr_spmv_hot", "pr_spmv"):
         if name in kernels:
             per_iter_us = 1000.0 * kernels[name]["total_ms"] / max(1, iters)
             t = traffic.get(name)
@@ -317,8 +317,8 @@ def main() -> int:
                 spmv_bytes, per_iter_us, t,
                 "bytes_per_launch = 4(n+1) + 4m + 8n (offsets, targets, x read once, y written once); "
                 "avg_launch_us is per iteration that ran (launches after convergence return at once)"
-                + ("; the push-pull kernel reads the hubs' lists only inside the push set (77% of the "
-                   "target stream at R-MAT 22)" if name == "pr_spmv_push" else ""))
+                + ("; the push-pull kernels read the hubs' lists only inside the push set (79% of the "
+                   "target stream at R-MAT 22)" if name in ("pr_spmv_merge", "pr_spmv_push") else ""))
     for name in ("tc_hash", "tc_count"):
         if name in kernels:
             rooflines[name] = roofline_entry(tc_merge_bytes, kernels[name]["avg_us"], traffic.get(name),
@@ -887,9 +887,10 @@ def pagerank_precision(g, offsets, targets, dev) -> dict:
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    dev.profile(True, only=["pr_spmv_push", "pr_spmv_hot", "pr_spmv"])
+    spmv_kernels = ("pr_spmv_merge", "pr_spmv_push", "pr_spmv_hot", "pr_spmv")
+    dev.profile(True, only=list(spmv_kernels))
     r = g.pagerank_numpy(0.85, 50, 1e-7)
-    name = next(k for k in ("pr_spmv_push", "pr_spmv_hot", "pr_spmv") if dev.kernel_time(k)[1])
+    name = next(k for k in spmv_kernels if dev.kernel_time(k)[1])
     ms, launches = dev.kernel_time(name)
     dev.profile(False)
     _, it, conv = oracle.pagerank(offsets, targets, 0.85, 50, 1e-7)

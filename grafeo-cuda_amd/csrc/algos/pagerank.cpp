@@ -115,21 +115,29 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
 
 }  // namespace
 
-// Which SpMV runs (GC_PR_KERNEL): on a symmetric graph the push-pull kernel
-// (default, pr_spmv_push); otherwise, or with GC_PR_KERNEL=hot, the persistent
-// pull kernel with the hot contributions in LDS (pr_spmv_hot); with
-// GC_PR_KERNEL=rows one workgroup per row tile (pr_spmv). Measured on MI355X,
-// 17 iterations at R-MAT scale 22: hot 11.3 ms against rows 14.0 ms; scale
-// 20: 3.0 against 3.8 ms. (A source-partitioned pass, one part of the sources
-// per XCD with eight FP32 partial sums per row, halved the L2 requests but was
-// no faster: 12.1 against 10.2-10.5 ms.)
-enum class PrKernel { kPush, kHot, kRows };
+// Which SpMV runs (GC_PR_KERNEL): on a symmetric graph the push-pull
+// iterations, by default with the nonzero-split pull over the rows below the
+// push set (pr_spmv_merge, pr_merge_finish), with GC_PR_KERNEL=push with row
+// tiles of G lanes per row (pr_spmv_push, pr_push_finish); otherwise, or with
+// GC_PR_KERNEL=hot, the persistent pull kernel with the hot contributions in
+// LDS (pr_spmv_hot); with GC_PR_KERNEL=rows one workgroup per row tile
+// (pr_spmv). Measured on MI355X at R-MAT scale 22, per iteration that ran:
+// merge 380 us, push 401 us (both with the default sets), hot 527 us. The
+// row tiles fill 70% of their lane slots there (a row of degree d on G lanes
+// takes ceil(d / 8G) steps of 8G slots); the chunks of 512 entries fill all.
+// (Earlier, 17 iterations at scale 22: hot 11.3 ms against rows 14.0 ms; a
+// source-partitioned pass, one part of the sources per XCD with eight FP32
+// partial sums per row, halved the L2 requests but was no faster: 12.1
+// against 10.2-10.5 ms.)
+enum class PrKernel { kPush, kMerge, kHot, kRows };
 
 static PrKernel choose_kernel(const Graph& g) {
   const char* env = std::getenv("GC_PR_KERNEL");
   if (env != nullptr && std::strcmp(env, "rows") == 0) return PrKernel::kRows;
   if (env != nullptr && std::strcmp(env, "hot") == 0) return PrKernel::kHot;
-  return g.symmetric ? PrKernel::kPush : PrKernel::kHot;
+  if (!g.symmetric) return PrKernel::kHot;
+  if (env != nullptr && std::strcmp(env, "push") == 0) return PrKernel::kPush;
+  return PrKernel::kMerge;
 }
 
 static uint32_t env_u32(const char* name, uint32_t fallback) {
@@ -137,16 +145,18 @@ static uint32_t env_u32(const char* name, uint32_t fallback) {
   return env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : fallback;
 }
 
-// Push-pull iterations on a symmetric graph (pr_spmv_push, pr_push_finish,
-// pr_finalize; the kernels' comments give the method). One 1024-lane
-// workgroup per CU holds push_count 8-byte accumulators and hot_count FP32
-// contributions in LDS: 8,192 and 22,528 (152 KB of the 160 KB). At R-MAT
-// scale 22 that leaves 36% of the entries to gather from global memory
-// (58% for the pull kernel's 19,456 hot ids) and skips 23% of the target
-// stream (the hubs' entries outside the push set). GC_PR_PUSH and GC_PR_HOT
+// Push-pull iterations on a symmetric graph (pr_spmv_merge and
+// pr_merge_finish, or pr_spmv_push and pr_push_finish; then pr_finalize; the
+// kernels' comments give the method). One 1024-lane workgroup per CU holds
+// push_count 8-byte accumulators and hot_count FP32 contributions in LDS:
+// 6,144 and 26,624 (152 KB of the 160 KB). At R-MAT scale 22 that leaves
+// 36.5% of the entries to gather from global memory (58% for the pull
+// kernel's 19,456 hot ids) and skips 21% of the target stream (the hubs'
+// entries outside the push set). 8,192 and 22,528 measured 385 us per
+// iteration (merge) and 423 us (push) there. GC_PR_PUSH and GC_PR_HOT
 // override the two sizes.
 void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float tolerance, float* d_out,
-                   uint32_t* iterations, int32_t* converged) {
+                   uint32_t* iterations, int32_t* converged, bool merge) {
   Device& dev = *g.dev;
   const uint32_t n = g.n;
   const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
@@ -154,21 +164,29 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   const uint32_t wgs = std::min(2u, std::max(1u, env_u32("GC_PR_WGS", 1u)));
   // lane groups take this many rows of a tile each, their loads in flight together (1 or 2,
   // GC_PR_ROWS; pr_spmv_push's tiles have this many slices). Measured at R-MAT scale 22,
-  // one box: 357 us per iteration with 1 row, 382 with 2.
+  // one box: 357 us per launch with 1 row (420 per iteration that ran), 382 with 2.
   const uint32_t rows_per_group = env_u32("GC_PR_ROWS", 1u) >= 2 ? 2u : 1u;
   const size_t lds_cap = std::min<size_t>(dev.max_lds_per_block, 160 * 1024) / wgs - 1024;  // static LDS
   // at most 1/3 of the LDS for the accumulators, the rest for the contributions
   const uint32_t push_cap = static_cast<uint32_t>(lds_cap / 3 / 8);
-  const uint32_t push_count = std::min({n, std::max(1u, env_u32("GC_PR_PUSH", 8192u)), push_cap});
+  const uint32_t push_count = std::min({n, std::max(1u, env_u32("GC_PR_PUSH", 6144u)), push_cap});
   const uint32_t hot_cap = static_cast<uint32_t>((lds_cap - 8ull * push_count) / 4);
-  const uint32_t hot_count = std::min({n, std::max(push_count, env_u32("GC_PR_HOT", 22528u)), hot_cap});
+  const uint32_t hot_count = std::min({n, std::max(push_count, env_u32("GC_PR_HOT", 26624u)), hot_cap});
   const uint32_t push_base = n - push_count;
   const uint32_t shmem = static_cast<uint32_t>(8ull * push_count + 4ull * hot_count);
   const PrSegments segs = plan_segments(g.rl.host_offsets, push_base, kPrHotThreads, false,
                                         env_u32("GC_PR_SHIFT", 2u), rows_per_group);
   const uint32_t push_tiles = (push_count + 15) / 16;  // 16 hub rows (one wave each) per tile
   const uint32_t grid = cu * wgs;
-  const uint32_t finish_blocks = (push_count + 15) / 16;
+  const uint32_t hub_blocks = (push_count + 15) / 16;
+  // the nonzero-split pull: rows [zero_rows, push_base) own the entries [0, entries)
+  const uint32_t zero_rows = std::min(g.rl.dangling, push_base);
+  const uint32_t entries = merge ? g.rl.host_offsets[push_base] : 0u;
+  if (merge && g.rl.host_offsets[zero_rows] != 0)
+    fail_kernel("PageRank: the degree-ordered rows do not start with the rows of degree 0");
+  const uint32_t chunks = static_cast<uint32_t>((static_cast<uint64_t>(entries) + 511) / 512);
+  const uint32_t fix_blocks = (chunks + 255) / 256;
+  const uint32_t finish_blocks = hub_blocks + fix_blocks;
   const uint32_t partial_blocks = grid + finish_blocks;
   float* rank[2] = {g.buffer<float>("pr.rank0", sizeof(float) * n),
                     g.buffer<float>("pr.rank1", sizeof(float) * n)};
@@ -181,6 +199,20 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   auto* push_partials =
       g.buffer<unsigned long long>("pr.push_partials", sizeof(unsigned long long) * grid * push_count);
   fill_words(dev, work, 4, 0);
+  uint32_t* starts = nullptr;
+  uint32_t* chunk_row = nullptr;
+  double* head_partial = nullptr;
+  double* tail_partial = nullptr;
+  uint32_t* tail_row = nullptr;
+  if (merge) {
+    const uint64_t words = std::max<uint64_t>(1, (static_cast<uint64_t>(entries) + 31) / 32);
+    starts = g.buffer<uint32_t>("pr.starts", sizeof(uint32_t) * words);
+    const size_t nc = std::max<uint32_t>(1, chunks);
+    chunk_row = g.buffer<uint32_t>("pr.chunk_row", sizeof(uint32_t) * nc);
+    head_partial = g.buffer<double>("pr.head_partial", sizeof(double) * nc);
+    tail_partial = g.buffer<double>("pr.tail_partial", sizeof(double) * nc);
+    tail_row = g.buffer<uint32_t>("pr.tail_row", sizeof(uint32_t) * nc);
+  }
 
   const float r0 = 1.0f / static_cast<float>(n);
   const uint32_t* off = g.rl.offsets.as<const uint32_t>();
@@ -190,26 +222,53 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
          contrib[0]);
   launch(dev, get_kernel(dev, "pagerank", "pr_push_split"), (push_count + 255) / 256, 256u, 0, off, tgt, n,
          push_count, psplit);
+  if (merge && chunks > 0) {
+    fill_words(dev, starts, (static_cast<uint64_t>(entries) + 31) / 32, 0);
+    Grid1D sgrid = thread_per_item(push_base - zero_rows);
+    launch(dev, get_kernel(dev, "pagerank", "pr_merge_starts"), sgrid.blocks, sgrid.threads, 0, off, zero_rows,
+           push_base, starts);
+    Grid1D cgrid = thread_per_item(chunks);
+    launch(dev, get_kernel(dev, "pagerank", "pr_merge_chunks"), cgrid.blocks, cgrid.threads, 0, off, zero_rows,
+           push_base, chunks, chunk_row);
+  }
   PrState init{static_cast<double>(g.rl.dangling) * static_cast<double>(r0), 0.0, 0, 0};
   copy_to_device(dev, state, &init, sizeof(init));
 
-  Kernel spmv = get_kernel(dev, "pagerank", "pr_spmv_push");
-  Kernel finish = get_kernel(dev, "pagerank", "pr_push_finish");
+  Kernel spmv = get_kernel(dev, "pagerank", merge ? "pr_spmv_merge" : "pr_spmv_push");
+  Kernel finish = get_kernel(dev, "pagerank", merge ? "pr_merge_finish" : "pr_push_finish");
   Kernel finalize = get_kernel(dev, "pagerank", "pr_finalize");
   const double d = static_cast<double>(damping);
   const double tol = static_cast<double>(tolerance);
   PrState host_state = init;
   for (uint32_t it = 0; it < max_iterations; ++it) {
     const int in = it & 1, out = in ^ 1;
-    hipError_t err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt,
-                                static_cast<const float*>(contrib[in]), static_cast<const float*>(rank[in]),
-                                contrib[out], rank[out], n, d, segs, static_cast<const PrState*>(state),
-                                hot_count, push_count, static_cast<const uint32_t*>(psplit), push_tiles,
-                                rows_per_group, push_partials, partials);
-    if (err != hipSuccess) fail_kernel(std::string("pr_spmv_push launch failed: ") + hipGetErrorString(err));
-    launch(dev, finish, finish_blocks, 256u, 0, off, static_cast<const float*>(rank[in]), contrib[out], rank[out],
-           n, d, static_cast<const PrState*>(state), push_count,
-           static_cast<const unsigned long long*>(push_partials), grid, partials, grid);
+    hipError_t err;
+    if (merge) {
+      err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, static_cast<const float*>(contrib[in]),
+                       static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d,
+                       static_cast<const PrState*>(state), hot_count, push_count,
+                       static_cast<const uint32_t*>(psplit), entries, chunks, zero_rows,
+                       reinterpret_cast<const uint8_t*>(starts), static_cast<const uint32_t*>(chunk_row),
+                       head_partial, tail_partial, tail_row, push_partials, partials);
+    } else {
+      err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, static_cast<const float*>(contrib[in]),
+                       static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d, segs,
+                       static_cast<const PrState*>(state), hot_count, push_count,
+                       static_cast<const uint32_t*>(psplit), push_tiles, rows_per_group, push_partials, partials);
+    }
+    if (err != hipSuccess)
+      fail_kernel(std::string(merge ? "pr_spmv_merge" : "pr_spmv_push") + " launch failed: " + hipGetErrorString(err));
+    if (merge) {
+      launch(dev, finish, finish_blocks, 256u, 0, off, static_cast<const float*>(rank[in]), contrib[out],
+             rank[out], n, d, static_cast<const PrState*>(state), push_count,
+             static_cast<const unsigned long long*>(push_partials), grid, hub_blocks, chunks,
+             static_cast<const double*>(head_partial), static_cast<const double*>(tail_partial),
+             static_cast<const uint32_t*>(tail_row), partials, grid);
+    } else {
+      launch(dev, finish, finish_blocks, 256u, 0, off, static_cast<const float*>(rank[in]), contrib[out],
+             rank[out], n, d, static_cast<const PrState*>(state), push_count,
+             static_cast<const unsigned long long*>(push_partials), grid, partials, grid);
+    }
     launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials), partial_blocks, tol, work);
     if ((it + 1) % kPrBatch == 0 || it + 1 == max_iterations) {
       copy_to_host(dev, dev.pinned, state, sizeof(PrState));
@@ -233,8 +292,8 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   prepare(g);
   const uint32_t n = g.n;
   const PrKernel kind = choose_kernel(g);
-  if (kind == PrKernel::kPush) {
-    pagerank_push(g, damping, max_iterations, tolerance, d_out, iterations, converged);
+  if (kind == PrKernel::kPush || kind == PrKernel::kMerge) {
+    pagerank_push(g, damping, max_iterations, tolerance, d_out, iterations, converged, kind == PrKernel::kMerge);
     return;
   }
   const bool rows_kernel = kind == PrKernel::kRows;

@@ -795,3 +795,333 @@ extern "C" __global__ __launch_bounds__(256) void pr_push_finish(
     partials[2 * slot + 1] = d1;
   }
 }
+
+// ---- push-pull with a nonzero-split pull (GC_PR_KERNEL=merge) --------------
+//
+// Same push set, LDS sets and hub rows as pr_spmv_push; only the pull over the
+// rows below P is cut differently. Rows are in ascending degree order, so the
+// rows of degree 0 come first and own no entry, and the rows [z, push_base)
+// own the one run of entries [0, E), every row at least one. That run is cut
+// into chunks of PRM_CHUNK = 512 entries: one wave per chunk and eight
+// consecutive entries per lane (two 16-byte target loads), so every lane has
+// eight useful gathers in flight whatever the row lengths (row tiles of G
+// lanes per row fill 70% of their lane slots at R-MAT scale 22).
+//
+// A per-graph bitmap marks the entries that start a row (bit e set iff
+// e = offsets[r]); chunk_row[c] is the row of the chunk's first entry minus
+// one if that entry starts a row, so the row of entry e of the chunk is
+// chunk_row[c] + (starts in [c * 512, e]). A lane sums each row's part of its
+// eight entries (in reverse entry order); a segmented suffix scan across the
+// wave adds the parts of the lanes that follow up to the next start, so a
+// row that starts in the chunk and ends in it is finished by the lane that
+// holds its start. The part before the chunk's first start (head) and the
+// chunk's last row if it may run on (tail) go to head_partial, tail_partial
+// and tail_row (PRM_NONE: the chunk holds no start); pr_merge_finish joins
+// them in chunk order. Every sum has a fixed order, so the result is
+// deterministic.
+
+#define PRM_CHUNK 512
+#define PRM_NONE 0xFFFFFFFFu
+
+// starts (u32 words, zeroed): bit offsets[r] for every row r in [first, last).
+extern "C" __global__ __launch_bounds__(256) void pr_merge_starts(const uint32_t* __restrict__ offsets,
+                                                                  uint32_t first, uint32_t last,
+                                                                  uint32_t* __restrict__ starts) {
+  const uint64_t r = static_cast<uint64_t>(first) + global_thread();
+  if (r >= last) return;
+  const uint32_t e = offsets[r];
+  atomicOr(starts + (e >> 5), 1u << (e & 31));
+}
+
+// chunk_row[c] for the chunks of [0, offsets[last]): the largest row r in
+// [first, last) with offsets[r] <= c * 512, minus one if offsets[r] == c * 512
+// (unsigned: first - 1 wraps when first = 0, and the popcounts wrap it back).
+extern "C" __global__ __launch_bounds__(256) void pr_merge_chunks(const uint32_t* __restrict__ offsets,
+                                                                  uint32_t first, uint32_t last,
+                                                                  uint32_t chunks,
+                                                                  uint32_t* __restrict__ chunk_row) {
+  const uint64_t c = global_thread();
+  if (c >= chunks) return;
+  const uint32_t e0 = static_cast<uint32_t>(c) * PRM_CHUNK;
+  uint32_t lo = first, hi = last - 1;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+    if (offsets[mid] <= e0) lo = mid;
+    else hi = mid - 1;
+  }
+  chunk_row[c] = lo - (offsets[lo] == e0 ? 1u : 0u);
+}
+
+typedef uint32_t prm_u32x4 __attribute__((ext_vector_type(4)));
+
+// One chunk on one wave (every lane calls it: it shuffles).
+__device__ __forceinline__ void prm_chunk(uint32_t c, uint32_t entries, const uint32_t* __restrict__ targets,
+                                          const uint8_t* __restrict__ starts, const uint32_t* __restrict__ chunk_row,
+                                          const float* __restrict__ contrib_in, const float* __restrict__ s_hot,
+                                          uint32_t hot_base, unsigned long long* __restrict__ s_push,
+                                          uint32_t push_base, const float* __restrict__ rank_in,
+                                          float* __restrict__ contrib_out, float* __restrict__ rank_out,
+                                          double base, double damping, double& diff,
+                                          double* __restrict__ head_partial, double* __restrict__ tail_partial,
+                                          uint32_t* __restrict__ tail_row) {
+  const uint32_t lane = lane_id();
+  const uint32_t e = c * PRM_CHUNK + lane * 8;
+  const uint32_t nv = e < entries ? min(entries - e, 8u) : 0u;
+  uint32_t t[8];
+  if (nv == 8) {
+    const prm_u32x4* p = reinterpret_cast<const prm_u32x4*>(targets + e);
+    const prm_u32x4 a = __builtin_nontemporal_load(p);
+    const prm_u32x4 b = __builtin_nontemporal_load(p + 1);
+    t[0] = a.x; t[1] = a.y; t[2] = a.z; t[3] = a.w;
+    t[4] = b.x; t[5] = b.y; t[6] = b.z; t[7] = b.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = static_cast<uint32_t>(k) < nv ? targets[e + k] : hot_base;
+  }
+  uint32_t bits = nv ? static_cast<uint32_t>(starts[e >> 3]) : 0u;
+  bits &= (1u << nv) - 1u;
+  float x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = t[k] >= hot_base ? s_hot[t[k] - hot_base] : contrib_in[t[k]];
+  // rows of the lane's entries: rbase + starts in [e, e + k]; the exclusive
+  // prefix of the lanes' start counts (0..8) from four bit-slice ballots
+  const uint32_t pc = __popc(bits);
+  uint32_t below = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) below += lanes_below(__ballot((pc >> b) & 1u)) << b;
+  const uint32_t rbase = chunk_row[c] + below;
+  // each row's own contribution for its pushes (loaded at entry 0 and at every
+  // start) and the input rank of every row that starts here, all in flight together
+  float cst[8], rin[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t rk = rbase + __popc(bits & ((2u << k) - 1u));
+    const bool start = (bits >> k) & 1u;
+    cst[k] = (k == 0 || start) ? contrib_in[rk] : 0.0f;
+    rin[k] = start ? rank_in[rk] : 0.0f;
+  }
+  float cur = cst[0];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((bits >> k) & 1u) cur = cst[k];
+    if (static_cast<uint32_t>(k) < nv && t[k] >= push_base) atomicAdd(s_push + (t[k] - push_base), pr_to_fixed(cur));
+  }
+  // the parts of the lane's rows, last entry first: seg[k] / len[k] = sum and
+  // count of entries k .. (next start or the lane's end) - 1
+  double seg[8];
+  uint32_t len[8];
+  double acc = 0.0;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int k = 7; k >= 0; --k) {
+    if (static_cast<uint32_t>(k) < nv) {
+      acc += static_cast<double>(x[k]);
+      ++cnt;
+    }
+    seg[k] = acc;
+    len[k] = cnt;
+    if ((bits >> k) & 1u) {
+      acc = 0.0;
+      cnt = 0;
+    }
+  }
+  // segmented suffix scan over the lanes: C = sum of the heads (entries before
+  // a lane's first start) from this lane up to and including the first lane at
+  // or after it that holds a start; which lanes hold one comes from a ballot
+  const uint64_t smask = __ballot(bits != 0u);
+  const uint64_t from_me = smask >> lane;  // bit i: lane + i holds a start
+  double C = acc;
+#pragma unroll
+  for (int o = 1; o < GC_WAVE; o <<= 1) {
+    const double oc = __shfl_down(C, o);
+    const uint64_t window = from_me & ((1ull << o) - 1ull);  // a start in [lane, lane + o)
+    if (window == 0ull && lane + o < GC_WAVE) C += oc;
+  }
+  double nc = __shfl_down(C, 1);
+  const uint64_t after = lane + 1 < GC_WAVE ? smask >> (lane + 1) : 0ull;
+  const uint32_t next = lane + 1 + (after ? static_cast<uint32_t>(__builtin_ctzll(after)) : 0u);
+  const uint32_t first_pos = bits ? static_cast<uint32_t>(__builtin_ctz(bits)) : 0u;
+  const uint32_t next_pos = __shfl(first_pos, static_cast<int>(min(next, GC_WAVE - 1u)));
+  if (lane == GC_WAVE - 1) nc = 0.0;
+  if (lane == 0) {
+    head_partial[c] = C;
+    if (smask == 0ull) tail_row[c] = PRM_NONE;
+  }
+  const uint32_t last = bits ? 31u - __clz(bits) : 8u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((bits >> k) & 1u) {
+      const uint32_t row = rbase + __popc(bits & ((2u << k) - 1u));
+      double total = seg[k];
+      uint32_t deg = len[k];
+      bool whole = true;
+      if (static_cast<uint32_t>(k) == last) {
+        total += nc;
+        // the row runs to the first start of lane `next`
+        deg = (8u * next + next_pos) - (8u * lane + last);
+        whole = after != 0ull;
+      }
+      if (whole) {
+        const float r = static_cast<float>(base + damping * total);
+        rank_out[row] = r;
+        contrib_out[row] = r / static_cast<float>(deg);
+        diff += fabs(static_cast<double>(r) - static_cast<double>(rin[k]));
+      } else {
+        tail_row[c] = row;
+        tail_partial[c] = total;
+      }
+    }
+  }
+}
+
+// Waves take items round-robin over the whole grid: the push_count hub rows
+// (pushes of c[h] into the hubs of its list inside P, as pr_spmv_push), then
+// the chunks, then the rows of degree 0 in runs of 512 (eight per lane: rank
+// = base, contribution 0, dangling). Per-workgroup dangling and L1 partials
+// and LDS accumulators go out as in pr_spmv_push.
+extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_merge(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t hot_count, uint32_t push_count,
+    const uint32_t* __restrict__ psplit, uint32_t entries, uint32_t chunks, uint32_t zero_rows,
+    const uint8_t* __restrict__ starts, const uint32_t* __restrict__ chunk_row,
+    double* __restrict__ head_partial, double* __restrict__ tail_partial, uint32_t* __restrict__ tail_row,
+    unsigned long long* __restrict__ push_partials, double* __restrict__ partials) {
+  extern __shared__ unsigned long long s_dyn[];
+  unsigned long long* s_push = s_dyn;
+  float* s_hot = reinterpret_cast<float*>(s_dyn + push_count);
+  __shared__ double s_part[2][PRP_THREADS / GC_WAVE];
+  if (state->done) return;
+  const uint32_t hot_base = n - hot_count;
+  const uint32_t push_base = n - push_count;
+  for (uint32_t i = threadIdx.x; i < hot_count; i += PRP_THREADS) s_hot[i] = contrib_in[hot_base + i];
+  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) s_push[i] = 0ull;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GC_WAVE);
+  const uint32_t waves = gridDim.x * (PRP_THREADS / GC_WAVE);
+  const uint32_t zero_items = (zero_rows + PRM_CHUNK - 1) / PRM_CHUNK;
+  const uint32_t items = push_count + chunks + zero_items;
+  for (uint32_t item = blockIdx.x * (PRP_THREADS / GC_WAVE) + wave; item < items; item += waves) {
+    if (item < push_count) {
+      const uint32_t h = n - 1 - item;
+      const unsigned long long cfix = pr_to_fixed(s_hot[h - hot_base]);
+      const uint32_t e = offsets[h + 1];
+      uint32_t i = psplit[h - push_base] + lane_id();
+      for (; i + 7 * GC_WAVE < e; i += 8 * GC_WAVE) {
+        uint32_t t[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = load_stream(targets + i + j * GC_WAVE);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) atomicAdd(s_push + (t[j] - push_base), cfix);
+      }
+      for (; i < e; i += GC_WAVE) atomicAdd(s_push + (load_stream(targets + i) - push_base), cfix);
+    } else if (item < push_count + chunks) {
+      prm_chunk(item - push_count, entries, targets, starts, chunk_row, contrib_in, s_hot, hot_base, s_push,
+                push_base, rank_in, contrib_out, rank_out, base, damping, diff, head_partial, tail_partial,
+                tail_row);
+    } else {
+      const uint32_t r0 = (item - push_count - chunks) * PRM_CHUNK + lane_id() * 8;
+      const float r = static_cast<float>(base);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t row = r0 + k;
+        if (row < zero_rows) {
+          rank_out[row] = r;
+          contrib_out[row] = 0.0f;
+          dangling += static_cast<double>(r);
+          diff += fabs(static_cast<double>(r) - static_cast<double>(rank_in[row]));
+        }
+      }
+    }
+  }
+  dangling = wave_sum(dangling);
+  diff = wave_sum(diff);
+  if (lane_id() == 0) {
+    s_part[0][wave] = dangling;
+    s_part[1][wave] = diff;
+  }
+  __syncthreads();  // also: every push of this workgroup has landed
+  unsigned long long* out = push_partials + static_cast<uint64_t>(blockIdx.x) * push_count;
+  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) out[i] = s_push[i];
+  if (threadIdx.x == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    for (int w = 0; w < PRP_THREADS / GC_WAVE; ++w) {
+      d0 += s_part[0][w];
+      d1 += s_part[1][w];
+    }
+    partials[2 * static_cast<uint64_t>(blockIdx.x)] = d0;
+    partials[2 * static_cast<uint64_t>(blockIdx.x) + 1] = d1;
+  }
+}
+
+// The hub rows as pr_push_finish (blocks [0, hub_blocks), 16 hubs each), then
+// one lane per chunk (blocks from hub_blocks on, 256 chunks each): a chunk's
+// tail row gets the heads of the chunks after it, up to and including the
+// first chunk that holds a start, and is finished there. Per-block dangling
+// and L1 partials go to partials from partial_base on.
+extern "C" __global__ __launch_bounds__(256) void pr_merge_finish(
+    const uint32_t* __restrict__ offsets, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t push_count,
+    const unsigned long long* __restrict__ push_partials, uint32_t parts, uint32_t hub_blocks,
+    uint32_t chunks, const double* __restrict__ head_partial, const double* __restrict__ tail_partial,
+    const uint32_t* __restrict__ tail_row, double* __restrict__ partials, uint32_t partial_base) {
+  __shared__ unsigned long long s_sum[256];
+  __shared__ double s_part[2][256 / GC_WAVE];
+  if (state->done) return;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  if (blockIdx.x < hub_blocks) {
+    const uint32_t hub = threadIdx.x % PRF_HUBS, j = threadIdx.x / PRF_HUBS;
+    const uint32_t i = blockIdx.x * PRF_HUBS + hub;
+    unsigned long long sum = 0ull;
+    if (i < push_count) {
+      for (uint32_t w = j; w < parts; w += 256 / PRF_HUBS) sum += push_partials[static_cast<uint64_t>(w) * push_count + i];
+    }
+    s_sum[threadIdx.x] = sum;
+    __syncthreads();
+    if (j == 0 && i < push_count) {
+      unsigned long long y = 0ull;
+      for (uint32_t q = 0; q < 256 / PRF_HUBS; ++q) y += s_sum[q * PRF_HUBS + hub];
+      const uint32_t row = n - push_count + i;
+      pr_finish_row(row, offsets[row + 1] - offsets[row], static_cast<double>(y) / PRP_FIX_SCALE, base, damping,
+                    rank_in, rank_out, contrib_out, dangling, diff);
+    }
+  } else {
+    const uint32_t c = (blockIdx.x - hub_blocks) * 256 + threadIdx.x;
+    if (c < chunks) {
+      const uint32_t row = tail_row[c];
+      if (row != PRM_NONE) {
+        double total = tail_partial[c];
+        for (uint32_t k = c + 1; k < chunks; ++k) {
+          total += head_partial[k];
+          if (tail_row[k] != PRM_NONE) break;
+        }
+        pr_finish_row(row, offsets[row + 1] - offsets[row], total, base, damping, rank_in, rank_out,
+                      contrib_out, dangling, diff);
+      }
+    }
+  }
+  dangling = wave_sum(dangling);
+  diff = wave_sum(diff);
+  if (lane_id() == 0) {
+    s_part[0][threadIdx.x / GC_WAVE] = dangling;
+    s_part[1][threadIdx.x / GC_WAVE] = diff;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    for (int w = 0; w < 256 / GC_WAVE; ++w) {
+      d0 += s_part[0][w];
+      d1 += s_part[1][w];
+    }
+    const uint64_t slot = partial_base + blockIdx.x;
+    partials[2 * slot] = d0;
+    partials[2 * slot + 1] = d1;
+  }
+}

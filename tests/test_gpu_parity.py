@@ -61,10 +61,13 @@ def check_tc_partitions(name, g, expected, monkeypatch):
 
 def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
     """Every SpMV kernel against the oracle: the default (push-pull on a symmetric graph, with
-    its push and hot sets at their defaults, at a few dozen ids, and at one id), the pull
-    kernel with its hot set in LDS (default size and a set smaller than the graph), and one
-    workgroup per row tile."""
+    its push and hot sets at their defaults, at a few dozen ids, and at one id), the same with
+    the nonzero-split pull (merge), the row-tiled push-pull (push), the pull kernel with its hot
+    set in LDS (default size and a set smaller than the graph), and one workgroup per row
+    tile."""
     cases = ((None, None, None), (None, "64", "96"), (None, "1", "1"), (None, "37", None),
+             ("merge", None, None), ("merge", "64", "96"), ("merge", "1", "1"), ("merge", "37", None),
+             ("push", None, None), ("push", "37", None),
              ("rows", None, None), ("hot", None, "1024"), ("hot", None, None))
     for kernel, push, hot in cases:
         for var, value in (("GC_PR_KERNEL", kernel), ("GC_PR_PUSH", push), ("GC_PR_HOT", hot)):
@@ -154,6 +157,12 @@ def test_random_graphs(device, monkeypatch):
         check_bfs(name, off, tgt, g, 0)
         check_bfs(name, off, tgt, g, n - 1)
         check_pagerank(name, off, tgt, g)
+        with monkeypatch.context() as mp:
+            # one pushed hub, so the nonzero-split pull covers the other rows in one short chunk
+            mp.setenv("GC_PR_KERNEL", "merge")
+            mp.setenv("GC_PR_PUSH", "1")
+            mp.setenv("GC_PR_HOT", "1")
+            check_pagerank(name + " (merge, push=1)", off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch, ks=(3, 4, 5))
         check_truss_numbers(name, off, tgt, g)
         check_msbfs(name, off, tgt, g, list(range(n)) + [0, n - 1])
