@@ -104,6 +104,9 @@ enum class FixedRule { kBeamer, kBeamerChain };
 // less than a top-down level with its read-back; the cap bounds path-like
 // graphs, where bottom-up rescans the unvisited vertices at every level).
 constexpr uint32_t kTailBottomUp = 3;
+// Counter slots per set of the bottom-up runs (BFS_SLOTS of bfs.hip): each
+// workgroup adds into one, the publication adds them up.
+constexpr uint32_t kBuSlots = 64;
 // Publication slots in the pinned area, one per sequence-number parity: a
 // level launched ahead publishes into the other slot than the one read.
 constexpr size_t kPubStride = 128;
@@ -123,7 +126,8 @@ struct Traversal {
   uint32_t* start;
   uint32_t* scan_buf;
   BfsChainState* chain_state;
-  BfsCounters* bu_counters;  // the two counter sets that bottom-up runs alternate between
+  BfsCounters* bu_counters;  // the three counter sets (kBuSlots slots each) that bottom-up runs rotate through
+  BfsCounters* lvl;          // the counters (kBuSlots slots) of the levels run one at a time
   const uint32_t* off;
   const uint32_t* tgt;
   BfsPublished* pub;      // pinned host memory the levels' publications land in (two slots)
@@ -152,6 +156,8 @@ struct Traversal {
   // bottom-up needs no conversion). Kept for symmetric graphs only.
   bool synced = false;
   bool first_level = true;   // start[0] = 0 from bfs_init: the first level needs no scan
+  uint32_t bu_runs = 0;      // bottom-up runs started (bfs_init clears the counter sets for the first)
+  uint32_t tail_levels = kTailBottomUp;  // GC_BFS_TAIL
 
   Traversal(Graph& graph, uint32_t src)
       : g(graph), dev(*graph.dev), n(graph.n), words64((static_cast<uint64_t>(graph.n) + 63) / 64) {
@@ -164,7 +170,8 @@ struct Traversal {
     start = g.buffer<uint32_t>("bfs.start", sizeof(uint32_t) * n);
     scan_buf = g.buffer<uint32_t>("bfs.scan", sizeof(uint32_t) * scan_scratch_words(n));
     chain_state = g.buffer<BfsChainState>("bfs.chain", sizeof(BfsChainState));
-    bu_counters = g.buffer<BfsCounters>("bfs.bu_counters", 2 * sizeof(BfsCounters));
+    bu_counters = g.buffer<BfsCounters>("bfs.bu_counters", 3 * kBuSlots * sizeof(BfsCounters));
+    lvl = g.buffer<BfsCounters>("bfs.lvl_counters", kBuSlots * sizeof(BfsCounters));
     off = g.rl.offsets.as<const uint32_t>();
     tgt = g.rl.targets.as<const uint32_t>();
     k_td = get_kernel(dev, "bfs", "bfs_topdown_edges");
@@ -186,10 +193,11 @@ struct Traversal {
     // four entries per lane and store, a grid-stride loop over at most 16 workgroups per CU
     Grid1D vgrid = thread_per_item(std::max<uint64_t>(n / 4, 1));
     launch(dev, get_kernel(dev, "bfs", "bfs_init"), std::min(vgrid.blocks, cu * 16), vgrid.threads, 0, dist, n,
-           src, queue[0], &rb->counters[0], bu_counters, start, bitmap[0], bitmap[1], words64);
+           src, queue[0], lvl, bu_counters, start, bitmap[0], bitmap[1], words64);
     synced = g.symmetric;
     m_f = hoff[src + 1] - hoff[src];
     if (const char* a = std::getenv("GC_BFS_ALPHA")) alpha = std::max(1.0, std::atof(a));
+    if (const char* a = std::getenv("GC_BFS_TAIL")) tail_levels = static_cast<uint32_t>(std::strtoul(a, nullptr, 10));
     if (const char* r = std::getenv("GC_BFS_RULE")) {
       if (std::strcmp(r, "beamer") == 0) rule = FixedRule::kBeamer;
       else if (std::strcmp(r, "chain") == 0) rule = FixedRule::kBeamerChain;
@@ -249,7 +257,7 @@ struct Traversal {
     const SchedNorm norm = g.bfs_sched != nullptr ? g.bfs_sched->norm : SchedNorm{};
     // the kernel drops the chained levels itself when the frontier is too large
     const uint32_t mask = (1u << kTopDown) | (g.symmetric ? (1u << kBottomUp) : 0u) | (1u << kChained);
-    launch(dev, k_decide, 1u, 64u, 0, params, norm, level_state(), &rb->counters[0], from_counters ? 1u : 0u,
+    launch(dev, k_decide, 1u, 64u, 0, params, norm, level_state(), lvl, kBuSlots, from_counters ? 1u : 0u,
            mask, kChainCap, static_cast<unsigned long long>(kChainEdges), &rb->decision,
            publish ? slot_dev(seq) : static_cast<BfsPublished*>(nullptr), seq);
   }
@@ -312,7 +320,7 @@ struct Traversal {
     in_bitmap = true;
   }
 
-  // Launches one top-down or bottom-up level (counters into rb->counters[0]).
+  // Launches one top-down or bottom-up level (counters into lvl).
   void launch_level(uint32_t choice) {
     const int32_t next_level = level + 1;
     const bool first = first_level;
@@ -324,7 +332,7 @@ struct Traversal {
         // the source alone: bfs_init wrote its degree prefix start[0] = 0
       } else if (n_f <= kSmallFrontier) {
         launch(dev, k_scan_small, 1u, 1024u, 0, off, static_cast<const uint32_t*>(queue[cur]),
-               static_cast<uint32_t>(n_f), start, &rb->counters[0]);
+               static_cast<uint32_t>(n_f), start, lvl);
       } else {
         Grid1D qg = thread_per_item(n_f);
         launch(dev, k_deg, qg.blocks, qg.threads, 0, off, static_cast<const uint32_t*>(queue[cur]),
@@ -337,10 +345,10 @@ struct Traversal {
       launch(dev, k_td, std::max(1u, std::min(tg.blocks, cu * 32)), tg.threads, 0, off, tgt, dist,
              static_cast<const uint32_t*>(queue[cur]), static_cast<uint32_t>(n_f),
              static_cast<const uint32_t*>(start), static_cast<uint32_t>(m_f), queue[cur ^ 1],
-             &rb->counters[0], next_level, cur_bits, next_bits);
+             lvl, next_level, cur_bits, next_bits);
     } else {
       to_bitmap();
-      launch_bottom_up(cur, next_level, &rb->counters[0]);
+      launch_bottom_up(cur, next_level, lvl, kBuSlots);
     }
   }
 
@@ -348,21 +356,24 @@ struct Traversal {
   // grid depends on n only, not on the frontier. Its counts go to `counters`;
   // with `pub_slot`, it first publishes the finished counters `pub` of the
   // level before it under sequence number `seq`.
-  void launch_bottom_up(int in, int32_t next_level, BfsCounters* counters, BfsCounters* pub = nullptr,
-                        BfsPublished* pub_slot = nullptr, uint32_t seq = 0) {
+  BfsCounters* bu_set(int set) { return bu_counters + static_cast<size_t>(set) * kBuSlots; }
+
+  void launch_bottom_up(int in, int32_t next_level, BfsCounters* counters, uint32_t slots,
+                        BfsCounters* pub = nullptr, BfsPublished* pub_slot = nullptr, uint32_t seq = 0,
+                        BfsCounters* clear = nullptr) {
     Grid1D bg = wave_per_item(words64);
     // the rows of degree 0 come first in the degree order: skip their chunks
     const uint32_t first_chunk = g.rl.dangling / 64;
     launch(dev, k_bu, std::min(bg.blocks, cu * 8), bg.threads, 0, off, tgt, top_entries(), dist,
            reinterpret_cast<const uint32_t*>(bitmap[in]), bitmap[in ^ 1], n, first_chunk, counters,
-           next_level, pub, pub_slot, seq);
+           next_level, pub, pub_slot, seq, clear, slots);
     synced = false;  // the queues no longer hold the frontier
   }
 
   // Publishes (and clears) the level counters under a new sequence number.
   uint32_t publish_counters() {
     const uint32_t seq = ++dev.publish_seq;
-    launch(dev, k_publish, 1u, 64u, 0, &rb->counters[0], slot_dev(seq), seq);
+    launch(dev, k_publish, 1u, 64u, 0, lvl, kBuSlots, slot_dev(seq), seq);
     return seq;
   }
 
@@ -374,19 +385,21 @@ struct Traversal {
   // stays bottom-up and then for at most kTailBottomUp levels; returns when
   // the traversal ends or with the next level to run top-down. `on_level` is
   // called after each level's counters are applied.
-  // The levels alternate between the two bu_counters sets; each level
-  // launched ahead publishes the set of the level before it (so no
-  // publishing launch sits between two levels), and the run's last level is
-  // published by bfs_publish. A set is cleared by its publication; a level
-  // launched after the traversal ended counts nothing, and bfs_init clears
-  // both sets anyway.
+  // The levels rotate through the three bu_counters sets; each level
+  // launched ahead counts into the next set, publishes the set of the level
+  // before it (so no publishing launch sits between two levels) and clears
+  // the set after that, and the run's last level is published by
+  // bfs_publish. A level launched ahead after the traversal has ended sees
+  // the empty count of the level before it and returns at once. bfs_init
+  // clears the sets for the first run; a later run clears them first.
   template <class OnLevel>
   void run_bottom_up(OnLevel&& on_level) {
     to_bitmap();
+    if (bu_runs++ > 0) fill_words(dev, bu_counters, 3 * kBuSlots * sizeof(BfsCounters) / 4, 0);
     int in = cur;
     int32_t lvl = level;  // the level whose frontier is bitmap[in]
     int set = 0;          // the counter set of the oldest level not yet read
-    launch_bottom_up(in, lvl + 1, &bu_counters[set]);
+    launch_bottom_up(in, lvl + 1, bu_set(set), kBuSlots);
     in ^= 1;
     ++lvl;
     uint32_t seq = ++dev.publish_seq;  // the publication of that level's set
@@ -394,11 +407,12 @@ struct Traversal {
     bool ahead = true;
     for (;;) {
       if (ahead) {
-        launch_bottom_up(in, lvl + 1, &bu_counters[set ^ 1], &bu_counters[set], slot_dev(seq), seq);
+        launch_bottom_up(in, lvl + 1, bu_set((set + 1) % 3), kBuSlots, bu_set(set), slot_dev(seq), seq,
+                         bu_set((set + 2) % 3));
         in ^= 1;
         ++lvl;
       } else {
-        launch(dev, k_publish, 1u, 64u, 0, &bu_counters[set], slot_dev(seq), seq);
+        launch(dev, k_publish, 1u, 64u, 0, bu_set(set), kBuSlots, slot_dev(seq), seq);
       }
       wait_published(seq);
       BfsCounters counters;
@@ -408,8 +422,8 @@ struct Traversal {
       if (done || !ahead) break;
       // the level after the one just launched: bottom-up while the rule says so, then the tail
       if (static_cast<double>(n_f) < static_cast<double>(n) / kBeta) ++tail;
-      ahead = tail < kTailBottomUp;
-      set ^= 1;
+      ahead = tail < tail_levels;
+      set = (set + 1) % 3;
       seq = ++dev.publish_seq;
     }
     top_down_dir = true;

@@ -40,9 +40,12 @@ struct BfsPublished {
   uint32_t pad[3];
 };
 
-// Also clears the level counters (the top-down ones, which every level's
-// publication clears again for the next one, and the two of the bottom-up
-// runs), sets start[0] = 0 (the degree prefix of the one-vertex first
+// Counter slots per set of the bottom-up runs (see block_counters_add_slot).
+#define BFS_SLOTS 64
+
+// Also clears the level counters (the BFS_SLOTS slots of the levels run one
+// at a time, which every level's publication clears again for the next one,
+// and the three sets of the bottom-up runs), sets start[0] = 0 (the degree prefix of the one-vertex first
 // frontier, so the first level needs no scan), and sets up the frontier
 // bitmaps that top-down levels keep in step with their queues: bits0 = the
 // source alone, bits1 = empty (`words` 64-bit words each).
@@ -69,33 +72,53 @@ extern "C" __global__ __launch_bounds__(256) void bfs_init(int32_t* __restrict__
   if (global_thread() == 0) {
     queue[0] = source;
     start[0] = 0;
-    counters->next_size = 0;
-    counters->next_edges = 0;
-    for (int i = 0; i < 2; ++i) {
-      bu_counters[i].next_size = 0;
-      bu_counters[i].next_edges = 0;
-    }
+  }
+  if (global_thread() < BFS_SLOTS) {
+    counters[global_thread()].next_size = 0;
+    counters[global_thread()].pad = 0;
+    counters[global_thread()].next_edges = 0;
+  }
+  if (global_thread() < 3 * BFS_SLOTS) {
+    bu_counters[global_thread()].next_size = 0;
+    bu_counters[global_thread()].next_edges = 0;
   }
 }
 
-// Copies a finished level's counters to `host` (one lane), clears them for
-// the next level, then sets host->seq = seq after a system-scope fence, so
-// the counters are visible before the sequence word.
-__device__ __forceinline__ void publish_counters(BfsCounters* __restrict__ counters,
-                                                 BfsPublished* __restrict__ host, uint32_t seq) {
-  const BfsCounters c = *counters;
-  host->counters = c;
-  counters->next_size = 0;
-  counters->next_edges = 0;
-  __threadfence_system();
-  __hip_atomic_store(&host->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+// Copies a finished level's counters, the sum of its `slots` counter slots,
+// to `host`, clears the slots for the next level unless `keep`, then sets
+// host->seq = seq after a system-scope fence, so the counters are visible
+// before the sequence word. Called by one whole wave (slots <= 64).
+__device__ __forceinline__ void publish_counters(BfsCounters* __restrict__ counters, uint32_t slots,
+                                                 BfsPublished* __restrict__ host, uint32_t seq,
+                                                 bool keep = false) {
+  const uint32_t lane = lane_id();
+  uint32_t c = 0;
+  unsigned long long e = 0;
+  if (lane < slots) {
+    c = counters[lane].next_size;
+    e = counters[lane].next_edges;
+  }
+  c = wave_sum(c);
+  e = wave_sum(e);
+  if (!keep && lane < slots) {
+    counters[lane].next_size = 0;
+    counters[lane].pad = 0;
+    counters[lane].next_edges = 0;
+  }
+  if (lane == 0) {
+    host->counters.next_size = c;
+    host->counters.pad = 0;
+    host->counters.next_edges = e;
+    __threadfence_system();
+    __hip_atomic_store(&host->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
-// Publishes the level's counters to the host (one lane).
-extern "C" __global__ __launch_bounds__(64) void bfs_publish(BfsCounters* __restrict__ counters,
+// Publishes the level's counters (`slots` slots) to the host (one wave).
+extern "C" __global__ __launch_bounds__(64) void bfs_publish(BfsCounters* __restrict__ counters, uint32_t slots,
                                                              BfsPublished* __restrict__ host, uint32_t seq) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  publish_counters(counters, host, seq);
+  if (blockIdx.x != 0) return;
+  publish_counters(counters, slots, host, seq);
 }
 
 // top[k * n + v] = the k-th last entry of row v (k < 4); past the row's start,
@@ -139,6 +162,16 @@ __device__ __forceinline__ void block_counters_add(uint32_t count, unsigned long
   }
 }
 
+// As block_counters_add into slot blockIdx.x % slots of `counters`: device-
+// scope atomics on one word serialize across the XCDs (a bottom-up level
+// whose 2,048 workgroups all found vertices took 40-60 us more than one
+// whose workgroups found none), so the bottom-up runs spread them over
+// BFS_SLOTS words and their publication adds the slots up.
+__device__ __forceinline__ void block_counters_add_slot(uint32_t count, unsigned long long edges,
+                                                        BfsCounters* __restrict__ counters, uint32_t slots) {
+  block_counters_add(count, edges, counters + blockIdx.x % slots);
+}
+
 // A wave takes two chunks of 64 vertices at a time (chunk and chunk + waves),
 // their loads interleaved: dist of both, then the four candidates and row
 // bounds of both, then the candidates' frontier words, so that each wave has
@@ -150,11 +183,26 @@ extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
     const uint32_t* __restrict__ top, int32_t* __restrict__ dist, const uint32_t* __restrict__ front,
     uint64_t* __restrict__ next, uint32_t n, uint32_t first_chunk, BfsCounters* __restrict__ counters,
     int32_t next_level, BfsCounters* __restrict__ pub_counters, BfsPublished* __restrict__ pub_host,
-    uint32_t pub_seq) {
+    uint32_t pub_seq, BfsCounters* __restrict__ clear_counters, uint32_t slots) {
   // A level launched ahead publishes the counters of the level before it
   // (finished, since launches on the stream run in order): no publishing
-  // launch of its own between the two levels.
-  if (pub_host != nullptr && blockIdx.x == 0 && threadIdx.x == 0) publish_counters(pub_counters, pub_host, pub_seq);
+  // launch of its own between the two levels. The runs rotate three counter
+  // sets of `slots` slots each: this level counts into one, publishes the one
+  // before it (left as it is, every wave reads it) and clears the one of two
+  // levels back, which the next level counts into. If the level before found
+  // nothing, the traversal has ended and this level returns at once.
+  if (pub_host != nullptr) {
+    const uint32_t lane_slot = lane_id();
+    const bool any = __ballot(lane_slot < slots && pub_counters[lane_slot].next_size != 0) != 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < GC_WAVE) {
+      publish_counters(pub_counters, slots, pub_host, pub_seq, true);
+      if (clear_counters != nullptr && lane_slot < slots) {
+        clear_counters[lane_slot].next_size = 0;
+        clear_counters[lane_slot].next_edges = 0;
+      }
+    }
+    if (!any) return;
+  }
   const uint32_t lane = lane_id();
   uint32_t found_count = 0;  // this lane's discoveries and their degrees
   unsigned long long found_edges = 0;
@@ -228,7 +276,7 @@ extern "C" __global__ __launch_bounds__(256) void bfs_bottomup(
       found_edges += deg;
     }
   }
-  block_counters_add(found_count, found_edges, counters);
+  block_counters_add_slot(found_count, found_edges, counters, slots);
 }
 
 // degree[i] = degree of queue[i] (input of the frontier's edge prefix sum).
@@ -255,6 +303,7 @@ extern "C" __global__ __launch_bounds__(FS_THREADS) void bfs_frontier_scan_small
   if (threadIdx.x == 0) {
     s_carry = 0;
     counters->next_size = 0;
+    counters->pad = 0;
     counters->next_edges = 0;
   }
   __syncthreads();
@@ -304,7 +353,12 @@ extern "C" __global__ __launch_bounds__(256) void bfs_topdown_edges(
   // so the lane of its first edge clears its bit.
   // The workgroup's discoveries of one block of 4 x TD_EDGES_PER_WAVE edges are
   // gathered in LDS and appended with one atomic (one per wave and 64 edges,
-  // all on one counter, serialized: a 370K-edge level took 368 us).
+  // all on one counter, serialized: a 370K-edge level took 368 us) on the
+  // size of counter slot 0; the edges go to the BFS_SLOTS counter slots.
+  // (Appending only while the queue held fewer than 16,384 entries, with the
+  // level's size counted in the slots and larger frontiers left in next_bits,
+  // was slower: the test of the queue counter before each append is one more
+  // access to its contended line; 0.22 against 0.185 ms per R-MAT 22 source.)
   __shared__ uint32_t s_list[4 * TD_EDGES_PER_WAVE];
   __shared__ uint32_t s_count, s_base;
   const uint32_t lane = lane_id();
@@ -368,7 +422,7 @@ extern "C" __global__ __launch_bounds__(256) void bfs_topdown_edges(
     if (threadIdx.x == 0) s_count = 0;
     __syncthreads();
   }
-  block_counters_add(0u, found_edges, counters);
+  block_counters_add_slot(0u, found_edges, counters, BFS_SLOTS);
 }
 
 // Sets the bitmap bits of the queue's vertices (the bitmap is cleared first).

@@ -211,19 +211,33 @@ __device__ __forceinline__ float lg(double x) { return static_cast<float>(log2(x
 // null only the features are written; otherwise the allowed choice
 // (allowed_mask bit c; choice 2 also needs n_f <= chain_cap and m_f <=
 // chain_edge_cap) with the
-// smallest predicted log-time. With `host` (and use_counters), lane 0 also
+// smallest predicted log-time. The counters are the sum of `slots` counter
+// slots (slots <= 64). With `host` (and use_counters), the kernel also
 // publishes the counters and the decision there as bfs_publish does, clears
-// the counters for the next level and sets host->seq = seq last.
+// the slots for the next level and sets host->seq = seq last.
 extern "C" __global__ __launch_bounds__(64) void sched_bfs_decide(
     const float* __restrict__ params, MlpNorm norm, BfsLevelState st,
-    BfsCountersView* __restrict__ counters, uint32_t use_counters, uint32_t allowed_mask,
+    BfsCountersView* __restrict__ counters, uint32_t slots, uint32_t use_counters, uint32_t allowed_mask,
     uint32_t chain_cap, unsigned long long chain_edge_cap, SchedDecision* __restrict__ out,
     BfsPublishedView* __restrict__ host, uint32_t seq) {
   if (blockIdx.x != 0) return;
   const uint32_t lane = lane_id();
   // every lane derives the same state and features
+  BfsCountersView c{0u, 0u, 0ull};
   if (use_counters) {
-    const BfsCountersView c = *counters;
+    uint32_t size = 0;
+    unsigned long long edges = 0;
+    if (lane < slots) {
+      size = counters[lane].next_size;
+      edges = counters[lane].next_edges;
+    }
+    c.next_size = wave_sum(size);
+    c.next_edges = wave_sum(edges);
+    if (host != nullptr && lane < slots) {  // each lane clears the slot it read
+      counters[lane].next_size = 0;
+      counters[lane].pad = 0;  // the top-down queue counter
+      counters[lane].next_edges = 0;
+    }
     st.prev_n_f = st.n_f;
     st.n_f = c.next_size;
     st.m_f = c.next_edges;
@@ -285,12 +299,10 @@ extern "C" __global__ __launch_bounds__(64) void sched_bfs_decide(
   }
   out->choice = best == 0xFFFFFFFFu ? 0u : best;
   if (host != nullptr && use_counters) {
-    host->counters = *counters;
+    host->counters = c;
     for (int i = 0; i < MLP_IN; ++i) host->decision.features[i] = f[i];
     for (int k = 0; k < MLP_OUT; ++k) host->decision.predicted[k] = y[k];
     host->decision.choice = best == 0xFFFFFFFFu ? 0u : best;
-    counters->next_size = 0;
-    counters->next_edges = 0;
     __threadfence_system();
     __hip_atomic_store(&host->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
