@@ -114,13 +114,24 @@ def roofline_entry(bytes_per_launch: float, avg_us: float, traffic=None, note: s
     return out
 
 
-def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bits: int = 32768) -> dict:
+def tc_top_bits() -> int:
+    """The width of tc_bitmap16's masks of the top ids, as the library reads it (GC_TC_TOP,
+    default 4096; 0: no masks)."""
+    env = os.environ.get("GC_TC_TOP")
+    return int(env) if env is not None else 4096
+
+
+def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bits: int = 32768,
+                          top_bits: int | None = None) -> dict:
     """Bytes tc_bitmap16 reads per launch, by the model that fits the kernel (on the degree-ordered
     DAG the kernel runs on, `rl` = GpuGraph.relabelled_numpy()): offsets and the DAG split read
     once, 4(n+1) + 4n; L(u) at 4 bytes per entry once per part it is split into (rows of at
     least `split_min` DAG entries run in `parts` parts, shorter rows once); and for every DAG
     edge u -> v the list L(v) at its stored width: its entries among the top `hot_bits` ids as
-    16-bit offsets in whole 16-byte octs of eight, its other entries at 4 bytes."""
+    16-bit offsets in whole 16-byte octs of eight, its other entries at 4 bytes; a list whose
+    entries among the top `top_bits` ids fill more octs than their mask (top_bits / 8 bytes)
+    holds the mask instead of those offsets."""
+    top_bits = tc_top_bits() if top_bits is None else top_bits
     off = rl["offsets"].astype(np.int64)
     fwd = rl["fwd"].astype(np.int64)
     tgt = rl["targets"]
@@ -134,12 +145,21 @@ def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bi
     hot = np.bincount(row, weights=(dag_v >= hot_base), minlength=n).astype(np.int64)
     cold = dplus - hot
     octs = (hot + 7) // 8
+    masked_rows = 0
+    if top_bits:
+        top_octs = top_bits // 128
+        top_base = max(hot_base, ((n - top_bits) + 31) & ~31 if n > top_bits else 0)
+        top = np.bincount(row, weights=(dag_v >= top_base), minlength=n).astype(np.int64)
+        masked = (top >> 3) > top_octs
+        octs = np.where(masked, (hot - top + 7) // 8 + top_octs, octs)
+        masked_rows = int(masked.sum())
     indeg = np.bincount(dag_v, minlength=n).astype(np.int64)
     streamed = int((indeg * (16 * octs + 4 * cold)).sum())
     row_reads = int((4 * dplus * np.where(dplus >= split_min, parts, 1)).sum())
     total = 4 * (n + 1) + 4 * n + row_reads + streamed
     return {"bytes": total, "list_bytes": streamed, "list_bytes_32bit": int((indeg * 4 * dplus).sum()),
-            "row_bytes": row_reads, "hot_base": int(hot_base),
+            "row_bytes": row_reads, "hot_base": int(hot_base), "top_bits": int(top_bits),
+            "masked_rows": masked_rows,
             "hot_fraction_of_streamed_entries": float((indeg * hot).sum() / max(1, (indeg * dplus).sum()))}
 
 
@@ -332,7 +352,9 @@ def main() -> int:
                 model, kernels[name]["avg_us"], traffic.get(name),
                 "stored-width model: 4(n+1) + 4n, L(u) at 4 bytes once per part it is split into, and "
                 "for every DAG edge u->v the list L(v) at its stored width (entries among the top 32768 "
-                "ids as 16-bit offsets in 16-byte octs of eight, others at 4 bytes); the merge model "
+                "ids as 16-bit offsets in 16-byte octs of eight, others at 4 bytes, and the entries among "
+                f"the top {tc16['top_bits']} ids as a {tc16['top_bits'] // 8}-byte mask where that is "
+                f"smaller: {tc16['masked_rows']} lists); the merge model "
                 f"(both lists at 4 bytes, {tc_merge_bytes} bytes) would count data the kernel never reads")
             rooflines[name]["merge_model_bytes"] = tc_merge_bytes
     if "tc_wedge" in kernels:

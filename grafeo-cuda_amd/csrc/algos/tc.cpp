@@ -70,11 +70,37 @@ bool tc_hot16() {
   return env == nullptr || env[0] != '0';
 }
 
+// Masks of the top ids in the 16-bit lists (tc_bitmap16): GC_TC_TOP ids (a
+// power of two from 128 to 8192; 0: no masks), default 4096, i.e. 32 octs
+// (512 bytes) per mask. At R-MAT scale 22 the top 1024 ids alone are 51% of
+// the entries the kernel streams (29.4 GB as offsets per count) and the top
+// 2048 ids 69%. Measured tc_bitmap16 per launch with masks of 0 / 1024 /
+// 2048 / 4096 / 8192 ids: R-MAT 22 8.55 / 7.16 / 6.74 / 6.69 / 7.17 ms, R-MAT
+// 24 83.5 / 78.7 / 74.9 / 74.0 / 75.5 ms, R-MAT 20 1.85 / 1.67 / 1.62 / 1.63 /
+// 1.76 ms.
+uint32_t tc_top_octs() {
+  const char* env = std::getenv("GC_TC_TOP");
+  const uint32_t bits = env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : 4096u;
+  if (bits == 0) return 0;
+  if (bits < 128 || bits > 8192 || (bits & (bits - 1)) != 0)
+    fail_kernel("GC_TC_TOP must be 0 or a power of two from 128 to 8192");
+  return bits / 128;
+}
+
+// First id of the masked top ids: the top `octs` * 128 ids, rounded up to a
+// multiple of 32 (so its bitmap words start at a word boundary), and never
+// below hot_base.
+uint32_t tc_top_base(uint32_t n, uint32_t hot_base, uint32_t octs) {
+  const uint32_t bits = octs * 128u;
+  const uint32_t t = n > bits ? ((n - bits) + 31u) & ~31u : 0u;
+  return std::max(t, hot_base);
+}
+
 // The 16-bit hot lists of tc_bitmap16, built once per graph (again if the
-// parts change): at most (m + 7n) / 8 + 1 octs of 16 bytes, and 12 bytes per
-// vertex.
-void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts) {
-  if (g.tc_hot16_ready && g.tc_hot16_parts == parts) return;
+// parts or the masks change): at most (m + 7n) / 8 + 1 octs of 16 bytes (a
+// mask replaces more octs than it takes), and 12 bytes per vertex.
+void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts, uint32_t top_octs) {
+  if (g.tc_hot16_ready && g.tc_hot16_parts == parts && g.tc_hot16_top == top_octs) return;
   Device& dev = *g.dev;
   const uint32_t n = g.n;
   if (n == 0) return;  // no rows, nothing to count
@@ -85,22 +111,28 @@ void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts) {
   g.tc_split.allocate(sizeof(uint32_t) * rows);
   g.tc_hw.allocate(sizeof(uint32_t) * 2 * rows);
   g.tc_words.allocate(sizeof(uint16_t) * 8 * max_quads);
-  DeviceMemory qoff, scan;
+  DeviceMemory qoff, scan, tsplit;
   qoff.allocate(sizeof(uint32_t) * rows);
   scan.allocate(sizeof(uint32_t) * scan_scratch_words(rows));
+  tsplit.allocate(sizeof(uint32_t) * (top_octs != 0 ? rows : 1));
+  const uint32_t top_base = tc_top_base(n, hot_base, top_octs);
   const uint32_t* offsets = g.rl.offsets.as<const uint32_t>();
   const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
   Grid1D vg = thread_per_item(rows);
   launch(dev, get_kernel(dev, "tc", "tc_hot16_rows"), vg.blocks, vg.threads, 0, g.rl.fwd.as<const uint32_t>(),
-         offsets + 1, tgt, n, hot_base, parts, g.tc_split.as<uint32_t>(), qoff.as<uint32_t>());
+         offsets + 1, tgt, n, hot_base, parts, g.tc_split.as<uint32_t>(), qoff.as<uint32_t>(), top_base, top_octs,
+         tsplit.as<uint32_t>());
   exclusive_scan_u32(dev, qoff.as<uint32_t>(), n, scan.as<uint32_t>());
+  if (top_octs != 0) fill_words(dev, g.tc_words.as<uint32_t>(), 4 * max_quads, 0);  // the masks are ORed in
   Grid1D eg = thread_per_item(std::max<uint64_t>(m, n));
   launch(dev, get_kernel(dev, "tc", "tc_hot16_fill"), eg.blocks, eg.threads, 0, offsets, tgt, n, m, hot_base,
          parts, static_cast<const uint32_t*>(g.tc_split.as<uint32_t>()), static_cast<const uint32_t*>(qoff.as<uint32_t>()),
-         g.tc_hw.as<uint32_t>() /* uint2[n] */, g.tc_words.as<uint16_t>());
-  dev.sync();  // qoff and scan are released on return
+         g.tc_hw.as<uint32_t>() /* uint2[n] */, g.tc_words.as<uint16_t>(), top_base, top_octs,
+         static_cast<const uint32_t*>(tsplit.as<uint32_t>()));
+  dev.sync();  // qoff, scan and tsplit are released on return
   g.tc_hot16_ready = true;
   g.tc_hot16_parts = parts;
+  g.tc_hot16_top = top_octs;
 }
 
 // Shortest L(u) that tc_bitmap splits over the parts (shorter rows run whole,
@@ -169,12 +201,15 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
       if (no_alloc) launch_untimed(dev, k, grid_for(tasks), 4 * kWaveSize, 0, args...);
       else launch(dev, k, grid_for(tasks), 4 * kWaveSize, 0, args...);
     };
+    // the no-alloc call uses the lists as they were built (their masks included)
+    const uint32_t top_octs = no_alloc ? g.tc_hot16_top : tc_top_octs();
     const bool hot16 = tc_hot16() && (!no_alloc || (g.tc_hot16_ready && g.tc_hot16_parts == parts));
     if (hot16) {
-      if (!no_alloc) ensure_hot16(g, hot_base, parts);
+      if (!no_alloc) ensure_hot16(g, hot_base, parts, top_octs);
+      const uint32_t top_word0 = (tc_top_base(n, hot_base, top_octs) - hot_base) / 32u;
       go("tc_bitmap16", fwd, end, tgt, g.tc_split.as<const uint32_t>(),
          g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
-         row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result);
+         row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result, top_word0, top_octs);
     } else {
       go("tc_bitmap", fwd, end, tgt, row_hi, row_lo, hot_base, single, light_chunk, parts, tc_split_min(),
          counters, result);

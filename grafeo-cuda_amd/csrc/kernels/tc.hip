@@ -424,6 +424,15 @@ __device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, 
 // goes with the lane loads, not their width), and no entry needs a mask or a
 // clamp.
 #define TCB_PAD16 TCB_HOT_BITS
+// Masks of the top ids: a list L(v) whose entries among the top ids (ids from
+// top_base on, top_octs * 128 of them) would take more octs as offsets than
+// their mask takes (top_octs octs) keeps only the entries below top_base as
+// offsets and stores the mask after them; hw[v].y then carries TC_TOP_FLAG and
+// counts the mask's octs. The mask's bits and the row's bitmap words from
+// top_word0 = (top_base - hot_base) / 32 on cover the same ids, so a masked
+// top part adds popcount(mask & bitmap words): 128 bytes per list at
+// top_octs = 8 against two bytes per top entry.
+#define TC_TOP_FLAG 0x80000000u
 
 __device__ __forceinline__ uint32_t hot16_hit(const uint32_t* s_bits, uint32_t y) {
   return word_bit(s_bits[y >> 5], y);
@@ -519,7 +528,8 @@ __device__ __forceinline__ void tc_bitmap_body(
     const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint4* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
-    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits) {
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits,
+    uint32_t top_word0 = 0, uint32_t top_octs = 0) {
   const uint32_t positions = n - row_lo;  // rows [row_lo, n)
   const uint32_t lane = lane_id();
   for (uint32_t j = lane; j < TCB_BLOOM_BASE + TCB_BLOOM_WORDS; j += GC_WAVE) s_bits[j] = 0;
@@ -612,14 +622,20 @@ __device__ __forceinline__ void tc_bitmap_body(
             // kHot16: [st, st + len) are the octs of L(v)'s hot part and
             // [cst, cst + clen) its cold part in targets (walked only when
             // the row has a cold prefix); otherwise all of L(v) in targets
-            uint32_t st = 0, len = 0, cst = 0, clen = 0;
+            uint32_t st = 0, len = 0, cst = 0, clen = 0, mst = 0;
+            bool masked = false;  // L(v)'s top part is a mask (TC_TOP_FLAG)
             if (lane < nv) {
               const uint32_t v = A[cv + lane];
               if (whole || v % parts == part) {
                 if (kHot16) {
                   const uint2 h = hw[v];
                   st = h.x;
-                  len = h.y;
+                  len = h.y & ~TC_TOP_FLAG;
+                  if (h.y & TC_TOP_FLAG) {
+                    masked = true;
+                    len -= top_octs;
+                    mst = st + len;
+                  }
                   if (c != 0) {
                     cst = lo[v];
                     clen = split[v] - cst;
@@ -648,6 +664,33 @@ __device__ __forceinline__ void tc_bitmap_body(
               } else {
                 if (kHot16) local += probe16_list(quads + s1, lj, s_bits);
                 else local += probe_list(targets + s1, lj, s_bits, hot_base, A, c);
+              }
+            }
+            if (kHot16 && top_octs != 0) {
+              // the masks of the top ids: top_octs lanes per mask (one oct,
+              // four words, each), 64 / top_octs masks per step, against the
+              // same words of the row's bitmap
+              unsigned long long mrows = __ballot(masked);
+              const uint32_t per = GC_WAVE / top_octs;
+              const uint32_t grp = lane / top_octs, chunk = lane % top_octs;
+              const uint32_t* ubw = s_bits + top_word0 + 4 * chunk;  // (top_word0 need not be a multiple of 4)
+              const uint4 ub = make_uint4(ubw[0], ubw[1], ubw[2], ubw[3]);
+              while (mrows != 0) {
+                uint32_t mine = 0;
+                bool take = false;
+                for (uint32_t g = 0; g < per && mrows != 0; ++g) {
+                  const uint32_t j = static_cast<uint32_t>(__ffsll(static_cast<long long>(mrows)) - 1);
+                  mrows &= mrows - 1;
+                  if (grp == g) {
+                    mine = j;
+                    take = true;
+                  }
+                }
+                const uint32_t base = static_cast<uint32_t>(__shfl(static_cast<int>(mst), static_cast<int>(mine)));
+                if (take) {
+                  const uint4 mk = quads[base + chunk];
+                  local += __popc(mk.x & ub.x) + __popc(mk.y & ub.y) + __popc(mk.z & ub.z) + __popc(mk.w & ub.w);
+                }
               }
             }
             if (kHot16 && c != 0) {
@@ -697,10 +740,11 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
     const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint4* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
-    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result) {
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t top_word0,
+    uint32_t top_octs) {
   __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
   tc_bitmap_body<true>(lo, hi, targets, split, hw, quads, n, row_lo, hot_base, heavy, light_chunk, parts,
-                       split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE]);
+                       split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE], top_word0, top_octs);
 }
 
 // The hot lists of tc_bitmap16, in two passes around an exclusive scan of
@@ -718,12 +762,23 @@ __device__ __forceinline__ uint32_t part_slot(uint32_t v, uint32_t n, uint32_t p
 
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_rows(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
-    uint32_t n, uint32_t hot_base, uint32_t parts, uint32_t* __restrict__ split, uint32_t* __restrict__ nquads) {
+    uint32_t n, uint32_t hot_base, uint32_t parts, uint32_t* __restrict__ split, uint32_t* __restrict__ nquads,
+    uint32_t top_base, uint32_t top_octs, uint32_t* __restrict__ tsplit) {
   for (uint64_t v = global_thread(); v < n; v += grid_threads()) {
     const uint32_t b = hi[v];
     const uint32_t sp = lower_bound_u32(targets, lo[v], b, hot_base);
     split[v] = sp;
-    nquads[part_slot(static_cast<uint32_t>(v), n, parts)] = (b - sp + 7) >> 3;
+    uint32_t end = b, extra = 0;
+    if (top_octs != 0) {
+      // a mask of top_octs octs when the top part would take more octs as offsets
+      const uint32_t ts = lower_bound_u32(targets, sp, b, top_base);
+      if (((b - ts) >> 3) > top_octs) {
+        end = ts;
+        extra = top_octs;
+      }
+      tsplit[v] = end;
+    }
+    nquads[part_slot(static_cast<uint32_t>(v), n, parts)] = ((end - sp + 7) >> 3) + extra;
   }
 }
 
@@ -744,23 +799,33 @@ __device__ __forceinline__ uint64_t quad_slot(uint32_t e, uint32_t len) {
 extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n, uint64_t m,
     uint32_t hot_base, uint32_t parts, const uint32_t* __restrict__ split, const uint32_t* __restrict__ qoff,
-    uint2* __restrict__ hw, uint16_t* __restrict__ half) {
+    uint2* __restrict__ hw, uint16_t* __restrict__ half, uint32_t top_base, uint32_t top_octs,
+    const uint32_t* __restrict__ tsplit) {
   const uint64_t total = m > n ? m : static_cast<uint64_t>(n);
   for (uint64_t i = global_thread(); i < total; i += grid_threads()) {
     if (i < m) {
       const uint32_t v = row_of_entry(offsets, n, static_cast<uint32_t>(i));
       const uint32_t sp = split[v];
-      const uint32_t end = offsets[v + 1];
-      if (i >= sp && i < end)
-        half[8ull * qoff[part_slot(v, n, parts)] + quad_slot(static_cast<uint32_t>(i) - sp, end - sp)] =
-            static_cast<uint16_t>(targets[i] - hot_base);
+      const uint32_t row_end = offsets[v + 1];
+      const uint32_t end = top_octs != 0 ? tsplit[v] : row_end;  // the offsets' part
+      const uint64_t q8 = 8ull * qoff[part_slot(v, n, parts)];
+      if (i >= sp && i < end) {
+        half[q8 + quad_slot(static_cast<uint32_t>(i) - sp, end - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
+      } else if (i >= end && i < row_end) {
+        // a masked top entry: its bit in the mask after the offsets' octs (zeroed beforehand)
+        const uint32_t y = targets[i] - top_base;
+        uint32_t* mask = reinterpret_cast<uint32_t*>(half + q8 + 8ull * ((end - sp + 7) >> 3));
+        atomicOr(mask + (y >> 5), 1u << (y & 31));
+      }
     }
     if (i < n) {
       const uint32_t v = static_cast<uint32_t>(i);
       const uint32_t q = qoff[part_slot(v, n, parts)];
-      const uint32_t len = offsets[v + 1] - split[v];
+      const uint32_t row_end = offsets[v + 1];
+      const uint32_t end = top_octs != 0 ? tsplit[v] : row_end;
+      const uint32_t len = end - split[v];
       const uint32_t nq = (len + 7) >> 3;
-      hw[v] = make_uint2(q, nq);
+      hw[v] = end < row_end ? make_uint2(q, (nq + top_octs) | TC_TOP_FLAG) : make_uint2(q, nq);
       // the last block's free slots (at most seven) hold the pad
       if (len & 7u) {
         const uint32_t last = (len - 1) >> 9;

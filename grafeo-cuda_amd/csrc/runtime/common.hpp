@@ -347,6 +347,7 @@ struct Graph {
   DeviceMemory tc_words;      // 16-byte octs of eight offsets of all rows, part by part
   bool tc_hot16_ready = false;
   uint32_t tc_hot16_parts = 0;  // the middle-vertex parts the words are grouped by
+  uint32_t tc_hot16_top = 0;    // octs per mask of the top ids in the words (0: no masks)
   // The learned triangle-count schedule (method per degree bucket): a function
   // of the graph and the network, kept with the network and version it came from.
   std::vector<uint32_t> tc_choice;

@@ -13,7 +13,7 @@ The committed model (sched_model.json next to this file) was produced by
 
     python3 -m grafeo_cuda.scheduler train --out grafeo-cuda_amd/grafeo_cuda/sched_model.json
 
-on one MI355X, from R-MAT scales 16..22 (edge factor 16) and square 4-neighbour grids with 256,
+on one MI355X, from R-MAT scales 16..23 (edge factor 16) and square 4-neighbour grids with 256,
 512, 1024, 2048 and 2896 rows (the training set below). It is evaluated on graphs it was not trained
 on: R-MAT scale 24 and the 4096 x 4096 grid (bench.py --workload scheduler).
 """
@@ -56,7 +56,7 @@ def nonzero_sources(off, count: int, seed: int) -> list[int]:
 
 def training_graphs(quick: bool = False):
     """(name, offsets, targets, BFS sources) of the training set."""
-    scales = (16, 18) if quick else (16, 17, 18, 19, 20, 21, 22)
+    scales = (16, 18) if quick else (16, 17, 18, 19, 20, 21, 22, 23)
     # 2896 x 2896 (8.4M vertices) is the first square size here at which the binary-search
     # intersection beats the bitmap on the grid's degree-4 rows (2571 against 2923 us)
     rows = (256, 512) if quick else (256, 512, 1024, 2048, 2896)

@@ -57,14 +57,17 @@ def test_rmat20_suite(device, golden, monkeypatch):
     g = grafeo_cuda.GpuGraph.from_csr(off, tgt, device)
     assert g.triangle_count()["total_triangles"] == e["triangles"]
     # both list layouts of the bitmap kernel (16-bit hot lists, the default, and 32-bit lists) with
-    # the middle vertices in 1, 3 and 8 parts: at n = 2^20 the rows have cold prefixes (ids below
-    # the top 32768), which the smaller parity graphs do not
-    for hot16 in ("0", "1"):
+    # the middle vertices in 1, 3 and 8 parts, and the 16-bit lists with the masks of the top ids
+    # off and at 128, 2048 and 8192 ids: at n = 2^20 the rows have cold prefixes (ids below the top
+    # 32768), which the smaller parity graphs do not
+    for hot16, top in (("0", "0"), ("1", "0"), ("1", "128"), ("1", "1024"), ("1", "2048"), ("1", "8192")):
         for parts in ("1", "3", "8"):
             monkeypatch.setenv("GC_TC_HOT16", hot16)
+            monkeypatch.setenv("GC_TC_TOP", top)
             monkeypatch.setenv("GC_TC_PARTS", parts)
-            assert g.triangle_count()["total_triangles"] == e["triangles"], f"hot16={hot16} parts={parts}"
+            assert g.triangle_count()["total_triangles"] == e["triangles"], f"hot16={hot16} top={top} parts={parts}"
     monkeypatch.delenv("GC_TC_HOT16")
+    monkeypatch.delenv("GC_TC_TOP")
     monkeypatch.delenv("GC_TC_PARTS")
 
     kt = g.k_truss_numpy(4)

@@ -31,6 +31,13 @@ def upload(device, off, tgt):
 def check_triangles(name, device, off, tgt, g):
     expected = oracle.triangle_count(off, tgt)
     assert g.triangle_count()["total_triangles"] == expected, f"{name}: triangle_count"
+    # the bitmap kernel's masks of the top ids: off, and at 128 ids (masks on short lists)
+    for top in ("0", "128"):
+        os.environ["GC_TC_TOP"] = top
+        try:
+            assert g.triangle_count()["total_triangles"] == expected, f"{name}: triangle_count (top={top})"
+        finally:
+            del os.environ["GC_TC_TOP"]
     # every triangle method (hub bitmap, hash probes, binary-search lists, middle-vertex
     # wedges) must agree
     for method in ("hash", "lists", "wedge"):
@@ -49,12 +56,17 @@ def check_triangles(name, device, off, tgt, g):
 
 def check_tc_partitions(name, g, expected, monkeypatch):
     """The bitmap kernel splits the middle vertices into parts (one per XCD or pair of XCDs) and
-    runs short rows whole: every split must give the same count."""
-    for parts in ("1", "2", "3", "8"):
-        for split in ("0", "64", "4294967295"):
-            monkeypatch.setenv("GC_TC_PARTS", parts)
-            monkeypatch.setenv("GC_TC_SPLIT", split)
-            assert g.triangle_count()["total_triangles"] == expected, f"{name}: parts={parts} split={split}"
+    runs short rows whole, and masks the top ids of long lists: every split and mask width must
+    give the same count."""
+    for top in ("0", "128", "1024", "8192"):
+        for parts in ("1", "2", "3", "8"):
+            for split in ("0", "64", "4294967295"):
+                monkeypatch.setenv("GC_TC_TOP", top)
+                monkeypatch.setenv("GC_TC_PARTS", parts)
+                monkeypatch.setenv("GC_TC_SPLIT", split)
+                assert g.triangle_count()["total_triangles"] == expected, \
+                    f"{name}: top={top} parts={parts} split={split}"
+    monkeypatch.delenv("GC_TC_TOP")
     monkeypatch.delenv("GC_TC_PARTS")
     monkeypatch.delenv("GC_TC_SPLIT")
 
