@@ -130,7 +130,7 @@ def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bi
     edge u -> v the list L(v) at its stored width: its entries among the top `hot_bits` ids as
     16-bit offsets in whole 16-byte octs of eight, its other entries at 4 bytes; a list whose
     entries among the top `top_bits` ids fill more octs than their mask (top_bits / 8 bytes)
-    holds the mask instead of those offsets."""
+    holds the mask instead of those offsets (from the mask's oct of id v + 1 on)."""
     top_bits = tc_top_bits() if top_bits is None else top_bits
     off = rl["offsets"].astype(np.int64)
     fwd = rl["fwd"].astype(np.int64)
@@ -150,8 +150,11 @@ def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bi
         top_octs = top_bits // 128
         top_base = max(hot_base, ((n - top_bits) + 31) & ~31 if n > top_bits else 0)
         top = np.bincount(row, weights=(dag_v >= top_base), minlength=n).astype(np.int64)
-        masked = (top >> 3) > top_octs
-        octs = np.where(masked, (hot - top + 7) // 8 + top_octs, octs)
+        # a mask starts at the oct of id max(v + 1, top_base): L(v) holds ids above v only
+        ids = np.arange(n, dtype=np.int64)
+        mask_octs = top_octs - np.where(ids + 1 > top_base, (ids + 1 - top_base) >> 7, 0)
+        masked = (top >> 3) > mask_octs
+        octs = np.where(masked, (hot - top + 7) // 8 + mask_octs, octs)
         masked_rows = int(masked.sum())
     indeg = np.bincount(dag_v, minlength=n).astype(np.int64)
     streamed = int((indeg * (16 * octs + 4 * cold)).sum())
@@ -353,7 +356,7 @@ def main() -> int:
                 "stored-width model: 4(n+1) + 4n, L(u) at 4 bytes once per part it is split into, and "
                 "for every DAG edge u->v the list L(v) at its stored width (entries among the top 32768 "
                 "ids as 16-bit offsets in 16-byte octs of eight, others at 4 bytes, and the entries among "
-                f"the top {tc16['top_bits']} ids as a {tc16['top_bits'] // 8}-byte mask where that is "
+                f"the top {tc16['top_bits']} ids as a mask of at most {tc16['top_bits'] // 8} bytes where that is "
                 f"smaller: {tc16['masked_rows']} lists); the merge model "
                 f"(both lists at 4 bytes, {tc_merge_bytes} bytes) would count data the kernel never reads")
             rooflines[name]["merge_model_bytes"] = tc_merge_bytes

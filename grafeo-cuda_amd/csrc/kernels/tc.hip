@@ -426,13 +426,20 @@ __device__ __forceinline__ uint32_t probe_pair(const uint32_t* L1, uint32_t l1, 
 #define TCB_PAD16 TCB_HOT_BITS
 // Masks of the top ids: a list L(v) whose entries among the top ids (ids from
 // top_base on, top_octs * 128 of them) would take more octs as offsets than
-// their mask takes (top_octs octs) keeps only the entries below top_base as
-// offsets and stores the mask after them; hw[v].y then carries TC_TOP_FLAG and
-// counts the mask's octs. The mask's bits and the row's bitmap words from
-// top_word0 = (top_base - hot_base) / 32 on cover the same ids, so a masked
-// top part adds popcount(mask & bitmap words): 128 bytes per list at
-// top_octs = 8 against two bytes per top entry.
+// their mask takes keeps only the entries below top_base as offsets and stores
+// the mask after them; hw[v].y then carries TC_TOP_FLAG and counts the mask's
+// octs too. L(v) holds ids above v only, so the mask starts at the oct of id
+// max(v + 1, top_base) (tc_mask_first; the octs below it would be zero) and
+// runs to the end of the top ids. The mask's bits and the row's bitmap words
+// from top_word0 = (top_base - hot_base) / 32 on cover the same ids, so a
+// masked top part adds popcount(mask & bitmap words): at most 512 bytes per
+// list at top_octs = 32 against two bytes per top entry.
 #define TC_TOP_FLAG 0x80000000u
+
+// The first stored oct of row v's mask among the top_octs octs of the top ids.
+__device__ __forceinline__ uint32_t tc_mask_first(uint32_t v, uint32_t top_base) {
+  return v + 1 > top_base ? (v + 1 - top_base) >> 7 : 0u;
+}
 
 __device__ __forceinline__ uint32_t hot16_hit(const uint32_t* s_bits, uint32_t y) {
   return word_bit(s_bits[y >> 5], y);
@@ -622,7 +629,7 @@ __device__ __forceinline__ void tc_bitmap_body(
             // kHot16: [st, st + len) are the octs of L(v)'s hot part and
             // [cst, cst + clen) its cold part in targets (walked only when
             // the row has a cold prefix); otherwise all of L(v) in targets
-            uint32_t st = 0, len = 0, cst = 0, clen = 0, mst = 0;
+            uint32_t st = 0, len = 0, cst = 0, clen = 0, mst = 0, mfirst = 0;
             bool masked = false;  // L(v)'s top part is a mask (TC_TOP_FLAG)
             if (lane < nv) {
               const uint32_t v = A[cv + lane];
@@ -633,7 +640,8 @@ __device__ __forceinline__ void tc_bitmap_body(
                   len = h.y & ~TC_TOP_FLAG;
                   if (h.y & TC_TOP_FLAG) {
                     masked = true;
-                    len -= top_octs;
+                    mfirst = tc_mask_first(v, hot_base + 32u * top_word0);
+                    len -= top_octs - mfirst;
                     mst = st + len;
                   }
                   if (c != 0) {
@@ -687,8 +695,9 @@ __device__ __forceinline__ void tc_bitmap_body(
                   }
                 }
                 const uint32_t base = static_cast<uint32_t>(__shfl(static_cast<int>(mst), static_cast<int>(mine)));
-                if (take) {
-                  const uint4 mk = quads[base + chunk];
+                const uint32_t first = static_cast<uint32_t>(__shfl(static_cast<int>(mfirst), static_cast<int>(mine)));
+                if (take && chunk >= first) {
+                  const uint4 mk = quads[base + chunk - first];
                   local += __popc(mk.x & ub.x) + __popc(mk.y & ub.y) + __popc(mk.z & ub.z) + __popc(mk.w & ub.w);
                 }
               }
@@ -770,11 +779,12 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_rows(
     split[v] = sp;
     uint32_t end = b, extra = 0;
     if (top_octs != 0) {
-      // a mask of top_octs octs when the top part would take more octs as offsets
+      // a mask (its octs from tc_mask_first on) when the top part would take more octs as offsets
       const uint32_t ts = lower_bound_u32(targets, sp, b, top_base);
-      if (((b - ts) >> 3) > top_octs) {
+      const uint32_t mask_octs = top_octs - tc_mask_first(static_cast<uint32_t>(v), top_base);
+      if (((b - ts) >> 3) > mask_octs) {
         end = ts;
-        extra = top_octs;
+        extra = mask_octs;
       }
       tsplit[v] = end;
     }
@@ -815,7 +825,7 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
         // a masked top entry: its bit in the mask after the offsets' octs (zeroed beforehand)
         const uint32_t y = targets[i] - top_base;
         uint32_t* mask = reinterpret_cast<uint32_t*>(half + q8 + 8ull * ((end - sp + 7) >> 3));
-        atomicOr(mask + (y >> 5), 1u << (y & 31));
+        atomicOr(mask + ((y >> 5) - 4u * tc_mask_first(v, top_base)), 1u << (y & 31));
       }
     }
     if (i < n) {
@@ -825,7 +835,8 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
       const uint32_t end = top_octs != 0 ? tsplit[v] : row_end;
       const uint32_t len = end - split[v];
       const uint32_t nq = (len + 7) >> 3;
-      hw[v] = end < row_end ? make_uint2(q, (nq + top_octs) | TC_TOP_FLAG) : make_uint2(q, nq);
+      hw[v] = end < row_end ? make_uint2(q, (nq + top_octs - tc_mask_first(v, top_base)) | TC_TOP_FLAG)
+                            : make_uint2(q, nq);
       // the last block's free slots (at most seven) hold the pad
       if (len & 7u) {
         const uint32_t last = (len - 1) >> 9;
