@@ -242,8 +242,10 @@ def test_rmat_small(device, monkeypatch, scale):
 
 def test_bfs_schedule_switches(device, monkeypatch):
     """Distances under other fixed-rule settings against the oracle: bottom-up levels launched
-    ahead or one per read-back (GC_BFS_AHEAD), and Beamer's top-down -> bottom-up threshold at
-    1 (bottom-up as soon as allowed), 14 (the default) and 1000 (almost never), on the catalog,
+    ahead or one per read-back (GC_BFS_AHEAD), Beamer's top-down -> bottom-up threshold at 1
+    (bottom-up as soon as allowed), 14 (the default) and 1000 (almost never), and the bottom-up
+    tail after Beamer's rule would return at 0 and 3 levels (GC_BFS_TAIL; with 0 one traversal
+    can start several bottom-up runs, each clearing the runs' counter sets first), on the catalog,
     a grid (many levels) and R-MAT 14 (hubs). The schedule changes which kernels run, never
     the distances."""
     graphs = [(name, off, tgt) for name, off, tgt in catalog()]
@@ -256,13 +258,16 @@ def test_bfs_schedule_switches(device, monkeypatch):
         expected = {s: oracle.bfs(off, tgt, s) for s in sources}
         for ahead in ("0", "1"):
             for alpha in ("1", "14", "1000"):
-                monkeypatch.setenv("GC_BFS_AHEAD", ahead)
-                monkeypatch.setenv("GC_BFS_ALPHA", alpha)
-                for s in sources:
-                    dist, reach, depth = expected[s]
-                    r = g.bfs_numpy(s)
-                    tag = f"{name} from {s} (ahead={ahead}, alpha={alpha})"
-                    assert np.array_equal(r["distances"], dist), tag
-                    assert (r["reachable"], r["max_depth"]) == (reach, depth), tag
+                for tail in ("0", "3"):
+                    monkeypatch.setenv("GC_BFS_AHEAD", ahead)
+                    monkeypatch.setenv("GC_BFS_ALPHA", alpha)
+                    monkeypatch.setenv("GC_BFS_TAIL", tail)
+                    for s in sources:
+                        dist, reach, depth = expected[s]
+                        r = g.bfs_numpy(s)
+                        tag = f"{name} from {s} (ahead={ahead}, alpha={alpha}, tail={tail})"
+                        assert np.array_equal(r["distances"], dist), tag
+                        assert (r["reachable"], r["max_depth"]) == (reach, depth), tag
     monkeypatch.delenv("GC_BFS_AHEAD")
     monkeypatch.delenv("GC_BFS_ALPHA")
+    monkeypatch.delenv("GC_BFS_TAIL")
