@@ -348,6 +348,7 @@ struct Graph {
   bool tc_hot16_ready = false;
   uint32_t tc_hot16_parts = 0;  // the middle-vertex parts the words are grouped by
   uint32_t tc_hot16_top = 0;    // octs per mask of the top ids in the words (0: no masks)
+  uint32_t pr_merge_push = 0;   // push set size the PageRank row-start bitmap was built for (0: none)
   // The learned triangle-count schedule (method per degree bucket): a function
   // of the graph and the network, kept with the network and version it came from.
   std::vector<uint32_t> tc_choice;

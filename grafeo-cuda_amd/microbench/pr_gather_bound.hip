@@ -6,7 +6,8 @@
 //    flattened into one stream with no row structure: the target stream the kernel reads, each
 //    entry gathered from LDS (the top `hot` ids) or from global memory, with and without the LDS
 //    pushes for the top `push` ids; then with every gather served by LDS, then the stream alone.
-// 3. The library's own pr_spmv_push time per iteration on the same graph.
+// 3. The library's own push-pull SpMV time per iteration on the same graph (pr_spmv_merge, the
+//    default, and pr_spmv_push, the row-tiled kernel, with GC_PR_KERNEL=push).
 //
 // Build and run (repo root, after build.py):
 //   hipcc --offload-arch=gfx950 -O3 -std=c++17 -Xclang -target-feature -Xclang -packed-fp32-ops \
@@ -143,7 +144,7 @@ int main() {
   hipDeviceProp_t prop;
   CK(hipGetDeviceProperties(&prop, 0));
   const int cu = prop.multiProcessorCount;
-  std::printf("%s: %d CUs, L2 %d bytes per XCD\n", prop.name, cu, prop.l2CacheSize);
+  std::printf("%s: %d CUs, L2 %d bytes per XCD\n", prop.gcnArchName, cu, prop.l2CacheSize);
 
   // 1. uniform random gathers, 128 M per launch, 8 workgroups of 256 per CU
   const uint64_t N = 128ull << 20;
@@ -187,7 +188,8 @@ int main() {
   std::vector<uint32_t> off(n + 1ull), tgt(m);
   CK(hipMemcpy(off.data(), roff, 4ull * (n + 1), hipMemcpyDeviceToHost));
   CK(hipMemcpy(tgt.data(), rtgt, 4ull * m, hipMemcpyDeviceToHost));
-  // the library's default LDS split at this size: 8192 pushed hubs, 22528 contributions cached
+  // an LDS split of the push-pull kernels at this size (8192 pushed hubs, 22528 contributions
+  // cached; the default is 6144 and 26624, the same 152 KB)
   const uint32_t push_count = 8192, hot_count = 22528;
   const uint32_t push_base = n - push_count, hot_base = n - hot_count;
   // the entries pr_spmv_push streams: every row's list, but only the part of a pushed hub's list
@@ -240,11 +242,14 @@ int main() {
   GK(gc_pagerank_device(g, 0.85f, 20, 1e-6f, d_rank, &it, &conv, &secs));
   GK(gc_profile_enable(dev, 1));
   GK(gc_pagerank_device(g, 0.85f, 20, 1e-6f, d_rank, &it, &conv, &secs));
-  double total_ms;
-  uint64_t launches;
-  GK(gc_profile_query(dev, "pr_spmv_push", &total_ms, &launches));
-  std::printf("library pr_spmv_push: %.1f us per iteration that ran (%u iterations, %llu launches)\n",
-              1000.0 * total_ms / it, it, static_cast<unsigned long long>(launches));
+  for (const char* name : {"pr_spmv_merge", "pr_spmv_push"}) {
+    double total_ms;
+    uint64_t launches;
+    GK(gc_profile_query(dev, name, &total_ms, &launches));
+    if (launches != 0)
+      std::printf("library %s: %.1f us per iteration that ran (%u iterations, %llu launches)\n", name,
+                  1000.0 * total_ms / it, it, static_cast<unsigned long long>(launches));
+  }
   CK(hipFree(d_rank));
   CK(hipFree(d_stream));
   CK(hipFree(d_table));
