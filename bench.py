@@ -54,8 +54,8 @@ KERNEL_FAMILIES = {
     "bfs": ("bfs_init", "bfs_topdown_edges", "bfs_frontier_degrees", "bfs_frontier_scan_small", "bfs_bottomup",
             "bfs_queue_to_bitmap", "bfs_bitmap_to_queue", "bfs_publish", "bfs_chain_expand", "bfs_chain_advance",
             "bfs_top_build", "bfs_chain"),
-    "msbfs": ("ms_seed", "ms_seed_distances", "ms_topdown", "ms_bottomup_thread", "ms_bottomup_wave",
-              "ms_finalize", "ms_write_distances", "ms_summarize"),
+    "msbfs": ("ms_seed", "ms_seed_levels", "ms_topdown", "ms_bottomup_thread", "ms_bottomup_wave",
+              "ms_finalize", "ms_write_levels", "ms_levels_to_dist", "ms_write_distances", "ms_summarize"),
     "ktruss": ("kt_wedge_support", "kt_truss_frontier", "kt_truss_peel", "kt_truss_commit", "kt_truss_decrement",
                "kt_truss_values", "kt_filter", "kt_support_direct", "kt_reverse_index", "kt_canonical_index",
                "kt_canonical_reverse", "kt_emit_flags_direct", "kt_emit_flags_state"),

@@ -9,6 +9,7 @@
 // Patterson (SC 2012) applied to the union of the frontiers (alpha = 14,
 // beta = 24); non-symmetric CSRs stay top-down.
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -34,6 +35,7 @@ constexpr uint32_t kMaxSources = 128;
 constexpr uint32_t kWaveDegree = 32;  // rows of at least this degree get a wave in bottom-up steps
 constexpr double kAlpha = 14.0;
 constexpr double kBeta = 24.0;
+constexpr int32_t kMaxByteLevel = 254;  // MS_MAX_BYTE_LEVEL of msbfs.hip: 0xFF marks "not reached"
 
 // One traversal of up to 128 sources (caller ids). d_dist holds count rows of
 // n distances; d_reach / d_depth count entries, zeroed here.
@@ -54,7 +56,10 @@ void msbfs_chunk(Graph& g, const uint32_t* sources, uint32_t count, int32_t* d_d
   fill_words(dev, seen, mask_words, 0);
   fill_words(dev, visit, mask_words, 0);
   fill_words(dev, next, mask_words, 0);
-  fill_words(dev, d_dist, static_cast<uint64_t>(count) * n, 0xFFFFFFFFu);
+  // the levels as bytes (ms_write_levels), written out as distances at the end
+  auto* lv = g.buffer<uint8_t>("ms.levels", 128ull * std::max<uint32_t>(n, 1));
+  fill_words(dev, lv, 32ull * n, 0xFFFFFFFFu);
+  bool byte_levels = true;
 
   std::vector<uint32_t> staged(2 * kMaxSources, 0);
   std::vector<uint32_t> frontier;
@@ -71,8 +76,8 @@ void msbfs_chunk(Graph& g, const uint32_t* sources, uint32_t count, int32_t* d_d
   copy_to_device(dev, queue, frontier.data(), sizeof(uint32_t) * frontier.size());
   launch(dev, get_kernel(dev, "msbfs", "ms_seed"), 1u, 64u, 0,
          static_cast<const uint32_t*>(src_dev + kMaxSources), count, seen, visit);
-  launch(dev, get_kernel(dev, "msbfs", "ms_seed_distances"), (count + 63) / 64, 64u, 0,
-         static_cast<const uint32_t*>(src_dev), count, n, d_dist);
+  launch(dev, get_kernel(dev, "msbfs", "ms_seed_levels"), (count + 63) / 64, 64u, 0,
+         static_cast<const uint32_t*>(src_dev + kMaxSources), count, lv);
 
   const unsigned long long active_lo = count >= 64 ? ~0ull : ((1ull << count) - 1);
   const unsigned long long active_hi =
@@ -93,8 +98,17 @@ void msbfs_chunk(Graph& g, const uint32_t* sources, uint32_t count, int32_t* d_d
   Kernel k_buw = get_kernel(dev, "msbfs", "ms_bottomup_wave");
   Kernel k_fin = get_kernel(dev, "msbfs", "ms_finalize");
   Kernel k_dist = get_kernel(dev, "msbfs", "ms_write_distances");
+  Kernel k_lv = get_kernel(dev, "msbfs", "ms_write_levels");
+  Kernel k_lv2d = get_kernel(dev, "msbfs", "ms_levels_to_dist");
   const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
   const uint32_t resident = cu * 8;  // 256-lane workgroups
+  const uint32_t out_blocks = std::min<uint32_t>((n + 255) / 256, resident * 4);
+  // four distances per 16-byte store where n and the caller's buffer allow it
+  const uint32_t vec4 = (n % 4 == 0 && (reinterpret_cast<uintptr_t>(d_dist) & 15u) == 0) ? 1u : 0u;
+  auto levels_to_dist = [&] {
+    launch(dev, k_lv2d, out_blocks, 256u, 0, g.rl.inv.as<const uint32_t>(), n, count,
+           static_cast<const uint8_t*>(lv), d_dist, vec4);
+  };
   uint64_t n_f = frontier.size();
   uint64_t m_u = static_cast<uint64_t>(g.m) - std::min<uint64_t>(g.m, m_f);
   bool top_down = true;
@@ -131,12 +145,24 @@ void msbfs_chunk(Graph& g, const uint32_t* sources, uint32_t count, int32_t* d_d
     std::memcpy(&c, dev.pinned, sizeof(c));
     if (c.frontier_vertices == 0) break;
     ++level;
-    launch(dev, k_dist, std::min<uint32_t>((n + 255) / 256, resident * 4), 256u, 0,
-           g.rl.inv.as<const uint32_t>(), n, static_cast<const SourceMask*>(visit), level, d_dist);
+    if (byte_levels && level > kMaxByteLevel) {
+      // deeper than a byte holds: the levels so far become distances, and later
+      // levels write distances directly
+      levels_to_dist();
+      byte_levels = false;
+    }
+    if (byte_levels) {
+      launch(dev, k_lv, out_blocks, 256u, 0, n, static_cast<const SourceMask*>(visit),
+             static_cast<uint32_t>(level), lv);
+    } else {
+      launch(dev, k_dist, out_blocks, 256u, 0, g.rl.inv.as<const uint32_t>(), n,
+             static_cast<const SourceMask*>(visit), level, d_dist);
+    }
     n_f = c.frontier_vertices;
     m_f = c.frontier_edges;
     m_u = c.unexplored_edges;
   }
+  if (byte_levels) levels_to_dist();
   fill_words(dev, d_reach, count, 0);
   fill_words(dev, d_depth, count, 0);
   const uint32_t per_source = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 64));

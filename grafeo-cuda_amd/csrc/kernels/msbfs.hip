@@ -64,13 +64,6 @@ extern "C" __global__ __launch_bounds__(64) void ms_seed(const uint32_t* __restr
   }
 }
 
-// distances[s * n + source_s] = 0 for every source (caller's ids).
-extern "C" __global__ __launch_bounds__(64) void ms_seed_distances(
-    const uint32_t* __restrict__ sources, uint32_t count, uint32_t n, int32_t* __restrict__ dist) {
-  const uint32_t s = threadIdx.x + blockIdx.x * blockDim.x;
-  if (s < count) dist[static_cast<uint64_t>(s) * n + sources[s]] = 0;
-}
-
 // Top-down: one wave per frontier vertex, lanes over its list.
 extern "C" __global__ __launch_bounds__(256) void ms_topdown(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
@@ -245,6 +238,101 @@ extern "C" __global__ __launch_bounds__(256) void ms_write_distances(
       m.hi &= m.hi - 1;
       dist[static_cast<uint64_t>(s) * n + u] = level;
     }
+  }
+}
+
+// The distances as levels in a byte matrix lv[v * 128 + s] (relabelled v,
+// 0xFF = not reached yet), set at each level by one lane per vertex for the
+// sources that reached it there: a vertex's bytes share one 128-byte line, so
+// a level's stores land on one line per newly reached vertex instead of one
+// per (source, vertex) pair scattered over 128 rows of caller ids (2.5 of 10.4
+// ms at R-MAT 22 with 128 sources). ms_levels_to_dist then writes every row
+// once, coalesced, in caller ids; traversals deeper than MS_MAX_BYTE_LEVEL
+// levels switch to ms_write_distances from there on.
+#define MS_MAX_BYTE_LEVEL 254
+
+extern "C" __global__ __launch_bounds__(256) void ms_write_levels(uint32_t n, const ulonglong2* __restrict__ visit,
+                                                                  uint32_t level, uint8_t* __restrict__ lv) {
+  // eight lanes per vertex, sixteen bytes (sources 16c .. 16c + 15) each: a wave rewrites
+  // eight consecutive 128-byte rows, only the chunks that gained a source
+  const uint32_t c = lane_id() & 7u;
+  const uint64_t groups = grid_threads() / 8;
+  for (uint64_t vv = global_thread() / 8; vv < n; vv += groups) {
+    const Mask m = load_mask(&visit[vv]);
+    const uint32_t bits = static_cast<uint32_t>(((c < 4 ? m.lo : m.hi) >> (16u * (c & 3u))) & 0xFFFFu);
+    if (bits == 0) continue;
+    uint4* chunk = reinterpret_cast<uint4*>(lv + 128ull * vv) + c;
+    uint4 q = *chunk;
+    uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) {
+      if ((bits >> b) & 1u) {
+        const uint32_t sh = 8u * (b & 3u);
+        w[b >> 2] = (w[b >> 2] & ~(0xFFu << sh)) | (level << sh);
+      }
+    }
+    *chunk = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// lv bytes of the sources at level 0 (relabelled ids in sources_rl).
+extern "C" __global__ __launch_bounds__(64) void ms_seed_levels(const uint32_t* __restrict__ sources_rl,
+                                                                uint32_t count, uint8_t* __restrict__ lv) {
+  const uint32_t s = threadIdx.x + blockIdx.x * blockDim.x;
+  if (s < count) lv[128ull * sources_rl[s] + s] = 0;
+}
+
+// dist[s * n + u] for every source s < count and caller vertex u, from the
+// bytes of relabelled vertex inv[u] (0xFF: -1). With `vec4` (n a multiple of 4
+// and dist 16-byte aligned, checked by the host), a lane takes four consecutive caller vertices and writes the four distances of each
+// source with one 16-byte store (the sources in two halves of 64, 64 bytes of
+// each row in registers); otherwise one lane per vertex and 4-byte stores.
+__device__ __forceinline__ int32_t ms_level_dist(uint32_t word, uint32_t s) {
+  const uint32_t b = (word >> (8u * (s & 3u))) & 0xFFu;
+  return b == 0xFFu ? -1 : static_cast<int32_t>(b);
+}
+
+extern "C" __global__ __launch_bounds__(256) void ms_levels_to_dist(const uint32_t* __restrict__ inv, uint32_t n,
+                                                                    uint32_t count, const uint8_t* __restrict__ lv,
+                                                                    int32_t* __restrict__ dist, uint32_t vec4) {
+  if (vec4) {
+    const uint64_t quads = n / 4;
+    for (uint64_t qq = global_thread(); qq < quads; qq += grid_threads()) {
+      const uint64_t u0 = 4 * qq;
+      const uint32_t* rows[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) rows[j] = reinterpret_cast<const uint32_t*>(lv + 128ull * inv[u0 + j]);
+#pragma unroll
+      for (uint32_t half = 0; half < 2; ++half) {
+        if (64 * half >= count) break;
+        uint32_t w[4][16];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint4* r = reinterpret_cast<const uint4*>(rows[j] + 16 * half);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint4 x = r[k];
+            w[j][4 * k] = x.x;
+            w[j][4 * k + 1] = x.y;
+            w[j][4 * k + 2] = x.z;
+            w[j][4 * k + 3] = x.w;
+          }
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < 64; ++t) {
+          const uint32_t s = 64 * half + t;
+          if (s >= count) break;
+          const int4 d = make_int4(ms_level_dist(w[0][t >> 2], t), ms_level_dist(w[1][t >> 2], t),
+                                   ms_level_dist(w[2][t >> 2], t), ms_level_dist(w[3][t >> 2], t));
+          *reinterpret_cast<int4*>(dist + static_cast<uint64_t>(s) * n + u0) = d;
+        }
+      }
+    }
+    return;
+  }
+  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(lv + 128ull * inv[uu]);
+    for (uint32_t s = 0; s < count; ++s) dist[static_cast<uint64_t>(s) * n + uu] = ms_level_dist(row[s >> 2], s);
   }
 }
 

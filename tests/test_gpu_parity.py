@@ -271,3 +271,14 @@ def test_bfs_schedule_switches(device, monkeypatch):
     monkeypatch.delenv("GC_BFS_AHEAD")
     monkeypatch.delenv("GC_BFS_ALPHA")
     monkeypatch.delenv("GC_BFS_TAIL")
+
+
+def test_msbfs_deep_graphs(device):
+    """Multi-source BFS deeper than 254 levels: the levels kept as bytes are written out as
+    distances at level 255 and the later levels written directly (a 600-vertex path, depth up to
+    599, and a 300 x 2 grid, depth up to 300), against the oracle."""
+    path = [(i, i + 1) for i in range(599)]
+    for name, (off, tgt) in (("path 600", undirected(600, path)), ("grid 300x2", grid_graph(300, 2))):
+        g = upload(device, off, tgt)
+        n = len(off) - 1
+        check_msbfs(name, off, tgt, g, [0, n // 3, n // 2, n - 1])
