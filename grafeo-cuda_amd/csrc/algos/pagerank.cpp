@@ -10,8 +10,11 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <stdexcept>
 
 #include "../runtime/common.hpp"
+#include "pr_bin_layout.hpp"
 
 namespace gc {
 
@@ -129,7 +132,7 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
 // source-partitioned pass, one part of the sources per XCD with eight FP32
 // partial sums per row, halved the L2 requests but was no faster: 12.1
 // against 10.2-10.5 ms.)
-enum class PrKernel { kPush, kMerge, kHot, kRows };
+enum class PrKernel { kPush, kMerge, kBin, kHot, kRows };
 
 static PrKernel choose_kernel(const Graph& g) {
   const char* env = std::getenv("GC_PR_KERNEL");
@@ -137,7 +140,8 @@ static PrKernel choose_kernel(const Graph& g) {
   if (env != nullptr && std::strcmp(env, "hot") == 0) return PrKernel::kHot;
   if (!g.symmetric) return PrKernel::kHot;
   if (env != nullptr && std::strcmp(env, "push") == 0) return PrKernel::kPush;
-  return PrKernel::kMerge;
+  if (env != nullptr && std::strcmp(env, "merge") == 0) return PrKernel::kMerge;
+  return PrKernel::kBin;
 }
 
 static uint32_t env_u32(const char* name, uint32_t fallback) {
@@ -145,20 +149,115 @@ static uint32_t env_u32(const char* name, uint32_t fallback) {
   return env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : fallback;
 }
 
-// Push-pull iterations on a symmetric graph (pr_spmv_merge and
-// pr_merge_finish, or pr_spmv_push and pr_push_finish; then pr_finalize; the
-// kernels' comments give the method). One 1024-lane workgroup per CU holds
-// push_count 8-byte accumulators and hot_count FP32 contributions in LDS:
-// 6,144 and 26,624 (152 KB of the 160 KB). At R-MAT scale 22 that leaves
-// 36.5% of the entries to gather from global memory (58% for the pull
-// kernel's 19,456 hot ids) and skips 21% of the target stream (the hubs'
-// entries outside the push set). 8,192 and 22,528 measured 385 us per
-// iteration (merge) and 423 us (push) there. GC_PR_PUSH and GC_PR_HOT
-// override the two sizes.
+// The binned layout of the cold entries (pr_bin_layout.hpp) for the push set
+// [n - push_count, n) and the hot set [n - hot_count, n) of a prepared
+// symmetric graph: built on the host from the relabelled CSR (copied back
+// once), uploaded, and kept with the graph until a call asks for other sizes.
+static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, uint32_t stage, uint32_t bin_rows,
+                           uint32_t bin_values) {
+  if (g.pr_bin && g.pr_bin->push_count == push_count && g.pr_bin->hot_count == hot_count &&
+      g.pr_bin->stage == stage && g.pr_bin->bin_rows == bin_rows && g.pr_bin->bin_values == bin_values)
+    return *g.pr_bin;
+  g.pr_bin.reset();
+  Stopwatch sw;
+  Device& dev = *g.dev;
+  std::vector<uint32_t> tgt(std::max<uint32_t>(g.m, 1));
+  if (g.m > 0) copy_to_host(dev, tgt.data(), g.rl.targets.get(), sizeof(uint32_t) * g.m);
+  PrBinLayout h;
+  try {
+    h = build_pr_bin_layout(g.rl.host_offsets.data(), tgt.data(), g.n, g.rl.dangling, push_count, hot_count,
+                            stage, bin_rows, bin_values,
+                            [](uint64_t count, const std::function<void(uint64_t, uint64_t)>& fn) {
+                              parallel_for(count, fn, 1);
+                            });
+  } catch (const std::runtime_error& e) {
+    fail_kernel(e.what());
+  }
+  std::vector<uint32_t>().swap(tgt);
+  auto plan = std::unique_ptr<PrBinPlan>(new PrBinPlan());
+  PrBinPlan& p = *plan;
+  p.push_count = push_count;
+  p.hot_count = hot_count;
+  p.stage = stage;
+  p.bin_rows = bin_rows;
+  p.bin_values = bin_values;
+  p.zero_rows = h.zero_rows;
+  p.push_base = h.push_base;
+  p.hot_base = h.hot_base;
+  p.hot_entries = static_cast<uint32_t>(h.hot16.size());
+  p.chunks = static_cast<uint32_t>((static_cast<uint64_t>(p.hot_entries) + 511) / 512);
+  p.subs = h.subs();
+  p.bins = h.bins();
+  p.values = h.bin_val.empty() ? 0 : h.bin_val.back();
+  p.cold_entries = h.cold_entries;
+  p.slot_rows = 8ull * h.block_group.size();
+  auto up = [&](DeviceMemory& m, const void* src, size_t bytes) {
+    m.allocate(std::max<size_t>(bytes, 16));
+    if (bytes > 0) copy_to_device(dev, m.get(), src, bytes);
+  };
+  // the 16-bit stream padded to whole 512-entry chunks with entries that read the zero
+  h.hot16.resize(static_cast<size_t>(p.chunks) * 512, static_cast<uint16_t>(hot_count));
+  up(p.hot16, h.hot16.data(), sizeof(uint16_t) * h.hot16.size());
+  up(p.hoff, h.hoff.data(), sizeof(uint32_t) * h.hoff.size());
+  up(p.slots, h.slots.data(), sizeof(uint16_t) * h.slots.size());
+  up(p.block_group, h.block_group.data(), sizeof(uint32_t) * h.block_group.size());
+  up(p.sub, h.sub.data(), sizeof(uint32_t) * h.sub.size());
+  up(p.run_first, h.run_first.data(), sizeof(uint32_t) * h.run_first.size());
+  up(p.run_info, h.run_info.data(), sizeof(uint32_t) * h.run_info.size());
+  up(p.run_dst, h.run_dst.data(), sizeof(uint32_t) * h.run_dst.size());
+  up(p.bin_row, h.bin_row.data(), sizeof(uint32_t) * h.bin_row.size());
+  up(p.bin_val, h.bin_val.data(), sizeof(uint32_t) * h.bin_val.size());
+  up(p.uloc, h.uloc.data(), sizeof(uint16_t) * h.uloc.size());
+  up(p.hub_pieces, h.hub_pieces.data(), sizeof(uint32_t) * h.hub_pieces.size());
+  p.pieces = h.pieces();
+  p.runs = h.run_info.size();
+  for (uint32_t i = 0; i < p.pieces; ++i) {
+    const uint32_t row = h.hub_pieces[2 * i], b = h.hub_pieces[2 * i + 1];
+    p.piece_entries += std::min(b + 512u, g.rl.host_offsets[row + 1]) - b;
+  }
+  const uint64_t vals = std::max<uint64_t>(p.values, 8);
+  p.vals.allocate(sizeof(float) * vals);
+  fill_words(dev, p.vals.get(), vals, 0);  // the padding values stay 0
+  const uint32_t pull = p.push_base - p.zero_rows;
+  p.cold_sum.allocate(sizeof(double) * std::max<uint32_t>(pull, 1));
+  fill_words(dev, p.cold_sum.get(), 2ull * std::max<uint32_t>(pull, 1), 0);  // stays 0 without cold entries
+  const uint64_t words = std::max<uint64_t>(1, static_cast<uint64_t>(p.chunks) * 16);  // whole chunks
+  p.starts.allocate(sizeof(uint32_t) * words);
+  fill_words(dev, p.starts.get(), words, 0);
+  p.chunk_row.allocate(sizeof(uint32_t) * std::max<uint32_t>(p.chunks, 1));
+  if (pull > 0) {
+    const uint32_t* d_hoff = p.hoff.as<const uint32_t>();
+    Grid1D sgrid = thread_per_item(pull);
+    launch(dev, get_kernel(dev, "pagerank", "pr_merge_starts"), sgrid.blocks, sgrid.threads, 0, d_hoff,
+           p.zero_rows, p.push_base, p.starts.as<uint32_t>());
+    Grid1D cgrid = thread_per_item(p.chunks);
+    launch(dev, get_kernel(dev, "pagerank", "pr_merge_chunks"), cgrid.blocks, cgrid.threads, 0, d_hoff,
+           p.zero_rows, p.push_base, p.chunks, p.chunk_row.as<uint32_t>());
+  }
+  dev.sync();
+  p.build_seconds = sw.seconds();
+  g.pr_bin = std::move(plan);
+  return *g.pr_bin;
+}
+
+// Push-pull iterations on a symmetric graph (pr_spmv_bin with pr_bin_scatter
+// and pr_bin_accum before it, or pr_spmv_merge, both then pr_merge_finish; or
+// pr_spmv_push and pr_push_finish; then pr_finalize; the kernels' comments give
+// the method). One 1024-lane workgroup per CU holds push_count 8-byte
+// accumulators and hot_count FP32 contributions in LDS: 6,144 and 26,624 (152
+// KB of the 160 KB). At R-MAT scale 22 that leaves 36.5% of the entries cold
+// (below the hot set: binned, or gathered from global memory by the merge and
+// push kernels; 58% for the pull kernel's 19,456 hot ids) and skips 21% of the
+// target stream (the hubs' entries outside the push set). 8,192 and 22,528
+// measured 385 us per iteration (merge) and 423 us (push) there. GC_PR_PUSH
+// and GC_PR_HOT override the two sizes; GC_PR_STAGE (staged entries per
+// scatter workgroup, at least 512), GC_PR_BIN_ROWS and GC_PR_BIN_VALUES (rows
+// and values per accumulation bin) the binned layout's.
 void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float tolerance, float* d_out,
-                   uint32_t* iterations, int32_t* converged, bool merge) {
+                   uint32_t* iterations, int32_t* converged, PrKernel kind) {
   Device& dev = *g.dev;
   const uint32_t n = g.n;
+  const bool merge = kind == PrKernel::kMerge, bin = kind == PrKernel::kBin;
   const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
   // workgroups per CU (GC_PR_WGS, 1 or 2): each holds its own sets in its share of the LDS
   const uint32_t wgs = std::min(2u, std::max(1u, env_u32("GC_PR_WGS", 1u)));
@@ -166,14 +265,17 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   // GC_PR_ROWS; pr_spmv_push's tiles have this many slices). Measured at R-MAT scale 22,
   // one box: 357 us per launch with 1 row (420 per iteration that ran), 382 with 2.
   const uint32_t rows_per_group = env_u32("GC_PR_ROWS", 1u) >= 2 ? 2u : 1u;
-  const size_t lds_cap = std::min<size_t>(dev.max_lds_per_block, 160 * 1024) / wgs - 1024;  // static LDS
-  // at most 1/3 of the LDS for the accumulators, the rest for the contributions
+  const size_t lds_limit = std::min<size_t>(dev.max_lds_per_block, 160 * 1024);
+  // less the static LDS (pr_spmv_bin: 8 KB of staged row totals besides the partials)
+  const size_t lds_cap = lds_limit / wgs - (bin ? 9216 : 1024);
+  // at most 1/3 of the LDS for the accumulators, the rest for the contributions (and the
+  // binned stream's zero)
   const uint32_t push_cap = static_cast<uint32_t>(lds_cap / 3 / 8);
   const uint32_t push_count = std::min({n, std::max(1u, env_u32("GC_PR_PUSH", 6144u)), push_cap});
-  const uint32_t hot_cap = static_cast<uint32_t>((lds_cap - 8ull * push_count) / 4);
+  const uint32_t hot_cap = static_cast<uint32_t>((lds_cap - 8ull * push_count) / 4) - 1u;
   const uint32_t hot_count = std::min({n, std::max(push_count, env_u32("GC_PR_HOT", 26624u)), hot_cap});
   const uint32_t push_base = n - push_count;
-  const uint32_t shmem = static_cast<uint32_t>(8ull * push_count + 4ull * hot_count);
+  const uint32_t shmem = static_cast<uint32_t>(8ull * push_count + 4ull * (hot_count + (bin ? 1u : 0u)));
   const PrSegments segs = plan_segments(g.rl.host_offsets, push_base, kPrHotThreads, false,
                                         env_u32("GC_PR_SHIFT", 2u), rows_per_group);
   const uint32_t push_tiles = (push_count + 15) / 16;  // 16 hub rows (one wave each) per tile
@@ -181,9 +283,21 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   const uint32_t hub_blocks = (push_count + 15) / 16;
   // the nonzero-split pull: rows [zero_rows, push_base) own the entries [0, entries)
   const uint32_t zero_rows = std::min(g.rl.dangling, push_base);
-  const uint32_t entries = merge ? g.rl.host_offsets[push_base] : 0u;
-  if (merge && g.rl.host_offsets[zero_rows] != 0)
+  if ((merge || bin) && g.rl.host_offsets[zero_rows] != 0)
     fail_kernel("PageRank: the degree-ordered rows do not start with the rows of degree 0");
+  PrBinPlan* plan = nullptr;
+  if (bin) {
+    // Defaults measured at R-MAT scale 22 (per launch, scatter + accumulation): 36,864 staged
+    // entries and 65,536 values per bin 136 us; 40,448 and 196,608 124 us (longer runs per bin).
+    const uint32_t stage = std::max(512u, std::min(env_u32("GC_PR_STAGE", 40448u),
+                                                   static_cast<uint32_t>(lds_limit / 4 - 1)));
+    const uint32_t bin_rows = std::max(1u, std::min(env_u32("GC_PR_BIN_ROWS", 16384u),
+                                                    static_cast<uint32_t>(lds_limit / 8 - 1)));
+    const uint32_t bin_values = std::max(8u, env_u32("GC_PR_BIN_VALUES", 196608u));
+    plan = &bin_plan(g, push_count, hot_count, stage, bin_rows, bin_values);
+    plan->grid = cu * wgs;
+  }
+  const uint32_t entries = bin ? plan->hot_entries : (merge ? g.rl.host_offsets[push_base] : 0u);
   const uint32_t chunks = static_cast<uint32_t>((static_cast<uint64_t>(entries) + 511) / 512);
   const uint32_t fix_blocks = (chunks + 255) / 256;
   const uint32_t finish_blocks = hub_blocks + fix_blocks;
@@ -204,14 +318,20 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   double* head_partial = nullptr;
   double* tail_partial = nullptr;
   uint32_t* tail_row = nullptr;
-  if (merge) {
-    const uint64_t words = std::max<uint64_t>(1, (static_cast<uint64_t>(entries) + 31) / 32);
-    starts = g.buffer<uint32_t>("pr.starts", sizeof(uint32_t) * words);
+  if (merge || bin) {
     const size_t nc = std::max<uint32_t>(1, chunks);
-    chunk_row = g.buffer<uint32_t>("pr.chunk_row", sizeof(uint32_t) * nc);
     head_partial = g.buffer<double>("pr.head_partial", sizeof(double) * nc);
     tail_partial = g.buffer<double>("pr.tail_partial", sizeof(double) * nc);
     tail_row = g.buffer<uint32_t>("pr.tail_row", sizeof(uint32_t) * nc);
+  }
+  if (merge) {
+    const uint64_t words = std::max<uint64_t>(1, (static_cast<uint64_t>(entries) + 31) / 32);
+    starts = g.buffer<uint32_t>("pr.starts", sizeof(uint32_t) * words);
+    chunk_row = g.buffer<uint32_t>("pr.chunk_row", sizeof(uint32_t) * std::max<uint32_t>(1, chunks));
+  }
+  if (bin) {
+    starts = plan->starts.as<uint32_t>();
+    chunk_row = plan->chunk_row.as<uint32_t>();
   }
 
   const float r0 = 1.0f / static_cast<float>(n);
@@ -220,8 +340,9 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   Grid1D igrid = thread_per_item(n);
   launch(dev, get_kernel(dev, "pagerank", "pr_init"), igrid.blocks, igrid.threads, 0, off, n, r0, rank[0],
          contrib[0]);
-  launch(dev, get_kernel(dev, "pagerank", "pr_push_split"), (push_count + 255) / 256, 256u, 0, off, tgt, n,
-         push_count, psplit);
+  if (!bin)  // the binned layout holds the hubs' entries inside the push set as pieces
+    launch(dev, get_kernel(dev, "pagerank", "pr_push_split"), (push_count + 255) / 256, 256u, 0, off, tgt, n,
+           push_count, psplit);
   // the row-start bitmap and the chunks' first rows depend on the graph and the push set only:
   // built once per graph (78 us per call at R-MAT 22 when rebuilt each time)
   if (merge && chunks > 0 && g.pr_merge_push != push_count) {
@@ -238,16 +359,44 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   PrState init{static_cast<double>(g.rl.dangling) * static_cast<double>(r0), 0.0, 0, 0};
   copy_to_device(dev, state, &init, sizeof(init));
 
-  Kernel spmv = get_kernel(dev, "pagerank", merge ? "pr_spmv_merge" : "pr_spmv_push");
-  Kernel finish = get_kernel(dev, "pagerank", merge ? "pr_merge_finish" : "pr_push_finish");
+  const char* spmv_name = bin ? "pr_spmv_bin" : (merge ? "pr_spmv_merge" : "pr_spmv_push");
+  Kernel spmv = get_kernel(dev, "pagerank", spmv_name);
+  Kernel finish = get_kernel(dev, "pagerank", merge || bin ? "pr_merge_finish" : "pr_push_finish");
   Kernel finalize = get_kernel(dev, "pagerank", "pr_finalize");
+  Kernel scatter{}, accum{};
+  if (bin) {
+    scatter = get_kernel(dev, "pagerank", "pr_bin_scatter");
+    accum = get_kernel(dev, "pagerank", "pr_bin_accum");
+  }
   const double d = static_cast<double>(damping);
   const double tol = static_cast<double>(tolerance);
   PrState host_state = init;
   for (uint32_t it = 0; it < max_iterations; ++it) {
     const int in = it & 1, out = in ^ 1;
     hipError_t err;
-    if (merge) {
+    if (bin) {
+      if (plan->cold_entries > 0) {
+        err = try_launch(dev, scatter, plan->subs, kPrHotThreads, 4u * (plan->stage + 1),
+                         static_cast<const float*>(contrib[in]), plan->zero_rows, plan->hot_base,
+                         plan->slots.as<const uint16_t>(), plan->block_group.as<const uint32_t>(),
+                         plan->sub.as<const uint32_t>(), plan->run_first.as<const uint32_t>(),
+                         plan->run_info.as<const uint32_t>(), plan->run_dst.as<const uint32_t>(),
+                         plan->vals.as<float>(), static_cast<const PrState*>(state));
+        if (err != hipSuccess) fail_kernel(std::string("pr_bin_scatter launch failed: ") + hipGetErrorString(err));
+        err = try_launch(dev, accum, plan->bins, kPrHotThreads, 8u * (plan->bin_rows + 1),
+                         plan->vals.as<const float>(), plan->uloc.as<const uint16_t>(),
+                         plan->bin_row.as<const uint32_t>(), plan->bin_val.as<const uint32_t>(), plan->zero_rows,
+                         plan->cold_sum.as<double>(), static_cast<const PrState*>(state));
+        if (err != hipSuccess) fail_kernel(std::string("pr_bin_accum launch failed: ") + hipGetErrorString(err));
+      }
+      err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, plan->hot16.as<const uint16_t>(),
+                       plan->cold_sum.as<const double>(), static_cast<const float*>(contrib[in]),
+                       static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d,
+                       static_cast<const PrState*>(state), hot_count, push_count,
+                       plan->hub_pieces.as<const uint32_t>(), plan->pieces, chunks, zero_rows,
+                       reinterpret_cast<const uint8_t*>(starts), static_cast<const uint32_t*>(chunk_row),
+                       head_partial, tail_partial, tail_row, push_partials, partials);
+    } else if (merge) {
       err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, static_cast<const float*>(contrib[in]),
                        static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d,
                        static_cast<const PrState*>(state), hot_count, push_count,
@@ -260,9 +409,8 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
                        static_cast<const PrState*>(state), hot_count, push_count,
                        static_cast<const uint32_t*>(psplit), push_tiles, rows_per_group, push_partials, partials);
     }
-    if (err != hipSuccess)
-      fail_kernel(std::string(merge ? "pr_spmv_merge" : "pr_spmv_push") + " launch failed: " + hipGetErrorString(err));
-    if (merge) {
+    if (err != hipSuccess) fail_kernel(std::string(spmv_name) + " launch failed: " + hipGetErrorString(err));
+    if (merge || bin) {
       launch(dev, finish, finish_blocks, 256u, 0, off, static_cast<const float*>(rank[in]), contrib[out],
              rank[out], n, d, static_cast<const PrState*>(state), push_count,
              static_cast<const unsigned long long*>(push_partials), grid, hub_blocks, chunks,
@@ -296,8 +444,8 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   prepare(g);
   const uint32_t n = g.n;
   const PrKernel kind = choose_kernel(g);
-  if (kind == PrKernel::kPush || kind == PrKernel::kMerge) {
-    pagerank_push(g, damping, max_iterations, tolerance, d_out, iterations, converged, kind == PrKernel::kMerge);
+  if (kind == PrKernel::kPush || kind == PrKernel::kMerge || kind == PrKernel::kBin) {
+    pagerank_push(g, damping, max_iterations, tolerance, d_out, iterations, converged, kind);
     return;
   }
   const bool rows_kernel = kind == PrKernel::kRows;
@@ -467,6 +615,25 @@ gc_status gc_pagerank_device(gc_graph* h, float damping, uint32_t max_iterations
     if (iterations) *iterations = it;
     if (converged) *converged = conv;
     if (secs) *secs = sw.seconds();
+  });
+}
+
+gc_status gc_pagerank_layout(gc_graph* h, uint64_t* stats, uint32_t count) {
+  return guard([&] {
+    gc::GraphCall call(h);
+    if (stats == nullptr && count > 0) gc::fail_kernel("stats is null");
+    const gc::PrBinPlan* p = call.g->pr_bin.get();
+    uint64_t v[19] = {};
+    if (p != nullptr) {
+      const uint64_t vals[19] = {p->push_count, p->hot_count, p->stage, p->bin_rows, p->bin_values,
+                                 p->cold_entries, p->values, p->slot_rows, p->hot_entries, p->chunks,
+                                 p->subs, p->bins, p->runs, p->pieces, p->piece_entries,
+                                 static_cast<uint64_t>(p->push_base - p->zero_rows),
+                                 static_cast<uint64_t>(p->hot_base > p->zero_rows ? p->hot_base - p->zero_rows : 0),
+                                 static_cast<uint64_t>(p->build_seconds * 1e9), p->grid};
+      std::memcpy(v, vals, sizeof(v));
+    }
+    for (uint32_t i = 0; i < count && i < 19; ++i) stats[i] = v[i];
   });
 }
 

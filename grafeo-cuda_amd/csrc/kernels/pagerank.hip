@@ -38,6 +38,25 @@ struct PrSegments {
   uint32_t total_blocks;
 };
 
+// dst[i] = src[i] for i < count (count >= 1) by the `threads` lanes of a
+// workgroup: U loads per lane issued before their LDS stores, clamped rather
+// than branched, so a pass costs one memory round trip (a plain loop waits for
+// each load before its store: about 26 round trips for the hot sets here).
+template <int U>
+__device__ __forceinline__ void lds_fill_f32(float* dst, const float* __restrict__ src, uint32_t count,
+                                             uint32_t threads) {
+  for (uint32_t base = 0; base < count; base += threads * U) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src[min(base + u * threads + threadIdx.x, count - 1)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = base + u * threads + threadIdx.x;
+      if (i < count) dst[i] = v[u];
+    }
+  }
+}
+
 struct PrState {
   double dangling;     // D of the current input ranks
   double diff;         // L1 change of the last iteration
@@ -389,7 +408,7 @@ extern "C" __global__ __launch_bounds__(PRH_THREADS) void pr_spmv_hot(
   __shared__ double s_part[2][PRH_THREADS / GC_WAVE];
   if (state->done) return;
   const uint32_t hot_base = n - hot_count;
-  for (uint32_t i = threadIdx.x; i < hot_count; i += PRH_THREADS) s_hot[i] = contrib_in[hot_base + i];
+  if (hot_count > 0) lds_fill_f32<16>(s_hot, contrib_in + hot_base, hot_count, PRH_THREADS);
   const double inv_n = 1.0 / static_cast<double>(n);
   const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
   double dangling = 0.0, diff = 0.0;
@@ -643,7 +662,7 @@ extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_push(
   if (state->done) return;
   const uint32_t hot_base = n - hot_count;
   const uint32_t push_base = n - push_count;
-  for (uint32_t i = threadIdx.x; i < hot_count; i += PRP_THREADS) s_hot[i] = contrib_in[hot_base + i];
+  lds_fill_f32<16>(s_hot, contrib_in + hot_base, hot_count, PRP_THREADS);
   for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) s_push[i] = 0ull;
   const double inv_n = 1.0 / static_cast<double>(n);
   const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
@@ -974,70 +993,51 @@ __device__ __forceinline__ void prm_chunk(uint32_t c, uint32_t entries, const ui
   }
 }
 
-// Waves take items round-robin over the whole grid: the push_count hub rows
-// (pushes of c[h] into the hubs of its list inside P, as pr_spmv_push), then
-// the chunks, then the rows of degree 0 in runs of 512 (eight per lane: rank
-// = base, contribution 0, dangling). Per-workgroup dangling and L1 partials
-// and LDS accumulators go out as in pr_spmv_push.
-extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_merge(
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
-    const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
-    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
-    const PrState* __restrict__ state, uint32_t hot_count, uint32_t push_count,
-    const uint32_t* __restrict__ psplit, uint32_t entries, uint32_t chunks, uint32_t zero_rows,
-    const uint8_t* __restrict__ starts, const uint32_t* __restrict__ chunk_row,
-    double* __restrict__ head_partial, double* __restrict__ tail_partial, uint32_t* __restrict__ tail_row,
-    unsigned long long* __restrict__ push_partials, double* __restrict__ partials) {
-  extern __shared__ unsigned long long s_dyn[];
-  unsigned long long* s_push = s_dyn;
-  float* s_hot = reinterpret_cast<float*>(s_dyn + push_count);
-  __shared__ double s_part[2][PRP_THREADS / GC_WAVE];
-  if (state->done) return;
-  const uint32_t hot_base = n - hot_count;
-  const uint32_t push_base = n - push_count;
-  for (uint32_t i = threadIdx.x; i < hot_count; i += PRP_THREADS) s_hot[i] = contrib_in[hot_base + i];
-  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) s_push[i] = 0ull;
-  const double inv_n = 1.0 / static_cast<double>(n);
-  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
-  double dangling = 0.0, diff = 0.0;
-  __syncthreads();
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GC_WAVE);
-  const uint32_t waves = gridDim.x * (PRP_THREADS / GC_WAVE);
-  const uint32_t zero_items = (zero_rows + PRM_CHUNK - 1) / PRM_CHUNK;
-  const uint32_t items = push_count + chunks + zero_items;
-  for (uint32_t item = blockIdx.x * (PRP_THREADS / GC_WAVE) + wave; item < items; item += waves) {
-    if (item < push_count) {
-      const uint32_t h = n - 1 - item;
-      const unsigned long long cfix = pr_to_fixed(s_hot[h - hot_base]);
-      const uint32_t e = offsets[h + 1];
-      uint32_t i = psplit[h - push_base] + lane_id();
-      for (; i + 7 * GC_WAVE < e; i += 8 * GC_WAVE) {
-        uint32_t t[8];
+// Hub row h (one wave): push c[h] into the hubs of its list inside P (from
+// psplit, its first such entry), as pr_spmv_push does.
+__device__ __forceinline__ void prm_hub_item(uint32_t h, const uint32_t* __restrict__ offsets,
+                                             const uint32_t* __restrict__ targets,
+                                             const uint32_t* __restrict__ psplit, const float* __restrict__ s_hot,
+                                             uint32_t hot_base, unsigned long long* __restrict__ s_push,
+                                             uint32_t push_base) {
+  const unsigned long long cfix = pr_to_fixed(s_hot[h - hot_base]);
+  const uint32_t e = offsets[h + 1];
+  uint32_t i = psplit[h - push_base] + lane_id();
+  for (; i + 7 * GC_WAVE < e; i += 8 * GC_WAVE) {
+    uint32_t t[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) t[j] = load_stream(targets + i + j * GC_WAVE);
+    for (int j = 0; j < 8; ++j) t[j] = load_stream(targets + i + j * GC_WAVE);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) atomicAdd(s_push + (t[j] - push_base), cfix);
-      }
-      for (; i < e; i += GC_WAVE) atomicAdd(s_push + (load_stream(targets + i) - push_base), cfix);
-    } else if (item < push_count + chunks) {
-      prm_chunk(item - push_count, entries, targets, starts, chunk_row, contrib_in, s_hot, hot_base, s_push,
-                push_base, rank_in, contrib_out, rank_out, base, damping, diff, head_partial, tail_partial,
-                tail_row);
-    } else {
-      const uint32_t r0 = (item - push_count - chunks) * PRM_CHUNK + lane_id() * 8;
-      const float r = static_cast<float>(base);
+    for (int j = 0; j < 8; ++j) atomicAdd(s_push + (t[j] - push_base), cfix);
+  }
+  for (; i < e; i += GC_WAVE) atomicAdd(s_push + (load_stream(targets + i) - push_base), cfix);
+}
+
+// Rows of degree 0, run `item` of 512 (eight per lane): rank = base,
+// contribution 0, dangling.
+__device__ __forceinline__ void prm_zero_item(uint32_t item, uint32_t zero_rows, double base,
+                                              const float* __restrict__ rank_in, float* __restrict__ rank_out,
+                                              float* __restrict__ contrib_out, double& dangling, double& diff) {
+  const uint32_t r0 = item * PRM_CHUNK + lane_id() * 8;
+  const float r = static_cast<float>(base);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint32_t row = r0 + k;
-        if (row < zero_rows) {
-          rank_out[row] = r;
-          contrib_out[row] = 0.0f;
-          dangling += static_cast<double>(r);
-          diff += fabs(static_cast<double>(r) - static_cast<double>(rank_in[row]));
-        }
-      }
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t row = r0 + k;
+    if (row < zero_rows) {
+      rank_out[row] = r;
+      contrib_out[row] = 0.0f;
+      dangling += static_cast<double>(r);
+      diff += fabs(static_cast<double>(r) - static_cast<double>(rank_in[row]));
     }
   }
+}
+
+// End of a push-pull workgroup: its dangling and L1 partials and its LDS
+// accumulators go out (one slot per workgroup).
+__device__ __forceinline__ void prm_end(uint32_t wave, const unsigned long long* __restrict__ s_push,
+                                        uint32_t push_count, unsigned long long* __restrict__ push_partials,
+                                        double* __restrict__ partials, double dangling, double diff,
+                                        double (*s_part)[PRP_THREADS / GC_WAVE]) {
   dangling = wave_sum(dangling);
   diff = wave_sum(diff);
   if (lane_id() == 0) {
@@ -1056,6 +1056,50 @@ extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_merge(
     partials[2 * static_cast<uint64_t>(blockIdx.x)] = d0;
     partials[2 * static_cast<uint64_t>(blockIdx.x) + 1] = d1;
   }
+}
+
+// Waves take items round-robin over the whole grid: the push_count hub rows
+// (prm_hub_item), then the chunks, then the rows of degree 0 in runs of 512
+// (prm_zero_item). Per-workgroup dangling and L1 partials and LDS
+// accumulators go out as in pr_spmv_push.
+extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_merge(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t hot_count, uint32_t push_count,
+    const uint32_t* __restrict__ psplit, uint32_t entries, uint32_t chunks, uint32_t zero_rows,
+    const uint8_t* __restrict__ starts, const uint32_t* __restrict__ chunk_row,
+    double* __restrict__ head_partial, double* __restrict__ tail_partial, uint32_t* __restrict__ tail_row,
+    unsigned long long* __restrict__ push_partials, double* __restrict__ partials) {
+  extern __shared__ unsigned long long s_dyn[];
+  unsigned long long* s_push = s_dyn;
+  float* s_hot = reinterpret_cast<float*>(s_dyn + push_count);
+  __shared__ double s_part[2][PRP_THREADS / GC_WAVE];
+  if (state->done) return;
+  const uint32_t hot_base = n - hot_count;
+  const uint32_t push_base = n - push_count;
+  lds_fill_f32<16>(s_hot, contrib_in + hot_base, hot_count, PRP_THREADS);
+  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) s_push[i] = 0ull;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GC_WAVE);
+  const uint32_t waves = gridDim.x * (PRP_THREADS / GC_WAVE);
+  const uint32_t zero_items = (zero_rows + PRM_CHUNK - 1) / PRM_CHUNK;
+  const uint32_t items = push_count + chunks + zero_items;
+  for (uint32_t item = blockIdx.x * (PRP_THREADS / GC_WAVE) + wave; item < items; item += waves) {
+    if (item < push_count) {
+      prm_hub_item(n - 1 - item, offsets, targets, psplit, s_hot, hot_base, s_push, push_base);
+    } else if (item < push_count + chunks) {
+      prm_chunk(item - push_count, entries, targets, starts, chunk_row, contrib_in, s_hot, hot_base, s_push,
+                push_base, rank_in, contrib_out, rank_out, base, damping, diff, head_partial, tail_partial,
+                tail_row);
+    } else {
+      prm_zero_item(item - push_count - chunks, zero_rows, base, rank_in, rank_out, contrib_out, dangling, diff);
+    }
+  }
+  prm_end(wave, s_push, push_count, push_partials, partials, dangling, diff, s_part);
 }
 
 // The hub rows as pr_push_finish (blocks [0, hub_blocks), 16 hubs each), then
@@ -1124,4 +1168,443 @@ extern "C" __global__ __launch_bounds__(256) void pr_merge_finish(
     partials[2 * slot] = d0;
     partials[2 * slot + 1] = d1;
   }
+}
+
+// ---- binned cold entries (symmetric graphs, the default there) -------------
+//
+// In the push-pull iteration the entries of the rows below P whose target lies
+// below H (the cold entries: 47 M of the 128 M entries at R-MAT scale 22) are
+// random 4-byte gathers from global memory, and those run at about 190 G/s
+// whatever cache level holds the line: the L1 looks up one tag per distinct
+// line, up to 64 per gather instruction. Here c[t] is propagated instead, along
+// a layout fixed once per graph:
+//   pr_bin_scatter: the cold sources (rows z .. H-1, in groups of 64
+//     consecutive ids; entry j of source z + 64 g + lane sits in slot row
+//     (g, j) of a static array of 16-bit LDS slots, eight slot rows to one
+//     16-byte load per lane) are cut into sub-ranges of consecutive slot rows
+//     holding at most `stage` real entries. A workgroup stores the c[t] of
+//     each entry of its sub-range into that entry's LDS slot (the entries
+//     ordered by destination bin), then copies each bin's run to its place in
+//     the bin's part of the value buffer.
+//   pr_bin_accum: one workgroup per bin (a range of at most 16,384 rows and
+//     about 64 K values) adds each value into the 64-bit fixed-point LDS
+//     accumulator of its row (a static 16-bit row index per value) and writes
+//     every row's sum of cold contributions.
+//   pr_spmv_bin: the nonzero-split pull of pr_spmv_merge over the other
+//     entries only, stored as 16-bit offsets from H (a row with none holds one
+//     offset equal to hot_count, which reads a zero), each row adding its cold
+//     sum. Hub rows, pushes and the rows of degree 0 are as in pr_spmv_merge.
+// Per cold entry an iteration moves 2 (slot) + 4 (value stored) + 4 (value
+// read) + 2 (row index) bytes in coalesced streams instead of one gather; the
+// fixed-point sums are exact, so the result stays deterministic.
+
+#define PRB_THREADS 1024
+#define PRB_WAVES (PRB_THREADS / GC_WAVE)
+
+typedef uint16_t prb_u16x8 __attribute__((ext_vector_type(8)));
+typedef float prb_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t prb_u32x2 __attribute__((ext_vector_type(2)));
+
+// Raw buffer stores whose lanes that must not store take an offset past the
+// buffer's end, which the hardware drops: the store is issued by every lane
+// with no branch around it, so the count of memory operations in flight is
+// the same on every path and the waits for earlier loads stay exact (the
+// vector memory counter counts stores too). Buffers stay below 2 GB.
+#define PRB_OOB 0x80000000u
+#define PRB_RSRC_FLAGS 0x00020000  // raw buffer, 32-bit data format (gfx9)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t prb_rsrc(void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, static_cast<int>(min(bytes, PRB_OOB)), PRB_RSRC_FLAGS);
+}
+
+__device__ __forceinline__ void prb_st32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, static_cast<int>(on ? byte_off : PRB_OOB), 0, 0);
+}
+
+__device__ __forceinline__ void prb_st64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(prb_u32x2, v), r,
+                                        static_cast<int>(on ? byte_off : PRB_OOB), 0, 0);
+}
+
+// The output buffers of pr_spmv_bin's chunks.
+struct PrbOut {
+  __amdgpu_buffer_rsrc_t rank, contrib, head, tail, tail_row;
+};
+
+// Workgroup s: slot blocks [sub[s], sub[s+1]) (eight slot rows of one group
+// each: block b holds, for lane l, the slots of its rows at slots[64 b + l];
+// block_group[b] its group); then runs [run_first[s], run_first[s+1]): run r
+// copies staging [info & 0xFFFF, + (info >> 16)) to vals[run_dst[r] ...].
+// Dynamic LDS: stage + 1 floats (slot `stage` takes the padding). Each wave
+// takes a contiguous share of the blocks, PRB_BATCH blocks per step with the
+// groups of the next step loaded in the same round trip, and a contiguous
+// share of the runs, whose descriptors it loads 64 at a time.
+#define PRB_BATCH 4
+extern "C" __global__ __launch_bounds__(PRB_THREADS) void pr_bin_scatter(
+    const float* __restrict__ contrib_in, uint32_t src_base, uint32_t src_end,
+    const prb_u16x8* __restrict__ slots, const uint32_t* __restrict__ block_group,
+    const uint32_t* __restrict__ sub, const uint32_t* __restrict__ run_first,
+    const uint32_t* __restrict__ run_info, const uint32_t* __restrict__ run_dst, float* __restrict__ vals,
+    const PrState* __restrict__ state) {
+  extern __shared__ float s_stage[];
+  if (state->done) return;
+  const uint32_t s = blockIdx.x;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GC_WAVE);
+  const uint32_t lane = lane_id();
+  const uint32_t b0 = sub[s], b1 = sub[s + 1];
+  const uint32_t per = (b1 - b0 + PRB_WAVES - 1) / PRB_WAVES;
+  const uint32_t w0 = min(b1, b0 + wave * per), w1 = min(b1, w0 + per);
+  uint32_t g[PRB_BATCH];
+#pragma unroll
+  for (int q = 0; q < PRB_BATCH; ++q) g[q] = w0 + q < w1 ? block_group[w0 + q] : 0u;
+  for (uint32_t blk = w0; blk < w1; blk += PRB_BATCH) {
+    prb_u16x8 sl[PRB_BATCH];
+    float c[PRB_BATCH];
+#pragma unroll
+    for (int q = 0; q < PRB_BATCH; ++q) {
+      if (blk + q < w1) {
+        const uint32_t t = src_base + g[q] * GC_WAVE + lane;
+        c[q] = t < src_end ? contrib_in[t] : 0.0f;
+        sl[q] = __builtin_nontemporal_load(slots + static_cast<uint64_t>(blk + q) * GC_WAVE + lane);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PRB_BATCH; ++q) {
+      const uint32_t nb = blk + PRB_BATCH + q;
+      g[q] = nb < w1 ? block_group[nb] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < PRB_BATCH; ++q) {
+      if (blk + q < w1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s_stage[sl[q][i]] = c[q];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t r0 = run_first[s], r1 = run_first[s + 1];
+  const uint32_t rper = (r1 - r0 + PRB_WAVES - 1) / PRB_WAVES;
+  const uint32_t q0 = min(r1, r0 + wave * rper), q1 = min(r1, q0 + rper);
+  for (uint32_t at = q0; at < q1; at += GC_WAVE) {
+    const uint32_t mine = at + lane;
+    const uint32_t info = mine < q1 ? run_info[mine] : 0u;
+    const uint32_t dst = mine < q1 ? run_dst[mine] : 0u;
+    const uint32_t cnt = min(static_cast<uint32_t>(GC_WAVE), q1 - at);
+#pragma unroll 4
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t inf = __builtin_amdgcn_readlane(info, j);
+      const uint32_t d = __builtin_amdgcn_readlane(dst, j);
+      const uint32_t from = inf & 0xFFFFu, len = inf >> 16;
+      for (uint32_t i = lane; i < len; i += GC_WAVE) vals[d + i] = s_stage[from + i];
+    }
+  }
+}
+
+// Workgroup b: rows [bin_row[b], bin_row[b+1]) and values [bin_val[b],
+// bin_val[b+1]) (multiples of eight; padding values are 0 with row index
+// equal to the bin's row count). cold_sum[row - row_base] = sum of the row's
+// values. Dynamic LDS: (rows + 1) u64.
+extern "C" __global__ __launch_bounds__(PRB_THREADS) void pr_bin_accum(
+    const float* __restrict__ vals, const uint16_t* __restrict__ uloc, const uint32_t* __restrict__ bin_row,
+    const uint32_t* __restrict__ bin_val, uint32_t row_base, double* __restrict__ cold_sum,
+    const PrState* __restrict__ state) {
+  extern __shared__ unsigned long long s_acc[];
+  if (state->done) return;
+  const uint32_t b = blockIdx.x;
+  const uint32_t row0 = bin_row[b], rows = bin_row[b + 1] - row0;
+  const uint32_t v0 = bin_val[b], v1 = bin_val[b + 1];
+  for (uint32_t i = threadIdx.x; i <= rows; i += PRB_THREADS) s_acc[i] = 0ull;
+  __syncthreads();
+  constexpr uint32_t kStep = PRB_THREADS * 8;
+  uint32_t p = v0 + threadIdx.x * 8;
+  // two groups of eight values per lane in flight
+  for (; p + kStep < v1; p += 2 * kStep) {
+    const prb_f32x4 a0 = __builtin_nontemporal_load(reinterpret_cast<const prb_f32x4*>(vals + p));
+    const prb_f32x4 a1 = __builtin_nontemporal_load(reinterpret_cast<const prb_f32x4*>(vals + p + 4));
+    const prb_u16x8 u0 = __builtin_nontemporal_load(reinterpret_cast<const prb_u16x8*>(uloc + p));
+    const prb_f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const prb_f32x4*>(vals + p + kStep));
+    const prb_f32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const prb_f32x4*>(vals + p + kStep + 4));
+    const prb_u16x8 u1 = __builtin_nontemporal_load(reinterpret_cast<const prb_u16x8*>(uloc + p + kStep));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      atomicAdd(s_acc + u0[k], pr_to_fixed(a0[k]));
+      atomicAdd(s_acc + u0[k + 4], pr_to_fixed(a1[k]));
+      atomicAdd(s_acc + u1[k], pr_to_fixed(c0[k]));
+      atomicAdd(s_acc + u1[k + 4], pr_to_fixed(c1[k]));
+    }
+  }
+  if (p < v1) {
+    const prb_f32x4 a0 = __builtin_nontemporal_load(reinterpret_cast<const prb_f32x4*>(vals + p));
+    const prb_f32x4 a1 = __builtin_nontemporal_load(reinterpret_cast<const prb_f32x4*>(vals + p + 4));
+    const prb_u16x8 u0 = __builtin_nontemporal_load(reinterpret_cast<const prb_u16x8*>(uloc + p));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      atomicAdd(s_acc + u0[k], pr_to_fixed(a0[k]));
+      atomicAdd(s_acc + u0[k + 4], pr_to_fixed(a1[k]));
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < rows; i += PRB_THREADS)
+    cold_sum[row0 - row_base + i] = static_cast<double>(s_acc[i]) / PRP_FIX_SCALE;
+}
+
+// A chunk's loads: the lane's eight 16-bit entries and its row-start byte,
+// and the data of row R0 + lane (R0 = chunk_row[c], passed in): its own
+// contribution, input rank, cold sum and degree, which serve the chunk's
+// first 64 rows. The stream and its start bits are padded to whole chunks
+// (entries hot_count, which read a zero, and no start) and the rows are
+// clamped to the pull rows, so every load is in bounds and none needs a
+// branch: a branch would make the waits for these loads in flight
+// conservative.
+struct PrbChunkIn {
+  prb_u16x8 t;
+  uint32_t bits;
+  uint32_t row0;
+  float cst, rin;
+  double cold;
+  uint32_t off0, off1;
+};
+
+__device__ __forceinline__ PrbChunkIn prb_load(uint32_t c, uint32_t row0, const uint16_t* __restrict__ hot16,
+                                               const uint8_t* __restrict__ starts,
+                                               const uint32_t* __restrict__ offsets,
+                                               const double* __restrict__ cold_sum,
+                                               const float* __restrict__ contrib_in,
+                                               const float* __restrict__ rank_in, uint32_t row_base,
+                                               uint32_t row_last) {
+  PrbChunkIn in;
+  const uint32_t lane = lane_id();
+  const uint32_t e = c * PRM_CHUNK + lane * 8;
+  in.t = __builtin_nontemporal_load(reinterpret_cast<const prb_u16x8*>(hot16 + e));
+  in.bits = starts[e >> 3];
+  in.row0 = row0;
+  const uint32_t row = min(row0 + lane, row_last);  // R0 = -1 (first row starts chunk 0) clamps too
+  in.cst = contrib_in[row];
+  in.rin = rank_in[row];
+  in.cold = cold_sum[max(row, row_base) - row_base];
+  in.off0 = offsets[row];
+  in.off1 = offsets[row + 1];
+  return in;
+}
+
+// One chunk of the 16-bit stream on one wave (every lane calls it: it
+// shuffles), from its loads. The segment sums are those of prm_chunk, with
+// entries t = id - H (t < hot_count: s_hot[t]; t == hot_count: the zero of a
+// row without such entries). The chunk's rows are R0 + rel, rel = 0 .. S (S
+// row starts in the chunk; rel 0 is the row running in from the chunk
+// before, rel S the tail, which may run on). Instead of loading the data of
+// each start's row where the start lies (one dependent load per start), the
+// wave loads rows R0 + rel0 + lane for pages of 64 rows at once: their own
+// contributions (passed to the entries for the pushes with a lane shuffle),
+// input ranks, degrees and cold sums. The lanes holding starts stage their
+// rows' totals in the wave's 64 LDS doubles (s_tot), and the row lanes finish
+// rows 1 .. S-1 and hand the tail's partial (with its cold sum) to
+// pr_merge_finish.
+__device__ __forceinline__ void prb_chunk(uint32_t c, const PrbChunkIn& in, const uint32_t* __restrict__ offsets,
+                                          const double* __restrict__ cold_sum, uint32_t row_base,
+                                          const float* __restrict__ contrib_in, const float* __restrict__ s_hot,
+                                          uint32_t hot_count, unsigned long long* __restrict__ s_push,
+                                          uint32_t push_off, const float* __restrict__ rank_in,
+                                          const PrbOut& out, double base, double damping, double& diff,
+                                          double* __restrict__ s_tot) {
+  const uint32_t lane = lane_id();
+  const uint32_t bits = in.bits;
+  uint32_t t[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) t[k] = in.t[k];
+  float x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = s_hot[t[k]];
+  const uint32_t pc = __popc(bits);
+  uint32_t below = 0, starts_total = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const uint64_t bb = __ballot((pc >> b) & 1u);
+    below += lanes_below(bb) << b;
+    starts_total += static_cast<uint32_t>(__popcll(bb)) << b;
+  }
+  const uint32_t S = __builtin_amdgcn_readfirstlane(starts_total);
+  // the parts of the lane's rows, last entry first: seg[k] = sum of entries
+  // k .. (next start or the lane's end) - 1
+  double seg[8];
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 7; k >= 0; --k) {
+    acc += static_cast<double>(x[k]);  // entries past the stream read the zero
+    seg[k] = acc;
+    if ((bits >> k) & 1u) acc = 0.0;
+  }
+  const uint64_t smask = __ballot(bits != 0u);
+  const uint64_t from_me = smask >> lane;
+  double C = acc;
+#pragma unroll
+  for (int o = 1; o < GC_WAVE; o <<= 1) {
+    const double oc = __shfl_down(C, o);
+    const uint64_t window = from_me & ((1ull << o) - 1ull);
+    if (window == 0ull && lane + o < GC_WAVE) C += oc;
+  }
+  double nc = __shfl_down(C, 1);
+  if (lane == GC_WAVE - 1) nc = 0.0;
+  prb_st64(out.head, c * 8u, lane == 0, C);
+  prb_st32(out.tail_row, c * 4u, lane == 0 && smask == 0ull, PRM_NONE);
+  const uint32_t last = bits ? 31u - __clz(bits) : 8u;
+  uint32_t relk[8];  // row of entry k, relative to R0
+#pragma unroll
+  for (int k = 0; k < 8; ++k) relk[k] = below + __popc(bits & ((2u << k) - 1u));
+  const uint32_t R0 = in.row0;
+  // rows R0 + rel0 + lane: the first 64 from the chunk's loads, further pages (chunks
+  // of more than 63 rows) loaded here
+  auto page = [&](uint32_t rel0, uint32_t row, float cst_l, float rin_l, double cold_l, uint32_t deg_l) {
+    const uint32_t rel = rel0 + lane;
+    // each row lane converts its row's contribution once; the entries take it by shuffle
+    const unsigned long long cfix_l = pr_to_fixed(cst_l);
+    const uint32_t cfix_lo = static_cast<uint32_t>(cfix_l), cfix_hi = static_cast<uint32_t>(cfix_l >> 32);
+    const uint32_t push_count = hot_count - push_off;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int src = static_cast<int>((relk[k] - rel0) & (GC_WAVE - 1));
+      const uint32_t lo = __shfl(cfix_lo, src), hi = __shfl(cfix_hi, src);
+      if (relk[k] - rel0 < GC_WAVE && t[k] - push_off < push_count)
+        atomicAdd(s_push + (t[k] - push_off), (static_cast<unsigned long long>(hi) << 32) | lo);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (((bits >> k) & 1u) && relk[k] - rel0 < GC_WAVE)
+        s_tot[relk[k] - rel0] = seg[k] + (static_cast<uint32_t>(k) == last ? nc : 0.0);
+    }
+    wave_lds_sync();
+    const bool fin = rel >= 1 && rel < S;  // rows that end in the chunk
+    const bool tail = rel >= 1 && rel == S;
+    const double total = s_tot[lane] + cold_l;
+    const float r = static_cast<float>(base + damping * total);
+    prb_st32(out.rank, row * 4u, fin, __builtin_bit_cast(uint32_t, r));
+    prb_st32(out.contrib, row * 4u, fin, __builtin_bit_cast(uint32_t, r / static_cast<float>(deg_l)));
+    diff += fin ? fabs(static_cast<double>(r) - static_cast<double>(rin_l)) : 0.0;
+    prb_st32(out.tail_row, c * 4u, tail, row);
+    prb_st64(out.tail, c * 8u, tail, total);
+    wave_lds_sync();
+  };
+  page(0, R0 + lane, in.cst, in.rin, in.cold, in.off1 - in.off0);
+  for (uint32_t rel0 = GC_WAVE; rel0 <= S; rel0 += GC_WAVE) {
+    const uint32_t row = R0 + min(rel0 + lane, S);  // lanes past the last row repeat it
+    page(rel0, row, contrib_in[row], rank_in[row], cold_sum[row - row_base], offsets[row + 1] - offsets[row]);
+  }
+}
+
+// Hub piece (one wave): up to 512 entries [b, min(b + 512, end of row h)) of
+// hub row h that point into P, each taking c[h] (pushes into the LDS
+// accumulators, as prm_hub_item); eight loads per lane in flight, clamped to
+// the piece.
+__device__ __forceinline__ void prb_hub_piece(uint32_t h, uint32_t b, const uint32_t* __restrict__ offsets,
+                                              const uint32_t* __restrict__ targets, const float* __restrict__ s_hot,
+                                              uint32_t hot_base, unsigned long long* __restrict__ s_push,
+                                              uint32_t push_base) {
+  const unsigned long long cfix = pr_to_fixed(s_hot[h - hot_base]);
+  const uint32_t e = min(b + PRM_CHUNK, offsets[h + 1]);
+  uint32_t t[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) t[j] = load_stream(targets + min(b + j * GC_WAVE + lane_id(), e - 1));
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (b + j * GC_WAVE + lane_id() < e) atomicAdd(s_push + (t[j] - push_base), cfix);
+}
+
+// pr_spmv_merge over the 16-bit stream (prb_chunk; the stream and its start
+// bits padded to whole chunks). Dynamic LDS: push_count
+// u64 accumulators, then hot_count + 1 contributions (the last one 0); static
+// LDS: 8.25 KB (the waves' staged row totals and the partials).
+extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_bin(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const uint16_t* __restrict__ hot16, const double* __restrict__ cold_sum,
+    const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t hot_count, uint32_t push_count,
+    const uint32_t* __restrict__ pieces, uint32_t piece_count, uint32_t chunks, uint32_t zero_rows,
+    const uint8_t* __restrict__ starts, const uint32_t* __restrict__ chunk_row,
+    double* __restrict__ head_partial, double* __restrict__ tail_partial, uint32_t* __restrict__ tail_row,
+    unsigned long long* __restrict__ push_partials, double* __restrict__ partials) {
+  extern __shared__ unsigned long long s_dyn[];
+  unsigned long long* s_push = s_dyn;
+  float* s_hot = reinterpret_cast<float*>(s_dyn + push_count);
+  __shared__ double s_part[2][PRP_THREADS / GC_WAVE];
+  __shared__ double s_tot[PRP_THREADS / GC_WAVE][GC_WAVE];  // each wave's staged row totals
+  if (state->done) return;
+  const uint32_t hot_base = n - hot_count;
+  const uint32_t push_base = n - push_count;
+  lds_fill_f32<16>(s_hot, contrib_in + hot_base, hot_count, PRP_THREADS);
+  if (threadIdx.x == 0) s_hot[hot_count] = 0.0f;
+  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) s_push[i] = 0ull;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GC_WAVE);
+  const uint32_t waves = gridDim.x * (PRP_THREADS / GC_WAVE);
+  const uint32_t zero_items = (zero_rows + PRM_CHUNK - 1) / PRM_CHUNK;
+  const uint32_t items = piece_count + chunks + zero_items;
+  // the wave's items in order: hub pieces, then chunks, then rows of degree 0
+  uint32_t item = blockIdx.x * (PRP_THREADS / GC_WAVE) + wave;
+  for (; item < piece_count; item += waves)
+    prb_hub_piece(pieces[2 * item], pieces[2 * item + 1], offsets, targets, s_hot, hot_base, s_push, push_base);
+  if (item < piece_count + chunks) {
+    uint32_t c = item - piece_count;
+    PrbOut out;
+    out.rank = prb_rsrc(rank_out, n * 4u);
+    out.contrib = prb_rsrc(contrib_out, n * 4u);
+    out.head = prb_rsrc(head_partial, chunks * 8u);
+    out.tail = prb_rsrc(tail_partial, chunks * 8u);
+    out.tail_row = prb_rsrc(tail_row, chunks * 4u);
+    auto process = [&](uint32_t ch, const PrbChunkIn& in) {
+      prb_chunk(ch, in, offsets, cold_sum, zero_rows, contrib_in, s_hot, hot_count, s_push, push_base - hot_base,
+                rank_in, out, base, damping, diff, s_tot[wave]);
+    };
+    // Loads run two chunks ahead of the processing (the last chunk's loads are
+    // issued again past the end), and the first chunk is processed before the
+    // loop, so that both ways into the loop have the same operations after
+    // the loads it waits for: the waits then cover those loads and not the
+    // stores of the chunk before (the vector memory counter is in order). The
+    // first rows of the wave's chunks c, c + waves, ... come 64 at a time in
+    // one load (lane k: the k-th of them).
+    const uint32_t last = chunks - 1;
+    const uint32_t row_last = push_base - 1;
+    const uint32_t c0 = c;
+    uint32_t k = 0;  // index of chunk c among the wave's chunks
+    uint32_t rows_v = 0;
+    auto rows_batch = [&](uint32_t from) {
+      const uint64_t ck = static_cast<uint64_t>(from) + static_cast<uint64_t>(lane_id()) * waves;
+      rows_v = chunk_row[min(ck, static_cast<uint64_t>(last))];
+    };
+    // row0 of the wave's chunk number j (j - k < 64 after the batch of chunk k was loaded)
+    auto row_of = [&](uint32_t j) {
+      return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rows_v), static_cast<int>(j & 63u)));
+    };
+    auto load = [&](uint32_t j) {
+      const uint32_t ch = min(c0 + j * waves, last);
+      return prb_load(ch, row_of(j), hot16, starts, offsets, cold_sum, contrib_in, rank_in, zero_rows, row_last);
+    };
+    rows_batch(c0);
+    PrbChunkIn a = load(0);
+    PrbChunkIn b = load(1);
+    process(c, a);
+    a = load(2);
+    c += waves;
+    ++k;
+    while (c < chunks) {
+      process(c, b);
+      if (((k + 2) & 63u) == 0u) rows_batch(c0 + (k + 2) * waves);
+      b = load(k + 2);
+      c += waves;
+      ++k;
+      if (c >= chunks) break;
+      process(c, a);
+      if (((k + 2) & 63u) == 0u) rows_batch(c0 + (k + 2) * waves);
+      a = load(k + 2);
+      c += waves;
+      ++k;
+    }
+    item = piece_count + c;
+  }
+  for (; item < items; item += waves)
+    prm_zero_item(item - piece_count - chunks, zero_rows, base, rank_in, rank_out, contrib_out, dangling, diff);
+  prm_end(wave, s_push, push_count, push_partials, partials, dangling, diff, s_part);
 }

@@ -306,6 +306,32 @@ struct GraphExecs {
   }
 };
 
+// The static layout of PageRank's binned cold entries (pagerank.hip,
+// pr_bin_scatter / pr_bin_accum / pr_spmv_bin) for one push set, hot set and
+// set of sizes; built once per graph and kept with it.
+struct PrBinPlan {
+  // key
+  uint32_t push_count = 0, hot_count = 0, stage = 0, bin_rows = 0, bin_values = 0;
+  // rows [zero_rows, push_base) pull; sources [zero_rows, hot_base) are cold
+  uint32_t zero_rows = 0, push_base = 0, hot_base = 0;
+  uint32_t hot_entries = 0;  // length of the 16-bit stream
+  uint32_t chunks = 0;       // 512-entry chunks of the 16-bit stream
+  uint32_t subs = 0;         // scatter sub-ranges (workgroups)
+  uint32_t bins = 0;         // accumulation bins (workgroups)
+  uint64_t values = 0;       // length of the value buffer (with padding)
+  uint64_t cold_entries = 0; // real cold entries
+  uint64_t slot_rows = 0;    // slot rows (padded to whole blocks of eight)
+  double build_seconds = 0.0;
+  DeviceMemory hot16, hoff, starts, chunk_row;          // 16-bit stream, row starts, chunk rows
+  DeviceMemory slots, block_group, sub, run_first;      // scatter layout
+  DeviceMemory run_info, run_dst;                       // runs: staging range, destination
+  DeviceMemory bin_row, bin_val, uloc, vals, cold_sum;  // bins, row indices, values, row sums
+  DeviceMemory hub_pieces;                              // (hub row, first entry) of the pieces
+  uint32_t pieces = 0;
+  uint64_t runs = 0, piece_entries = 0;
+  uint32_t grid = 0;  // workgroups of the last pr_spmv_bin launches
+};
+
 struct Graph {
   Device* dev = nullptr;  // holds one reference
   Sched* bfs_sched = nullptr;  // learned BFS schedule (one reference), or null: the fixed rule
@@ -349,6 +375,7 @@ struct Graph {
   uint32_t tc_hot16_parts = 0;  // the middle-vertex parts the words are grouped by
   uint32_t tc_hot16_top = 0;    // octs per mask of the top ids in the words (0: no masks)
   uint32_t pr_merge_push = 0;   // push set size the PageRank row-start bitmap was built for (0: none)
+  std::unique_ptr<PrBinPlan> pr_bin;  // PageRank's binned layout (null until the first binned call)
   // The learned triangle-count schedule (method per degree bucket): a function
   // of the graph and the network, kept with the network and version it came from.
   std::vector<uint32_t> tc_choice;

@@ -103,6 +103,7 @@ def _load() -> ctypes.CDLL:
             c_int,
             [c_void_p, c_float, c_uint32, c_float, c_void_p, u32p, i32p, POINTER(c_double)],
         ),
+        "gc_pagerank_layout": (c_int, [c_void_p, POINTER(c_uint64), c_uint32]),
         "gc_k_truss": (
             c_int,
             [c_void_p, c_uint32, POINTER(u32p), POINTER(c_size_t), u32p, POINTER(c_double)],
@@ -177,7 +178,7 @@ EXPORTED = [
     "gc_graph_num_nodes", "gc_graph_num_edges", "gc_graph_device_offsets",
     "gc_graph_device_targets", "gc_graph_prepare", "gc_graph_is_symmetric", "gc_graph_dag_stats",
     "gc_device_malloc", "gc_device_free", "gc_device_copy_to_host", "gc_triangle_count", "gc_triangle_count_noalloc",
-    "gc_triangle_count_oriented", "gc_bfs", "gc_bfs_device", "gc_pagerank", "gc_pagerank_device",
+    "gc_triangle_count_oriented", "gc_bfs", "gc_bfs_device", "gc_pagerank", "gc_pagerank_device", "gc_pagerank_layout",
     "gc_k_truss", "gc_truss_numbers", "gc_msbfs", "gc_msbfs_device", "gc_batch_from_csrs",
     "gc_batch_free", "gc_batch_num_graphs", "gc_batch_triangle_count", "gc_batch_set_config",
     "gc_batch_get_config", "gc_batch_tune", "gc_sched_new", "gc_sched_free", "gc_sched_get",

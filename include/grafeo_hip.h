@@ -251,6 +251,16 @@ gc_status gc_pagerank_device(gc_graph* g, float damping, uint32_t max_iterations
                              float* d_ranks, uint32_t* iterations, int32_t* converged,
                              double* secs);
 
+/* Sizes of the binned layout the last gc_pagerank / gc_pagerank_device call on a
+ * symmetric graph used (the default SpMV; zeros if none was built). Diagnostic,
+ * no reference counterpart: the benchmark states its kernels' bytes from them.
+ * Fills min(count, 19) values: push set, hot set, staged entries per scatter
+ * workgroup, rows and values per bin (the limits), cold entries, value buffer
+ * length, slot rows, 16-bit stream entries, stream chunks, scatter sub-ranges,
+ * bins, runs, hub pieces, hub-piece entries, pull rows, cold sources, the
+ * layout's build time in nanoseconds, and the pull kernel's workgroups. */
+gc_status gc_pagerank_layout(gc_graph* g, uint64_t* stats, uint32_t count);
+
 /* Multi-source BFS (an addition with no reference function; for each source
  * it returns what gc_bfs returns, bfs::bfs, crates/grafeo-cuda-algos/src/bfs.rs:42-131).
  * `distances` (host) receives num_sources rows of num_nodes int32 distances,

@@ -72,23 +72,29 @@ def check_tc_partitions(name, g, expected, monkeypatch):
 
 
 def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
-    """Every SpMV kernel against the oracle: the default (push-pull on a symmetric graph, with
-    its push and hot sets at their defaults, at a few dozen ids, and at one id), the same with
-    the nonzero-split pull (merge), the row-tiled push-pull (push), the pull kernel with its hot
-    set in LDS (default size and a set smaller than the graph), and one workgroup per row
-    tile."""
+    """Every SpMV kernel against the oracle: the default (push-pull on a symmetric graph with the
+    binned cold entries, with its push and hot sets at their defaults, at a few dozen ids, and at
+    one id; and with small staging buffers and bins, so that the scatter takes many sub-ranges
+    and the rows many bins), the same with the nonzero-split pull that gathers the cold entries
+    (merge), the row-tiled push-pull (push), the pull kernel with its hot set in LDS (default size
+    and a set smaller than the graph), and one workgroup per row tile."""
+    pr_vars = ("GC_PR_KERNEL", "GC_PR_PUSH", "GC_PR_HOT", "GC_PR_STAGE", "GC_PR_BIN_ROWS", "GC_PR_BIN_VALUES")
     cases = ((None, None, None), (None, "64", "96"), (None, "1", "1"), (None, "37", None),
+             ("bin", "1", "1", "512", "1", "8"), ("bin", "3", "40", "512", "7", "100"),
+             ("bin", None, None, "1000", "64", "64"),
              ("merge", None, None), ("merge", "64", "96"), ("merge", "1", "1"), ("merge", "37", None),
              ("push", None, None), ("push", "37", None),
              ("rows", None, None), ("hot", None, "1024"), ("hot", None, None))
-    for kernel, push, hot in cases:
-        for var, value in (("GC_PR_KERNEL", kernel), ("GC_PR_PUSH", push), ("GC_PR_HOT", hot)):
+    for case in cases:
+        values = case + (None,) * (len(pr_vars) - len(case))
+        for var, value in zip(pr_vars, values):
             if value is None:
                 monkeypatch.delenv(var, raising=False)
             else:
                 monkeypatch.setenv(var, value)
-        check_pagerank(f"{name} (kernel={kernel}, push={push}, hot={hot})", off, tgt, g)
-    for var in ("GC_PR_KERNEL", "GC_PR_PUSH", "GC_PR_HOT"):
+        label = ", ".join(f"{v}={x}" for v, x in zip(pr_vars, values) if x is not None)
+        check_pagerank(f"{name} ({label or 'defaults'})", off, tgt, g)
+    for var in pr_vars:
         monkeypatch.delenv(var, raising=False)
 
 
@@ -175,6 +181,10 @@ def test_random_graphs(device, monkeypatch):
             mp.setenv("GC_PR_PUSH", "1")
             mp.setenv("GC_PR_HOT", "1")
             check_pagerank(name + " (merge, push=1)", off, tgt, g)
+            # the same with the cold entries binned, one row per bin
+            mp.setenv("GC_PR_KERNEL", "bin")
+            mp.setenv("GC_PR_BIN_ROWS", "1")
+            check_pagerank(name + " (bin, push=1, one row per bin)", off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch, ks=(3, 4, 5))
         check_truss_numbers(name, off, tgt, g)
         check_msbfs(name, off, tgt, g, list(range(n)) + [0, n - 1])
@@ -282,3 +292,44 @@ def test_msbfs_deep_graphs(device):
         g = upload(device, off, tgt)
         n = len(off) - 1
         check_msbfs(name, off, tgt, g, [0, n // 3, n // 2, n - 1])
+
+
+def test_pagerank_layout_sizes(device, monkeypatch):
+    """The binned layout's reported sizes (gc_pagerank_layout) against counts taken here from
+    the degree-ordered CSR: cold entries (entries of the pull rows below the hot set), pull rows,
+    cold sources, the hub-piece entries (entries of the push-set rows inside the push set) and
+    the 16-bit stream (each pull row's entries in the hot set, at least one)."""
+    from ctypes import c_uint64
+
+    from grafeo_cuda._native import check, lib
+
+    off, tgt = rmat_graph(14, 16, seed=1)
+    g = upload(device, off, tgt)
+    for push, hot in ((None, None), ("64", "96"), ("1", "1")):
+        for var, value in (("GC_PR_PUSH", push), ("GC_PR_HOT", hot)):
+            if value is None:
+                monkeypatch.delenv(var, raising=False)
+            else:
+                monkeypatch.setenv(var, value)
+        check_pagerank(f"RMAT-14 layout (push={push}, hot={hot})", off, tgt, g)
+        vals = (c_uint64 * 19)()
+        check(lib.gc_pagerank_layout(g._handle, vals, 19))
+        lay = dict(zip(("push", "hot", "stage", "bin_rows", "bin_values", "cold", "values", "slot_rows",
+                        "stream", "chunks", "subs", "bins", "runs", "pieces", "piece_entries", "pull",
+                        "sources", "build_ns", "grid"), (int(v) for v in vals)))
+        rl = g.relabelled_numpy()
+        roff, rtgt = rl["offsets"].astype(np.int64), rl["targets"].astype(np.int64)
+        n = roff.size - 1
+        pb, hb = n - lay["push"], n - lay["hot"]
+        deg = np.diff(roff)
+        z = min(int(np.count_nonzero(deg == 0)), pb)
+        row = np.repeat(np.arange(n), deg)
+        pull = (row >= z) & (row < pb)
+        assert lay["cold"] == int(np.count_nonzero(pull & (rtgt < hb)))
+        assert lay["pull"] == pb - z and lay["sources"] == max(hb - z, 0)
+        assert lay["piece_entries"] == int(np.count_nonzero((row >= pb) & (rtgt >= pb)))
+        hot_per_row = np.bincount(row[pull & (rtgt >= hb)], minlength=n)[z:pb]
+        assert lay["stream"] == int(np.maximum(hot_per_row, 1).sum())
+        assert lay["chunks"] == (lay["stream"] + 511) // 512
+        assert lay["values"] >= lay["cold"] and lay["values"] % 8 == 0 and lay["slot_rows"] % 8 == 0
+        assert lay["grid"] >= 1

@@ -1,5 +1,5 @@
 """Builds and runs the host-only C++ unit tests (module cache with a fake loader, launch
-geometry); no GPU involved."""
+geometry, PageRank's binned layout replayed against direct row sums); no GPU involved."""
 
 from __future__ import annotations
 
@@ -17,3 +17,13 @@ def test_native_units(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "all native unit tests passed" in out.stdout
+
+
+def test_pagerank_bin_layout(tmp_path):
+    exe = tmp_path / "pr_bin"
+    src = os.path.join(REPO, "tests", "native", "test_pr_bin.cpp")
+    inc = os.path.join(REPO, "grafeo-cuda_amd", "csrc", "algos")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", f"-I{inc}", src, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "pr_bin layout tests passed" in out.stdout
