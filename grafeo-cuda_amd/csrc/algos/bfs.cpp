@@ -579,15 +579,14 @@ struct BfsOutcome {
 };
 
 // The fixed rule as the learned rule's fallback: the network's choice stands
-// only where it predicts it at least `margin` (in ln microseconds; 0.26, about
-// 30%, GC_SCHED_MARGIN) cheaper than the fixed rule's choice for the same
-// state, and the fixed rule decides elsewhere. A network trained on R-MAT
-// scales 16-22 went bottom-up from a 763-vertex frontier at scale 24 (0.50
-// against 0.21 ms for that level); the margin keeps such extrapolations from
-// overriding the rule unless the predicted gain is large.
+// only where it predicts it at least `margin` (in ln microseconds; 0.15, about
+// 16%, GC_SCHED_MARGIN) cheaper than the fixed rule's choice for the same
+// state, and the fixed rule decides elsewhere. 0.15 was the best of 0, 0.05,
+// 0.1, 0.15 and 0.26 on R-MAT scale 22 with seed 1, a graph outside the
+// training set (every margin there stayed 5-8% slower than the fixed rule).
 static float learned_margin() {
   const char* env = std::getenv("GC_SCHED_MARGIN");
-  return env != nullptr ? static_cast<float>(std::atof(env)) : 0.26f;
+  return env != nullptr ? static_cast<float>(std::atof(env)) : 0.15f;
 }
 
 static uint32_t with_fallback(Traversal& t, uint32_t learned, float margin) {
@@ -672,10 +671,13 @@ BfsOutcome bfs(Graph& g, uint32_t source, int32_t* d_out) {
 // Training data: at every level, the features of the decision and the time of
 // each allowed choice from the same state (dist and frontier saved and
 // restored around each trial); then the fastest choice is run for one level.
-// Each choice is timed the way bfs() runs it when chosen, per level: chained
-// levels as one captured chain divided by the levels it built, bottom-up as
-// the run of levels launched ahead that the choice starts (unless
-// GC_BFS_AHEAD=0) divided by its levels.
+// Each choice is timed the way bfs() runs it when chosen: chained levels as
+// one captured chain divided by the levels it built; bottom-up as the run of
+// levels launched ahead that the choice starts (unless GC_BFS_AHEAD=0), up to
+// the publication of its first level: that level is what the choice adds, as
+// the run's later levels are those the fixed rule takes too. (The run's mean
+// per level would hide a first level from a small frontier, which scans
+// nearly every list: 0.5 ms at R-MAT scale 24 against a 0.12 ms mean.)
 uint32_t bfs_level_timings(Graph& g, uint32_t source, uint32_t max_levels, float* features,
                            float* times_us) {
   prepare(g);
@@ -737,9 +739,11 @@ uint32_t bfs_level_timings(Graph& g, uint32_t source, uint32_t max_levels, float
           const uint32_t built = t.run_chain(kChainLevels);
           us = static_cast<float>(sw.seconds() * 1e6) / static_cast<float>(std::max(1u, built));
         } else if (c == kBottomUp && ahead) {
-          uint32_t built = 0;
-          t.run_bottom_up([&] { ++built; });
-          us = static_cast<float>(sw.seconds() * 1e6) / static_cast<float>(std::max(1u, built));
+          float first = -1.0f;
+          t.run_bottom_up([&] {
+            if (first < 0.0f) first = static_cast<float>(sw.seconds() * 1e6);
+          });
+          us = first >= 0.0f ? first : static_cast<float>(sw.seconds() * 1e6);
         } else {
           t.step(c, 1, false, nullptr);
           us = static_cast<float>(sw.seconds() * 1e6);

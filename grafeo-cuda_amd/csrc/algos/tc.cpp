@@ -273,18 +273,20 @@ const std::vector<BucketInfo>& bucket_info(Graph& g) {
   if (bounds.size() >= 2) {
     const uint32_t buckets = static_cast<uint32_t>(bounds.size() - 1);
     uint32_t* d_bounds = g.buffer<uint32_t>("tc.bucket_bounds", sizeof(uint32_t) * bounds.size());
-    auto* d_stats = g.buffer<unsigned long long>("tc.bucket_stats", sizeof(unsigned long long) * 4 * buckets);
+    auto* d_stats = g.buffer<unsigned long long>("tc.bucket_stats", sizeof(unsigned long long) * 5 * buckets);
     copy_to_device(dev, d_bounds, bounds.data(), sizeof(uint32_t) * bounds.size());
-    fill_words(dev, d_stats, 8ull * buckets, 0);
+    fill_words(dev, d_stats, 10ull * buckets, 0);
     const uint32_t grid = std::max(1, dev.cu_count) * 8u;
+    const uint32_t n = g.n;
+    const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;  // as the bitmap kernels
     launch(dev, get_kernel(dev, "tc", "tc_bucket_stats"), grid, 256u, 0, g.rl.fwd.as<const uint32_t>(),
            g.rl.offsets.as<const uint32_t>() + 1, g.rl.targets.as<const uint32_t>(),
-           static_cast<const uint32_t*>(d_bounds), buckets, d_stats);
-    std::vector<unsigned long long> stats(4 * buckets);
+           static_cast<const uint32_t*>(d_bounds), buckets, hot_base, d_stats);
+    std::vector<unsigned long long> stats(5 * buckets);
     copy_to_host(dev, stats.data(), d_stats, sizeof(unsigned long long) * stats.size());
     for (uint32_t b = 0; b < buckets; ++b) {
-      info.push_back({bounds[b], bounds[b + 1], stats[4 * b], stats[4 * b + 1], stats[4 * b + 2],
-                      stats[4 * b + 3]});
+      info.push_back({bounds[b], bounds[b + 1], stats[5 * b], stats[5 * b + 1], stats[5 * b + 2],
+                      stats[5 * b + 3], stats[5 * b + 4]});
     }
   }
   g.tc_buckets.assign(info.begin(), info.end());
@@ -310,7 +312,10 @@ void bucket_features(const Graph& g, const std::vector<BucketInfo>& all, size_t 
   f[7] = lg2(degree_lo);
   f[8] = lg2(g.n - k.row_lo);
   f[9] = lg2(static_cast<double>(k.probes) / std::max<uint64_t>(k.entries, 1));
-  f[10] = lg2(static_cast<double>(g.rl.oriented_entries));
+  // the share of the bucket's probes among the bitmap kernels' hot ids, whose lists the
+  // bitmap kernels stream from 16-bit words (the rest are binary-searched): near 1 on
+  // R-MAT graphs, near 0 on large grids, where the binary-search lists win
+  f[10] = static_cast<float>(static_cast<double>(k.hot_probes) / static_cast<double>(std::max<uint64_t>(k.probes, 1)));
   f[11] = lg2(static_cast<double>(total_probes));
 }
 

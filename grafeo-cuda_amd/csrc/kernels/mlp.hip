@@ -245,19 +245,24 @@ extern "C" __global__ __launch_bounds__(64) void sched_bfs_decide(
     st.reached += c.next_size;
     st.level += 1;
   }
+  // The graph's size enters only as the fractions of it the state holds (a
+  // network trained on smaller graphs then sees a larger graph's states within
+  // the ranges it was trained on); the frontier's own size stays absolute, as
+  // it decides between per-level overheads and per-level work.
+  const double n1 = static_cast<double>(st.n) + 1.0, m1 = static_cast<double>(st.m) + 1.0;
   float f[MLP_IN];
-  f[0] = lg(st.n);
-  f[1] = lg(static_cast<double>(st.m));
+  f[0] = static_cast<float>(log2((static_cast<double>(st.n_f) + 1.0) / n1));
+  f[1] = static_cast<float>(log2((static_cast<double>(st.m_f) + 1.0) / m1));
   f[2] = lg(st.n_f);
   f[3] = lg(static_cast<double>(st.m_f));
-  f[4] = lg(static_cast<double>(st.m_u));
-  f[5] = lg(static_cast<double>(st.reached));
+  f[4] = static_cast<float>(log2((static_cast<double>(st.m_u) + 1.0) / m1));
+  f[5] = static_cast<float>(log2((static_cast<double>(st.reached) + 1.0) / n1));
   f[6] = static_cast<float>(log2((static_cast<double>(st.m_f) + 1.0) / (static_cast<double>(st.m_u) + 1.0)));
   f[7] = static_cast<float>(log2((static_cast<double>(st.n_f) + 1.0) / (static_cast<double>(st.prev_n_f) + 1.0)));
   f[8] = lg(st.level);
   f[9] = st.bitmap ? 1.0f : 0.0f;
   f[10] = lg(static_cast<double>(st.m) / (st.n > 0 ? st.n : 1));
-  f[11] = lg(st.max_degree);
+  f[11] = static_cast<float>(log2((static_cast<double>(st.max_degree) + 1.0) * n1 / m1));  // hub size / mean degree
   if (lane < MLP_IN) out->features[lane] = f[lane];
   // chained levels take bounded frontiers only
   if (st.n_f > chain_cap || st.m_f > chain_edge_cap) allowed_mask &= ~4u;

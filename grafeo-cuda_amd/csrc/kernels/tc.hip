@@ -868,7 +868,7 @@ extern "C" __global__ __launch_bounds__(64) void tc_wedge(
 extern "C" __global__ __launch_bounds__(256) void tc_bucket_stats(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
     const uint32_t* __restrict__ targets, const uint32_t* __restrict__ bucket_lo, uint32_t buckets,
-    unsigned long long* __restrict__ stats) {
+    uint32_t hot_base, unsigned long long* __restrict__ stats) {
   const uint32_t first = bucket_lo[0], last = bucket_lo[buckets];
   const uint64_t stride = grid_threads();
   const uint64_t steps = (static_cast<uint64_t>(last - first) + stride - 1) / stride;
@@ -876,7 +876,7 @@ extern "C" __global__ __launch_bounds__(256) void tc_bucket_stats(
     const uint64_t uu = first + global_thread() + k * stride;
     const bool valid = uu < last;
     uint32_t b = 0;
-    unsigned long long d = 0, probes = 0;
+    unsigned long long d = 0, probes = 0, hot = 0;
     if (valid) {
       const uint32_t u = static_cast<uint32_t>(uu);
       uint32_t l = 0, r = buckets;  // last bucket whose first row is <= u
@@ -891,6 +891,7 @@ extern "C" __global__ __launch_bounds__(256) void tc_bucket_stats(
       for (uint32_t i = a; i < e; ++i) {
         const uint32_t v = targets[i];
         probes += hi[v] - lo[v];
+        hot += hi[v] - lower_bound_u32(targets, lo[v], hi[v], hot_base);  // L(v) among the hot ids
       }
     }
     const uint32_t b0 = __shfl(b, 0);
@@ -899,6 +900,7 @@ extern "C" __global__ __launch_bounds__(256) void tc_bucket_stats(
       const unsigned long long rows = wave_sum(valid ? 1ull : 0ull);
       const unsigned long long entries = wave_sum(d);
       const unsigned long long items = wave_sum(probes);
+      const unsigned long long hot_items = wave_sum(hot);
       unsigned long long mx = d;
 #pragma unroll
       for (int o = GC_WAVE / 2; o > 0; o >>= 1) {
@@ -906,16 +908,18 @@ extern "C" __global__ __launch_bounds__(256) void tc_bucket_stats(
         mx = other > mx ? other : mx;
       }
       if (lane_id() == 0) {
-        atomicAdd(&stats[4 * b0 + 0], rows);
-        atomicAdd(&stats[4 * b0 + 1], entries);
-        atomicAdd(&stats[4 * b0 + 2], items);
-        atomicMax(&stats[4 * b0 + 3], mx);
+        atomicAdd(&stats[5 * b0 + 0], rows);
+        atomicAdd(&stats[5 * b0 + 1], entries);
+        atomicAdd(&stats[5 * b0 + 2], items);
+        atomicMax(&stats[5 * b0 + 3], mx);
+        atomicAdd(&stats[5 * b0 + 4], hot_items);
       }
     } else if (valid) {
-      atomicAdd(&stats[4 * b + 0], 1ull);
-      atomicAdd(&stats[4 * b + 1], d);
-      atomicAdd(&stats[4 * b + 2], probes);
-      atomicMax(&stats[4 * b + 3], d);
+      atomicAdd(&stats[5 * b + 0], 1ull);
+      atomicAdd(&stats[5 * b + 1], d);
+      atomicAdd(&stats[5 * b + 2], probes);
+      atomicMax(&stats[5 * b + 3], d);
+      atomicAdd(&stats[5 * b + 4], hot);
     }
   }
 }

@@ -342,6 +342,7 @@ struct Graph {
   struct TcBucket {
     uint32_t row_lo, row_hi;
     uint64_t rows, entries, probes, max_out;
+    uint64_t hot_probes;  // probes among the bitmap kernels' hot ids (the top 32,768)
   };
   std::vector<TcBucket> tc_buckets;
   bool tc_buckets_ready = false;

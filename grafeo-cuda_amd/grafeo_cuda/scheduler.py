@@ -13,8 +13,8 @@ The committed model (sched_model.json next to this file) was produced by
 
     python3 -m grafeo_cuda.scheduler train --out grafeo-cuda_amd/grafeo_cuda/sched_model.json
 
-on one MI355X, from R-MAT scales 16..23 (edge factor 16) and square 4-neighbour grids with 256,
-512, 1024, 2048 and 2896 rows (the training set below). It is evaluated on graphs it was not trained
+on one MI355X, from R-MAT scales 16..22 (edge factor 16) and 4-neighbour grids of 256², 512²,
+1024², 2048², 2896², 2048 x 4096 and 2896 x 4096 vertices (the training set below). It is evaluated on graphs it was not trained
 on: R-MAT scale 24 and the 4096 x 4096 grid (bench.py --workload scheduler).
 """
 
@@ -56,17 +56,21 @@ def nonzero_sources(off, count: int, seed: int) -> list[int]:
 
 def training_graphs(quick: bool = False):
     """(name, offsets, targets, BFS sources) of the training set."""
-    scales = (16, 18) if quick else (16, 17, 18, 19, 20, 21, 22, 23)
+    scales = (16, 18) if quick else (16, 17, 18, 19, 20, 21, 22)
     # 2896 x 2896 (8.4M vertices) is the first square size here at which the binary-search
     # intersection beats the bitmap on the grid's degree-4 rows (2571 against 2923 us)
-    rows = (256, 512) if quick else (256, 512, 1024, 2048, 2896)
+    # and rectangular grids of 8.4 M and 11.9 M vertices, where the bitmap kernels see even
+    # fewer of their hot ids
+    rows = ((256, 256), (512, 512)) if quick else ((256, 256), (512, 512), (1024, 1024), (2048, 2048),
+                                                   (2896, 2896), (2048, 4096), (2896, 4096))
     for s in scales:
         off, tgt = rmat(s, 16, seed=100 + s)
         yield f"rmat{s}", off, tgt, nonzero_sources(off, 3, s)
-    for r in rows:
-        off, tgt = grid(r, r)
-        n = r * r
-        yield f"grid{r}", off, tgt, [0, n // 2 + r // 2, int(np.random.default_rng(r).integers(0, n))]
+    for r, c in rows:
+        off, tgt = grid(r, c)
+        n = r * c
+        name = f"grid{r}" if r == c else f"grid{r}x{c}"
+        yield name, off, tgt, [0, n // 2 + c // 2, int(np.random.default_rng(n).integers(0, n))]
 
 
 def _targets(times_us: np.ndarray):
@@ -109,6 +113,16 @@ def collect(device: GpuDevice, graphs) -> dict:
             "names": names}
 
 
+def centered(t, mask):
+    """Targets minus their mean over the allowed choices of each sample. A decision needs only
+    the order of its choices' costs; without each state's common level (which grows with the
+    graph) the network does not have to extrapolate absolute times to graphs larger than those
+    it was trained on."""
+    allowed = (mask > 0).astype(np.float64)
+    mean = (t * allowed).sum(axis=1) / np.maximum(allowed.sum(axis=1), 1.0)
+    return (t - mean[:, None] * allowed).astype(np.float32)
+
+
 def fit(device: GpuDevice, x, t, mask, steps: int = 3000, lr: float = 3e-3, seed: int = 7):
     """A GpuScheduler trained on (x, t, mask): returns (scheduler, loss before, loss after)."""
     s = GpuScheduler(device, seed=seed)
@@ -148,15 +162,21 @@ def main(argv=None) -> int:
     tr.add_argument("--out", default=MODEL)
     tr.add_argument("--quick", action="store_true", help="a small training set (tests, smoke)")
     tr.add_argument("--steps", type=int, default=3000)
+    tr.add_argument("--bfs-targets", choices=["centered", "absolute"], default="centered",
+                    help="BFS targets: ln-times less their per-level mean (default), or ln-times")
     args = ap.parse_args(argv)
     device = GpuDevice(0)
     t0 = time.time()
     data = collect(device, training_graphs(args.quick))
     info = {"device": device.name, "collect_seconds": time.time() - t0, "steps": args.steps,
-            "training_set": data["names"]}
+            "training_set": data["names"], "bfs_targets": args.bfs_targets}
     result = {}
     for key in ("bfs", "tc"):
         x, t, mask = data[key]
+        # BFS decisions compare predictions only with each other (and a margin in ln-time);
+        # triangle-count decisions also weigh a predicted saving in microseconds
+        if key == "bfs" and args.bfs_targets == "centered":
+            t = centered(t, mask)
         s, before, after = fit(device, x, t, mask, steps=args.steps)
         info[key] = {"samples": int(len(x)), "loss_before": before, "loss_after": after}
         result[key] = s
