@@ -49,7 +49,8 @@ METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
 KERNEL_FAMILIES = {
     "tc": ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count"),
-    "pagerank": ("pr_spmv_bin", "pr_bin_scatter", "pr_bin_accum", "pr_spmv_merge", "pr_merge_finish",
+    "pagerank": ("pr_spmv_slice", "pr_slice_finish", "pr_spmv_bin", "pr_bin_scatter", "pr_bin_accum",
+                 "pr_spmv_merge", "pr_merge_finish",
                  "pr_spmv_push", "pr_push_finish", "pr_spmv_hot", "pr_spmv",
                  "pr_hub_finish", "pr_finalize", "pr_init", "pr_push_split", "pr_merge_starts", "pr_merge_chunks"),
     "bfs": ("bfs_init", "bfs_topdown_edges", "bfs_frontier_degrees", "bfs_frontier_scan_small", "bfs_bottomup",
@@ -67,13 +68,19 @@ KERNEL_FAMILIES = {
 KERNELS = tuple(k for ks in KERNEL_FAMILIES.values() for k in ks)
 # The kernels timed with events in a default run: those the rooflines report. Each timed launch
 # adds two event records to the stream (about 2 ms per step if every launch were timed).
-ROOFLINE_KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_bin", "pr_bin_scatter",
-                    "pr_bin_accum", "pr_spmv_merge", "pr_spmv_push", "pr_spmv_hot", "pr_spmv")
-# The binned SpMV (the default on symmetric graphs): three kernels per iteration.
-PR_BIN_KERNELS = ("pr_bin_scatter", "pr_bin_accum", "pr_spmv_bin")
+ROOFLINE_KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_slice", "pr_slice_finish",
+                    "pr_spmv_bin", "pr_merge_finish", "pr_bin_scatter", "pr_bin_accum", "pr_spmv_merge",
+                    "pr_spmv_push", "pr_spmv_hot", "pr_spmv")
+# The binned SpMV (the default on symmetric graphs), four kernels per iteration: the scatter and
+# the accumulation of the cold entries, the pull over the hot stream (sliced, the default, or
+# chunked: GC_PR_KERNEL=bin) and the finishing of the hub rows and of the rows the pull leaves
+# partial.
+PR_BIN_SETS = (("pr_bin_scatter", "pr_bin_accum", "pr_spmv_slice", "pr_slice_finish"),
+               ("pr_bin_scatter", "pr_bin_accum", "pr_spmv_bin", "pr_merge_finish"))
 PR_LAYOUT_KEYS = ("push_count", "hot_count", "stage", "bin_rows", "bin_values", "cold_entries", "values",
                   "slot_rows", "hot_entries", "chunks", "subs", "bins", "runs", "pieces", "piece_entries",
-                  "pull_rows", "cold_sources", "build_ns", "grid")
+                  "pull_rows", "cold_sources", "build_ns", "grid", "sliced", "slices", "slice_entries",
+                  "slice_items", "long_slices", "partial_slots")
 
 
 def pagerank_layout(g) -> dict:
@@ -92,18 +99,34 @@ def pagerank_bin_models(lay: dict, zero_rows: int) -> dict:
       written (4 bytes);
     - pr_bin_accum: each value and its 16-bit row index (6 bytes) and each pull row's cold sum
       written (8 bytes);
-    - pr_spmv_bin: the 16-bit stream in whole 512-entry chunks with a start bit per entry; per
-      pull row its contribution, input rank, cold sum, offset (20 bytes) and rank and
-      contribution written (8); the hub pieces' 4-byte entries; per workgroup the hot set read
-      (4 bytes per id) and the push accumulators written (8 per hub); the rows of degree 0
-      (rank read, rank and contribution written: 12 bytes)."""
+    - pr_spmv_slice (sliced layout): the sliced 16-bit entries with their padding, 16 bytes per
+      work item; per pull row its contribution, input rank, cold sum, offset (20 bytes) and rank
+      and contribution written (8), less the rows of the wide slices, which write a partial sum
+      (8 bytes per row and piece) instead; the hub pieces' 4-byte entries; per workgroup the hot
+      set read (4 bytes per id) and the push accumulators written (8 per hub); the rows of
+      degree 0 (rank read, rank and contribution written: 12 bytes);
+    - pr_slice_finish: the push accumulators of every workgroup read (8 bytes per hub each) and
+      the hub rows finished (offsets, input rank, rank and contribution: 16 bytes); the wide
+      slices' rows: their partials read, cold sum, offsets and input rank (16 bytes) and rank and
+      contribution written (8);
+    - pr_spmv_bin (chunked stream): the 16-bit stream in whole 512-entry chunks with a start bit
+      per entry, the row data as pr_spmv_slice, 20 bytes of head, tail and tail row per chunk;
+    - pr_merge_finish: the push accumulators and hub rows as pr_slice_finish, the chunks' heads,
+      tails and tail rows read (20 bytes per chunk) and a row finished per tail (24 bytes)."""
     stream = 512 * lay["chunks"]
+    per_wg = lay["grid"] * (4 * lay["hot_count"] + 8 * lay["push_count"])
+    hubs = lay["grid"] * 8 * lay["push_count"] + 16 * lay["push_count"]
+    wide_rows = 64 * lay["long_slices"]
     return {
         "pr_bin_scatter": 2 * 64 * lay["slot_rows"] + 4 * (lay["slot_rows"] // 8) + 4 * lay["cold_sources"]
         + 8 * lay["runs"] + 4 * lay["cold_entries"],
         "pr_bin_accum": 6 * lay["values"] + 8 * lay["pull_rows"],
-        "pr_spmv_bin": 2 * stream + stream // 8 + 28 * lay["pull_rows"] + 4 * lay["piece_entries"]
-        + lay["grid"] * (4 * lay["hot_count"] + 8 * lay["push_count"]) + 12 * zero_rows,
+        "pr_spmv_slice": 2 * lay["slice_entries"] + 16 * lay["slice_items"] + 28 * lay["pull_rows"]
+        - 8 * wide_rows + 512 * lay["partial_slots"] + 4 * lay["piece_entries"] + per_wg + 12 * zero_rows,
+        "pr_slice_finish": hubs + 512 * lay["partial_slots"] + 24 * wide_rows,
+        "pr_spmv_bin": 2 * stream + stream // 8 + 28 * lay["pull_rows"] + 20 * lay["chunks"]
+        + 4 * lay["piece_entries"] + per_wg + 12 * zero_rows,
+        "pr_merge_finish": hubs + 20 * lay["chunks"] + 24 * min(lay["chunks"], lay["pull_rows"]),
     }
 # two-sided 95% quantiles of Student's t by degrees of freedom
 T975 = {1: 12.706, 2: 4.303, 3: 3.182, 4: 2.776, 5: 2.571, 6: 2.447, 7: 2.365, 8: 2.306, 9: 2.262, 10: 2.228}
@@ -379,22 +402,23 @@ def main() -> int:
                 "avg_launch_us is per iteration that ran (launches after convergence return at once)"
                 + ("; the push-pull kernels read the hubs' lists only inside the push set (79% of the "
                    "target stream at R-MAT 22)" if name in ("pr_spmv_merge", "pr_spmv_push") else ""))
-    if all(k in kernels for k in PR_BIN_KERNELS):
+    bin_set = next((ks for ks in PR_BIN_SETS if all(k in kernels for k in ks)), None)
+    if bin_set is not None:
         lay = pagerank_layout(g)
         zero_rows = int(np.count_nonzero(np.diff(offsets) == 0))
         models = pagerank_bin_models(lay, zero_rows)
-        per_iter = {k: 1000.0 * kernels[k]["total_ms"] / max(1, iters) for k in PR_BIN_KERNELS}
-        tr = [traffic.get(k) for k in PR_BIN_KERNELS]
-        t_sum = (sum(t * kernels[k]["launches"] / max(1, iters) for t, k in zip(tr, PR_BIN_KERNELS))
+        per_iter = {k: 1000.0 * kernels[k]["total_ms"] / max(1, iters) for k in bin_set}
+        tr = [traffic.get(k) for k in bin_set]
+        t_sum = (sum(t * kernels[k]["launches"] / max(1, iters) for t, k in zip(tr, bin_set))
                  if all(tr) else None)
         rooflines["pr_spmv_binned"] = roofline_entry(
             spmv_bytes, sum(per_iter.values()), t_sum,
-            "the SpMV of the default PageRank (binned cold entries): pr_bin_scatter + pr_bin_accum + "
-            "pr_spmv_bin per iteration that ran, against bytes_per_launch = 4(n+1) + 4m + 8n (offsets, "
+            "the SpMV of the default PageRank (binned cold entries): " + " + ".join(bin_set)
+            + " per iteration that ran, against bytes_per_launch = 4(n+1) + 4m + 8n (offsets, "
             "targets, x read once, y written once)")
         rooflines["pr_spmv_binned"]["kernels_us"] = per_iter
         rooflines["pr_spmv_binned"]["layout"] = lay
-        for k in PR_BIN_KERNELS:
+        for k in bin_set:
             t = traffic.get(k)
             rooflines[k] = roofline_entry(
                 models[k], per_iter[k], t * kernels[k]["launches"] / max(1, iters) if t else None,
@@ -970,12 +994,13 @@ def pagerank_precision(g, offsets, targets, dev) -> dict:
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    spmv_kernels = ("pr_spmv_merge", "pr_spmv_push", "pr_spmv_hot", "pr_spmv") + PR_BIN_KERNELS
+    spmv_kernels = ("pr_spmv_merge", "pr_spmv_push", "pr_spmv_hot", "pr_spmv") + PR_BIN_SETS[0] + PR_BIN_SETS[1]
     dev.profile(True, only=list(spmv_kernels))
     r = g.pagerank_numpy(0.85, 50, 1e-7)
-    if dev.kernel_time("pr_spmv_bin")[1]:
-        name = "+".join(PR_BIN_KERNELS)
-        ms = sum(dev.kernel_time(k)[0] for k in PR_BIN_KERNELS)
+    bin_set = next((ks for ks in PR_BIN_SETS if all(dev.kernel_time(k)[1] for k in ks)), None)
+    if bin_set is not None:
+        name = "+".join(bin_set)
+        ms = sum(dev.kernel_time(k)[0] for k in bin_set)
     else:
         name = next(k for k in spmv_kernels if dev.kernel_time(k)[1])
         ms, _ = dev.kernel_time(name)

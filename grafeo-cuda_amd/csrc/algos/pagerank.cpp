@@ -15,6 +15,7 @@
 
 #include "../runtime/common.hpp"
 #include "pr_bin_layout.hpp"
+#include "pr_slice_layout.hpp"
 
 namespace gc {
 
@@ -119,20 +120,25 @@ PrSegments plan_segments(const std::vector<uint32_t>& off, uint32_t n, uint32_t 
 }  // namespace
 
 // Which SpMV runs (GC_PR_KERNEL): on a symmetric graph the push-pull
-// iterations, by default with the nonzero-split pull over the rows below the
-// push set (pr_spmv_merge, pr_merge_finish), with GC_PR_KERNEL=push with row
-// tiles of G lanes per row (pr_spmv_push, pr_push_finish); otherwise, or with
-// GC_PR_KERNEL=hot, the persistent pull kernel with the hot contributions in
-// LDS (pr_spmv_hot); with GC_PR_KERNEL=rows one workgroup per row tile
-// (pr_spmv). Measured on MI355X at R-MAT scale 22, per iteration that ran:
-// merge 380 us, push 401 us (both with the default sets), hot 527 us. The
-// row tiles fill 70% of their lane slots there (a row of degree d on G lanes
-// takes ceil(d / 8G) steps of 8G slots); the chunks of 512 entries fill all.
+// iterations, by default with the cold entries binned (pr_bin_scatter,
+// pr_bin_accum) and the hot stream sliced one row per lane (pr_spmv_slice,
+// pr_slice_finish); with GC_PR_KERNEL=bin the same bins and the hot stream in
+// 512-entry chunks (pr_spmv_bin, pr_merge_finish); with GC_PR_KERNEL=merge the
+// nonzero-split pull that gathers the cold entries (pr_spmv_merge,
+// pr_merge_finish), with GC_PR_KERNEL=push row tiles of G lanes per row
+// (pr_spmv_push, pr_push_finish); otherwise, or with GC_PR_KERNEL=hot, the
+// persistent pull kernel with the hot contributions in LDS (pr_spmv_hot); with
+// GC_PR_KERNEL=rows one workgroup per row tile (pr_spmv). Measured on MI355X
+// at R-MAT scale 22, per iteration that ran: slice 226 us and bin 266 us
+// (scatter, accumulation, pull and finishing together), merge 380 us, push 401
+// us (pull alone, both with the default sets), hot 527 us. The row tiles fill
+// 70% of their lane slots there (a row of degree d on G lanes takes
+// ceil(d / 8G) steps of 8G slots); the chunks of 512 entries fill all.
 // (Earlier, 17 iterations at scale 22: hot 11.3 ms against rows 14.0 ms; a
 // source-partitioned pass, one part of the sources per XCD with eight FP32
 // partial sums per row, halved the L2 requests but was no faster: 12.1
 // against 10.2-10.5 ms.)
-enum class PrKernel { kPush, kMerge, kBin, kHot, kRows };
+enum class PrKernel { kPush, kMerge, kBin, kSlice, kHot, kRows };
 
 static PrKernel choose_kernel(const Graph& g) {
   const char* env = std::getenv("GC_PR_KERNEL");
@@ -141,7 +147,8 @@ static PrKernel choose_kernel(const Graph& g) {
   if (!g.symmetric) return PrKernel::kHot;
   if (env != nullptr && std::strcmp(env, "push") == 0) return PrKernel::kPush;
   if (env != nullptr && std::strcmp(env, "merge") == 0) return PrKernel::kMerge;
-  return PrKernel::kBin;
+  if (env != nullptr && std::strcmp(env, "bin") == 0) return PrKernel::kBin;
+  return PrKernel::kSlice;
 }
 
 static uint32_t env_u32(const char* name, uint32_t fallback) {
@@ -154,9 +161,10 @@ static uint32_t env_u32(const char* name, uint32_t fallback) {
 // symmetric graph: built on the host from the relabelled CSR (copied back
 // once), uploaded, and kept with the graph until a call asks for other sizes.
 static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, uint32_t stage, uint32_t bin_rows,
-                           uint32_t bin_values) {
+                           uint32_t bin_values, bool sliced, uint32_t cap_groups, uint32_t waves) {
   if (g.pr_bin && g.pr_bin->push_count == push_count && g.pr_bin->hot_count == hot_count &&
-      g.pr_bin->stage == stage && g.pr_bin->bin_rows == bin_rows && g.pr_bin->bin_values == bin_values)
+      g.pr_bin->stage == stage && g.pr_bin->bin_rows == bin_rows && g.pr_bin->bin_values == bin_values &&
+      g.pr_bin->sliced == sliced && (!sliced || (g.pr_bin->cap_groups == cap_groups && g.pr_bin->waves == waves)))
     return *g.pr_bin;
   g.pr_bin.reset();
   Stopwatch sw;
@@ -195,10 +203,41 @@ static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, ui
     m.allocate(std::max<size_t>(bytes, 16));
     if (bytes > 0) copy_to_device(dev, m.get(), src, bytes);
   };
-  // the 16-bit stream padded to whole 512-entry chunks with entries that read the zero
-  h.hot16.resize(static_cast<size_t>(p.chunks) * 512, static_cast<uint16_t>(hot_count));
-  up(p.hot16, h.hot16.data(), sizeof(uint16_t) * h.hot16.size());
-  up(p.hoff, h.hoff.data(), sizeof(uint32_t) * h.hoff.size());
+  p.sliced = sliced;
+  if (sliced) {
+    std::vector<uint32_t> hub_len(h.pieces());
+    for (uint32_t i = 0; i < h.pieces(); ++i) {
+      const uint32_t row = h.hub_pieces[2 * i], b = h.hub_pieces[2 * i + 1];
+      hub_len[i] = std::min(b + 512u, g.rl.host_offsets[row + 1]) - b;
+    }
+    PrSliceLayout sl;
+    try {
+      sl = build_pr_slice_layout(h.hoff.data(), h.hot16.data(), h.zero_rows, h.push_base, hot_count, h.hub_pieces,
+                                 hub_len, cap_groups, waves,
+                                 [](uint64_t count, const std::function<void(uint64_t, uint64_t)>& fn) {
+                                   parallel_for(count, fn, 1);
+                                 });
+    } catch (const std::runtime_error& e) {
+      fail_kernel(e.what());
+    }
+    p.cap_groups = cap_groups;
+    p.waves = waves;
+    p.slices = sl.slices;
+    p.slice_items = sl.item_count();
+    p.long_count = sl.long_count();
+    p.partial_slots = sl.partial_slots;
+    p.slice_entries = sl.slice_off.back();  // without the padding past the last slice
+    up(p.slice_data, sl.data.data(), sizeof(uint16_t) * sl.data.size());
+    up(p.items, sl.items.data(), sizeof(uint32_t) * sl.items.size());
+    up(p.wave_first, sl.wave_first.data(), sizeof(uint32_t) * sl.wave_first.size());
+    up(p.long_slices, sl.long_slices.data(), sizeof(uint32_t) * sl.long_slices.size());
+    p.slice_partial.allocate(sizeof(double) * 64 * std::max<uint32_t>(sl.partial_slots, 1));
+  } else {
+    // the 16-bit stream padded to whole 512-entry chunks with entries that read the zero
+    h.hot16.resize(static_cast<size_t>(p.chunks) * 512, static_cast<uint16_t>(hot_count));
+    up(p.hot16, h.hot16.data(), sizeof(uint16_t) * h.hot16.size());
+    up(p.hoff, h.hoff.data(), sizeof(uint32_t) * h.hoff.size());
+  }
   up(p.slots, h.slots.data(), sizeof(uint16_t) * h.slots.size());
   up(p.block_group, h.block_group.data(), sizeof(uint32_t) * h.block_group.size());
   up(p.sub, h.sub.data(), sizeof(uint32_t) * h.sub.size());
@@ -221,11 +260,13 @@ static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, ui
   const uint32_t pull = p.push_base - p.zero_rows;
   p.cold_sum.allocate(sizeof(double) * std::max<uint32_t>(pull, 1));
   fill_words(dev, p.cold_sum.get(), 2ull * std::max<uint32_t>(pull, 1), 0);  // stays 0 without cold entries
-  const uint64_t words = std::max<uint64_t>(1, static_cast<uint64_t>(p.chunks) * 16);  // whole chunks
-  p.starts.allocate(sizeof(uint32_t) * words);
-  fill_words(dev, p.starts.get(), words, 0);
-  p.chunk_row.allocate(sizeof(uint32_t) * std::max<uint32_t>(p.chunks, 1));
-  if (pull > 0) {
+  if (!sliced) {
+    const uint64_t words = std::max<uint64_t>(1, static_cast<uint64_t>(p.chunks) * 16);  // whole chunks
+    p.starts.allocate(sizeof(uint32_t) * words);
+    fill_words(dev, p.starts.get(), words, 0);
+    p.chunk_row.allocate(sizeof(uint32_t) * std::max<uint32_t>(p.chunks, 1));
+  }
+  if (!sliced && pull > 0) {
     const uint32_t* d_hoff = p.hoff.as<const uint32_t>();
     Grid1D sgrid = thread_per_item(pull);
     launch(dev, get_kernel(dev, "pagerank", "pr_merge_starts"), sgrid.blocks, sgrid.threads, 0, d_hoff,
@@ -240,10 +281,10 @@ static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, ui
   return *g.pr_bin;
 }
 
-// Push-pull iterations on a symmetric graph (pr_spmv_bin with pr_bin_scatter
-// and pr_bin_accum before it, or pr_spmv_merge, both then pr_merge_finish; or
-// pr_spmv_push and pr_push_finish; then pr_finalize; the kernels' comments give
-// the method). One 1024-lane workgroup per CU holds push_count 8-byte
+// Push-pull iterations on a symmetric graph (pr_bin_scatter and pr_bin_accum,
+// then pr_spmv_slice and pr_slice_finish or pr_spmv_bin and pr_merge_finish;
+// or pr_spmv_merge and pr_merge_finish; or pr_spmv_push and pr_push_finish;
+// then pr_finalize; the kernels' comments give the method). One 1024-lane workgroup per CU holds push_count 8-byte
 // accumulators and hot_count FP32 contributions in LDS: 6,144 and 26,624 (152
 // KB of the 160 KB). At R-MAT scale 22 that leaves 36.5% of the entries cold
 // (below the hot set: binned, or gathered from global memory by the merge and
@@ -257,7 +298,8 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
                    uint32_t* iterations, int32_t* converged, PrKernel kind) {
   Device& dev = *g.dev;
   const uint32_t n = g.n;
-  const bool merge = kind == PrKernel::kMerge, bin = kind == PrKernel::kBin;
+  const bool merge = kind == PrKernel::kMerge, bin = kind == PrKernel::kBin, slice = kind == PrKernel::kSlice;
+  const bool binned = bin || slice;  // the cold entries binned (pr_bin_scatter, pr_bin_accum)
   const uint32_t cu = static_cast<uint32_t>(std::max(1, dev.cu_count));
   // workgroups per CU (GC_PR_WGS, 1 or 2): each holds its own sets in its share of the LDS
   const uint32_t wgs = std::min(2u, std::max(1u, env_u32("GC_PR_WGS", 1u)));
@@ -275,7 +317,7 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   const uint32_t hot_cap = static_cast<uint32_t>((lds_cap - 8ull * push_count) / 4) - 1u;
   const uint32_t hot_count = std::min({n, std::max(push_count, env_u32("GC_PR_HOT", 26624u)), hot_cap});
   const uint32_t push_base = n - push_count;
-  const uint32_t shmem = static_cast<uint32_t>(8ull * push_count + 4ull * (hot_count + (bin ? 1u : 0u)));
+  const uint32_t shmem = static_cast<uint32_t>(8ull * push_count + 4ull * (hot_count + (binned ? 1u : 0u)));
   const PrSegments segs = plan_segments(g.rl.host_offsets, push_base, kPrHotThreads, false,
                                         env_u32("GC_PR_SHIFT", 2u), rows_per_group);
   const uint32_t push_tiles = (push_count + 15) / 16;  // 16 hub rows (one wave each) per tile
@@ -283,10 +325,10 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   const uint32_t hub_blocks = (push_count + 15) / 16;
   // the nonzero-split pull: rows [zero_rows, push_base) own the entries [0, entries)
   const uint32_t zero_rows = std::min(g.rl.dangling, push_base);
-  if ((merge || bin) && g.rl.host_offsets[zero_rows] != 0)
+  if ((merge || binned) && g.rl.host_offsets[zero_rows] != 0)
     fail_kernel("PageRank: the degree-ordered rows do not start with the rows of degree 0");
   PrBinPlan* plan = nullptr;
-  if (bin) {
+  if (binned) {
     // Defaults measured at R-MAT scale 22 (per launch, scatter + accumulation): 36,864 staged
     // entries and 65,536 values per bin 136 us; 40,448 and 196,608 124 us (longer runs per bin).
     const uint32_t stage = std::max(512u, std::min(env_u32("GC_PR_STAGE", 40448u),
@@ -294,12 +336,17 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
     const uint32_t bin_rows = std::max(1u, std::min(env_u32("GC_PR_BIN_ROWS", 16384u),
                                                     static_cast<uint32_t>(lds_limit / 8 - 1)));
     const uint32_t bin_values = std::max(8u, env_u32("GC_PR_BIN_VALUES", 196608u));
-    plan = &bin_plan(g, push_count, hot_count, stage, bin_rows, bin_values);
+    // pr_spmv_slice: slices wider than this many groups of eight entries per lane are cut
+    // into pieces (GC_PR_SLICE_CAP)
+    const uint32_t cap_groups = std::max(1u, std::min(env_u32("GC_PR_SLICE_CAP", 16u), 4096u));
+    plan = &bin_plan(g, push_count, hot_count, stage, bin_rows, bin_values, slice, cap_groups,
+                     grid * (kPrHotThreads / 64));
     plan->grid = cu * wgs;
   }
   const uint32_t entries = bin ? plan->hot_entries : (merge ? g.rl.host_offsets[push_base] : 0u);
   const uint32_t chunks = static_cast<uint32_t>((static_cast<uint64_t>(entries) + 511) / 512);
-  const uint32_t fix_blocks = (chunks + 255) / 256;
+  // finishing blocks after the hubs': 256 chunk tails each, or four wide slices
+  const uint32_t fix_blocks = slice ? (plan->long_count + 3) / 4 : (chunks + 255) / 256;
   const uint32_t finish_blocks = hub_blocks + fix_blocks;
   const uint32_t partial_blocks = grid + finish_blocks;
   float* rank[2] = {g.buffer<float>("pr.rank0", sizeof(float) * n),
@@ -340,7 +387,7 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   Grid1D igrid = thread_per_item(n);
   launch(dev, get_kernel(dev, "pagerank", "pr_init"), igrid.blocks, igrid.threads, 0, off, n, r0, rank[0],
          contrib[0]);
-  if (!bin)  // the binned layout holds the hubs' entries inside the push set as pieces
+  if (!binned)  // the binned layout holds the hubs' entries inside the push set as pieces
     launch(dev, get_kernel(dev, "pagerank", "pr_push_split"), (push_count + 255) / 256, 256u, 0, off, tgt, n,
            push_count, psplit);
   // the row-start bitmap and the chunks' first rows depend on the graph and the push set only:
@@ -359,12 +406,13 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   PrState init{static_cast<double>(g.rl.dangling) * static_cast<double>(r0), 0.0, 0, 0};
   copy_to_device(dev, state, &init, sizeof(init));
 
-  const char* spmv_name = bin ? "pr_spmv_bin" : (merge ? "pr_spmv_merge" : "pr_spmv_push");
+  const char* spmv_name = slice ? "pr_spmv_slice" : bin ? "pr_spmv_bin" : (merge ? "pr_spmv_merge" : "pr_spmv_push");
   Kernel spmv = get_kernel(dev, "pagerank", spmv_name);
-  Kernel finish = get_kernel(dev, "pagerank", merge || bin ? "pr_merge_finish" : "pr_push_finish");
+  Kernel finish = get_kernel(dev, "pagerank",
+                             slice ? "pr_slice_finish" : (merge || bin ? "pr_merge_finish" : "pr_push_finish"));
   Kernel finalize = get_kernel(dev, "pagerank", "pr_finalize");
   Kernel scatter{}, accum{};
-  if (bin) {
+  if (binned) {
     scatter = get_kernel(dev, "pagerank", "pr_bin_scatter");
     accum = get_kernel(dev, "pagerank", "pr_bin_accum");
   }
@@ -374,7 +422,7 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   for (uint32_t it = 0; it < max_iterations; ++it) {
     const int in = it & 1, out = in ^ 1;
     hipError_t err;
-    if (bin) {
+    if (binned) {
       if (plan->cold_entries > 0) {
         err = try_launch(dev, scatter, plan->subs, kPrHotThreads, 4u * (plan->stage + 1),
                          static_cast<const float*>(contrib[in]), plan->zero_rows, plan->hot_base,
@@ -389,6 +437,14 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
                          plan->cold_sum.as<double>(), static_cast<const PrState*>(state));
         if (err != hipSuccess) fail_kernel(std::string("pr_bin_accum launch failed: ") + hipGetErrorString(err));
       }
+      if (slice) {
+        err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, plan->slice_data.as<const uint16_t>(),
+                         plan->items.as<const uint4>(), plan->wave_first.as<const uint32_t>(),
+                         plan->cold_sum.as<const double>(), static_cast<const float*>(contrib[in]),
+                         static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d,
+                         static_cast<const PrState*>(state), hot_count, push_count, zero_rows,
+                         plan->slice_partial.as<double>(), push_partials, partials);
+      } else {
       err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, plan->hot16.as<const uint16_t>(),
                        plan->cold_sum.as<const double>(), static_cast<const float*>(contrib[in]),
                        static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d,
@@ -396,6 +452,7 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
                        plan->hub_pieces.as<const uint32_t>(), plan->pieces, chunks, zero_rows,
                        reinterpret_cast<const uint8_t*>(starts), static_cast<const uint32_t*>(chunk_row),
                        head_partial, tail_partial, tail_row, push_partials, partials);
+      }
     } else if (merge) {
       err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, static_cast<const float*>(contrib[in]),
                        static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d,
@@ -410,7 +467,13 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
                        static_cast<const uint32_t*>(psplit), push_tiles, rows_per_group, push_partials, partials);
     }
     if (err != hipSuccess) fail_kernel(std::string(spmv_name) + " launch failed: " + hipGetErrorString(err));
-    if (merge || bin) {
+    if (slice) {
+      launch(dev, finish, finish_blocks, 256u, 0, off, static_cast<const float*>(rank[in]), contrib[out],
+             rank[out], n, d, static_cast<const PrState*>(state), push_count,
+             static_cast<const unsigned long long*>(push_partials), grid, hub_blocks,
+             plan->long_slices.as<const uint32_t>(), plan->long_count, plan->slice_partial.as<const double>(),
+             plan->cold_sum.as<const double>(), zero_rows, partials, grid);
+    } else if (merge || bin) {
       launch(dev, finish, finish_blocks, 256u, 0, off, static_cast<const float*>(rank[in]), contrib[out],
              rank[out], n, d, static_cast<const PrState*>(state), push_count,
              static_cast<const unsigned long long*>(push_partials), grid, hub_blocks, chunks,
@@ -444,7 +507,7 @@ void pagerank(Graph& g, float damping, uint32_t max_iterations, float tolerance,
   prepare(g);
   const uint32_t n = g.n;
   const PrKernel kind = choose_kernel(g);
-  if (kind == PrKernel::kPush || kind == PrKernel::kMerge || kind == PrKernel::kBin) {
+  if (kind == PrKernel::kPush || kind == PrKernel::kMerge || kind == PrKernel::kBin || kind == PrKernel::kSlice) {
     pagerank_push(g, damping, max_iterations, tolerance, d_out, iterations, converged, kind);
     return;
   }
@@ -623,17 +686,19 @@ gc_status gc_pagerank_layout(gc_graph* h, uint64_t* stats, uint32_t count) {
     gc::GraphCall call(h);
     if (stats == nullptr && count > 0) gc::fail_kernel("stats is null");
     const gc::PrBinPlan* p = call.g->pr_bin.get();
-    uint64_t v[19] = {};
+    uint64_t v[25] = {};
     if (p != nullptr) {
-      const uint64_t vals[19] = {p->push_count, p->hot_count, p->stage, p->bin_rows, p->bin_values,
+      const uint64_t vals[25] = {p->push_count, p->hot_count, p->stage, p->bin_rows, p->bin_values,
                                  p->cold_entries, p->values, p->slot_rows, p->hot_entries, p->chunks,
                                  p->subs, p->bins, p->runs, p->pieces, p->piece_entries,
                                  static_cast<uint64_t>(p->push_base - p->zero_rows),
                                  static_cast<uint64_t>(p->hot_base > p->zero_rows ? p->hot_base - p->zero_rows : 0),
-                                 static_cast<uint64_t>(p->build_seconds * 1e9), p->grid};
+                                 static_cast<uint64_t>(p->build_seconds * 1e9), p->grid,
+                                 p->sliced ? 1u : 0u, p->slices, p->slice_entries, p->slice_items,
+                                 p->long_count, p->partial_slots};
       std::memcpy(v, vals, sizeof(v));
     }
-    for (uint32_t i = 0; i < count && i < 19; ++i) stats[i] = v[i];
+    for (uint32_t i = 0; i < count && i < 25; ++i) stats[i] = v[i];
   });
 }
 

@@ -1608,3 +1608,254 @@ extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_bin(
     prm_zero_item(item - piece_count - chunks, zero_rows, base, rank_in, rank_out, contrib_out, dangling, diff);
   prm_end(wave, s_push, push_count, push_partials, partials, dangling, diff, s_part);
 }
+
+// ---- sliced hot stream (GC_PR_KERNEL=slice) --------------------------------
+//
+// The pull over the hot entries with one lane per row: slice s holds rows
+// z + 64 s + l (lane l), its entries interleaved so that group g (eight
+// entries per lane, 16 bytes) sits at [512 g + 8 l ..] and, after the G
+// groups, the remainder of R entries per lane (R = 0, 2, 4, 6) at
+// [512 G + R l ..]; entries hot_count (a row's end, rows past the last pull
+// row) read the zero held after the hot set. A lane sums its row in FP64 and
+// pushes its own c into the LDS accumulators for entries in P, with no
+// exchange between lanes, where the chunked stream of pr_spmv_bin needs a
+// segmented scan across the wave and a shuffle per entry. Widths of the 64
+// rows differ, and the padding is what that costs: 1.24x the hot entries at
+// R-MAT scale 22.
+//
+// Items (four words, pr_slice_layout.hpp): a whole slice (its rows finished
+// here), a piece of a wide slice (its 64 sums go to slice_partial, and
+// pr_slice_finish adds a row's pieces in order) or a hub piece (pushes only,
+// as prb_hub_piece). Wave w takes items [wave_first[w], wave_first[w + 1]),
+// dealt on the host longest first to the wave with the least work, its hub
+// pieces first; then the rows of degree 0 in runs of 512, round-robin.
+//
+// The slice items run one ahead: the next item's row data, its first four
+// groups (clamped to its last group) and its remainder are loaded before the
+// current item is summed, so a wave keeps ten loads in flight while it works;
+// an item of more than four groups loads the rest itself. The descriptors
+// come by scalar loads two items ahead. The row data and the stores carry no
+// branch (clamped loads, buffer stores dropped out of range), so that the
+// waits for the loads ahead stay exact.
+typedef uint32_t prs_u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+struct PrsIn {
+  prb_u16x8 v[4];
+  prs_u32x4a4 rem;
+  float cst, rin;
+  double cold;
+  uint32_t off0, off1;
+};
+
+__device__ __forceinline__ PrsIn prs_load(uint32_t x, uint32_t y, uint32_t zw, const uint16_t* __restrict__ data,
+                                          const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
+                                          const double* __restrict__ cold_sum, const uint32_t* __restrict__ offsets,
+                                          uint32_t zero_rows, uint32_t push_base) {
+  PrsIn in;
+  const uint32_t lane = lane_id();
+  const uint32_t rc = min(zero_rows + 64u * (x & 0x3FFFFFFFu) + lane, push_base - 1);
+  const uint32_t ng = zw & 0xFFFFu, R = zw >> 16;
+  const uint32_t glast = ng > 0 ? ng - 1 : 0u;
+  const prb_u16x8* grp = reinterpret_cast<const prb_u16x8*>(data + y) + lane;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) in.v[q] = __builtin_nontemporal_load(grp + 64u * min(static_cast<uint32_t>(q), glast));
+  in.rem = *reinterpret_cast<const prs_u32x4a4*>(data + y + 512u * ng + R * lane);
+  in.cst = contrib_in[rc];
+  in.rin = rank_in[rc];
+  in.cold = cold_sum[rc - zero_rows];
+  in.off0 = offsets[rc];
+  in.off1 = offsets[rc + 1];
+  return in;
+}
+
+__device__ __forceinline__ void prs_entry(uint32_t t, const float* __restrict__ s_hot,
+                                          unsigned long long* __restrict__ s_push, uint32_t push_off,
+                                          uint32_t push_count, unsigned long long cfix, double& acc) {
+  acc += static_cast<double>(s_hot[t]);
+  if (t - push_off < push_count) atomicAdd(s_push + (t - push_off), cfix);
+}
+
+// Sums one slice item from its loads: the first four groups, the remainder,
+// then (loaded here) the further groups; a whole slice's rows are finished, a
+// piece's sums written to its slot.
+__device__ __forceinline__ void prs_item(uint32_t x, uint32_t y, uint32_t zw, uint32_t slot, const PrsIn& in,
+                                         const uint16_t* __restrict__ data, const float* __restrict__ s_hot,
+                                         unsigned long long* __restrict__ s_push, uint32_t push_off,
+                                         uint32_t push_count, uint32_t zero_rows, uint32_t push_base,
+                                         const PrbOut& out, double* __restrict__ slice_partial, double base,
+                                         double damping, double& diff) {
+  const uint32_t lane = lane_id();
+  const uint32_t ng = zw & 0xFFFFu, R = zw >> 16;
+  const unsigned long long cfix = pr_to_fixed(in.cst);
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (static_cast<uint32_t>(q) < ng) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) prs_entry(in.v[q][i], s_hot, s_push, push_off, push_count, cfix, acc);
+    }
+  }
+  const prb_u16x8 rem = __builtin_bit_cast(prb_u16x8, in.rem);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if (static_cast<uint32_t>(i) < R) prs_entry(rem[i], s_hot, s_push, push_off, push_count, cfix, acc);
+  }
+  // the row data is taken here, before any further loads, while the waits for it are exact on
+  // every path
+  __asm__ volatile("" ::"v"(in.cold), "v"(in.rin), "v"(in.off0), "v"(in.off1));
+  if (ng > 4) {
+    const prb_u16x8* grp = reinterpret_cast<const prb_u16x8*>(data + y) + lane;
+    for (uint32_t g = 4; g < ng; g += 4) {
+      prb_u16x8 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = __builtin_nontemporal_load(grp + 64u * min(g + q, ng - 1));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (g + q < ng) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) prs_entry(v[q][i], s_hot, s_push, push_off, push_count, cfix, acc);
+        }
+      }
+    }
+  }
+  const uint32_t row = zero_rows + 64u * (x & 0x3FFFFFFFu) + lane;
+  if ((x >> 30) == 0u) {
+    const bool valid = row < push_base;
+    const float r = static_cast<float>(base + damping * (acc + in.cold));
+    prb_st32(out.rank, row * 4u, valid, __builtin_bit_cast(uint32_t, r));
+    prb_st32(out.contrib, row * 4u, valid,
+             __builtin_bit_cast(uint32_t, r / static_cast<float>(in.off1 - in.off0)));
+    diff += valid ? fabs(static_cast<double>(r) - static_cast<double>(in.rin)) : 0.0;
+  } else {
+    slice_partial[static_cast<uint64_t>(slot) * GC_WAVE + lane] = acc;
+  }
+}
+
+// Dynamic LDS: push_count u64 accumulators, then hot_count + 1 contributions
+// (the last one 0), as pr_spmv_bin.
+extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_slice(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const uint16_t* __restrict__ data, const uint4* __restrict__ items, const uint32_t* __restrict__ wave_first,
+    const double* __restrict__ cold_sum, const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t hot_count, uint32_t push_count, uint32_t zero_rows,
+    double* __restrict__ slice_partial, unsigned long long* __restrict__ push_partials,
+    double* __restrict__ partials) {
+  extern __shared__ unsigned long long s_dyn[];
+  unsigned long long* s_push = s_dyn;
+  float* s_hot = reinterpret_cast<float*>(s_dyn + push_count);
+  __shared__ double s_part[2][PRP_THREADS / GC_WAVE];
+  if (state->done) return;
+  const uint32_t hot_base = n - hot_count;
+  const uint32_t push_base = n - push_count;
+  const uint32_t push_off = hot_count - push_count;
+  lds_fill_f32<16>(s_hot, contrib_in + hot_base, hot_count, PRP_THREADS);
+  if (threadIdx.x == 0) s_hot[hot_count] = 0.0f;
+  for (uint32_t i = threadIdx.x; i < push_count; i += PRP_THREADS) s_push[i] = 0ull;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GC_WAVE);
+  const uint32_t gw = blockIdx.x * (PRP_THREADS / GC_WAVE) + wave;
+  const uint32_t k0 = wave_first[gw], k1 = wave_first[gw + 1];
+  if (k0 < k1) {
+    PrbOut out;
+    out.rank = prb_rsrc(rank_out, n * 4u);
+    out.contrib = prb_rsrc(contrib_out, n * 4u);
+    uint32_t k = k0;
+    for (; k < k1; ++k) {
+      const uint4 it = items[k];
+      if ((it.x >> 30) != 2u) break;
+      prb_hub_piece(it.y, it.z, offsets, targets, s_hot, hot_base, s_push, push_base);
+    }
+    if (k < k1) {
+      // slice items, one ahead; the last item's loads are issued again past the end
+      uint4 da = items[k];
+      uint4 db = items[min(k + 1, k1 - 1)];
+      PrsIn a = prs_load(da.x, da.y, da.z, data, contrib_in, rank_in, cold_sum, offsets, zero_rows, push_base);
+      while (true) {
+        const uint4 dc = items[min(k + 2, k1 - 1)];
+        PrsIn b = prs_load(db.x, db.y, db.z, data, contrib_in, rank_in, cold_sum, offsets, zero_rows, push_base);
+        prs_item(da.x, da.y, da.z, da.w, a, data, s_hot, s_push, push_off, push_count, zero_rows, push_base, out,
+                 slice_partial, base, damping, diff);
+        if (++k >= k1) break;
+        da = db;
+        db = dc;
+        a = b;
+      }
+    }
+  }
+  const uint32_t zero_items = (zero_rows + PRM_CHUNK - 1) / PRM_CHUNK;
+  for (uint32_t item = gw; item < zero_items; item += gridDim.x * (PRP_THREADS / GC_WAVE))
+    prm_zero_item(item, zero_rows, base, rank_in, rank_out, contrib_out, dangling, diff);
+  prm_end(wave, s_push, push_count, push_partials, partials, dangling, diff, s_part);
+}
+
+// The hub rows as pr_push_finish (blocks [0, hub_blocks), 16 hubs each), then
+// the rows of the wide slices (blocks from hub_blocks on, four slices of 64
+// rows each): long_slices holds (slice, first partial slot, pieces); a row's
+// sum is its pieces' sums in order plus its cold sum. Per-block dangling and
+// L1 partials go to partials from partial_base on.
+extern "C" __global__ __launch_bounds__(256) void pr_slice_finish(
+    const uint32_t* __restrict__ offsets, const float* __restrict__ rank_in,
+    float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
+    const PrState* __restrict__ state, uint32_t push_count,
+    const unsigned long long* __restrict__ push_partials, uint32_t parts, uint32_t hub_blocks,
+    const uint32_t* __restrict__ long_slices, uint32_t long_count, const double* __restrict__ slice_partial,
+    const double* __restrict__ cold_sum, uint32_t zero_rows, double* __restrict__ partials,
+    uint32_t partial_base) {
+  __shared__ unsigned long long s_sum[256];
+  __shared__ double s_part[2][256 / GC_WAVE];
+  if (state->done) return;
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double base = (1.0 - damping) * inv_n + damping * state->dangling * inv_n;
+  double dangling = 0.0, diff = 0.0;
+  if (blockIdx.x < hub_blocks) {
+    const uint32_t hub = threadIdx.x % PRF_HUBS, j = threadIdx.x / PRF_HUBS;
+    const uint32_t i = blockIdx.x * PRF_HUBS + hub;
+    unsigned long long sum = 0ull;
+    if (i < push_count) {
+      for (uint32_t w = j; w < parts; w += 256 / PRF_HUBS) sum += push_partials[static_cast<uint64_t>(w) * push_count + i];
+    }
+    s_sum[threadIdx.x] = sum;
+    __syncthreads();
+    if (j == 0 && i < push_count) {
+      unsigned long long y = 0ull;
+      for (uint32_t q = 0; q < 256 / PRF_HUBS; ++q) y += s_sum[q * PRF_HUBS + hub];
+      const uint32_t row = n - push_count + i;
+      pr_finish_row(row, offsets[row + 1] - offsets[row], static_cast<double>(y) / PRP_FIX_SCALE, base, damping,
+                    rank_in, rank_out, contrib_out, dangling, diff);
+    }
+  } else {
+    const uint32_t q = (blockIdx.x - hub_blocks) * (256 / GC_WAVE) + threadIdx.x / GC_WAVE;
+    if (q < long_count) {
+      const uint32_t s = long_slices[3 * q], p0 = long_slices[3 * q + 1], np = long_slices[3 * q + 2];
+      const uint32_t row = zero_rows + 64u * s + lane_id();
+      if (row < n - push_count) {
+        double acc = 0.0;
+        for (uint32_t p = 0; p < np; ++p) acc += slice_partial[static_cast<uint64_t>(p0 + p) * GC_WAVE + lane_id()];
+        pr_finish_row(row, offsets[row + 1] - offsets[row], acc + cold_sum[row - zero_rows], base, damping, rank_in,
+                      rank_out, contrib_out, dangling, diff);
+      }
+    }
+  }
+  dangling = wave_sum(dangling);
+  diff = wave_sum(diff);
+  if (lane_id() == 0) {
+    s_part[0][threadIdx.x / GC_WAVE] = dangling;
+    s_part[1][threadIdx.x / GC_WAVE] = diff;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    for (int w = 0; w < 256 / GC_WAVE; ++w) {
+      d0 += s_part[0][w];
+      d1 += s_part[1][w];
+    }
+    const uint64_t slot = partial_base + blockIdx.x;
+    partials[2 * slot] = d0;
+    partials[2 * slot + 1] = d1;
+  }
+}

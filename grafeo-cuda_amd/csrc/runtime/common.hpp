@@ -330,6 +330,12 @@ struct PrBinPlan {
   uint32_t pieces = 0;
   uint64_t runs = 0, piece_entries = 0;
   uint32_t grid = 0;  // workgroups of the last pr_spmv_bin launches
+  // the sliced hot stream (pr_slice_layout.hpp), when the plan was built for pr_spmv_slice
+  bool sliced = false;
+  uint32_t cap_groups = 0, waves = 0;  // key
+  uint32_t slices = 0, slice_items = 0, long_count = 0, partial_slots = 0;
+  uint64_t slice_entries = 0;
+  DeviceMemory slice_data, items, wave_first, long_slices, slice_partial;
 };
 
 struct Graph {

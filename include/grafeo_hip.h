@@ -254,11 +254,13 @@ gc_status gc_pagerank_device(gc_graph* g, float damping, uint32_t max_iterations
 /* Sizes of the binned layout the last gc_pagerank / gc_pagerank_device call on a
  * symmetric graph used (the default SpMV; zeros if none was built). Diagnostic,
  * no reference counterpart: the benchmark states its kernels' bytes from them.
- * Fills min(count, 19) values: push set, hot set, staged entries per scatter
+ * Fills min(count, 25) values: push set, hot set, staged entries per scatter
  * workgroup, rows and values per bin (the limits), cold entries, value buffer
  * length, slot rows, 16-bit stream entries, stream chunks, scatter sub-ranges,
  * bins, runs, hub pieces, hub-piece entries, pull rows, cold sources, the
- * layout's build time in nanoseconds, and the pull kernel's workgroups. */
+ * layout's build time in nanoseconds, the pull kernel's workgroups; then 1 if
+ * the hot stream is sliced (GC_PR_KERNEL=slice), slices, sliced entries (with
+ * padding), work items, wide slices and their pieces. */
 gc_status gc_pagerank_layout(gc_graph* g, uint64_t* stats, uint32_t count);
 
 /* Multi-source BFS (an addition with no reference function; for each source

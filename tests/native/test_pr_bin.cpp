@@ -1,9 +1,12 @@
-// Host-only test of PageRank's binned layout (csrc/algos/pr_bin_layout.hpp):
-// on random symmetric graphs in degree order it replays the indexing of
-// pr_bin_scatter, pr_bin_accum and pr_spmv_bin (pagerank.hip) with integer
+// Host-only test of PageRank's binned layout (csrc/algos/pr_bin_layout.hpp)
+// and of the sliced hot stream built from it (pr_slice_layout.hpp): on random
+// symmetric graphs in degree order it replays the indexing of pr_bin_scatter,
+// pr_bin_accum, pr_spmv_bin and pr_spmv_slice (pagerank.hip) with integer
 // contributions and checks that every pull row's cold sum plus its 16-bit
 // stream sum equals the sum over its neighbours, that every staged slot and
-// every value is written exactly once, and that padding stays inert.
+// every value is written exactly once, that the slices hand each lane its
+// row's hot entries in order, every sliced entry read once and every item
+// dealt to one wave, and that padding stays inert.
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
@@ -15,6 +18,7 @@
 #include <vector>
 
 #include "pr_bin_layout.hpp"
+#include "pr_slice_layout.hpp"
 
 namespace {
 
@@ -85,6 +89,123 @@ void serial(uint64_t count, const std::function<void(uint64_t, uint64_t)>& fn) {
     const uint64_t e = std::min(count, b + 1 + next_u64() % 97);
     fn(b, e);
     b = e;
+  }
+}
+
+// pr_spmv_slice's reads of the sliced layout, replayed for every item
+void check_slices(const Csr& g, const gc::PrBinLayout& p, uint32_t hot, uint32_t cap, uint32_t waves,
+                  const std::string& tag0) {
+  const std::string tag = tag0 + " cap=" + std::to_string(cap) + " waves=" + std::to_string(waves);
+  std::vector<uint32_t> hub_len(p.pieces());
+  for (uint32_t i = 0; i < p.pieces(); ++i) {
+    const uint32_t h = p.hub_pieces[2 * i], b = p.hub_pieces[2 * i + 1];
+    hub_len[i] = std::min(b + 512u, g.off[h + 1]) - b;
+  }
+  const gc::PrSliceLayout sl = gc::build_pr_slice_layout(p.hoff.data(), p.hot16.data(), p.zero_rows, p.push_base,
+                                                         hot, p.hub_pieces, hub_len, cap, waves, serial);
+  const uint32_t z = p.zero_rows, pb = p.push_base, S = sl.slices;
+  check(S == (pb - z + 63) / 64, tag + ": slice count");
+  check(sl.slice_off.size() == S + 1u && sl.slice_off[S] + gc::kPrSlicePad == sl.data.size(), tag + ": slice offsets");
+  check(sl.wave_first.size() == waves + 1u && sl.wave_first[0] == 0 && sl.wave_first[waves] == sl.item_count(),
+        tag + ": wave ranges");
+  for (uint32_t w = 0; w < waves; ++w) check(sl.wave_first[w] <= sl.wave_first[w + 1], tag + ": wave ranges ordered");
+  std::vector<uint32_t> reads(sl.data.size(), 0);
+  std::vector<uint32_t> hub_seen(p.pieces(), 0);
+  // per slice: its pieces by partial slot (or the whole slice), each lane's entries in order
+  std::vector<std::vector<std::pair<uint32_t, std::vector<std::vector<uint16_t>>>>> got(S);
+  std::vector<uint32_t> whole(S, 0), slot_used(sl.partial_slots, 0);
+  for (uint32_t k = 0; k < sl.item_count(); ++k) {
+    const uint32_t* it = &sl.items[4 * static_cast<size_t>(k)];
+    const uint32_t kind = it[0] >> 30, idx = it[0] & 0x3FFFFFFFu;
+    if (kind == gc::kPrSliceHub) {
+      check(idx < p.pieces() && it[1] == p.hub_pieces[2 * idx] && it[2] == p.hub_pieces[2 * idx + 1],
+            tag + ": hub item");
+      if (idx < p.pieces()) ++hub_seen[idx];
+      continue;
+    }
+    check(kind <= 1 && idx < S, tag + ": slice item");
+    if (kind > 1 || idx >= S) continue;
+    const uint32_t ng = it[2] & 0xFFFFu, R = it[2] >> 16;
+    check(R <= 6 && R % 2 == 0 && ng <= cap, tag + ": piece size");
+    std::vector<std::vector<uint16_t>> lanes(64);
+    for (uint32_t l = 0; l < 64; ++l) {
+      for (uint32_t q = 0; q < ng; ++q)
+        for (uint32_t i = 0; i < 8; ++i) {
+          const uint64_t at = static_cast<uint64_t>(it[1]) + 512ull * q + 8 * l + i;
+          check(at < sl.data.size(), tag + ": group read in range");
+          if (at >= sl.data.size()) continue;
+          ++reads[at];
+          lanes[l].push_back(sl.data[at]);
+        }
+      // the remainder: one 16-byte load per lane, its first R entries used
+      const uint64_t rem = static_cast<uint64_t>(it[1]) + 512ull * ng + static_cast<uint64_t>(R) * l;
+      check(rem + 8 <= sl.data.size(), tag + ": remainder load in range");
+      for (uint32_t i = 0; i < R && rem + i < sl.data.size(); ++i) {
+        ++reads[rem + i];
+        lanes[l].push_back(sl.data[rem + i]);
+      }
+      // the four group loads of a batch, clamped to the item's last group (to group 0 without groups)
+      for (uint32_t q = 0; q < 4; ++q) {
+        const uint64_t at = static_cast<uint64_t>(it[1]) + 512ull * std::min(q, ng > 0 ? ng - 1 : 0u) + 8 * l;
+        check(at + 8 <= sl.data.size(), tag + ": clamped group load in range");
+      }
+    }
+    if (kind == 0) {
+      ++whole[idx];
+      got[idx].push_back({0u, lanes});
+    } else {
+      check(it[3] < sl.partial_slots, tag + ": partial slot in range");
+      if (it[3] < sl.partial_slots) ++slot_used[it[3]];
+      got[idx].push_back({it[3], lanes});
+    }
+  }
+  for (uint32_t i = 0; i < p.pieces(); ++i) check(hub_seen[i] == 1, tag + ": hub piece dealt once");
+  for (size_t i = 0; i < reads.size(); ++i) check(reads[i] == (i < sl.slice_off[S] ? 1u : 0u), tag + ": sliced entry read once");
+  // each wave's hub pieces come before its slice items
+  for (uint32_t w = 0; w < waves; ++w) {
+    bool slices_seen = false;
+    for (uint32_t k = sl.wave_first[w]; k < sl.wave_first[w + 1]; ++k) {
+      const bool hub = (sl.items[4 * static_cast<size_t>(k)] >> 30) == gc::kPrSliceHub;
+      check(!(hub && slices_seen), tag + ": hub pieces first in a wave");
+      slices_seen = slices_seen || !hub;
+    }
+  }
+  for (uint32_t q = 0; q < sl.partial_slots; ++q) check(slot_used[q] == 1, tag + ": partial slot written once");
+  std::vector<uint32_t> is_long(S, 0);
+  for (uint32_t q = 0; q < sl.long_count(); ++q) {
+    const uint32_t s = sl.long_slices[3 * q], p0 = sl.long_slices[3 * q + 1], np = sl.long_slices[3 * q + 2];
+    check(s < S && whole[s] == 0 && got[s].size() == np, tag + ": wide slice pieces");
+    if (s >= S) continue;
+    is_long[s] = 1;
+    std::sort(got[s].begin(), got[s].end(),
+              [](const std::pair<uint32_t, std::vector<std::vector<uint16_t>>>& a,
+                 const std::pair<uint32_t, std::vector<std::vector<uint16_t>>>& b) { return a.first < b.first; });
+    for (uint32_t j = 0; j < got[s].size(); ++j) check(got[s][j].first == p0 + j, tag + ": wide slice slots in order");
+  }
+  for (uint32_t s = 0; s < S; ++s) {
+    check(is_long[s] || (whole[s] == 1 && got[s].size() == 1), tag + ": every slice once");
+    for (uint32_t l = 0; l < 64; ++l) {
+      std::vector<uint16_t> seq;
+      for (const auto& piece : got[s]) seq.insert(seq.end(), piece.second[l].begin(), piece.second[l].end());
+      const uint64_t r = static_cast<uint64_t>(z) + 64ull * s + l;
+      std::vector<uint16_t> want;
+      if (r < pb) {
+        for (uint32_t q = p.hoff[r]; q < p.hoff[r + 1]; ++q)
+          if (p.hot16[q] != hot) want.push_back(p.hot16[q]);
+      }
+      std::vector<uint16_t> real;
+      for (uint16_t t : seq) {
+        if (t != hot) real.push_back(t);
+      }
+      check(real == want, tag + ": lane entries are the row's hot entries in order");
+      // padding only after the row's entries
+      bool pad = false, ok = true;
+      for (uint16_t t : seq) {
+        if (t == hot) pad = true;
+        else if (pad) ok = false;
+      }
+      check(ok, tag + ": padding after the entries");
+    }
   }
 }
 
@@ -187,6 +308,12 @@ void run_case(const Csr& g, uint32_t push, uint32_t hot, uint32_t stage, uint32_
     uint64_t expect = 0;
     for (uint32_t q = g.off[r]; q < g.off[r + 1]; ++q) expect += val[g.tgt[q]];
     check(cold[r - z] + hot_sum == expect, tag + ": row " + std::to_string(r) + " sum");
+  }
+  if (stage == 512 && bin_rows == 16384) {
+    const uint32_t caps[] = {1, 2, 4};
+    const uint32_t wave_counts[] = {1, 7, 4096};
+    for (uint32_t cap : caps)
+      for (uint32_t waves : wave_counts) check_slices(g, p, hot, cap, waves, tag);
   }
 }
 

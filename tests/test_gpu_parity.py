@@ -73,15 +73,20 @@ def check_tc_partitions(name, g, expected, monkeypatch):
 
 def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
     """Every SpMV kernel against the oracle: the default (push-pull on a symmetric graph with the
-    binned cold entries, with its push and hot sets at their defaults, at a few dozen ids, and at
-    one id; and with small staging buffers and bins, so that the scatter takes many sub-ranges
-    and the rows many bins), the same with the nonzero-split pull that gathers the cold entries
-    (merge), the row-tiled push-pull (push), the pull kernel with its hot set in LDS (default size
+    binned cold entries and the hot stream sliced one row per lane, with its push and hot sets at
+    their defaults, at a few dozen ids, and at one id; with pieces of one or two groups of eight
+    entries, so that the wide slices are cut and finished from their pieces; and with small
+    staging buffers and bins, so that the scatter takes many sub-ranges and the rows many bins),
+    the same bins with the hot stream in 512-entry chunks (bin), the nonzero-split pull that
+    gathers the cold entries (merge), the row-tiled push-pull (push), the pull kernel with its hot set in LDS (default size
     and a set smaller than the graph), and one workgroup per row tile."""
-    pr_vars = ("GC_PR_KERNEL", "GC_PR_PUSH", "GC_PR_HOT", "GC_PR_STAGE", "GC_PR_BIN_ROWS", "GC_PR_BIN_VALUES")
+    pr_vars = ("GC_PR_KERNEL", "GC_PR_PUSH", "GC_PR_HOT", "GC_PR_STAGE", "GC_PR_BIN_ROWS", "GC_PR_BIN_VALUES",
+               "GC_PR_SLICE_CAP")
     cases = ((None, None, None), (None, "64", "96"), (None, "1", "1"), (None, "37", None),
              ("bin", "1", "1", "512", "1", "8"), ("bin", "3", "40", "512", "7", "100"),
              ("bin", None, None, "1000", "64", "64"),
+             ("slice", None, None), ("slice", "64", "96", None, None, None, "1"), ("slice", "1", "1", "512", "1", "8"),
+             ("slice", "3", "40", "512", "7", "100", "2"),
              ("merge", None, None), ("merge", "64", "96"), ("merge", "1", "1"), ("merge", "37", None),
              ("push", None, None), ("push", "37", None),
              ("rows", None, None), ("hot", None, "1024"), ("hot", None, None))
@@ -185,6 +190,10 @@ def test_random_graphs(device, monkeypatch):
             mp.setenv("GC_PR_KERNEL", "bin")
             mp.setenv("GC_PR_BIN_ROWS", "1")
             check_pagerank(name + " (bin, push=1, one row per bin)", off, tgt, g)
+            # the hot stream sliced, slices cut into pieces of one group
+            mp.setenv("GC_PR_KERNEL", "slice")
+            mp.setenv("GC_PR_SLICE_CAP", "1")
+            check_pagerank(name + " (slice, push=1, one row per bin, pieces of one group)", off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch, ks=(3, 4, 5))
         check_truss_numbers(name, off, tgt, g)
         check_msbfs(name, off, tgt, g, list(range(n)) + [0, n - 1])
@@ -333,3 +342,23 @@ def test_pagerank_layout_sizes(device, monkeypatch):
         assert lay["chunks"] == (lay["stream"] + 511) // 512
         assert lay["values"] >= lay["cold"] and lay["values"] % 8 == 0 and lay["slot_rows"] % 8 == 0
         assert lay["grid"] >= 1
+        # the sliced hot stream: 64 rows per slice, each as wide as its widest row's hot entries,
+        # in groups of eight and pairs
+        monkeypatch.setenv("GC_PR_KERNEL", "slice")
+        monkeypatch.setenv("GC_PR_SLICE_CAP", "4")
+        check_pagerank(f"RMAT-14 sliced layout (push={push}, hot={hot})", off, tgt, g)
+        monkeypatch.delenv("GC_PR_KERNEL")
+        monkeypatch.delenv("GC_PR_SLICE_CAP")
+        vals = (c_uint64 * 25)()
+        check(lib.gc_pagerank_layout(g._handle, vals, 25))
+        sliced, slices, entries, items, wide, slots = (int(v) for v in vals[19:25])
+        assert sliced == 1 and slices == (pb - z + 63) // 64
+        width = np.zeros(slices * 64, np.int64)
+        width[:pb - z] = hot_per_row
+        width = width.reshape(slices, 64).max(axis=1)
+        width += width & 1
+        assert entries == int((512 * (width // 8) + 64 * (width % 8)).sum())
+        cut = width // 8 > 4  # wider than pieces of 4 groups
+        assert wide == int(np.count_nonzero(cut))
+        assert slots == int(((width[cut] // 8 + 3) // 4).sum())
+        assert items == slices - wide + slots + lay["pieces"]
