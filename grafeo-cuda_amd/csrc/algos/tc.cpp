@@ -70,6 +70,21 @@ bool tc_hot16() {
   return env == nullptr || env[0] != '0';
 }
 
+// tc_bitmap16 walks the hot parts of at most GC_TC_QMAX octs (default 32) a
+// quarter wave each, GC_TC_QUARTERS octs per lane and step (1 to 4, default
+// 2; 0: every list a wave step, as the longer ones). Measured at R-MAT scale
+// 22 per launch: a wave step per list 7.13 ms; quarters with 2 octs per lane
+// up to 16 / 24 / 32 / 40 / 63 octs 7.04 / 6.89 / 6.86 / 6.89 / 7.06 ms; 1
+// oct per lane up to 63 octs 7.65 ms (one load in flight per lane against
+// two for the lockstep pairs), 4 up to 63 6.92 ms.
+uint32_t tc_quarters() {
+  const char* env = std::getenv("GC_TC_QUARTERS");
+  const uint32_t d = env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : 2u;
+  const char* qmax = std::getenv("GC_TC_QMAX");
+  const uint32_t m = qmax != nullptr ? static_cast<uint32_t>(std::strtoul(qmax, nullptr, 10)) : 32u;
+  return std::min(d, 4u) | std::min(m, 4096u) << 8;
+}
+
 // Masks of the top ids in the 16-bit lists (tc_bitmap16): GC_TC_TOP ids (a
 // power of two from 128 to 8192; 0: no masks), default 4096, i.e. 32 octs
 // (512 bytes) per mask. At R-MAT scale 22 the top 1024 ids alone are 51% of
@@ -209,7 +224,8 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
       const uint32_t top_word0 = (tc_top_base(n, hot_base, top_octs) - hot_base) / 32u;
       go("tc_bitmap16", fwd, end, tgt, g.tc_split.as<const uint32_t>(),
          g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
-         row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result, top_word0, top_octs);
+         row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result, top_word0, top_octs,
+         tc_quarters());
     } else {
       go("tc_bitmap", fwd, end, tgt, row_hi, row_lo, hot_base, single, light_chunk, parts, tc_split_min(),
          counters, result);

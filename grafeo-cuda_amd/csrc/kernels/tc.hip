@@ -493,6 +493,54 @@ __device__ __forceinline__ uint32_t probe16_pair(const uint4* O1, uint32_t n1, c
   return local;
 }
 
+// Hits of the hot parts of the lists of the lanes in `lists` (lane j's list:
+// octs [st, st + len) of O), a quarter wave (16 lanes, D octs each per step)
+// per list: a quarter whose list has ended takes the next one, so the steps'
+// lanes mostly carry entries. (One list per wave step, as probe16_list, fills
+// half the lanes at R-MAT scale 22, where 99% of the list visits have fewer
+// than 64 octs and most 17 to 64.)
+template <int D>
+__device__ __forceinline__ uint32_t probe16_quarters(const uint4* O, unsigned long long lists, uint32_t st,
+                                                     uint32_t len, const uint32_t* s_bits) {
+  const uint32_t lane = lane_id();
+  const uint32_t quarter = lane >> 4, ql = lane & 15u;
+  uint32_t cur = 0, end = 0;  // the quarter's next oct and end (the same on its 16 lanes)
+  uint32_t local = 0;
+  for (;;) {
+    // the quarters without work take the next lists, in quarter order
+    const unsigned long long idle = __ballot(cur >= end);
+    uint32_t free4 = static_cast<uint32_t>((idle & 1ull) | ((idle >> 15) & 2ull) | ((idle >> 30) & 4ull) |
+                                           ((idle >> 45) & 8ull));
+    while (free4 != 0 && lists != 0) {
+      const uint32_t fq = static_cast<uint32_t>(__ffs(free4) - 1);
+      free4 &= free4 - 1;
+      const int jv = __ffsll(static_cast<long long>(lists)) - 1;
+      lists &= lists - 1;
+      const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+      const uint32_t l1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
+      if (quarter == fq) {
+        cur = s1;
+        end = s1 + l1;
+      }
+    }
+    if (__ballot(cur < end) == 0) break;
+    // D octs per lane: lanes past their list's end read its last oct again (a quarter that never
+    // had one, oct 0)
+    uint4 x[D];
+    bool act[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const uint32_t i = cur + 16u * d + ql;
+      act[d] = i < end;
+      x[d] = O[act[d] ? i : (end > 0 ? end - 1 : 0u)];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) local += act[d] ? hot16x8_hits(s_bits, x[d]) : 0u;
+    cur += 16u * D;
+  }
+  return local;
+}
+
 // Hits of the cold part L[0, lj) of a list (every entry below hot_base)
 // among the row's cold prefix A[0, c): the Bloom filter, then a search.
 __device__ __forceinline__ uint32_t probe_cold(const uint32_t* L, uint32_t lj, const uint32_t* s_bits,
@@ -536,7 +584,7 @@ __device__ __forceinline__ void tc_bitmap_body(
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits,
-    uint32_t top_word0 = 0, uint32_t top_octs = 0) {
+    uint32_t top_word0 = 0, uint32_t top_octs = 0, uint32_t quarters = 0) {
   const uint32_t positions = n - row_lo;  // rows [row_lo, n)
   const uint32_t lane = lane_id();
   for (uint32_t j = lane; j < TCB_BLOOM_BASE + TCB_BLOOM_WORDS; j += GC_WAVE) s_bits[j] = 0;
@@ -654,8 +702,22 @@ __device__ __forceinline__ void tc_bitmap_body(
                 }
               }
             }
-            // the lanes holding a non-empty list to walk
+            // the lanes holding a non-empty list to walk; with the 16-bit lists, those of at most
+            // quarters >> 8 octs go a quarter wave each, (quarters & 0xFF) octs per lane and step
+            // (0: every list a wave step)
             unsigned long long pending = __ballot(len != 0);
+            if (kHot16 && (quarters & 0xFFu) != 0) {
+              const unsigned long long small = __ballot(len != 0 && len <= (quarters >> 8));
+              if (small != 0) {
+                switch (quarters & 0xFFu) {
+                  case 1: local += probe16_quarters<1>(quads, small, st, len, s_bits); break;
+                  case 3: local += probe16_quarters<3>(quads, small, st, len, s_bits); break;
+                  case 4: local += probe16_quarters<4>(quads, small, st, len, s_bits); break;
+                  default: local += probe16_quarters<2>(quads, small, st, len, s_bits); break;
+                }
+              }
+              pending &= ~small;
+            }
             while (pending != 0) {
               const int jv = __ffsll(static_cast<long long>(pending)) - 1;
               pending &= pending - 1;
@@ -750,10 +812,11 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t top_word0,
-    uint32_t top_octs) {
+    uint32_t top_octs, uint32_t quarters) {
   __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
   tc_bitmap_body<true>(lo, hi, targets, split, hw, quads, n, row_lo, hot_base, heavy, light_chunk, parts,
-                       split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE], top_word0, top_octs);
+                       split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE], top_word0, top_octs,
+                       quarters);
 }
 
 // The hot lists of tc_bitmap16, in two passes around an exclusive scan of

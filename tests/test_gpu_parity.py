@@ -56,8 +56,8 @@ def check_triangles(name, device, off, tgt, g):
 
 def check_tc_partitions(name, g, expected, monkeypatch):
     """The bitmap kernel splits the middle vertices into parts (one per XCD or pair of XCDs) and
-    runs short rows whole, and masks the top ids of long lists: every split and mask width must
-    give the same count."""
+    runs short rows whole, masks the top ids of long lists and walks short lists a quarter wave
+    each: every split, mask width and walk must give the same count."""
     for top in ("0", "128", "1024", "8192"):
         for parts in ("1", "2", "3", "8"):
             for split in ("0", "64", "4294967295"):
@@ -69,6 +69,17 @@ def check_tc_partitions(name, g, expected, monkeypatch):
     monkeypatch.delenv("GC_TC_TOP")
     monkeypatch.delenv("GC_TC_PARTS")
     monkeypatch.delenv("GC_TC_SPLIT")
+    # the short hot parts a quarter wave each, one to four octs per lane and step, up to a few
+    # list lengths (0: a wave step per list)
+    for quarters, qmax in (("0", None), ("1", "1"), ("1", "63"), ("2", "8"), ("3", "48"), ("4", "4096")):
+        monkeypatch.setenv("GC_TC_QUARTERS", quarters)
+        if qmax is None:
+            monkeypatch.delenv("GC_TC_QMAX", raising=False)
+        else:
+            monkeypatch.setenv("GC_TC_QMAX", qmax)
+        assert g.triangle_count()["total_triangles"] == expected, f"{name}: quarters={quarters} qmax={qmax}"
+    monkeypatch.delenv("GC_TC_QUARTERS")
+    monkeypatch.delenv("GC_TC_QMAX")
 
 
 def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
