@@ -58,7 +58,7 @@ KERNEL_FAMILIES = {
             "bfs_top_build", "bfs_chain"),
     "msbfs": ("ms_seed", "ms_seed_levels", "ms_topdown", "ms_bottomup_thread", "ms_bottomup_wave",
               "ms_finalize", "ms_write_levels", "ms_levels_to_dist", "ms_write_distances", "ms_summarize"),
-    "ktruss": ("kt_wedge_support", "kt_truss_frontier", "kt_truss_peel", "kt_truss_commit", "kt_truss_decrement",
+    "ktruss": ("kt_bitmap_support", "kt_wedge_support", "kt_truss_frontier", "kt_truss_peel", "kt_truss_commit", "kt_truss_decrement",
                "kt_truss_values", "kt_filter", "kt_support_direct", "kt_reverse_index", "kt_canonical_index",
                "kt_canonical_reverse", "kt_emit_flags_direct", "kt_emit_flags_state"),
     "batch": ("bt_count",),
@@ -596,16 +596,18 @@ def reference_convention(dev, g, offsets, sources, args, m) -> dict:
     out["ktruss"].update({"gteps": m / out["ktruss"]["time_s"] / 1e9, "rounds": rounds,
                           "edges": len(kt["r"]["edges"]), "kernel_ms_per_call": kt_kernel_ms})
     # The survey's model: triangle-count bytes per support pass times passes, plus 5m per peel
-    # round. This library counts supports once (triangle enumeration on the DAG, tc_wedge's bytes)
-    # and then only peels.
+    # round. This library counts the supports once, by the triangle walk on the hub bitmap over the
+    # 32-bit lists (tc_bitmap's stored bytes, plus a 4-byte read-modify-write of one support per
+    # triangle), and then only peels.
     n = offsets.size - 1
-    from grafeo_cuda._native import check, lib
-    m_o, inter, wedges = c_uint64(), c_uint64(), c_uint64()
-    check(lib.gc_graph_dag_stats(g._handle, byref(m_o), byref(inter), byref(wedges)))
-    kt_bytes = (4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value) + 5 * m * rounds
+    tcm = tc_stored_width_bytes(g.relabelled_numpy())
+    triangles = g.triangle_count()["total_triangles"]
+    kt_bytes = (4 * (n + 1) + 4 * n + tcm["row_bytes"] + tcm["list_bytes_32bit"] + 8 * triangles
+                + 5 * m * rounds)
     out["rooflines"] = {"ktruss": roofline_entry(
         kt_bytes, 1000.0 * kt_kernel_ms,
-        note="one support pass at tc_wedge's bytes (4(n+1) + 4n + 4m + 4 wedges) plus 5m per peel round; "
+        note="one support pass at the bitmap walk's stored bytes (4(n+1) + 4n + L(u) at 4 bytes once per "
+             "part + L(v) at 4 bytes per DAG edge u->v + 8 bytes per triangle) plus 5m per peel round; "
              "avg_launch_us is the kernels' event time per k-truss call (k = 4)")}
     log(f"reference convention: {json.dumps({k: v for k, v in out.items() if k != 'rooflines'})[:400]}")
     return out

@@ -346,6 +346,14 @@ const char* tc_method() {
 
 }  // namespace
 
+void tc_bitmap_schedule(const Graph& g, uint32_t* hot_base, uint32_t* heavy, uint32_t* parts, uint32_t* split_min) {
+  const uint32_t n = g.n;
+  *hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
+  *heavy = tc_heavy_rows(g, 0, n);
+  *parts = tc_parts();
+  *split_min = tc_split_min();
+}
+
 // Triangles of a symmetric graph by enumeration at the middle vertex of the
 // relabelled DAG (tc_wedge).
 uint64_t run_tc_wedge(Graph& g) {

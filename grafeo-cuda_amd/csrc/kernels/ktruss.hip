@@ -103,6 +103,201 @@ extern "C" __global__ __launch_bounds__(64) void kt_wedge_support(
                    counters, nullptr, support);
 }
 
+// ---- support on the hub bitmap (kt_bitmap_support) ------------------------
+//
+// The same supports as kt_wedge_support, from the walk of tc_bitmap (tc.hip)
+// over the 32-bit lists: a wave holds L(u) ∩ [hot_base, n) of row u as a
+// 32768-bit LDS bitmap (and a Bloom filter of its cold prefix) and walks
+// L(v) for every v in L(u); a hit w is the triangle u < v < w. Its three
+// forward entries gain one support each:
+//  - v->w is the probed entry itself (one global atomic per triangle, at a
+//    position the walk already holds);
+//  - u->v takes the list's hits at once (one global atomic per list);
+//  - u->w is counted in a 16-bit LDS counter per hot entry of L(u) (the
+//    entry's rank: the rank of the first entry in w's bitmap word, kept per
+//    word, plus the bits below w's in that word), added to the support once
+//    per row; rows with more than KTB_COUNTERS hot entries, and cold w (found
+//    by a search of the prefix), take a global atomic.
+// The wedge enumeration probes all 7.2e9 DAG wedges at R-MAT scale 22 (295
+// ms); the bitmap walk tests 2.9e10 list entries, but at one LDS read each.
+// Rows and parts are scheduled as in tc_bitmap (heavy rows one at a time,
+// the others in chunks; rows of at least split_min entries walked once per
+// middle-vertex part, each part's counters on a line of their own).
+#define KTB_WAVES 4
+#define KTB_HOT_BITS 32768
+#define KTB_HOT_WORDS (KTB_HOT_BITS / 32)
+#define KTB_BLOOM_LOG2 12
+#define KTB_BLOOM_WORDS ((1 << KTB_BLOOM_LOG2) / 32)
+#define KTB_BLOOM_BASE (KTB_HOT_WORDS + 1)  // after the bitmap's zero word
+#define KTB_BITS_WORDS (KTB_BLOOM_BASE + KTB_BLOOM_WORDS)
+#define KTB_COUNTERS 2048  // 16-bit u->w counters per wave, two per word
+#define KTB_COUNTER_STRIDE 64
+
+struct KtbLds {
+  uint32_t bits[KTB_BITS_WORDS];
+  uint16_t wrank[KTB_HOT_WORDS];  // rank among L(u)'s hot entries of the first one in each word
+  uint32_t cnt[KTB_COUNTERS / 2];
+};
+
+__device__ __forceinline__ uint32_t ktb_bloom_hash(uint32_t w) {
+  return (w * 0x9E3779B1u) >> (32 - KTB_BLOOM_LOG2);
+}
+
+// Hits of entries [s1, s1 + lj) of targets (the list L(v)) among the row's
+// sets, with their supports: v->w at the entry, u->w in the row's counters
+// (or at A's entry, base + position), per lane.
+__device__ __forceinline__ uint32_t ktb_walk(const uint32_t* __restrict__ targets, uint32_t s1, uint32_t lj,
+                                             KtbLds& L, uint32_t hot_base, const uint32_t* __restrict__ A,
+                                             uint32_t c, uint32_t row_e, bool lds_cnt,
+                                             uint32_t* __restrict__ support) {
+  const uint32_t lane = lane_id();
+  uint32_t local = 0;
+  for (uint32_t kk = 0; kk < lj; kk += 4 * GC_WAVE) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = targets[s1 + min(kk + q * GC_WAVE + lane, lj - 1)];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t idx = kk + q * GC_WAVE + lane;
+      const uint32_t y = w[q] - hot_base;
+      const uint32_t word = L.bits[min(y >> 5, static_cast<uint32_t>(KTB_HOT_WORDS))];
+      const uint32_t bit = y & 31u;
+      if (idx < lj && ((word >> bit) & 1u)) {
+        atomicAdd(&support[s1 + idx], 1u);
+        const uint32_t r = L.wrank[y >> 5] + static_cast<uint32_t>(__popc(word & ((1u << bit) - 1u)));
+        if (lds_cnt) atomicAdd(&L.cnt[r >> 1], 1u << (16u * (r & 1u)));
+        else atomicAdd(&support[row_e + c + r], 1u);
+        ++local;
+      } else if (c != 0 && idx < lj && w[q] < hot_base) {
+        const uint32_t h = ktb_bloom_hash(w[q]);
+        if ((L.bits[KTB_BLOOM_BASE + (h >> 5)] >> (h & 31u)) & 1u) {
+          const uint32_t p = lower_bound_u32(A, 0, c, w[q]);
+          if (p < c && A[p] == w[q]) {
+            atomicAdd(&support[s1 + idx], 1u);
+            atomicAdd(&support[row_e + p], 1u);
+            ++local;
+          }
+        }
+      }
+    }
+  }
+  return local;
+}
+
+// lo[x] .. hi[x]: the forward entries of row x (the relabelled CSR's fwd and
+// offsets + 1); rows [0, n), taken from the top. support[e] (zeroed) gains
+// the triangles of every forward entry e. counters: parts x
+// KTB_COUNTER_STRIDE zeroed words.
+extern "C" __global__ __launch_bounds__(KTB_WAVES * GC_WAVE) void kt_bitmap_support(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
+    uint32_t n, uint32_t hot_base, uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
+    uint32_t* __restrict__ counters, uint32_t* __restrict__ support) {
+  __shared__ KtbLds s_all[KTB_WAVES];
+  KtbLds& L = s_all[threadIdx.x / GC_WAVE];
+  const uint32_t lane = lane_id();
+  for (uint32_t j = lane; j < KTB_BITS_WORDS; j += GC_WAVE) L.bits[j] = 0;
+  for (uint32_t j = lane; j < KTB_COUNTERS / 2; j += GC_WAVE) L.cnt[j] = 0;
+  wave_lds_sync();
+  const uint32_t home = blockIdx.x % parts;
+  for (uint32_t qp = 0; qp < parts; ++qp) {
+    const uint32_t part = (home + qp) % parts;
+    uint32_t* part_counters = counters + KTB_COUNTER_STRIDE * part;
+    bool heavy_left = heavy > 0;
+    for (;;) {
+      uint32_t kb = n, ke = n;
+      if (lane == 0) {
+        if (heavy_left) {
+          const uint32_t k = atomicAdd(&part_counters[0], 1u);
+          if (k < heavy) {
+            kb = k;
+            ke = k + 1;
+          } else {
+            heavy_left = false;
+          }
+        }
+        if (!heavy_left && kb == n) {
+          const uint32_t cc = atomicAdd(&part_counters[1], light_chunk);
+          const uint64_t b = static_cast<uint64_t>(heavy) + cc;
+          if (b < n) {
+            kb = static_cast<uint32_t>(b);
+            const uint64_t e = b + light_chunk;
+            ke = e < n ? static_cast<uint32_t>(e) : n;
+          }
+        }
+      }
+      kb = __shfl(kb, 0);
+      ke = __shfl(ke, 0);
+      if (kb >= n) break;
+      for (uint32_t kc = kb; kc < ke; kc += GC_WAVE) {
+        uint32_t row_lo_e = 0, row_a = 0;
+        bool take = false;
+        if (kc + lane < ke) {
+          const uint32_t u = n - 1 - (kc + lane);
+          row_lo_e = lo[u];
+          row_a = hi[u] - row_lo_e;
+          take = row_a != 0 && (row_a >= split_min || u % parts == part);
+        }
+        unsigned long long rows = __ballot(take);
+        while (rows != 0) {
+          const int jr = __ffsll(static_cast<long long>(rows)) - 1;
+          rows &= rows - 1;
+          const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(row_a, jr));
+          const uint32_t row_e = static_cast<uint32_t>(__builtin_amdgcn_readlane(row_lo_e, jr));
+          const uint32_t* A = targets + row_e;
+          const bool whole = a < split_min;
+          // the cold prefix [0, c) (entries below hot_base) and the hot suffix's bits and ranks
+          const uint32_t c = __builtin_amdgcn_readfirstlane(lower_bound_u32(A, 0, a, hot_base));
+          for (uint32_t j = c + lane; j < a; j += GC_WAVE) {
+            const uint32_t y = A[j] - hot_base;
+            atomicOr(&L.bits[y >> 5], 1u << (y & 31u));
+            if (j == c || ((A[j - 1] - hot_base) >> 5) != (y >> 5)) L.wrank[y >> 5] = static_cast<uint16_t>(j - c);
+          }
+          for (uint32_t j = lane; j < c; j += GC_WAVE) {
+            const uint32_t h = ktb_bloom_hash(A[j]);
+            atomicOr(&L.bits[KTB_BLOOM_BASE + (h >> 5)], 1u << (h & 31u));
+          }
+          wave_lds_sync();
+          const bool lds_cnt = a - c <= KTB_COUNTERS;
+          for (uint32_t cv = 0; cv < a; cv += GC_WAVE) {
+            const uint32_t nv = min(static_cast<uint32_t>(GC_WAVE), a - cv);
+            uint32_t st = 0, len = 0;
+            if (lane < nv) {
+              const uint32_t v = A[cv + lane];
+              if (whole || v % parts == part) {
+                st = lo[v];
+                len = hi[v] - st;
+              }
+            }
+            unsigned long long pending = __ballot(len != 0);
+            while (pending != 0) {
+              const int jv = __ffsll(static_cast<long long>(pending)) - 1;
+              pending &= pending - 1;
+              const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+              const uint32_t lj = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
+              const uint32_t hits = wave_sum(ktb_walk(targets, s1, lj, L, hot_base, A, c, row_e, lds_cnt, support));
+              if (lane == 0 && hits != 0) atomicAdd(&support[row_e + cv + static_cast<uint32_t>(jv)], hits);
+            }
+          }
+          wave_lds_sync();
+          if (lds_cnt) {
+            for (uint32_t r = lane; r < a - c; r += GC_WAVE) {
+              const uint32_t x = (L.cnt[r >> 1] >> (16u * (r & 1u))) & 0xFFFFu;
+              if (x != 0) atomicAdd(&support[row_e + c + r], x);
+            }
+          }
+          wave_lds_sync();
+          if (lds_cnt) {
+            for (uint32_t j = lane; j < (a - c + 1) / 2; j += GC_WAVE) L.cnt[j] = 0;
+          }
+          for (uint32_t j = c + lane; j < a; j += GC_WAVE) L.bits[(A[j] - hot_base) >> 5] = 0;
+          for (uint32_t j = lane; j < c; j += GC_WAVE) L.bits[KTB_BLOOM_BASE + (ktb_bloom_hash(A[j]) >> 5)] = 0;
+          wave_lds_sync();
+        }
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ uint32_t has_loop(const uint32_t* __restrict__ offsets,
                                              const uint32_t* __restrict__ targets,
                                              const uint32_t* __restrict__ fwd, uint32_t x) {

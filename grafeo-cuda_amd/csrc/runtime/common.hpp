@@ -429,6 +429,12 @@ void exclusive_scan_u32(Device& dev, uint32_t* data, uint64_t count, uint32_t* b
 // and 4096 entries (rows above that search global memory instead).
 uint32_t wedge_table_bits(const Relabelled& rl);
 
+// The row schedule of the bitmap triangle kernels over all rows of a prepared
+// graph (tc.cpp): the first hot id (the top 32768, a multiple of 32), the rows
+// handed out one at a time, the middle-vertex parts and the shortest row split
+// over them (GC_TC_HEAVY_DEGREE, GC_TC_PARTS, GC_TC_SPLIT).
+void tc_bitmap_schedule(const Graph& g, uint32_t* hot_base, uint32_t* heavy, uint32_t* parts, uint32_t* split_min);
+
 // Builds the derived layouts if not done yet (on the device; GC_PREPARE=host
 // runs the host restatement instead).
 void prepare(Graph& g);

@@ -138,16 +138,23 @@ def check_pagerank(name, off, tgt, g):
 
 
 def check_k_truss(name, off, tgt, g, monkeypatch, ks=range(3, 9), both_paths=True, fast_oracle=False):
-    """Both GPU paths (the DAG rounds and the per-entry rounds) against the oracle."""
+    """Both GPU paths (the DAG rounds, with the supports from the bitmap walk and from the wedge
+    enumeration, and the per-entry rounds) against the oracle."""
+    paths = (("0", None), ("0", "wedge"), ("1", None)) if both_paths else (("0", None),)
     for k in ks:
         edges, iterations = oracle.k_truss(off, tgt, k, fast=fast_oracle)
-        for direct in (("0", "1") if both_paths else ("0",)):
+        for direct, support in paths:
             monkeypatch.setenv("GC_KTRUSS_DIRECT", direct)
+            if support is None:
+                monkeypatch.delenv("GC_KTRUSS_SUPPORT", raising=False)
+            else:
+                monkeypatch.setenv("GC_KTRUSS_SUPPORT", support)
             r = g.k_truss(k)
-            assert r["edges"] == edges, f"{name}: {k}-truss (direct={direct})"
-            assert r["iterations"] == iterations, f"{name}: {k}-truss rounds (direct={direct})"
+            assert r["edges"] == edges, f"{name}: {k}-truss (direct={direct}, support={support})"
+            assert r["iterations"] == iterations, f"{name}: {k}-truss rounds (direct={direct}, support={support})"
             assert r["k"] == k
     monkeypatch.delenv("GC_KTRUSS_DIRECT")
+    monkeypatch.delenv("GC_KTRUSS_SUPPORT", raising=False)
 
 
 def check_msbfs(name, off, tgt, g, sources):
@@ -159,12 +166,18 @@ def check_msbfs(name, off, tgt, g, sources):
         assert r["reachable"][row] == reach and r["max_depth"][row] == depth, f"{name}: MS-BFS {source}"
 
 
-def check_truss_numbers(name, off, tgt, g):
+def check_truss_numbers(name, off, tgt, g, monkeypatch=None):
+    """Trussness against the oracle's peeling; with monkeypatch, also from the wedge supports."""
     edges, truss = oracle.truss_numbers(off, tgt)
-    r = g.truss_numbers_numpy()
-    assert [(int(u), int(v)) for u, v in r["edges"]] == edges, f"{name}: truss edges"
-    assert np.array_equal(r["truss"], truss), f"{name}: truss numbers"
-    assert r["max_truss"] == (int(truss.max()) if truss.size else 0), f"{name}: max truss"
+    for support in ((None, "wedge") if monkeypatch is not None else (None,)):
+        if support is not None:
+            monkeypatch.setenv("GC_KTRUSS_SUPPORT", support)
+        r = g.truss_numbers_numpy()
+        assert [(int(u), int(v)) for u, v in r["edges"]] == edges, f"{name}: truss edges"
+        assert np.array_equal(r["truss"], truss), f"{name}: truss numbers (support={support})"
+        assert r["max_truss"] == (int(truss.max()) if truss.size else 0), f"{name}: max truss"
+    if monkeypatch is not None:
+        monkeypatch.delenv("GC_KTRUSS_SUPPORT", raising=False)
 
 
 def test_catalog(device, monkeypatch):
@@ -175,7 +188,7 @@ def test_catalog(device, monkeypatch):
             check_bfs(name, off, tgt, g, source)
         check_pagerank(name, off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch)
-        check_truss_numbers(name, off, tgt, g)
+        check_truss_numbers(name, off, tgt, g, monkeypatch)
         check_msbfs(name, off, tgt, g, list(range(len(off) - 1)))
 
 
@@ -206,7 +219,7 @@ def test_random_graphs(device, monkeypatch):
             mp.setenv("GC_PR_SLICE_CAP", "1")
             check_pagerank(name + " (slice, push=1, one row per bin, pieces of one group)", off, tgt, g)
         check_k_truss(name, off, tgt, g, monkeypatch, ks=(3, 4, 5))
-        check_truss_numbers(name, off, tgt, g)
+        check_truss_numbers(name, off, tgt, g, monkeypatch)
         check_msbfs(name, off, tgt, g, list(range(n)) + [0, n - 1])
 
 
@@ -246,7 +259,7 @@ def test_er_4096(device, monkeypatch):
         check_bfs("ER-4096", off, tgt, g, source)
     check_pagerank("ER-4096", off, tgt, g)
     check_k_truss("ER-4096", off, tgt, g, monkeypatch, ks=(3, 4))
-    check_truss_numbers("ER-4096", off, tgt, g)
+    check_truss_numbers("ER-4096", off, tgt, g, monkeypatch)
     # 130 sources: one full traversal of 128 and one of 2, with repeats
     sources = [int(x) for x in np.random.default_rng(4096).integers(0, 4096, size=127)] + [0, 0, 4095]
     check_msbfs("ER-4096", off, tgt, g, sources)
@@ -266,7 +279,7 @@ def test_rmat_small(device, monkeypatch, scale):
     # the per-entry path merges full lists (hub x hub), so it is checked at the small scale only
     check_k_truss(f"RMAT-{scale}", off, tgt, g, monkeypatch, ks=(3, 4, 6), both_paths=scale <= 10,
                   fast_oracle=scale > 10)
-    check_truss_numbers(f"RMAT-{scale}", off, tgt, g)
+    check_truss_numbers(f"RMAT-{scale}", off, tgt, g, monkeypatch)
     check_msbfs(f"RMAT-{scale}", off, tgt, g, [int(x) for x in np.random.default_rng(scale).integers(0, n, size=128)])
 
 
