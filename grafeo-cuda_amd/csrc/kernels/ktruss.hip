@@ -79,13 +79,21 @@ extern "C" __global__ __launch_bounds__(256) void kt_emit_flags_direct(
 
 // ------------------------------------------------------------------- DAG --
 
+// The row loops of the index kernels below: one row per wave (grid-stride
+// over the waves), its entries over the lanes, so a hub row of 1e5 entries is
+// not one thread's serial loop (one thread per row took 11.5 ms at R-MAT scale
+// 22, the hubs' threads running alone at the end).
+__device__ __forceinline__ uint64_t kt_wave() { return global_thread() / GC_WAVE; }
+__device__ __forceinline__ uint64_t kt_waves() { return grid_threads() / GC_WAVE; }
+
 // rev[i] = position of u in row v for every forward entry i = (u -> v).
 extern "C" __global__ __launch_bounds__(256) void kt_reverse_index(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ fwd, uint32_t n, uint32_t* __restrict__ rev) {
-  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
+  for (uint64_t uu = kt_wave(); uu < n; uu += kt_waves()) {
     const uint32_t u = static_cast<uint32_t>(uu);
-    for (uint32_t i = fwd[u]; i < offsets[u + 1]; ++i) {
+    const uint32_t e = offsets[u + 1];
+    for (uint32_t i = fwd[u] + lane_id(); i < e; i += GC_WAVE) {
       const uint32_t v = targets[i];
       rev[i] = lower_bound_u32(targets, offsets[v], fwd[v], u);
     }
@@ -334,16 +342,12 @@ __device__ __forceinline__ uint32_t has_loop(const uint32_t* __restrict__ offset
 extern "C" __global__ __launch_bounds__(256) void kt_canonical_index(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ fwd, uint32_t n,
     uint32_t* __restrict__ canon, uint8_t* __restrict__ state) {
-  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
+  for (uint64_t uu = kt_wave(); uu < n; uu += kt_waves()) {
     const uint32_t u = static_cast<uint32_t>(uu);
     const uint32_t f = fwd[u], e = offsets[u + 1];
-    for (uint32_t i = offsets[u]; i < f; ++i) {
-      canon[i] = i;  // overwritten below by the forward entry for non-loops
-      state[i] = static_cast<uint8_t>(KT_NONE);
-    }
-    for (uint32_t i = f; i < e; ++i) {
-      canon[i] = i;
-      state[i] = static_cast<uint8_t>(KT_ALIVE);
+    for (uint32_t i = offsets[u] + lane_id(); i < e; i += GC_WAVE) {
+      canon[i] = i;  // entries below f: overwritten by kt_canonical_reverse for non-loops
+      state[i] = static_cast<uint8_t>(i < f ? KT_NONE : KT_ALIVE);
     }
   }
 }
@@ -353,9 +357,10 @@ extern "C" __global__ __launch_bounds__(256) void kt_canonical_index(
 extern "C" __global__ __launch_bounds__(256) void kt_canonical_reverse(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ fwd,
     const uint32_t* __restrict__ rev, uint32_t n, uint32_t* __restrict__ canon) {
-  for (uint64_t uu = global_thread(); uu < n; uu += grid_threads()) {
+  for (uint64_t uu = kt_wave(); uu < n; uu += kt_waves()) {
     const uint32_t u = static_cast<uint32_t>(uu);
-    for (uint32_t i = fwd[u]; i < offsets[u + 1]; ++i) canon[rev[i]] = i;
+    const uint32_t e = offsets[u + 1];
+    for (uint32_t i = fwd[u] + lane_id(); i < e; i += GC_WAVE) canon[rev[i]] = i;
   }
 }
 
