@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
 KERNEL_FAMILIES = {
-    "tc": ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count"),
+    "tc": ("tc_bitmap16", "tc_bitmap16_w4", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count"),
     "pagerank": ("pr_spmv_slice", "pr_slice_finish", "pr_spmv_bin", "pr_bin_scatter", "pr_bin_accum",
                  "pr_spmv_merge", "pr_merge_finish",
                  "pr_spmv_push", "pr_push_finish", "pr_spmv_hot", "pr_spmv",

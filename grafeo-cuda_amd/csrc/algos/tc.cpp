@@ -77,6 +77,13 @@ bool tc_hot16() {
 // up to 16 / 24 / 32 / 40 / 63 octs 7.04 / 6.89 / 6.86 / 6.89 / 7.06 ms; 1
 // oct per lane up to 63 octs 7.65 ms (one load in flight per lane against
 // two for the lockstep pairs), 4 up to 63 6.92 ms.
+// Waves per workgroup of the 16-bit bitmap kernel: GC_TC_WG = 1 (tc_bitmap16,
+// the bitmap at LDS address 0; the default) or 4 (tc_bitmap16_w4).
+uint32_t tc_wg_waves() {
+  const char* env = std::getenv("GC_TC_WG");
+  return env != nullptr && env[0] == '4' ? 4u : 1u;
+}
+
 uint32_t tc_quarters() {
   const char* env = std::getenv("GC_TC_QUARTERS");
   const uint32_t d = env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : 2u;
@@ -211,10 +218,12 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     const uint32_t single = tc_heavy_rows(g, row_lo, row_hi), parts = tc_parts();
     const uint64_t tasks =
         parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
-    auto go = [&](const char* name, auto... args) {
+    // workgroups of `waves` waves, as many waves in all as grid_for's four-wave workgroups
+    auto go = [&](const char* name, uint32_t waves, auto... args) {
       const Kernel k = get_kernel(dev, "tc", name);
-      if (no_alloc) launch_untimed(dev, k, grid_for(tasks), 4 * kWaveSize, 0, args...);
-      else launch(dev, k, grid_for(tasks), 4 * kWaveSize, 0, args...);
+      const uint32_t blocks = grid_for(tasks) * (4 / waves);
+      if (no_alloc) launch_untimed(dev, k, blocks, waves * kWaveSize, 0, args...);
+      else launch(dev, k, blocks, waves * kWaveSize, 0, args...);
     };
     // the no-alloc call uses the lists as they were built (their masks included)
     const uint32_t top_octs = no_alloc ? g.tc_hot16_top : tc_top_octs();
@@ -222,12 +231,13 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
     if (hot16) {
       if (!no_alloc) ensure_hot16(g, hot_base, parts, top_octs);
       const uint32_t top_word0 = (tc_top_base(n, hot_base, top_octs) - hot_base) / 32u;
-      go("tc_bitmap16", fwd, end, tgt, g.tc_split.as<const uint32_t>(),
+      const bool w1 = tc_wg_waves() == 1;
+      go(w1 ? "tc_bitmap16" : "tc_bitmap16_w4", w1 ? 1u : 4u, fwd, end, tgt, g.tc_split.as<const uint32_t>(),
          g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
          row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result, top_word0, top_octs,
          tc_quarters());
     } else {
-      go("tc_bitmap", fwd, end, tgt, row_hi, row_lo, hot_base, single, light_chunk, parts, tc_split_min(),
+      go("tc_bitmap", 4u, fwd, end, tgt, row_hi, row_lo, hot_base, single, light_chunk, parts, tc_split_min(),
          counters, result);
     }
   }

@@ -445,8 +445,11 @@ __device__ __forceinline__ uint32_t hot16_hit(const uint32_t* s_bits, uint32_t y
   return word_bit(s_bits[y >> 5], y);
 }
 
+// Two offsets in one word: the low one's bit is taken with x itself (v_bfe_u32
+// reads only the offset's low five bits, the low offset's), its word index
+// with one shift and mask; the high one's word index is x >> 21 (no mask).
 __device__ __forceinline__ uint32_t hot16x2_hits(const uint32_t* s_bits, uint32_t x) {
-  return hot16_hit(s_bits, x & 0xFFFFu) + hot16_hit(s_bits, x >> 16);
+  return word_bit(s_bits[(x >> 5) & 0x7FFu], x) + word_bit(s_bits[x >> 21], x >> 16);
 }
 
 __device__ __forceinline__ uint32_t hot16x8_hits(const uint32_t* s_bits, uint4 x) {
@@ -806,7 +809,7 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap(
 // half the bytes and an eighth of the loads for the entries that carry almost
 // all of the streaming, no clamp and no cold test per hot entry, and the cold
 // parts walked apart, only for the rows that have a cold prefix.
-extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
+extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16_w4(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint4* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
@@ -817,6 +820,22 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16(
   tc_bitmap_body<true>(lo, hi, targets, split, hw, quads, n, row_lo, hot_base, heavy, light_chunk, parts,
                        split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE], top_word0, top_octs,
                        quarters);
+}
+
+// The same with one-wave workgroups (the default; tc_bitmap16_w4 above with
+// four): the wave's bitmap is the workgroup's only LDS, at address 0, so a
+// lookup's address needs no per-wave base (eight adds per oct fewer). At
+// R-MAT scale 22, 6.75 against 6.82 ms on one box.
+extern "C" __global__ __launch_bounds__(GC_WAVE) void tc_bitmap16(
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint4* __restrict__ quads,
+    uint32_t n, uint32_t row_lo, uint32_t hot_base,
+    uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
+    uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t top_word0,
+    uint32_t top_octs, uint32_t quarters) {
+  __shared__ uint32_t s_bits[TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
+  tc_bitmap_body<true>(lo, hi, targets, split, hw, quads, n, row_lo, hot_base, heavy, light_chunk, parts,
+                       split_min, counters, result, s_bits, top_word0, top_octs, quarters);
 }
 
 // The hot lists of tc_bitmap16, in two passes around an exclusive scan of

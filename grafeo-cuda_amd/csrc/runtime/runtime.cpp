@@ -191,6 +191,7 @@ void noalloc_setup(Device& dev) {
     // the kernels of gc_triangle_count_noalloc, loaded now so that it loads nothing
     (void)get_kernel(dev, "util", "fill_u32");
     (void)get_kernel(dev, "tc", "tc_bitmap16");
+    (void)get_kernel(dev, "tc", "tc_bitmap16_w4");
     (void)get_kernel(dev, "tc", "tc_bitmap");
   } catch (const Error& e) {
     dev.noalloc_error = e.message;

@@ -80,6 +80,10 @@ def check_tc_partitions(name, g, expected, monkeypatch):
         assert g.triangle_count()["total_triangles"] == expected, f"{name}: quarters={quarters} qmax={qmax}"
     monkeypatch.delenv("GC_TC_QUARTERS")
     monkeypatch.delenv("GC_TC_QMAX")
+    # four-wave workgroups (each wave's bitmap at its own LDS base) against the one-wave default
+    monkeypatch.setenv("GC_TC_WG", "4")
+    assert g.triangle_count()["total_triangles"] == expected, f"{name}: four-wave workgroups"
+    monkeypatch.delenv("GC_TC_WG")
 
 
 def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
