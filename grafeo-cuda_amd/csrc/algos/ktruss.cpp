@@ -121,6 +121,12 @@ void support_pass(Graph& g, uint32_t* support, uint32_t* counters) {
          heavy, 64u, parts, split_min, work, support);
 }
 
+// Lanes per frontier edge of kt_truss_peel over `blocks` 256-lane workgroups:
+// 16 when the batch gives every 16-lane group an edge, else a whole wave.
+uint32_t peel_width(uint32_t size, uint32_t blocks) {
+  return static_cast<uint64_t>(size) >= static_cast<uint64_t>(blocks) * 256u / 16u ? 16u : 64u;
+}
+
 uint32_t rounds_direct(Graph& g, uint32_t threshold, uint8_t* flags) {
   Device& dev = *g.dev;
   const uint32_t n = g.n, m = g.m;
@@ -212,7 +218,8 @@ uint32_t rounds_dag(Graph& g, uint32_t threshold, uint8_t* flags) {
       if (size > 0) {
         fill_words(dev, counters + 2, 1, 0);
         launch(dev, k_peel, grid, 256u, 0, off, tgt, static_cast<const uint32_t*>(canon), n,
-               static_cast<const uint32_t*>(queue[cur]), size, state, support, t, queue[cur ^ 1], counters + 2);
+               static_cast<const uint32_t*>(queue[cur]), size, state, support, t, queue[cur ^ 1], counters + 2,
+               peel_width(size, grid));
         copy_to_host(dev, dev.pinned, counters + 2, sizeof(uint32_t));
         std::memcpy(&next, dev.pinned, sizeof(next));
         Grid1D cg = thread_per_item(std::max(size, next));
@@ -291,7 +298,7 @@ uint32_t truss_peel(Graph& g, uint32_t* support) {
       fill_words(dev, counters + 2, 1, 0);
       launch(dev, k_peel, peel_grid, 256u, 0, off, tgt, static_cast<const uint32_t*>(canon), n,
              static_cast<const uint32_t*>(queue[cur]), size, state, support, t, queue[cur ^ 1],
-             counters + 2);
+             counters + 2, peel_width(size, peel_grid));
       copy_to_host(dev, dev.pinned, counters + 2, sizeof(uint32_t));
       uint32_t next = 0;
       std::memcpy(&next, dev.pinned, sizeof(next));

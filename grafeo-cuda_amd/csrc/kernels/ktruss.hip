@@ -439,17 +439,21 @@ __device__ __forceinline__ void kt_truss_decrement(uint32_t x, uint32_t t,
   }
 }
 
-// One batch: one wave per frontier edge (u, v) walks the shorter of the two
-// full rows and binary-searches each neighbour w in the longer one.
+// One batch: `width` lanes (16 or 64) per frontier edge (u, v) walk the
+// shorter of the two full rows and binary-search each neighbour w in the
+// longer one. Large batches take 16 lanes per edge: the shorter row is short
+// for most of their edges, which a whole wave per edge left mostly idle (32
+// against 14 ms for the 8 rounds of the 4-truss at R-MAT scale 22); the few
+// edges of the late trussness levels, between hubs, take a whole wave.
 extern "C" __global__ __launch_bounds__(256) void kt_truss_peel(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ canon, uint32_t n, const uint32_t* __restrict__ queue,
     uint32_t queue_size, uint8_t* __restrict__ state, uint32_t* __restrict__ support, uint32_t t,
-    uint32_t* __restrict__ next_queue, uint32_t* __restrict__ next_count) {
-  const uint32_t lane = lane_id();
-  const uint64_t wave = global_thread() / GC_WAVE;
-  const uint64_t waves = grid_threads() / GC_WAVE;
-  for (uint64_t q = wave; q < queue_size; q += waves) {
+    uint32_t* __restrict__ next_queue, uint32_t* __restrict__ next_count, uint32_t width) {
+  const uint32_t ql = lane_id() & (width - 1u);
+  const uint64_t group = global_thread() / width;
+  const uint64_t groups = grid_threads() / width;
+  for (uint64_t q = group; q < queue_size; q += groups) {
     const uint32_t e = queue[q];
     const uint32_t u = row_of_entry(offsets, n, e);
     const uint32_t v = targets[e];
@@ -458,7 +462,7 @@ extern "C" __global__ __launch_bounds__(256) void kt_truss_peel(
     const uint32_t l = du <= dv ? v : u;
     const uint32_t s_begin = offsets[s], s_end = offsets[s + 1];
     const uint32_t l_begin = offsets[l], l_end = offsets[l + 1];
-    for (uint32_t es = s_begin + lane; es < s_end; es += GC_WAVE) {
+    for (uint32_t es = s_begin + ql; es < s_end; es += width) {
       const uint32_t w = targets[es];
       if (w == u || w == v) continue;
       const uint32_t p = lower_bound_u32(targets, l_begin, l_end, w);
