@@ -111,7 +111,16 @@ extern "C" __global__ __launch_bounds__(256) void ms_bottomup_thread(
 }
 
 // Bottom-up for vertices [begin, end) of high degree: one wave per vertex,
-// 64 neighbours per step, stopping once the missing sources are covered.
+// up to 512 neighbours per step (up to eight per lane in flight, as many
+// groups of 64 as the list has left; a lane past the list's end reads its
+// last neighbour again, which ORs nothing new), stopping once the missing
+// sources are covered. The masks are ORed across the wave once per step: with
+// 128 sources a vertex seldom covers all its missing ones early, and one
+// gather per lane and an OR across the wave per 64 neighbours left the level
+// bound by their round trips (at R-MAT scale 22 with 128 sources, the five
+// bottom-up levels: 6.3 ms that way, 3.97 with four groups per step, 3.64
+// with eight).
+#define MSB_GROUPS 8  // groups of 64 neighbours per step
 extern "C" __global__ __launch_bounds__(256) void ms_bottomup_wave(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t begin,
     uint32_t end, unsigned long long active_lo, unsigned long long active_hi,
@@ -126,12 +135,19 @@ extern "C" __global__ __launch_bounds__(256) void ms_bottomup_wave(
     if ((miss_lo | miss_hi) == 0) continue;
     Mask acc{0, 0};
     const uint32_t stop = offsets[w + 1];
-    for (uint32_t base = offsets[w]; base < stop; base += GC_WAVE) {
-      const uint32_t i = base + lane;
-      if (i < stop) {
-        const Mask mv = load_mask(&visit[targets[i]]);
-        acc.lo |= mv.lo;
-        acc.hi |= mv.hi;
+    for (uint32_t base = offsets[w]; base < stop; base += MSB_GROUPS * GC_WAVE) {
+      const uint32_t left = stop - base;  // wave-uniform: the groups of 64 past it are skipped
+      uint32_t v[MSB_GROUPS];
+#pragma unroll
+      for (int q = 0; q < MSB_GROUPS; ++q)
+        if (q * GC_WAVE < left) v[q] = targets[min(base + q * GC_WAVE + lane, stop - 1)];
+#pragma unroll
+      for (int q = 0; q < MSB_GROUPS; ++q) {
+        if (q * GC_WAVE < left) {
+          const Mask mv = load_mask(&visit[v[q]]);
+          acc.lo |= mv.lo;
+          acc.hi |= mv.hi;
+        }
       }
       acc = wave_or(acc);
       if ((acc.lo & miss_lo) == miss_lo && (acc.hi & miss_hi) == miss_hi) break;
