@@ -102,7 +102,7 @@ def pagerank_bin_models(lay: dict, zero_rows: int) -> dict:
     - pr_spmv_slice (sliced layout): the sliced 16-bit entries with their padding, 16 bytes per
       work item; per pull row its contribution, input rank, cold sum, offset (20 bytes) and rank
       and contribution written (8), less the rows of the wide slices, which write a partial sum
-      (8 bytes per row and piece) instead; the hub pieces' 4-byte entries; per workgroup the hot
+      (8 bytes per row and piece) instead; the hub pieces' 16-bit entries (512 per piece); per workgroup the hot
       set read (4 bytes per id) and the push accumulators written (8 per hub); the rows of
       degree 0 (rank read, rank and contribution written: 12 bytes);
     - pr_slice_finish: the push accumulators of every workgroup read (8 bytes per hub each) and
@@ -122,7 +122,7 @@ def pagerank_bin_models(lay: dict, zero_rows: int) -> dict:
         + 8 * lay["runs"] + 4 * lay["cold_entries"],
         "pr_bin_accum": 6 * lay["values"] + 8 * lay["pull_rows"],
         "pr_spmv_slice": 2 * lay["slice_entries"] + 16 * lay["slice_items"] + 28 * lay["pull_rows"]
-        - 8 * wide_rows + 512 * lay["partial_slots"] + 4 * lay["piece_entries"] + per_wg + 12 * zero_rows,
+        - 8 * wide_rows + 512 * lay["partial_slots"] + 1024 * lay["pieces"] + per_wg + 12 * zero_rows,
         "pr_slice_finish": hubs + 512 * lay["partial_slots"] + 24 * wide_rows,
         "pr_spmv_bin": 2 * stream + stream // 8 + 28 * lay["pull_rows"] + 20 * lay["chunks"]
         + 4 * lay["piece_entries"] + per_wg + 12 * zero_rows,

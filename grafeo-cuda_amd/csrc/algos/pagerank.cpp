@@ -181,7 +181,6 @@ static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, ui
   } catch (const std::runtime_error& e) {
     fail_kernel(e.what());
   }
-  std::vector<uint32_t>().swap(tgt);
   auto plan = std::unique_ptr<PrBinPlan>(new PrBinPlan());
   PrBinPlan& p = *plan;
   p.push_count = push_count;
@@ -205,15 +204,10 @@ static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, ui
   };
   p.sliced = sliced;
   if (sliced) {
-    std::vector<uint32_t> hub_len(h.pieces());
-    for (uint32_t i = 0; i < h.pieces(); ++i) {
-      const uint32_t row = h.hub_pieces[2 * i], b = h.hub_pieces[2 * i + 1];
-      hub_len[i] = std::min(b + 512u, g.rl.host_offsets[row + 1]) - b;
-    }
     PrSliceLayout sl;
     try {
       sl = build_pr_slice_layout(h.hoff.data(), h.hot16.data(), h.zero_rows, h.push_base, hot_count, h.hub_pieces,
-                                 hub_len, cap_groups, waves,
+                                 g.rl.host_offsets.data(), tgt.data(), cap_groups, waves,
                                  [](uint64_t count, const std::function<void(uint64_t, uint64_t)>& fn) {
                                    parallel_for(count, fn, 1);
                                  });
@@ -231,8 +225,11 @@ static PrBinPlan& bin_plan(Graph& g, uint32_t push_count, uint32_t hot_count, ui
     up(p.items, sl.items.data(), sizeof(uint32_t) * sl.items.size());
     up(p.wave_first, sl.wave_first.data(), sizeof(uint32_t) * sl.wave_first.size());
     up(p.long_slices, sl.long_slices.data(), sizeof(uint32_t) * sl.long_slices.size());
+    up(p.hub16, sl.hub16.data(), sizeof(uint16_t) * sl.hub16.size());
     p.slice_partial.allocate(sizeof(double) * 64 * std::max<uint32_t>(sl.partial_slots, 1));
-  } else {
+  }
+  std::vector<uint32_t>().swap(tgt);
+  if (!sliced) {
     // the 16-bit stream padded to whole 512-entry chunks with entries that read the zero
     h.hot16.resize(static_cast<size_t>(p.chunks) * 512, static_cast<uint16_t>(hot_count));
     up(p.hot16, h.hot16.data(), sizeof(uint16_t) * h.hot16.size());
@@ -438,7 +435,8 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
         if (err != hipSuccess) fail_kernel(std::string("pr_bin_accum launch failed: ") + hipGetErrorString(err));
       }
       if (slice) {
-        err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, tgt, plan->slice_data.as<const uint16_t>(),
+        err = try_launch(dev, spmv, grid, kPrHotThreads, shmem, off, plan->hub16.as<const uint16_t>(),
+                         plan->slice_data.as<const uint16_t>(),
                          plan->items.as<const uint4>(), plan->wave_first.as<const uint32_t>(),
                          plan->cold_sum.as<const double>(), static_cast<const float*>(contrib[in]),
                          static_cast<const float*>(rank[in]), contrib[out], rank[out], n, d,

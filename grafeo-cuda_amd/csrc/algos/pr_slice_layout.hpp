@@ -22,7 +22,9 @@
 //   slice piece (kind 1): first entry of its groups, groups | R << 16,
 //     partial slot (64 doubles written there; the last piece of a slice takes
 //     its remainder, R = 0 on the others);
-//   hub piece (kind 2): hub row, first entry, unused.
+//   hub piece (kind 2): hub row, first entry, first of its 512 entries in
+//     hub16 (the piece's entries as offsets from push_base, 0xFFFF past the
+//     piece's end).
 // Slices wider than cap_groups groups are cut into pieces of cap_groups
 // groups; long_slices lists (slice, first partial slot, pieces) for the
 // finishing kernel, which adds a row's pieces in order. Items are dealt to
@@ -49,23 +51,39 @@ struct PrSliceLayout {
   std::vector<uint32_t> items;        // 4 words per item
   std::vector<uint32_t> wave_first;   // [waves + 1]
   std::vector<uint32_t> long_slices;  // (slice, first partial slot, pieces) triples
+  std::vector<uint16_t> hub16;        // [hub pieces * 512]
   uint32_t item_count() const { return static_cast<uint32_t>(items.size() / 4); }
   uint32_t long_count() const { return static_cast<uint32_t>(long_slices.size() / 3); }
 };
 
 constexpr uint32_t kPrSliceWhole = 0, kPrSlicePiece = 1, kPrSliceHub = 2;
 constexpr uint32_t kPrSlicePad = 1024;  // entries past the last slice
+constexpr uint16_t kPrHubNone = 0xFFFF;  // a hub piece's entry past its end
 
-// hub_len[i]: the entry count of hub piece i (hub_pieces holds (row, first entry) pairs).
+// hub_pieces holds (hub row, first entry) pairs of the CSR (off, tgt): up to
+// 512 entries from there, in the row, all of them at or above push_base.
 template <class ParFor>
 PrSliceLayout build_pr_slice_layout(const uint32_t* hoff, const uint16_t* hot16, uint32_t z, uint32_t push_base,
                                     uint32_t hot_count, const std::vector<uint32_t>& hub_pieces,
-                                    const std::vector<uint32_t>& hub_len, uint32_t cap_groups, uint32_t waves,
+                                    const uint32_t* off, const uint32_t* tgt, uint32_t cap_groups, uint32_t waves,
                                     ParFor par) {
   if (cap_groups == 0 || cap_groups >= 0xFFFFu || waves == 0 || hot_count >= 0xFFFFu || push_base < z ||
-      hub_len.size() * 2 != hub_pieces.size())
+      hub_pieces.size() % 2 != 0)
     throw std::runtime_error("PageRank: sliced layout sizes out of range");
   PrSliceLayout p;
+  const size_t pieces = hub_pieces.size() / 2;
+  std::vector<uint32_t> hub_len(pieces);
+  p.hub16.assign(pieces * 512, kPrHubNone);
+  for (size_t i = 0; i < pieces; ++i) {
+    const uint32_t h = hub_pieces[2 * i], b = hub_pieces[2 * i + 1];
+    hub_len[i] = std::min(b + 512u, off[h + 1]) - b;
+    for (uint32_t j = 0; j < hub_len[i]; ++j) {
+      const uint32_t t = tgt[b + j];
+      if (t < push_base || t - push_base >= kPrHubNone)
+        throw std::runtime_error("PageRank: a hub piece's entry lies outside the push set");
+      p.hub16[512 * i + j] = static_cast<uint16_t>(t - push_base);
+    }
+  }
   const uint32_t pull = push_base - z;
   const uint32_t S = (pull + 63) / 64;
   p.slices = S;
@@ -138,7 +156,8 @@ PrSliceLayout build_pr_slice_layout(const uint32_t* hoff, const uint16_t* hot16,
     }
   }
   for (size_t i = 0; i < hub_len.size(); ++i) {
-    all.push_back({{kPrSliceHub << 30 | static_cast<uint32_t>(i), hub_pieces[2 * i], hub_pieces[2 * i + 1], 0},
+    all.push_back({{kPrSliceHub << 30 | static_cast<uint32_t>(i), hub_pieces[2 * i], hub_pieces[2 * i + 1],
+                    static_cast<uint32_t>(512 * i)},
                    kHubCost + (hub_len[i] + 63) / 64});
   }
   if (all.size() >= (1u << 30)) throw std::runtime_error("PageRank: too many sliced items");

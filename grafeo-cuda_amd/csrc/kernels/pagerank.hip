@@ -1731,10 +1731,23 @@ __device__ __forceinline__ void prs_item(uint32_t x, uint32_t y, uint32_t zw, ui
   }
 }
 
+// Hub piece (one wave) of the sliced layout: its up to 512 entries in P as
+// 16-bit offsets from push_base (0xFFFF past the piece's end), eight per lane
+// in one 16-byte load, each taking c[h] (the pushes of prb_hub_piece).
+__device__ __forceinline__ void prs_hub_piece(uint32_t h, uint32_t first, const uint16_t* __restrict__ hub16,
+                                              const float* __restrict__ s_hot, uint32_t hot_base,
+                                              unsigned long long* __restrict__ s_push) {
+  const unsigned long long cfix = pr_to_fixed(s_hot[h - hot_base]);
+  const prb_u16x8 t = __builtin_nontemporal_load(reinterpret_cast<const prb_u16x8*>(hub16 + first) + lane_id());
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (t[k] != 0xFFFFu) atomicAdd(s_push + t[k], cfix);
+}
+
 // Dynamic LDS: push_count u64 accumulators, then hot_count + 1 contributions
 // (the last one 0), as pr_spmv_bin.
 extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_slice(
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets,
+    const uint32_t* __restrict__ offsets, const uint16_t* __restrict__ hub16,
     const uint16_t* __restrict__ data, const uint4* __restrict__ items, const uint32_t* __restrict__ wave_first,
     const double* __restrict__ cold_sum, const float* __restrict__ contrib_in, const float* __restrict__ rank_in,
     float* __restrict__ contrib_out, float* __restrict__ rank_out, uint32_t n, double damping,
@@ -1768,7 +1781,7 @@ extern "C" __global__ __launch_bounds__(PRP_THREADS) void pr_spmv_slice(
     for (; k < k1; ++k) {
       const uint4 it = items[k];
       if ((it.x >> 30) != 2u) break;
-      prb_hub_piece(it.y, it.z, offsets, targets, s_hot, hot_base, s_push, push_base);
+      prs_hub_piece(it.y, it.w, hub16, s_hot, hot_base, s_push);
     }
     if (k < k1) {
       // slice items, one ahead; the last item's loads are issued again past the end

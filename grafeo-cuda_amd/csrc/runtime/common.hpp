@@ -335,7 +335,7 @@ struct PrBinPlan {
   uint32_t cap_groups = 0, waves = 0;  // key
   uint32_t slices = 0, slice_items = 0, long_count = 0, partial_slots = 0;
   uint64_t slice_entries = 0;
-  DeviceMemory slice_data, items, wave_first, long_slices, slice_partial;
+  DeviceMemory slice_data, items, wave_first, long_slices, slice_partial, hub16;
 };
 
 struct Graph {

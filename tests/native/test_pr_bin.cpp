@@ -96,13 +96,20 @@ void serial(uint64_t count, const std::function<void(uint64_t, uint64_t)>& fn) {
 void check_slices(const Csr& g, const gc::PrBinLayout& p, uint32_t hot, uint32_t cap, uint32_t waves,
                   const std::string& tag0) {
   const std::string tag = tag0 + " cap=" + std::to_string(cap) + " waves=" + std::to_string(waves);
-  std::vector<uint32_t> hub_len(p.pieces());
-  for (uint32_t i = 0; i < p.pieces(); ++i) {
-    const uint32_t h = p.hub_pieces[2 * i], b = p.hub_pieces[2 * i + 1];
-    hub_len[i] = std::min(b + 512u, g.off[h + 1]) - b;
-  }
   const gc::PrSliceLayout sl = gc::build_pr_slice_layout(p.hoff.data(), p.hot16.data(), p.zero_rows, p.push_base,
-                                                         hot, p.hub_pieces, hub_len, cap, waves, serial);
+                                                         hot, p.hub_pieces, g.off.data(), g.tgt.data(), cap, waves,
+                                                         serial);
+  // the hub pieces' entries as offsets from push_base, 0xFFFF past each piece's end
+  check(sl.hub16.size() == static_cast<size_t>(p.pieces()) * 512, tag + ": hub16 size");
+  for (uint32_t i = 0; i < p.pieces() && sl.hub16.size() == static_cast<size_t>(p.pieces()) * 512; ++i) {
+    const uint32_t h = p.hub_pieces[2 * i], b = p.hub_pieces[2 * i + 1];
+    const uint32_t len = std::min(b + 512u, g.off[h + 1]) - b;
+    for (uint32_t j = 0; j < 512; ++j) {
+      const uint16_t x = sl.hub16[512 * static_cast<size_t>(i) + j];
+      if (j < len) check(x == g.tgt[b + j] - p.push_base, tag + ": hub16 entry");
+      else check(x == gc::kPrHubNone, tag + ": hub16 padding");
+    }
+  }
   const uint32_t z = p.zero_rows, pb = p.push_base, S = sl.slices;
   check(S == (pb - z + 63) / 64, tag + ": slice count");
   check(sl.slice_off.size() == S + 1u && sl.slice_off[S] + gc::kPrSlicePad == sl.data.size(), tag + ": slice offsets");
@@ -118,7 +125,8 @@ void check_slices(const Csr& g, const gc::PrBinLayout& p, uint32_t hot, uint32_t
     const uint32_t* it = &sl.items[4 * static_cast<size_t>(k)];
     const uint32_t kind = it[0] >> 30, idx = it[0] & 0x3FFFFFFFu;
     if (kind == gc::kPrSliceHub) {
-      check(idx < p.pieces() && it[1] == p.hub_pieces[2 * idx] && it[2] == p.hub_pieces[2 * idx + 1],
+      check(idx < p.pieces() && it[1] == p.hub_pieces[2 * idx] && it[2] == p.hub_pieces[2 * idx + 1] &&
+                it[3] == 512 * idx,
             tag + ": hub item");
       if (idx < p.pieces()) ++hub_seen[idx];
       continue;
