@@ -824,9 +824,12 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16_w4
 
 // The same with one-wave workgroups (the default; tc_bitmap16_w4 above with
 // four): the wave's bitmap is the workgroup's only LDS, at address 0, so a
-// lookup's address needs no per-wave base (eight adds per oct fewer). At
-// R-MAT scale 22, 6.75 against 6.82 ms on one box.
-extern "C" __global__ __launch_bounds__(GC_WAVE) void tc_bitmap16(
+// lookup's address needs no per-wave base (eight adds per oct fewer). The
+// kernel waits on its list loads two thirds of the time, so it is built for
+// eight waves per SIMD (its SGPRs held it to seven; the compiler now keeps 24
+// of them in VGPR lanes). At R-MAT scale 22 on one box: 5.92 ms, against 6.79
+// for tc_bitmap16_w4 (6.75 against 6.82 at seven waves).
+extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) void tc_bitmap16(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
     const uint32_t* __restrict__ split, const uint2* __restrict__ hw, const uint4* __restrict__ quads,
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
