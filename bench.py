@@ -48,7 +48,8 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
 KERNEL_FAMILIES = {
-    "tc": ("tc_bitmap16", "tc_bitmap16_w4", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count"),
+    "tc": ("tc_bitmap16", "tc_mid16", "tc_bitmap16_w4", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count",
+           "tc_mid_rows", "tc_mid_fill", "tc_mid_records", "tc_mid_rowwork"),
     "pagerank": ("pr_spmv_slice", "pr_slice_finish", "pr_spmv_bin", "pr_bin_scatter", "pr_bin_accum",
                  "pr_spmv_merge", "pr_merge_finish",
                  "pr_spmv_push", "pr_push_finish", "pr_spmv_hot", "pr_spmv",
@@ -68,7 +69,7 @@ KERNEL_FAMILIES = {
 KERNELS = tuple(k for ks in KERNEL_FAMILIES.values() for k in ks)
 # The kernels timed with events in a default run: those the rooflines report. Each timed launch
 # adds two event records to the stream (about 2 ms per step if every launch were timed).
-ROOFLINE_KERNELS = ("tc_bitmap16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_slice", "pr_slice_finish",
+ROOFLINE_KERNELS = ("tc_bitmap16", "tc_mid16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_slice", "pr_slice_finish",
                     "pr_spmv_bin", "pr_merge_finish", "pr_bin_scatter", "pr_bin_accum", "pr_spmv_merge",
                     "pr_spmv_push", "pr_spmv_hot", "pr_spmv")
 # The binned SpMV (the default on symmetric graphs), four kernels per iteration: the scatter and
@@ -181,17 +182,33 @@ def tc_top_bits() -> int:
     return int(env) if env is not None else 4096
 
 
-def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bits: int = 32768,
-                          top_bits: int | None = None) -> dict:
-    """Bytes tc_bitmap16 reads per launch, by the model that fits the kernel (on the degree-ordered
-    DAG the kernel runs on, `rl` = GpuGraph.relabelled_numpy()): offsets and the DAG split read
-    once, 4(n+1) + 4n; L(u) at 4 bytes per entry once per part it is split into (rows of at
-    least `split_min` DAG entries run in `parts` parts, shorter rows once); and for every DAG
-    edge u -> v the list L(v) at its stored width: its entries among the top `hot_bits` ids as
-    16-bit offsets in whole 16-byte octs of eight, its other entries at 4 bytes; a list whose
-    entries among the top `top_bits` ids fill more octs than their mask (top_bits / 8 bytes)
-    holds the mask instead of those offsets (from the mask's oct of id v + 1 on)."""
+def tc_env(name: str, default: int) -> int:
+    env = os.environ.get(name)
+    return int(env) if env is not None else default
+
+
+def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int | None = None, hot_bits: int = 32768,
+                          top_bits: int | None = None, mid: bool | None = None, cost: int | None = None,
+                          unit: int | None = None) -> dict:
+    """Bytes the default triangle count reads per launch (tc_bitmap16, then tc_mid16), by the model
+    that fits the kernels (on the degree-ordered DAG they run on, `rl` = GpuGraph.relabelled_numpy()).
+
+    Every DAG edge u -> v is counted on one side. tc_bitmap16's side: L(v) at its stored width (its
+    entries among the top `hot_bits` ids as 16-bit offsets in whole 16-byte octs of eight, its other
+    entries at 4 bytes; a list whose entries among the top `top_bits` ids fill more octs than their
+    mask holds the mask instead, from the mask's oct of id v + 1 on), plus, once per launch, the
+    offsets and the DAG split, 4(n+1) + 4n, and L(u) at 4 bytes per entry once per part it is split
+    into (rows of at least `split_min` DAG entries run in `parts` parts, shorter rows once) for the
+    rows with an edge left on this side, and one bit per DAG entry (which side). tc_mid16's side
+    (`mid`, the library's default; an edge u -> v with v among the hot ids whose walk is empty or
+    whose walk's octs plus `cost` are fewer than the octs of L(v)): the walk, the 16-byte octs of
+    the part of L(u) above v in natural order; an 8-byte record per entry of the hot rows; and L(v)
+    at 4 bytes twice (its bits set, then cleared) per work unit of `unit` entries of v's row."""
     top_bits = tc_top_bits() if top_bits is None else top_bits
+    mid = (os.environ.get("GC_TC_MID", "1") != "0") if mid is None else mid
+    cost = tc_env("GC_TC_MID_COST", 2) if cost is None else cost
+    unit = max(64, tc_env("GC_TC_MID_UNIT", 8192)) if unit is None else unit
+    split_min = tc_env("GC_TC_SPLIT", 1024 if mid else 256) if split_min is None else split_min
     off = rl["offsets"].astype(np.int64)
     fwd = rl["fwd"].astype(np.int64)
     tgt = rl["targets"]
@@ -200,8 +217,10 @@ def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bi
     dplus = off[1:] - fwd
     # DAG entries: targets[fwd[u] .. off[u+1]) of every row u
     row = np.repeat(np.arange(n, dtype=np.int32), dplus)
-    starts = np.repeat(fwd - np.concatenate(([0], np.cumsum(dplus)[:-1])), dplus)
+    dag_first = np.concatenate(([0], np.cumsum(dplus)[:-1]))
+    starts = np.repeat(fwd - dag_first, dplus)
     dag_v = tgt[np.arange(row.size, dtype=np.int64) + starts].astype(np.int64)
+    del starts
     hot = np.bincount(row, weights=(dag_v >= hot_base), minlength=n).astype(np.int64)
     cold = dplus - hot
     octs = (hot + 7) // 8
@@ -216,14 +235,37 @@ def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bi
         masked = (top >> 3) > mask_octs
         octs = np.where(masked, (hot - top + 7) // 8 + mask_octs, octs)
         masked_rows = int(masked.sum())
-    indeg = np.bincount(dag_v, minlength=n).astype(np.int64)
+    mid_bytes = 0
+    mid_edges = 0
+    if mid:
+        k = np.arange(row.size, dtype=np.int64) - np.repeat(dag_first, dplus)  # place in L(u)
+        last = k == dplus[row] - 1
+        # octs from the one holding the entry after v to the end of u's hot part
+        walk = np.where(last, 0, (hot[row] + 7) // 8 - (k - cold[row] + 1) // 8)
+        is_mid = (dag_v >= hot_base) & (last | (walk + cost < octs[dag_v]))
+        del k, last
+        walk_octs = int(walk[is_mid].sum())
+        del walk
+        mid_edges = int(is_mid.sum())
+        units = np.zeros(n, np.int64)
+        deg = np.diff(off)
+        units[hot_base:] = (deg[hot_base:] + unit - 1) // unit
+        mid_bytes = 16 * walk_octs + 8 * int(off[n] - off[hot_base]) + int((8 * dplus * units).sum())
+        row_side = ~is_mid
+        del is_mid
+    else:
+        row_side = np.ones(row.size, bool)
+    indeg = np.bincount(dag_v[row_side], minlength=n).astype(np.int64)
     streamed = int((indeg * (16 * octs + 4 * cold)).sum())
-    row_reads = int((4 * dplus * np.where(dplus >= split_min, parts, 1)).sum())
-    total = 4 * (n + 1) + 4 * n + row_reads + streamed
-    return {"bytes": total, "list_bytes": streamed, "list_bytes_32bit": int((indeg * 4 * dplus).sum()),
-            "row_bytes": row_reads, "hot_base": int(hot_base), "top_bits": int(top_bits),
-            "masked_rows": masked_rows,
-            "hot_fraction_of_streamed_entries": float((indeg * hot).sum() / max(1, (indeg * dplus).sum()))}
+    rowwork = (dplus >= 2) & (np.bincount(row[row_side], minlength=n) > 0)
+    row_reads = int((4 * dplus * np.where(dplus >= split_min, parts, 1) * rowwork).sum())
+    row_total = 4 * (n + 1) + 4 * n + row_reads + streamed + (row.size // 8 if mid else 0)
+    indeg_all = np.bincount(dag_v, minlength=n).astype(np.int64)
+    return {"bytes": row_total + mid_bytes, "row_side_bytes": row_total, "mid_side_bytes": mid_bytes,
+            "mid": bool(mid), "mid_edges": mid_edges, "dag_edges": int(row.size), "list_bytes": streamed,
+            "list_bytes_32bit": int((indeg * 4 * dplus).sum()), "row_bytes": row_reads, "hot_base": int(hot_base),
+            "top_bits": int(top_bits), "masked_rows": masked_rows, "split_min": int(split_min),
+            "hot_fraction_of_streamed_entries": float((indeg_all * hot).sum() / max(1, (indeg_all * dplus).sum()))}
 
 
 def dag_merge_bytes(off: np.ndarray, tgt: np.ndarray) -> int:
@@ -385,7 +427,8 @@ def main() -> int:
     tc_merge_bytes = 4 * (n + 1) + 4 * m_o.value + 4 * inter.value
     wedge_bytes = 4 * (n + 1) + 4 * n + 4 * m + 4 * wedges.value
     t0 = time.time()
-    tc16 = tc_stored_width_bytes(g.relabelled_numpy())
+    tc16 = tc_stored_width_bytes(g.relabelled_numpy(), mid=os.environ.get("GC_TC_MID", "1") != "0"
+                                 and os.environ.get("GC_TC_HOT16", "1") != "0")
     log(f"tc_bitmap16 model: {tc16['bytes'] / 1e9:.2f} GB per launch ({time.time() - t0:.1f}s)")
     traffic = load_traffic(f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}")
     rooflines = {}
@@ -428,20 +471,32 @@ def main() -> int:
         if name in kernels:
             rooflines[name] = roofline_entry(tc_merge_bytes, kernels[name]["avg_us"], traffic.get(name),
                                              "merge model: 4(n+1) + 4 m_o + 4 sum over DAG edges (d+u + d+v)")
+    tc_note = ("stored-width model (bench.py tc_stored_width_bytes): 4(n+1) + 4n, L(u) at 4 bytes once per "
+               "part it is split into, and for every DAG edge u->v counted by tc_bitmap16 the list L(v) at its "
+               "stored width (entries among the top 32768 ids as 16-bit offsets in 16-byte octs of eight, "
+               f"others at 4 bytes, the entries among the top {tc16['top_bits']} ids as a mask of at most "
+               f"{tc16['top_bits'] // 8} bytes where that is smaller: {tc16['masked_rows']} lists)")
+    if tc16["mid"]:
+        tc_note += (f"; the {tc16['mid_edges']} of {tc16['dag_edges']} DAG edges u->v counted by tc_mid16 at v: "
+                    "the octs of L(u) above v (16-byte octs, natural order), an 8-byte record per entry of the "
+                    "hot rows, L(v) at 4 bytes twice per work unit, and one bit per DAG entry")
+    tc_note += f"; the merge model (both lists at 4 bytes, {tc_merge_bytes} bytes) would count data never read"
     for name in ("tc_bitmap16", "tc_bitmap"):
         if name in kernels:
             # tc_bitmap streams the same lists at 4 bytes per entry
-            model = tc16["bytes"] if name == "tc_bitmap16" else (
+            model = tc16["row_side_bytes"] if name == "tc_bitmap16" else (
                 tc16["bytes"] - tc16["list_bytes"] + tc16["list_bytes_32bit"])
-            rooflines[name] = roofline_entry(
-                model, kernels[name]["avg_us"], traffic.get(name),
-                "stored-width model: 4(n+1) + 4n, L(u) at 4 bytes once per part it is split into, and "
-                "for every DAG edge u->v the list L(v) at its stored width (entries among the top 32768 "
-                "ids as 16-bit offsets in 16-byte octs of eight, others at 4 bytes, and the entries among "
-                f"the top {tc16['top_bits']} ids as a mask of at most {tc16['top_bits'] // 8} bytes where that is "
-                f"smaller: {tc16['masked_rows']} lists); the merge model "
-                f"(both lists at 4 bytes, {tc_merge_bytes} bytes) would count data the kernel never reads")
+            rooflines[name] = roofline_entry(model, kernels[name]["avg_us"], traffic.get(name), tc_note)
             rooflines[name]["merge_model_bytes"] = tc_merge_bytes
+    if "tc_mid16" in kernels and "tc_bitmap16" in kernels:
+        rooflines["tc_mid16"] = roofline_entry(tc16["mid_side_bytes"], kernels["tc_mid16"]["avg_us"],
+                                               traffic.get("tc_mid16"), tc_note)
+        # the count as one unit: both kernels, once each per count
+        t_pair = [traffic.get("tc_bitmap16"), traffic.get("tc_mid16")]
+        rooflines["tc_bitmap16+tc_mid16"] = roofline_entry(
+            tc16["bytes"], kernels["tc_bitmap16"]["avg_us"] + kernels["tc_mid16"]["avg_us"],
+            sum(t_pair) if all(t_pair) else None, tc_note)
+        rooflines["tc_bitmap16+tc_mid16"]["merge_model_bytes"] = tc_merge_bytes
     if "tc_wedge" in kernels:
         rooflines["tc_wedge"] = roofline_entry(wedge_bytes, kernels["tc_wedge"]["avg_us"], traffic.get("tc_wedge"))
     # BFS over the whole call per source (every level's kernels, launch gaps and the read-back of
@@ -450,8 +505,12 @@ def main() -> int:
         bfs_bytes, 1000.0 * per_algo["bfs_ms_per_source"], bfs_call_traffic(
             f"rmat{args.scale}_ef{args.edge_factor}_seed{args.seed}"),
         "4(n+1) + 4m + 4n per traversal; avg_launch_us is the whole BFS call per source")
-    # The dominant kernel: the one with the most event time in the profiled step.
-    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
+    # The dominant kernel: the one with the most event time in the profiled step (the triangle
+    # count's two kernels, which split its edges between them, as one).
+    units = {k: v["total_ms"] for k, v in kernels.items()}
+    if "tc_bitmap16" in units and "tc_mid16" in units:
+        units["tc_bitmap16+tc_mid16"] = units.pop("tc_bitmap16") + units.pop("tc_mid16")
+    dominant = max(units, key=units.get) if units else None
     roofline = dict(rooflines.get(dominant) or {})
     if roofline:
         roofline["kernel"] = dominant
@@ -600,7 +659,7 @@ def reference_convention(dev, g, offsets, sources, args, m) -> dict:
     # 32-bit lists (tc_bitmap's stored bytes, plus a 4-byte read-modify-write of one support per
     # triangle), and then only peels.
     n = offsets.size - 1
-    tcm = tc_stored_width_bytes(g.relabelled_numpy())
+    tcm = tc_stored_width_bytes(g.relabelled_numpy(), mid=False, split_min=tc_env("GC_TC_SPLIT", 256))
     triangles = g.triangle_count()["total_triangles"]
     kt_bytes = (4 * (n + 1) + 4 * n + tcm["row_bytes"] + tcm["list_bytes_32bit"] + 8 * triangles
                 + 5 * m * rounds)

@@ -157,13 +157,118 @@ void ensure_hot16(Graph& g, uint32_t hot_base, uint32_t parts, uint32_t top_octs
   g.tc_hot16_top = top_octs;
 }
 
+// First relabelled row of degree >= d (rows are sorted by degree), or n.
+uint32_t first_row_of_degree(const Graph& g, uint64_t d) {
+  const std::vector<uint32_t>& off = g.rl.host_offsets;
+  uint32_t lo = 0, hi = g.n;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (off[mid + 1] - off[mid] >= d) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// The middle-vertex side of tc_bitmap16 (tc_mid16) unless GC_TC_MID=0. An edge
+// u->v goes to it when the octs of its walk plus GC_TC_MID_COST (default 2) are
+// fewer than the octs tc_bitmap16 streams for L(v); GC_TC_MID_UNIT (default
+// 8192) entries of a hot row's in-list make one work unit. Measured at R-MAT
+// scale 22 (rows split at 1024 entries): 5.97 ms without the middle side, 5.30
+// with it; costs 0, 2, 8 and 32 octs within 5%; units of 512, 2048, 8192 and
+// 32768 entries 5.96, 5.56, 5.30 and 5.30 ms (2048 and 512 at split 256).
+bool tc_mid() {
+  const char* env = std::getenv("GC_TC_MID");
+  return env == nullptr || std::strcmp(env, "0") != 0;
+}
+
+uint32_t tc_mid_cost() {
+  const char* env = std::getenv("GC_TC_MID_COST");
+  return env == nullptr ? 2u : static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
+}
+
+uint32_t tc_mid_unit() {
+  const char* env = std::getenv("GC_TC_MID_UNIT");
+  const uint32_t u = env == nullptr ? 8192u : static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
+  return std::max(64u, u);
+}
+
+// The middle side's lists, records, edge bits and units, built once per graph
+// after the 16-bit lists (again if the parts, the masks or the per-edge cost
+// change, since they decide which edges go where). At R-MAT scale 22: 16-byte
+// octs of the DAG rows' hot parts (about 160 MB), 8 bytes per entry of the top
+// 32768 rows, one bit per CSR entry.
+void ensure_mid(Graph& g, uint32_t hot_base, uint32_t parts, uint32_t top_octs, uint32_t cost) {
+  const uint32_t unit = tc_mid_unit();
+  if (g.tc_mid_ready && g.tc_mid_key[0] == parts && g.tc_mid_key[1] == top_octs && g.tc_mid_key[2] == cost &&
+      g.tc_mid_key[3] == unit)
+    return;
+  g.tc_mid_ready = false;
+  Device& dev = *g.dev;
+  const uint32_t n = g.n;
+  const std::vector<uint32_t>& off = g.rl.host_offsets;
+  const uint64_t m = off[n];
+  const uint32_t* offsets = g.rl.offsets.as<const uint32_t>();
+  const uint32_t* tgt = g.rl.targets.as<const uint32_t>();
+  const uint64_t rows = std::max<uint64_t>(n, 1);
+  DeviceMemory hq, scan;
+  hq.allocate(sizeof(uint32_t) * (rows + 1));
+  scan.allocate(sizeof(uint32_t) * scan_scratch_words(rows + 1));
+  Grid1D vg = thread_per_item(rows);
+  launch(dev, get_kernel(dev, "tc", "tc_mid_rows"), vg.blocks, vg.threads, 0, g.tc_split.as<const uint32_t>(),
+         offsets + 1, n, hq.as<uint32_t>());
+  fill_words(dev, hq.as<uint32_t>() + n, 1, 0);
+  exclusive_scan_u32(dev, hq.as<uint32_t>(), static_cast<uint64_t>(n) + 1, scan.as<uint32_t>());
+  uint32_t total_octs = 0;
+  copy_to_host(dev, &total_octs, hq.as<uint32_t>() + n, sizeof(uint32_t));
+  g.tc_mid_octs.ensure(16ull * (static_cast<uint64_t>(total_octs) + 1));
+  Grid1D eg = thread_per_item(std::max<uint64_t>(m, n));
+  launch(dev, get_kernel(dev, "tc", "tc_mid_fill"), eg.blocks, eg.threads, 0, offsets, tgt, n, m, hot_base,
+         g.tc_split.as<const uint32_t>(), static_cast<const uint32_t*>(hq.as<uint32_t>()),
+         g.tc_mid_octs.as<uint16_t>());
+  const uint32_t rec_base = off[hot_base];
+  const uint64_t recs = m - rec_base;
+  g.tc_mid_rec.ensure(sizeof(uint32_t) * 2 * std::max<uint64_t>(recs, 1));
+  const uint64_t bit_words = (m + 31) / 32 + 1;
+  g.tc_midbits.ensure(sizeof(uint32_t) * bit_words);
+  fill_words(dev, g.tc_midbits.as<uint32_t>(), bit_words, 0);
+  if (recs > 0) {
+    Grid1D rg = thread_per_item(recs);
+    launch(dev, get_kernel(dev, "tc", "tc_mid_records"), rg.blocks, rg.threads, 0, offsets, tgt,
+           g.rl.fwd.as<const uint32_t>(), n, hot_base, g.tc_split.as<const uint32_t>(),
+           g.tc_hw.as<const uint32_t>() /* uint2[n] */, static_cast<const uint32_t*>(hq.as<uint32_t>()), cost,
+           g.tc_mid_rec.as<uint32_t>() /* uint2 */, g.tc_midbits.as<uint32_t>());
+  }
+  g.tc_rowwork.ensure(rows);
+  launch(dev, get_kernel(dev, "tc", "tc_mid_rowwork"), vg.blocks, vg.threads, 0, g.rl.fwd.as<const uint32_t>(),
+         offsets + 1, n, static_cast<const uint32_t*>(g.tc_midbits.as<uint32_t>()), g.tc_rowwork.as<uint8_t>());
+  // units: the hot rows' entry ranges in pieces of at most tc_mid_unit() entries, top row first
+  std::vector<uint32_t> units;
+  for (uint32_t v = n; v-- > hot_base;) {
+    for (uint32_t b = off[v]; b < off[v + 1]; b += unit) {
+      units.insert(units.end(), {v, b, std::min<uint32_t>(off[v + 1], b + unit), 0u});
+    }
+  }
+  g.tc_mid_num_units = static_cast<uint32_t>(units.size() / 4);
+  g.tc_mid_units.ensure(sizeof(uint32_t) * std::max<size_t>(units.size(), 4));
+  if (!units.empty()) copy_to_device(dev, g.tc_mid_units.as<uint32_t>(), units.data(), sizeof(uint32_t) * units.size());
+  dev.sync();  // hq and scan are released on return
+  g.tc_mid_rec_base = rec_base;
+  g.tc_mid_key[0] = parts;
+  g.tc_mid_key[1] = top_octs;
+  g.tc_mid_key[2] = cost;
+  g.tc_mid_key[3] = unit;
+  g.tc_mid_ready = true;
+}
+
 // Shortest L(u) that tc_bitmap splits over the parts (shorter rows run whole,
 // once): GC_TC_SPLIT, default 256. With the 16-bit lists, R-MAT scale 22
 // counted in 8.92 ms at 256 against 9.13 at 64 (scale 24: 86.0 against
 // 87.3 ms; scale 20 unchanged): a split row builds its bitmap once per part.
-uint32_t tc_split_min() {
+// With the middle side on (mid), whose walks take most of the long lists, the
+// default is 1024: 5.30 ms at R-MAT scale 22 against 5.56 at 256 (64: 5.99).
+uint32_t tc_split_min(bool mid = false) {
   const char* env = std::getenv("GC_TC_SPLIT");
-  if (env == nullptr) return 256;
+  if (env == nullptr) return mid ? 1024 : 256;
   return static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
 }
 
@@ -232,10 +337,24 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
       if (!no_alloc) ensure_hot16(g, hot_base, parts, top_octs);
       const uint32_t top_word0 = (tc_top_base(n, hot_base, top_octs) - hot_base) / 32u;
       const bool w1 = tc_wg_waves() == 1;
+      // the middle side takes its edges when the launch covers every row that can close a
+      // triangle (the rows below the first of degree 2 have at most one DAG entry)
+      const bool mid = !no_alloc && tc_mid() && row_hi == n && row_lo <= first_row_of_degree(g, 2);
+      if (mid) ensure_mid(g, hot_base, parts, top_octs, tc_mid_cost());
+      const uint32_t* midbits = mid ? g.tc_midbits.as<const uint32_t>() : nullptr;
+      const uint8_t* rowwork = mid ? g.tc_rowwork.as<const uint8_t>() : nullptr;
       go(w1 ? "tc_bitmap16" : "tc_bitmap16_w4", w1 ? 1u : 4u, fwd, end, tgt, g.tc_split.as<const uint32_t>(),
          g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
-         row_lo, hot_base, single, light_chunk, parts, tc_split_min(), counters, result, top_word0, top_octs,
-         tc_quarters());
+         row_lo, hot_base, single, light_chunk, parts, tc_split_min(mid), counters, result, top_word0, top_octs,
+         tc_quarters(), midbits, rowwork);
+      if (mid && g.tc_mid_num_units > 0) {
+        // one-wave workgroups, as many as tc_bitmap16's, on work counter word 2 (unused by the parts)
+        const uint32_t blocks = std::min<uint32_t>(grid_for(g.tc_mid_num_units * 4ull) * 4u, g.tc_mid_num_units);
+        launch(dev, get_kernel(dev, "tc", "tc_mid16"), blocks, kWaveSize, 0, g.rl.offsets.as<const uint32_t>(), tgt,
+               fwd, hot_base, g.tc_mid_rec.as<const uint32_t>() /* uint2 */, g.tc_mid_rec_base,
+               g.tc_mid_octs.as<const uint32_t>() /* uint4 */, g.tc_mid_units.as<const uint32_t>() /* uint4 */,
+               g.tc_mid_num_units, tc_quarters(), counters + 2, result);
+      }
     } else {
       go("tc_bitmap", 4u, fwd, end, tgt, row_hi, row_lo, hot_base, single, light_chunk, parts, tc_split_min(),
          counters, result);
@@ -249,18 +368,6 @@ uint32_t* tc_scratch(Graph& g) {
   uint32_t* scratch = g.buffer<uint32_t>("tc.scratch", sizeof(uint32_t) * words);
   fill_words(*g.dev, scratch, words, 0);
   return scratch;
-}
-
-// First relabelled row of degree >= d (rows are sorted by degree), or n.
-uint32_t first_row_of_degree(const Graph& g, uint64_t d) {
-  const std::vector<uint32_t>& off = g.rl.host_offsets;
-  uint32_t lo = 0, hi = g.n;
-  while (lo < hi) {
-    const uint32_t mid = lo + (hi - lo) / 2;
-    if (off[mid + 1] - off[mid] >= d) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo;
 }
 
 // Degree buckets of the relabelled rows: bucket b holds the rows of degree in

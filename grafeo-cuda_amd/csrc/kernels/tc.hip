@@ -587,7 +587,8 @@ __device__ __forceinline__ void tc_bitmap_body(
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t* s_bits,
-    uint32_t top_word0 = 0, uint32_t top_octs = 0, uint32_t quarters = 0) {
+    uint32_t top_word0 = 0, uint32_t top_octs = 0, uint32_t quarters = 0,
+    const uint32_t* __restrict__ midbits = nullptr, const uint8_t* __restrict__ rowwork = nullptr) {
   const uint32_t positions = n - row_lo;  // rows [row_lo, n)
   const uint32_t lane = lane_id();
   for (uint32_t j = lane; j < TCB_BLOOM_BASE + TCB_BLOOM_WORDS; j += GC_WAVE) s_bits[j] = 0;
@@ -633,14 +634,17 @@ __device__ __forceinline__ void tc_bitmap_body(
           const uint32_t u = n - 1 - (kc + lane);
           row_lo_e = lo[u];
           row_a = hi[u] - row_lo_e;
-          take = row_a != 0 && (row_a >= split_min || u % parts == part);
+          // a row with one DAG entry v closes no triangle (v is not in L(v)); with the middle
+          // side on, rowwork[u] = 0 marks a row whose every entry that can close one went there
+          take = row_a >= 2 && (rowwork == nullptr || rowwork[u] != 0) && (row_a >= split_min || u % parts == part);
         }
         unsigned long long rows = __ballot(take);
         while (rows != 0) {
           const int jr = __ffsll(static_cast<long long>(rows)) - 1;
           rows &= rows - 1;
           const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(row_a, jr));
-          const uint32_t* A = targets + static_cast<uint32_t>(__builtin_amdgcn_readlane(row_lo_e, jr));
+          const uint32_t a0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(row_lo_e, jr));
+          const uint32_t* A = targets + a0;
           const bool whole = a < split_min;  // processed by one part, every v
           // cold prefix [0, c): entries below hot_base; set the hot suffix's bits
           uint32_t c;
@@ -684,7 +688,10 @@ __device__ __forceinline__ void tc_bitmap_body(
             bool masked = false;  // L(v)'s top part is a mask (TC_TOP_FLAG)
             if (lane < nv) {
               const uint32_t v = A[cv + lane];
-              if (whole || v % parts == part) {
+              // an entry u->v whose bit is set in midbits is counted by tc_mid16 at v
+              const uint32_t e = a0 + cv + lane;
+              const bool mid = midbits != nullptr && ((midbits[e >> 5] >> (e & 31u)) & 1u) != 0;
+              if (!mid && (whole || v % parts == part)) {
                 if (kHot16) {
                   const uint2 h = hw[v];
                   st = h.x;
@@ -815,11 +822,12 @@ extern "C" __global__ __launch_bounds__(TCB_WAVES * GC_WAVE) void tc_bitmap16_w4
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t top_word0,
-    uint32_t top_octs, uint32_t quarters) {
+    uint32_t top_octs, uint32_t quarters, const uint32_t* __restrict__ midbits,
+    const uint8_t* __restrict__ rowwork) {
   __shared__ uint32_t s_bits_all[TCB_WAVES][TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
   tc_bitmap_body<true>(lo, hi, targets, split, hw, quads, n, row_lo, hot_base, heavy, light_chunk, parts,
                        split_min, counters, result, s_bits_all[threadIdx.x / GC_WAVE], top_word0, top_octs,
-                       quarters);
+                       quarters, midbits, rowwork);
 }
 
 // The same with one-wave workgroups (the default; tc_bitmap16_w4 above with
@@ -835,10 +843,11 @@ extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per
     uint32_t n, uint32_t row_lo, uint32_t hot_base,
     uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
     uint32_t* __restrict__ counters, unsigned long long* __restrict__ result, uint32_t top_word0,
-    uint32_t top_octs, uint32_t quarters) {
+    uint32_t top_octs, uint32_t quarters, const uint32_t* __restrict__ midbits,
+    const uint8_t* __restrict__ rowwork) {
   __shared__ uint32_t s_bits[TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
   tc_bitmap_body<true>(lo, hi, targets, split, hw, quads, n, row_lo, hot_base, heavy, light_chunk, parts,
-                       split_min, counters, result, s_bits, top_word0, top_octs, quarters);
+                       split_min, counters, result, s_bits, top_word0, top_octs, quarters, midbits, rowwork);
 }
 
 // The hot lists of tc_bitmap16, in two passes around an exclusive scan of
@@ -932,6 +941,186 @@ extern "C" __global__ __launch_bounds__(256) void tc_hot16_fill(
       }
     }
   }
+}
+
+// ---- middle-vertex walk for the heavy lists (tc_mid16) ----
+//
+// A triangle u < v < w of the DAG (u->v, u->w, v->w) can be found from either
+// side of its edge u->v: tc_bitmap16 streams all of L(v) against the bitmap of
+// L(u); tc_mid16 streams the part of L(u) above v against the bitmap of L(v).
+// Both give |L(u) ∩ L(v)| (every entry of L(v) is above v). The second walk is
+// short wherever v sits near the end of L(u) and L(v) is long, which at R-MAT
+// scale 22 holds for 71% of the DAG edges: their share of tc_bitmap16's
+// streaming falls from 2.9e10 entries to 4.7e9 walked by tc_mid16. Each edge
+// u->v with v among the hot ids is given to the cheaper side once per graph
+// (tc_mid_records): the middle side when the octs of its walk, plus a fixed
+// cost per edge, are fewer than the octs tc_bitmap16 streams for L(v). Its
+// bit is then set in midbits (by DAG entry), which tc_bitmap16 skips.
+//
+// tc_mid16's lists are the DAG rows' hot parts in natural order, 16-bit
+// offsets from hot_base, each row from an oct boundary and padded to the next
+// with TCB_PAD16 (tc_mid_rows, then tc_mid_fill around an exclusive scan of
+// the octs). A walk starts at the oct holding the first entry above v: the
+// entries before it in that oct are ids of L(u) up to v, never in L(v) (whose
+// ids are all above v), and the pads read the bitmap's zero word, so no entry
+// needs a mask.
+
+// Octs of row v's natural-order hot list.
+extern "C" __global__ __launch_bounds__(256) void tc_mid_rows(const uint32_t* __restrict__ split,
+                                                              const uint32_t* __restrict__ hi, uint32_t n,
+                                                              uint32_t* __restrict__ nocts) {
+  for (uint64_t v = global_thread(); v < n; v += grid_threads()) nocts[v] = (hi[v] - split[v] + 7u) >> 3;
+}
+
+// Every hot DAG entry's offset at its place (hq[v] = the row's first oct) and
+// the pads of each row's last oct.
+extern "C" __global__ __launch_bounds__(256) void tc_mid_fill(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, uint32_t n, uint64_t m,
+    uint32_t hot_base, const uint32_t* __restrict__ split, const uint32_t* __restrict__ hq,
+    uint16_t* __restrict__ half) {
+  const uint64_t total = m > n ? m : static_cast<uint64_t>(n);
+  for (uint64_t i = global_thread(); i < total; i += grid_threads()) {
+    if (i < m) {
+      const uint32_t v = row_of_entry(offsets, n, static_cast<uint32_t>(i));
+      const uint32_t sp = split[v];
+      if (i >= sp) half[8ull * hq[v] + (i - sp)] = static_cast<uint16_t>(targets[i] - hot_base);
+    }
+    if (i < n) {
+      const uint32_t v = static_cast<uint32_t>(i);
+      const uint32_t len = offsets[v + 1] - split[v];
+      for (uint32_t r = len; r < ((len + 7u) & ~7u); ++r) half[8ull * hq[v] + r] = static_cast<uint16_t>(TCB_PAD16);
+    }
+  }
+}
+
+// One record per entry i of the hot rows v in [hot_base, n) (rec[i - rec_base]):
+// for an entry u of v's in-list (u < v), the walk {first oct, octs} of the part of
+// L(u) above v when the edge u->v goes to the middle side, else {0, 0}; entries
+// of v's own DAG part and self-loops get {0, 0}. row_octs(v) = hw[v]'s oct count
+// (the offsets' octs and, for a masked list, the mask's): what tc_bitmap16 streams
+// for L(v). An edge whose walk is empty (v the last entry of L(u)) closes no
+// triangle and always goes to the middle side, where it costs nothing.
+extern "C" __global__ __launch_bounds__(256) void tc_mid_records(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
+    uint32_t n, uint32_t hot_base, const uint32_t* __restrict__ split, const uint2* __restrict__ hw,
+    const uint32_t* __restrict__ hq, uint32_t ovh, uint2* __restrict__ rec, uint32_t* __restrict__ midbits) {
+  const uint32_t rec_base = offsets[hot_base];
+  const uint64_t total = static_cast<uint64_t>(offsets[n]) - rec_base;
+  for (uint64_t k = global_thread(); k < total; k += grid_threads()) {
+    const uint32_t i = rec_base + static_cast<uint32_t>(k);
+    uint32_t lo = hot_base, hi = n - 1;  // the row of entry i: the largest v with offsets[v] <= i
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+      if (offsets[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t v = lo;
+    uint2 r = make_uint2(0u, 0u);
+    const uint32_t u = targets[i];
+    if (u < v) {
+      const uint32_t ue = offsets[u + 1];
+      const uint32_t j = lower_bound_u32(targets, fwd[u], ue, v);
+      if (j < ue && targets[j] == v) {
+        const uint32_t p = j - split[u];  // v's place in u's hot list (v >= hot_base)
+        const uint32_t hot_octs = (ue - split[u] + 7u) >> 3;
+        const uint32_t walk = j + 1 == ue ? 0u : hot_octs - ((p + 1) >> 3);
+        const uint32_t row_octs = hw[v].y & ~TC_TOP_FLAG;
+        if (walk == 0 || walk + ovh < row_octs) {
+          r = make_uint2(hq[u] + ((p + 1) >> 3), walk);
+          atomicOr(&midbits[j >> 5], 1u << (j & 31u));
+        }
+      }
+    }
+    rec[k] = r;
+  }
+}
+
+// rowwork[u] = 1 when row u has a DAG entry u->v that tc_bitmap16 still walks
+// (its bit clear in midbits) and at least two DAG entries, else 0.
+extern "C" __global__ __launch_bounds__(256) void tc_mid_rowwork(const uint32_t* __restrict__ lo,
+                                                                 const uint32_t* __restrict__ hi, uint32_t n,
+                                                                 const uint32_t* __restrict__ midbits,
+                                                                 uint8_t* __restrict__ rowwork) {
+  for (uint64_t u = global_thread(); u < n; u += grid_threads()) {
+    const uint32_t b = lo[u], e = hi[u];
+    uint8_t work = 0;
+    if (e - b >= 2) {
+      for (uint32_t j = b; j < e && work == 0; ++j) work = ((midbits[j >> 5] >> (j & 31u)) & 1u) == 0 ? 1 : 0;
+    }
+    rowwork[u] = work;
+  }
+}
+
+// The middle side's count: units {v, first entry, end entry, 0} of the hot rows'
+// in-lists, heaviest first from one work counter; per unit the wave sets L(v)'s
+// bits in its LDS bitmap (L(v) is all hot: v >= hot_base), walks the records'
+// lists (those of at most quarters >> 8 octs a quarter wave each, the others two
+// at a time a wave step each, as tc_bitmap16 walks L(v)), then clears the bits.
+extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) void tc_mid16(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
+    uint32_t hot_base, const uint2* __restrict__ rec, uint32_t rec_base, const uint4* __restrict__ octs,
+    const uint4* __restrict__ units, uint32_t num_units, uint32_t quarters, uint32_t* __restrict__ counter,
+    unsigned long long* __restrict__ result) {
+  __shared__ uint32_t s_bits[TCB_HOT_WORDS + 1];
+  const uint32_t lane = lane_id();
+  for (uint32_t j = lane; j < TCB_HOT_WORDS + 1; j += GC_WAVE) s_bits[j] = 0;
+  wave_lds_sync();
+  unsigned long long count = 0;
+  for (;;) {
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(counter, 1u);
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+    if (k >= num_units) break;
+    const uint4 unit = units[k];
+    const uint32_t v = __builtin_amdgcn_readfirstlane(unit.x);
+    const uint32_t b = __builtin_amdgcn_readfirstlane(unit.y), e = __builtin_amdgcn_readfirstlane(unit.z);
+    const uint32_t lb = fwd[v], le = offsets[v + 1];
+    for (uint32_t j = lb + lane; j < le; j += GC_WAVE) {
+      const uint32_t y = targets[j] - hot_base;
+      atomicOr(&s_bits[y >> 5], 1u << (y & 31u));
+    }
+    wave_lds_sync();
+    uint32_t local = 0;
+    for (uint32_t i0 = b; i0 < e; i0 += GC_WAVE) {
+      const uint32_t i = i0 + lane;
+      const uint2 r = i < e ? rec[i - rec_base] : make_uint2(0u, 0u);
+      const uint32_t st = r.x, len = r.y;
+      unsigned long long pending = __ballot(len != 0);
+      if ((quarters & 0xFFu) != 0) {
+        const unsigned long long small = __ballot(len != 0 && len <= (quarters >> 8));
+        if (small != 0) {
+          switch (quarters & 0xFFu) {
+            case 1: local += probe16_quarters<1>(octs, small, st, len, s_bits); break;
+            case 3: local += probe16_quarters<3>(octs, small, st, len, s_bits); break;
+            case 4: local += probe16_quarters<4>(octs, small, st, len, s_bits); break;
+            default: local += probe16_quarters<2>(octs, small, st, len, s_bits); break;
+          }
+        }
+        pending &= ~small;
+      }
+      while (pending != 0) {
+        const int jv = __ffsll(static_cast<long long>(pending)) - 1;
+        pending &= pending - 1;
+        const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+        const uint32_t l1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
+        if (pending != 0) {
+          const int jv2 = __ffsll(static_cast<long long>(pending)) - 1;
+          pending &= pending - 1;
+          const uint32_t s2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv2));
+          const uint32_t l2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv2));
+          local += probe16_pair(octs + s1, l1, octs + s2, l2, s_bits);
+        } else {
+          local += probe16_list(octs + s1, l1, s_bits);
+        }
+      }
+    }
+    count += local;
+    wave_lds_sync();
+    for (uint32_t j = lb + lane; j < le; j += GC_WAVE) s_bits[(targets[j] - hot_base) >> 5] = 0;
+    wave_lds_sync();
+  }
+  count = wave_sum(count);
+  if (lane == 0 && count != 0) atomicAdd(result, count);
 }
 
 // Triangle count of a symmetric graph on its degree-ordered relabelling

@@ -381,6 +381,18 @@ struct Graph {
   bool tc_hot16_ready = false;
   uint32_t tc_hot16_parts = 0;  // the middle-vertex parts the words are grouped by
   uint32_t tc_hot16_top = 0;    // octs per mask of the top ids in the words (0: no masks)
+  // The middle-vertex side of the 16-bit count (tc_mid16): the DAG rows' hot parts in
+  // natural order, one record per entry of the hot rows, the bits of the DAG entries
+  // it counts, and its work units.
+  DeviceMemory tc_mid_octs;     // 16-byte octs, each row from an oct boundary
+  DeviceMemory tc_mid_rec;      // uint2 per entry of rows [hot_base, n): {first oct, octs}
+  DeviceMemory tc_midbits;      // one bit per CSR entry (set on the DAG entries counted at v)
+  DeviceMemory tc_mid_units;    // uint4 {v, first entry, end entry, 0}
+  DeviceMemory tc_rowwork;      // u8 per row: 1 when tc_bitmap16 still has entries of the row to walk
+  uint32_t tc_mid_num_units = 0;
+  uint32_t tc_mid_rec_base = 0;
+  bool tc_mid_ready = false;
+  uint32_t tc_mid_key[4] = {0, 0, 0, 0};  // parts, top octs, per-edge cost and unit the build is for
   uint32_t pr_merge_push = 0;   // push set size the PageRank row-start bitmap was built for (0: none)
   std::unique_ptr<PrBinPlan> pr_bin;  // PageRank's binned layout (null until the first binned call)
   // The learned triangle-count schedule (method per degree bucket): a function

@@ -84,6 +84,26 @@ def check_tc_partitions(name, g, expected, monkeypatch):
     monkeypatch.setenv("GC_TC_WG", "4")
     assert g.triangle_count()["total_triangles"] == expected, f"{name}: four-wave workgroups"
     monkeypatch.delenv("GC_TC_WG")
+    # the middle-vertex side (tc_mid16) off; taking every edge it may (per-edge cost 0); only the
+    # edges whose walk is empty (a cost no walk beats); in units of 64 entries; with the masks,
+    # parts and quarter walks it shares with tc_bitmap16 changed
+    for mid, cost, unit, top, parts, quarters in (("0", None, None, None, None, None),
+                                                  ("1", "0", None, None, None, None),
+                                                  ("1", "4294967295", None, None, None, None),
+                                                  ("1", "0", "64", None, None, None),
+                                                  ("1", "1", "64", "0", "3", None),
+                                                  ("1", "0", None, "1024", "8", "0"),
+                                                  ("1", "2", "100", "128", "1", "1")):
+        for var, val in (("GC_TC_MID", mid), ("GC_TC_MID_COST", cost), ("GC_TC_MID_UNIT", unit),
+                         ("GC_TC_TOP", top), ("GC_TC_PARTS", parts), ("GC_TC_QUARTERS", quarters)):
+            if val is None:
+                monkeypatch.delenv(var, raising=False)
+            else:
+                monkeypatch.setenv(var, val)
+        assert g.triangle_count()["total_triangles"] == expected, \
+            f"{name}: mid={mid} cost={cost} unit={unit} top={top} parts={parts} quarters={quarters}"
+    for var in ("GC_TC_MID", "GC_TC_MID_COST", "GC_TC_MID_UNIT", "GC_TC_TOP", "GC_TC_PARTS", "GC_TC_QUARTERS"):
+        monkeypatch.delenv(var, raising=False)
 
 
 def check_pagerank_kernels(name, off, tgt, g, monkeypatch):
