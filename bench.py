@@ -48,8 +48,8 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS + HBM GB/s (TC, BFS, PageRank) RMAT-22; learned vs fixed sched"
 GOLDEN = os.path.join(REPO, "tests", "golden", "rmat_counts.json")
 KERNEL_FAMILIES = {
-    "tc": ("tc_bitmap16", "tc_mid16", "tc_bitmap16_w4", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count",
-           "tc_mid_rows", "tc_mid_fill", "tc_mid_records", "tc_mid_rowwork"),
+    "tc": ("tc_bitmap16", "tc_mid16", "tc_mid32", "tc_bitmap16_w4", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count",
+           "tc_mid_rows", "tc_mid_fill", "tc_mid_records", "tc_mid32_records", "tc_mid_rowwork"),
     "pagerank": ("pr_spmv_slice", "pr_slice_finish", "pr_spmv_bin", "pr_bin_scatter", "pr_bin_accum",
                  "pr_spmv_merge", "pr_merge_finish",
                  "pr_spmv_push", "pr_push_finish", "pr_spmv_hot", "pr_spmv",
@@ -69,7 +69,7 @@ KERNEL_FAMILIES = {
 KERNELS = tuple(k for ks in KERNEL_FAMILIES.values() for k in ks)
 # The kernels timed with events in a default run: those the rooflines report. Each timed launch
 # adds two event records to the stream (about 2 ms per step if every launch were timed).
-ROOFLINE_KERNELS = ("tc_bitmap16", "tc_mid16", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_slice", "pr_slice_finish",
+ROOFLINE_KERNELS = ("tc_bitmap16", "tc_mid16", "tc_mid32", "tc_bitmap", "tc_hash", "tc_wedge", "tc_count", "pr_spmv_slice", "pr_slice_finish",
                     "pr_spmv_bin", "pr_merge_finish", "pr_bin_scatter", "pr_bin_accum", "pr_spmv_merge",
                     "pr_spmv_push", "pr_spmv_hot", "pr_spmv")
 # The binned SpMV (the default on symmetric graphs), four kernels per iteration: the scatter and
@@ -243,14 +243,26 @@ def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int | None = None
         # octs from the one holding the entry after v to the end of u's hot part
         walk = np.where(last, 0, (hot[row] + 7) // 8 - (k - cold[row] + 1) // 8)
         is_mid = (dag_v >= hot_base) & (last | (walk + cost < octs[dag_v]))
-        del k, last
         walk_octs = int(walk[is_mid].sum())
         del walk
-        mid_edges = int(is_mid.sum())
         units = np.zeros(n, np.int64)
         deg = np.diff(off)
         units[hot_base:] = (deg[hot_base:] + unit - 1) // unit
-        mid_bytes = 16 * walk_octs + 8 * int(off[n] - off[hot_base]) + int((8 * dplus * units).sum())
+        mid_bytes = 16 * walk_octs + 8 * int(off[n] - off[hot_base])
+        if os.environ.get("GC_TC_MID32", "0") != "0":
+            # the 32-bit tier (tc_mid32): v in [mid_base, hot_base), walks of 32-bit entries
+            ids = max(1, tc_env("GC_TC_MID32_IDS", 131072))
+            mid_base = n - ids if n > ids else 0
+            suffix = dplus[row] - k - 1
+            band = (dag_v >= mid_base) & (dag_v < hot_base)
+            is32 = band & (last | ((suffix + 3) // 4 + cost < octs[dag_v] + (cold[dag_v] + 3) // 4))
+            mid_bytes += 4 * int(suffix[is32].sum()) + 8 * int(off[hot_base] - off[mid_base])
+            units[mid_base:hot_base] = (deg[mid_base:hot_base] + unit - 1) // unit
+            is_mid |= is32
+            del suffix, band, is32
+        del k, last
+        mid_edges = int(is_mid.sum())
+        mid_bytes += int((8 * dplus * units).sum())
         row_side = ~is_mid
         del is_mid
     else:
@@ -478,8 +490,9 @@ def main() -> int:
                f"{tc16['top_bits'] // 8} bytes where that is smaller: {tc16['masked_rows']} lists)")
     if tc16["mid"]:
         tc_note += (f"; the {tc16['mid_edges']} of {tc16['dag_edges']} DAG edges u->v counted by tc_mid16 at v: "
-                    "the octs of L(u) above v (16-byte octs, natural order), an 8-byte record per entry of the "
-                    "hot rows, L(v) at 4 bytes twice per work unit, and one bit per DAG entry")
+                    "the octs of L(u) above v (16-byte octs, natural order; 4-byte entries in the 32-bit tier "
+                    "when it is on), an 8-byte record per entry of the rows it serves, L(v) at 4 bytes twice "
+                    "per work unit, and one bit per DAG entry")
     tc_note += f"; the merge model (both lists at 4 bytes, {tc_merge_bytes} bytes) would count data never read"
     for name in ("tc_bitmap16", "tc_bitmap"):
         if name in kernels:
@@ -489,14 +502,15 @@ def main() -> int:
             rooflines[name] = roofline_entry(model, kernels[name]["avg_us"], traffic.get(name), tc_note)
             rooflines[name]["merge_model_bytes"] = tc_merge_bytes
     if "tc_mid16" in kernels and "tc_bitmap16" in kernels:
-        rooflines["tc_mid16"] = roofline_entry(tc16["mid_side_bytes"], kernels["tc_mid16"]["avg_us"],
-                                               traffic.get("tc_mid16"), tc_note)
-        # the count as one unit: both kernels, once each per count
-        t_pair = [traffic.get("tc_bitmap16"), traffic.get("tc_mid16")]
-        rooflines["tc_bitmap16+tc_mid16"] = roofline_entry(
-            tc16["bytes"], kernels["tc_bitmap16"]["avg_us"] + kernels["tc_mid16"]["avg_us"],
-            sum(t_pair) if all(t_pair) else None, tc_note)
-        rooflines["tc_bitmap16+tc_mid16"]["merge_model_bytes"] = tc_merge_bytes
+        mids = [k for k in ("tc_mid16", "tc_mid32") if k in kernels]
+        mid_us = sum(kernels[k]["avg_us"] for k in mids)
+        rooflines["+".join(mids)] = roofline_entry(tc16["mid_side_bytes"], mid_us, None, tc_note)
+        # the count as one unit: every kernel of it, once each per count
+        pair = ["tc_bitmap16"] + mids
+        t_pair = [traffic.get(k) for k in pair]
+        rooflines["+".join(pair)] = roofline_entry(
+            tc16["bytes"], sum(kernels[k]["avg_us"] for k in pair), sum(t_pair) if all(t_pair) else None, tc_note)
+        rooflines["+".join(pair)]["merge_model_bytes"] = tc_merge_bytes
     if "tc_wedge" in kernels:
         rooflines["tc_wedge"] = roofline_entry(wedge_bytes, kernels["tc_wedge"]["avg_us"], traffic.get("tc_wedge"))
     # BFS over the whole call per source (every level's kernels, launch gaps and the read-back of
@@ -509,7 +523,8 @@ def main() -> int:
     # count's two kernels, which split its edges between them, as one).
     units = {k: v["total_ms"] for k, v in kernels.items()}
     if "tc_bitmap16" in units and "tc_mid16" in units:
-        units["tc_bitmap16+tc_mid16"] = units.pop("tc_bitmap16") + units.pop("tc_mid16")
+        pair = ["tc_bitmap16"] + [k for k in ("tc_mid16", "tc_mid32") if k in units]
+        units["+".join(pair)] = sum(units.pop(k) for k in pair)
     dominant = max(units, key=units.get) if units else None
     roofline = dict(rooflines.get(dominant) or {})
     if roofline:

@@ -50,9 +50,10 @@ uint32_t tc_parts() {
 // Rows handed out one at a time by tc_bitmap: the leading rows of [row_lo,
 // row_hi) in descending order whose degree is at least GC_TC_HEAVY_DEGREE
 // (default 64), at most 65536; the rest go in chunks of 64.
-uint32_t tc_heavy_rows(const Graph& g, uint32_t row_lo, uint32_t row_hi) {
+uint32_t tc_heavy_rows(const Graph& g, uint32_t row_lo, uint32_t row_hi, bool mid = false) {
   const char* env = std::getenv("GC_TC_HEAVY_DEGREE");
-  const uint32_t degree = env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : 64u;
+  // with the middle side on, 1024 (R-MAT scale 22: 5.10 ms against 5.28 at 64, 256 or 16)
+  const uint32_t degree = env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : (mid ? 1024u : 64u);
   const std::vector<uint32_t>& off = g.rl.host_offsets;
   // rows are sorted by degree: the first row of degree >= `degree`
   uint32_t a = row_lo, b = row_hi;
@@ -181,9 +182,34 @@ bool tc_mid() {
   return env == nullptr || std::strcmp(env, "0") != 0;
 }
 
+// The 32-bit middle tier (tc_mid32: middle vertices among the top
+// GC_TC_MID32_IDS ids, default 131072, but below the hot ids) with
+// GC_TC_MID32=1. Off by default: at R-MAT scale 22 it takes 7.5 M edges (1.8e9
+// streamed entries) from tc_bitmap16, which then runs 1.0 ms faster, but its
+// own walks take 1.39 ms (short walks a quarter wave each, a global round trip
+// per step, and the searches of L(v)'s cold prefix): 5.84 against 5.29 ms.
+bool tc_mid32() {
+  const char* env = std::getenv("GC_TC_MID32");
+  return env != nullptr && std::strcmp(env, "0") != 0;
+}
+
+uint32_t tc_mid32_ids() {
+  const char* env = std::getenv("GC_TC_MID32_IDS");
+  return env == nullptr ? 131072u : static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
+}
+
 uint32_t tc_mid_cost() {
   const char* env = std::getenv("GC_TC_MID_COST");
   return env == nullptr ? 2u : static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
+}
+
+// Work units a wave of tc_mid16 / tc_mid32 takes per atomic on their shared
+// counter: GC_TC_MID_GRAB (default 1). The units come heaviest first, so a wave
+// that takes several at once holds the tail: at R-MAT scale 22, 1, 8 and 32
+// units per atomic took 5.29, 8.2 and 24.4 ms.
+uint32_t tc_mid_grab() {
+  const char* env = std::getenv("GC_TC_MID_GRAB");
+  return env == nullptr ? 1u : std::max(1u, static_cast<uint32_t>(std::strtoul(env, nullptr, 10)));
 }
 
 uint32_t tc_mid_unit() {
@@ -199,8 +225,9 @@ uint32_t tc_mid_unit() {
 // 32768 rows, one bit per CSR entry.
 void ensure_mid(Graph& g, uint32_t hot_base, uint32_t parts, uint32_t top_octs, uint32_t cost) {
   const uint32_t unit = tc_mid_unit();
+  const uint32_t tier32 = tc_mid32() ? std::max(1u, tc_mid32_ids()) : 0u;
   if (g.tc_mid_ready && g.tc_mid_key[0] == parts && g.tc_mid_key[1] == top_octs && g.tc_mid_key[2] == cost &&
-      g.tc_mid_key[3] == unit)
+      g.tc_mid_key[3] == unit && g.tc_mid_key[4] == tier32)
     return;
   g.tc_mid_ready = false;
   Device& dev = *g.dev;
@@ -231,12 +258,46 @@ void ensure_mid(Graph& g, uint32_t hot_base, uint32_t parts, uint32_t top_octs, 
   const uint64_t bit_words = (m + 31) / 32 + 1;
   g.tc_midbits.ensure(sizeof(uint32_t) * bit_words);
   fill_words(dev, g.tc_midbits.as<uint32_t>(), bit_words, 0);
+  // the rows (of the hot ones, then of the 32-bit tier's) with a non-empty walk: only they get units
+  DeviceMemory walks;
+  std::vector<uint8_t> has_walk(n - hot_base, 0);
+  const uint64_t walk_words = (std::max<uint64_t>(n, 1) + 3) / 4;
+  walks.allocate(4 * walk_words);
+  fill_words(dev, walks.as<uint32_t>(), walk_words, 0);
   if (recs > 0) {
     Grid1D rg = thread_per_item(recs);
     launch(dev, get_kernel(dev, "tc", "tc_mid_records"), rg.blocks, rg.threads, 0, offsets, tgt,
            g.rl.fwd.as<const uint32_t>(), n, hot_base, g.tc_split.as<const uint32_t>(),
            g.tc_hw.as<const uint32_t>() /* uint2[n] */, static_cast<const uint32_t*>(hq.as<uint32_t>()), cost,
-           g.tc_mid_rec.as<uint32_t>() /* uint2 */, g.tc_midbits.as<uint32_t>());
+           g.tc_mid_rec.as<uint32_t>() /* uint2 */, g.tc_midbits.as<uint32_t>(), walks.as<uint8_t>());
+    copy_to_host(dev, has_walk.data(), walks.as<uint8_t>(), has_walk.size());
+  }
+  // the 32-bit tier: rows [mid_base, hot_base), when the graph has ids below the hot ones
+  const uint32_t mid_base = n > tier32 ? n - tier32 : 0u;
+  g.tc_mid32_num_units = 0;
+  g.tc_mid32_base = mid_base;
+  g.tc_mid32_rec_base = off[mid_base];
+  std::vector<uint32_t> units32;
+  if (tier32 && mid_base < hot_base && off[hot_base] > off[mid_base]) {
+    const uint64_t recs32 = off[hot_base] - off[mid_base];
+    g.tc_mid32_rec.ensure(sizeof(uint32_t) * 2 * recs32);
+    Grid1D rg = thread_per_item(recs32);
+    fill_words(dev, walks.as<uint32_t>(), walk_words, 0);
+    launch(dev, get_kernel(dev, "tc", "tc_mid32_records"), rg.blocks, rg.threads, 0, offsets, tgt,
+           g.rl.fwd.as<const uint32_t>(), mid_base, hot_base, g.tc_split.as<const uint32_t>(),
+           g.tc_hw.as<const uint32_t>() /* uint2[n] */, cost, g.tc_mid32_rec.as<uint32_t>() /* uint2 */,
+           g.tc_midbits.as<uint32_t>(), walks.as<uint8_t>());
+    std::vector<uint8_t> has_walk32(hot_base - mid_base, 0);
+    copy_to_host(dev, has_walk32.data(), walks.as<uint8_t>(), has_walk32.size());
+    for (uint32_t v = hot_base; v-- > mid_base;) {
+      if (!has_walk32[v - mid_base]) continue;
+      for (uint32_t b = off[v]; b < off[v + 1]; b += unit) {
+        units32.insert(units32.end(), {v, b, std::min<uint32_t>(off[v + 1], b + unit), 0u});
+      }
+    }
+    g.tc_mid32_num_units = static_cast<uint32_t>(units32.size() / 4);
+    g.tc_mid32_units.ensure(sizeof(uint32_t) * std::max<size_t>(units32.size(), 4));
+    copy_to_device(dev, g.tc_mid32_units.as<uint32_t>(), units32.data(), sizeof(uint32_t) * units32.size());
   }
   g.tc_rowwork.ensure(rows);
   launch(dev, get_kernel(dev, "tc", "tc_mid_rowwork"), vg.blocks, vg.threads, 0, g.rl.fwd.as<const uint32_t>(),
@@ -244,6 +305,7 @@ void ensure_mid(Graph& g, uint32_t hot_base, uint32_t parts, uint32_t top_octs, 
   // units: the hot rows' entry ranges in pieces of at most tc_mid_unit() entries, top row first
   std::vector<uint32_t> units;
   for (uint32_t v = n; v-- > hot_base;) {
+    if (!has_walk[v - hot_base]) continue;
     for (uint32_t b = off[v]; b < off[v + 1]; b += unit) {
       units.insert(units.end(), {v, b, std::min<uint32_t>(off[v + 1], b + unit), 0u});
     }
@@ -257,6 +319,7 @@ void ensure_mid(Graph& g, uint32_t hot_base, uint32_t parts, uint32_t top_octs, 
   g.tc_mid_key[1] = top_octs;
   g.tc_mid_key[2] = cost;
   g.tc_mid_key[3] = unit;
+  g.tc_mid_key[4] = tier32;
   g.tc_mid_ready = true;
 }
 
@@ -320,9 +383,9 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
   } else {
     // hot range [hot_base, n): at most kHotBits ids, hot_base a multiple of 32
     const uint32_t hot_base = n > kHotBits ? ((n - kHotBits) + 31u) & ~31u : 0u;
-    const uint32_t single = tc_heavy_rows(g, row_lo, row_hi), parts = tc_parts();
-    const uint64_t tasks =
-        parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
+    const uint32_t parts = tc_parts();
+    uint32_t single = tc_heavy_rows(g, row_lo, row_hi);
+    uint64_t tasks = parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
     // workgroups of `waves` waves, as many waves in all as grid_for's four-wave workgroups
     auto go = [&](const char* name, uint32_t waves, auto... args) {
       const Kernel k = get_kernel(dev, "tc", name);
@@ -339,10 +402,17 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
       const bool w1 = tc_wg_waves() == 1;
       // the middle side takes its edges when the launch covers every row that can close a
       // triangle (the rows below the first of degree 2 have at most one DAG entry)
-      const bool mid = !no_alloc && tc_mid() && row_hi == n && row_lo <= first_row_of_degree(g, 2);
+      bool mid = !no_alloc && tc_mid() && row_hi == n && row_lo <= first_row_of_degree(g, 2);
       if (mid) ensure_mid(g, hot_base, parts, top_octs, tc_mid_cost());
+      // no walk on either middle kernel (e.g. a grid, where every L(v) is short): the edge bits would
+      // only skip edges that close no triangle, at a load per edge (6.36 against 6.02 ms, 4096 x 4096 grid)
+      mid = mid && g.tc_mid_num_units + g.tc_mid32_num_units > 0;
       const uint32_t* midbits = mid ? g.tc_midbits.as<const uint32_t>() : nullptr;
       const uint8_t* rowwork = mid ? g.tc_rowwork.as<const uint8_t>() : nullptr;
+      if (mid) {
+        single = tc_heavy_rows(g, row_lo, row_hi, true);
+        tasks = parts * (single + (static_cast<uint64_t>(rows - single) + light_chunk - 1) / light_chunk);
+      }
       go(w1 ? "tc_bitmap16" : "tc_bitmap16_w4", w1 ? 1u : 4u, fwd, end, tgt, g.tc_split.as<const uint32_t>(),
          g.tc_hw.as<const uint32_t>() /* uint2[n] */, g.tc_words.as<const uint32_t>() /* uint4 octs */, row_hi,
          row_lo, hot_base, single, light_chunk, parts, tc_split_min(mid), counters, result, top_word0, top_octs,
@@ -353,7 +423,15 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
         launch(dev, get_kernel(dev, "tc", "tc_mid16"), blocks, kWaveSize, 0, g.rl.offsets.as<const uint32_t>(), tgt,
                fwd, hot_base, g.tc_mid_rec.as<const uint32_t>() /* uint2 */, g.tc_mid_rec_base,
                g.tc_mid_octs.as<const uint32_t>() /* uint4 */, g.tc_mid_units.as<const uint32_t>() /* uint4 */,
-               g.tc_mid_num_units, tc_quarters(), counters + 2, result);
+               g.tc_mid_num_units, tc_quarters(), tc_mid_grab(), counters + 2, result);
+      }
+      if (mid && g.tc_mid32_num_units > 0) {
+        // one-wave workgroups, as tc_mid16's, on work counter word 3
+        const uint32_t blocks = std::min<uint32_t>(grid_for(g.tc_mid32_num_units * 4ull) * 4u, g.tc_mid32_num_units);
+        launch(dev, get_kernel(dev, "tc", "tc_mid32"), blocks, kWaveSize, 0, g.rl.offsets.as<const uint32_t>(), tgt,
+               fwd, g.tc_split.as<const uint32_t>(), hot_base, g.tc_mid32_rec.as<const uint32_t>() /* uint2 */,
+               g.tc_mid32_rec_base, g.tc_mid32_units.as<const uint32_t>() /* uint4 */, g.tc_mid32_num_units,
+               tc_mid_grab(), counters + 3, result);
       }
     } else {
       go("tc_bitmap", 4u, fwd, end, tgt, row_hi, row_lo, hot_base, single, light_chunk, parts, tc_split_min(),

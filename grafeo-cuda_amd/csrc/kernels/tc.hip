@@ -1000,10 +1000,13 @@ extern "C" __global__ __launch_bounds__(256) void tc_mid_fill(
 // (the offsets' octs and, for a masked list, the mask's): what tc_bitmap16 streams
 // for L(v). An edge whose walk is empty (v the last entry of L(u)) closes no
 // triangle and always goes to the middle side, where it costs nothing.
+// walks[v - hot_base] = 1 marks the rows with a non-empty walk (zeroed first):
+// only they get work units.
 extern "C" __global__ __launch_bounds__(256) void tc_mid_records(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
     uint32_t n, uint32_t hot_base, const uint32_t* __restrict__ split, const uint2* __restrict__ hw,
-    const uint32_t* __restrict__ hq, uint32_t ovh, uint2* __restrict__ rec, uint32_t* __restrict__ midbits) {
+    const uint32_t* __restrict__ hq, uint32_t ovh, uint2* __restrict__ rec, uint32_t* __restrict__ midbits,
+    uint8_t* __restrict__ walks) {
   const uint32_t rec_base = offsets[hot_base];
   const uint64_t total = static_cast<uint64_t>(offsets[n]) - rec_base;
   for (uint64_t k = global_thread(); k < total; k += grid_threads()) {
@@ -1028,6 +1031,7 @@ extern "C" __global__ __launch_bounds__(256) void tc_mid_records(
         if (walk == 0 || walk + ovh < row_octs) {
           r = make_uint2(hq[u] + ((p + 1) >> 3), walk);
           atomicOr(&midbits[j >> 5], 1u << (j & 31u));
+          if (walk != 0) walks[v - hot_base] = 1;  // the row has a walk: it needs a unit
         }
       }
     }
@@ -1052,24 +1056,29 @@ extern "C" __global__ __launch_bounds__(256) void tc_mid_rowwork(const uint32_t*
 }
 
 // The middle side's count: units {v, first entry, end entry, 0} of the hot rows'
-// in-lists, heaviest first from one work counter; per unit the wave sets L(v)'s
+// in-lists, heaviest first from one work counter, `grab` units per atomic (the
+// atomics of all waves on one counter serialize across the XCDs); per unit the wave sets L(v)'s
 // bits in its LDS bitmap (L(v) is all hot: v >= hot_base), walks the records'
 // lists (those of at most quarters >> 8 octs a quarter wave each, the others two
 // at a time a wave step each, as tc_bitmap16 walks L(v)), then clears the bits.
 extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) void tc_mid16(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
     uint32_t hot_base, const uint2* __restrict__ rec, uint32_t rec_base, const uint4* __restrict__ octs,
-    const uint4* __restrict__ units, uint32_t num_units, uint32_t quarters, uint32_t* __restrict__ counter,
-    unsigned long long* __restrict__ result) {
+    const uint4* __restrict__ units, uint32_t num_units, uint32_t quarters, uint32_t grab,
+    uint32_t* __restrict__ counter, unsigned long long* __restrict__ result) {
   __shared__ uint32_t s_bits[TCB_HOT_WORDS + 1];
   const uint32_t lane = lane_id();
   for (uint32_t j = lane; j < TCB_HOT_WORDS + 1; j += GC_WAVE) s_bits[j] = 0;
   wave_lds_sync();
   unsigned long long count = 0;
-  for (;;) {
-    uint32_t k = 0;
-    if (lane == 0) k = atomicAdd(counter, 1u);
-    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+  uint32_t k = 0, k_end = 0;
+  for (;; ++k) {
+    if (k >= k_end) {  // the next `grab` units, one atomic on the shared counter for them all
+      uint32_t g0 = 0;
+      if (lane == 0) g0 = atomicAdd(counter, grab);
+      k = __builtin_amdgcn_readfirstlane(__shfl(g0, 0));
+      k_end = min(k + grab, num_units);
+    }
     if (k >= num_units) break;
     const uint4 unit = units[k];
     const uint32_t v = __builtin_amdgcn_readfirstlane(unit.x);
@@ -1117,6 +1126,152 @@ extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per
     count += local;
     wave_lds_sync();
     for (uint32_t j = lb + lane; j < le; j += GC_WAVE) s_bits[(targets[j] - hot_base) >> 5] = 0;
+    wave_lds_sync();
+  }
+  count = wave_sum(count);
+  if (lane == 0 && count != 0) atomicAdd(result, count);
+}
+
+// ---- the middle side below the hot ids (tc_mid32) ----
+//
+// The edges u->v whose v lies just below the hot ids, in [mid_base, hot_base)
+// (the host's choice of mid_base: by default the top 131072 ids), go to the middle side by the
+// same rule, with the costs in 16-byte units: the walk at 4 bytes an entry
+// against L(v)'s hot octs plus its cold entries at 4 bytes. Their walks are the
+// 32-bit entries of L(u) above v, read in place from targets. L(v) is held as
+// tc_bitmap16 holds L(u): its hot part in the wave's LDS bitmap, its cold
+// prefix (ids in (v, hot_base)) behind a Bloom filter and searched in place.
+// Records of the entries of rows v in [mid_base, hot_base) (rec[i - rec_base]):
+// {first entry, entries} of the walk of L(u) above v, or {0, 0}; sets midbits,
+// and walks[v - mid_base] = 1 for the rows with a non-empty walk.
+extern "C" __global__ __launch_bounds__(256) void tc_mid32_records(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
+    uint32_t mid_base, uint32_t hot_base, const uint32_t* __restrict__ split, const uint2* __restrict__ hw,
+    uint32_t ovh, uint2* __restrict__ rec, uint32_t* __restrict__ midbits, uint8_t* __restrict__ walks) {
+  const uint32_t rec_base = offsets[mid_base];
+  const uint64_t total = static_cast<uint64_t>(offsets[hot_base]) - rec_base;
+  for (uint64_t k = global_thread(); k < total; k += grid_threads()) {
+    const uint32_t i = rec_base + static_cast<uint32_t>(k);
+    uint32_t lo = mid_base, hi = hot_base - 1;  // the row of entry i
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+      if (offsets[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t v = lo;
+    uint2 r = make_uint2(0u, 0u);
+    const uint32_t u = targets[i];
+    if (u < v) {
+      const uint32_t ue = offsets[u + 1];
+      const uint32_t j = lower_bound_u32(targets, fwd[u], ue, v);
+      if (j < ue && targets[j] == v) {
+        const uint32_t walk = ue - j - 1;
+        const uint32_t row_cost = (hw[v].y & ~TC_TOP_FLAG) + ((split[v] - fwd[v] + 3u) >> 2);
+        if (walk == 0 || ((walk + 3u) >> 2) + ovh < row_cost) {
+          r = make_uint2(j + 1, walk);
+          atomicOr(&midbits[j >> 5], 1u << (j & 31u));
+          if (walk != 0) walks[v - mid_base] = 1;
+        }
+      }
+    }
+    rec[k] = r;
+  }
+}
+
+// Hits of the walks T[st, st + len) of the lanes in `lists`, a quarter wave
+// per walk, four entries per lane and step (the quarter's lanes on consecutive
+// entries; a quarter whose walk has ended takes the next one), each entry
+// tested as tc_bitmap tests one (entry_hit: the clamped bitmap lookup, and
+// for a cold id, when L(v) has a cold prefix A[0, c), the filter and search).
+__device__ __forceinline__ uint32_t probe32_quarters(const uint32_t* __restrict__ T, unsigned long long lists,
+                                                    uint32_t st, uint32_t len, const uint32_t* s_bits,
+                                                    uint32_t hot_base, const uint32_t* A, uint32_t c) {
+  const uint32_t lane = lane_id();
+  const uint32_t quarter = lane >> 4, ql = lane & 15u;
+  uint32_t cur = 0, end = 0;
+  uint32_t local = 0;
+  for (;;) {
+    const unsigned long long idle = __ballot(cur >= end);
+    uint32_t free4 = static_cast<uint32_t>((idle & 1ull) | ((idle >> 15) & 2ull) | ((idle >> 30) & 4ull) |
+                                           ((idle >> 45) & 8ull));
+    while (free4 != 0 && lists != 0) {
+      const uint32_t fq = static_cast<uint32_t>(__ffs(free4) - 1);
+      free4 &= free4 - 1;
+      const int jv = __ffsll(static_cast<long long>(lists)) - 1;
+      lists &= lists - 1;
+      const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+      const uint32_t l1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
+      if (quarter == fq) {
+        cur = s1;
+        end = s1 + l1;
+      }
+    }
+    if (__ballot(cur < end) == 0) break;
+    uint32_t w[4];
+    bool act[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const uint32_t i = cur + 16u * d + ql;
+      act[d] = i < end;
+      w[d] = T[act[d] ? i : (end > 0 ? end - 1 : 0u)];
+    }
+#pragma unroll
+    for (int d = 0; d < 4; ++d) local += act[d] ? entry_hit(s_bits, w[d], hot_base, A, c) : 0u;
+    cur += 64u;
+  }
+  return local;
+}
+
+// The count of the edges given to this side: units {v, first entry, end entry,
+// 0} of the rows [mid_base, hot_base) from one work counter, `grab` units per
+// atomic; per unit the wave
+// sets L(v)'s bits (hot part) and filter bits (cold prefix), walks the unit's
+// records 64 at a time, then clears both.
+extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) void tc_mid32(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
+    const uint32_t* __restrict__ split, uint32_t hot_base, const uint2* __restrict__ rec, uint32_t rec_base,
+    const uint4* __restrict__ units, uint32_t num_units, uint32_t grab, uint32_t* __restrict__ counter,
+    unsigned long long* __restrict__ result) {
+  __shared__ uint32_t s_bits[TCB_BLOOM_BASE + TCB_BLOOM_WORDS];
+  const uint32_t lane = lane_id();
+  for (uint32_t j = lane; j < TCB_BLOOM_BASE + TCB_BLOOM_WORDS; j += GC_WAVE) s_bits[j] = 0;
+  wave_lds_sync();
+  unsigned long long count = 0;
+  uint32_t k = 0, k_end = 0;
+  for (;; ++k) {
+    if (k >= k_end) {  // the next `grab` units, one atomic on the shared counter for them all
+      uint32_t g0 = 0;
+      if (lane == 0) g0 = atomicAdd(counter, grab);
+      k = __builtin_amdgcn_readfirstlane(__shfl(g0, 0));
+      k_end = min(k + grab, num_units);
+    }
+    if (k >= num_units) break;
+    const uint4 unit = units[k];
+    const uint32_t v = __builtin_amdgcn_readfirstlane(unit.x);
+    const uint32_t b = __builtin_amdgcn_readfirstlane(unit.y), e = __builtin_amdgcn_readfirstlane(unit.z);
+    const uint32_t lb = fwd[v], le = offsets[v + 1];
+    const uint32_t c = __builtin_amdgcn_readfirstlane(split[v] - lb);  // L(v)'s cold prefix A[0, c)
+    const uint32_t* A = targets + lb;
+    for (uint32_t j = lb + c + lane; j < le; j += GC_WAVE) {
+      const uint32_t y = targets[j] - hot_base;
+      atomicOr(&s_bits[y >> 5], 1u << (y & 31u));
+    }
+    for (uint32_t j = lane; j < c; j += GC_WAVE) {
+      const uint32_t h = bloom_hash(A[j]);
+      atomicOr(&s_bits[TCB_BLOOM_BASE + (h >> 5)], 1u << (h & 31u));
+    }
+    wave_lds_sync();
+    uint32_t local = 0;
+    for (uint32_t i0 = b; i0 < e; i0 += GC_WAVE) {
+      const uint32_t i = i0 + lane;
+      const uint2 r = i < e ? rec[i - rec_base] : make_uint2(0u, 0u);
+      const unsigned long long lists = __ballot(r.y != 0);
+      if (lists != 0) local += probe32_quarters(targets, lists, r.x, r.y, s_bits, hot_base, A, c);
+    }
+    count += local;
+    wave_lds_sync();
+    for (uint32_t j = lb + c + lane; j < le; j += GC_WAVE) s_bits[(targets[j] - hot_base) >> 5] = 0;
+    for (uint32_t j = lane; j < c; j += GC_WAVE) s_bits[TCB_BLOOM_BASE + (bloom_hash(A[j]) >> 5)] = 0;
     wave_lds_sync();
   }
   count = wave_sum(count);

@@ -389,10 +389,15 @@ struct Graph {
   DeviceMemory tc_midbits;      // one bit per CSR entry (set on the DAG entries counted at v)
   DeviceMemory tc_mid_units;    // uint4 {v, first entry, end entry, 0}
   DeviceMemory tc_rowwork;      // u8 per row: 1 when tc_bitmap16 still has entries of the row to walk
+  DeviceMemory tc_mid32_rec;    // uint2 per entry of rows [mid_base, hot_base): {first entry, entries}
+  DeviceMemory tc_mid32_units;  // uint4 {v, first entry, end entry, 0}
+  uint32_t tc_mid32_num_units = 0;
+  uint32_t tc_mid32_rec_base = 0;
+  uint32_t tc_mid32_base = 0;
   uint32_t tc_mid_num_units = 0;
   uint32_t tc_mid_rec_base = 0;
   bool tc_mid_ready = false;
-  uint32_t tc_mid_key[4] = {0, 0, 0, 0};  // parts, top octs, per-edge cost and unit the build is for
+  uint32_t tc_mid_key[5] = {0, 0, 0, 0, 0};  // parts, top octs, per-edge cost, unit, 32-bit tier on
   uint32_t pr_merge_push = 0;   // push set size the PageRank row-start bitmap was built for (0: none)
   std::unique_ptr<PrBinPlan> pr_bin;  // PageRank's binned layout (null until the first binned call)
   // The learned triangle-count schedule (method per degree bucket): a function

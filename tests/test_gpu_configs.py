@@ -69,13 +69,18 @@ def test_rmat20_suite(device, golden, monkeypatch):
     monkeypatch.delenv("GC_TC_HOT16")
     monkeypatch.delenv("GC_TC_TOP")
     monkeypatch.delenv("GC_TC_PARTS")
-    # the middle-vertex side off, at per-edge costs 0, 2 (default) and 16 octs, in units of 64 entries
-    for mid, cost, unit in (("0", "2", "2048"), ("1", "0", "2048"), ("1", "16", "2048"), ("1", "2", "64")):
+    # the middle-vertex sides off (both, or the 32-bit tier), at per-edge costs 0, 2 (default) and 16
+    # octs, in units of 64 entries
+    for mid, mid32, cost, unit in (("0", "1", "2", "2048"), ("1", "0", "0", "2048"), ("1", "1", "0", "2048"),
+                                   ("1", "1", "16", "2048"), ("1", "1", "2", "64")):
         monkeypatch.setenv("GC_TC_MID", mid)
+        monkeypatch.setenv("GC_TC_MID32", mid32)
         monkeypatch.setenv("GC_TC_MID_COST", cost)
         monkeypatch.setenv("GC_TC_MID_UNIT", unit)
-        assert g.triangle_count()["total_triangles"] == e["triangles"], f"mid={mid} cost={cost} unit={unit}"
+        assert g.triangle_count()["total_triangles"] == e["triangles"], \
+            f"mid={mid} mid32={mid32} cost={cost} unit={unit}"
     monkeypatch.delenv("GC_TC_MID")
+    monkeypatch.delenv("GC_TC_MID32")
     monkeypatch.delenv("GC_TC_MID_COST")
     monkeypatch.delenv("GC_TC_MID_UNIT")
 

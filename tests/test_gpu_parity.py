@@ -307,6 +307,53 @@ def test_rmat_small(device, monkeypatch, scale):
     check_msbfs(f"RMAT-{scale}", off, tgt, g, [int(x) for x in np.random.default_rng(scale).integers(0, n, size=128)])
 
 
+@pytest.mark.parametrize("scale", [16, 18])
+def test_rmat_tc_sides(device, monkeypatch, scale):
+    """The triangle count's three sides on graphs with ids below the top 32768 (R-MAT 16: the
+    16-bit middle side over the top half, the 32-bit tier over the rest) and below the top 131072
+    (R-MAT 18: rows on no middle side too): each side off, every edge it may take (per-edge cost
+    0), only the empty walks, small work units, and the parts and masks changed."""
+    off, tgt = rmat_graph(scale, 16, seed=5)
+    expected = oracle.triangle_count(off, tgt, fast=True)
+    g = upload(device, off, tgt)
+    assert g.triangle_count()["total_triangles"] == expected, f"RMAT-{scale}: default"
+    names = ("GC_TC_MID", "GC_TC_MID32", "GC_TC_MID_COST", "GC_TC_MID_UNIT", "GC_TC_PARTS", "GC_TC_TOP")
+    for case in (("0", None, None, None, None, None), ("1", "0", None, None, None, None),
+                 ("1", "1", None, None, None, None),
+                 ("1", "1", "0", None, None, None), ("1", "1", "4294967295", None, None, None),
+                 ("1", "1", "0", "64", "3", None), ("1", "0", "1", "100", "8", "0"),
+                 ("1", "1", "2", "64", "1", "1024")):
+        for var, val in zip(names, case):
+            if val is None:
+                monkeypatch.delenv(var, raising=False)
+            else:
+                monkeypatch.setenv(var, val)
+        assert g.triangle_count()["total_triangles"] == expected, f"RMAT-{scale}: {dict(zip(names, case))}"
+    for var in names:
+        monkeypatch.delenv(var, raising=False)
+    # the 32-bit tier over every row below the hot ids, and over a band narrower than the default
+    monkeypatch.setenv("GC_TC_MID32", "1")
+    for ids in ("4294967295", "40000"):
+        for cost in ("0", "2"):
+            monkeypatch.setenv("GC_TC_MID32_IDS", ids)
+            monkeypatch.setenv("GC_TC_MID_COST", cost)
+            assert g.triangle_count()["total_triangles"] == expected, f"RMAT-{scale}: ids={ids} cost={cost}"
+    monkeypatch.delenv("GC_TC_MID32_IDS")
+    monkeypatch.delenv("GC_TC_MID_COST")
+    # the middle kernels' units taken one and three per atomic
+    for grab in ("1", "3"):
+        monkeypatch.setenv("GC_TC_MID_GRAB", grab)
+        monkeypatch.setenv("GC_TC_MID_UNIT", "64")
+        assert g.triangle_count()["total_triangles"] == expected, f"RMAT-{scale}: grab={grab}"
+    for var in ("GC_TC_MID_GRAB", "GC_TC_MID_UNIT", "GC_TC_MID32"):
+        monkeypatch.delenv(var)
+    # every row taken singly, and none (rows in chunks only)
+    for heavy in ("1", "4294967295"):
+        monkeypatch.setenv("GC_TC_HEAVY_DEGREE", heavy)
+        assert g.triangle_count()["total_triangles"] == expected, f"RMAT-{scale}: heavy degree {heavy}"
+    monkeypatch.delenv("GC_TC_HEAVY_DEGREE")
+
+
 def test_bfs_schedule_switches(device, monkeypatch):
     """Distances under other fixed-rule settings against the oracle: bottom-up levels launched
     ahead or one per read-back (GC_BFS_AHEAD), Beamer's top-down -> bottom-up threshold at 1
