@@ -544,18 +544,56 @@ __device__ __forceinline__ uint32_t probe16_quarters(const uint4* O, unsigned lo
   return local;
 }
 
-// Hits of the cold part L[0, lj) of a list (every entry below hot_base)
-// among the row's cold prefix A[0, c): the Bloom filter, then a search.
-__device__ __forceinline__ uint32_t probe_cold(const uint32_t* L, uint32_t lj, const uint32_t* s_bits,
-                                               const uint32_t* A, uint32_t c) {
+// Hits of the cold parts T[st, st + len) of the lanes' lists (every entry
+// below hot_base) among the row's cold prefix A[0, c): the Bloom filter, then
+// a search. A quarter wave per part, D entries per lane and step (a quarter whose part
+// has ended takes the next one): the parts are short (26 entries on average
+// at R-MAT scale 22), and one per wave step left most lanes idle and put a
+// global round trip per part on the wave's path.
+template <int D>
+__device__ __forceinline__ uint32_t probe_cold_quarters(const uint32_t* __restrict__ T, unsigned long long lists,
+                                                        uint32_t st, uint32_t len, const uint32_t* s_bits,
+                                                        const uint32_t* A, uint32_t c) {
+  const uint32_t lane = lane_id();
+  const uint32_t quarter = lane >> 4, ql = lane & 15u;
+  uint32_t cur = 0, end = 0;
   uint32_t local = 0;
-  for (uint32_t k = lane_id(); k < lj; k += GC_WAVE) {
-    const uint32_t w = L[k];
-    const uint32_t h = bloom_hash(w);
-    if ((s_bits[TCB_BLOOM_BASE + (h >> 5)] >> (h & 31)) & 1u) {
-      const uint32_t p = lower_bound_u32(A, 0, c, w);
-      local += (p < c && A[p] == w) ? 1u : 0u;
+  for (;;) {
+    const unsigned long long idle = __ballot(cur >= end);
+    uint32_t free4 = static_cast<uint32_t>((idle & 1ull) | ((idle >> 15) & 2ull) | ((idle >> 30) & 4ull) |
+                                           ((idle >> 45) & 8ull));
+    while (free4 != 0 && lists != 0) {
+      const uint32_t fq = static_cast<uint32_t>(__ffs(free4) - 1);
+      free4 &= free4 - 1;
+      const int jv = __ffsll(static_cast<long long>(lists)) - 1;
+      lists &= lists - 1;
+      const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+      const uint32_t l1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
+      if (quarter == fq) {
+        cur = s1;
+        end = s1 + l1;
+      }
     }
+    if (__ballot(cur < end) == 0) break;
+    uint32_t w[D];
+    bool act[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const uint32_t i = cur + 16u * d + ql;
+      act[d] = i < end;
+      w[d] = T[act[d] ? i : (end > 0 ? end - 1 : 0u)];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (act[d]) {
+        const uint32_t h = bloom_hash(w[d]);
+        if ((s_bits[TCB_BLOOM_BASE + (h >> 5)] >> (h & 31)) & 1u) {
+          const uint32_t p = lower_bound_u32(A, 0, c, w[d]);
+          local += (p < c && A[p] == w[d]) ? 1u : 0u;
+        }
+      }
+    }
+    cur += 16u * D;
   }
   return local;
 }
@@ -775,14 +813,8 @@ __device__ __forceinline__ void tc_bitmap_body(
               }
             }
             if (kHot16 && c != 0) {
-              unsigned long long cold = __ballot(clen != 0);
-              while (cold != 0) {
-                const int jv = __ffsll(static_cast<long long>(cold)) - 1;
-                cold &= cold - 1;
-                const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(cst, jv));
-                const uint32_t lj = static_cast<uint32_t>(__builtin_amdgcn_readlane(clen, jv));
-                local += probe_cold(targets + s1, lj, s_bits, A, c);
-              }
+              const unsigned long long cold = __ballot(clen != 0);
+              if (cold != 0) local += probe_cold_quarters<2>(targets, cold, cst, clen, s_bits, A, c);
             }
           }
           count += local;
