@@ -90,7 +90,11 @@ uint32_t tc_quarters() {
   const uint32_t d = env != nullptr ? static_cast<uint32_t>(std::strtoul(env, nullptr, 10)) : 2u;
   const char* qmax = std::getenv("GC_TC_QMAX");
   const uint32_t m = qmax != nullptr ? static_cast<uint32_t>(std::strtoul(qmax, nullptr, 10)) : 32u;
-  return std::min(d, 4u) | std::min(m, 4096u) << 8;
+  // the cold parts' walk in tc_bitmap16: GC_TC_COLD_GROUP lanes per part (4, 8 or 16; at
+  // R-MAT scale 22 16 measured best: 4.79 ms against 4.85 with 8 and 4.89 with 4)
+  const char* cg = std::getenv("GC_TC_COLD_GROUP");
+  const uint32_t g = cg != nullptr ? static_cast<uint32_t>(std::strtoul(cg, nullptr, 10)) : 16u;
+  return std::min(d, 4u) | std::min(m, 4096u) << 8 | std::min(g, 255u) << 24;
 }
 
 // Masks of the top ids in the 16-bit lists (tc_bitmap16): GC_TC_TOP ids (a
@@ -210,6 +214,15 @@ uint32_t tc_mid_cost() {
 uint32_t tc_mid_grab() {
   const char* env = std::getenv("GC_TC_MID_GRAB");
   return env == nullptr ? 1u : std::max(1u, static_cast<uint32_t>(std::strtoul(env, nullptr, 10)));
+}
+
+// Lanes per walk of at most GC_TC_QMAX octs in tc_mid16: GC_TC_MID_GROUP = 4,
+// 8 or 16 (two octs per lane and step), or 0 for tc_bitmap16's quarter walk.
+// Half the walks hold at most 7 octs: at R-MAT scale 22, 4.79 ms with 8 lanes
+// against 4.81 (4), 4.87 (16) and 4.91 (the quarter walk).
+uint32_t tc_mid_group() {
+  const char* env = std::getenv("GC_TC_MID_GROUP");
+  return env == nullptr ? 8u : static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
 }
 
 uint32_t tc_mid_unit() {
@@ -423,7 +436,7 @@ void launch_method(Graph& g, uint32_t method, uint32_t row_lo, uint32_t row_hi, 
         launch(dev, get_kernel(dev, "tc", "tc_mid16"), blocks, kWaveSize, 0, g.rl.offsets.as<const uint32_t>(), tgt,
                fwd, hot_base, g.tc_mid_rec.as<const uint32_t>() /* uint2 */, g.tc_mid_rec_base,
                g.tc_mid_octs.as<const uint32_t>() /* uint4 */, g.tc_mid_units.as<const uint32_t>() /* uint4 */,
-               g.tc_mid_num_units, tc_quarters(), tc_mid_grab(), counters + 2, result);
+               g.tc_mid_num_units, tc_quarters(), tc_mid_grab(), tc_mid_group(), counters + 2, result);
       }
       if (mid && g.tc_mid32_num_units > 0) {
         // one-wave workgroups, as tc_mid16's, on work counter word 3

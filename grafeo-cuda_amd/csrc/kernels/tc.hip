@@ -544,32 +544,77 @@ __device__ __forceinline__ uint32_t probe16_quarters(const uint4* O, unsigned lo
   return local;
 }
 
-// Hits of the cold parts T[st, st + len) of the lanes' lists (every entry
-// below hot_base) among the row's cold prefix A[0, c): the Bloom filter, then
-// a search. A quarter wave per part, D entries per lane and step (a quarter whose part
-// has ended takes the next one): the parts are short (26 entries on average
-// at R-MAT scale 22), and one per wave step left most lanes idle and put a
-// global round trip per part on the wave's path.
-template <int D>
-__device__ __forceinline__ uint32_t probe_cold_quarters(const uint32_t* __restrict__ T, unsigned long long lists,
-                                                        uint32_t st, uint32_t len, const uint32_t* s_bits,
-                                                        const uint32_t* A, uint32_t c) {
+// Lead lanes of the groups of G lanes (bit g * G for group g).
+template <int G>
+__device__ __forceinline__ constexpr unsigned long long group_leads() {
+  return G == 4 ? 0x1111111111111111ull : G == 8 ? 0x0101010101010101ull : G == 16 ? 0x0001000100010001ull
+                                                                                    : 0x0000000100000001ull;
+}
+
+// As probe16_quarters with groups of G lanes (4, 8, 16 or 32) per list, D octs
+// per lane and step: 64 / G lists in flight per step, so that many short
+// lists cost one step together rather than one round of four each.
+template <int G, int D>
+__device__ __forceinline__ uint32_t probe16_groups(const uint4* __restrict__ O, unsigned long long lists,
+                                                   uint32_t st, uint32_t len, const uint32_t* s_bits) {
   const uint32_t lane = lane_id();
-  const uint32_t quarter = lane >> 4, ql = lane & 15u;
+  const uint32_t grp = lane / G, gl = lane % G;
   uint32_t cur = 0, end = 0;
   uint32_t local = 0;
   for (;;) {
-    const unsigned long long idle = __ballot(cur >= end);
-    uint32_t free4 = static_cast<uint32_t>((idle & 1ull) | ((idle >> 15) & 2ull) | ((idle >> 30) & 4ull) |
-                                           ((idle >> 45) & 8ull));
-    while (free4 != 0 && lists != 0) {
-      const uint32_t fq = static_cast<uint32_t>(__ffs(free4) - 1);
-      free4 &= free4 - 1;
+    unsigned long long idle = __ballot(cur >= end) & group_leads<G>();
+    while (idle != 0 && lists != 0) {
+      const uint32_t fg = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1) / G;
+      idle &= idle - 1;
       const int jv = __ffsll(static_cast<long long>(lists)) - 1;
       lists &= lists - 1;
       const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
       const uint32_t l1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
-      if (quarter == fq) {
+      if (grp == fg) {
+        cur = s1;
+        end = s1 + l1;
+      }
+    }
+    if (__ballot(cur < end) == 0) break;
+    uint4 x[D];
+    bool act[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const uint32_t i = cur + static_cast<uint32_t>(G) * d + gl;
+      act[d] = i < end;
+      x[d] = O[act[d] ? i : (end > 0 ? end - 1 : 0u)];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) local += act[d] ? hot16x8_hits(s_bits, x[d]) : 0u;
+    cur += static_cast<uint32_t>(G) * D;
+  }
+  return local;
+}
+
+// Hits of the cold parts T[st, st + len) of the lanes' lists (every entry
+// below hot_base) among the row's cold prefix A[0, c): the Bloom filter, then
+// a search. G lanes per part, D entries per lane and step (a group whose part
+// has ended takes the next one): the parts are short (26 entries on average
+// at R-MAT scale 22), and one per wave step left most lanes idle and put a
+// global round trip per part on the wave's path.
+template <int G, int D>
+__device__ __forceinline__ uint32_t probe_cold_groups(const uint32_t* __restrict__ T, unsigned long long lists,
+                                                      uint32_t st, uint32_t len, const uint32_t* s_bits,
+                                                      const uint32_t* A, uint32_t c) {
+  const uint32_t lane = lane_id();
+  const uint32_t grp = lane / G, gl = lane % G;
+  uint32_t cur = 0, end = 0;
+  uint32_t local = 0;
+  for (;;) {
+    unsigned long long idle = __ballot(cur >= end) & group_leads<G>();
+    while (idle != 0 && lists != 0) {
+      const uint32_t fg = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1) / G;
+      idle &= idle - 1;
+      const int jv = __ffsll(static_cast<long long>(lists)) - 1;
+      lists &= lists - 1;
+      const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(st, jv));
+      const uint32_t l1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(len, jv));
+      if (grp == fg) {
         cur = s1;
         end = s1 + l1;
       }
@@ -579,7 +624,7 @@ __device__ __forceinline__ uint32_t probe_cold_quarters(const uint32_t* __restri
     bool act[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const uint32_t i = cur + 16u * d + ql;
+      const uint32_t i = cur + static_cast<uint32_t>(G) * d + gl;
       act[d] = i < end;
       w[d] = T[act[d] ? i : (end > 0 ? end - 1 : 0u)];
     }
@@ -593,7 +638,7 @@ __device__ __forceinline__ uint32_t probe_cold_quarters(const uint32_t* __restri
         }
       }
     }
-    cur += 16u * D;
+    cur += static_cast<uint32_t>(G) * D;
   }
   return local;
 }
@@ -755,7 +800,7 @@ __device__ __forceinline__ void tc_bitmap_body(
             // (0: every list a wave step)
             unsigned long long pending = __ballot(len != 0);
             if (kHot16 && (quarters & 0xFFu) != 0) {
-              const unsigned long long small = __ballot(len != 0 && len <= (quarters >> 8));
+              const unsigned long long small = __ballot(len != 0 && len <= ((quarters >> 8) & 0xFFFFu));
               if (small != 0) {
                 switch (quarters & 0xFFu) {
                   case 1: local += probe16_quarters<1>(quads, small, st, len, s_bits); break;
@@ -814,7 +859,14 @@ __device__ __forceinline__ void tc_bitmap_body(
             }
             if (kHot16 && c != 0) {
               const unsigned long long cold = __ballot(clen != 0);
-              if (cold != 0) local += probe_cold_quarters<2>(targets, cold, cst, clen, s_bits, A, c);
+              if (cold != 0) {
+                // quarters >> 24 lanes per part (4 or 8; else 16)
+                switch (quarters >> 24) {
+                  case 4: local += probe_cold_groups<4, 4>(targets, cold, cst, clen, s_bits, A, c); break;
+                  case 8: local += probe_cold_groups<8, 2>(targets, cold, cst, clen, s_bits, A, c); break;
+                  default: local += probe_cold_groups<16, 2>(targets, cold, cst, clen, s_bits, A, c); break;
+                }
+              }
             }
           }
           count += local;
@@ -1096,7 +1148,7 @@ extern "C" __global__ __launch_bounds__(256) void tc_mid_rowwork(const uint32_t*
 extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) void tc_mid16(
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
     uint32_t hot_base, const uint2* __restrict__ rec, uint32_t rec_base, const uint4* __restrict__ octs,
-    const uint4* __restrict__ units, uint32_t num_units, uint32_t quarters, uint32_t grab,
+    const uint4* __restrict__ units, uint32_t num_units, uint32_t quarters, uint32_t grab, uint32_t group,
     uint32_t* __restrict__ counter, unsigned long long* __restrict__ result) {
   __shared__ uint32_t s_bits[TCB_HOT_WORDS + 1];
   const uint32_t lane = lane_id();
@@ -1128,13 +1180,20 @@ extern "C" __global__ __launch_bounds__(GC_WAVE) __attribute__((amdgpu_waves_per
       const uint32_t st = r.x, len = r.y;
       unsigned long long pending = __ballot(len != 0);
       if ((quarters & 0xFFu) != 0) {
-        const unsigned long long small = __ballot(len != 0 && len <= (quarters >> 8));
+        const unsigned long long small = __ballot(len != 0 && len <= ((quarters >> 8) & 0xFFFFu));
         if (small != 0) {
-          switch (quarters & 0xFFu) {
-            case 1: local += probe16_quarters<1>(octs, small, st, len, s_bits); break;
-            case 3: local += probe16_quarters<3>(octs, small, st, len, s_bits); break;
-            case 4: local += probe16_quarters<4>(octs, small, st, len, s_bits); break;
-            default: local += probe16_quarters<2>(octs, small, st, len, s_bits); break;
+          // `group` lanes per walk (4, 8 or 16), two octs per lane and step; 0: quarters as tc_bitmap16
+          switch (group) {
+            case 4: local += probe16_groups<4, 2>(octs, small, st, len, s_bits); break;
+            case 8: local += probe16_groups<8, 2>(octs, small, st, len, s_bits); break;
+            case 16: local += probe16_groups<16, 2>(octs, small, st, len, s_bits); break;
+            default:
+              switch (quarters & 0xFFu) {
+                case 1: local += probe16_quarters<1>(octs, small, st, len, s_bits); break;
+                case 3: local += probe16_quarters<3>(octs, small, st, len, s_bits); break;
+                case 4: local += probe16_quarters<4>(octs, small, st, len, s_bits); break;
+                default: local += probe16_quarters<2>(octs, small, st, len, s_bits); break;
+              }
           }
         }
         pending &= ~small;

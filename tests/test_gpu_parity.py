@@ -347,6 +347,19 @@ def test_rmat_tc_sides(device, monkeypatch, scale):
         assert g.triangle_count()["total_triangles"] == expected, f"RMAT-{scale}: grab={grab}"
     for var in ("GC_TC_MID_GRAB", "GC_TC_MID_UNIT", "GC_TC_MID32"):
         monkeypatch.delenv(var)
+    # the middle kernel's short walks 4, 8 or 16 lanes each or a quarter wave (0), up to 4096 octs;
+    # the cold parts' walk in tc_bitmap16 4, 8 or 16 lanes per part
+    for group, cold, qmax in (("0", "4", None), ("4", "8", None), ("8", "16", "4096"), ("16", "4", "1")):
+        monkeypatch.setenv("GC_TC_MID_GROUP", group)
+        monkeypatch.setenv("GC_TC_COLD_GROUP", cold)
+        if qmax is None:
+            monkeypatch.delenv("GC_TC_QMAX", raising=False)
+        else:
+            monkeypatch.setenv("GC_TC_QMAX", qmax)
+        assert g.triangle_count()["total_triangles"] == expected, \
+            f"RMAT-{scale}: mid group {group} cold group {cold} qmax {qmax}"
+    for var in ("GC_TC_MID_GROUP", "GC_TC_COLD_GROUP", "GC_TC_QMAX"):
+        monkeypatch.delenv(var, raising=False)
     # every row taken singly, and none (rows in chunks only)
     for heavy in ("1", "4294967295"):
         monkeypatch.setenv("GC_TC_HEAVY_DEGREE", heavy)
