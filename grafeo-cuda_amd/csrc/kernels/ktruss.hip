@@ -195,11 +195,12 @@ __device__ __forceinline__ uint32_t ktb_walk(const uint32_t* __restrict__ target
 // lo[x] .. hi[x]: the forward entries of row x (the relabelled CSR's fwd and
 // offsets + 1); rows [0, n), taken from the top. support[e] (zeroed) gains
 // the triangles of every forward entry e. counters: parts x
-// KTB_COUNTER_STRIDE zeroed words.
+// KTB_COUNTER_STRIDE zeroed words. The edges u->v whose bit is set in
+// midbits (null: none) are left to kt_mid_support.
 extern "C" __global__ __launch_bounds__(KTB_WAVES * GC_WAVE) void kt_bitmap_support(
     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, const uint32_t* __restrict__ targets,
     uint32_t n, uint32_t hot_base, uint32_t heavy, uint32_t light_chunk, uint32_t parts, uint32_t split_min,
-    uint32_t* __restrict__ counters, uint32_t* __restrict__ support) {
+    uint32_t* __restrict__ counters, uint32_t* __restrict__ support, const uint32_t* __restrict__ midbits) {
   __shared__ KtbLds s_all[KTB_WAVES];
   KtbLds& L = s_all[threadIdx.x / GC_WAVE];
   const uint32_t lane = lane_id();
@@ -271,7 +272,10 @@ extern "C" __global__ __launch_bounds__(KTB_WAVES * GC_WAVE) void kt_bitmap_supp
             uint32_t st = 0, len = 0;
             if (lane < nv) {
               const uint32_t v = A[cv + lane];
-              if (whole || v % parts == part) {
+              // an entry u->v whose bit is set in midbits is walked by kt_mid_support at v
+              const uint32_t e = row_e + cv + lane;
+              const bool mid = midbits != nullptr && ((midbits[e >> 5] >> (e & 31u)) & 1u) != 0;
+              if (!mid && (whole || v % parts == part)) {
                 st = lo[v];
                 len = hi[v] - st;
               }
@@ -303,6 +307,159 @@ extern "C" __global__ __launch_bounds__(KTB_WAVES * GC_WAVE) void kt_bitmap_supp
         }
       }
     }
+  }
+}
+
+// ---- the middle side of the supports (kt_mid_support) ----
+//
+// As in the triangle count (tc_mid16), a DAG edge u->v with v among the hot
+// ids can be walked from v: the entries of L(u) above v against the bitmap
+// of L(v), whose entries are all hot. Each hit w is the triangle u < v < w:
+// u->w is the walked entry itself (a global atomic), v->w a 16-bit LDS counter
+// per entry of L(v) (its rank, as kt_bitmap_support ranks L(u)'s entries),
+// added once per unit, and u->v takes the walk's hits at once. Which side
+// walks an edge is decided once per graph (kt_mid_records): the middle side
+// when the walk, plus `ovh` entries, is shorter than L(v), or empty.
+
+// Records of the entries of rows v in [hot_base, n) (rec[i - offsets[hot_base]]):
+// {first entry, entries} of the walk of L(u) above v, or {0, 0}; sets
+// midbits and walks[v - hot_base] = 1 for the rows with a non-empty walk.
+extern "C" __global__ __launch_bounds__(256) void kt_mid_records(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
+    uint32_t n, uint32_t hot_base, uint32_t ovh, uint2* __restrict__ rec, uint32_t* __restrict__ midbits,
+    uint8_t* __restrict__ walks) {
+  const uint32_t rec_base = offsets[hot_base];
+  const uint64_t total = static_cast<uint64_t>(offsets[n]) - rec_base;
+  for (uint64_t k = global_thread(); k < total; k += grid_threads()) {
+    const uint32_t i = rec_base + static_cast<uint32_t>(k);
+    uint32_t lo = hot_base, hi = n - 1;  // the row of entry i
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+      if (offsets[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t v = lo;
+    uint2 r = make_uint2(0u, 0u);
+    const uint32_t u = targets[i];
+    if (u < v) {
+      const uint32_t ue = offsets[u + 1];
+      const uint32_t j = lower_bound_u32(targets, fwd[u], ue, v);
+      if (j < ue && targets[j] == v) {
+        const uint32_t walk = ue - j - 1;
+        if (walk == 0 || walk + ovh < offsets[v + 1] - fwd[v]) {
+          r = make_uint2(j + 1, walk);
+          atomicOr(&midbits[j >> 5], 1u << (j & 31u));
+          if (walk != 0) walks[v - hot_base] = 1;
+        }
+      }
+    }
+    rec[k] = r;
+  }
+}
+
+// Units {v, first entry, end entry, 0} of the hot rows' in-lists, one per
+// atomic on `counter`; per unit the wave sets L(v)'s bits and word ranks,
+// walks the records' walks a quarter wave each (four entries per lane and
+// step), then adds the v->w counters and clears its sets.
+extern "C" __global__ __launch_bounds__(GC_WAVE) void kt_mid_support(
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ targets, const uint32_t* __restrict__ fwd,
+    uint32_t hot_base, const uint2* __restrict__ rec, uint32_t rec_base, const uint4* __restrict__ units,
+    uint32_t num_units, uint32_t* __restrict__ counter, uint32_t* __restrict__ support) {
+  __shared__ uint32_t s_bits[KTB_HOT_WORDS + 1];
+  __shared__ uint16_t s_wrank[KTB_HOT_WORDS];
+  __shared__ uint32_t s_cnt[KTB_COUNTERS / 2];
+  const uint32_t lane = lane_id();
+  const uint32_t quarter = lane >> 4, ql = lane & 15u;
+  for (uint32_t j = lane; j < KTB_HOT_WORDS + 1; j += GC_WAVE) s_bits[j] = 0;
+  for (uint32_t j = lane; j < KTB_COUNTERS / 2; j += GC_WAVE) s_cnt[j] = 0;
+  wave_lds_sync();
+  for (;;) {
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(counter, 1u);
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+    if (k >= num_units) break;
+    const uint4 unit = units[k];
+    const uint32_t v = __builtin_amdgcn_readfirstlane(unit.x);
+    const uint32_t b = __builtin_amdgcn_readfirstlane(unit.y), e = __builtin_amdgcn_readfirstlane(unit.z);
+    const uint32_t lb = fwd[v], le = offsets[v + 1];
+    const uint32_t a = le - lb;  // L(v): all hot (v >= hot_base)
+    const bool lds_cnt = a <= KTB_COUNTERS;
+    for (uint32_t j = lb + lane; j < le; j += GC_WAVE) {
+      const uint32_t y = targets[j] - hot_base;
+      atomicOr(&s_bits[y >> 5], 1u << (y & 31u));
+      if (j == lb || ((targets[j - 1] - hot_base) >> 5) != (y >> 5)) s_wrank[y >> 5] = static_cast<uint16_t>(j - lb);
+    }
+    wave_lds_sync();
+    for (uint32_t i0 = b; i0 < e; i0 += GC_WAVE) {
+      const uint32_t i = i0 + lane;
+      const uint2 r = i < e ? rec[i - rec_base] : make_uint2(0u, 0u);
+      unsigned long long lists = __ballot(r.y != 0);
+      uint32_t cur = 0, end = 0, edge = 0, hits = 0;  // the quarter's walk, its edge u->v, its hits
+      for (;;) {
+        const unsigned long long idle = __ballot(cur >= end);
+        uint32_t free4 = static_cast<uint32_t>((idle & 1ull) | ((idle >> 15) & 2ull) | ((idle >> 30) & 4ull) |
+                                               ((idle >> 45) & 8ull));
+        while (free4 != 0 && lists != 0) {
+          const uint32_t fq = static_cast<uint32_t>(__ffs(free4) - 1);
+          free4 &= free4 - 1;
+          const int jv = __ffsll(static_cast<long long>(lists)) - 1;
+          lists &= lists - 1;
+          const uint32_t s1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(r.x, jv));
+          const uint32_t l1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(r.y, jv));
+          if (quarter == fq) {
+            cur = s1;
+            end = s1 + l1;
+            edge = s1 - 1;
+            hits = 0;
+          }
+        }
+        if (__ballot(cur < end) == 0) break;
+        uint32_t w[4];
+        bool act[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          const uint32_t idx = cur + 16u * d + ql;
+          act[d] = idx < end;
+          w[d] = targets[act[d] ? idx : (end > 0 ? end - 1 : 0u)];
+        }
+        uint32_t h = 0;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          const uint32_t y = w[d] - hot_base;  // w > v >= hot_base for a walked entry
+          const uint32_t word = s_bits[min(y >> 5, static_cast<uint32_t>(KTB_HOT_WORDS))];
+          const uint32_t bit = y & 31u;
+          if (act[d] && ((word >> bit) & 1u)) {
+            atomicAdd(&support[cur + 16u * d + ql], 1u);  // u->w
+            const uint32_t rk = s_wrank[y >> 5] + static_cast<uint32_t>(__popc(word & ((1u << bit) - 1u)));
+            if (lds_cnt) atomicAdd(&s_cnt[rk >> 1], 1u << (16u * (rk & 1u)));  // v->w
+            else atomicAdd(&support[lb + rk], 1u);
+            ++h;
+          }
+        }
+        // the quarter's hits this step (lanes of a quarter hold the same walk)
+        h += __shfl_xor(h, 8);
+        h += __shfl_xor(h, 4);
+        h += __shfl_xor(h, 2);
+        h += __shfl_xor(h, 1);
+        const bool had = cur < end;
+        hits += had ? h : 0u;
+        cur += 64u;
+        if (had && cur >= end && ql == 0 && hits != 0) atomicAdd(&support[edge], hits);  // u->v
+      }
+    }
+    wave_lds_sync();
+    if (lds_cnt) {
+      for (uint32_t rk = lane; rk < a; rk += GC_WAVE) {
+        const uint32_t x = (s_cnt[rk >> 1] >> (16u * (rk & 1u))) & 0xFFFFu;
+        if (x != 0) atomicAdd(&support[lb + rk], x);
+      }
+    }
+    wave_lds_sync();
+    if (lds_cnt) {
+      for (uint32_t j = lane; j < (a + 1) / 2; j += GC_WAVE) s_cnt[j] = 0;
+    }
+    for (uint32_t j = lb + lane; j < le; j += GC_WAVE) s_bits[(targets[j] - hot_base) >> 5] = 0;
+    wave_lds_sync();
   }
 }
 

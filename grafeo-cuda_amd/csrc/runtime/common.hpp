@@ -371,6 +371,15 @@ struct Graph {
   bool fwd_original_ready = false;
   DeviceMemory ktruss_rev;    // u32[m] reverse entry of each forward relabelled entry
   bool ktruss_rev_ready = false;
+  // The middle side of the k-truss supports (kt_mid_support): one record per entry of the hot
+  // rows, the bits of the DAG entries it walks, its work units.
+  DeviceMemory kt_mid_rec;      // uint2 per entry of rows [hot_base, n): {first entry, entries}
+  DeviceMemory kt_midbits;      // one bit per CSR entry
+  DeviceMemory kt_mid_units;    // uint4 {v, first entry, end entry, 0}
+  uint32_t kt_mid_num_units = 0;
+  uint32_t kt_mid_rec_base = 0;
+  bool kt_mid_ready = false;
+  uint32_t kt_mid_key[2] = {0, 0};  // per-edge cost and unit the build is for
   DeviceMemory bfs_top;       // u32[4][n] the four highest neighbours of each relabelled row
   bool bfs_top_ready = false;
   // The DAG lists' entries among the top 32768 ids as 16-bit offsets, eight to

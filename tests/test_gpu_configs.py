@@ -84,10 +84,17 @@ def test_rmat20_suite(device, golden, monkeypatch):
     monkeypatch.delenv("GC_TC_MID_COST")
     monkeypatch.delenv("GC_TC_MID_UNIT")
 
-    kt = g.k_truss_numpy(4)
-    assert len(kt["edges"]) == e["k_truss_4"]["edges"]
-    assert kt["iterations"] == e["k_truss_4"]["iterations"]
-    assert hashlib.sha256(kt["edges"].astype("<u4").tobytes()).hexdigest() == e["k_truss_4"]["sha256"]
+    # the 4-truss with the supports' middle side (default), without it, and with it taking every edge
+    # it may (cost 0)
+    for mid, cost in (("1", "16"), ("0", "16"), ("1", "0")):
+        monkeypatch.setenv("GC_KTRUSS_MID", mid)
+        monkeypatch.setenv("GC_KTRUSS_MID_COST", cost)
+        kt = g.k_truss_numpy(4)
+        assert len(kt["edges"]) == e["k_truss_4"]["edges"], f"mid={mid} cost={cost}"
+        assert kt["iterations"] == e["k_truss_4"]["iterations"]
+        assert hashlib.sha256(kt["edges"].astype("<u4").tobytes()).hexdigest() == e["k_truss_4"]["sha256"]
+    monkeypatch.delenv("GC_KTRUSS_MID")
+    monkeypatch.delenv("GC_KTRUSS_MID_COST")
 
     # truss numbers: the edges with trussness >= 4 are exactly the golden 4-truss, and the
     # edges with trussness >= max are the max-truss (checked against the GPU k-truss)

@@ -162,23 +162,27 @@ def check_pagerank(name, off, tgt, g):
 
 
 def check_k_truss(name, off, tgt, g, monkeypatch, ks=range(3, 9), both_paths=True, fast_oracle=False):
-    """Both GPU paths (the DAG rounds, with the supports from the bitmap walk and from the wedge
+    """Both GPU paths (the DAG rounds, with the supports from the bitmap walk with and without its
+    middle side (every edge it may take at cost 0, units of 64 entries) and from the wedge
     enumeration, and the per-entry rounds) against the oracle."""
-    paths = (("0", None), ("0", "wedge"), ("1", None)) if both_paths else (("0", None),)
+    paths = ((("0", None, None, None, None), ("0", None, "0", None, None), ("0", None, None, "0", "64"),
+              ("0", "wedge", None, None, None), ("1", None, None, None, None)) if both_paths
+             else (("0", None, None, None, None), ("0", None, "0", None, None)))
+    names = ("GC_KTRUSS_DIRECT", "GC_KTRUSS_SUPPORT", "GC_KTRUSS_MID", "GC_KTRUSS_MID_COST", "GC_KTRUSS_MID_UNIT")
     for k in ks:
         edges, iterations = oracle.k_truss(off, tgt, k, fast=fast_oracle)
-        for direct, support in paths:
-            monkeypatch.setenv("GC_KTRUSS_DIRECT", direct)
-            if support is None:
-                monkeypatch.delenv("GC_KTRUSS_SUPPORT", raising=False)
-            else:
-                monkeypatch.setenv("GC_KTRUSS_SUPPORT", support)
+        for path in paths:
+            for var, val in zip(names, path):
+                if val is None:
+                    monkeypatch.delenv(var, raising=False)
+                else:
+                    monkeypatch.setenv(var, val)
             r = g.k_truss(k)
-            assert r["edges"] == edges, f"{name}: {k}-truss (direct={direct}, support={support})"
-            assert r["iterations"] == iterations, f"{name}: {k}-truss rounds (direct={direct}, support={support})"
+            assert r["edges"] == edges, f"{name}: {k}-truss ({dict(zip(names, path))})"
+            assert r["iterations"] == iterations, f"{name}: {k}-truss rounds ({dict(zip(names, path))})"
             assert r["k"] == k
-    monkeypatch.delenv("GC_KTRUSS_DIRECT")
-    monkeypatch.delenv("GC_KTRUSS_SUPPORT", raising=False)
+    for var in names:
+        monkeypatch.delenv(var, raising=False)
 
 
 def check_msbfs(name, off, tgt, g, sources):
@@ -193,8 +197,11 @@ def check_msbfs(name, off, tgt, g, sources):
 def check_truss_numbers(name, off, tgt, g, monkeypatch=None):
     """Trussness against the oracle's peeling; with monkeypatch, also from the wedge supports."""
     edges, truss = oracle.truss_numbers(off, tgt)
-    for support in ((None, "wedge") if monkeypatch is not None else (None,)):
-        if support is not None:
+    for support in ((None, "wedge", "nomid") if monkeypatch is not None else (None,)):
+        if support == "nomid":  # the bitmap walk's supports without its middle side
+            monkeypatch.delenv("GC_KTRUSS_SUPPORT", raising=False)
+            monkeypatch.setenv("GC_KTRUSS_MID", "0")
+        elif support is not None:
             monkeypatch.setenv("GC_KTRUSS_SUPPORT", support)
         r = g.truss_numbers_numpy()
         assert [(int(u), int(v)) for u, v in r["edges"]] == edges, f"{name}: truss edges"
@@ -202,6 +209,7 @@ def check_truss_numbers(name, off, tgt, g, monkeypatch=None):
         assert r["max_truss"] == (int(truss.max()) if truss.size else 0), f"{name}: max truss"
     if monkeypatch is not None:
         monkeypatch.delenv("GC_KTRUSS_SUPPORT", raising=False)
+        monkeypatch.delenv("GC_KTRUSS_MID", raising=False)
 
 
 def test_catalog(device, monkeypatch):
@@ -360,6 +368,18 @@ def test_rmat_tc_sides(device, monkeypatch, scale):
             f"RMAT-{scale}: mid group {group} cold group {cold} qmax {qmax}"
     for var in ("GC_TC_MID_GROUP", "GC_TC_COLD_GROUP", "GC_TC_QMAX"):
         monkeypatch.delenv(var, raising=False)
+    # the k-truss supports with ids below the hot ones: the bitmap walk with and without its middle side
+    kedges, kit = oracle.k_truss(off, tgt, 4, fast=True)
+    for mid, cost in (("1", None), ("0", None), ("1", "0")):
+        monkeypatch.setenv("GC_KTRUSS_MID", mid)
+        if cost is None:
+            monkeypatch.delenv("GC_KTRUSS_MID_COST", raising=False)
+        else:
+            monkeypatch.setenv("GC_KTRUSS_MID_COST", cost)
+        r = g.k_truss(4)
+        assert r["edges"] == kedges and r["iterations"] == kit, f"RMAT-{scale}: 4-truss mid={mid} cost={cost}"
+    monkeypatch.delenv("GC_KTRUSS_MID")
+    monkeypatch.delenv("GC_KTRUSS_MID_COST", raising=False)
     # every row taken singly, and none (rows in chunks only)
     for heavy in ("1", "4294967295"):
         monkeypatch.setenv("GC_TC_HEAVY_DEGREE", heavy)
