@@ -59,7 +59,7 @@ KERNEL_FAMILIES = {
             "bfs_top_build", "bfs_chain"),
     "msbfs": ("ms_seed", "ms_seed_levels", "ms_topdown", "ms_bottomup_thread", "ms_bottomup_wave",
               "ms_finalize", "ms_write_levels", "ms_levels_to_dist", "ms_write_distances", "ms_summarize"),
-    "ktruss": ("kt_bitmap_support", "kt_wedge_support", "kt_truss_frontier", "kt_truss_peel", "kt_truss_commit", "kt_truss_decrement",
+    "ktruss": ("kt_bitmap_support", "kt_mid_support", "kt_mid_records", "kt_wedge_support", "kt_truss_frontier", "kt_truss_peel", "kt_truss_commit", "kt_truss_decrement",
                "kt_truss_values", "kt_filter", "kt_support_direct", "kt_reverse_index", "kt_canonical_index",
                "kt_canonical_reverse", "kt_emit_flags_direct", "kt_emit_flags_state"),
     "batch": ("bt_count",),
@@ -278,6 +278,47 @@ def tc_stored_width_bytes(rl: dict, parts: int = 4, split_min: int | None = None
             "list_bytes_32bit": int((indeg * 4 * dplus).sum()), "row_bytes": row_reads, "hot_base": int(hot_base),
             "top_bits": int(top_bits), "masked_rows": masked_rows, "split_min": int(split_min),
             "hot_fraction_of_streamed_entries": float((indeg_all * hot).sum() / max(1, (indeg_all * dplus).sum()))}
+
+
+def ktruss_support_bytes(rl: dict, parts: int = 4, split_min: int = 256, hot_bits: int = 32768) -> dict:
+    """Bytes of one k-truss support pass by the model that fits its kernels (on the degree-ordered
+    DAG, `rl` = GpuGraph.relabelled_numpy()): kt_bitmap_support reads the offsets and the DAG split,
+    4(n+1) + 4n, L(u) at 4 bytes once per part for the rows with an edge on its side, one bit per
+    DAG entry, and L(v) at 4 bytes for every DAG edge u->v on its side; kt_mid_support (unless
+    GC_KTRUSS_MID=0) takes the edges u->v with v among the hot ids whose walk of L(u) above v, plus
+    GC_KTRUSS_MID_COST entries (16), is shorter than L(v) or empty, and reads their walks at 4 bytes
+    an entry, an 8-byte record per entry of the hot rows, and L(v) twice per work unit."""
+    mid = os.environ.get("GC_KTRUSS_MID", "1") != "0"
+    cost = tc_env("GC_KTRUSS_MID_COST", 16)
+    unit = max(64, tc_env("GC_KTRUSS_MID_UNIT", 8192))
+    off = rl["offsets"].astype(np.int64)
+    fwd = rl["fwd"].astype(np.int64)
+    tgt = rl["targets"]
+    n = off.size - 1
+    hot_base = ((n - hot_bits) + 31) & ~31 if n > hot_bits else 0
+    dplus = off[1:] - fwd
+    row = np.repeat(np.arange(n, dtype=np.int32), dplus)
+    dag_first = np.concatenate(([0], np.cumsum(dplus)[:-1]))
+    dag_v = tgt[np.arange(row.size, dtype=np.int64) + np.repeat(fwd - dag_first, dplus)].astype(np.int64)
+    mid_bytes = 0
+    if mid and hot_base < n:
+        k = np.arange(row.size, dtype=np.int64) - np.repeat(dag_first, dplus)
+        suffix = dplus[row] - k - 1
+        del k
+        is_mid = (dag_v >= hot_base) & ((suffix == 0) | (suffix + cost < dplus[dag_v]))
+        units = np.zeros(n, np.int64)
+        deg = np.diff(off)
+        units[hot_base:] = (deg[hot_base:] + unit - 1) // unit
+        mid_bytes = (4 * int(suffix[is_mid].sum()) + 8 * int(off[n] - off[hot_base]) + int((8 * dplus * units).sum()))
+        row_side = ~is_mid
+        del suffix, is_mid
+    else:
+        row_side = np.ones(row.size, bool)
+    streamed = int((4 * dplus[dag_v[row_side]]).sum())
+    rowwork = np.bincount(row[row_side], minlength=n) > 0
+    row_reads = int((4 * dplus * np.where(dplus >= split_min, parts, 1) * rowwork).sum())
+    row_bytes = 4 * (n + 1) + 4 * n + row_reads + streamed + (row.size // 8 if mid else 0)
+    return {"bytes": row_bytes + mid_bytes, "row_side_bytes": row_bytes, "mid_side_bytes": mid_bytes}
 
 
 def dag_merge_bytes(off: np.ndarray, tgt: np.ndarray) -> int:
@@ -673,16 +714,15 @@ def reference_convention(dev, g, offsets, sources, args, m) -> dict:
     # round. This library counts the supports once, by the triangle walk on the hub bitmap over the
     # 32-bit lists (tc_bitmap's stored bytes, plus a 4-byte read-modify-write of one support per
     # triangle), and then only peels.
-    n = offsets.size - 1
-    tcm = tc_stored_width_bytes(g.relabelled_numpy(), mid=False, split_min=tc_env("GC_TC_SPLIT", 256))
+    ktm = ktruss_support_bytes(g.relabelled_numpy(), split_min=tc_env("GC_TC_SPLIT", 256))
     triangles = g.triangle_count()["total_triangles"]
-    kt_bytes = (4 * (n + 1) + 4 * n + tcm["row_bytes"] + tcm["list_bytes_32bit"] + 8 * triangles
-                + 5 * m * rounds)
+    kt_bytes = ktm["bytes"] + 8 * triangles + 5 * m * rounds
     out["rooflines"] = {"ktruss": roofline_entry(
         kt_bytes, 1000.0 * kt_kernel_ms,
-        note="one support pass at the bitmap walk's stored bytes (4(n+1) + 4n + L(u) at 4 bytes once per "
-             "part + L(v) at 4 bytes per DAG edge u->v + 8 bytes per triangle) plus 5m per peel round; "
-             "avg_launch_us is the kernels' event time per k-truss call (k = 4)")}
+        note="one support pass (bench.py ktruss_support_bytes: each DAG edge's supports walked from its "
+             f"cheaper end, {ktm['row_side_bytes']} bytes on the row side, {ktm['mid_side_bytes']} on the middle "
+             "side) + 8 bytes per triangle, plus 5m per peel round; avg_launch_us is the kernels' event time "
+             "per k-truss call (k = 4)")}
     log(f"reference convention: {json.dumps({k: v for k, v in out.items() if k != 'rooflines'})[:400]}")
     return out
 
