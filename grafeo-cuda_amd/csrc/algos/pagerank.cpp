@@ -8,6 +8,7 @@
 // reference sums in FP32 with FP32 atomics), and the convergence test runs on
 // the device, the host reading the state once per batch of iterations.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -25,6 +26,21 @@ constexpr uint32_t kPrThreads = 256;     // pr_spmv workgroup
 constexpr uint32_t kPrHotThreads = 1024; // pr_spmv_hot workgroup
 constexpr uint32_t kPrMaxSegments = 24;
 constexpr uint32_t kPrBatch = 8;  // iterations launched between state reads
+
+// Iterations to launch before the next state read. Until two reads are in, a
+// whole batch; then the L1 change's decay per iteration, measured between the
+// last two reads, predicts when it falls below the tolerance, and the batch
+// stops there (at least one iteration, at most kPrBatch), so that few of the
+// launches after convergence (which return at once) are made. The read
+// schedule changes nothing computed: only how many empty launches follow.
+uint32_t pr_next_batch(double prev_diff, uint32_t prev_it, double diff, uint32_t it, double tol) {
+  if (prev_it == 0 || it <= prev_it || !(diff > 0.0) || !(prev_diff > diff) || !(tol > 0.0)) return kPrBatch;
+  const double rate = std::pow(diff / prev_diff, 1.0 / static_cast<double>(it - prev_it));  // in (0, 1)
+  if (!(rate > 0.0) || !(rate < 1.0)) return kPrBatch;
+  if (diff < tol) return 1;
+  const double need = std::ceil(std::log(tol / diff) / std::log(rate));
+  return static_cast<uint32_t>(std::min<double>(kPrBatch, std::max(1.0, need)));
+}
 constexpr uint32_t kPrExact = 0x80000000u;  // group flag: rows of one exact degree
 constexpr uint32_t kPrExactMax = 16;        // exact-degree segments for degrees 0..15
 constexpr uint32_t kPrHubLimit = 1024;      // rows of at least this degree are summed in chunks
@@ -416,6 +432,10 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
   const double d = static_cast<double>(damping);
   const double tol = static_cast<double>(tolerance);
   PrState host_state = init;
+  // the state is read back after iteration next_read (and after the last one)
+  uint32_t next_read = kPrBatch;
+  double prev_diff = 0.0;
+  uint32_t prev_it = 0;
   for (uint32_t it = 0; it < max_iterations; ++it) {
     const int in = it & 1, out = in ^ 1;
     hipError_t err;
@@ -483,10 +503,13 @@ void pagerank_push(Graph& g, float damping, uint32_t max_iterations, float toler
              static_cast<const unsigned long long*>(push_partials), grid, partials, grid);
     }
     launch(dev, finalize, 1u, 1024u, 0, state, static_cast<const double*>(partials), partial_blocks, tol, work);
-    if ((it + 1) % kPrBatch == 0 || it + 1 == max_iterations) {
+    if (it + 1 == next_read || it + 1 == max_iterations) {
       copy_to_host(dev, dev.pinned, state, sizeof(PrState));
       std::memcpy(&host_state, dev.pinned, sizeof(PrState));
       if (host_state.done) break;
+      next_read = it + 1 + pr_next_batch(prev_diff, prev_it, host_state.diff, host_state.iterations, tol);
+      prev_diff = host_state.diff;
+      prev_it = host_state.iterations;
     }
   }
   const int latest = host_state.iterations & 1;
